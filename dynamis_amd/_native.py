@@ -1,0 +1,98 @@
+"""ctypes binding of libdynamis_amd.so, the HIP library behind the C ABI of include/dynamis.h.
+
+The library is built in-tree (dynamis_amd/build.py). Loading fails loudly when it is missing:
+there is no CPU fallback anywhere in the product.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import threading
+
+LIBRARY = pathlib.Path(__file__).resolve().parent / "libdynamis_amd.so"
+
+c_u32 = ctypes.c_uint32
+c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
+c_int = ctypes.c_int
+
+# name -> (restype, argtypes); every symbol include/dynamis.h declares.
+SIGNATURES: dict[str, tuple[object, list[object]]] = {
+    "dyn_version": (c_int, []),
+    "dyn_last_error": (ctypes.c_char_p, []),
+    "dyn_device_count": (c_int, [ctypes.POINTER(c_int)]),
+    "dyn_set_device": (c_int, [c_int]),
+    "dyn_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size_t]),
+    "dyn_free": (c_int, [c_void_p]),
+    "dyn_memcpy_htod": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "dyn_memcpy_dtoh": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "dyn_memcpy_dtod": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "dyn_memset": (c_int, [c_void_p, c_int, c_size_t]),
+    "dyn_stream_create": (c_int, [ctypes.POINTER(c_void_p)]),
+    "dyn_stream_destroy": (c_int, [c_void_p]),
+    "dyn_stream_synchronize": (c_int, [c_void_p]),
+    "dyn_event_create": (c_int, [ctypes.POINTER(c_void_p)]),
+    "dyn_event_destroy": (c_int, [c_void_p]),
+    "dyn_event_record": (c_int, [c_void_p, c_void_p]),
+    "dyn_event_elapsed_ms": (c_int, [ctypes.POINTER(ctypes.c_float), c_void_p, c_void_p]),
+    "dyn_profile_enable": (c_int, [c_int]),
+    "dyn_profile_collect": (c_int, [ctypes.POINTER(c_int)]),
+    "dyn_profile_entry": (c_int, [c_int, ctypes.c_char_p, c_size_t, ctypes.POINTER(c_u32),
+                                  ctypes.POINTER(ctypes.c_double)]),
+    "dyn_sort_units_for": (c_u32, [c_u32]),
+    "dyn_sort_key_words": (c_u32, [c_u32]),
+    "dyn_sort_scratch_bytes": (c_size_t, [c_u32, c_u32]),
+    "dyn_sort_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_void_p,
+                               c_void_p, c_u32, c_u32, c_u32, c_void_p, c_size_t]),
+    "dyn_broadphase_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),
+    "dyn_broadphase": (c_int, [c_void_p, c_void_p]),
+}
+
+
+class DynamisError(RuntimeError):
+    """A C-ABI entry point returned a negative status."""
+
+    def __init__(self, name: str, code: int, message: str):
+        super().__init__(f"{name} failed ({code}): {message}")
+        self.code = code
+
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIBRARY.exists():
+                raise RuntimeError(
+                    f"{LIBRARY} is missing: build the HIP library first "
+                    "(python -c 'import __graft_entry__ as g; g.build()')")
+            handle = ctypes.CDLL(str(LIBRARY))
+            for name, (restype, argtypes) in SIGNATURES.items():
+                fn = getattr(handle, name)
+                fn.restype = restype
+                fn.argtypes = argtypes
+            _lib = handle
+        return _lib
+
+
+def check(name: str, code: int) -> None:
+    if code != 0:
+        message = lib().dyn_last_error().decode(errors="replace")
+        raise DynamisError(name, code, message)
+
+
+def call(name: str, *args) -> None:
+    """Calls an int-status entry point and raises DynamisError on failure."""
+    check(name, getattr(lib(), name)(*args))
+
+
+def device_count() -> int:
+    n = c_int(0)
+    call("dyn_device_count", ctypes.byref(n))
+    return n.value

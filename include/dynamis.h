@@ -1,0 +1,145 @@
+/*
+ * dynamis.h - C ABI of the MI355X-native rigid-body step library (libdynamis_amd.so).
+ *
+ * The reference runs every stage of `World::step` as WGSL programs dispatched through wgpu
+ * (crates/dynamis-gpu/src/recorder.rs:29-39, `ComputeRecorder::record`). This library replaces
+ * that dispatch layer and the WGSL programs with hand-written gfx950 HIP kernels behind the flat
+ * entry points below. Conventions:
+ *   - every entry point returns an int status: DYN_OK (0) or a negative DYN_E_* code, with a
+ *     message from dyn_last_error(); nothing throws or aborts across the ABI;
+ *   - pointers named d_* or documented as "device" are HIP device pointers; `stream` is a
+ *     hipStream_t passed as void* (NULL = the default stream); work is enqueued asynchronously
+ *     unless the comment says otherwise;
+ *   - record layouts are the reference's (dyn_abi.h).
+ */
+#ifndef DYNAMIS_H
+#define DYNAMIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dyn_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DYN_OK 0
+#define DYN_E_INVALID (-1)   /* an argument violates the documented contract */
+#define DYN_E_HIP (-2)       /* the HIP runtime reported an error */
+#define DYN_E_CAPACITY (-3)  /* a caller-provided buffer is too small */
+#define DYN_E_NODEVICE (-4)  /* no usable gfx950 device */
+
+/* ---- plumbing: device, memory, streams (the reference's GpuContext / GpuBuffer,
+ *      crates/dynamis-gpu/src/context.rs:185, src/buffer.rs:6) -------------------------------- */
+
+/* Library ABI version; bumps when a signature below changes. */
+int dyn_version(void);
+/* Text of the last error raised on the calling thread ("" when none). */
+const char* dyn_last_error(void);
+/* Number of visible HIP devices (0 on a machine without a GPU; never an error then). */
+int dyn_device_count(int* count);
+int dyn_set_device(int ordinal);
+int dyn_malloc(void** d_ptr, size_t bytes);
+int dyn_free(void* d_ptr);
+/* Synchronous copies between host and device memory. */
+int dyn_memcpy_htod(void* d_dst, const void* src, size_t bytes);
+int dyn_memcpy_dtoh(void* dst, const void* d_src, size_t bytes);
+int dyn_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes);
+int dyn_memset(void* d_dst, int value, size_t bytes);
+int dyn_stream_create(void** stream);
+int dyn_stream_destroy(void* stream);
+int dyn_stream_synchronize(void* stream);
+/* Events for timing a region of a stream on that stream's own clock. */
+int dyn_event_create(void** event);
+int dyn_event_destroy(void* event);
+int dyn_event_record(void* event, void* stream);
+int dyn_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---- kernel timing (the reference's `profile` feature: per-pass timestamps,
+ *      crates/dynamis-gpu/src/timing.rs:22-164, crates/dynamis-pass/src/schedule.rs:119-136) ----
+ * While enabled, every kernel the library launches is bracketed by events on its own stream.
+ * dyn_profile_read synchronizes those events and reports, per kernel name, the launch count and
+ * the summed device time; it then forgets them. Disabled by default. */
+int dyn_profile_enable(int on);
+/* Number of distinct kernel names recorded since the last read. */
+int dyn_profile_collect(int* kernels);
+/* Entry i (0 <= i < kernels) of the last collect: name (NUL-terminated, truncated to
+ * name_capacity), launch count and total milliseconds. */
+int dyn_profile_entry(int i, char* name, size_t name_capacity, uint32_t* launches,
+                      double* total_ms);
+
+/* ---- radix sort (crates/dynamis-sort/src/radix.rs) ---------------------------------------- */
+
+/* Replaces `units_for` (radix.rs:63-67): ceil(live/256)+64 clamped to [1, 512]. The sort below
+ * produces the same order for every plan; the value is accepted for interface parity. */
+uint32_t dyn_sort_units_for(uint32_t live);
+/* Replaces `key_words` (radix.rs:71-74): bytes needed to order indices below `elements`. */
+uint32_t dyn_sort_key_words(uint32_t elements);
+/* Scratch bytes dyn_sort_pairs needs for lanes of `capacity` words and `digit_passes` passes. */
+size_t dyn_sort_scratch_bytes(uint32_t capacity, uint32_t digit_passes);
+/*
+ * Replaces `RadixSort::sort` (radix.rs:397-476) with the `SortChannels` contract of
+ * radix.rs:76-85: a stable LSD sort of the records [base, base+len) of three u32 lanes
+ * (major, minor, payload), where len = min(*d_count, capacity - base) and base = *d_base (0 when
+ * d_base is NULL), both read on the device. The order is by the low `major_words` bytes of
+ * major, then the low `minor_words` bytes of minor (minor digits are applied first, then major,
+ * radix.rs:312-317); equal keys keep their input order; all three lanes are permuted and the
+ * result is left in the primaries. `plan` is the reference's unit budget (1..512) and does not
+ * change the result. Errors: major_words + minor_words == 0 (the reference asserts the same,
+ * radix.rs:404-407), words > 4, or scratch smaller than dyn_sort_scratch_bytes.
+ */
+int dyn_sort_pairs(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t* d_payload,
+                   uint32_t capacity, const uint32_t* d_count, const uint32_t* d_base,
+                   uint32_t plan, uint32_t major_words, uint32_t minor_words,
+                   void* d_scratch, size_t scratch_bytes);
+
+/* ---- broadphase (crates/dynamis-broadphase/src/passes.rs, crates/dynamis-rigid/shaders/
+ *      collider_entries.wgsl, crates/dynamis-shader/shaders/grid_index.wgsl) -------------------- */
+
+typedef struct dyn_broadphase_io {
+    /* inputs (device) */
+    const dyn_aabb* d_aabbs;       /* K swept boxes, one per collider */
+    const uint32_t* d_groups;      /* K owner ids; colliders of one owner never pair */
+    const uint32_t* d_flags;       /* K DYN_BP_* words (dyn_abi.h), or NULL for "all mobile
+                                      and awake" */
+    const dyn_aabb* d_resolution;  /* K boxes that set the base cell (the reference's upright
+                                      tight boxes), or NULL to take the base cell from d_aabbs */
+    uint32_t colliders;            /* K */
+    /* outputs (device) */
+    uint32_t* d_entry_keys;        /* sorted cell keys, capacity entry_capacity */
+    uint32_t* d_entry_order;       /* entry slot carried by the sort */
+    dyn_grid_entry* d_entries;     /* entry records indexed by slot */
+    uint32_t entry_capacity;
+    uint32_t* d_pair_major;        /* min collider index of each candidate pair */
+    uint32_t* d_pair_minor;        /* max collider index */
+    uint32_t pair_capacity;
+    uint32_t* d_counters;          /* DYN_COUNTER_BLOCK_WORDS words; ENTRIES, PAIRS,
+                                      REFUSED_PAIRS, ENTRY_FAULTS, GRID_LEVELS, COARSE_ACTIVE,
+                                      GRID_SCALE and GRID_EXTENT are written */
+    uint32_t sort_pairs;           /* non-zero: leave the pairs in canonical (min, max) order, as
+                                      the narrowphase's pair sort does (narrowphase.rs:116-125) */
+    uint32_t _reserved;
+    void* d_scratch;
+    size_t scratch_bytes;
+} dyn_broadphase_io;
+
+/* Scratch bytes dyn_broadphase needs. */
+size_t dyn_broadphase_scratch_bytes(uint32_t colliders, uint32_t entry_capacity,
+                                    uint32_t pair_capacity);
+/*
+ * One broadphase pass over the awake region: grid resolution (broadphase_aabb.wgsl:21-24,
+ * core.wgsl:109-126), entry emission (collider_entries.wgsl:9, collider_emit.wgsl:6-63), the
+ * region sort (passes.rs:76-89: 4 key bytes, payload = slot), coarser-level links
+ * (level_links.wgsl) and same-level cell mates (cell_pairs.wgsl) with the pair filter of
+ * pair_emit.wgsl. Entry slots are assigned in collider order, so equal keys leave the sort in
+ * collider order. Pairs beyond pair_capacity are counted in REFUSED_PAIRS, entries beyond a
+ * collider's 8-cell budget or beyond entry_capacity in ENTRY_FAULTS, as the reference does.
+ */
+int dyn_broadphase(void* stream, const dyn_broadphase_io* io);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DYNAMIS_H */

@@ -1,0 +1,116 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) for tests and the CPU baseline.
+
+The oracle is the checker: tests compare the HIP library against it. It is built from
+oracle/*.c with the Makefile there when the shared object is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+from dynamis_amd import abi
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_SO = ORACLE_DIR / "_build" / "liboracle.so"
+
+_u32 = ctypes.c_uint32
+_p = ctypes.c_void_p
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return ORACLE_SO
+
+
+class Oracle:
+    def __init__(self, handle: ctypes.CDLL):
+        self.h = handle
+        h = handle
+        h.oracle_units_for.restype = _u32
+        h.oracle_units_for.argtypes = [_u32]
+        h.oracle_key_words.restype = _u32
+        h.oracle_key_words.argtypes = [_u32]
+        h.oracle_sort_pairs.restype = ctypes.c_int
+        h.oracle_sort_pairs.argtypes = [_p, _p, _p, _u32, _u32, _u32, _u32, _u32]
+        h.oracle_cell_hash.restype = _u32
+        h.oracle_cell_hash.argtypes = [ctypes.c_int32] * 3
+        h.oracle_cell_key.restype = _u32
+        h.oracle_cell_key.argtypes = [_u32] + [ctypes.c_int32] * 3
+        h.oracle_grid_cell_size.restype = ctypes.c_float
+        h.oracle_grid_cell_size.argtypes = [_u32, _u32]
+        h.oracle_shape_levels.restype = _u32
+        h.oracle_shape_levels.argtypes = [_p, ctypes.c_float]
+        h.oracle_broadphase.restype = _u32
+        h.oracle_broadphase.argtypes = [_p, _p, _p, _p, _u32, _p, _p, _p, _u32, _p, _p, _u32,
+                                        _p, _u32]
+
+    def units_for(self, live: int) -> int:
+        return int(self.h.oracle_units_for(live))
+
+    def key_words(self, elements: int) -> int:
+        return int(self.h.oracle_key_words(elements & 0xFFFFFFFF))
+
+    def sort_pairs(self, major, minor, payload, major_words, minor_words, count=None, base=0,
+                   capacity=None):
+        major = np.array(major, dtype=np.uint32)
+        minor = np.array(minor, dtype=np.uint32)
+        payload = np.array(payload, dtype=np.uint32)
+        cap = len(major) if capacity is None else capacity
+        cnt = len(major) if count is None else count
+        rc = self.h.oracle_sort_pairs(major.ctypes.data, minor.ctypes.data, payload.ctypes.data,
+                                      cap, cnt, base, major_words, minor_words)
+        if rc != 0:
+            raise ValueError("oracle_sort_pairs refused the request")
+        return major, minor, payload
+
+    def cell_hash(self, x, y, z) -> int:
+        return int(self.h.oracle_cell_hash(x, y, z))
+
+    def cell_key(self, level, x, y, z) -> int:
+        return int(self.h.oracle_cell_key(level, x, y, z))
+
+    def broadphase(self, boxes, groups=None, flags=None, resolution=None, entry_capacity=None,
+                   pair_capacity=None, sort_pairs=True):
+        boxes = np.ascontiguousarray(boxes, dtype=abi.AABB)
+        k = len(boxes)
+        ecap = 8 * max(k, 1) if entry_capacity is None else entry_capacity
+        pcap = 8 * max(k, 1) if pair_capacity is None else pair_capacity
+        keys = np.zeros(max(ecap, 1), np.uint32)
+        order = np.zeros(max(ecap, 1), np.uint32)
+        entries = np.zeros(max(ecap, 1), abi.GRID_ENTRY)
+        major = np.zeros(max(pcap, 1), np.uint32)
+        minor = np.zeros(max(pcap, 1), np.uint32)
+        counters = np.zeros(abi.COUNTER_BLOCK_WORDS, np.uint32)
+        g = None if groups is None else np.ascontiguousarray(groups, dtype=np.uint32)
+        f = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint32)
+        r = None if resolution is None else np.ascontiguousarray(resolution, dtype=abi.AABB)
+        live = self.h.oracle_broadphase(
+            boxes.ctypes.data, None if g is None else g.ctypes.data,
+            None if f is None else f.ctypes.data, None if r is None else r.ctypes.data, k,
+            keys.ctypes.data, order.ctypes.data, entries.ctypes.data, ecap, major.ctypes.data,
+            minor.ctypes.data, pcap, counters.ctypes.data, 1 if sort_pairs else 0)
+        pairs = min(abi.counter(counters, "PAIRS"), pcap)
+        return {
+            "entry_keys": keys[:live].copy(),
+            "entry_order": order[:live].copy(),
+            "entries": entries[:min(abi.counter(counters, "ENTRIES"), ecap)].copy(),
+            "pairs": np.stack([major[:pairs], minor[:pairs]], axis=1),
+            "counters": counters,
+        }
+
+
+_oracle: Oracle | None = None
+
+
+def load() -> Oracle:
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            build()
+        _oracle = Oracle(ctypes.CDLL(str(ORACLE_SO)))
+    return _oracle
