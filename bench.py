@@ -39,8 +39,10 @@ def kernel_bytes(name: str, k: int, entries: int, pairs: int) -> float:
     key_pay = 8.0
     if name == "entry_sort_pass":        # read key+slot, write key+slot
         return 2.0 * key_pay * entries
-    if name == "bp_pairs_kernel":        # key, slot, record per entry; pairs written
-        return 40.0 * entries + 8.0 * pairs
+    if name == "bp_mates_kernel":        # every sorted key read once; staged pairs written
+        return 4.0 * entries + 8.0 * pairs
+    if name == "bp_links_kernel":        # box + packed count per collider; pairs written
+        return 28.0 * k + 8.0 * pairs
     if name == "bp_emit_kernel":         # box + packed count in; key, slot, record out
         return 28.0 * k + 40.0 * entries
     if name == "bp_count_kernel":        # box in, packed count out

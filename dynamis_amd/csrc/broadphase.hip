@@ -23,6 +23,7 @@ namespace {
 
 constexpr int kBpThreads = 256;
 constexpr uint32_t kEntryPasses = 4;  // 4 key bytes, passes.rs:88
+constexpr uint32_t kResolutionBlocks = 1024;
 
 __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
     return counters + slot * DYN_COUNTER_STRIDE_WORDS;
@@ -30,46 +31,90 @@ __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
 
 
 
-__global__ void bp_reset_kernel(uint32_t* counters) {
+__global__ void bp_reset_kernel(uint32_t* counters, uint32_t* overflow_n) {
     const uint32_t slots[] = {DYN_COUNTER_ENTRIES,     DYN_COUNTER_PAIRS,
                               DYN_COUNTER_REFUSED_PAIRS, DYN_COUNTER_ENTRY_FAULTS,
                               DYN_COUNTER_GRID_LEVELS, DYN_COUNTER_COARSE_ACTIVE,
                               DYN_COUNTER_GRID_SCALE,  DYN_COUNTER_GRID_EXTENT};
     const uint32_t t = threadIdx.x;
     if (t < sizeof(slots) / sizeof(slots[0])) *ctr(counters, slots[t]) = 0;
+    if (t == 0) *overflow_n = 0;
 }
 
 // broadphase_aabb.wgsl:21-24: the base cell is the smallest resolution box's largest side.
+// Each block reduces a grid-strided share to one (scale, extent) partial; a single workgroup then
+// folds the partials. No two blocks ever update the same word.
 __global__ void __launch_bounds__(kBpThreads) bp_resolution_kernel(const dyn_aabb* __restrict__ boxes,
-                                                                   uint32_t n, uint32_t* counters) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                                   uint32_t n, uint32_t* partials) {
+    __shared__ uint32_t s_red[2][kBpThreads / kWave];
     uint32_t scale = 0, extent = 0;
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 lo = reinterpret_cast<const float4*>(boxes)[2 * i];
         const float4 hi = reinterpret_cast<const float4*>(boxes)[2 * i + 1];
         const float3 mn = make_float3(lo.x, lo.y, lo.z);
         const float3 mx = make_float3(hi.x, hi.y, hi.z);
-        scale = __float_as_uint(grid_scale_of(mn, mx));
-        extent = __float_as_uint(fmaxf(grid_extent_of(mn, mx), 0.0f));
+        scale = max(scale, __float_as_uint(grid_scale_of(mn, mx)));
+        extent = max(extent, __float_as_uint(fmaxf(grid_extent_of(mn, mx), 0.0f)));
     }
     scale = wave_max(scale);
     extent = wave_max(extent);
+    const uint32_t wave = threadIdx.x / kWave;
     if (lane_id() == 0) {
-        atomicMax(ctr(counters, DYN_COUNTER_GRID_SCALE), scale);
-        atomicMax(ctr(counters, DYN_COUNTER_GRID_EXTENT), extent);
+        s_red[0][wave] = scale;
+        s_red[1][wave] = extent;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBpThreads / kWave; ++w) {
+            scale = max(scale, s_red[0][w]);
+            extent = max(extent, s_red[1][w]);
+        }
+        partials[2 * blockIdx.x] = scale;
+        partials[2 * blockIdx.x + 1] = extent;
     }
 }
 
-// Per collider: level, cell count, emitted entries (<= 8), faults, coarse activity, levels.
+// Folds the resolution partials into GRID_SCALE / GRID_EXTENT (both are float bits of
+// non-negative values, so the integer maximum is the float maximum).
+__global__ void __launch_bounds__(1024) bp_resolution_fold_kernel(const uint32_t* __restrict__ partials,
+                                                                  uint32_t blocks, uint32_t* counters) {
+    __shared__ uint32_t s_red[2][1024 / kWave];
+    uint32_t scale = 0, extent = 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+        scale = max(scale, partials[2 * b]);
+        extent = max(extent, partials[2 * b + 1]);
+    }
+    scale = wave_max(scale);
+    extent = wave_max(extent);
+    const uint32_t wave = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        s_red[0][wave] = scale;
+        s_red[1][wave] = extent;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 1024 / kWave; ++w) {
+            scale = max(scale, s_red[0][w]);
+            extent = max(extent, s_red[1][w]);
+        }
+        *ctr(counters, DYN_COUNTER_GRID_SCALE) = scale;
+        *ctr(counters, DYN_COUNTER_GRID_EXTENT) = extent;
+    }
+}
+
+// Per collider: level, cell count, emitted entries (<= 8). Per block: the entry total and the
+// block's share of the level mask, cell faults and coarse activity (folded by bp_scan_kernel).
 __global__ void __launch_bounds__(kBpThreads) bp_count_kernel(const dyn_aabb* __restrict__ aabbs,
                                                               const uint32_t* __restrict__ flags,
-                                                              uint32_t n, uint32_t* counters,
+                                                              uint32_t n, const uint32_t* counters,
                                                               uint32_t* cnt_level,
-                                                              uint32_t* block_totals) {
+                                                              uint32_t* block_totals,
+                                                              uint32_t* block_stats) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
+    __shared__ uint32_t s_stats[3][kBpThreads / kWave];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float base = grid_cell_size(*ctr(counters, DYN_COUNTER_GRID_SCALE),
-                                      *ctr(counters, DYN_COUNTER_GRID_EXTENT));
+    const float base = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
+                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
     uint32_t emitted = 0, levels = 0, faults = 0, coarse = 0;
     if (i < n) {
         const float4 lo = reinterpret_cast<const float4*>(aabbs)[2 * i];
@@ -89,26 +134,63 @@ __global__ void __launch_bounds__(kBpThreads) bp_count_kernel(const dyn_aabb* __
     levels = wave_or(levels);
     faults = wave_sum(faults);
     coarse = wave_sum(coarse);
+    const uint32_t wave = threadIdx.x / kWave;
     if (lane_id() == 0) {
-        if (levels) atomicOr(ctr(counters, DYN_COUNTER_GRID_LEVELS), levels);
-        if (faults) atomicAdd(ctr(counters, DYN_COUNTER_ENTRY_FAULTS), faults);
-        if (coarse) atomicAdd(ctr(counters, DYN_COUNTER_COARSE_ACTIVE), coarse);
+        s_stats[0][wave] = levels;
+        s_stats[1][wave] = faults;
+        s_stats[2][wave] = coarse;
     }
     uint32_t total = 0;
     block_exclusive_sum<kBpThreads>(emitted, s_scan, &total);
-    if (threadIdx.x == 0) block_totals[blockIdx.x] = total;
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBpThreads / kWave; ++w) {
+            levels |= s_stats[0][w];
+            faults += s_stats[1][w];
+            coarse += s_stats[2][w];
+        }
+        block_totals[blockIdx.x] = total;
+        block_stats[3 * blockIdx.x] = levels;
+        block_stats[3 * blockIdx.x + 1] = faults;
+        block_stats[3 * blockIdx.x + 2] = coarse;
+    }
 }
 
 // One workgroup: exclusive scan of the block totals; entry count; sort control words.
 __global__ void __launch_bounds__(1024) bp_scan_kernel(const uint32_t* __restrict__ block_totals,
+                                                       const uint32_t* __restrict__ block_stats,
                                                        uint32_t blocks, uint32_t* block_offsets,
                                                        uint32_t* counters, uint32_t* sort_hist,
                                                        uint32_t* sort_ctl,
                                                        uint32_t entry_capacity) {
     __shared__ uint32_t s_scan[1024 / kWave];
+    __shared__ uint32_t s_stats[3][1024 / kWave];
     __shared__ uint32_t s_carry;
     if (threadIdx.x == 0) s_carry = 0;
+    uint32_t levels = 0, faults = 0, coarse = 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+        levels |= block_stats[3 * b];
+        faults += block_stats[3 * b + 1];
+        coarse += block_stats[3 * b + 2];
+    }
+    levels = wave_or(levels);
+    faults = wave_sum(faults);
+    coarse = wave_sum(coarse);
+    if (lane_id() == 0) {
+        s_stats[0][threadIdx.x / kWave] = levels;
+        s_stats[1][threadIdx.x / kWave] = faults;
+        s_stats[2][threadIdx.x / kWave] = coarse;
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 1024 / kWave; ++w) {
+            levels |= s_stats[0][w];
+            faults += s_stats[1][w];
+            coarse += s_stats[2][w];
+        }
+        *ctr(counters, DYN_COUNTER_GRID_LEVELS) = levels;
+        *ctr(counters, DYN_COUNTER_ENTRY_FAULTS) = faults;
+        *ctr(counters, DYN_COUNTER_COARSE_ACTIVE) = coarse;
+    }
     for (uint32_t first = 0; first < blocks; first += 1024) {
         const uint32_t b = first + threadIdx.x;
         const uint32_t v = b < blocks ? block_totals[b] : 0u;
@@ -243,24 +325,62 @@ __device__ __forceinline__ void key_run(const uint32_t* __restrict__ keys,
     *end = e;
 }
 
-__device__ __forceinline__ void emit_pair(uint32_t* counters, uint32_t* pair_major,
-                                          uint32_t* pair_minor, uint32_t pair_capacity,
-                                          uint32_t group_a, uint32_t info_a, uint32_t group_b,
-                                          uint32_t info_b) {
+// Candidate pairs are staged per workgroup in LDS (pair_emit.wgsl:10-31 appends each pair with a
+// device atomic on one counter; here the only atomics are on LDS). A workgroup's staged pairs
+// go to its own region of a staging array at the end of the kernel; bp_pairs_scan_kernel and
+// bp_pairs_copy_kernel then lay the regions out back to back. Pairs beyond a workgroup's LDS
+// room take a global overflow list.
+constexpr uint32_t kPairGroups = 1024;        // persistent workgroups of the cell-mate kernel
+constexpr uint32_t kLinkGroups = 4096;        // most workgroups of the level-link kernel
+constexpr uint32_t kStageCapacity = 512;      // pairs staged per workgroup
+
+struct PairStage {
+    uint32_t n;
+    uint32_t major[kStageCapacity];
+    uint32_t minor[kStageCapacity];
+};
+
+struct PairSink {
+    uint32_t* staged;      // (kPairGroups + kLinkGroups) * kStageCapacity * 2 words
+    uint32_t* counts;      // staged count of every region
+    uint32_t* overflow;    // overflow pairs, 2 words each
+    uint32_t* overflow_n;  // overflow count
+    uint32_t overflow_capacity;
+};
+
+__device__ __forceinline__ void stage_pair(PairStage& stage, const PairSink& sink,
+                                           uint32_t group_a, uint32_t info_a, uint32_t group_b,
+                                           uint32_t info_b) {
     // pair_emit.wgsl:10-31 (node identity is excluded by the callers).
     if (((info_a & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (((info_b & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (group_a == group_b) return;
     if (!(info_a & DYN_ENTRY_MOBILE) && !(info_b & DYN_ENTRY_MOBILE)) return;
-    const uint32_t slot = atomicAdd(ctr(counters, DYN_COUNTER_PAIRS), 1u);
-    if (slot < pair_capacity) {
-        const uint32_t a = info_a & DYN_ENTRY_INDEX_MASK;
-        const uint32_t b = info_b & DYN_ENTRY_INDEX_MASK;
-        pair_minor[slot] = max(a, b);
-        pair_major[slot] = min(a, b);
-    } else {
-        atomicAdd(ctr(counters, DYN_COUNTER_REFUSED_PAIRS), 1u);
+    const uint32_t a = info_a & DYN_ENTRY_INDEX_MASK;
+    const uint32_t b = info_b & DYN_ENTRY_INDEX_MASK;
+    const uint32_t slot = atomicAdd(&stage.n, 1u);
+    if (slot < kStageCapacity) {
+        stage.major[slot] = min(a, b);
+        stage.minor[slot] = max(a, b);
+        return;
     }
+    const uint32_t o = atomicAdd(sink.overflow_n, 1u);
+    if (o < sink.overflow_capacity) {
+        sink.overflow[2 * o] = min(a, b);
+        sink.overflow[2 * o + 1] = max(a, b);
+    }
+}
+
+// Writes a workgroup's staged pairs to its region (called by every thread after a barrier).
+__device__ __forceinline__ void flush_stage(const PairStage& stage, const PairSink& sink,
+                                            uint32_t region) {
+    const uint32_t n = min(stage.n, kStageCapacity);
+    uint32_t* out = sink.staged + static_cast<size_t>(region) * kStageCapacity * 2;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        out[2 * j] = stage.major[j];
+        out[2 * j + 1] = stage.minor[j];
+    }
+    if (threadIdx.x == 0) sink.counts[region] = n;
 }
 
 struct EntryView {
@@ -299,74 +419,196 @@ __device__ __forceinline__ bool mate_in_cell(const EntryView& a, const EntryView
     return same3(mc, cell);
 }
 
-// One thread per sorted entry: cell mates (cell_pairs.wgsl:20-50) when awake, coarser-level
-// links (level_links.wgsl:43-65) when primary.
-__global__ void __launch_bounds__(256) bp_pairs_kernel(
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
-    const dyn_grid_entry* __restrict__ entries, const uint32_t* __restrict__ dir,
-    uint32_t* counters, const uint32_t* sort_ctl, uint32_t* pair_major, uint32_t* pair_minor,
-    uint32_t pair_capacity) {
-    const uint32_t len = sort_ctl[kCtlLen];
-    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
-    if (index >= len) return;
+// Same-level cell mates (cell_pairs.wgsl:20-50) of one sorted entry. An entry's own key is the
+// key of the cell it searches, so its run is the block of equal keys around it; most entries have
+// no equal neighbour and finish after reading three coalesced keys.
+__device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
+                                           const uint32_t* __restrict__ keys,
+                                           const uint32_t* __restrict__ order,
+                                           const dyn_grid_entry* __restrict__ entries, float grid,
+                                           PairStage& stage, const PairSink& sink) {
     const uint32_t key = keys[index];
-    const uint32_t node = order[index];
-    const EntryView self = load_entry(entries, node);
+    const bool before = index > 0 && keys[index - 1] == key;
+    const bool after = index + 1 < len && keys[index + 1] == key;
+    if (!before && !after) return;
+    const EntryView self = load_entry(entries, order[index]);
     if (((self.info & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
-    const float grid = grid_cell_size(*ctr(counters, DYN_COUNTER_GRID_SCALE),
-                                      *ctr(counters, DYN_COUNTER_GRID_EXTENT));
-    const bool awake = (self.info & DYN_ENTRY_AWAKE) != 0u;
+    if (!(self.info & DYN_ENTRY_AWAKE)) return;
     // The entry's level is the level bits of its own key (emitted from shape_levels of this box).
-    const uint32_t level = key >> DYN_LEVEL_KEY_SHIFT;
+    const float cs = level_cell_size(key >> DYN_LEVEL_KEY_SHIFT, grid);
+    int32_t cell[3];
+    entry_cell(self.mn, self.info, cs, cell);
+    // Earlier mates in the run: only entries that are not awake pair from this side.
+    for (uint32_t e = index; e-- > 0 && keys[e] == key;) {
+        const EntryView o = load_entry(entries, order[e]);
+        if (o.info & DYN_ENTRY_AWAKE) continue;
+        if (mate_in_cell(self, o, cs, cell))
+            stage_pair(stage, sink, self.group, self.info, o.group, o.info);
+    }
+    for (uint32_t e = index + 1; e < len && keys[e] == key; ++e) {
+        const EntryView o = load_entry(entries, order[e]);
+        if (mate_in_cell(self, o, cs, cell))
+            stage_pair(stage, sink, self.group, self.info, o.group, o.info);
+    }
+}
 
-    if (awake) {
-        const float cs = level_cell_size(level, grid);
-        int32_t cell[3];
-        entry_cell(self.mn, self.info, cs, cell);
-        // Earlier mates in the run: only entries that are not awake pair from this side.
-        for (uint32_t e = index; e-- > 0 && keys[e] == key;) {
-            const uint32_t other = order[e];
-            const EntryView o = load_entry(entries, other);
-            if (o.info & DYN_ENTRY_AWAKE) continue;
-            if (mate_in_cell(self, o, cs, cell))
-                emit_pair(counters, pair_major, pair_minor, pair_capacity, self.group, self.info,
-                          o.group, o.info);
-        }
-        for (uint32_t e = index + 1; e < len && keys[e] == key; ++e) {
-            const uint32_t other = order[e];
-            const EntryView o = load_entry(entries, other);
-            if (mate_in_cell(self, o, cs, cell))
-                emit_pair(counters, pair_major, pair_minor, pair_capacity, self.group, self.info,
-                          o.group, o.info);
+__global__ void __launch_bounds__(256) bp_mates_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+    const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
+    const uint32_t* sort_ctl, PairSink sink) {
+    __shared__ PairStage stage;
+    if (threadIdx.x == 0) stage.n = 0;
+    __syncthreads();
+    const uint32_t len = sort_ctl[kCtlLen];
+    const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
+                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
+    for (uint32_t index = blockIdx.x * blockDim.x + threadIdx.x; index < len;
+         index += gridDim.x * blockDim.x) {
+        cell_mates(index, len, keys, order, entries, grid, stage, sink);
+    }
+    __syncthreads();
+    flush_stage(stage, sink, blockIdx.x);
+}
+
+// Coarser-level links (level_links.wgsl:43-65) of one collider. Every collider's primary entry
+// (cell offset 0) is its first slot, so the probe starts from the collider's own box, flags and
+// level instead of from the sorted array; a primary beyond capacity probes nothing, as in the
+// reference.
+__device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_t slot,
+                                            const dyn_aabb* __restrict__ aabbs,
+                                            const uint32_t* __restrict__ groups,
+                                            const uint32_t* __restrict__ flags,
+                                            const uint32_t* __restrict__ keys,
+                                            const uint32_t* __restrict__ order,
+                                            const dyn_grid_entry* __restrict__ entries,
+                                            const uint32_t* __restrict__ dir, bool coarse_active,
+                                            uint32_t occupied, float grid, uint32_t capacity,
+                                            PairStage& stage, const PairSink& sink) {
+    const uint32_t emitted = packed & 0xFFu;
+    if (emitted == 0u || slot >= capacity) return;
+    const uint32_t f = flags ? flags[i] : (DYN_BP_MOBILE | DYN_BP_AWAKE);
+    const bool awake = (f & DYN_BP_AWAKE) != 0u;
+    if (!awake && !coarse_active) return;
+    const uint32_t level = packed >> 8;
+    uint32_t coarser = occupied & ~((1u << min(level + 1u, 31u)) - 1u);
+    if (coarser == 0u) return;
+    EntryView self;
+    const float4 lo = reinterpret_cast<const float4*>(aabbs)[2 * i];
+    const float4 hi = reinterpret_cast<const float4*>(aabbs)[2 * i + 1];
+    self.mn = make_float3(lo.x, lo.y, lo.z);
+    self.mx = make_float3(hi.x, hi.y, hi.z);
+    self.group = groups ? groups[i] : i;
+    self.info = i | (DYN_ENTRY_KIND_COLLIDER << DYN_ENTRY_KIND_SHIFT) | DYN_ENTRY_PRIMARY;
+    if (awake) self.info |= DYN_ENTRY_AWAKE;
+    if (f & DYN_BP_MOBILE) self.info |= DYN_ENTRY_MOBILE;
+    while (coarser != 0u) {
+        const uint32_t link = 31u - static_cast<uint32_t>(__clz(coarser));
+        coarser &= ~(1u << link);
+        const float cs = level_cell_size(link, grid);
+        const Cells cells = grid_cells(self.mn, self.mx, cs);
+        const uint32_t count = cell_count(cells);
+        for (uint32_t ordinal = 0; ordinal < count; ++ordinal) {
+            int32_t cell[3];
+            cell_at(cells, ordinal, cell);
+            uint32_t first, end;
+            key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]), &first, &end);
+            for (uint32_t e = first; e < end; ++e) {
+                const uint32_t other = order[e];
+                if (other == slot) continue;
+                const EntryView o = load_entry(entries, other);
+                if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
+                if (mate_in_cell(self, o, cs, cell))
+                    stage_pair(stage, sink, self.group, self.info, o.group, o.info);
+            }
         }
     }
+}
 
-    if (self.info & DYN_ENTRY_PRIMARY) {
-        if (!awake && *ctr(counters, DYN_COUNTER_COARSE_ACTIVE) == 0u) return;
-        const uint32_t occupied = *ctr(counters, DYN_COUNTER_GRID_LEVELS);
-        const uint32_t shift = min(level + 1u, 31u);
-        uint32_t coarser = occupied & ~((1u << shift) - 1u);
-        while (coarser != 0u) {
-            const uint32_t link = 31u - static_cast<uint32_t>(__clz(coarser));
-            coarser &= ~(1u << link);
-            const float cs = level_cell_size(link, grid);
-            const Cells cells = grid_cells(self.mn, self.mx, cs);
-            const uint32_t count = cell_count(cells);
-            for (uint32_t ordinal = 0; ordinal < count; ++ordinal) {
-                int32_t cell[3];
-                cell_at(cells, ordinal, cell);
-                uint32_t first, end;
-                key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]), &first, &end);
-                for (uint32_t e = first; e < end; ++e) {
-                    const uint32_t other = order[e];
-                    if (other == node) continue;
-                    const EntryView o = load_entry(entries, other);
-                    if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
-                    if (mate_in_cell(self, o, cs, cell))
-                        emit_pair(counters, pair_major, pair_minor, pair_capacity, self.group,
-                                  self.info, o.group, o.info);
-                }
+__global__ void __launch_bounds__(kBpThreads) bp_links_kernel(
+    const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
+    const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
+    const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
+    const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
+    PairSink sink) {
+    __shared__ PairStage stage;
+    __shared__ uint32_t s_scan[kBpThreads / kWave];
+    if (threadIdx.x == 0) stage.n = 0;
+    __syncthreads();
+    const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
+    const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
+    const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
+                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
+    const uint32_t chunks = ceil_div(n, kBpThreads);
+    // Chunks of 256 colliders are the count kernel's blocks, so block_offsets applies per chunk.
+    for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        const uint32_t i = chunk * kBpThreads + threadIdx.x;
+        const uint32_t packed = i < n ? cnt_level[i] : 0u;
+        const uint32_t slot = block_offsets[chunk] +
+                              block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
+        if (i < n) {
+            level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir,
+                        coarse_active, occupied, grid, capacity, stage, sink);
+        }
+    }
+    __syncthreads();
+    flush_stage(stage, sink, kPairGroups + blockIdx.x);
+}
+
+// One workgroup: offsets of the staged regions, the pair count and the refused count.
+__global__ void __launch_bounds__(1024) bp_pairs_scan_kernel(const uint32_t* __restrict__ counts,
+                                                             uint32_t regions,
+                                                             uint32_t* region_offsets,
+                                                             const uint32_t* overflow_n,
+                                                             uint32_t* counters,
+                                                             uint32_t pair_capacity) {
+    __shared__ uint32_t s_scan[1024 / kWave];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    for (uint32_t first = 0; first < regions; first += 1024) {
+        const uint32_t r = first + threadIdx.x;
+        const uint32_t v = r < regions ? counts[r] : 0u;
+        uint32_t total = 0;
+        const uint32_t excl = block_exclusive_sum<1024>(v, s_scan, &total);
+        if (r < regions) region_offsets[r] = s_carry + excl;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t staged = s_carry;
+        const uint32_t total = staged + overflow_n[0];
+        region_offsets[regions] = staged;
+        counters[DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS] = total;
+        counters[DYN_COUNTER_REFUSED_PAIRS * DYN_COUNTER_STRIDE_WORDS] =
+            total > pair_capacity ? total - pair_capacity : 0u;
+    }
+}
+
+// Lays the staged regions (then the overflow list) out back to back in the pair stream.
+__global__ void __launch_bounds__(256) bp_pairs_copy_kernel(PairSink sink,
+                                                            const uint32_t* __restrict__ region_offsets,
+                                                            uint32_t regions, uint32_t* pair_major,
+                                                            uint32_t* pair_minor,
+                                                            uint32_t pair_capacity) {
+    for (uint32_t r = blockIdx.x; r < regions; r += gridDim.x) {
+        const uint32_t n = sink.counts[r];
+        const uint32_t at = region_offsets[r];
+        const uint32_t* in = sink.staged + static_cast<size_t>(r) * kStageCapacity * 2;
+        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+            if (at + j < pair_capacity) {
+                pair_major[at + j] = in[2 * j];
+                pair_minor[at + j] = in[2 * j + 1];
             }
+        }
+    }
+    const uint32_t staged = region_offsets[regions];
+    const uint32_t extra = min(sink.overflow_n[0], sink.overflow_capacity);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < extra;
+         j += gridDim.x * blockDim.x) {
+        if (staged + j < pair_capacity) {
+            pair_major[staged + j] = sink.overflow[2 * j];
+            pair_minor[staged + j] = sink.overflow[2 * j + 1];
         }
     }
 }
@@ -374,7 +616,14 @@ __global__ void __launch_bounds__(256) bp_pairs_kernel(
 struct BpLayout {
     uint32_t* cnt_level;
     uint32_t* block_totals;
+    uint32_t* block_stats;
     uint32_t* block_offsets;
+    uint32_t* res_partials;
+    uint32_t* staged;
+    uint32_t* stage_counts;
+    uint32_t* region_offsets;
+    uint32_t* overflow;
+    uint32_t* overflow_n;
     uint32_t* dir;
     uint32_t* pair_payload;
     SortScratch entry_sort;
@@ -389,7 +638,15 @@ BpLayout bp_layout(uint32_t colliders, uint32_t entry_capacity, uint32_t pair_ca
     const uint32_t blocks = ceil_div(colliders ? colliders : 1u, kBpThreads);
     l.cnt_level = carve.take<uint32_t>(colliders ? colliders : 1u);
     l.block_totals = carve.take<uint32_t>(blocks);
+    l.block_stats = carve.take<uint32_t>(3 * static_cast<size_t>(blocks));
     l.block_offsets = carve.take<uint32_t>(blocks);
+    l.res_partials = carve.take<uint32_t>(2 * kResolutionBlocks);
+    l.staged = carve.take<uint32_t>(static_cast<size_t>(kPairGroups + kLinkGroups) *
+                                    kStageCapacity * 2);
+    l.stage_counts = carve.take<uint32_t>(kPairGroups + kLinkGroups);
+    l.region_offsets = carve.take<uint32_t>(kPairGroups + kLinkGroups + 1);
+    l.overflow = carve.take<uint32_t>(2 * static_cast<size_t>(pair_capacity ? pair_capacity : 1u));
+    l.overflow_n = carve.take<uint32_t>(1);
     l.dir = carve.take<uint32_t>(static_cast<size_t>(kGridLevels) * kDirStride);
     l.pair_payload = carve.take<uint32_t>(pair_capacity ? pair_capacity : 1u);
     const uint32_t pair_passes = 2u * dyn_sort_key_words(colliders ? colliders : 1u);
@@ -437,7 +694,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     uint32_t* counters = io->d_counters;
     {
         ProfileScope scope("bp_reset_kernel", st);
-        bp_reset_kernel<<<1, 64, 0, st>>>(counters);
+        bp_reset_kernel<<<1, 64, 0, st>>>(counters, l.overflow_n);
     }
     DYN_LAUNCHED("bp_reset_kernel");
     const uint32_t blocks = ceil_div(n ? n : 1u, kBpThreads);
@@ -445,19 +702,28 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
         const dyn_aabb* res = io->d_resolution ? io->d_resolution : io->d_aabbs;
         {
             ProfileScope scope("bp_resolution_kernel", st);
-            bp_resolution_kernel<<<blocks, kBpThreads, 0, st>>>(res, n, counters);
+            bp_resolution_kernel<<<min(blocks, kResolutionBlocks), kBpThreads, 0, st>>>(
+                res, n, l.res_partials);
         }
         DYN_LAUNCHED("bp_resolution_kernel");
         {
+            ProfileScope scope("bp_resolution_fold_kernel", st);
+            bp_resolution_fold_kernel<<<1, 1024, 0, st>>>(l.res_partials,
+                                                          min(blocks, kResolutionBlocks), counters);
+        }
+        DYN_LAUNCHED("bp_resolution_fold_kernel");
+        {
             ProfileScope scope("bp_count_kernel", st);
             bp_count_kernel<<<blocks, kBpThreads, 0, st>>>(io->d_aabbs, io->d_flags, n, counters,
-                                                           l.cnt_level, l.block_totals);
+                                                           l.cnt_level, l.block_totals,
+                                                           l.block_stats);
         }
         DYN_LAUNCHED("bp_count_kernel");
     }
     {
         ProfileScope scope("bp_scan_kernel", st);
-        bp_scan_kernel<<<1, 1024, 0, st>>>(l.block_totals, n ? blocks : 0u, l.block_offsets,
+        bp_scan_kernel<<<1, 1024, 0, st>>>(l.block_totals, l.block_stats, n ? blocks : 0u,
+                                           l.block_offsets,
                                            counters, l.entry_sort.hist, l.entry_sort.ctl,
                                            io->entry_capacity);
     }
@@ -482,13 +748,38 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
                                                                         l.entry_sort.ctl, l.dir);
     }
     DYN_LAUNCHED("bp_directory_kernel");
+    const PairSink sink{l.staged, l.stage_counts, l.overflow, l.overflow_n, io->pair_capacity};
+    const uint32_t link_groups = min(blocks, kLinkGroups);
+    const uint32_t regions = kPairGroups + link_groups;
     {
-        ProfileScope scope("bp_pairs_kernel", st);
-        bp_pairs_kernel<<<ceil_div(io->entry_capacity, 256), 256, 0, st>>>(
-            io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters, l.entry_sort.ctl,
-            io->d_pair_major, io->d_pair_minor, io->pair_capacity);
+        ProfileScope scope("bp_links_kernel", st);
+        bp_links_kernel<<<link_groups, kBpThreads, 0, st>>>(
+            io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
+            io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
+            io->entry_capacity, sink);
     }
-    DYN_LAUNCHED("bp_pairs_kernel");
+    DYN_LAUNCHED("bp_links_kernel");
+    {
+        ProfileScope scope("bp_mates_kernel", st);
+        bp_mates_kernel<<<kPairGroups, 256, 0, st>>>(io->d_entry_keys, io->d_entry_order,
+                                                     io->d_entries, counters, l.entry_sort.ctl,
+                                                     sink);
+    }
+    DYN_LAUNCHED("bp_mates_kernel");
+    {
+        ProfileScope scope("bp_pairs_scan_kernel", st);
+        bp_pairs_scan_kernel<<<1, 1024, 0, st>>>(l.stage_counts, regions,
+                                                 l.region_offsets, l.overflow_n, counters,
+                                                 io->pair_capacity);
+    }
+    DYN_LAUNCHED("bp_pairs_scan_kernel");
+    {
+        ProfileScope scope("bp_pairs_copy_kernel", st);
+        bp_pairs_copy_kernel<<<kPairGroups, 256, 0, st>>>(sink, l.region_offsets,
+                                                         regions, io->d_pair_major,
+                                                         io->d_pair_minor, io->pair_capacity);
+    }
+    DYN_LAUNCHED("bp_pairs_copy_kernel");
     if (io->sort_pairs && io->pair_capacity) {
         const uint32_t words = dyn_sort_key_words(n);
         sort_digits(words, words, digits);

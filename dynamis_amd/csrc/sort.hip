@@ -104,9 +104,7 @@ __global__ void __launch_bounds__(256) sort_pass_kernel(
     const uint32_t ntiles = ceil_div(len, TILE);
     const uint64_t below = (1ull << lane) - 1ull;
 
-    // Global start of every digit: exclusive scan of this pass's histogram.
-    s_digit_base[t] = block_exclusive_sum<kSortThreads>(hist_pass[t], s_scan, nullptr);
-
+    bool based = false;
     for (;;) {
         if (t == 0) s_tile = atomicAdd(&ctl[kCtlTiles + pass], 1u);
 #pragma unroll
@@ -114,6 +112,12 @@ __global__ void __launch_bounds__(256) sort_pass_kernel(
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= ntiles) break;
+        if (!based) {
+            // Global start of every digit: exclusive scan of this pass's histogram (only
+            // workgroups that hold a tile need it).
+            s_digit_base[t] = block_exclusive_sum<kSortThreads>(hist_pass[t], s_scan, nullptr);
+            based = true;
+        }
         const uint32_t first = tile * TILE;
         const uint32_t tile_n = min(static_cast<uint32_t>(TILE), len - first);
         if (flags_next) flags_next[tile * kSortBins + t] = 0;
@@ -309,7 +313,8 @@ static int launch_passes(hipStream_t stream, uint32_t* major, uint32_t* minor, u
                          uint32_t capacity, const SortNames& names) {
     constexpr int TILE = kSortThreads * ITEMS;
     const uint32_t tiles = ceil_div(capacity ? capacity : 1u, TILE);
-    const uint32_t groups = tiles < 2048u ? tiles : 2048u;
+    // Persistent workgroups take tiles in order; about two per CU keep every CU busy.
+    const uint32_t groups = tiles < 1024u ? tiles : 1024u;
     // The look-back words are laid out for the narrowest tiling, so every tiling fits.
     const size_t flag_stride = static_cast<size_t>(s.max_tiles) * kSortBins;
     if (static_cast<size_t>(tiles) * kSortBins > flag_stride) {
