@@ -1,0 +1,45 @@
+/*
+ * ostep.h - CPU restatement of one rigid-body World::step. TEST INFRASTRUCTURE ONLY (see
+ * oracle.h). Serial FP32, one pass after another in the reference's pass order
+ * (crates/dynamis-world/tests/suite/pipeline.rs:6-49) for a world without joints, CCD, soft
+ * bodies, characters, vehicles, fields or command edits.
+ */
+#ifndef DYN_OSTEP_H
+#define DYN_OSTEP_H
+
+#include <stdint.h>
+
+#include "../include/dyn_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_world oracle_world;
+
+/* Bodies occupy rows [0, body_count); rows [0, dynamic_count) are the simulated ones
+ * (crates/dynamis-world/src/body.rs:292-321). Capacities bound the per-step streams. */
+oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_state* states,
+                                  const dyn_body_desc* descs, uint32_t body_count,
+                                  uint32_t dynamic_count, const dyn_collider* colliders,
+                                  const uint32_t* owners, uint32_t collider_count,
+                                  uint32_t contact_capacity);
+void oracle_world_destroy(oracle_world* w);
+/* One step with the given dt (params.dt / substep_dt are set from it). */
+void oracle_world_step(oracle_world* w, float dt);
+void oracle_world_states(const oracle_world* w, dyn_body_state* out);
+/* Contacts of the last step (after the solve), in canonical pair order; returns the count. */
+uint32_t oracle_world_contacts(const oracle_world* w, dyn_contact* out, uint32_t capacity);
+void oracle_world_counters(const oracle_world* w, uint32_t* out);
+/* Sorted candidate pairs of the last step; returns the count. */
+uint32_t oracle_world_pairs(const oracle_world* w, uint32_t* major, uint32_t* minor,
+                            uint32_t capacity);
+
+/* Host record encoding (crates/dynamis-model/src/mass.rs, crates/dynamis-abi/src/body.rs). */
+void oracle_inertia_inverse(const float* inertia, float* inverse);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
