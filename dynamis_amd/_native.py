@@ -48,6 +48,20 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
                                c_void_p, c_u32, c_u32, c_u32, c_void_p, c_size_t]),
     "dyn_broadphase_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),
     "dyn_broadphase": (c_int, [c_void_p, c_void_p]),
+    "dyn_config_default": (None, [c_void_p]),
+    "dyn_step_params_build": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p]),
+    "dyn_inertia_inverse": (c_int, [c_void_p, c_void_p]),
+    "dyn_encode_body": (c_int, [c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_void_p]),
+    "dyn_world_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_u32, c_u32, c_void_p, c_void_p, c_u32, c_u32]),
+    "dyn_world_destroy": (c_int, [c_void_p]),
+    "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
+    "dyn_world_wait": (c_int, [c_void_p]),
+    "dyn_world_stream": (c_void_p, [c_void_p]),
+    "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),
+    "dyn_world_read_counters": (c_int, [c_void_p, c_void_p]),
+    "dyn_world_read_contacts": (c_int, [c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    "dyn_world_read_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
 }
 
 

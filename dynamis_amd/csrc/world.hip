@@ -1,0 +1,1506 @@
+// One rigid-body World::step on gfx950: the pass sequence of the reference
+// (crates/dynamis-world/tests/suite/pipeline.rs:6-49) for worlds without joints, CCD, soft bodies,
+// characters, vehicles, fields or command edits, as a fixed sequence of HIP kernels on the
+// world's stream.
+//
+// Device choices that keep the reference's results:
+//  - every collider is emitted into one grid region per step with its owner's current awake
+//    flag; the pair set equals the reference's (overlapping boxes with an awake side);
+//  - per-step appends (contacts, resting manifolds) use prefix sums, never a shared atomic
+//    counter, so their order is the canonical one;
+//  - solver rows are all rows a resolving contact touches; a row whose accumulated words are
+//    zero is left alone, exactly as the reference's apply does. Inert (static, kinematic or
+//    sleeping) bodies receive exactly zero deltas, so their words are never touched, and their
+//    claim / contribution marks are plain stores rather than counted atomics (their counts are
+//    never read: the mass split of an inert body is frozen to zero);
+//  - the resting archive is kept sorted by (min body id, max body id) for the contact_begin
+//    lookup (the reference sorts an index over a free-list archive, commit.rs:278-295).
+#include <vector>
+
+#include "narrow.h"
+#include "sort.h"
+
+namespace dyn {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kIslandWake = 1u;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr float kHalfTurn = 3.141592653589793f;
+
+__device__ __forceinline__ uint32_t* cw(uint32_t* counters, uint32_t slot) {
+    return counters + slot * DYN_COUNTER_STRIDE_WORDS;
+}
+
+// counter.rs:75-128: slots cleared every step (reset_counters.wgsl)
+__constant__ uint32_t kPerStepSlots[] = {
+    DYN_COUNTER_ENTRIES, DYN_COUNTER_RESTING_REBUILD, DYN_COUNTER_RESTING_SORT,
+    DYN_COUNTER_IMMOVABLE_EMITTED, DYN_COUNTER_PAIRS, DYN_COUNTER_GRID_LEVELS,
+    DYN_COUNTER_CONTACTS, DYN_COUNTER_JOINTS, DYN_COUNTER_EVENTS, DYN_COUNTER_REFUSED_PAIRS,
+    DYN_COUNTER_REFUSED_EVENTS, DYN_COUNTER_ENTRY_FAULTS, DYN_COUNTER_REFUSED_RESTING,
+    DYN_COUNTER_SLEPT, DYN_COUNTER_WOKE, DYN_COUNTER_ACTIVE, DYN_COUNTER_WOKE_DEFERRED,
+    DYN_COUNTER_RESTING_GATHER, DYN_COUNTER_ISLANDS, DYN_COUNTER_ISLAND_FAULTS,
+    DYN_COUNTER_SOLVE_LINEAR_RESIDUAL, DYN_COUNTER_SOLVE_ANGULAR_RESIDUAL,
+    DYN_COUNTER_COARSE_ACTIVE, DYN_COUNTER_GRID_SCALE, DYN_COUNTER_GRID_EXTENT,
+    DYN_COUNTER_BREAKS, DYN_COUNTER_PARTICLE_REACH, DYN_COUNTER_COARSE_NEIGHBOURS,
+    DYN_COUNTER_LIVE, DYN_COUNTER_LIVE_FAULTS, DYN_COUNTER_ROW_FAULTS, DYN_COUNTER_SOFT_ACTIVE,
+    DYN_COUNTER_SOFT_SLEPT, DYN_COUNTER_SOFT_WOKE, DYN_COUNTER_REFUSED_CONTACTS,
+    DYN_COUNTER_IMPACTS, DYN_COUNTER_REFUSED_IMPACTS, DYN_COUNTER_SOFT_EVENTS,
+    DYN_COUNTER_REFUSED_SOFT_EVENTS};
+constexpr uint32_t kPerStepCount = sizeof(kPerStepSlots) / sizeof(kPerStepSlots[0]);
+
+// Device view of a world's buffers (passed by value to kernels).
+struct W {
+    dyn_step_params p;
+    uint32_t n, d, k;
+    dyn_body_state* st;
+    dyn_body_desc* de;
+    dyn_collider* co;
+    uint32_t* owner;
+    uint32_t* row_of_body;  // body id -> row, ids < id_cap
+    uint32_t id_cap;
+    uint32_t* wake;
+    uint32_t* activity;
+    uint32_t* motion;
+    uint32_t* counters;
+    dyn_aabb* aabbs;
+    dyn_aabb* res;
+    uint32_t* flags;
+    uint32_t* pmaj;
+    uint32_t* pmin;
+    uint32_t pair_cap;
+    dyn_contact* raw;
+    uint32_t* valid;
+    uint32_t* dest;
+    dyn_contact* contacts;
+    uint32_t* matched;
+    uint32_t contact_cap;
+    uint32_t* ncontacts;  // device word: contacts this step
+    dyn_contact* archive;
+    uint32_t* narchive;   // device word
+    dyn_contact* resting;
+    uint32_t* nresting;   // device word
+    dyn_contact* rest_tmp;
+    uint32_t* rest_major;
+    uint32_t* rest_minor;
+    uint32_t* rest_slot;
+    uint32_t* rest_flags;
+    uint32_t* parents;
+    uint32_t* istate;
+    uint32_t* block_counts;
+    uint32_t* blocks;
+    float* targets;
+    float* resolution;
+    uint32_t* vdeltas;
+    uint32_t* pdeltas;
+    uint32_t* contrib;
+    uint32_t* scan_blocks;   // per-block sums of compaction scans
+    uint32_t* scan_total;    // two words: totals of the last scans
+};
+
+__device__ __forceinline__ Body load_body(const W& w, uint32_t row) {
+    Body b;
+    b.s = w.st[row];
+    b.d = w.de[row];
+    return b;
+}
+
+// ---- counters, inputs, activity, begin_step -------------------------------------------------
+
+__global__ void k_reset_counters(W w) {
+    const uint32_t t = threadIdx.x;
+    if (t < kPerStepCount) *cw(w.counters, kPerStepSlots[t]) = 0u;
+}
+
+// clear_inputs.wgsl, activity.wgsl (all rows), begin_step.wgsl (dynamic rows)
+__global__ void __launch_bounds__(kThreads) k_begin(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool awake = false;
+    if (i < w.n) {
+        dyn_body_state s = w.st[i];
+        const dyn_body_desc d = w.de[i];
+        s.force[0] = s.force[1] = s.force[2] = 0.0f;
+        s.torque[0] = s.torque[1] = s.torque[2] = 0.0f;
+        awake = body_is_active(s, d);
+        w.activity[i] = awake ? 1u : 0u;
+        bool moving;
+        const bool driven = kind_moves(d) && !kind_simulates(d);
+        if (driven && (!zero3(ld3(s.velocity)) || !zero3(ld3(s.angular_velocity)))) {
+            moving = true;
+        } else if (!kind_simulates(d) || s.sleeping != 0u) {
+            moving = false;
+        } else {
+            const float sv = (d.flags & DYN_OVERRIDE_SLEEP_LINEAR) ? d.sleep_velocity : w.p.sleep_velocity;
+            const float sw = (d.flags & DYN_OVERRIDE_SLEEP_ANGULAR) ? d.sleep_angular_velocity
+                                                                   : w.p.sleep_angular_velocity;
+            moving = len3(ld3(s.velocity)) > sv || len3(ld3(s.angular_velocity)) > sw;
+        }
+        w.motion[i] = moving ? 1u : 0u;
+        if (i < w.d && s.sleeping == 0u) {
+            s.prev_position[0] = s.position[0];
+            s.prev_position[1] = s.position[1];
+            s.prev_position[2] = s.position[2];
+            for (int c = 0; c < 4; ++c) s.prev_orientation[c] = s.orientation[c];
+        }
+        w.st[i] = s;
+    }
+    if (__any(awake) && lane_id() == 0) *cw(w.counters, DYN_COUNTER_ACTIVE) = 1u;
+}
+
+// broadphase_aabb.wgsl:26-62, plus the entry flags of collider_emit.wgsl:31-63
+__global__ void __launch_bounds__(kThreads) k_aabb(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.k) return;
+    const uint32_t ow = w.owner[i];
+    const dyn_body_state s = w.st[ow];
+    const dyn_body_desc d = w.de[ow];
+    const dyn_collider c = w.co[i];
+    const Shape ws = world_collider(s, c);
+    v3 tmn, tmx;
+    world_aabb_of(ws, &tmn, &tmx);
+    Shape up = ws;
+    up.rotation = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    v3 rmn, rmx;
+    world_aabb_of(up, &rmn, &rmx);
+    v3 smn = tmn, smx = tmx;
+    if (kind_moves(d)) {
+        const v3 travel = absv(mul(ld3(s.velocity), w.p.dt));
+        const float spin = len3(mul(ld3(s.angular_velocity), w.p.dt));
+        const v3 com = add(ld3(s.position), qrot(ld4(s.orientation), ld3(d.com)));
+        const v3 center = add(ld3(s.position), qrot(ld4(s.orientation), ld3(c.local_offset)));
+        const float radius = len3(mul(sub(tmx, tmn), 0.5f)) + len3(sub(center, com));
+        const float reach_r = spin >= kHalfTurn ? 2.0f * radius : 2.0f * radius * sinf(0.5f * spin);
+        const float r = w.p.contact_margin + reach_r;
+        const v3 reach = mk(r, r, r);
+        smn = sub(sub(tmn, travel), reach);
+        smx = add(add(tmx, travel), reach);
+    }
+    float4* a = reinterpret_cast<float4*>(w.aabbs + i);
+    a[0] = make_float4(smn.x, smn.y, smn.z, 0.0f);
+    a[1] = make_float4(smx.x, smx.y, smx.z, 0.0f);
+    float4* r = reinterpret_cast<float4*>(w.res + i);
+    r[0] = make_float4(rmn.x, rmn.y, rmn.z, 0.0f);
+    r[1] = make_float4(rmx.x, rmx.y, rmx.z, 0.0f);
+    uint32_t f = 0;
+    if (kind_moves(d)) f |= DYN_BP_MOBILE;
+    if (w.activity[ow] != 0u || w.wake[ow] != 0u) f |= DYN_BP_AWAKE;
+    w.flags[i] = f;
+}
+
+// ---- narrowphase (narrowphase.wgsl:518-685) -------------------------------------------------
+
+__device__ __forceinline__ float material_combine(float a, float b, uint32_t mode) {
+    if (mode == 1u) return fmn(a, b);
+    if (mode == 2u) return fmx(a, b);
+    if (mode == 3u) return (a + b) * 0.5f;
+    return sqrtf(a * b);
+}
+
+__device__ __forceinline__ bool analytic(uint32_t kind) {
+    return kind == DYN_SHAPE_SPHERE || kind == DYN_SHAPE_CUBOID || kind == DYN_SHAPE_CAPSULE;
+}
+
+__device__ __forceinline__ bool scaled(const dyn_collider& c) {
+    return c.scale[0] != 1.0f || c.scale[1] != 1.0f || c.scale[2] != 1.0f;
+}
+
+__device__ __forceinline__ void mirror(Manifold& m) {
+    m.normal = neg(m.normal);
+}
+
+__global__ void __launch_bounds__(kThreads) k_narrow(W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    if (index >= live) return;
+    w.valid[index] = 0u;
+    const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+    const uint32_t fb = w.owner[fs], sb = w.owner[ss];
+    if (fb == DYN_NO_BODY || sb == DYN_NO_BODY || fb == sb) return;
+    const Body first = load_body(w, fb), second = load_body(w, sb);
+    if (!kind_moves(first.d) && !kind_moves(second.d)) return;
+    const dyn_collider fc = w.co[fs], sc = w.co[ss];
+    if (fc.kind == DYN_SHAPE_NONE || sc.kind == DYN_SHAPE_NONE) return;
+    const uint32_t g1 = fc.collision_group != DYN_NO_COLLISION_FILTER ? fc.collision_group : first.d.collision_group;
+    const uint32_t m1 = fc.collision_mask != DYN_NO_COLLISION_FILTER ? fc.collision_mask : first.d.collision_mask;
+    const uint32_t g2 = sc.collision_group != DYN_NO_COLLISION_FILTER ? sc.collision_group : second.d.collision_group;
+    const uint32_t m2 = sc.collision_mask != DYN_NO_COLLISION_FILTER ? sc.collision_mask : second.d.collision_mask;
+    if (!((g1 & m2) != 0u && (g2 & m1) != 0u)) return;
+    const bool sensor = (fc.flags & DYN_COLLIDER_SENSOR) || (sc.flags & DYN_COLLIDER_SENSOR);
+    const bool fw = fc.kind == DYN_SHAPE_PLANE, sw = sc.kind == DYN_SHAPE_PLANE;
+    if (fw && sw) return;
+    float margin = w.p.contact_margin;
+    if ((first.d.flags & DYN_BODY_CCD) || (second.d.flags & DYN_BODY_CCD)) {
+        margin = w.p.contact_margin + len3(sub(ld3(second.s.velocity), ld3(first.s.velocity))) * w.p.dt;
+    }
+    Manifold m;
+    m.count = 0;
+    m.normal = zero3v();
+    bool mirrored_features = false;
+    if (fw) {
+        const Shape scene = world_collider(first.s, fc), other = world_collider(second.s, sc);
+        if (!analytic(other.kind)) return;  // hull / cylinder planes take the GJK path
+        plane_convex(m, scene, other, margin);
+    } else if (sw) {
+        const Shape scene = world_collider(second.s, sc), other = world_collider(first.s, fc);
+        if (!analytic(other.kind)) return;
+        plane_convex(m, scene, other, margin);
+        mirror(m);
+        mirrored_features = true;
+    } else if (scaled(fc) || scaled(sc) || !analytic(fc.kind) || !analytic(sc.kind)) {
+        return;  // GJK/EPA pairs
+    } else {
+        const uint32_t a = fc.kind, b = sc.kind;
+        if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_SPHERE) sphere_sphere(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_CUBOID) sphere_box(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_CAPSULE) sphere_capsule(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_CUBOID && b == DYN_SHAPE_SPHERE) {
+            sphere_box(m, second, sc, first, fc, margin);
+            mirror(m);
+        } else if (a == DYN_SHAPE_CUBOID && b == DYN_SHAPE_CUBOID) box_box_sat(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_SPHERE) {
+            sphere_capsule(m, second, sc, first, fc, margin);
+            mirror(m);
+        } else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_CAPSULE) capsule_capsule(m, first, fc, second, sc, margin);
+        else return;  // capsule-box takes the GJK path
+    }
+    if (m.count == 0u) return;
+    dyn_contact c;
+    c.a = fs;
+    c.b = ss;
+    c.point_count = m.count;
+    c.sensor = sensor ? 1u : 0u;
+    c.first_body_id = first.s.body_id;
+    c.second_body_id = second.s.body_id;
+    c.first_generation = first.s.generation;
+    c.second_generation = second.s.generation;
+    st3(c.normal, m.normal);
+    c.events = (fc.flags & sc.flags) & (2u | 4u);
+    c.surface = 0xFFFFFFFFu;
+    c.triangle = 0xFFFFFFFFu;
+    c.friction = material_combine(fc.friction, sc.friction, w.p.friction_combine);
+    c.restitution = material_combine(fc.restitution, sc.restitution, w.p.restitution_combine);
+    c.rolling_friction = fmx(fc.rolling_friction, sc.rolling_friction);
+    c.spin_friction = fmx(fc.spin_friction, sc.spin_friction);
+    c.carried_normal = 0.0f;
+    c.carried_tangent = 0.0f;
+    c.relaxation = fc.relaxation + sc.relaxation;
+    c.damping_ratio = fmx(fc.damping_ratio, sc.damping_ratio);
+    c._pad_head[0] = c._pad_head[1] = 0u;
+    const q4 qa = qconj(ld4(first.s.orientation)), qb = qconj(ld4(second.s.orientation));
+    const v3 pa = ld3(first.s.position), pb = ld3(second.s.position);
+    for (uint32_t i = 0; i < 4; ++i) {
+        dyn_manifold_point& mp = c.points[i];
+        if (i < m.count) {
+            st3(mp.position, m.pos[i]);
+            mp.depth = m.depth[i];
+            st3(mp.local_a, qrot(qa, sub(m.pos[i], pa)));
+            st3(mp.local_b, qrot(qb, sub(m.pos[i], pb)));
+            mp.feature = mirrored_features ? feature_mirror(m.feature[i]) : m.feature[i];
+        } else {
+            mp.position[0] = mp.position[1] = mp.position[2] = 0.0f;
+            mp.depth = 0.0f;
+            mp.local_a[0] = mp.local_a[1] = mp.local_a[2] = 0.0f;
+            mp.local_b[0] = mp.local_b[1] = mp.local_b[2] = 0.0f;
+            mp.feature = 0u;
+        }
+        mp._pad_a = 0.0f;
+        mp._pad_tail = 0u;
+        mp.accumulated_normal = 0.0f;
+        mp.accumulated_tangent_1 = 0.0f;
+        mp.accumulated_tangent_2 = 0.0f;
+    }
+    w.raw[index] = c;
+    w.valid[index] = 1u;
+}
+
+// ---- stable flag compaction: per-block counts, one-workgroup scan, destination map ---------
+
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = kThreads * kScanItems;  // 1024 flags per block
+
+// flags[i] for i < *count (or < limit when count is null)
+__global__ void __launch_bounds__(kThreads) k_scan_count(const uint32_t* __restrict__ flags,
+                                                         const uint32_t* count, uint32_t limit,
+                                                         uint32_t* block_sums) {
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    const uint32_t n = count ? min(*count, limit) : limit;
+    const uint32_t first = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t i = first + j;
+        if (i < n && flags[i]) ++sum;
+    }
+    uint32_t total = 0;
+    block_exclusive_sum<kThreads>(sum, s_scan, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* block_sums, uint32_t blocks,
+                                                      uint32_t* total_out) {
+    __shared__ uint32_t s_scan[1024 / kWave];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    for (uint32_t first = 0; first < blocks; first += 1024) {
+        const uint32_t b = first + threadIdx.x;
+        const uint32_t v = b < blocks ? block_sums[b] : 0u;
+        uint32_t total = 0;
+        const uint32_t excl = block_exclusive_sum<1024>(v, s_scan, &total);
+        if (b < blocks) block_sums[b] = s_carry + excl;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total_out = s_carry;
+}
+
+// dest[i] = rank of flag i among set flags (kNoSlot when clear); the same partition as
+// k_scan_count, whose block sums k_scan_blocks turned into offsets.
+__global__ void __launch_bounds__(kThreads) k_scan_dest(const uint32_t* __restrict__ flags,
+                                                        const uint32_t* count, uint32_t limit,
+                                                        const uint32_t* __restrict__ block_offsets,
+                                                        uint32_t* dest) {
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    const uint32_t n = count ? min(*count, limit) : limit;
+    const uint32_t first = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    uint32_t f[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t i = first + j;
+        f[j] = (i < n && flags[i]) ? 1u : 0u;
+        sum += f[j];
+    }
+    uint32_t at = block_offsets[blockIdx.x] + block_exclusive_sum<kThreads>(sum, s_scan, nullptr);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t i = first + j;
+        if (i < n) {
+            dest[i] = f[j] ? at : kNoSlot;
+            at += f[j];
+        }
+    }
+}
+
+// compact_scatter.wgsl: valid raw contacts in pair order; carried impulses cleared (the raw
+// record already has them at zero).
+__global__ void __launch_bounds__(kThreads) k_contacts_scatter(W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    if (index >= live || !w.valid[index]) return;
+    const uint32_t at = w.dest[index];
+    if (at >= w.contact_cap) return;
+    w.contacts[at] = w.raw[index];
+    w.matched[at] = 0u;
+}
+
+__global__ void k_contacts_total(W w) {
+    if (threadIdx.x != 0) return;
+    const uint32_t total = w.scan_total[0];
+    *cw(w.counters, DYN_COUNTER_CONTACTS) = total;
+    const uint32_t kept = min(total, w.contact_cap);
+    *cw(w.counters, DYN_COUNTER_REFUSED_CONTACTS) = total - kept;
+    *w.ncontacts = kept;
+    *cw(w.counters, DYN_COUNTER_BLOCKS) = kept;
+}
+
+// ---- warm start and identity (identity.wgsl, contact_relay.wgsl, contact_begin.wgsl) ------
+
+__device__ __forceinline__ bool touches(const dyn_contact& c, float threshold) {
+    for (uint32_t i = 0; i < c.point_count; ++i) {
+        if (c.points[i].depth >= -threshold) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool same_pair(const dyn_contact& h, const dyn_contact& c) {
+    const bool roles = h.first_body_id == c.first_body_id && h.first_generation == c.first_generation &&
+                       h.second_body_id == c.second_body_id && h.second_generation == c.second_generation &&
+                       h.a == c.a && h.b == c.b;
+    const bool flipped = h.first_body_id == c.second_body_id && h.first_generation == c.second_generation &&
+                         h.second_body_id == c.first_body_id && h.second_generation == c.first_generation &&
+                         h.a == c.b && h.b == c.a;
+    return roles || flipped;
+}
+
+__device__ __forceinline__ void carry(dyn_contact& c) {
+    float n = 0.0f, t = 0.0f;
+    for (uint32_t i = 0; i < c.point_count; ++i) n = n + fmx(c.points[i].accumulated_normal, 0.0f);
+    for (uint32_t i = 0; i < c.point_count; ++i) {
+        const dyn_manifold_point& p = c.points[i];
+        t = t + sqrtf(p.accumulated_tangent_1 * p.accumulated_tangent_1 +
+                      p.accumulated_tangent_2 * p.accumulated_tangent_2);
+    }
+    c.carried_normal = n;
+    c.carried_tangent = t;
+}
+
+// identity.wgsl:85-117
+__device__ inline void relay_impulses(dyn_contact& r, const dyn_contact& held) {
+    bool taken[4] = {false, false, false, false};
+    for (uint32_t pi = 0; pi < r.point_count; ++pi) {
+        const dyn_manifold_point& point = r.points[pi];
+        uint32_t picked = kNoSlot;
+        float picked_distance = 0.0f;
+        for (uint32_t hi = 0; hi < held.point_count; ++hi) {
+            if (taken[hi]) continue;
+            const dyn_manifold_point& hp = held.points[hi];
+            if (!(hp.feature == point.feature || hp.feature == feature_mirror(point.feature))) continue;
+            const float d = len3(sub(ld3(hp.local_a), ld3(point.local_a))) +
+                            len3(sub(ld3(hp.local_b), ld3(point.local_b)));
+            if (picked == kNoSlot || d < picked_distance) {
+                picked = hi;
+                picked_distance = d;
+            }
+        }
+        if (picked == kNoSlot) continue;
+        taken[picked] = true;
+        r.points[pi].accumulated_normal = held.points[picked].accumulated_normal;
+        r.points[pi].accumulated_tangent_1 = held.points[picked].accumulated_tangent_1;
+        r.points[pi].accumulated_tangent_2 = held.points[picked].accumulated_tangent_2;
+    }
+}
+
+// body_row.wgsl:1-13
+__device__ __forceinline__ uint32_t resolve_row(const W& w, uint32_t id, uint32_t gen) {
+    if (id >= w.id_cap) return DYN_NO_BODY;
+    const uint32_t row = w.row_of_body[id];
+    if (row >= w.n) return DYN_NO_BODY;
+    const dyn_body_state& s = w.st[row];
+    if (s.body_id != id || s.generation != gen) return DYN_NO_BODY;
+    return row;
+}
+
+// contact_relay.wgsl:34-71 (events are not modelled: no collider here requests them)
+__global__ void __launch_bounds__(kThreads) k_relay(W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= *w.narchive) return;
+    const dyn_contact held = w.archive[index];
+    if (resolve_row(w, held.first_body_id, held.first_generation) == DYN_NO_BODY ||
+        resolve_row(w, held.second_body_id, held.second_generation) == DYN_NO_BODY) return;
+    const uint32_t count = *w.ncontacts;
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        const uint32_t ma = w.contacts[mid].a, mb = w.contacts[mid].b;
+        if (ma < held.a || (ma == held.a && mb < held.b)) lo = mid + 1u; else hi = mid;
+    }
+    if (lo >= count || w.contacts[lo].a != held.a || w.contacts[lo].b != held.b) return;
+    dyn_contact cur = w.contacts[lo];
+    if (!same_pair(held, cur)) return;
+    if (!(held.events & kContactAnnounced)) return;
+    w.matched[lo] = 1u;
+    relay_impulses(cur, held);
+    carry(cur);
+    cur.events |= kContactAnnounced;
+    w.contacts[lo] = cur;
+}
+
+// contact_begin.wgsl:41-71 over the resting archive sorted by (min id, max id)
+__global__ void __launch_bounds__(kThreads) k_contact_begin(W w, float slop) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= *w.ncontacts || w.matched[index]) return;
+    dyn_contact c = w.contacts[index];
+    const uint32_t kmaj = min(c.first_body_id, c.second_body_id);
+    const uint32_t kmin = max(c.first_body_id, c.second_body_id);
+    const uint32_t count = *w.nresting;
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        const uint32_t ma = w.rest_major[mid];
+        if (ma < kmaj || (ma == kmaj && w.rest_minor[mid] < kmin)) lo = mid + 1u; else hi = mid;
+    }
+    uint32_t slot = kNoSlot;
+    for (; lo < count && w.rest_major[lo] == kmaj && w.rest_minor[lo] == kmin; ++lo) {
+        if (same_pair(w.resting[lo], c)) {
+            slot = lo;
+            break;
+        }
+    }
+    if (slot != kNoSlot) {
+        const dyn_contact& rest = w.resting[slot];
+        if (rest.events & kContactAnnounced) {
+            c.events |= kContactAnnounced;
+            relay_impulses(c, rest);
+            carry(c);
+            c.events |= kContactAnnounced;
+        } else if (touches(c, slop)) {
+            c.events |= kContactAnnounced;
+        }
+        w.contacts[index] = c;
+        return;
+    }
+    if (touches(c, slop)) {
+        w.contacts[index].events = c.events | kContactAnnounced;
+    }
+}
+
+// ---- islands (island_*.wgsl) ------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kThreads) k_island_init(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.d) return;
+    w.parents[i] = i;
+    w.istate[i] = 0u;
+}
+
+__device__ __forceinline__ uint32_t island_find(const W& w, uint32_t node) {
+    uint32_t cur = node;
+    uint32_t parent = atomic_load_agent(&w.parents[cur]);
+    while (parent != cur) {
+        cur = parent;
+        parent = atomic_load_agent(&w.parents[cur]);
+    }
+    return cur;
+}
+
+// island_link.wgsl:13-45
+__device__ inline void island_couple(const W& w, uint32_t a, uint32_t b) {
+    if (b >= w.d && atomic_load_agent(&w.wake[b]) != 0u) atomicOr(&w.wake[a], 1u);
+    if (a >= w.d && atomic_load_agent(&w.wake[a]) != 0u) atomicOr(&w.wake[b], 1u);
+    if (a >= w.d || b >= w.d || a == b) return;
+    for (uint32_t attempt = 0; attempt < 256u; ++attempt) {
+        const uint32_t ra = island_find(w, a), rb = island_find(w, b);
+        if (ra == rb) return;
+        const uint32_t lower = min(ra, rb), higher = max(ra, rb);
+        uint32_t expected = higher;
+        if (__hip_atomic_compare_exchange_strong(&w.parents[higher], &expected, lower, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            return;
+        }
+    }
+    atomicAdd(cw(w.counters, DYN_COUNTER_ISLAND_FAULTS), 1u);
+}
+
+__global__ void __launch_bounds__(kThreads) k_island_link(W w, float slop) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *w.ncontacts) return;
+    const dyn_contact& c = w.contacts[i];
+    if (c.point_count == 0u || c.sensor == 1u || !touches(c, slop)) return;
+    island_couple(w, w.owner[c.a], w.owner[c.b]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_island_link_resting(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *w.nresting) return;
+    const dyn_contact& c = w.resting[i];
+    const uint32_t r1 = resolve_row(w, c.first_body_id, c.first_generation);
+    const uint32_t r2 = resolve_row(w, c.second_body_id, c.second_generation);
+    if (r1 == DYN_NO_BODY || r2 == DYN_NO_BODY) return;
+    island_couple(w, r1, r2);
+}
+
+// island_flatten.wgsl
+__global__ void __launch_bounds__(kThreads) k_island_flatten(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.d) return;
+    const uint32_t parent = atomic_load_agent(&w.parents[i]);
+    const uint32_t gp = atomic_load_agent(&w.parents[parent]);
+    atomic_store_agent(&w.parents[i], min(parent, gp));
+}
+
+// island_aggregate.wgsl + island_count.wgsl
+__global__ void __launch_bounds__(kThreads) k_island_aggregate(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t roots = 0;
+    if (i < w.d) {
+        const uint32_t root = w.parents[i];
+        if (w.wake[i] != 0u || w.motion[i] != 0u) atomicMax(&w.istate[root], kIslandWake);
+        roots = root == i ? 1u : 0u;
+    }
+    roots = wave_sum(roots);
+    if (lane_id() == 0 && roots) atomicAdd(cw(w.counters, DYN_COUNTER_ISLANDS), roots);
+}
+
+// island_wake.wgsl
+__global__ void __launch_bounds__(kThreads) k_island_wake(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t woke = 0;
+    if (i < w.d) {
+        const uint32_t root = w.parents[i];
+        if (w.st[i].sleeping != 0u && (w.istate[root] & kIslandWake)) {
+            woke = 1u;
+            w.st[i].sleeping = 0u;
+            w.st[i].sleep_timer = 0.0f;
+        }
+        w.wake[i] = 0u;
+    }
+    woke = wave_sum(woke);
+    if (lane_id() == 0 && woke) {
+        atomicAdd(cw(w.counters, DYN_COUNTER_WOKE), woke);
+        atomicAdd(cw(w.counters, DYN_COUNTER_WOKE_DEFERRED), woke);
+    }
+}
+
+// live.wgsl count, solver_reset.wgsl
+__global__ void __launch_bounds__(kThreads) k_solver_reset(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t live = 0;
+    if (i < w.n) {
+        w.block_counts[i] = 0u;
+        reinterpret_cast<float4*>(w.resolution)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < w.d && body_is_active(w.st[i], w.de[i])) live = 1u;
+    }
+    live = wave_sum(live);
+    if (lane_id() == 0 && live) atomicAdd(cw(w.counters, DYN_COUNTER_LIVE), live);
+}
+
+// ---- solver ------------------------------------------------------------------------------------
+
+__device__ __forceinline__ bool block_resolves(const dyn_contact& c) {
+    return c.point_count > 0u && c.sensor == 0u;
+}
+
+// solver_blocks.wgsl:24-48
+__global__ void __launch_bounds__(kThreads) k_blocks(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *w.ncontacts) return;
+    const dyn_contact& c = w.contacts[i];
+    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
+    w.blocks[2 * i] = fb;
+    w.blocks[2 * i + 1] = sb;
+    if (!block_resolves(c)) return;
+    const Body first = load_body(w, fb), second = load_body(w, sb);
+    const v3 normal = ld3(c.normal);
+    for (uint32_t pi = 0; pi < c.point_count; ++pi) {
+        const v3 pos = ld3(c.points[pi].position);
+        const float speed = dot3(relative_velocity(first, second, pos, pos), normal);
+        float target = 0.0f;
+        if (c.points[pi].depth < 0.0f) target = c.points[pi].depth / w.p.dt;
+        else if (speed < -w.p.restitution_threshold) target = -speed * c.restitution;
+        w.targets[i * 4u + pi] = target;
+    }
+    if (body_is_inert(first)) w.block_counts[fb] = 1u; else atomicAdd(&w.block_counts[fb], 1u);
+    if (body_is_inert(second)) w.block_counts[sb] = 1u; else atomicAdd(&w.block_counts[sb], 1u);
+}
+
+__global__ void __launch_bounds__(kThreads) k_rows_count(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t claimed = (i < w.n && w.block_counts[i] != 0u) ? 1u : 0u;
+    claimed = wave_sum(claimed);
+    if (lane_id() == 0 && claimed) atomicAdd(cw(w.counters, DYN_COUNTER_SOLVER_ROWS), claimed);
+}
+
+__device__ __forceinline__ void accumulate(uint32_t* deltas, uint32_t row, v3 lin, v3 ang, float scale) {
+    uint32_t* d = deltas + row * 8u;
+    atomicAdd(d + 0, solver_word(lin.x, scale));
+    atomicAdd(d + 1, solver_word(lin.y, scale));
+    atomicAdd(d + 2, solver_word(lin.z, scale));
+    atomicAdd(d + 3, solver_word(ang.x, scale));
+    atomicAdd(d + 4, solver_word(ang.y, scale));
+    atomicAdd(d + 5, solver_word(ang.z, scale));
+}
+
+__device__ __forceinline__ void make_tangents(v3 n, v3* t1, v3* t2) {
+    const v3 t = fabsf(n.x) > 0.9f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    *t1 = norm3(cross3(n, t));
+    *t2 = cross3(n, *t1);
+}
+
+__device__ __forceinline__ float momentum_mass(const Body& a, const Body& b, v3 pa, v3 pb, v3 axis) {
+    const v3 rax = cross3(sub(pa, body_com(a)), axis), rbx = cross3(sub(pb, body_com(b)), axis);
+    return a.d.inverse_mass + b.d.inverse_mass + dot3(rax, apply_inverse_inertia(a, rax)) +
+           dot3(rbx, apply_inverse_inertia(b, rbx));
+}
+
+__device__ __forceinline__ void apply_pair_impulse(Body& a, Body& b, v3 pa, v3 pb, v3 impulse) {
+    st3(a.s.velocity, sub(ld3(a.s.velocity), mul(impulse, a.d.inverse_mass)));
+    st3(a.s.angular_velocity,
+        sub(ld3(a.s.angular_velocity), apply_inverse_inertia(a, cross3(sub(pa, body_com(a)), impulse))));
+    st3(b.s.velocity, add(ld3(b.s.velocity), mul(impulse, b.d.inverse_mass)));
+    st3(b.s.angular_velocity,
+        add(ld3(b.s.angular_velocity), apply_inverse_inertia(b, cross3(sub(pb, body_com(b)), impulse))));
+}
+
+__device__ __forceinline__ v3 arm(const Body& b, const dyn_manifold_point& p, bool first) {
+    return add(qrot(ld4(b.s.orientation), ld3(first ? p.local_a : p.local_b)), ld3(b.s.position));
+}
+
+// solver_block_solve.wgsl:56-83: inert bodies receive exactly zero words (skipped).
+__device__ __forceinline__ void commit_block(const W& w, uint32_t fb, uint32_t sb, bool fa_inert,
+                                             bool sb_inert, v3 da, v3 sa, v3 db, v3 sbv,
+                                             bool residual) {
+    if (!fa_inert) accumulate(w.vdeltas, fb, da, sa, DYN_SOLVER_VELOCITY_SCALE);
+    if (!sb_inert) accumulate(w.vdeltas, sb, db, sbv, DYN_SOLVER_VELOCITY_SCALE);
+    if (!residual) return;
+    atomicMax(cw(w.counters, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL),
+              solver_word(fmx(len3(da), len3(db)), DYN_SOLVER_VELOCITY_SCALE));
+    atomicMax(cw(w.counters, DYN_COUNTER_SOLVE_ANGULAR_RESIDUAL),
+              solver_word(fmx(len3(sa), len3(sbv)), DYN_SOLVER_VELOCITY_SCALE));
+}
+
+// solver_contact_block.wgsl:195-233
+__global__ void __launch_bounds__(kThreads) k_warm(W w) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= *w.ncontacts) return;
+    const dyn_contact& contact = w.contacts[ci];
+    if (!block_resolves(contact)) return;
+    const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
+    const Body fl = load_body(w, fb), sl = load_body(w, sb);
+    Body first = fl, second = sl;
+    const bool fi = body_is_inert(fl), si = body_is_inert(sl);
+    if (fi) body_frozen(first);
+    if (si) body_frozen(second);
+    const v3 normal = ld3(contact.normal);
+    v3 t1, t2;
+    make_tangents(normal, &t1, &t2);
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        const dyn_manifold_point& p = contact.points[pi];
+        const v3 aa = arm(first, p, true), ab = arm(second, p, false);
+        const v3 impulse = add(add(mul(normal, p.accumulated_normal), mul(t1, p.accumulated_tangent_1)),
+                               mul(t2, p.accumulated_tangent_2));
+        apply_pair_impulse(first, second, aa, ab, impulse);
+    }
+    commit_block(w, fb, sb, fi, si, sub(ld3(first.s.velocity), ld3(fl.s.velocity)),
+                 sub(ld3(first.s.angular_velocity), ld3(fl.s.angular_velocity)),
+                 sub(ld3(second.s.velocity), ld3(sl.s.velocity)),
+                 sub(ld3(second.s.angular_velocity), ld3(sl.s.angular_velocity)), false);
+}
+
+__device__ __forceinline__ float normal_delta(const dyn_contact& c, float mass, float truth, float step,
+                                              float goal, float depth, float speed, float accumulated) {
+    if (c.relaxation <= 0.0f) return (goal - speed) / mass;
+    const float bias = fmx(depth / step, goal);
+    const float drag = 2.0f * c.damping_ratio * c.relaxation / step;
+    const float softness = c.relaxation * c.relaxation * truth / (step * step);
+    const float prior = speed - accumulated * truth;
+    const float wanted = (bias - (1.0f + drag) * prior) / (softness + (1.0f + drag) * truth);
+    return truth * (wanted - accumulated) / mass;
+}
+
+// solver_contact_block.wgsl:31-193
+__global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= *w.ncontacts) return;
+    const dyn_contact contact = w.contacts[ci];
+    if (!block_resolves(contact)) return;
+    const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
+    Body pf = load_body(w, fb), ps = load_body(w, sb);
+    Body split_f = pf, split_s = ps;
+    const float c1 = static_cast<float>(max(w.block_counts[fb], 1u));
+    const float c2 = static_cast<float>(max(w.block_counts[sb], 1u));
+    split_f.d.inverse_mass = split_f.d.inverse_mass * c1;
+    split_s.d.inverse_mass = split_s.d.inverse_mass * c2;
+    for (int k = 0; k < 6; ++k) {
+        split_f.d.inverse_inertia[k] = split_f.d.inverse_inertia[k] * c1;
+        split_s.d.inverse_inertia[k] = split_s.d.inverse_inertia[k] * c2;
+    }
+    const bool fi = body_is_inert(pf), si = body_is_inert(ps);
+    if (fi) {
+        body_frozen(pf);
+        body_frozen(split_f);
+    }
+    if (si) {
+        body_frozen(ps);
+        body_frozen(split_s);
+    }
+    if (fi && si) return;  // both inert: zero words (commit_block of zeros)
+    Body first = pf, second = ps;
+    const v3 normal = ld3(contact.normal);
+    v3 t1, t2;
+    make_tangents(normal, &t1, &t2);
+    dyn_contact updated = contact;
+    v3 reach_a = zero3v(), reach_b = zero3v();
+    const v3 com_f = body_com(pf), com_s = body_com(ps);
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        reach_a = add(reach_a, sub(arm(pf, contact.points[pi], true), com_f));
+        reach_b = add(reach_b, sub(arm(ps, contact.points[pi], false), com_s));
+    }
+    const float count = static_cast<float>(max(contact.point_count, 1u));
+    float normal_total = 0.0f;
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        normal_total = normal_total + fmx(contact.points[pi].accumulated_normal, 0.0f);
+    }
+    v3 impulses[4] = {zero3v(), zero3v(), zero3v(), zero3v()};
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        const dyn_manifold_point& p = contact.points[pi];
+        const v3 aa = arm(pf, p, true), ab = arm(ps, p, false);
+        float an = p.accumulated_normal, at1 = p.accumulated_tangent_1, at2 = p.accumulated_tangent_2;
+        const float nspeed = dot3(relative_velocity(pf, ps, aa, ab), normal);
+        const v3 arm_a = cross3(sub(aa, body_com(split_f)), normal);
+        const v3 arm_b = cross3(sub(ab, body_com(split_s)), normal);
+        const float nmass = count * (split_f.d.inverse_mass + split_s.d.inverse_mass) +
+                            dot3(apply_inverse_inertia(split_f, cross3(reach_a, normal)), arm_a) +
+                            dot3(apply_inverse_inertia(split_s, cross3(reach_b, normal)), arm_b);
+        const float truth = momentum_mass(pf, ps, aa, ab, normal);
+        const float delta = normal_delta(contact, nmass, truth, w.p.dt, w.targets[ci * 4u + pi], p.depth,
+                                         nspeed, an);
+        const float next_n = fmx(0.0f, an + delta);
+        impulses[pi] = add(impulses[pi], mul(normal, next_n - an));
+        an = next_n;
+        const float limit = contact.friction * an;
+        const float m1 = momentum_mass(split_f, split_s, aa, ab, t1) * count;
+        const float s1 = dot3(relative_velocity(pf, ps, aa, ab), t1);
+        const float next_1 = clampf(at1 - s1 / m1, -limit, limit);
+        impulses[pi] = add(impulses[pi], mul(t1, next_1 - at1));
+        at1 = next_1;
+        const float m2 = momentum_mass(split_f, split_s, aa, ab, t2) * count;
+        const float s2 = dot3(relative_velocity(pf, ps, aa, ab), t2);
+        const float remaining = sqrtf(fmx(limit * limit - at1 * at1, 0.0f));
+        const float next_2 = clampf(at2 - s2 / m2, -remaining, remaining);
+        impulses[pi] = add(impulses[pi], mul(t2, next_2 - at2));
+        at2 = next_2;
+        updated.points[pi].accumulated_normal = an;
+        updated.points[pi].accumulated_tangent_1 = at1;
+        updated.points[pi].accumulated_tangent_2 = at2;
+    }
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        const dyn_manifold_point& p = contact.points[pi];
+        apply_pair_impulse(first, second, arm(pf, p, true), arm(ps, p, false), impulses[pi]);
+    }
+    const v3 rel_spin = sub(ld3(second.s.angular_velocity), ld3(first.s.angular_velocity));
+    const v3 planar = sub(rel_spin, mul(normal, dot3(rel_spin, normal)));
+    const float planar_len = len3(planar);
+    if (planar_len > 1e-6f) {
+        const v3 axis = dv(planar, planar_len);
+        const float mass = dot3(axis, apply_inverse_inertia(split_f, axis)) +
+                           dot3(axis, apply_inverse_inertia(split_s, axis));
+        if (mass > 0.0f) {
+            const float lim = contact.rolling_friction * normal_total;
+            const float applied = clampf(-planar_len / mass, -lim, lim);
+            st3(first.s.angular_velocity,
+                sub(ld3(first.s.angular_velocity), apply_inverse_inertia(first, mul(axis, applied))));
+            st3(second.s.angular_velocity,
+                add(ld3(second.s.angular_velocity), apply_inverse_inertia(second, mul(axis, applied))));
+        }
+    }
+    const float twist = dot3(rel_spin, normal);
+    if (fabsf(twist) > 1e-6f) {
+        const float mass = dot3(normal, apply_inverse_inertia(split_f, normal)) +
+                           dot3(normal, apply_inverse_inertia(split_s, normal));
+        if (mass > 0.0f) {
+            const float lim = contact.spin_friction * normal_total;
+            const float applied = clampf(-twist / mass, -lim, lim);
+            st3(first.s.angular_velocity,
+                sub(ld3(first.s.angular_velocity), apply_inverse_inertia(first, mul(normal, applied))));
+            st3(second.s.angular_velocity,
+                add(ld3(second.s.angular_velocity), apply_inverse_inertia(second, mul(normal, applied))));
+        }
+    }
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        dyn_manifold_point& q = w.contacts[ci].points[pi];
+        q.accumulated_normal = updated.points[pi].accumulated_normal;
+        q.accumulated_tangent_1 = updated.points[pi].accumulated_tangent_1;
+        q.accumulated_tangent_2 = updated.points[pi].accumulated_tangent_2;
+    }
+    commit_block(w, fb, sb, fi, si, sub(ld3(first.s.velocity), ld3(pf.s.velocity)),
+                 sub(ld3(first.s.angular_velocity), ld3(pf.s.angular_velocity)),
+                 sub(ld3(second.s.velocity), ld3(ps.s.velocity)),
+                 sub(ld3(second.s.angular_velocity), ld3(ps.s.angular_velocity)), residual != 0u);
+}
+
+// solver_block_apply.wgsl over every claimed row
+__global__ void __launch_bounds__(kThreads) k_apply(W w) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= w.n || w.block_counts[row] == 0u) return;
+    uint32_t* d = w.vdeltas + row * 8u;
+    const uint4 lo = *reinterpret_cast<const uint4*>(d);
+    const uint2 hi = *reinterpret_cast<const uint2*>(d + 4);
+    if ((lo.x | lo.y | lo.z | lo.w | hi.x | hi.y) == 0u) return;
+    *reinterpret_cast<uint4*>(d) = make_uint4(0u, 0u, 0u, 0u);
+    *reinterpret_cast<uint2*>(d + 4) = make_uint2(0u, 0u);
+    const v3 lin = mk(solver_value(lo.x, DYN_SOLVER_VELOCITY_SCALE), solver_value(lo.y, DYN_SOLVER_VELOCITY_SCALE),
+                      solver_value(lo.z, DYN_SOLVER_VELOCITY_SCALE));
+    const v3 ang = mk(solver_value(lo.w, DYN_SOLVER_VELOCITY_SCALE), solver_value(hi.x, DYN_SOLVER_VELOCITY_SCALE),
+                      solver_value(hi.y, DYN_SOLVER_VELOCITY_SCALE));
+    if (zero3(lin) && zero3(ang)) return;
+    dyn_body_state& s = w.st[row];
+    st3(s.velocity, add(ld3(s.velocity), lin));
+    st3(s.angular_velocity, add(ld3(s.angular_velocity), ang));
+}
+
+// substep_integrate.wgsl:15-76 over the awake dynamic rows (no fields)
+__global__ void __launch_bounds__(kThreads) k_integrate(W w) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= w.d) return;
+    const dyn_body_desc d = w.de[row];
+    dyn_body_state s = w.st[row];
+    if (!body_is_active(s, d) || !kind_simulates(d)) return;
+    const float h = w.p.substep_dt;
+    const q4 q = ld4(s.orientation);
+    const v3 acc = add(mul(mk(w.p.gravity[0], w.p.gravity[1], w.p.gravity[2]), d.gravity_scale),
+                       mul(ld3(s.force), d.inverse_mass));
+    const v3 spin = ld3(s.angular_velocity);
+    st3(s.velocity, add(ld3(s.velocity), mul(acc, h)));
+    v3 gspin = spin;
+    const bool iso = d.inertia[1] == 0.0f && d.inertia[2] == 0.0f && d.inertia[4] == 0.0f &&
+                     d.inertia[0] == d.inertia[3] && d.inertia[3] == d.inertia[5];
+    if (!iso) {
+        const v3 body_spin = qrot(qconj(q), spin);
+        const v3 momentum = sym_apply(d.inertia, body_spin);
+        if (!zero3(cross3(body_spin, momentum))) {
+            v3 solved = momentum;
+            for (int it = 0; it < 4; ++it) {
+                const v3 mid = mul(add(momentum, solved), 0.5f);
+                solved = rotate_about(mul(neg(sym_apply(d.inverse_inertia, mid)), h), momentum);
+            }
+            gspin = qrot(q, sym_apply(d.inverse_inertia, solved));
+        }
+    }
+    const v3 torque_term = qrot(q, sym_apply(d.inverse_inertia, qrot(qconj(q), mul(ld3(s.torque), h))));
+    st3(s.angular_velocity, add(gspin, torque_term));
+    const float speed = len3(ld3(s.velocity));
+    if (speed > w.p.max_velocity) st3(s.velocity, mul(ld3(s.velocity), w.p.max_velocity / speed));
+    const float sspeed = len3(ld3(s.angular_velocity));
+    if (sspeed > w.p.max_angular_velocity)
+        st3(s.angular_velocity, mul(ld3(s.angular_velocity), w.p.max_angular_velocity / sspeed));
+    st3(s.velocity, dv(ld3(s.velocity), 1.0f + d.linear_damping * h));
+    st3(s.angular_velocity, dv(ld3(s.angular_velocity), 1.0f + d.angular_damping * h));
+    w.st[row] = s;
+}
+
+// substep_advance.wgsl
+__global__ void __launch_bounds__(kThreads) k_advance(W w) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= w.d) return;
+    dyn_body_state s = w.st[row];
+    if (!body_is_active(s, w.de[row])) return;
+    const float h = w.p.substep_dt;
+    st3(s.position, add(ld3(s.position), mul(ld3(s.velocity), h)));
+    const q4 spin = make_float4(s.angular_velocity[0], s.angular_velocity[1], s.angular_velocity[2], 0.0f);
+    const q4 q = ld4(s.orientation);
+    const q4 dq = qmul(spin, q);
+    const float k = 0.5f * h;
+    st4(s.orientation, qnorm(make_float4(q.x + k * dq.x, q.y + k * dq.y, q.z + k * dq.z, q.w + k * dq.w)));
+    w.st[row] = s;
+}
+
+// position_correction.wgsl:5-46, position_block.wgsl
+__global__ void __launch_bounds__(kThreads) k_position_block(W w) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= *w.ncontacts) return;
+    const dyn_contact& c = w.contacts[ci];
+    if (!block_resolves(c) || c.relaxation > 0.0f) return;
+    const uint32_t fr = w.blocks[2 * ci], sr = w.blocks[2 * ci + 1];
+    const dyn_body_desc& df = w.de[fr];
+    const dyn_body_desc& ds = w.de[sr];
+    const bool fi = !kind_simulates(df) || w.st[fr].sleeping != 0u;
+    const bool si = !kind_simulates(ds) || w.st[sr].sleeping != 0u;
+    const float imf = fi ? 0.0f : df.inverse_mass, ims = si ? 0.0f : ds.inverse_mass;
+    const v3 normal = ld3(c.normal);
+    const float4 rs = reinterpret_cast<const float4*>(w.resolution)[sr];
+    const float4 rf = reinterpret_cast<const float4*>(w.resolution)[fr];
+    const float resolved = dot3(normal, sub(mk(rs.x, rs.y, rs.z), mk(rf.x, rf.y, rf.z)));
+    v3 total = zero3v();
+    uint32_t contributing = 0;
+    for (uint32_t pi = 0; pi < c.point_count; ++pi) {
+        const float error = fmx(c.points[pi].depth - w.p.slop - resolved, 0.0f);
+        if (error <= 0.0f) continue;
+        const float k = imf + ims;
+        if (k == 0.0f) continue;
+        total = add(total, mul(normal, w.p.relaxation * error / k));
+        ++contributing;
+    }
+    if (contributing == 0u) return;
+    const v3 correction = dv(total, static_cast<float>(contributing));
+    const v3 la = mul(neg(correction), imf), lb = mul(correction, ims);
+    if (!(dot3(la, la) > 0.0f || dot3(lb, lb) > 0.0f)) return;
+    // Inert rows get exactly zero words; only their "contributed" mark matters.
+    if (fi) w.contrib[fr] = 1u; else atomicAdd(&w.contrib[fr], 1u);
+    if (si) w.contrib[sr] = 1u; else atomicAdd(&w.contrib[sr], 1u);
+    if (!fi) accumulate(w.pdeltas, fr, la, zero3v(), DYN_SOLVER_POSITION_SCALE);
+    if (!si) accumulate(w.pdeltas, sr, lb, zero3v(), DYN_SOLVER_POSITION_SCALE);
+}
+
+// position_apply.wgsl over every claimed row
+__global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= w.n || w.block_counts[row] == 0u) return;
+    const uint32_t contributing = w.contrib[row];
+    if (contributing == 0u) return;
+    w.contrib[row] = 0u;
+    uint32_t* d = w.pdeltas + row * 8u;
+    const uint4 lo = *reinterpret_cast<const uint4*>(d);
+    const uint2 hi = *reinterpret_cast<const uint2*>(d + 4);
+    *reinterpret_cast<uint4*>(d) = make_uint4(0u, 0u, 0u, 0u);
+    *reinterpret_cast<uint2*>(d + 4) = make_uint2(0u, 0u);
+    const v3 lin = mk(solver_value(lo.x, DYN_SOLVER_POSITION_SCALE), solver_value(lo.y, DYN_SOLVER_POSITION_SCALE),
+                      solver_value(lo.z, DYN_SOLVER_POSITION_SCALE));
+    const v3 ang = mk(solver_value(lo.w, DYN_SOLVER_POSITION_SCALE), solver_value(hi.x, DYN_SOLVER_POSITION_SCALE),
+                      solver_value(hi.y, DYN_SOLVER_POSITION_SCALE));
+    const float cf = static_cast<float>(contributing);
+    const v3 al = dv(lin, cf), aa = dv(ang, cf);
+    dyn_body_state& s = w.st[row];
+    st3(s.position, add(ld3(s.position), al));
+    st4(s.orientation, qnorm(qmul(make_float4(aa.x * 0.5f, aa.y * 0.5f, aa.z * 0.5f, 1.0f), ld4(s.orientation))));
+    float4& r = reinterpret_cast<float4*>(w.resolution)[row];
+    const v3 nr = add(mk(r.x, r.y, r.z), al);
+    r = make_float4(nr.x, nr.y, nr.z, 0.0f);
+}
+
+// island_sleep.wgsl
+__global__ void __launch_bounds__(kThreads) k_sleep(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t slept = 0;
+    if (i < w.d) {
+        dyn_body_state s = w.st[i];
+        if (s.sleeping == 0u && kind_simulates(w.de[i])) {
+            const uint32_t root = w.parents[i];
+            const bool disturbed = (w.istate[root] & kIslandWake) != 0u || w.wake[i] != 0u;
+            if (disturbed) {
+                s.sleep_timer = 0.0f;
+            } else {
+                s.sleep_timer = s.sleep_timer + w.p.dt;
+                if (s.sleep_timer >= w.p.sleep_time) {
+                    for (int c = 0; c < 3; ++c) {
+                        s.velocity[c] = 0.0f;
+                        s.angular_velocity[c] = 0.0f;
+                        s.force[c] = 0.0f;
+                        s.torque[c] = 0.0f;
+                        s.prev_position[c] = s.position[c];
+                    }
+                    for (int c = 0; c < 4; ++c) s.prev_orientation[c] = s.orientation[c];
+                    s.sleep_timer = 0.0f;
+                    s.sleeping = 1u;
+                    slept = 1u;
+                }
+            }
+            w.st[i] = s;
+        }
+    }
+    slept = wave_sum(slept);
+    if (lane_id() == 0 && slept) atomicAdd(cw(w.counters, DYN_COUNTER_SLEPT), slept);
+}
+
+// ---- commit: thaw, archive, static wake clear, freeze (commit.rs:187-209) ---------------
+
+// rest_flags[i] = keep resting manifold i (both bodies inactive at the step's start)
+__global__ void __launch_bounds__(kThreads) k_thaw(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *w.nresting) return;
+    const dyn_contact& c = w.resting[i];
+    const uint32_t r1 = resolve_row(w, c.first_body_id, c.first_generation);
+    const uint32_t r2 = resolve_row(w, c.second_body_id, c.second_generation);
+    const bool keep = r1 != DYN_NO_BODY && r2 != DYN_NO_BODY && w.activity[r1] == 0u && w.activity[r2] == 0u;
+    w.rest_flags[i] = keep ? 1u : 0u;
+}
+
+// contact_archive.wgsl + freeze_contacts.wgsl flags (freeze flags after the resting flags)
+__global__ void __launch_bounds__(kThreads) k_archive(W w, uint32_t* freeze_flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *w.ncontacts) return;
+    const dyn_contact c = w.contacts[i];
+    w.archive[i] = c;
+    const uint32_t f = w.owner[c.a], s = w.owner[c.b];
+    const bool frozen = !body_is_active(w.st[f], w.de[f]) && !body_is_active(w.st[s], w.de[s]);
+    freeze_flags[i] = frozen ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kThreads) k_archive_count(W w) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *w.narchive = *w.ncontacts;
+        *cw(w.counters, DYN_COUNTER_ARCHIVED) = *w.ncontacts;
+    }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.d && i < w.n) w.wake[i] = 0u;  // static_wake_clear.wgsl
+}
+
+// New resting list = kept manifolds then frozen manifolds, keyed by (min id, max id).
+__global__ void __launch_bounds__(kThreads) k_rest_gather(W w, const uint32_t* keep_dest,
+                                                          const uint32_t* keep_total,
+                                                          const uint32_t* freeze_dest,
+                                                          uint32_t* out_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t kept = *keep_total;
+    const uint32_t old = *w.nresting;
+    const uint32_t ncont = *w.ncontacts;
+    if (i == 0) {
+        // freeze_dest totals are written after the map by k_scan_blocks into scan_total[1]
+        const uint32_t total = kept + w.scan_total[1];
+        const uint32_t capped = min(total, w.contact_cap);
+        *out_count = capped;
+        *cw(w.counters, DYN_COUNTER_REFUSED_RESTING) = total - capped;
+        *cw(w.counters, DYN_COUNTER_RESTING) = capped;
+    }
+    if (i < old && keep_dest[i] != kNoSlot) {
+        const uint32_t at = keep_dest[i];
+        if (at < w.contact_cap) {
+            const dyn_contact& c = w.resting[i];
+            w.rest_tmp[at] = c;
+            w.rest_major[at] = min(c.first_body_id, c.second_body_id);
+            w.rest_minor[at] = max(c.first_body_id, c.second_body_id);
+            w.rest_slot[at] = at;
+        }
+    }
+    if (i < ncont && freeze_dest[i] != kNoSlot) {
+        const uint32_t at = kept + freeze_dest[i];
+        if (at < w.contact_cap) {
+            const dyn_contact& c = w.contacts[i];
+            w.rest_tmp[at] = c;
+            w.rest_major[at] = min(c.first_body_id, c.second_body_id);
+            w.rest_minor[at] = max(c.first_body_id, c.second_body_id);
+            w.rest_slot[at] = at;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_rest_place(W w, const uint32_t* count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *w.nresting = *count;
+    if (i >= *count) return;
+    w.resting[i] = w.rest_tmp[w.rest_slot[i]];
+}
+
+__global__ void k_consume(W w) {
+    if (threadIdx.x == 0) *cw(w.counters, DYN_COUNTER_STEP) += 1u;
+}
+
+}  // namespace
+
+}  // namespace dyn
+
+// ---- host side: the World object behind the C ABI ---------------------------------------------
+
+struct dyn_world {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    dyn::W w{};
+    uint32_t entry_cap = 0;
+    uint32_t* entry_keys = nullptr;
+    uint32_t* entry_order = nullptr;
+    dyn_grid_entry* entries = nullptr;
+    uint32_t* words = nullptr;        // small device words (counts)
+    uint32_t* freeze_flags = nullptr;
+    uint32_t* keep_dest = nullptr;
+    uint32_t* freeze_dest = nullptr;
+    void* bp_scratch = nullptr;
+    size_t bp_scratch_bytes = 0;
+    void* sort_scratch = nullptr;
+    size_t sort_scratch_bytes = 0;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+template <typename T>
+int alloc(dyn_world* world, T** ptr, size_t count) {
+    void* p = nullptr;
+    DYN_HIP(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
+    DYN_HIP(hipMemset(p, 0, (count ? count : 1) * sizeof(T)));
+    world->owned.push_back(p);
+    *ptr = static_cast<T*>(p);
+    return DYN_OK;
+}
+
+uint32_t blocks_for(uint32_t n) { return dyn::ceil_div(n ? n : 1u, dyn::kThreads); }
+
+int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_t limit,
+         uint32_t* dest, uint32_t* total) {
+    using namespace dyn;
+    hipStream_t st = world->stream;
+    const uint32_t blocks = ceil_div(limit ? limit : 1u, kScanBlock);
+    {
+        ProfileScope scope("k_scan_count", st);
+        k_scan_count<<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks);
+    }
+    DYN_LAUNCHED("k_scan_count");
+    {
+        ProfileScope scope("k_scan_blocks", st);
+        k_scan_blocks<<<1, 1024, 0, st>>>(world->w.scan_blocks, blocks, total);
+    }
+    DYN_LAUNCHED("k_scan_blocks");
+    {
+        ProfileScope scope("k_scan_dest", st);
+        k_scan_dest<<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks, dest);
+    }
+    DYN_LAUNCHED("k_scan_dest");
+    return DYN_OK;
+}
+
+#define LAUNCH(name, grid, ...)                                            \
+    do {                                                                   \
+        {                                                                  \
+            ::dyn::ProfileScope scope_(#name, world->stream);              \
+            name<<<(grid), ::dyn::kThreads, 0, world->stream>>>(__VA_ARGS__); \
+        }                                                                  \
+        DYN_LAUNCHED(#name);                                               \
+    } while (0)
+
+int world_step(dyn_world* world, float dt) {
+    using namespace dyn;
+    W& w = world->w;
+    w.p.dt = dt;
+    w.p.substep_dt = dt / static_cast<float>(w.p.substeps);
+    const uint32_t nb = blocks_for(w.n), db = blocks_for(w.d), kb = blocks_for(w.k);
+    const uint32_t pb = blocks_for(w.pair_cap), cb = blocks_for(w.contact_cap);
+    hipStream_t st = world->stream;
+    {
+        ProfileScope scope("k_reset_counters", st);
+        k_reset_counters<<<1, 64, 0, st>>>(w);
+    }
+    DYN_LAUNCHED("k_reset_counters");
+    LAUNCH(k_begin, nb, w);
+    LAUNCH(k_aabb, kb, w);
+    dyn_broadphase_io io{};
+    io.d_aabbs = w.aabbs;
+    io.d_groups = w.owner;
+    io.d_flags = w.flags;
+    io.d_resolution = w.res;
+    io.colliders = w.k;
+    io.d_entry_keys = world->entry_keys;
+    io.d_entry_order = world->entry_order;
+    io.d_entries = world->entries;
+    io.entry_capacity = world->entry_cap;
+    io.d_pair_major = w.pmaj;
+    io.d_pair_minor = w.pmin;
+    io.pair_capacity = w.pair_cap;
+    io.d_counters = w.counters;
+    io.sort_pairs = 1;
+    io.d_scratch = world->bp_scratch;
+    io.scratch_bytes = world->bp_scratch_bytes;
+    DYN_TRY(dyn_broadphase(st, &io));
+    LAUNCH(k_narrow, pb, w);
+    const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
+    DYN_TRY(scan(world, w.valid, pair_count, w.pair_cap, w.dest, &w.scan_total[0]));
+    LAUNCH(k_contacts_scatter, pb, w);
+    {
+        ProfileScope scope("k_contacts_total", st);
+        k_contacts_total<<<1, 64, 0, st>>>(w);
+    }
+    DYN_LAUNCHED("k_contacts_total");
+    LAUNCH(k_relay, cb, w);
+    LAUNCH(k_contact_begin, cb, w, w.p.slop);
+    LAUNCH(k_island_init, db, w);
+    LAUNCH(k_island_link, cb, w, w.p.slop);
+    LAUNCH(k_island_link_resting, cb, w);
+    uint32_t rounds = 0;
+    for (uint32_t v = w.d < 2u ? 2u : w.d; v > 1u; v >>= 1) ++rounds;
+    rounds += 1;  // ilog2(max(dynamic, 2)) + 1, crates/dynamis-rigid/src/lib.rs:79-81
+    for (uint32_t r = 0; r < rounds; ++r) LAUNCH(k_island_flatten, db, w);
+    LAUNCH(k_island_aggregate, db, w);
+    LAUNCH(k_island_wake, db, w);
+    LAUNCH(k_solver_reset, nb, w);
+    LAUNCH(k_blocks, cb, w);
+    LAUNCH(k_rows_count, nb, w);
+    for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
+        LAUNCH(k_integrate, db, w);
+        if (sub == 0) {
+            LAUNCH(k_warm, cb, w);
+            LAUNCH(k_apply, nb, w);
+        }
+        for (uint32_t it = 0; it < w.p.solve_iterations; ++it) {
+            // solver_block_solve.wgsl:85-87: the residual round is the one where the round
+            // counter (reset by integrate, bumped by every apply) equals the iteration count,
+            // which only the warm-started first substep reaches.
+            LAUNCH(k_solve, cb, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
+            LAUNCH(k_apply, nb, w);
+        }
+        LAUNCH(k_advance, db, w);
+        for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
+            LAUNCH(k_position_block, cb, w);
+            LAUNCH(k_position_apply, nb, w);
+        }
+    }
+    LAUNCH(k_sleep, db, w);
+    LAUNCH(k_thaw, cb, w);
+    LAUNCH(k_archive, cb, w, world->freeze_flags);
+    DYN_TRY(scan(world, w.rest_flags, w.nresting, w.contact_cap, world->keep_dest, &w.scan_total[2]));
+    DYN_TRY(scan(world, world->freeze_flags, w.ncontacts, w.contact_cap, world->freeze_dest, &w.scan_total[1]));
+    LAUNCH(k_archive_count, nb, w);
+    LAUNCH(k_rest_gather, cb, w, world->keep_dest, &w.scan_total[2], world->freeze_dest, &world->words[0]);
+    // Sort the new resting list by (min id, max id); payload = its slot.
+    const uint32_t words = dyn_sort_key_words(w.id_cap ? w.id_cap : 1u);
+    DYN_TRY(dyn_sort_pairs(st, w.rest_major, w.rest_minor, w.rest_slot, w.contact_cap, &world->words[0],
+                           nullptr, 1u, words, words, world->sort_scratch, world->sort_scratch_bytes));
+    LAUNCH(k_rest_place, cb, w, &world->words[0]);
+    {
+        ProfileScope scope("k_consume", st);
+        k_consume<<<1, 64, 0, st>>>(w);
+    }
+    DYN_LAUNCHED("k_consume");
+    return DYN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* params,
+                     const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
+                     uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
+                     uint32_t collider_count, uint32_t contact_capacity) {
+    using namespace dyn;
+    if (!out || !params || (body_count && (!states || !descs)) ||
+        (collider_count && (!colliders || !owners))) {
+        return fail(DYN_E_INVALID, "dyn_world_create: null argument");
+    }
+    if (dynamic_count > body_count) return fail(DYN_E_INVALID, "dyn_world_create: dynamic_count > body_count");
+    if (params->substeps == 0 || params->solve_iterations == 0 || params->position_iterations == 0) {
+        return fail(DYN_E_INVALID, "dyn_world_create: substeps and iterations must be positive");
+    }
+    for (uint32_t i = 0; i < collider_count; ++i) {
+        if (owners[i] >= body_count) return fail(DYN_E_INVALID, "dyn_world_create: collider owner out of range");
+    }
+    // the id -> row table beside the rows (pool.rs rows): ids are unique
+    uint32_t id_cap = 0;
+    for (uint32_t i = 0; i < body_count; ++i) {
+        if (states[i].body_id >= DYN_NO_BODY - 1u) return fail(DYN_E_INVALID, "dyn_world_create: body_id out of range");
+        id_cap = states[i].body_id + 1u > id_cap ? states[i].body_id + 1u : id_cap;
+    }
+    std::vector<uint32_t> row_of_body(id_cap, DYN_NO_BODY);
+    for (uint32_t i = 0; i < body_count; ++i) {
+        if (row_of_body[states[i].body_id] != DYN_NO_BODY) return fail(DYN_E_INVALID, "dyn_world_create: duplicate body_id");
+        row_of_body[states[i].body_id] = i;
+    }
+    *out = nullptr;
+    dyn_world* world = new dyn_world();
+    auto cleanup = [&](int rc) {
+        for (void* p : world->owned) (void)hipFree(p);
+        if (world->own_stream && world->stream) (void)hipStreamDestroy(world->stream);
+        delete world;
+        return rc;
+    };
+    if (stream) {
+        world->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&world->stream, hipStreamNonBlocking) != hipSuccess) {
+            return cleanup(fail(DYN_E_HIP, "dyn_world_create: stream"));
+        }
+        world->own_stream = true;
+    }
+    W& w = world->w;
+    w.id_cap = id_cap;
+    w.p = *params;
+    w.n = body_count;
+    w.d = dynamic_count;
+    w.k = collider_count;
+    w.pair_cap = contact_capacity;
+    w.contact_cap = contact_capacity;
+    world->entry_cap = DYN_MAX_CELLS_PER_COLLIDER * (collider_count ? collider_count : 1u);
+    int rc = DYN_OK;
+#define A(ptr, count) if (rc == DYN_OK) rc = alloc(world, &(ptr), (count))
+    A(w.st, body_count);
+    A(w.de, body_count);
+    A(w.co, collider_count);
+    A(w.owner, collider_count);
+    A(w.row_of_body, id_cap);
+    A(w.wake, body_count);
+    A(w.activity, body_count);
+    A(w.motion, body_count);
+    A(w.counters, DYN_COUNTER_BLOCK_WORDS);
+    A(w.aabbs, collider_count);
+    A(w.res, collider_count);
+    A(w.flags, collider_count);
+    A(w.pmaj, contact_capacity);
+    A(w.pmin, contact_capacity);
+    A(w.raw, contact_capacity);
+    A(w.valid, contact_capacity);
+    A(w.dest, contact_capacity);
+    A(w.contacts, contact_capacity);
+    A(w.matched, contact_capacity);
+    A(w.archive, contact_capacity);
+    A(w.resting, contact_capacity);
+    A(w.rest_tmp, contact_capacity);
+    A(w.rest_major, contact_capacity);
+    A(w.rest_minor, contact_capacity);
+    A(w.rest_slot, contact_capacity);
+    A(w.rest_flags, contact_capacity);
+    A(w.parents, body_count);
+    A(w.istate, body_count);
+    A(w.block_counts, body_count);
+    A(w.blocks, 2u * static_cast<size_t>(contact_capacity));
+    A(w.targets, 4u * static_cast<size_t>(contact_capacity));
+    A(w.resolution, 4u * static_cast<size_t>(body_count));
+    A(w.vdeltas, 8u * static_cast<size_t>(body_count));
+    A(w.pdeltas, 8u * static_cast<size_t>(body_count));
+    A(w.contrib, body_count);
+    A(w.scan_blocks, ceil_div(contact_capacity ? contact_capacity : 1u, kScanBlock) + 1u);
+    A(w.scan_total, 4);
+    A(world->words, 8);
+    A(world->entry_keys, world->entry_cap);
+    A(world->entry_order, world->entry_cap);
+    A(world->entries, world->entry_cap);
+    A(world->freeze_flags, contact_capacity);
+    A(world->keep_dest, contact_capacity);
+    A(world->freeze_dest, contact_capacity);
+#undef A
+    if (rc != DYN_OK) return cleanup(rc);
+    w.ncontacts = world->words + 1;
+    w.narchive = world->words + 2;
+    w.nresting = world->words + 3;
+    world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(collider_count, world->entry_cap, contact_capacity);
+    world->sort_scratch_bytes = dyn_sort_scratch_bytes(contact_capacity, 2u * dyn_sort_key_words(id_cap ? id_cap : 1u));
+    if (hipMalloc(&world->bp_scratch, world->bp_scratch_bytes) != hipSuccess ||
+        hipMalloc(&world->sort_scratch, world->sort_scratch_bytes) != hipSuccess) {
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: scratch"));
+    }
+    world->owned.push_back(world->bp_scratch);
+    world->owned.push_back(world->sort_scratch);
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = DYN_E_HIP;
+    };
+    up(w.st, states, sizeof(dyn_body_state) * body_count);
+    up(w.de, descs, sizeof(dyn_body_desc) * body_count);
+    up(w.co, colliders, sizeof(dyn_collider) * collider_count);
+    up(w.row_of_body, row_of_body.data(), sizeof(uint32_t) * id_cap);
+    up(w.owner, owners, 4u * collider_count);
+    if (rc != DYN_OK) return cleanup(fail(DYN_E_HIP, "dyn_world_create: upload"));
+    *out = world;
+    return DYN_OK;
+}
+
+int dyn_world_destroy(dyn_world* world) {
+    if (!world) return DYN_OK;
+    (void)hipStreamSynchronize(world->stream);
+    for (void* p : world->owned) (void)hipFree(p);
+    if (world->own_stream) (void)hipStreamDestroy(world->stream);
+    delete world;
+    return DYN_OK;
+}
+
+int dyn_world_step(dyn_world* world, float dt) {
+    if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_step: null world");
+    if (!(dt > 0.0f)) return dyn::fail(DYN_E_INVALID, "dyn_world_step: dt must be positive");
+    return world_step(world, dt);
+}
+
+int dyn_world_wait(dyn_world* world) {
+    if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_wait: null world");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    return DYN_OK;
+}
+
+void* dyn_world_stream(dyn_world* world) { return world ? world->stream : nullptr; }
+
+int dyn_world_read_states(dyn_world* world, dyn_body_state* out) {
+    if (!world || !out) return dyn::fail(DYN_E_INVALID, "dyn_world_read_states: null argument");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    if (world->w.n) DYN_HIP(hipMemcpy(out, world->w.st, sizeof(dyn_body_state) * world->w.n, hipMemcpyDeviceToHost));
+    return DYN_OK;
+}
+
+int dyn_world_read_counters(dyn_world* world, uint32_t* out) {
+    if (!world || !out) return dyn::fail(DYN_E_INVALID, "dyn_world_read_counters: null argument");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    DYN_HIP(hipMemcpy(out, world->w.counters, 4u * DYN_COUNTER_BLOCK_WORDS, hipMemcpyDeviceToHost));
+    return DYN_OK;
+}
+
+int dyn_world_read_contacts(dyn_world* world, dyn_contact* out, uint32_t capacity, uint32_t* count) {
+    if (!world || !count) return dyn::fail(DYN_E_INVALID, "dyn_world_read_contacts: null argument");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    uint32_t n = 0;
+    DYN_HIP(hipMemcpy(&n, world->w.ncontacts, 4u, hipMemcpyDeviceToHost));
+    *count = n;
+    const uint32_t copy = n < capacity ? n : capacity;
+    if (copy && out) DYN_HIP(hipMemcpy(out, world->w.contacts, sizeof(dyn_contact) * copy, hipMemcpyDeviceToHost));
+    return DYN_OK;
+}
+
+int dyn_world_read_pairs(dyn_world* world, uint32_t* major, uint32_t* minor, uint32_t capacity,
+                         uint32_t* count) {
+    if (!world || !count) return dyn::fail(DYN_E_INVALID, "dyn_world_read_pairs: null argument");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    uint32_t pairs = 0;
+    DYN_HIP(hipMemcpy(&pairs, world->w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS, 4u,
+                      hipMemcpyDeviceToHost));
+    const uint32_t n = pairs < world->w.pair_cap ? pairs : world->w.pair_cap;
+    *count = n;
+    const uint32_t copy = n < capacity ? n : capacity;
+    if (copy && major) DYN_HIP(hipMemcpy(major, world->w.pmaj, 4u * copy, hipMemcpyDeviceToHost));
+    if (copy && minor) DYN_HIP(hipMemcpy(minor, world->w.pmin, 4u * copy, hipMemcpyDeviceToHost));
+    return DYN_OK;
+}
+
+}  // extern "C"

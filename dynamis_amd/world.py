@@ -1,0 +1,263 @@
+"""Host side of the rigid-body step (dyn_world_* in include/dynamis.h).
+
+Mirrors the reference's `World` for the rigid path (crates/dynamis-world/src/lib.rs:86 new,
+src/body.rs:108 spawn, :326 read_state, src/step.rs:34 step, src/observation.rs:484 wait,
+src/readback.rs:95 inspect_contacts, :215 measured): bodies with one collider each are described
+on the host, encoded into the device records by the library (dyn_encode_body), and stepped on the
+GPU. Rows follow the reference's partition (crates/dynamis-world/src/body.rs:292-321): simulated
+bodies first, then static and kinematic ones; a handle is the spawn index.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from ._native import call, lib
+
+
+def default_config() -> np.ndarray:
+    """PhysicsConfig::default (crates/dynamis-world/src/config.rs:76-100) as a CONFIG record."""
+    config = np.zeros(1, dtype=abi.CONFIG)
+    lib().dyn_config_default(config.ctypes.data)
+    return config
+
+
+@dataclass
+class BodyDesc:
+    """One body with one collider at its origin (crates/dynamis-model/src/body.rs:89-108 and
+    collider.rs:52-68). `density` >= 0 derives the mass from the shape's volume
+    (BodyDesc::density); a negative density stands for None, and `mass` is then the body's
+    mass (BodyDesc::new's 1.0 by default); a zero mass makes a static body."""
+
+    shape: int
+    radius: float = 0.0
+    half_height: float = 0.0
+    half_extents: tuple = (0.0, 0.0, 0.0)
+    position: tuple = (0.0, 0.0, 0.0)
+    orientation: tuple = (0.0, 0.0, 0.0, 1.0)
+    velocity: tuple = (0.0, 0.0, 0.0)
+    angular_velocity: tuple = (0.0, 0.0, 0.0)
+    mass: float = 1.0
+    density: float = -1.0
+    friction: float = 0.5
+    restitution: float = 0.0
+    rolling_friction: float = 0.0
+    spin_friction: float = 0.0
+    gravity_scale: float = 1.0
+    linear_damping: float = -1.0
+    angular_damping: float = -1.0
+    kinematic: bool = False
+    collision_group: int = 1
+    collision_mask: int = abi.NO_COLLISION_FILTER
+
+    # constructors named after crates/dynamis-model/src/shape.rs
+    @staticmethod
+    def sphere(radius: float, **kw) -> "BodyDesc":
+        return BodyDesc(shape=abi.SHAPE_SPHERE, radius=radius, **kw)
+
+    @staticmethod
+    def cuboid(half_extents, **kw) -> "BodyDesc":
+        return BodyDesc(shape=abi.SHAPE_CUBOID, half_extents=tuple(half_extents), **kw)
+
+    @staticmethod
+    def capsule(radius: float, half_height: float, **kw) -> "BodyDesc":
+        return BodyDesc(shape=abi.SHAPE_CAPSULE, radius=radius, half_height=half_height, **kw)
+
+    @staticmethod
+    def plane(**kw) -> "BodyDesc":
+        kw.setdefault("mass", 0.0)
+        return BodyDesc(shape=abi.SHAPE_PLANE, **kw)
+
+    def fixed(self) -> "BodyDesc":
+        """A static body: no mass, never moves (BodyKind::of with zero mass)."""
+        self.density = -1.0
+        self.mass = 0.0
+        return self
+
+    def spec(self) -> np.ndarray:
+        s = np.zeros(1, dtype=abi.BODY_SPEC)
+        for name in abi.BODY_SPEC.names:
+            value = getattr(self, name)
+            s[name] = int(value) if name == "kinematic" else value
+        return s
+
+
+@dataclass
+class Scene:
+    """Device records of a set of bodies, in row order, ready for dyn_world_create."""
+
+    params: np.ndarray
+    states: np.ndarray
+    descs: np.ndarray
+    colliders: np.ndarray
+    owners: np.ndarray
+    dynamic_count: int
+    rows: np.ndarray                 # handle -> row
+    handles: np.ndarray = field(default=None)  # row -> handle
+
+
+def spawn_rows(simulates: list[bool]) -> list[int]:
+    """Row of each spawned body after the reference's spawn bookkeeping for a fresh world:
+    a body is appended at the end of the rows (pool.rs:118-127 with compact removal), and a
+    simulated body is swapped with the first non-simulated row (body.rs:292-321
+    migrate_partition). Returns row -> handle."""
+    alive: list[int] = []
+    dynamic = 0
+    for handle, sim in enumerate(simulates):
+        slot = len(alive)
+        alive.append(handle)
+        if sim:
+            if slot != dynamic:
+                alive[slot], alive[dynamic] = alive[dynamic], alive[slot]
+            dynamic += 1
+    return alive
+
+
+def encode(bodies: list[BodyDesc], config: np.ndarray | None = None) -> Scene:
+    """Encodes bodies spawned in this order into device records with the library's host
+    encoder. Body handle ids are the spawn indices; collider slots follow spawn order (the
+    collider arena, collider.rs:180-214); rows follow the simulated-first partition."""
+    config = default_config() if config is None else config
+    specs = [b.spec() for b in bodies]
+    n = len(specs)
+    enc_states = np.zeros(n, dtype=abi.BODY_STATE)
+    enc_descs = np.zeros(n, dtype=abi.BODY_DESC)
+    colliders = np.zeros(n, dtype=abi.COLLIDER)
+    simulates = []
+    for handle, spec in enumerate(specs):
+        call("dyn_encode_body", spec.ctypes.data, config.ctypes.data, handle, 0,
+             enc_states[handle:handle + 1].ctypes.data, enc_descs[handle:handle + 1].ctypes.data,
+             colliders[handle:handle + 1].ctypes.data)
+        d = enc_descs[handle]
+        simulates.append(bool(d["inverse_mass"] > 0.0 and not (d["flags"] & abi.BODY_KINEMATIC)))
+    order = spawn_rows(simulates)
+    rows = np.zeros(n, dtype=np.uint32)
+    for row, handle in enumerate(order):
+        rows[handle] = row
+    states = enc_states[order].copy() if n else enc_states
+    descs = enc_descs[order].copy() if n else enc_descs
+    dynamic_count = int(sum(simulates))
+    params = np.zeros(1, dtype=abi.STEP_PARAMS)
+    call("dyn_step_params_build", config.ctypes.data, n, dynamic_count, n, params.ctypes.data)
+    return Scene(params=params, states=states, descs=descs, colliders=colliders,
+                 owners=rows.copy(), dynamic_count=dynamic_count, rows=rows,
+                 handles=np.asarray(order, dtype=np.uint32))
+
+
+class World:
+    """A device world. Bodies are spawned before the first step; `step(dt)` enqueues one step
+    on the world's stream, reads wait for it."""
+
+    def __init__(self, config: np.ndarray | None = None, contact_capacity: int | None = None):
+        self.config = default_config() if config is None else config.copy()
+        self.contact_capacity = contact_capacity
+        self._bodies: list[BodyDesc] = []
+        self._handle = ctypes.c_void_p()
+        self.scene: Scene | None = None
+
+    # -- bodies ------------------------------------------------------------------------------
+    def spawn(self, desc: BodyDesc) -> int:
+        if self.scene is not None:
+            raise RuntimeError("bodies are spawned before the first step of a world")
+        self._bodies.append(desc)
+        return len(self._bodies) - 1
+
+    def count(self) -> int:
+        return len(self._bodies)
+
+    def _ensure(self) -> None:
+        if self.scene is not None:
+            return
+        if not self._bodies:
+            raise RuntimeError("a world needs at least one body")
+        self.scene = encode(self._bodies, self.config)
+        self.create(self.scene, self.contact_capacity)
+
+    def create(self, scene: Scene, contact_capacity: int | None = None) -> None:
+        """Creates the device world from already-encoded records."""
+        self.scene = scene
+        n = len(scene.states)
+        capacity = int(contact_capacity if contact_capacity is not None else max(64, 8 * n))
+        call("dyn_world_create", ctypes.byref(self._handle), None, scene.params.ctypes.data,
+             scene.states.ctypes.data, scene.descs.ctypes.data, n, scene.dynamic_count,
+             scene.colliders.ctypes.data, scene.owners.ctypes.data, n, capacity)
+        self.capacity = capacity
+
+    # -- stepping ----------------------------------------------------------------------------
+    def step(self, dt: float) -> None:
+        self._ensure()
+        call("dyn_world_step", self._handle, ctypes.c_float(dt))
+
+    def wait(self) -> None:
+        if self.scene is not None:
+            call("dyn_world_wait", self._handle)
+
+    # -- reads -------------------------------------------------------------------------------
+    def states(self) -> np.ndarray:
+        """Body states in row order."""
+        self._ensure()
+        out = np.zeros(len(self.scene.states), dtype=abi.BODY_STATE)
+        call("dyn_world_read_states", self._handle, out.ctypes.data)
+        return out
+
+    def read_state(self, handle: int) -> np.ndarray:
+        return self.states()[int(self.scene.rows[handle])]
+
+    def measured(self) -> dict[str, int]:
+        """Counters of the last step by name (crates/dynamis-abi/src/counter.rs slots)."""
+        self._ensure()
+        block = np.zeros(abi.COUNTER_BLOCK_WORDS, dtype=np.uint32)
+        call("dyn_world_read_counters", self._handle, block.ctypes.data)
+        return {name: abi.counter(block, name) for name in abi.COUNTER_NAMES}
+
+    def solve_residual(self) -> tuple[float, float]:
+        """Largest linear and angular velocity change of the last solve round of the first
+        substep (fixed-point words of crates/dynamis-world/src/readback.rs:219-229)."""
+        m = self.measured()
+        scale = float(abi.SOLVER_VELOCITY_SCALE)
+        return (np.int32(np.uint32(m["SOLVE_LINEAR_RESIDUAL"])) / scale,
+                np.int32(np.uint32(m["SOLVE_ANGULAR_RESIDUAL"])) / scale)
+
+    def inspect_contacts(self) -> np.ndarray:
+        """Contacts of the last step in canonical pair order (CONTACT records)."""
+        self._ensure()
+        out = np.zeros(self.capacity, dtype=abi.CONTACT)
+        count = ctypes.c_uint32()
+        call("dyn_world_read_contacts", self._handle, out.ctypes.data, self.capacity,
+             ctypes.byref(count))
+        return out[:count.value].copy()
+
+    def pairs(self) -> np.ndarray:
+        """Sorted candidate pairs of the last step as (P, 2) collider indices."""
+        self._ensure()
+        major = np.zeros(self.capacity, dtype=np.uint32)
+        minor = np.zeros(self.capacity, dtype=np.uint32)
+        count = ctypes.c_uint32()
+        call("dyn_world_read_pairs", self._handle, major.ctypes.data, minor.ctypes.data,
+             self.capacity, ctypes.byref(count))
+        return np.stack([major[:count.value], minor[:count.value]], axis=1)
+
+    def stream(self) -> int:
+        self._ensure()
+        return lib().dyn_world_stream(self._handle)
+
+    def close(self) -> None:
+        if self._handle.value:
+            call("dyn_world_destroy", self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self) -> "World":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()

@@ -138,6 +138,105 @@ size_t dyn_broadphase_scratch_bytes(uint32_t colliders, uint32_t entry_capacity,
  */
 int dyn_broadphase(void* stream, const dyn_broadphase_io* io);
 
+/* ---- host record encoding (crates/dynamis-model/src/config.rs:12-100 PhysicsConfig,
+ *      body.rs:89 BodyDesc; crates/dynamis-abi/src/body.rs:96-139, collider.rs:47-102,
+ *      param.rs:110-137; crates/dynamis-model/src/mass.rs) -------------------------------------- */
+
+/* PhysicsConfig (config.rs:12-33). Combine modes: 0 multiply (sqrt(a*b)), 1 min, 2 max, 3 average. */
+typedef struct dyn_config {
+    float gravity[3];
+    float damping;
+    float angular_damping;
+    uint32_t substeps;
+    uint32_t solve_iterations;
+    uint32_t position_iterations;
+    float relaxation;
+    float slop;
+    float contact_margin;
+    float restitution_threshold;
+    float max_velocity;
+    float max_angular_velocity;
+    float sleep_velocity;
+    float sleep_angular_velocity;
+    float sleep_time;
+    uint32_t friction_combine;
+    uint32_t restitution_combine;
+} dyn_config;
+
+/* A BodyDesc with one collider at the body origin (model/body.rs:89-108, collider.rs:52-68).
+ * density >= 0 selects BodyDesc::density, otherwise `mass` is the fixed mass (0 = static);
+ * damping values < 0 take the configuration's. */
+typedef struct dyn_body_spec {
+    uint32_t shape; /* DYN_SHAPE_SPHERE, _CUBOID, _CAPSULE or _PLANE */
+    float radius;
+    float half_height;
+    float half_extents[3];
+    float position[3];
+    float orientation[4];
+    float velocity[3];
+    float angular_velocity[3];
+    float mass;
+    float density;
+    float friction;
+    float restitution;
+    float rolling_friction;
+    float spin_friction;
+    float gravity_scale;
+    float linear_damping;
+    float angular_damping;
+    uint32_t kinematic;
+    uint32_t collision_group;
+    uint32_t collision_mask;
+} dyn_body_spec;
+
+/* PhysicsConfig::default (config.rs:76-100). */
+void dyn_config_default(dyn_config* config);
+/* StepParamsRecord::new without the step length (param.rs:110-137). */
+int dyn_step_params_build(const dyn_config* config, uint32_t body_count, uint32_t dynamic_count,
+                          uint32_t collider_count, dyn_step_params* out);
+/* inertia_inverse (mass.rs:259-280); fails on a tensor that is not positive definite. */
+int dyn_inertia_inverse(const float* inertia, float* inverse);
+/* BodyStateRecord::initial + BodyDescriptorRecord::build + ColliderRecord::build for one body
+ * (body.rs:34-58, :96-139, collider.rs:47-102). `body_id` is the body's handle id, which the
+ * state record carries; `collider_slot` is the collider's index within its body (0 for a body
+ * with one collider). */
+int dyn_encode_body(const dyn_body_spec* spec, const dyn_config* config, uint32_t body_id,
+                    uint32_t collider_slot, dyn_body_state* state, dyn_body_desc* desc,
+                    dyn_collider* collider);
+
+/* ---- world step (crates/dynamis-world/src/lib.rs:86 World::new, src/step.rs:34 World::step,
+ *      src/observation.rs:484 wait, src/body.rs:326 read_state, src/readback.rs:95
+ *      inspect_contacts, :215 measured) ---------------------------------------------------------- */
+
+typedef struct dyn_world dyn_world;
+
+/*
+ * Creates a world from device-format records (host arrays, uploaded once): bodies in rows
+ * [0, body_count) with the simulated ones first in [0, dynamic_count) (the reference's row
+ * partition, crates/dynamis-world/src/body.rs:292-321), each carrying a unique body_id (its
+ * handle id; the world keeps the id -> row table of body_row.wgsl), colliders in collider-slot
+ * order with the rows of their owners (upload.rs:133-175). `params` is the StepParams record (dt and substep_dt are set by each step).
+ * `stream` (hipStream_t) orders all of the world's work; NULL creates a private stream.
+ * contact_capacity bounds candidate pairs, contacts and resting manifolds.
+ */
+int dyn_world_create(dyn_world** world, void* stream, const dyn_step_params* params,
+                     const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
+                     uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
+                     uint32_t collider_count, uint32_t contact_capacity);
+int dyn_world_destroy(dyn_world* world);
+/* Enqueues one step of `dt` seconds on the world's stream (non-blocking). */
+int dyn_world_step(dyn_world* world, float dt);
+/* Blocks until every enqueued step has finished. */
+int dyn_world_wait(dyn_world* world);
+/* The world's hipStream_t. */
+void* dyn_world_stream(dyn_world* world);
+/* Synchronous reads (they wait for the stream first). */
+int dyn_world_read_states(dyn_world* world, dyn_body_state* out);
+int dyn_world_read_counters(dyn_world* world, uint32_t* out);
+int dyn_world_read_contacts(dyn_world* world, dyn_contact* out, uint32_t capacity, uint32_t* count);
+int dyn_world_read_pairs(dyn_world* world, uint32_t* major, uint32_t* minor, uint32_t capacity,
+                         uint32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

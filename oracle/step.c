@@ -50,6 +50,8 @@ struct oracle_world {
     dyn_body_desc* de;
     dyn_collider* co;
     uint32_t* owner;
+    uint32_t* row_of_body; /* body id -> row, ids < id_capacity */
+    uint32_t id_capacity;
     uint32_t* wake;
     uint32_t* admitted;
     uint32_t* activity;
@@ -468,7 +470,6 @@ static dyn_contact capsule_capsule(const body_t* a, const dyn_collider* ca, cons
     float radius_sum = ca->radius + cb->radius;
     manifold_emit(&c, sign_normalize(delta));
     if (distance > radius_sum + margin) return c;
-    v3 normal = sign_normalize(delta);
     float depth = radius_sum - distance;
     v3 point = vmul(vadd(pa, pb), 0.5f);
     manifold_push(&c, point, depth, FEATURE_POINT);
@@ -962,11 +963,14 @@ static dyn_contact relay_impulses(const dyn_contact* current, const dyn_contact*
     }
     return r;
 }
+/* body_row.wgsl:1-13 */
 static uint32_t resolve_row(const oracle_world* w, uint32_t id, uint32_t gen) {
-    if (id >= w->n) return DYN_NO_BODY;
-    const dyn_body_state* s = &w->st[id];
+    if (id >= w->id_capacity) return DYN_NO_BODY;
+    uint32_t row = w->row_of_body[id];
+    if (row >= w->n) return DYN_NO_BODY;
+    const dyn_body_state* s = &w->st[row];
     if (s->body_id != id || s->generation != gen) return DYN_NO_BODY;
-    return id;
+    return row;
 }
 static uint32_t current_slot(const oracle_world* w, const dyn_contact* c) {
     uint32_t lo = 0, hi = w->ncontacts;
@@ -1645,7 +1649,9 @@ void oracle_world_step(oracle_world* w, float dt) {
             block_apply(w);
         }
         for (uint32_t it = 0; it < w->p.solve_iterations; it++) {
-            int residual = it + 1u == w->p.solve_iterations;
+            /* solver_block_solve.wgsl:85-87 with the round counter of solver_block_apply.wgsl:
+               only the warm-started first substep reaches rounds == solve_iterations */
+            int residual = sub == 0 && it + 1u == w->p.solve_iterations;
             for (uint32_t ci = 0; ci < w->ncontacts; ci++) solve_block(w, ci, residual);
             block_apply(w);
         }
@@ -1688,7 +1694,7 @@ void oracle_world_step(oracle_world* w, float dt) {
 
 /* ---- lifetime and accessors ------------------------------------------------------------- */
 
-#define ALLOC(ptr, count) ((ptr) = calloc((count) ? (count) : 1u, sizeof *(ptr)))
+#define ALLOC(ptr, count) ((ptr) = calloc(((count) != 0u) ? (count) : 1u, sizeof *(ptr)))
 
 oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_state* states,
                                   const dyn_body_desc* descs, uint32_t body_count,
@@ -1708,6 +1714,14 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
     memcpy(w->de, descs, (size_t)body_count * sizeof *descs);
     memcpy(w->co, colliders, (size_t)collider_count * sizeof *colliders);
     memcpy(w->owner, owners, (size_t)collider_count * 4u);
+    /* the id -> row table the host keeps beside the rows (pool.rs rows) */
+    w->id_capacity = 0;
+    for (uint32_t i = 0; i < body_count; i++) {
+        if (states[i].body_id + 1u > w->id_capacity) w->id_capacity = states[i].body_id + 1u;
+    }
+    ALLOC(w->row_of_body, w->id_capacity);
+    for (uint32_t i = 0; i < w->id_capacity; i++) w->row_of_body[i] = DYN_NO_BODY;
+    for (uint32_t i = 0; i < body_count; i++) w->row_of_body[states[i].body_id] = i;
     ALLOC(w->wake, body_count);
     ALLOC(w->admitted, body_count);
     ALLOC(w->activity, body_count);
@@ -1745,7 +1759,7 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
 
 void oracle_world_destroy(oracle_world* w) {
     if (!w) return;
-    void* ptrs[] = {w->st, w->de, w->co, w->owner, w->wake, w->admitted, w->activity, w->motion,
+    void* ptrs[] = {w->st, w->de, w->co, w->owner, w->row_of_body, w->wake, w->admitted, w->activity, w->motion,
                     w->aabbs, w->res, w->flags, w->keys, w->order, w->entries, w->pmaj, w->pmin,
                     w->raw, w->valid, w->contacts, w->matched, w->archive, w->resting,
                     w->parents, w->istate, w->live, w->block_counts, w->blocks, w->rows,

@@ -48,6 +48,22 @@ class Oracle:
         h.oracle_broadphase.restype = _u32
         h.oracle_broadphase.argtypes = [_p, _p, _p, _p, _u32, _p, _p, _p, _u32, _p, _p, _u32,
                                         _p, _u32]
+        h.oracle_world_create.restype = _p
+        h.oracle_world_create.argtypes = [_p, _p, _p, _u32, _u32, _p, _p, _u32, _u32]
+        h.oracle_world_destroy.restype = None
+        h.oracle_world_destroy.argtypes = [_p]
+        h.oracle_world_step.restype = None
+        h.oracle_world_step.argtypes = [_p, ctypes.c_float]
+        h.oracle_world_states.restype = None
+        h.oracle_world_states.argtypes = [_p, _p]
+        h.oracle_world_contacts.restype = _u32
+        h.oracle_world_contacts.argtypes = [_p, _p, _u32]
+        h.oracle_world_counters.restype = None
+        h.oracle_world_counters.argtypes = [_p, _p]
+        h.oracle_world_pairs.restype = _u32
+        h.oracle_world_pairs.argtypes = [_p, _p, _p, _u32]
+        h.oracle_inertia_inverse.restype = None
+        h.oracle_inertia_inverse.argtypes = [_p, _p]
 
     def units_for(self, live: int) -> int:
         return int(self.h.oracle_units_for(live))
@@ -102,6 +118,66 @@ class Oracle:
             "pairs": np.stack([major[:pairs], minor[:pairs]], axis=1),
             "counters": counters,
         }
+
+    def inertia_inverse(self, inertia):
+        tensor = np.ascontiguousarray(inertia, dtype=np.float32)
+        out = np.zeros(6, np.float32)
+        self.h.oracle_inertia_inverse(tensor.ctypes.data, out.ctypes.data)
+        return out
+
+    def world(self, scene, contact_capacity=None) -> "OracleWorld":
+        return OracleWorld(self, scene, contact_capacity)
+
+
+class OracleWorld:
+    """The oracle's serial restatement of World::step over the same device records."""
+
+    def __init__(self, oracle: Oracle, scene, contact_capacity=None):
+        self.o = oracle
+        n = len(scene.states)
+        self.n = n
+        self.capacity = int(contact_capacity if contact_capacity is not None else max(64, 8 * n))
+        self.scene = scene
+        self.w = oracle.h.oracle_world_create(
+            scene.params.ctypes.data, scene.states.ctypes.data, scene.descs.ctypes.data, n,
+            scene.dynamic_count, scene.colliders.ctypes.data, scene.owners.ctypes.data, n,
+            self.capacity)
+        if not self.w:
+            raise MemoryError("oracle_world_create failed")
+
+    def step(self, dt: float) -> None:
+        self.o.h.oracle_world_step(self.w, ctypes.c_float(dt))
+
+    def states(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=abi.BODY_STATE)
+        self.o.h.oracle_world_states(self.w, out.ctypes.data)
+        return out
+
+    def measured(self) -> dict:
+        block = np.zeros(abi.COUNTER_BLOCK_WORDS, dtype=np.uint32)
+        self.o.h.oracle_world_counters(self.w, block.ctypes.data)
+        return {name: abi.counter(block, name) for name in abi.COUNTER_NAMES}
+
+    def inspect_contacts(self) -> np.ndarray:
+        out = np.zeros(self.capacity, dtype=abi.CONTACT)
+        count = self.o.h.oracle_world_contacts(self.w, out.ctypes.data, self.capacity)
+        return out[:min(count, self.capacity)].copy()
+
+    def pairs(self) -> np.ndarray:
+        major = np.zeros(self.capacity, np.uint32)
+        minor = np.zeros(self.capacity, np.uint32)
+        count = self.o.h.oracle_world_pairs(self.w, major.ctypes.data, minor.ctypes.data,
+                                            self.capacity)
+        count = min(count, self.capacity)
+        return np.stack([major[:count], minor[:count]], axis=1)
+
+    def close(self) -> None:
+        if self.w:
+            self.o.h.oracle_world_destroy(self.w)
+            self.w = None
+
+    def __del__(self):
+        self.close()
 
 
 _oracle: Oracle | None = None

@@ -1,0 +1,149 @@
+"""Scenes and measurements shared by the step tests.
+
+The scenes restate the reference's world tests (crates/dynamis-world/tests/suite/*.rs) and its
+benchmark workloads; the helpers restate crates/dynamis-world/tests/suite/common.rs and the
+rotation measurements of rotation.rs:34-64.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from dynamis_amd.world import BodyDesc, default_config, encode
+
+DT = np.float32(1.0 / 60.0)  # common.rs:7
+
+
+def config(**fields) -> np.ndarray:
+    c = default_config()
+    for name, value in fields.items():
+        c[name] = value
+    return c
+
+
+def gravity_config(**fields) -> np.ndarray:
+    """common.rs:53-59: default configuration without damping."""
+    return config(damping=0.0, angular_damping=0.0, **fields)
+
+
+def static_config(**fields) -> np.ndarray:
+    """common.rs:44-51: no gravity, no damping."""
+    return config(gravity=(0.0, 0.0, 0.0), damping=0.0, angular_damping=0.0, **fields)
+
+
+def symplectic_fall(gravity: float, frames: int, initial_speed: float, substeps: int) -> float:
+    """common.rs:59-63, in FP32 as the reference evaluates it."""
+    f = np.float32
+    step = f(DT) / f(substeps)
+    count = f(frames * substeps)
+    return float(initial_speed * step * count - f(0.5) * f(gravity) * step * step * count
+                 * (count + f(1.0)))
+
+
+def floor_box(half=40.0) -> BodyDesc:
+    return BodyDesc.cuboid((half, 0.5, half), mass=0.0, position=(0.0, -0.5, 0.0))
+
+
+def tower(levels=16) -> list[BodyDesc]:
+    """dynamics.rs:177-213: a static floor and a tower of unit cubes spaced 1.0005 apart."""
+    bodies = [floor_box()]
+    for level in range(levels):
+        bodies.append(BodyDesc.cuboid((0.5, 0.5, 0.5), position=(0.0, 0.5 + level * 1.0005, 0.0),
+                                      friction=0.7, restitution=0.0))
+    return bodies
+
+
+def random_unit_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform rotations (Shoemake's subgroup algorithm) as (x, y, z, w) FP32."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    q = np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                  b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)], axis=1)
+    q = q.astype(np.float32)
+    return q / np.linalg.norm(q, axis=1, keepdims=True).astype(np.float32)
+
+
+def cube_drop(count: int, seed: int = 0, footprint: float = 20.0, low: float = 1.0,
+              high: float = 40.0) -> list[BodyDesc]:
+    """The 1,000-cube drop workload: cubes of half-extent 0.5 m, density 1000 kg/m^3, friction
+    0.5, restitution 0, centres uniform in a footprint x footprint square at heights
+    [low, high], uniform random orientations, above a static ground plane at y = 0."""
+    rng = np.random.default_rng(seed)
+    half = footprint / 2.0
+    xs = rng.uniform(-half, half, count).astype(np.float32)
+    zs = rng.uniform(-half, half, count).astype(np.float32)
+    ys = rng.uniform(low, high, count).astype(np.float32)
+    qs = random_unit_quaternions(rng, count)
+    bodies = [BodyDesc.plane()]
+    for i in range(count):
+        bodies.append(BodyDesc.cuboid((0.5, 0.5, 0.5), density=1000.0, friction=0.5,
+                                      restitution=0.0,
+                                      position=(float(xs[i]), float(ys[i]), float(zs[i])),
+                                      orientation=tuple(float(v) for v in qs[i])))
+    return bodies
+
+
+def mixed_pile(count: int, seed: int = 0, spacing: float = 2.0) -> list[BodyDesc]:
+    """Spheres, boxes and capsules on a jittered lattice above a plane (sizes of the mixed
+    convex pile: r ~ U[0.25, 0.75]; half-extents ~ U[0.2, 0.8]; capsule r ~ U[0.15, 0.4],
+    half-height ~ U[0.2, 0.8])."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(count / 4.0)))
+    bodies = [BodyDesc.plane()]
+    for i in range(count):
+        layer, cell = divmod(i, side * side)
+        gx, gz = divmod(cell, side)
+        jitter = rng.uniform(-0.1, 0.1, 3)
+        pos = (float((gx - side / 2) * spacing + jitter[0]), float(1.5 + layer * spacing + jitter[1]),
+               float((gz - side / 2) * spacing + jitter[2]))
+        kind = i % 3
+        if kind == 0:
+            bodies.append(BodyDesc.sphere(float(rng.uniform(0.25, 0.75)), position=pos,
+                                          density=1000.0))
+        elif kind == 1:
+            bodies.append(BodyDesc.cuboid(tuple(float(v) for v in rng.uniform(0.2, 0.8, 3)),
+                                          position=pos, density=1000.0))
+        else:
+            bodies.append(BodyDesc.capsule(float(rng.uniform(0.15, 0.4)),
+                                           float(rng.uniform(0.2, 0.8)), position=pos,
+                                           density=1000.0))
+    return bodies
+
+
+# rotation.rs:34-64 measurements, in float64 on the host
+def qrot(q, v):
+    q = np.asarray(q, np.float64)
+    v = np.asarray(v, np.float64)
+    u, s = q[:3], q[3]
+    return 2.0 * np.dot(u, v) * u + (s * s - np.dot(u, u)) * v + 2.0 * s * np.cross(u, v)
+
+
+def qconj(q):
+    return np.array([-q[0], -q[1], -q[2], q[3]], np.float64)
+
+
+def sym(t, v):
+    t = np.asarray(t, np.float64)
+    m = np.array([[t[0], t[1], t[2]], [t[1], t[3], t[4]], [t[2], t[4], t[5]]])
+    return m @ np.asarray(v, np.float64)
+
+
+def body_spin(state):
+    return qrot(qconj(state["orientation"]), state["angular_velocity"])
+
+
+def body_momentum(state, inertia):
+    return qrot(state["orientation"], sym(inertia, body_spin(state)))
+
+
+def kinetic_energy(state, inertia):
+    w = body_spin(state)
+    return 0.5 * float(np.dot(w, sym(inertia, w)))
+
+
+def angle_between(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    c = np.dot(a / np.linalg.norm(a), b / np.linalg.norm(b))
+    return float(np.arccos(np.clip(c, -1.0, 1.0)))
+
