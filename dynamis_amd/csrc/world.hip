@@ -636,6 +636,7 @@ __global__ void __launch_bounds__(kThreads) k_island_wake(W w) {
 // live.wgsl count, solver_reset.wgsl
 __global__ void __launch_bounds__(kThreads) k_solver_reset(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *cw(w.counters, DYN_COUNTER_SOLVER_ROWS) = 0u;  // maintained: k_rows_count refills it
     uint32_t live = 0;
     if (i < w.n) {
         w.block_counts[i] = 0u;
