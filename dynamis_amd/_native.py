@@ -51,6 +51,10 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_config_default": (None, [c_void_p]),
     "dyn_step_params_build": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p]),
     "dyn_inertia_inverse": (c_int, [c_void_p, c_void_p]),
+    "dyn_hull_build": (c_int, [c_void_p, c_u32, c_void_p, ctypes.POINTER(c_u32), c_void_p,
+                               ctypes.POINTER(c_u32), c_u32]),
+    "dyn_hull_solid": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
+    "dyn_world_upload_shapes": (c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_u32]),
     "dyn_encode_body": (c_int, [c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_void_p]),
     "dyn_world_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_u32, c_u32, c_void_p, c_void_p, c_u32, c_u32]),

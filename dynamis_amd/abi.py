@@ -139,7 +139,20 @@ BODY_SPEC = np.dtype([
     ("restitution", "<f4"), ("rolling_friction", "<f4"), ("spin_friction", "<f4"),
     ("gravity_scale", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),
     ("kinematic", "<u4"), ("collision_group", "<u4"), ("collision_mask", "<u4"),
+    ("source", "<u4"), ("solid_volume", "<f4"), ("solid_centroid", "<f4", 3),
+    ("solid_second", "<f4", 6),
 ])
+
+# records.wgsl:256-273 ShapeSource (80 bytes)
+SHAPE_SOURCE = np.dtype([
+    ("kind", "<u4"), ("vertex_offset", "<u4"), ("vertex_count", "<u4"),
+    ("triangle_offset", "<u4"), ("node_offset", "<u4"), ("node_count", "<u4"),
+    ("cell_offset", "<u4"), ("cell_count", "<u4"), ("grid_rows", "<u4"), ("grid_cols", "<u4"),
+    ("_pad0", "<u4"), ("_pad1", "<u4"), ("local_min", "<f4", 3), ("_pad2", "<f4"),
+    ("local_max", "<f4", 3), ("_pad3", "<f4"),
+])
+assert SHAPE_SOURCE.itemsize == 80
+LINEAR_SUPPORT_VERTICES = 4096
 
 assert STEP_PARAMS.itemsize == 176 and BODY_STATE.itemsize == 144
 assert BODY_DESC.itemsize == 128 and COLLIDER.itemsize == 112

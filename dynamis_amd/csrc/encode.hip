@@ -3,6 +3,7 @@
 // crates/dynamis-abi/src/body.rs:96-139 (BodyDescriptorRecord::build), :34-58 (BodyStateRecord::
 // initial), crates/dynamis-abi/src/collider.rs:47-102 (ColliderRecord::build),
 // crates/dynamis-model/src/mass.rs:34-96 (analytic solids), :155-210 (compute_mass_properties),
+// :282-342 (second-moment rotation),
 // :259-280 (inertia_inverse) and crates/dynamis-abi/src/param.rs:110-137 (StepParamsRecord::new).
 // Arithmetic is FP32 in the reference's operation order.
 #include <cmath>
@@ -16,12 +17,13 @@ constexpr float kPi = 3.14159265358979323846f;
 
 struct Solid {
     float volume;
+    float centroid[3];
     float second[6];  // unit second moment about the centroid: xx xy xz yy yz zz
     bool present;
 };
 
-// mass.rs:34-96
-Solid analytic_solid(const dyn_body_spec& b) {
+// mass.rs:34-96 (analytic shapes) and the hull's solid geometry carried by the spec
+Solid solid_of(const dyn_body_spec& b) {
     Solid s{};
     s.present = true;
     if (b.shape == DYN_SHAPE_SPHERE) {
@@ -49,10 +51,48 @@ Solid analytic_solid(const dyn_body_spec& b) {
         s.volume = total;
         const float m[6] = {lateral, 0.0f, 0.0f, axial, 0.0f, lateral};
         std::memcpy(s.second, m, sizeof m);
+    } else if (b.shape == DYN_SHAPE_CYLINDER) {
+        const float r = b.radius;
+        const float height = b.half_height * 2.0f;
+        const float lateral = r * r / 4.0f;
+        s.volume = kPi * r * r * height;
+        const float m[6] = {lateral, 0.0f, 0.0f, height * height / 12.0f, 0.0f, lateral};
+        std::memcpy(s.second, m, sizeof m);
+    } else if (b.shape == DYN_SHAPE_HULL) {
+        s.volume = b.solid_volume;
+        std::memcpy(s.centroid, b.solid_centroid, sizeof s.centroid);
+        std::memcpy(s.second, b.solid_second, sizeof s.second);
     } else {
         s.present = false;  // planes carry no solid
     }
     return s;
+}
+
+// mass.rs:282-342 for the collider's own rotation (the identity here: one collider at the body
+// origin), written out so the products and sums round as the reference's do.
+using M3 = float[3][3];
+void mat_mul(const M3 a, const M3 b, M3 out) {
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) out[r][c] = a[r][0] * b[0][c] + a[r][1] * b[1][c] + a[r][2] * b[2][c];
+    }
+}
+void second_moment_rotate(const float* second, const float* q, float* out) {
+    const float x = q[0], y = q[1], z = q[2], w = q[3];
+    const M3 r = {{1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y)},
+                  {2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x)},
+                  {2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y)}};
+    const M3 sm = {{second[0], second[1], second[2]}, {second[1], second[3], second[4]},
+                   {second[2], second[4], second[5]}};
+    const M3 rt = {{r[0][0], r[1][0], r[2][0]}, {r[0][1], r[1][1], r[2][1]}, {r[0][2], r[1][2], r[2][2]}};
+    M3 inner, m;
+    mat_mul(sm, rt, inner);
+    mat_mul(r, inner, m);
+    out[0] = m[0][0];
+    out[1] = (m[0][1] + m[1][0]) * 0.5f;
+    out[2] = (m[0][2] + m[2][0]) * 0.5f;
+    out[3] = m[1][1];
+    out[4] = (m[1][2] + m[2][1]) * 0.5f;
+    out[5] = m[2][2];
 }
 
 // mass.rs:259-280
@@ -99,22 +139,40 @@ int dyn_encode_body(const dyn_body_spec* spec, const dyn_config* config, uint32_
     }
     const dyn_body_spec& b = *spec;
     if (b.shape != DYN_SHAPE_SPHERE && b.shape != DYN_SHAPE_CUBOID && b.shape != DYN_SHAPE_CAPSULE &&
-        b.shape != DYN_SHAPE_PLANE) {
+        b.shape != DYN_SHAPE_CYLINDER && b.shape != DYN_SHAPE_HULL && b.shape != DYN_SHAPE_PLANE) {
         return fail(DYN_E_INVALID, "dyn_encode_body: shape kind not supported by this encoder");
     }
-    const Solid solid = analytic_solid(b);
-    // BodyDesc::effective_mass (model/body.rs:363-368) and the body kind
+    if (b.shape == DYN_SHAPE_HULL && !(b.solid_volume > 0.0f)) {
+        return fail(DYN_E_INVALID, "dyn_encode_body: a hull carries its solid geometry (dyn_hull_solid)");
+    }
+    const Solid solid = solid_of(b);
+    // BodyDesc::effective_mass (model/body.rs:363-368): scale volume (1) x solid volume
     const float volume = solid.present ? 1.0f * 1.0f * 1.0f * solid.volume : 0.0f;
     const float mass = b.density >= 0.0f ? b.density * volume : b.mass;
     const bool kinematic = b.kinematic != 0u;
     const bool simulates = !kinematic && mass > 0.0f;
-    // compute_mass_properties for one collider at the body origin, unrotated, unscaled
+    // compute_mass_properties (mass.rs:155-210) for one collider at the body origin, unrotated
+    // and unscaled
+    float com[3] = {0.0f, 0.0f, 0.0f};
     float inertia[6] = {0, 0, 0, 0, 0, 0};
     float inverse[6] = {0, 0, 0, 0, 0, 0};
     if (solid.present && volume > 0.0f && mass > 0.0f) {
+        const float identity[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        // centroid_of: offset (0) + rotate(identity, scale (1) x centroid)
+        float at[3];
+        for (int a = 0; a < 3; ++a) at[a] = 0.0f + 1.0f * solid.centroid[a];
+        for (int a = 0; a < 3; ++a) com[a] = (0.0f + at[a] * volume) / volume;
         const float shape_mass = mass * volume / volume;
+        float scaled[6];
+        for (int i = 0; i < 6; ++i) scaled[i] = solid.second[i] * 1.0f * 1.0f;
+        float rotated[6];
+        second_moment_rotate(scaled, identity, rotated);
+        const float d[3] = {at[0] - com[0], at[1] - com[1], at[2] - com[2]};
+        const float translated[6] = {rotated[0] + d[0] * d[0], rotated[1] + d[0] * d[1],
+                                     rotated[2] + d[0] * d[2], rotated[3] + d[1] * d[1],
+                                     rotated[4] + d[1] * d[2], rotated[5] + d[2] * d[2]};
         float second[6];
-        for (int i = 0; i < 6; ++i) second[i] = 0.0f + solid.second[i] * shape_mass;
+        for (int i = 0; i < 6; ++i) second[i] = 0.0f + translated[i] * shape_mass;
         const float xx = second[0], xy = second[1], xz = second[2], yy = second[3], yz = second[4],
                     zz = second[5];
         const float in[6] = {yy + zz, -xy, -xz, xx + zz, -yz, xx + yy};
@@ -138,14 +196,17 @@ int dyn_encode_body(const dyn_body_spec* spec, const dyn_config* config, uint32_
     desc->flags = kinematic ? DYN_BODY_KINEMATIC : 0u;
     desc->collision_group = b.collision_group;
     desc->collision_mask = b.collision_mask;
+    std::memcpy(desc->com, com, sizeof com);
     std::memcpy(desc->inertia, inertia, sizeof inertia);
     std::memcpy(desc->inverse_inertia, inverse, sizeof inverse);
     std::memset(collider, 0, sizeof *collider);
     collider->slot = collider_slot;
     collider->kind = b.shape;
     collider->flags = 2u;  // ContactEventMode::BeginEnd, the ColliderDesc default
-    collider->radius = (b.shape == DYN_SHAPE_SPHERE || b.shape == DYN_SHAPE_CAPSULE) ? b.radius : 0.0f;
-    collider->half_height = b.shape == DYN_SHAPE_CAPSULE ? b.half_height : 0.0f;
+    const bool round = b.shape == DYN_SHAPE_SPHERE || b.shape == DYN_SHAPE_CAPSULE || b.shape == DYN_SHAPE_CYLINDER;
+    collider->radius = round ? b.radius : 0.0f;
+    collider->half_height = (b.shape == DYN_SHAPE_CAPSULE || b.shape == DYN_SHAPE_CYLINDER) ? b.half_height : 0.0f;
+    collider->source = b.shape == DYN_SHAPE_HULL ? b.source : 0u;
     collider->relaxation = 0.0f;     // infinite contact frequency
     collider->damping_ratio = 1.0f;  // ColliderDesc default
     collider->impact_force = INFINITY;

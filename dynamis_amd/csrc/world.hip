@@ -60,6 +60,9 @@ struct W {
     uint32_t* owner;
     uint32_t* row_of_body;  // body id -> row, ids < id_cap
     uint32_t id_cap;
+    const dyn_shape_source* sources;  // hull shape sources and their float4 vertex rows
+    const float4* vertices;
+    uint32_t source_count;
     uint32_t* wake;
     uint32_t* activity;
     uint32_t* motion;
@@ -1170,6 +1173,12 @@ struct dyn_world {
     void* sort_scratch = nullptr;
     size_t sort_scratch_bytes = 0;
     std::vector<void*> owned;
+    void* shape_sources = nullptr;   // owned separately: replaced by each shape upload
+    void* shape_vertices = nullptr;
+    uint32_t collider_count = 0;
+    std::vector<uint32_t> host_kinds;    // collider kinds and sources, for upload validation
+    std::vector<uint32_t> host_sources;
+    uint32_t hull_colliders = 0;
 };
 
 namespace {
@@ -1360,6 +1369,14 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
         }
         world->own_stream = true;
     }
+    world->collider_count = collider_count;
+    world->host_kinds.resize(collider_count);
+    world->host_sources.resize(collider_count);
+    for (uint32_t i = 0; i < collider_count; ++i) {
+        world->host_kinds[i] = colliders[i].kind;
+        world->host_sources[i] = colliders[i].source;
+        if (colliders[i].kind == DYN_SHAPE_HULL) ++world->hull_colliders;
+    }
     W& w = world->w;
     w.id_cap = id_cap;
     w.p = *params;
@@ -1441,10 +1458,49 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     return DYN_OK;
 }
 
+int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
+                            uint32_t source_count, const float* vertices, uint32_t vertex_count) {
+    using namespace dyn;
+    if (!world || (source_count && !sources) || (vertex_count && !vertices)) {
+        return fail(DYN_E_INVALID, "dyn_world_upload_shapes: null argument");
+    }
+    for (uint32_t i = 0; i < source_count; ++i) {
+        if (sources[i].vertex_count > DYN_LINEAR_SUPPORT_VERTICES) {
+            return fail(DYN_E_INVALID, "dyn_world_upload_shapes: a hull holds more vertices than the linear support scan");
+        }
+        if (sources[i].vertex_count == 0 ||
+            static_cast<uint64_t>(sources[i].vertex_offset) + sources[i].vertex_count > vertex_count) {
+            return fail(DYN_E_INVALID, "dyn_world_upload_shapes: a source reaches past the vertex array");
+        }
+    }
+    for (uint32_t i = 0; i < world->collider_count; ++i) {
+        if (world->host_kinds[i] == DYN_SHAPE_HULL && world->host_sources[i] >= source_count) {
+            return fail(DYN_E_INVALID, "dyn_world_upload_shapes: a hull collider names a missing source");
+        }
+    }
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    if (world->shape_sources) DYN_HIP(hipFree(world->shape_sources));
+    if (world->shape_vertices) DYN_HIP(hipFree(world->shape_vertices));
+    world->shape_sources = nullptr;
+    world->shape_vertices = nullptr;
+    const size_t sb = sizeof(dyn_shape_source) * (source_count ? source_count : 1u);
+    const size_t vb = 16u * static_cast<size_t>(vertex_count ? vertex_count : 1u);
+    DYN_HIP(hipMalloc(&world->shape_sources, sb));
+    DYN_HIP(hipMalloc(&world->shape_vertices, vb));
+    if (source_count) DYN_HIP(hipMemcpy(world->shape_sources, sources, sizeof(dyn_shape_source) * source_count, hipMemcpyHostToDevice));
+    if (vertex_count) DYN_HIP(hipMemcpy(world->shape_vertices, vertices, 16u * static_cast<size_t>(vertex_count), hipMemcpyHostToDevice));
+    world->w.sources = static_cast<const dyn_shape_source*>(world->shape_sources);
+    world->w.vertices = static_cast<const float4*>(world->shape_vertices);
+    world->w.source_count = source_count;
+    return DYN_OK;
+}
+
 int dyn_world_destroy(dyn_world* world) {
     if (!world) return DYN_OK;
     (void)hipStreamSynchronize(world->stream);
     for (void* p : world->owned) (void)hipFree(p);
+    if (world->shape_sources) (void)hipFree(world->shape_sources);
+    if (world->shape_vertices) (void)hipFree(world->shape_vertices);
     if (world->own_stream) (void)hipStreamDestroy(world->stream);
     delete world;
     return DYN_OK;
@@ -1453,6 +1509,9 @@ int dyn_world_destroy(dyn_world* world) {
 int dyn_world_step(dyn_world* world, float dt) {
     if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_step: null world");
     if (!(dt > 0.0f)) return dyn::fail(DYN_E_INVALID, "dyn_world_step: dt must be positive");
+    if (world->hull_colliders && world->shape_sources == nullptr) {
+        return dyn::fail(DYN_E_INVALID, "dyn_world_step: hull colliders need dyn_world_upload_shapes first");
+    }
     return world_step(world, dt);
 }
 

@@ -19,6 +19,56 @@ from . import abi
 from ._native import call, lib
 
 
+@dataclass
+class HullSource:
+    """A hull shape source: vertices, outward triangles and its solid geometry."""
+
+    vertices: np.ndarray      # (V, 3) float32
+    triangles: np.ndarray     # (T, 3) uint32
+    volume: float
+    centroid: np.ndarray      # (3,) float32
+    second: np.ndarray        # (6,) float32 unit second moment about the centroid
+
+
+def build_hull(points) -> HullSource:
+    """Quickhull over a point set and the hull's solid geometry (World::add_hull_from_mesh,
+    crates/dynamis-world/src/shape.rs:50-57; crates/dynamis-hull/src/{hull,solid}.rs)."""
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    cap = max(8, 2 * len(pts) + 8)
+    verts = np.zeros((cap, 3), np.float32)
+    tris = np.zeros((cap, 3), np.uint32)
+    nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+    call("dyn_hull_build", pts.ctypes.data, len(pts), verts.ctypes.data, ctypes.byref(nv),
+         tris.ctypes.data, ctypes.byref(nt), cap)
+    verts, tris = verts[:nv.value].copy(), tris[:nt.value].copy()
+    volume = np.zeros(1, np.float32)
+    centroid = np.zeros(3, np.float32)
+    second = np.zeros(6, np.float32)
+    call("dyn_hull_solid", verts.ctypes.data, len(verts), tris.ctypes.data, len(tris),
+         volume.ctypes.data, centroid.ctypes.data, second.ctypes.data)
+    return HullSource(vertices=verts, triangles=tris, volume=float(volume[0]), centroid=centroid,
+                      second=second)
+
+
+def shape_records(hulls: list[HullSource]) -> tuple[np.ndarray, np.ndarray]:
+    """ShapeSource records and float4 vertex rows of a list of hulls (source id = index)."""
+    sources = np.zeros(len(hulls), dtype=abi.SHAPE_SOURCE)
+    rows = []
+    offset = 0
+    for i, h in enumerate(hulls):
+        sources[i]["kind"] = abi.SHAPE_HULL
+        sources[i]["vertex_offset"] = offset
+        sources[i]["vertex_count"] = len(h.vertices)
+        sources[i]["local_min"] = h.vertices.min(axis=0)
+        sources[i]["local_max"] = h.vertices.max(axis=0)
+        r = np.zeros((len(h.vertices), 4), np.float32)
+        r[:, :3] = h.vertices
+        rows.append(r)
+        offset += len(h.vertices)
+    vertices = np.concatenate(rows) if rows else np.zeros((0, 4), np.float32)
+    return sources, np.ascontiguousarray(vertices)
+
+
 def default_config() -> np.ndarray:
     """PhysicsConfig::default (crates/dynamis-world/src/config.rs:76-100) as a CONFIG record."""
     config = np.zeros(1, dtype=abi.CONFIG)
@@ -53,6 +103,7 @@ class BodyDesc:
     kinematic: bool = False
     collision_group: int = 1
     collision_mask: int = abi.NO_COLLISION_FILTER
+    source: int = 0
 
     # constructors named after crates/dynamis-model/src/shape.rs
     @staticmethod
@@ -68,6 +119,16 @@ class BodyDesc:
         return BodyDesc(shape=abi.SHAPE_CAPSULE, radius=radius, half_height=half_height, **kw)
 
     @staticmethod
+    def cylinder(radius: float, half_height: float, **kw) -> "BodyDesc":
+        return BodyDesc(shape=abi.SHAPE_CYLINDER, radius=radius, half_height=half_height, **kw)
+
+    @staticmethod
+    def hull(source: int, **kw) -> "BodyDesc":
+        """A body whose collider is hull shape source `source` (an index into the hulls
+        passed to `encode` / `World.add_hull`)."""
+        return BodyDesc(shape=abi.SHAPE_HULL, source=source, **kw)
+
+    @staticmethod
     def plane(**kw) -> "BodyDesc":
         kw.setdefault("mass", 0.0)
         return BodyDesc(shape=abi.SHAPE_PLANE, **kw)
@@ -78,11 +139,20 @@ class BodyDesc:
         self.mass = 0.0
         return self
 
-    def spec(self) -> np.ndarray:
+    def spec(self, hulls: list[HullSource] | None = None) -> np.ndarray:
         s = np.zeros(1, dtype=abi.BODY_SPEC)
         for name in abi.BODY_SPEC.names:
+            if name.startswith("solid_"):
+                continue
             value = getattr(self, name)
             s[name] = int(value) if name == "kinematic" else value
+        if self.shape == abi.SHAPE_HULL:
+            if hulls is None or not 0 <= self.source < len(hulls):
+                raise ValueError(f"hull body names shape source {self.source}, which is not given")
+            h = hulls[self.source]
+            s["solid_volume"] = h.volume
+            s["solid_centroid"] = h.centroid
+            s["solid_second"] = h.second
         return s
 
 
@@ -98,6 +168,8 @@ class Scene:
     dynamic_count: int
     rows: np.ndarray                 # handle -> row
     handles: np.ndarray = field(default=None)  # row -> handle
+    sources: np.ndarray = field(default=None)  # SHAPE_SOURCE records
+    vertices: np.ndarray = field(default=None)  # (V, 4) float32 rows
 
 
 def spawn_rows(simulates: list[bool]) -> list[int]:
@@ -117,12 +189,13 @@ def spawn_rows(simulates: list[bool]) -> list[int]:
     return alive
 
 
-def encode(bodies: list[BodyDesc], config: np.ndarray | None = None) -> Scene:
+def encode(bodies: list[BodyDesc], config: np.ndarray | None = None,
+           hulls: list[HullSource] | None = None) -> Scene:
     """Encodes bodies spawned in this order into device records with the library's host
     encoder. Body handle ids are the spawn indices; collider slots follow spawn order (the
     collider arena, collider.rs:180-214); rows follow the simulated-first partition."""
     config = default_config() if config is None else config
-    specs = [b.spec() for b in bodies]
+    specs = [b.spec(hulls) for b in bodies]
     n = len(specs)
     enc_states = np.zeros(n, dtype=abi.BODY_STATE)
     enc_descs = np.zeros(n, dtype=abi.BODY_DESC)
@@ -143,9 +216,10 @@ def encode(bodies: list[BodyDesc], config: np.ndarray | None = None) -> Scene:
     dynamic_count = int(sum(simulates))
     params = np.zeros(1, dtype=abi.STEP_PARAMS)
     call("dyn_step_params_build", config.ctypes.data, n, dynamic_count, n, params.ctypes.data)
+    sources, vertices = shape_records(hulls or [])
     return Scene(params=params, states=states, descs=descs, colliders=colliders,
                  owners=rows.copy(), dynamic_count=dynamic_count, rows=rows,
-                 handles=np.asarray(order, dtype=np.uint32))
+                 handles=np.asarray(order, dtype=np.uint32), sources=sources, vertices=vertices)
 
 
 class World:
@@ -156,6 +230,7 @@ class World:
         self.config = default_config() if config is None else config.copy()
         self.contact_capacity = contact_capacity
         self._bodies: list[BodyDesc] = []
+        self._hulls: list[HullSource] = []
         self._handle = ctypes.c_void_p()
         self.scene: Scene | None = None
 
@@ -169,12 +244,19 @@ class World:
     def count(self) -> int:
         return len(self._bodies)
 
+    def add_hull(self, points) -> int:
+        """Builds a hull shape source from points and returns its id (World::add_hull_from_mesh)."""
+        if self.scene is not None:
+            raise RuntimeError("shape sources are added before the first step of a world")
+        self._hulls.append(build_hull(points))
+        return len(self._hulls) - 1
+
     def _ensure(self) -> None:
         if self.scene is not None:
             return
         if not self._bodies:
             raise RuntimeError("a world needs at least one body")
-        self.scene = encode(self._bodies, self.config)
+        self.scene = encode(self._bodies, self.config, self._hulls)
         self.create(self.scene, self.contact_capacity)
 
     def create(self, scene: Scene, contact_capacity: int | None = None) -> None:
@@ -186,6 +268,9 @@ class World:
              scene.states.ctypes.data, scene.descs.ctypes.data, n, scene.dynamic_count,
              scene.colliders.ctypes.data, scene.owners.ctypes.data, n, capacity)
         self.capacity = capacity
+        if scene.sources is not None and len(scene.sources):
+            call("dyn_world_upload_shapes", self._handle, scene.sources.ctypes.data,
+                 len(scene.sources), scene.vertices.ctypes.data, len(scene.vertices))
 
     # -- stepping ----------------------------------------------------------------------------
     def step(self, dt: float) -> None:

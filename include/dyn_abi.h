@@ -131,6 +131,28 @@ typedef struct dyn_collider {
     float spin_friction;
 } dyn_collider;
 
+/* ShapeSource, records.wgsl:256-273 (80 bytes): where a hull's vertices (float4 rows) live.
+ * Hulls of at most DYN_LINEAR_SUPPORT_VERTICES vertices are supported (the reference's linear
+ * support scan); triangles, BVH nodes and height-field cells are not read by this path. */
+typedef struct dyn_shape_source {
+    uint32_t kind;
+    uint32_t vertex_offset;
+    uint32_t vertex_count;
+    uint32_t triangle_offset;
+    uint32_t node_offset;
+    uint32_t node_count;
+    uint32_t cell_offset;
+    uint32_t cell_count;
+    uint32_t grid_rows;
+    uint32_t grid_cols;
+    uint32_t _pad0;
+    uint32_t _pad1;
+    float local_min[3];
+    float _pad2;
+    float local_max[3];
+    float _pad3;
+} dyn_shape_source;
+
 /* Aabb, records.wgsl:299-304 (32 bytes). */
 typedef struct dyn_aabb {
     float min[3];
@@ -196,6 +218,7 @@ static_assert(sizeof(dyn_body_state) == 144, "BodyState is 144 bytes");
 static_assert(sizeof(dyn_body_desc) == 128, "BodyDescriptor is 128 bytes");
 static_assert(sizeof(dyn_collider) == 112, "Collider is 112 bytes");
 static_assert(sizeof(dyn_aabb) == 32, "Aabb is 32 bytes");
+static_assert(sizeof(dyn_shape_source) == 80, "ShapeSource is 80 bytes");
 static_assert(sizeof(dyn_grid_entry) == 32, "GridEntry is 32 bytes");
 static_assert(sizeof(dyn_manifold_point) == 64, "ManifoldPoint is 64 bytes");
 static_assert(sizeof(dyn_contact) == 352, "Contact is 352 bytes");
@@ -205,6 +228,7 @@ _Static_assert(sizeof(dyn_body_state) == 144, "BodyState is 144 bytes");
 _Static_assert(sizeof(dyn_body_desc) == 128, "BodyDescriptor is 128 bytes");
 _Static_assert(sizeof(dyn_collider) == 112, "Collider is 112 bytes");
 _Static_assert(sizeof(dyn_aabb) == 32, "Aabb is 32 bytes");
+_Static_assert(sizeof(dyn_shape_source) == 80, "ShapeSource is 80 bytes");
 _Static_assert(sizeof(dyn_grid_entry) == 32, "GridEntry is 32 bytes");
 _Static_assert(sizeof(dyn_manifold_point) == 64, "ManifoldPoint is 64 bytes");
 _Static_assert(sizeof(dyn_contact) == 352, "Contact is 352 bytes");
@@ -225,6 +249,7 @@ _Static_assert(sizeof(dyn_contact) == 352, "Contact is 352 bytes");
 #define DYN_SHAPE_CYLINDER 4u
 #define DYN_SHAPE_HULL 5u
 #define DYN_SHAPE_PLANE 9u
+#define DYN_LINEAR_SUPPORT_VERTICES 4096u
 #define DYN_COLLIDER_SENSOR 1u
 
 #define DYN_MAX_CELLS_PER_AXIS 2u

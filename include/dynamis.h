@@ -163,7 +163,8 @@ typedef struct dyn_config {
     uint32_t restitution_combine;
 } dyn_config;
 
-/* A BodyDesc with one collider at the body origin (model/body.rs:89-108, collider.rs:52-68).
+/* A BodyDesc with one collider at the body origin (model/body.rs:89-108, collider.rs:52-68);
+ * shape DYN_SHAPE_SPHERE, _CUBOID, _CAPSULE, _CYLINDER, _HULL or _PLANE.
  * density >= 0 selects BodyDesc::density, otherwise `mass` is the fixed mass (0 = static);
  * damping values < 0 take the configuration's. */
 typedef struct dyn_body_spec {
@@ -187,6 +188,11 @@ typedef struct dyn_body_spec {
     uint32_t kinematic;
     uint32_t collision_group;
     uint32_t collision_mask;
+    /* DYN_SHAPE_HULL only: the shape source index and its solid geometry (dyn_hull_solid) */
+    uint32_t source;
+    float solid_volume;
+    float solid_centroid[3];
+    float solid_second[6];
 } dyn_body_spec;
 
 /* PhysicsConfig::default (config.rs:76-100). */
@@ -194,6 +200,16 @@ void dyn_config_default(dyn_config* config);
 /* StepParamsRecord::new without the step length (param.rs:110-137). */
 int dyn_step_params_build(const dyn_config* config, uint32_t body_count, uint32_t dynamic_count,
                           uint32_t collider_count, dyn_step_params* out);
+/* Quickhull over `count` points (xyz triples), as World::add_hull_from_mesh builds a hull
+ * (crates/dynamis-hull/src/hull.rs:51-288): hull vertices (xyz, in order of first use) and
+ * outward triangles. Fails on collinear or coplanar points, or when either output exceeds
+ * `capacity` entries. */
+int dyn_hull_build(const float* points, uint32_t count, float* vertices, uint32_t* vertex_count,
+                   uint32_t* triangles, uint32_t* triangle_count, uint32_t capacity);
+/* Volume, centroid and unit second moment about the centroid ([xx, xy, xz, yy, yz, zz]) of a
+ * closed triangle hull, accumulated in FP64 (crates/dynamis-hull/src/solid.rs:62-105). */
+int dyn_hull_solid(const float* vertices, uint32_t vertex_count, const uint32_t* triangles,
+                   uint32_t triangle_count, float* volume, float* centroid, float* second);
 /* inertia_inverse (mass.rs:259-280); fails on a tensor that is not positive definite. */
 int dyn_inertia_inverse(const float* inertia, float* inverse);
 /* BodyStateRecord::initial + BodyDescriptorRecord::build + ColliderRecord::build for one body
@@ -224,6 +240,13 @@ int dyn_world_create(dyn_world** world, void* stream, const dyn_step_params* par
                      uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
                      uint32_t collider_count, uint32_t contact_capacity);
 int dyn_world_destroy(dyn_world* world);
+/* Uploads the hull shape sources colliders name in their `source` field, with their vertices as
+ * float4 rows (x, y, z, 0) (the reference's shape pool flush, crates/dynamis-world/src/
+ * shape_pool.rs:243-279). Replaces any earlier upload; call before a step that needs them.
+ * Fails when a source holds more than DYN_LINEAR_SUPPORT_VERTICES vertices or reaches past
+ * the vertex array, or when a hull collider names a source that is not there. */
+int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
+                            uint32_t source_count, const float* vertices, uint32_t vertex_count);
 /* Enqueues one step of `dt` seconds on the world's stream (non-blocking). */
 int dyn_world_step(dyn_world* world, float dt);
 /* Blocks until every enqueued step has finished. */

@@ -25,6 +25,10 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
                                   const uint32_t* owners, uint32_t collider_count,
                                   uint32_t contact_capacity);
 void oracle_world_destroy(oracle_world* w);
+/* Hull shape sources (records.wgsl ShapeSource) and their vertices as float4 rows; 0 on success,
+ * -1 when a source exceeds the linear support bound or its vertex range. */
+int oracle_world_shapes(oracle_world* w, const dyn_shape_source* sources, uint32_t source_count,
+                        const float* vertices, uint32_t vertex_count);
 /* One step with the given dt (params.dt / substep_dt are set from it). */
 void oracle_world_step(oracle_world* w, float dt);
 void oracle_world_states(const oracle_world* w, dyn_body_state* out);
@@ -34,6 +38,11 @@ void oracle_world_counters(const oracle_world* w, uint32_t* out);
 /* Sorted candidate pairs of the last step; returns the count. */
 uint32_t oracle_world_pairs(const oracle_world* w, uint32_t* major, uint32_t* minor,
                             uint32_t capacity);
+
+/* GJK / convex_hit of two colliders at given body states (test hook, no world). */
+int oracle_convex_query(const dyn_shape_source* sources, uint32_t source_count, const float* vertices,
+                        uint32_t vertex_count, const dyn_collider* ca, const dyn_body_state* sa,
+                        const dyn_collider* cb, const dyn_body_state* sb, float* out);
 
 /* Host record encoding (crates/dynamis-model/src/mass.rs, crates/dynamis-abi/src/body.rs). */
 void oracle_inertia_inverse(const float* inertia, float* inverse);

@@ -52,6 +52,10 @@ struct oracle_world {
     uint32_t* owner;
     uint32_t* row_of_body; /* body id -> row, ids < id_capacity */
     uint32_t id_capacity;
+    dyn_shape_source* sources; /* hull shape sources and their float4 vertex rows */
+    uint32_t source_count;
+    float* vertices;
+    uint32_t vertex_count;
     uint32_t* wake;
     uint32_t* admitted;
     uint32_t* activity;
@@ -186,6 +190,7 @@ typedef struct {
     float radius, half_height;
     v3 center, half_extents, scale;
     q4 rotation;
+    uint32_t source;
 } world_shape;
 
 static world_shape world_collider(const dyn_body_state* s, const dyn_collider* c) {
@@ -198,15 +203,35 @@ static world_shape world_collider(const dyn_body_state* s, const dyn_collider* c
     ws.half_extents = vload(c->half_extents);
     ws.rotation = qmul(q, qload(c->local_rotation));
     ws.scale = vload(c->scale);
+    ws.source = c->source;
     return ws;
 }
 
 static v3 shape_axis(const world_shape* ws) { return qrot(ws->rotation, V(0.0f, 1.0f, 0.0f)); }
 
-static void world_aabb_of(const world_shape* ws, v3* mn, v3* mx) {
+/* core.wgsl:490-503: bounds of a shape that reads a shape source */
+static void world_shape_bounds(const oracle_world* w, const world_shape* ws, v3* mn, v3* mx) {
+    const dyn_shape_source* src = &w->sources[ws->source];
+    v3 scaled_min = vmulv(vload(src->local_min), ws->scale);
+    v3 scaled_max = vmulv(vload(src->local_max), ws->scale);
+    v3 center = vadd(ws->center, qrot(ws->rotation, vmul(vadd(scaled_min, scaled_max), 0.5f)));
+    v3 half = vmul(vsub(scaled_max, scaled_min), 0.5f);
+    v3 extent = vadd(vadd(vabs(qrot(ws->rotation, V(half.x, 0.0f, 0.0f))),
+                          vabs(qrot(ws->rotation, V(0.0f, half.y, 0.0f)))),
+                     vabs(qrot(ws->rotation, V(0.0f, 0.0f, half.z))));
+    *mn = vsub(center, extent);
+    *mx = vadd(center, extent);
+}
+
+/* core.wgsl:505-536 */
+static void world_aabb_of(const oracle_world* w, const world_shape* ws, v3* mn, v3* mx) {
     if (ws->kind == DYN_SHAPE_PLANE) {
         *mn = vsub(ws->center, V(1e6f, 1e6f, 1e6f));
         *mx = vadd(ws->center, V(1e6f, 1e6f, 1e6f));
+        return;
+    }
+    if (ws->kind == DYN_SHAPE_HULL) {
+        world_shape_bounds(w, ws, mn, mx);
         return;
     }
     v3 extent = V(0.0f, 0.0f, 0.0f);
@@ -696,9 +721,31 @@ static dyn_contact box_box_sat(const body_t* a, const dyn_collider* ca, const bo
     return c;
 }
 
-/* ---- plane vs convex (narrowphase.wgsl:485-512, convex.wgsl:639-900) -------------------- */
+/* ---- shape sources (records.wgsl:256-273) and support maps (convex.wgsl:1-91) --------------- */
 
-static v3 support_local(const world_shape* ws, v3 d) {
+#define LINEAR_SUPPORT_VERTICES 4096u
+#define CAP_ALIGNMENT 0.93f
+#define CYLINDER_CAP_POINTS 8u
+#define CYLINDER_SIDE_POINTS 16u
+#define CYLINDER_SIDE_FACE 2u
+#define FEATURE_INDEX_LIMIT (1u << 12)
+#define EMPTY_RING_FACE FEATURE_INDEX_LIMIT
+#define NO_TRIANGLE 0xFFFFFFFFu
+#define FLT_BIG 3.402823466e38f
+
+/* cos / sin of f32(i) * 0.78539816 for the eight cylinder cap points (convex.wgsl:836-840),
+ * correctly rounded to FP32 */
+static const float kCapCos[8] = {0x1.0p+0f, 0x1.6a09e6p-1f, -0x1.777a5cp-25f, -0x1.6a09e6p-1f,
+                                 -0x1.0p+0f, -0x1.6a09e2p-1f, 0x1.99bc5cp-27f, 0x1.6a09eep-1f};
+static const float kCapSin[8] = {0x0.0p+0f, 0x1.6a09e6p-1f, 0x1.0p+0f, 0x1.6a09e6p-1f,
+                                 -0x1.777a5cp-24f, -0x1.6a09eap-1f, -0x1.0p+0f, -0x1.6a09dep-1f};
+
+static v3 hull_vertex(const oracle_world* w, const dyn_shape_source* src, uint32_t i) {
+    const float* v = &w->vertices[4u * (size_t)(src->vertex_offset + i)];
+    return V(v[0], v[1], v[2]);
+}
+
+static v3 support_local(const oracle_world* w, const world_shape* ws, v3 d) {
     v3 n = vnorm(d);
     if (ws->kind == DYN_SHAPE_SPHERE) return vmul(n, ws->radius);
     if (ws->kind == DYN_SHAPE_CUBOID) {
@@ -709,65 +756,832 @@ static v3 support_local(const world_shape* ws, v3 d) {
         v3 tip = vmul(vmul(V(0.0f, 1.0f, 0.0f), ws->half_height), d.y > 0.0f ? 1.0f : -1.0f);
         return vadd(tip, vmul(n, ws->radius));
     }
-    return V(0.0f, 0.0f, 0.0f);
+    if (ws->kind == DYN_SHAPE_CYLINDER) {
+        v3 base = vmul(vmul(V(0.0f, 1.0f, 0.0f), ws->half_height), d.y > 0.0f ? 1.0f : -1.0f);
+        v3 radial = vsub(d, V(0.0f, d.y, 0.0f));
+        if (vlen(radial) < 1e-6f) return base;
+        return vadd(base, vmul(vnorm(radial), ws->radius));
+    }
+    if (ws->kind != DYN_SHAPE_HULL || ws->source >= w->source_count) return V(0.0f, 0.0f, 0.0f);
+    /* the linear support scan (hulls above LINEAR_SUPPORT_VERTICES are refused at upload) */
+    const dyn_shape_source* src = &w->sources[ws->source];
+    float best_dot = -FLT_BIG;
+    v3 best = V(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < src->vertex_count; i++) {
+        v3 v = hull_vertex(w, src, i);
+        float s = vdot(v, d);
+        if (s > best_dot) {
+            best_dot = s;
+            best = v;
+        }
+    }
+    return best;
 }
 
-static v3 support(const world_shape* ws, v3 direction) {
+static v3 support(const oracle_world* w, const world_shape* ws, v3 direction) {
     v3 ld = qrot(qconj(ws->rotation), direction);
     v3 sd = vmulv(ld, ws->scale);
-    v3 local = support_local(ws, sd);
+    v3 local = support_local(w, ws, sd);
     return vadd(ws->center, qrot(ws->rotation, vmulv(local, ws->scale)));
 }
 
-/* shape_feature for the kinds modelled here (convex.wgsl:639-670, 795-819, 856-889) */
-static uint32_t shape_feature(const world_shape* ws, v3 direction, v3* out, uint32_t* ids) {
-    if (ws->kind == DYN_SHAPE_CUBOID) {
-        v3 axes[3] = {qrot(ws->rotation, V(1.0f, 0.0f, 0.0f)), qrot(ws->rotation, V(0.0f, 1.0f, 0.0f)),
-                      qrot(ws->rotation, V(0.0f, 0.0f, 1.0f))};
-        uint32_t best = 0;
-        for (uint32_t i = 1; i < 3; i++) {
-            if (fabsf(vdot(axes[i], direction)) > fabsf(vdot(axes[best], direction))) best = i;
-        }
-        float sign = vdot(axes[best], direction) < 0.0f ? -1.0f : 1.0f;
-        v3 face = vmul(axes[best], sign);
-        v3 center = vadd(ws->center, vmul(face, vcomp(ws->half_extents, best)));
-        v3 u1 = axes[(best + 1u) % 3u], u2 = axes[(best + 2u) % 3u];
-        float e1 = vcomp(ws->half_extents, (best + 1u) % 3u);
-        float e2 = vcomp(ws->half_extents, (best + 2u) % 3u);
-        out[0] = vadd(vadd(center, vmul(u1, e1)), vmul(u2, e2));
-        ids[0] = box_corner_id(best, sign, 1.0f, 1.0f);
-        out[1] = vadd(vsub(center, vmul(u1, e1)), vmul(u2, e2));
-        ids[1] = box_corner_id(best, sign, -1.0f, 1.0f);
-        out[2] = vsub(vsub(center, vmul(u1, e1)), vmul(u2, e2));
-        ids[2] = box_corner_id(best, sign, -1.0f, -1.0f);
-        out[3] = vsub(vadd(center, vmul(u1, e1)), vmul(u2, e2));
-        ids[3] = box_corner_id(best, sign, 1.0f, -1.0f);
-        return 4;
+/* ---- GJK (convex.wgsl:93-282, 439-521) ----------------------------------------------------- */
+
+typedef struct { v3 w, a, b; } simplex_point;
+
+typedef struct {
+    v3 v;
+    float lambdas[4];
+    int penetrating;
+    v3 point_a, point_b;
+} simplex_result;
+
+typedef struct {
+    float distance;
+    v3 point_a, point_b, normal;
+    int penetrating;
+} convex_closest_t;
+
+typedef struct {
+    float distance;
+    v3 point, normal;
+    uint32_t triangle;
+} shape_hit;
+
+typedef struct {
+    int valid;
+    v3 normal;
+    float depth;
+    v3 point;
+} epa_result;
+
+static v3 closest_on_triangle(v3 origin, v3 a, v3 b, v3 c, v3* weights) {
+    v3 ab = vsub(b, a), ac = vsub(c, a), ap = vsub(origin, a);
+    float d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        *weights = V(1.0f, 0.0f, 0.0f);
+        return a;
     }
-    if (ws->kind == DYN_SHAPE_CAPSULE) {
-        v3 ld = vmulv(qrot(qconj(ws->rotation), direction), ws->scale);
-        float cap = ld.y > 0.0f ? 1.0f : -1.0f;
-        v3 radial = vmul(sign_normalize(ld), ws->radius);
-        if (fabsf(ld.y) > 0.93f * vlen(ld)) {
-            v3 local = vadd(V(0.0f, cap * ws->half_height, 0.0f), radial);
-            out[0] = vadd(ws->center, qrot(ws->rotation, vmulv(local, ws->scale)));
-            ids[0] = cap > 0.0f ? 1u : 0u;
-            return 1;
-        }
-        v3 upper = vadd(V(0.0f, ws->half_height, 0.0f), radial);
-        v3 lower = vadd(V(0.0f, -ws->half_height, 0.0f), radial);
-        out[0] = vadd(ws->center, qrot(ws->rotation, vmulv(upper, ws->scale)));
-        out[1] = vadd(ws->center, qrot(ws->rotation, vmulv(lower, ws->scale)));
-        ids[0] = 0u;
-        ids[1] = 1u;
-        return 2;
+    v3 bp = vsub(origin, b);
+    float d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
+        *weights = V(0.0f, 1.0f, 0.0f);
+        return b;
     }
-    out[0] = support(ws, direction);
-    ids[0] = 0u;
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        float t = d1 / (d1 - d3);
+        *weights = V(1.0f - t, t, 0.0f);
+        return vadd(a, vmul(ab, t));
+    }
+    v3 cp = vsub(origin, c);
+    float d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
+        *weights = V(0.0f, 0.0f, 1.0f);
+        return c;
+    }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        float t = d2 / (d2 - d6);
+        *weights = V(1.0f - t, 0.0f, t);
+        return vadd(a, vmul(ac, t));
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *weights = V(0.0f, 1.0f - t, t);
+        return vadd(b, vmul(vsub(c, b), t));
+    }
+    float denom = 1.0f / (va + vb + vc);
+    float v = vb * denom, wgt = vc * denom;
+    *weights = V(1.0f - v - wgt, v, wgt);
+    return vadd(vadd(a, vmul(ab, v)), vmul(ac, wgt));
+}
+
+static float face_span(v3 a, v3 b, v3 c) {
+    return fmaxo(fmaxo(vlen(vsub(b, a)), vlen(vsub(c, a))), vlen(vsub(c, b)));
+}
+
+/* the four faces of a tetrahedron, in the reference's order */
+static const uint32_t kTetFaces[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+
+static int tetra_contains(const simplex_point* tet, v3 origin) {
+    for (uint32_t f = 0; f < 4; f++) {
+        v3 a = tet[kTetFaces[f][0]].w, b = tet[kTetFaces[f][1]].w, c = tet[kTetFaces[f][2]].w;
+        v3 normal = vcross(vsub(b, a), vsub(c, a));
+        float span = face_span(a, b, c);
+        if (vlen(normal) <= 1e-6f * span * span) return 0;
+        if (vdot(normal, vsub(origin, a)) > 0.0f) return 0;
+    }
     return 1;
 }
 
-static dyn_contact plane_convex(const world_shape* plane, const world_shape* convex, float margin) {
+static int simplex_holds(const simplex_point* s, uint32_t count, v3 point, float span) {
+    float tolerance = (1e-5f * span) * (1e-5f * span);
+    for (uint32_t i = 0; i < count; i++) {
+        v3 offset = vsub(s[i].w, point);
+        if (vdot(offset, offset) <= tolerance) return 1;
+    }
+    return 0;
+}
+
+static uint32_t simplex_reduce(simplex_point* s, const float* lambdas) {
+    simplex_point kept[4];
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < 4; i++) {
+        if (lambdas[i] > 1e-6f) kept[count++] = s[i];
+    }
+    if (count == 0u) {
+        kept[0] = s[0];
+        count = 1;
+    }
+    for (uint32_t i = 0; i < count; i++) s[i] = kept[i];
+    return count;
+}
+
+static simplex_result simplex_closest(const simplex_point* s, uint32_t count) {
+    simplex_result r;
+    r.v = V(FLT_BIG, FLT_BIG, FLT_BIG);
+    for (int i = 0; i < 4; i++) r.lambdas[i] = 0.0f;
+    r.penetrating = count == 4u && tetra_contains(s, V(0.0f, 0.0f, 0.0f));
+    r.point_a = V(0.0f, 0.0f, 0.0f);
+    r.point_b = V(0.0f, 0.0f, 0.0f);
+    if (r.penetrating) {
+        r.v = V(0.0f, 0.0f, 0.0f);
+        return r;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        float d = vlen(s[i].w);
+        if (d < vlen(r.v)) {
+            r.v = s[i].w;
+            for (int k = 0; k < 4; k++) r.lambdas[k] = 0.0f;
+            r.lambdas[i] = 1.0f;
+        }
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        for (uint32_t j = i + 1; j < count; j++) {
+            v3 a = s[i].w, b = s[j].w;
+            v3 ab = vsub(b, a);
+            float denom = vdot(ab, ab);
+            if (denom < 1e-12f) continue;
+            float t = fclamp(-vdot(a, ab) / denom, 0.0f, 1.0f);
+            v3 point = vadd(a, vmul(ab, t));
+            if (vlen(point) < vlen(r.v)) {
+                r.v = point;
+                for (int k = 0; k < 4; k++) r.lambdas[k] = 0.0f;
+                r.lambdas[i] = 1.0f - t;
+                r.lambdas[j] = t;
+            }
+        }
+    }
+    if (count >= 3u) {
+        for (uint32_t i = 0; i < count; i++) {
+            for (uint32_t j = i + 1; j < count; j++) {
+                for (uint32_t k = j + 1; k < count; k++) {
+                    v3 weights;
+                    v3 point = closest_on_triangle(V(0.0f, 0.0f, 0.0f), s[i].w, s[j].w, s[k].w, &weights);
+                    if (vlen(point) < vlen(r.v)) {
+                        r.v = point;
+                        for (int q = 0; q < 4; q++) r.lambdas[q] = 0.0f;
+                        r.lambdas[i] = weights.x;
+                        r.lambdas[j] = weights.y;
+                        r.lambdas[k] = weights.z;
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        r.point_a = vadd(r.point_a, smul(r.lambdas[i], s[i].a));
+        r.point_b = vadd(r.point_b, smul(r.lambdas[i], s[i].b));
+    }
+    return r;
+}
+
+static simplex_point support_pair(const oracle_world* w, const world_shape* a, const world_shape* b, v3 dir) {
+    simplex_point p;
+    p.a = support(w, a, dir);
+    p.b = support(w, b, vneg(dir));
+    p.w = vsub(p.a, p.b);
+    return p;
+}
+
+static float shape_scale(const world_shape* ws) {
+    float scale = ws->radius + ws->half_height;
+    scale = fmaxo(scale, fmaxo(fmaxo(ws->half_extents.x, ws->half_extents.y), ws->half_extents.z));
+    return fmaxo(scale, 1e-4f) * fmaxo(fmaxo(ws->scale.x, ws->scale.y), ws->scale.z);
+}
+
+static v3 simplex_expand_dir(const simplex_point* s, uint32_t count) {
+    if (count == 1u) return V(1.0f, 0.0f, 0.0f);
+    if (count == 2u) {
+        v3 edge = vsub(s[1].w, s[0].w);
+        v3 d = vcross(edge, V(1.0f, 0.0f, 0.0f));
+        if (vlen(d) < 1e-6f) d = vcross(edge, V(0.0f, 1.0f, 0.0f));
+        return vnorm(d);
+    }
+    v3 normal = vcross(vsub(s[1].w, s[0].w), vsub(s[2].w, s[0].w));
+    if (vlen(normal) > 1e-8f) return vnorm(normal);
+    return V(0.0f, 1.0f, 0.0f);
+}
+
+static convex_closest_t convex_closest(const oracle_world* w, const world_shape* first,
+                                       const world_shape* second, simplex_point* simplex,
+                                       uint32_t* out_count) {
+    uint32_t count = 0;
+    convex_closest_t r;
+    r.distance = 0.0f;
+    r.point_a = first->center;
+    r.point_b = second->center;
+    r.normal = sign_normalize(vsub(second->center, first->center));
+    r.penetrating = 0;
+    v3 dir = vsub(second->center, first->center);
+    if (vlen(dir) < 1e-8f) dir = V(1.0f, 0.0f, 0.0f);
+    simplex[0] = support_pair(w, first, second, dir);
+    count = 1;
+    simplex_result closest = simplex_closest(simplex, count);
+    float tiny = 1e-4f * fmino(shape_scale(first), shape_scale(second));
+    for (uint32_t iter = 0; iter < 24u; iter++) {
+        if (closest.penetrating || vlen(closest.v) < tiny) {
+            while (count < 4u) {
+                v3 d = simplex_expand_dir(simplex, count);
+                simplex[count] = support_pair(w, first, second, d);
+                count++;
+            }
+            r.penetrating = 1;
+            r.distance = 0.0f;
+            r.point_a = closest.point_a;
+            r.point_b = closest.point_b;
+            break;
+        }
+        simplex_point sp = support_pair(w, first, second, vneg(closest.v));
+        v3 nv = vneg(closest.v);
+        if (vdot(sp.w, nv) <= vdot(closest.v, nv) + 1e-4f * fmaxo(1.0f, vdot(closest.v, closest.v))) {
+            r.distance = vlen(closest.v);
+            r.point_a = closest.point_a;
+            r.point_b = closest.point_b;
+            if (r.distance > 1e-6f) r.normal = sign_normalize(vsub(closest.point_b, closest.point_a));
+            break;
+        }
+        count = simplex_reduce(simplex, closest.lambdas);
+        if (simplex_holds(simplex, count, sp.w, shape_scale(second))) {
+            r.distance = vlen(closest.v);
+            r.point_a = closest.point_a;
+            r.point_b = closest.point_b;
+            if (r.distance > 1e-6f) r.normal = sign_normalize(vsub(closest.point_b, closest.point_a));
+            break;
+        }
+        simplex[count] = sp;
+        count++;
+        closest = simplex_closest(simplex, count);
+    }
+    *out_count = count;
+    return r;
+}
+
+/* ---- EPA (convex.wgsl:286-437) -------------------------------------------------------------- */
+
+typedef struct { uint32_t a, b, c; v3 normal; } epa_face;
+
+static epa_result epa_invalid(void) {
+    epa_result r;
+    r.valid = 0;
+    r.normal = V(0.0f, 1.0f, 0.0f);
+    r.depth = 0.0f;
+    r.point = V(0.0f, 0.0f, 0.0f);
+    return r;
+}
+
+static epa_face epa_make_face(uint32_t a, uint32_t b, uint32_t c, v3 unit, v3 centroid) {
+    epa_face f;
+    f.a = a;
+    f.b = b;
+    f.c = c;
+    f.normal = vdot(unit, centroid) > 0.0f ? vneg(unit) : unit;
+    return f;
+}
+
+static epa_result epa_tetrahedron(const oracle_world* w, const world_shape* first,
+                                  const world_shape* second, const simplex_point* simplex,
+                                  uint32_t count) {
+    simplex_point polytope[64];
+    epa_face new_faces[64];
+    epa_face faces[128];
+    uint32_t visible[128];
+    uint32_t vertex_count = 0;
+    for (uint32_t i = 0; i < count; i++) polytope[vertex_count++] = simplex[i];
+    uint32_t face_count = 0;
+    if (vertex_count != 4u) return epa_invalid();
+    for (uint32_t i = 0; i < 4; i++) {
+        uint32_t a = kTetFaces[i][0], b = kTetFaces[i][1], c = kTetFaces[i][2];
+        v3 normal = vcross(vsub(polytope[b].w, polytope[a].w), vsub(polytope[c].w, polytope[a].w));
+        float length_n = vlen(normal);
+        if (length_n < 1e-10f) return epa_invalid();
+        v3 unit = vdiv(normal, length_n);
+        v3 centroid = vmul(vadd(vadd(polytope[a].w, polytope[b].w), polytope[c].w), 1.0f / 3.0f);
+        faces[face_count++] = epa_make_face(a, b, c, unit, centroid);
+    }
+    float best_dist = 0.0f;
+    uint32_t best_face = 0;
+    for (uint32_t iter = 0; iter < 32u; iter++) {
+        best_dist = FLT_BIG;
+        best_face = 0;
+        for (uint32_t i = 0; i < face_count; i++) {
+            float dist = vdot(faces[i].normal, polytope[faces[i].a].w);
+            if (dist < best_dist && vlen(faces[i].normal) > 0.5f) {
+                best_dist = dist;
+                best_face = i;
+            }
+        }
+        v3 probe = support_pair(w, first, second, faces[best_face].normal).w;
+        float gain = vdot(probe, faces[best_face].normal) - best_dist;
+        if (gain < 1e-4f) break;
+        if (vertex_count >= 64u) break;
+        polytope[vertex_count].w = probe;
+        polytope[vertex_count].a = support(w, first, faces[best_face].normal);
+        polytope[vertex_count].b = support(w, second, vneg(faces[best_face].normal));
+        vertex_count++;
+        uint32_t visible_count = 0;
+        for (uint32_t i = 0; i < face_count; i++) {
+            v3 vert = polytope[faces[i].a].w;
+            if (vdot(faces[i].normal, vsub(probe, vert)) > 1e-6f) visible[visible_count++] = i;
+        }
+        uint32_t new_count = 0;
+        for (uint32_t i = 0; i < visible_count; i++) {
+            epa_face face = faces[visible[i]];
+            uint32_t edges[3][2] = {{face.a, face.b}, {face.b, face.c}, {face.c, face.a}};
+            for (uint32_t e = 0; e < 3; e++) {
+                uint32_t start = edges[e][0], end = edges[e][1];
+                int shared = 0;
+                for (uint32_t j = 0; j < visible_count; j++) {
+                    if (j == i) continue;
+                    epa_face other = faces[visible[j]];
+                    if ((other.a == end && other.b == start) || (other.b == end && other.c == start) ||
+                        (other.c == end && other.a == start)) {
+                        shared = 1;
+                    }
+                }
+                if (!shared && new_count < 64u) {
+                    uint32_t apex = vertex_count - 1u;
+                    v3 normal = vcross(vsub(polytope[end].w, polytope[apex].w),
+                                       vsub(polytope[start].w, polytope[apex].w));
+                    float length_n = vlen(normal);
+                    if (length_n > 1e-10f) {
+                        v3 unit = vdiv(normal, length_n);
+                        v3 centroid = vmul(vadd(vadd(polytope[apex].w, polytope[start].w), polytope[end].w),
+                                           1.0f / 3.0f);
+                        new_faces[new_count++] = epa_make_face(apex, start, end, unit, centroid);
+                    }
+                }
+            }
+        }
+        uint32_t write = 0, seen = 0;
+        for (uint32_t i = 0; i < face_count; i++) {
+            if (seen < visible_count && visible[seen] == i) {
+                seen++;
+                continue;
+            }
+            faces[write++] = faces[i];
+        }
+        face_count = write;
+        if (face_count + new_count > 128u) break;
+        for (uint32_t i = 0; i < new_count; i++) faces[face_count++] = new_faces[i];
+    }
+    best_dist = FLT_BIG;
+    for (uint32_t i = 0; i < face_count; i++) {
+        float dist = vdot(faces[i].normal, polytope[faces[i].a].w);
+        if (dist < best_dist && vlen(faces[i].normal) > 0.5f) {
+            best_dist = dist;
+            best_face = i;
+        }
+    }
+    epa_result r;
+    r.valid = best_dist < FLT_BIG;
+    r.normal = faces[best_face].normal;
+    r.depth = best_dist;
+    r.point = polytope[faces[best_face].a].a;
+    return r;
+}
+
+/* ---- penetration fallbacks and convex_hit (convex.wgsl:523-618) ------------------------------ */
+
+static float support_projection_depth(const oracle_world* w, const world_shape* a, const world_shape* b, v3 dir) {
+    v3 n = vnorm(dir);
+    return vdot(vsub(support(w, a, n), support(w, b, vneg(n))), n);
+}
+
+static epa_result convex_penetration_probe(const oracle_world* w, const world_shape* first,
+                                           const world_shape* second, const simplex_point* s,
+                                           uint32_t count) {
+    epa_result best;
+    best.valid = 0;
+    best.normal = V(0.0f, 1.0f, 0.0f);
+    best.depth = FLT_BIG;
+    best.point = vmul(vadd(first->center, second->center), 0.5f);
+    v3 probed[16];
+    uint32_t probe_count = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        for (uint32_t j = i + 1; j < count; j++) {
+            for (uint32_t k = j + 1; k < count; k++) {
+                v3 normal = vcross(vsub(s[j].w, s[i].w), vsub(s[k].w, s[i].w));
+                if (vlen(normal) > 1e-8f && probe_count < 16u) probed[probe_count++] = normal;
+            }
+        }
+    }
+    v3 center_dir = vsub(second->center, first->center);
+    if (vlen(center_dir) > 1e-8f && probe_count < 16u) probed[probe_count++] = center_dir;
+    const v3 axes[6] = {V(1.0f, 0.0f, 0.0f), V(-1.0f, 0.0f, 0.0f), V(0.0f, 1.0f, 0.0f),
+                        V(0.0f, -1.0f, 0.0f), V(0.0f, 0.0f, 1.0f), V(0.0f, 0.0f, -1.0f)};
+    for (uint32_t i = 0; i < 6; i++) {
+        if (probe_count < 16u) probed[probe_count++] = axes[i];
+    }
+    for (uint32_t i = 0; i < probe_count; i++) {
+        float depth = support_projection_depth(w, first, second, probed[i]);
+        if (depth < best.depth) {
+            best.depth = depth;
+            best.normal = vnorm(probed[i]);
+        }
+    }
+    best.valid = best.depth < FLT_BIG;
+    v3 pa = support(w, first, best.normal), pb = support(w, second, vneg(best.normal));
+    best.point = vmul(vadd(pa, pb), 0.5f);
+    return best;
+}
+
+static shape_hit make_hit(float distance, v3 point, v3 normal) {
+    shape_hit h;
+    h.distance = distance;
+    h.point = point;
+    h.normal = normal;
+    h.triangle = NO_TRIANGLE;
+    return h;
+}
+
+static shape_hit convex_hit(const oracle_world* w, const world_shape* first, const world_shape* second) {
+    simplex_point simplex[4];
+    uint32_t count = 0;
+    convex_closest_t closest = convex_closest(w, first, second, simplex, &count);
+    v3 centers = vsub(second->center, first->center);
+    if (!closest.penetrating) {
+        epa_result probe = convex_penetration_probe(w, first, second, simplex, count);
+        if (probe.depth > 0.0f) {
+            v3 normal = probe.normal;
+            if (vdot(normal, centers) < 0.0f) normal = vneg(normal);
+            return make_hit(-probe.depth, vmul(vadd(closest.point_a, closest.point_b), 0.5f), normal);
+        }
+        if (closest.distance > 0.0f) {
+            return make_hit(closest.distance, vmul(vadd(closest.point_a, closest.point_b), 0.5f), closest.normal);
+        }
+        return make_hit(FLT_BIG, V(0.0f, 0.0f, 0.0f), V(0.0f, 0.0f, 0.0f)); /* NO_HIT */
+    }
+    epa_result epa = epa_tetrahedron(w, first, second, simplex, count);
+    shape_hit result;
+    if (epa.valid && epa.depth > 0.0f && epa.depth < FLT_BIG && vlen(epa.normal) > 0.5f &&
+        epa.depth < support_projection_depth(w, first, second, epa.normal) +
+                        0.1f * fmino(shape_scale(first), shape_scale(second))) {
+        v3 pa = support(w, first, epa.normal), pb = support(w, second, vneg(epa.normal));
+        result = make_hit(-epa.depth, vmul(vadd(pa, pb), 0.5f), epa.normal);
+    } else {
+        epa_result probe = convex_penetration_probe(w, first, second, simplex, count);
+        v3 pa = support(w, first, probe.normal), pb = support(w, second, vneg(probe.normal));
+        result = make_hit(-probe.depth, vmul(vadd(pa, pb), 0.5f), probe.normal);
+    }
+    if (vdot(result.normal, centers) < 0.0f) result.normal = vneg(result.normal);
+    return result;
+}
+
+/* ---- feature rings (convex.wgsl:625-900) ----------------------------------------------------- */
+
+static uint32_t box_feature_ring(const world_shape* ws, v3 direction, v3* out, uint32_t* ids,
+                                 uint32_t* out_face) {
+    v3 axes[3] = {qrot(ws->rotation, V(1.0f, 0.0f, 0.0f)), qrot(ws->rotation, V(0.0f, 1.0f, 0.0f)),
+                  qrot(ws->rotation, V(0.0f, 0.0f, 1.0f))};
+    uint32_t best = 0;
+    for (uint32_t i = 1; i < 3; i++) {
+        if (fabsf(vdot(axes[i], direction)) > fabsf(vdot(axes[best], direction))) best = i;
+    }
+    float sign = vdot(axes[best], direction) < 0.0f ? -1.0f : 1.0f;
+    v3 face = vmul(axes[best], sign);
+    v3 center = vadd(ws->center, vmul(face, vcomp(ws->half_extents, best)));
+    v3 u1 = axes[(best + 1u) % 3u], u2 = axes[(best + 2u) % 3u];
+    float e1 = vcomp(ws->half_extents, (best + 1u) % 3u);
+    float e2 = vcomp(ws->half_extents, (best + 2u) % 3u);
+    out[0] = vadd(vadd(center, vmul(u1, e1)), vmul(u2, e2));
+    ids[0] = box_corner_id(best, sign, 1.0f, 1.0f);
+    out[1] = vadd(vsub(center, vmul(u1, e1)), vmul(u2, e2));
+    ids[1] = box_corner_id(best, sign, -1.0f, 1.0f);
+    out[2] = vsub(vsub(center, vmul(u1, e1)), vmul(u2, e2));
+    ids[2] = box_corner_id(best, sign, -1.0f, -1.0f);
+    out[3] = vsub(vadd(center, vmul(u1, e1)), vmul(u2, e2));
+    ids[3] = box_corner_id(best, sign, 1.0f, -1.0f);
+    *out_face = FEATURE_FACE_BIT | (best * 2u + (sign > 0.0f ? 1u : 0u));
+    return 4;
+}
+
+/* Angular order about the ring normal: atan2(y, x) of the two points compared, decided exactly
+ * from the half plane and the sign of a cross product instead of two rounded arctangents
+ * (ring_of, convex.wgsl:696-711). Groups in increasing angle: -pi, (-pi, 0), [0, pi), pi. */
+static int angle_group(float x, float y) {
+    if (y == 0.0f && x < 0.0f) return signbit(y) ? 0 : 3;
+    if (y < 0.0f) return 1;
+    return 2;
+}
+
+static int angle_before(float bx, float by, float ax, float ay) {
+    if (bx == 0.0f && by == 0.0f) bx = 1.0f;
+    if (ax == 0.0f && ay == 0.0f) ax = 1.0f;
+    int gb = angle_group(bx, by), ga = angle_group(ax, ay);
+    if (gb != ga) return gb < ga;
+    if (gb == 0 || gb == 3) return 0;
+    return bx * ay - by * ax > 0.0f;
+}
+
+static uint32_t ring_of(const oracle_world* w, const world_shape* ws, v3 direction, v3* points,
+                        uint32_t* ids, uint32_t count, v3* out, uint32_t* out_ids) {
+    if (count <= 1u) {
+        out[0] = support(w, ws, direction);
+        out_ids[0] = ids[0];
+        return 1;
+    }
+    v3 n = sign_normalize(direction);
+    v3 center = V(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < count; i++) center = vadd(center, points[i]);
+    center = vdiv(center, (float)count);
+    v3 u = fabsf(n.x) > 0.9f ? vnorm(vcross(n, V(0.0f, 1.0f, 0.0f))) : vnorm(vcross(n, V(1.0f, 0.0f, 0.0f)));
+    v3 v = vcross(n, u);
+    for (uint32_t i = 0; i < count; i++) {
+        for (uint32_t j = i + 1; j < count; j++) {
+            v3 pa = vsub(points[i], center), pb = vsub(points[j], center);
+            if (angle_before(vdot(pb, u), vdot(pb, v), vdot(pa, u), vdot(pa, v))) {
+                v3 tp = points[i];
+                points[i] = points[j];
+                points[j] = tp;
+                uint32_t ti = ids[i];
+                ids[i] = ids[j];
+                ids[j] = ti;
+            }
+        }
+    }
+    uint32_t keep = count < FEATURE_MAX ? count : FEATURE_MAX;
+    for (uint32_t i = 0; i < keep; i++) {
+        out[i] = points[i];
+        out_ids[i] = ids[i];
+    }
+    return keep;
+}
+
+/* convex.wgsl:723-793, the linear path */
+static uint32_t hull_feature_ring(const oracle_world* w, const world_shape* ws, v3 direction, v3* out,
+                                  uint32_t* out_ids, uint32_t* out_face) {
+    const dyn_shape_source* src = &w->sources[ws->source];
+    v3 d = vmulv(qrot(qconj(ws->rotation), direction), ws->scale);
+    float best_dot = vdot(support_local(w, ws, d), d);
+    v3 extent = vsub(vload(src->local_max), vload(src->local_min));
+    float eps = 1e-3f * fmaxo(fmaxo(extent.x, extent.y), extent.z) *
+                fmaxo(fmaxo(ws->scale.x, ws->scale.y), ws->scale.z);
+    v3 points[16];
+    uint32_t ids[16];
+    uint32_t count = 0, first_id = EMPTY_RING_FACE;
+    for (uint32_t i = 0; i < src->vertex_count && count < 16u; i++) {
+        v3 v = hull_vertex(w, src, i);
+        if (vdot(v, d) >= best_dot - eps) {
+            points[count] = vadd(ws->center, qrot(ws->rotation, vmulv(v, ws->scale)));
+            ids[count] = i;
+            first_id = first_id < i ? first_id : i;
+            count++;
+        }
+    }
+    *out_face = FEATURE_FACE_BIT | first_id;
+    return ring_of(w, ws, direction, points, ids, count, out, out_ids);
+}
+
+/* convex.wgsl:795-819 */
+static uint32_t capsule_feature(const world_shape* ws, v3 direction, v3* out, uint32_t* ids, uint32_t* out_face) {
+    v3 ld = vmulv(qrot(qconj(ws->rotation), direction), ws->scale);
+    float cap = ld.y > 0.0f ? 1.0f : -1.0f;
+    v3 radial = vmul(sign_normalize(ld), ws->radius);
+    *out_face = FEATURE_FACE_BIT | 0u;
+    if (fabsf(ld.y) > CAP_ALIGNMENT * vlen(ld)) {
+        v3 local = vadd(V(0.0f, cap * ws->half_height, 0.0f), radial);
+        out[0] = vadd(ws->center, qrot(ws->rotation, vmulv(local, ws->scale)));
+        ids[0] = cap > 0.0f ? 1u : 0u;
+        return 1;
+    }
+    v3 upper = vadd(V(0.0f, ws->half_height, 0.0f), radial);
+    v3 lower = vadd(V(0.0f, -ws->half_height, 0.0f), radial);
+    out[0] = vadd(ws->center, qrot(ws->rotation, vmulv(upper, ws->scale)));
+    out[1] = vadd(ws->center, qrot(ws->rotation, vmulv(lower, ws->scale)));
+    ids[0] = 0u;
+    ids[1] = 1u;
+    return 2;
+}
+
+/* convex.wgsl:821-854 */
+static uint32_t cylinder_feature(const world_shape* ws, v3 direction, v3* out, uint32_t* ids, uint32_t* out_face) {
+    v3 axis = shape_axis(ws);
+    float alignment = vdot(axis, vnorm(direction));
+    if (fabsf(alignment) > CAP_ALIGNMENT) {
+        float sign = alignment < 0.0f ? -1.0f : 1.0f;
+        uint32_t cap = sign > 0.0f ? 1u : 0u;
+        v3 cap_center = vadd(ws->center, vmul(vmul(axis, ws->half_height), sign));
+        v3 u = vnorm(vcross(axis, V(1.0f, 0.0f, 0.0f)));
+        if (vlen(vcross(axis, V(1.0f, 0.0f, 0.0f))) < 1e-6f) u = vnorm(vcross(axis, V(0.0f, 0.0f, 1.0f)));
+        v3 v = vcross(axis, u);
+        for (uint32_t i = 0; i < CYLINDER_CAP_POINTS; i++) {
+            out[i] = vadd(cap_center, vmul(vadd(vmul(u, kCapCos[i]), vmul(v, kCapSin[i])), ws->radius));
+            ids[i] = cap * CYLINDER_CAP_POINTS + i;
+        }
+        *out_face = FEATURE_FACE_BIT | cap;
+        return CYLINDER_CAP_POINTS;
+    }
+    v3 side = sign_normalize(vsub(direction, vmul(axis, vdot(direction, axis))));
+    out[0] = vadd(vadd(ws->center, vmul(axis, ws->half_height)), vmul(side, ws->radius));
+    ids[0] = CYLINDER_SIDE_POINTS;
+    out[1] = vadd(vsub(ws->center, vmul(axis, ws->half_height)), vmul(side, ws->radius));
+    ids[1] = CYLINDER_SIDE_POINTS + 1u;
+    *out_face = FEATURE_FACE_BIT | CYLINDER_SIDE_FACE;
+    return 2;
+}
+
+/* convex.wgsl:856-889 */
+static uint32_t shape_feature(const oracle_world* w, const world_shape* ws, v3 direction, v3* out,
+                              uint32_t* ids, uint32_t* out_face, int* ring) {
+    if (ws->kind == DYN_SHAPE_CUBOID) {
+        *ring = 1;
+        return box_feature_ring(ws, direction, out, ids, out_face);
+    }
+    if (ws->kind == DYN_SHAPE_HULL) {
+        uint32_t count = hull_feature_ring(w, ws, direction, out, ids, out_face);
+        *ring = count > 2u;
+        return count;
+    }
+    if (ws->kind == DYN_SHAPE_CAPSULE) {
+        *ring = 0;
+        return capsule_feature(ws, direction, out, ids, out_face);
+    }
+    if (ws->kind == DYN_SHAPE_CYLINDER) {
+        uint32_t count = cylinder_feature(ws, direction, out, ids, out_face);
+        *ring = count > 2u;
+        return count;
+    }
+    *ring = 0;
+    out[0] = support(w, ws, direction);
+    ids[0] = 0u;
+    *out_face = FEATURE_FACE_BIT | 0u;
+    return 1;
+}
+
+/* convex.wgsl:891-902 */
+static float feature_radius(const v3* points, uint32_t count) {
+    v3 center = V(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < count; i++) center = vadd(center, points[i]);
+    center = vdiv(center, (float)count);
+    float radius = 0.0f;
+    for (uint32_t i = 0; i < count; i++) radius = fmaxo(radius, vlen(vsub(points[i], center)));
+    return radius;
+}
+
+/* convex.wgsl:910-982: Sutherland-Hodgman clip of a feature against the side planes of a ring */
+static uint32_t clip_feature_to_ring(const v3* points, const uint32_t* ids, uint32_t count,
+                                     const v3* ring, uint32_t ring_count, v3 ref_dir, v3* held,
+                                     uint32_t* held_ids) {
+    v3 kept[MANIFOLD_CANDIDATES];
+    uint32_t kept_ids[MANIFOLD_CANDIDATES];
+    for (uint32_t i = 0; i < count; i++) {
+        held[i] = points[i];
+        held_ids[i] = ids[i];
+    }
+    uint32_t live = count;
+    for (uint32_t side = 0; side < ring_count; side++) {
+        v3 current = ring[side], next = ring[(side + 1u) % ring_count];
+        v3 plane_normal = vnorm(vcross(vsub(next, current), ref_dir));
+        uint32_t written = 0;
+        if (live == 2u) {
+            v3 p0 = held[0], p1 = held[1];
+            float s0 = vdot(vsub(p0, current), plane_normal) - CLIP_MARGIN;
+            float s1 = vdot(vsub(p1, current), plane_normal) - CLIP_MARGIN;
+            if (s0 <= 0.0f) {
+                kept[written] = p0;
+                kept_ids[written] = held_ids[0];
+                written++;
+            }
+            if (s0 * s1 < 0.0f && written < MANIFOLD_CANDIDATES) {
+                kept[written] = vadd(p0, vmul(vsub(p1, p0), s0 / (s0 - s1)));
+                kept_ids[written] = FEATURE_CLIP | (side * MANIFOLD_CANDIDATES);
+                written++;
+            }
+            if (s1 <= 0.0f && written < MANIFOLD_CANDIDATES) {
+                kept[written] = p1;
+                kept_ids[written] = held_ids[1];
+                written++;
+            }
+        } else {
+            for (uint32_t i = 0; i < live; i++) {
+                v3 cp = held[i], np = held[(i + 1u) % live];
+                float cs = vdot(vsub(cp, current), plane_normal) - CLIP_MARGIN;
+                float ns = vdot(vsub(np, current), plane_normal) - CLIP_MARGIN;
+                if (cs <= 0.0f && written < MANIFOLD_CANDIDATES) {
+                    kept[written] = cp;
+                    kept_ids[written] = held_ids[i];
+                    written++;
+                }
+                if (cs * ns < 0.0f && written < MANIFOLD_CANDIDATES) {
+                    kept[written] = vadd(cp, vmul(vsub(np, cp), cs / (cs - ns)));
+                    kept_ids[written] = FEATURE_CLIP | (side * MANIFOLD_CANDIDATES + i);
+                    written++;
+                }
+            }
+        }
+        live = written;
+        for (uint32_t i = 0; i < written; i++) {
+            held[i] = kept[i];
+            held_ids[i] = kept_ids[i];
+        }
+        if (live == 0u) break;
+    }
+    return live;
+}
+
+/* convex.wgsl:984-1071 */
+static int convex_pair_manifold(const oracle_world* w, const world_shape* first, const world_shape* second,
+                                v3 direction, dyn_contact* c) {
+    v3 fp[FEATURE_MAX], sp[FEATURE_MAX];
+    uint32_t fi[FEATURE_MAX], si[FEATURE_MAX];
+    uint32_t first_face = 0, second_face = 0;
+    int first_ring = 0, second_ring = 0;
+    uint32_t first_count = shape_feature(w, first, direction, fp, fi, &first_face, &first_ring);
+    uint32_t second_count = shape_feature(w, second, vneg(direction), sp, si, &second_face, &second_ring);
+    if (!first_ring && !second_ring) return 0;
+    int first_reference = first_ring;
+    if (first_ring && second_ring) {
+        first_reference = feature_radius(fp, first_count) >= feature_radius(sp, second_count);
+    }
+    const v3 *reference, *incident;
+    const uint32_t* incident_ids;
+    uint32_t reference_count, incident_count, reference_face;
+    v3 ref_dir;
+    if (first_reference) {
+        reference = fp;
+        reference_count = first_count;
+        reference_face = first_face;
+        incident = sp;
+        incident_ids = si;
+        incident_count = second_count;
+        ref_dir = direction;
+    } else {
+        reference = sp;
+        reference_count = second_count;
+        reference_face = second_face;
+        incident = fp;
+        incident_ids = fi;
+        incident_count = first_count;
+        ref_dir = vneg(direction);
+    }
+    v3 center = V(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = 0; i < reference_count; i++) center = vadd(center, reference[i]);
+    center = vdiv(center, (float)reference_count);
+    v3 deepest = V(0.0f, 0.0f, 0.0f);
+    float deepest_depth = -1e30f;
+    for (uint32_t i = 0; i < incident_count; i++) {
+        float depth = vdot(vsub(center, incident[i]), ref_dir);
+        if (depth > deepest_depth) {
+            deepest_depth = depth;
+            deepest = incident[i];
+        }
+    }
+    v3 clipped[MANIFOLD_CANDIDATES];
+    uint32_t clipped_ids[MANIFOLD_CANDIDATES];
+    uint32_t clipped_count = clip_feature_to_ring(incident, incident_ids, incident_count, reference,
+                                                  reference_count, ref_dir, clipped, clipped_ids);
+    candidate_t cand[MANIFOLD_CANDIDATES];
+    uint32_t cc = 0;
+    for (uint32_t i = 0; i < clipped_count && cc < MANIFOLD_CANDIDATES; i++) {
+        float depth = vdot(vsub(center, clipped[i]), ref_dir);
+        if (depth > 0.0f) {
+            cand[cc].position = vsub(clipped[i], vmul(ref_dir, depth * 0.5f));
+            cand[cc].depth = depth;
+            cand[cc].feature = first_reference ? feature_pair(FEATURE_VERTEX, reference_face, clipped_ids[i])
+                                               : feature_pair(FEATURE_VERTEX, clipped_ids[i], reference_face);
+            cc++;
+        }
+    }
+    if (cc == 0u) {
+        if (deepest_depth <= 0.0f) return 0;
+        manifold_push(c, vsub(deepest, vmul(ref_dir, deepest_depth * 0.5f)), deepest_depth, FEATURE_POINT);
+        return 1;
+    }
+    manifold_keep(c, cand, cc);
+    return 1;
+}
+
+/* narrowphase.wgsl:479-483 */
+static void manifold_from_hit(dyn_contact* c, shape_hit hit, float margin) {
+    if (hit.distance <= margin) manifold_push(c, hit.point, -hit.distance, FEATURE_POINT);
+}
+
+/* narrowphase.wgsl:485-512 */
+static dyn_contact plane_convex(const oracle_world* w, const world_shape* plane, const world_shape* convex,
+                               float margin) {
     dyn_contact c;
     memset(&c, 0, sizeof c);
     v3 n = qrot(plane->rotation, V(0.0f, 1.0f, 0.0f));
@@ -776,7 +1590,9 @@ static dyn_contact plane_convex(const world_shape* plane, const world_shape* con
     manifold_emit(&c, facing);
     v3 pts[FEATURE_MAX];
     uint32_t ids[FEATURE_MAX];
-    uint32_t count = shape_feature(convex, vneg(facing), pts, ids);
+    uint32_t feature = 0;
+    int ring = 0;
+    uint32_t count = shape_feature(w, convex, vneg(facing), pts, ids, &feature, &ring);
     candidate_t cand[MANIFOLD_CANDIDATES];
     uint32_t cc = 0;
     for (uint32_t i = 0; i < count; i++) {
@@ -809,7 +1625,7 @@ static int filters_intersect(uint32_t g1, uint32_t m1, uint32_t g2, uint32_t m2)
     return (g1 & m2) != 0u && (g2 & m1) != 0u;
 }
 
-static void narrow_work(oracle_world* w, uint32_t index, uint32_t* unsupported) {
+static void narrow_work(oracle_world* w, uint32_t index) {
     w->valid[index] = 0;
     uint32_t fs = w->pmaj[index], ss = w->pmin[index];
     uint32_t fb = w->owner[fs], sb = w->owner[ss];
@@ -831,23 +1647,27 @@ static void narrow_work(oracle_world* w, uint32_t index, uint32_t* unsupported) 
     int fw = fc->kind == DYN_SHAPE_PLANE, sw = sc->kind == DYN_SHAPE_PLANE;
     if (fw && sw) return;
     float margin = pair_margin(w, &first, &second);
+    world_shape wf = world_collider(&first.s, fc), wsd = world_collider(&second.s, sc);
     if (fw) {
-        world_shape scene = world_collider(&first.s, fc), other = world_collider(&second.s, sc);
-        if (!shape_analytic(other.kind)) { (*unsupported)++; return; }
-        c = plane_convex(&scene, &other, margin);
+        c = plane_convex(w, &wf, &wsd, margin);
         generated = c.point_count > 0u;
     } else if (sw) {
-        world_shape scene = world_collider(&second.s, sc), other = world_collider(&first.s, fc);
-        if (!shape_analytic(other.kind)) { (*unsupported)++; return; }
-        c = plane_convex(&scene, &other, margin);
+        c = plane_convex(w, &wsd, &wf, margin);
         c.normal[0] = -c.normal[0];
         c.normal[1] = -c.normal[1];
         c.normal[2] = -c.normal[2];
         for (uint32_t i = 0; i < c.point_count; i++) c.points[i].feature = feature_mirror(c.points[i].feature);
         generated = c.point_count > 0u;
-    } else if (scaled(fc) || scaled(sc) || !shape_analytic(fc->kind) || !shape_analytic(sc->kind)) {
-        (*unsupported)++;
-        return;
+    } else if (scaled(fc) || scaled(sc) || !shape_analytic(fc->kind) || !shape_analytic(sc->kind) ||
+               (fc->kind == DYN_SHAPE_CAPSULE && sc->kind == DYN_SHAPE_CUBOID) ||
+               (fc->kind == DYN_SHAPE_CUBOID && sc->kind == DYN_SHAPE_CAPSULE)) {
+        /* GJK / EPA with a clipped feature manifold (narrowphase.wgsl:594-628, 654-664) */
+        shape_hit hit = convex_hit(w, &wf, &wsd);
+        if (hit.distance <= margin) {
+            manifold_emit(&c, hit.normal);
+            generated = 1;
+            if (!convex_pair_manifold(w, &wf, &wsd, hit.normal, &c)) manifold_from_hit(&c, hit, margin);
+        }
     } else {
         uint32_t a = fc->kind, b = sc->kind;
         if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_SPHERE) c = sphere_sphere(&first, fc, &second, sc, margin);
@@ -865,7 +1685,6 @@ static void narrow_work(oracle_world* w, uint32_t index, uint32_t* unsupported) 
             c.normal[1] = -c.normal[1];
             c.normal[2] = -c.normal[2];
         } else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_CAPSULE) c = capsule_capsule(&first, fc, &second, sc, margin);
-        else { (*unsupported)++; return; } /* capsule-box goes through GJK/EPA */
         generated = 1;
     }
     if (!generated) return;
@@ -1540,12 +2359,12 @@ static void prepare_and_index(oracle_world* w) {
         const dyn_collider* c = &w->co[i];
         world_shape ws = world_collider(s, c);
         v3 tmn, tmx;
-        world_aabb_of(&ws, &tmn, &tmx);
+        world_aabb_of(w, &ws, &tmn, &tmx);
         world_shape up = ws;
         up.rotation.x = up.rotation.y = up.rotation.z = 0.0f;
         up.rotation.w = 1.0f;
         v3 rmn, rmx;
-        world_aabb_of(&up, &rmn, &rmx);
+        world_aabb_of(w, &up, &rmn, &rmx);
         vstore(w->res[i].min, rmn);
         vstore(w->res[i].max, rmx);
         v3 smn = tmn, smx = tmx;
@@ -1602,8 +2421,7 @@ void oracle_world_step(oracle_world* w, float dt) {
     prepare_and_index(w);
 
     /* narrowphase + compaction (compact_*.wgsl) */
-    uint32_t unsupported = 0;
-    for (uint32_t i = 0; i < w->npairs; i++) narrow_work(w, i, &unsupported);
+    for (uint32_t i = 0; i < w->npairs; i++) narrow_work(w, i);
     w->ncontacts = 0;
     for (uint32_t i = 0; i < w->npairs; i++) {
         if (!w->valid[i]) continue;
@@ -1689,7 +2507,6 @@ void oracle_world_step(oracle_world* w, float dt) {
     }
     *ctr(w, DYN_COUNTER_RESTING) = w->nresting;
     (*ctr(w, DYN_COUNTER_STEP))++;
-    (void)unsupported;
 }
 
 /* ---- lifetime and accessors ------------------------------------------------------------- */
@@ -1757,9 +2574,55 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
     return w;
 }
 
+int oracle_world_shapes(oracle_world* w, const dyn_shape_source* sources, uint32_t source_count,
+                        const float* vertices, uint32_t vertex_count) {
+    for (uint32_t i = 0; i < source_count; i++) {
+        if (sources[i].vertex_count > LINEAR_SUPPORT_VERTICES ||
+            (uint64_t)sources[i].vertex_offset + sources[i].vertex_count > vertex_count) {
+            return -1;
+        }
+    }
+    free(w->sources);
+    free(w->vertices);
+    ALLOC(w->sources, source_count);
+    ALLOC(w->vertices, 4u * (size_t)vertex_count);
+    memcpy(w->sources, sources, (size_t)source_count * sizeof *sources);
+    memcpy(w->vertices, vertices, 16u * (size_t)vertex_count);
+    w->source_count = source_count;
+    w->vertex_count = vertex_count;
+    return 0;
+}
+
+/* Test hook: GJK closest points and convex_hit of two colliders at given poses, over the given
+ * shape sources. out = {distance, point_a xyz, point_b xyz, normal xyz, penetrating,
+ * hit distance, hit normal xyz}. */
+int oracle_convex_query(const dyn_shape_source* sources, uint32_t source_count, const float* vertices,
+                        uint32_t vertex_count, const dyn_collider* ca, const dyn_body_state* sa,
+                        const dyn_collider* cb, const dyn_body_state* sb, float* out) {
+    oracle_world w;
+    memset(&w, 0, sizeof w);
+    w.sources = (dyn_shape_source*)sources;
+    w.source_count = source_count;
+    w.vertices = (float*)vertices;
+    w.vertex_count = vertex_count;
+    world_shape a = world_collider(sa, ca), b = world_collider(sb, cb);
+    simplex_point simplex[4];
+    uint32_t count = 0;
+    convex_closest_t c = convex_closest(&w, &a, &b, simplex, &count);
+    shape_hit hit = convex_hit(&w, &a, &b);
+    out[0] = c.distance;
+    vstore(out + 1, c.point_a);
+    vstore(out + 4, c.point_b);
+    vstore(out + 7, c.normal);
+    out[10] = c.penetrating ? 1.0f : 0.0f;
+    out[11] = hit.distance;
+    vstore(out + 12, hit.normal);
+    return 0;
+}
+
 void oracle_world_destroy(oracle_world* w) {
     if (!w) return;
-    void* ptrs[] = {w->st, w->de, w->co, w->owner, w->row_of_body, w->wake, w->admitted, w->activity, w->motion,
+    void* ptrs[] = {w->st, w->de, w->co, w->owner, w->row_of_body, w->sources, w->vertices, w->wake, w->admitted, w->activity, w->motion,
                     w->aabbs, w->res, w->flags, w->keys, w->order, w->entries, w->pmaj, w->pmin,
                     w->raw, w->valid, w->contacts, w->matched, w->archive, w->resting,
                     w->parents, w->istate, w->live, w->block_counts, w->blocks, w->rows,

@@ -62,6 +62,10 @@ class Oracle:
         h.oracle_world_counters.argtypes = [_p, _p]
         h.oracle_world_pairs.restype = _u32
         h.oracle_world_pairs.argtypes = [_p, _p, _p, _u32]
+        h.oracle_world_shapes.restype = ctypes.c_int
+        h.oracle_world_shapes.argtypes = [_p, _p, _u32, _p, _u32]
+        h.oracle_convex_query.restype = ctypes.c_int
+        h.oracle_convex_query.argtypes = [_p, _u32, _p, _u32, _p, _p, _p, _p, _p]
         h.oracle_inertia_inverse.restype = None
         h.oracle_inertia_inverse.argtypes = [_p, _p]
 
@@ -125,6 +129,24 @@ class Oracle:
         self.h.oracle_inertia_inverse(tensor.ctypes.data, out.ctypes.data)
         return out
 
+    def convex_query(self, scene, row_a, row_b):
+        """GJK closest points and convex_hit of the colliders of two rows of a scene (their
+        body states as encoded): a dict of distance, point_a, point_b, normal, penetrating,
+        hit_distance, hit_normal."""
+        out = np.zeros(15, np.float32)
+        ca = scene.colliders[[int(np.nonzero(scene.owners == row_a)[0][0])]]
+        cb = scene.colliders[[int(np.nonzero(scene.owners == row_b)[0][0])]]
+        sa = scene.states[[row_a]]
+        sb = scene.states[[row_b]]
+        src = scene.sources if scene.sources is not None and len(scene.sources) else np.zeros(1, abi.SHAPE_SOURCE)
+        verts = scene.vertices if scene.vertices is not None and len(scene.vertices) else np.zeros((1, 4), np.float32)
+        self.h.oracle_convex_query(src.ctypes.data, len(src), verts.ctypes.data, len(verts),
+                                   ca.ctypes.data, sa.ctypes.data, cb.ctypes.data, sb.ctypes.data,
+                                   out.ctypes.data)
+        return {"distance": float(out[0]), "point_a": out[1:4].copy(), "point_b": out[4:7].copy(),
+                "normal": out[7:10].copy(), "penetrating": bool(out[10]),
+                "hit_distance": float(out[11]), "hit_normal": out[12:15].copy()}
+
     def world(self, scene, contact_capacity=None) -> "OracleWorld":
         return OracleWorld(self, scene, contact_capacity)
 
@@ -144,6 +166,11 @@ class OracleWorld:
             self.capacity)
         if not self.w:
             raise MemoryError("oracle_world_create failed")
+        if scene.sources is not None and len(scene.sources):
+            rc = oracle.h.oracle_world_shapes(self.w, scene.sources.ctypes.data, len(scene.sources),
+                                              scene.vertices.ctypes.data, len(scene.vertices))
+            if rc != 0:
+                raise ValueError("oracle_world_shapes refused the shape sources")
 
     def step(self, dt: float) -> None:
         self.o.h.oracle_world_step(self.w, ctypes.c_float(dt))
