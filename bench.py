@@ -1,15 +1,22 @@
 #!/usr/bin/env python3
-"""Throughput benchmark of the rigid-body step path on one MI355X.
+"""Throughput benchmark of the rigid-body step on one MI355X.
 
-Workload: the isolated sort + broadphase pass over 1,048,576 boxes (centres uniform in a
-1000 m cube, per-axis half-extents log-uniform in [0.2, 2] m, FP32 boxes, seed 2): base-cell
-resolution, grid-entry emission, the 4-digit entry sort, cell-mate and level-link pair
-emission, and the canonical pair sort. One step = one such pass; body-steps/s = boxes x passes
-per second. Inputs are resident in HBM before the timed region.
+Workloads (`--workload`):
+  pile        (default) the mixed convex pile: 100,000 bodies, a quarter each of spheres, boxes,
+              capsules and ellipsoid hulls of 16-64 points, density 1000, on a jittered lattice in
+              a 60 x 60 m bin of four static walls on a plane; FP32, dt = 1/60, 8 solver
+              iterations (2 substeps, 10 position iterations); the run is 600 steps, of which the
+              first `--warmup` are untimed.
+  cubes       1,000 cubes (half-extent 0.5 m, density 1000, friction 0.5) dropped from 1-40 m over
+              a 20 x 20 m footprint onto a plane; 4 solver iterations; 300 steps.
+  broadphase  the isolated sort + broadphase pass over 1,048,576 boxes.
+One step = one World::step of every body (broadphase: one pass over every box); body-steps/s =
+bodies x timed steps / seconds. Inputs are resident in HBM before the timed region; the timed
+region is bracketed by events on the world's own stream.
 
-Prints ONE JSON line (rank 0). `roofline` covers the kernel with the largest share of device time
-in a separately profiled run of the same passes; `cpu_baseline` times the single-threaded C
-oracle on the same workload.
+Prints ONE JSON line. `roofline` covers the kernel with the largest share of device time in a
+separately profiled window of steps (every launch bracketed by events on its stream);
+`cpu_baseline` times the single-threaded C oracle on a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -28,17 +35,18 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "body-steps/s (100k convex pile) + batched vs plain; % HBM peak; bwd/fwd ratio"
+DT = np.float32(1.0 / 60.0)
 BOXES = 1 << 20
-SEED = 2
+BOX_SEED = 2
 
 
-def kernel_bytes(name: str, k: int, entries: int, pairs: int) -> float:
-    """Algorithmic HBM bytes of one launch of each broadphase kernel: the minimal payload the
-    launch must read and write (entry = key 4 + slot 4 + record 32 bytes; box = 24 bytes of
-    min/max; pair = 8 bytes)."""
-    key_pay = 8.0
+# ---- algorithmic bytes per launch ---------------------------------------------------------------
+
+def broadphase_bytes(name: str, k: int, entries: int, pairs: int) -> float:
+    """Minimal HBM payload of one launch of each broadphase kernel (entry = key 4 + slot 4 +
+    record 32 bytes; box = 24 bytes of min/max; pair = 8 bytes)."""
     if name == "entry_sort_pass":        # read key+slot, write key+slot
-        return 2.0 * key_pay * entries
+        return 16.0 * entries
     if name == "bp_mates_kernel":        # every sorted key read once; staged pairs written
         return 4.0 * entries + 8.0 * pairs
     if name == "bp_links_kernel":        # box + packed count per collider; pairs written
@@ -56,7 +64,108 @@ def kernel_bytes(name: str, k: int, entries: int, pairs: int) -> float:
     return 0.0
 
 
-def run_gpu(steps: int, warmup: int):
+def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: int, points: float,
+               rows: int) -> float:
+    """Minimal HBM payload of one launch of each step kernel, from the step's own counts:
+    N bodies, K colliders, E entries, P pairs, C contacts of m points on average, R solver rows.
+    A manifold is M(m) = 48 + 56 m bytes (normal, pair and material words plus per point anchor,
+    depth, impulses and feature); pose + velocity 52 B and body constants 44 B per body."""
+    manifold = 48.0 + 56.0 * points
+    if name == "k_solve":               # manifold, two bodies' pose/velocity/constants, targets,
+        return contacts * (manifold + 16.0 * points + 240.0)  # delta words
+    if name == "k_position_block":
+        return contacts * (48.0 + 20.0 * points + 272.0)
+    if name in ("k_apply", "k_position_apply"):
+        return rows * 72.0
+    if name == "k_narrow":              # two bodies and colliders per pair, manifold out
+        return pairs * 316.0 + contacts * manifold
+    if name == "k_narrow_convex":
+        return pairs * 316.0 + contacts * manifold
+    if name in ("k_integrate", "k_advance"):
+        return n * 126.0
+    if name in ("k_relay", "k_contact_begin", "k_contacts_scatter", "k_archive"):
+        return 2.0 * contacts * manifold
+    return broadphase_bytes(name, k, entries, pairs)
+
+
+# ---- workloads ------------------------------------------------------------------------------------
+
+def pile_scene():
+    from dynamis_amd.scenes import convex_pile
+    from dynamis_amd.world import default_config, encode
+    bodies, hulls = convex_pile(100_000, seed=3)
+    return encode(bodies, default_config(), hulls)
+
+
+def cubes_scene():
+    from dynamis_amd.scenes import cube_drop
+    from dynamis_amd.world import default_config, encode
+    cfg = default_config()
+    cfg["solve_iterations"] = 4
+    return encode(cube_drop(1000, seed=1), cfg)
+
+
+WORKLOADS = {
+    "pile": {"scene": pile_scene, "steps": 540, "warmup": 60, "run": 600,
+             "describe": "mixed convex pile: 100,000 bodies (25% each spheres r~U[0.25,0.75], boxes "
+                         "h~U[0.2,0.8]^3, capsules r~U[0.15,0.4] hh~U[0.2,0.8], hulls of 16-64 points "
+                         "on ellipsoids with semi-axes ~U[0.3,0.9] from 512 shape sources) on a "
+                         "jittered lattice in a 60x60 m walled bin on a plane, density 1000, "
+                         "dt=1/60, 8 solver iterations, 2 substeps, 10 position iterations"},
+    "cubes": {"scene": cubes_scene, "steps": 290, "warmup": 10, "run": 300,
+              "describe": "1,000 cubes (h=0.5 m, density 1000, friction 0.5) dropped from 1-40 m "
+                          "over 20x20 m onto a plane, dt=1/60, 4 solver iterations"},
+}
+
+
+def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
+    import ctypes
+
+    import dynamis_amd
+    from dynamis_amd._native import call
+    from dynamis_amd.device import profile_collect, profile_enable
+    from dynamis_amd.world import World
+
+    if dynamis_amd.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    call("dyn_set_device", 0)
+    scene = WORKLOADS[workload]["scene"]()
+    world = World()
+    world.create(scene)
+    for _ in range(warmup):
+        world.step(DT)
+    world.wait()
+    stream = ctypes.c_void_p(world.stream())
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    call("dyn_event_create", ctypes.byref(e0))
+    call("dyn_event_create", ctypes.byref(e1))
+    call("dyn_event_record", e0, stream)
+    for _ in range(steps):
+        world.step(DT)
+    call("dyn_event_record", e1, stream)
+    world.wait()
+    ms = ctypes.c_float()
+    call("dyn_event_elapsed_ms", ctypes.byref(ms), e0, e1)
+    m = world.measured()
+    contacts = world.inspect_contacts()
+    points = float(np.mean(contacts["point_count"])) if len(contacts) else 0.0
+    # a profiled window of further steps
+    profile_enable(True)
+    for _ in range(profile_steps):
+        world.step(DT)
+    world.wait()
+    profile_enable(False)
+    stages = profile_collect()
+    m2 = world.measured()
+    counts = {"n": len(scene.states), "k": len(scene.colliders), "entries": m2["ENTRIES"],
+              "pairs": m2["PAIRS"], "contacts": m2["CONTACTS"], "points": points,
+              "rows": m2["SOLVER_ROWS"]}
+    final_states = world.states()
+    world.close()
+    return ms.value, len(scene.states), stages, counts, m, scene, final_states
+
+
+def run_broadphase(steps: int, warmup: int):
     import dynamis_amd
     from dynamis_amd import abi
     from dynamis_amd.broadphase import Broadphase
@@ -66,7 +175,7 @@ def run_gpu(steps: int, warmup: int):
     if dynamis_amd.device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
     dynamis_amd._native.call("dyn_set_device", 0)
-    boxes = uniform_boxes(BOXES, 1000.0, 0.2, 2.0, SEED)
+    boxes = uniform_boxes(BOXES, 1000.0, 0.2, 2.0, BOX_SEED)
     stream = Stream()
     bp = Broadphase(BOXES, stream=stream)
     bp.upload(boxes)
@@ -76,7 +185,6 @@ def run_gpu(steps: int, warmup: int):
     res = bp.read()
     entries = abi.counter(res.counters, "ENTRIES")
     pairs = abi.counter(res.counters, "PAIRS")
-
     start, stop = Event(), Event()
     start.record(stream)
     for _ in range(steps):
@@ -84,75 +192,127 @@ def run_gpu(steps: int, warmup: int):
     stop.record(stream)
     ms = start.elapsed_ms(stop)
     stream.synchronize()
-
-    # Same passes again with every launch bracketed by events on the bench stream.
     profile_enable(True)
-    for _ in range(steps):
+    for _ in range(min(steps, 10)):
         bp.run()
     stream.synchronize()
     profile_enable(False)
     stages = profile_collect()
-    return ms, entries, pairs, stages
+    counts = {"n": BOXES, "k": BOXES, "entries": entries, "pairs": pairs, "contacts": 0,
+              "points": 0.0, "rows": 0}
+    return ms, BOXES, stages, counts
 
 
-def run_cpu(passes: int):
+# ---- CPU baseline ------------------------------------------------------------------------------
+
+def cpu_world(scene, states, sample_steps: int):
+    import copy
+
+    from tests.oracle_binding import load
+    oracle = load()
+    start = copy.copy(scene)
+    start.states = states.copy()
+    world = oracle.world(start)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        world.step(DT)
+    secs = time.perf_counter() - t0
+    world.close()
+    return len(scene.states) * sample_steps / secs, secs
+
+
+def cpu_broadphase(passes: int):
     from dynamis_amd.scenes import uniform_boxes
     from tests.oracle_binding import load
-
     oracle = load()
-    boxes = uniform_boxes(BOXES, 1000.0, 0.2, 2.0, SEED)
+    boxes = uniform_boxes(BOXES, 1000.0, 0.2, 2.0, BOX_SEED)
     t0 = time.perf_counter()
     for _ in range(passes):
         oracle.broadphase(boxes)
-    return time.perf_counter() - t0
+    secs = time.perf_counter() - t0
+    return BOXES * passes / secs, secs
+
+
+# ---- report ------------------------------------------------------------------------------------------
+
+def stage_table(stages, counts, bytes_of):
+    total = max(sum(t for _, t in stages.values()), 1e-9)
+    table = {}
+    for name, (launches, total_ms) in stages.items():
+        avg_us = 1000.0 * total_ms / max(launches, 1)
+        nbytes = bytes_of(name)
+        table[name] = {"launches": launches, "avg_us": round(avg_us, 3), "share": round(total_ms / total, 4),
+                       "bytes": nbytes,
+                       "gbs": round(nbytes / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 and nbytes else 0.0}
+    dominant = max((n for n in table if table[n]["bytes"] > 0), key=lambda n: stages[n][1], default=None)
+    roofline = None
+    if dominant:
+        d = table[dominant]
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_us": d["avg_us"], "bytes_per_launch": d["bytes"],
+                    "share_of_device_time": d["share"]}
+    return table, roofline
 
 
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=50)
-    parser.add_argument("--warmup", type=int, default=10)
-    parser.add_argument("--cpu-passes", type=int, default=3,
-                        help="oracle passes timed for the CPU baseline (0 skips it)")
+    parser.add_argument("--workload", choices=["pile", "cubes", "broadphase"], default="pile")
+    parser.add_argument("--steps", type=int, default=None)
+    parser.add_argument("--warmup", type=int, default=None)
+    parser.add_argument("--profile-steps", type=int, default=5)
+    parser.add_argument("--cpu-sample", type=int, default=None,
+                        help="oracle steps (broadphase: passes) timed for the CPU baseline; 0 skips it")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(independent worlds would shard by world range; see DESIGN.md)", file=sys.stderr)
+              "(independent worlds would shard by world range)", file=sys.stderr)
         return 2
-    steps, warmup = max(args.steps, 1), max(args.warmup, 0)
-    ms, entries, pairs, stages = run_gpu(steps, warmup)
-    value = BOXES * steps / (ms / 1000.0)
-
-    table = {}
-    for name, (launches, total_ms) in stages.items():
-        avg_us = 1000.0 * total_ms / max(launches, 1)
-        nbytes = kernel_bytes(name, BOXES, entries, pairs)
-        table[name] = {"launches": launches, "avg_us": round(avg_us, 3),
-                       "share": round(total_ms / max(sum(t for _, t in stages.values()), 1e-9), 4),
-                       "bytes": nbytes,
-                       "gbs": round(nbytes / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 else 0.0}
-    dominant = max((n for n in table if table[n]["bytes"] > 0),
-                   key=lambda n: stages[n][1], default=None)
-    roofline = None
-    if dominant:
-        d = table[dominant]
-        achieved = d["gbs"]
-        roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved,
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "avg_us": d["avg_us"], "bytes_per_launch": d["bytes"]}
-
-    cpu = None
-    if args.cpu_passes > 0:
-        secs = run_cpu(args.cpu_passes)
-        cpu = {"value": BOXES * args.cpu_passes / secs, "unit": "body-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{args.cpu_passes} full passes of the same 2^20-box workload through "
-                         "the single-threaded C oracle (emission, sort, pairs, pair sort)"}
-
+    wl = args.workload
+    if wl == "broadphase":
+        steps = max(args.steps if args.steps is not None else 50, 1)
+        warmup = max(args.warmup if args.warmup is not None else 10, 0)
+        ms, bodies, stages, counts = run_broadphase(steps, warmup)
+        table, roofline = stage_table(stages, counts, lambda nm: broadphase_bytes(
+            nm, counts["k"], counts["entries"], counts["pairs"]))
+        sample = 3 if args.cpu_sample is None else args.cpu_sample
+        cpu = None
+        if sample > 0:
+            v, secs = cpu_broadphase(sample)
+            cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                   "sample": f"{sample} full passes of the same 2^20-box workload through the "
+                             f"single-threaded C oracle ({secs:.1f} s)"}
+        config = {"workload": "isolated sort + broadphase pass over 1,048,576 boxes (centres "
+                              "U[0,1000)^3 m, half-extents log-U[0.2,2] m)",
+                  "boxes": BOXES, "entries": counts["entries"], "pairs": counts["pairs"], "seed": BOX_SEED}
+        dtype = "u32"
+        measured = {}
+    else:
+        spec = WORKLOADS[wl]
+        steps = max(args.steps if args.steps is not None else spec["steps"], 1)
+        warmup = max(args.warmup if args.warmup is not None else spec["warmup"], 0)
+        ms, bodies, stages, counts, measured, scene, final = run_world(wl, steps, warmup, args.profile_steps)
+        table, roofline = stage_table(stages, counts, lambda nm: step_bytes(
+            nm, counts["n"], counts["k"], counts["entries"], counts["pairs"], counts["contacts"],
+            counts["points"], counts["rows"]))
+        sample = (8 if wl == "pile" else 100) if args.cpu_sample is None else args.cpu_sample
+        cpu = None
+        if sample > 0:
+            v, secs = cpu_world(scene, final, sample)
+            cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                   "sample": f"{sample} steps of the same scene through the single-threaded C "
+                             f"oracle, started from the device's body states at the end of the "
+                             f"timed and profiled steps ({secs:.1f} s)"}
+        config = {"workload": spec["describe"], "bodies": bodies, "steps_in_run": spec["run"],
+                  "seed": 3 if wl == "pile" else 1,
+                  "at_end": {k: measured[k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
+                                                       "RESTING", "ACTIVE", "REFUSED_PAIRS",
+                                                       "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
+        dtype = "f32"
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": bodies * steps / (ms / 1000.0),
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": steps,
@@ -161,13 +321,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": "isolated sort + broadphase pass over 1,048,576 boxes "
-                               "(centres U[0,1000)^3 m, half-extents log-U[0.2,2] m)",
-                   "boxes": BOXES, "entries": entries, "pairs": pairs, "seed": SEED,
-                   "step": "resolution + entry emission + 4-digit entry sort + pair emission + "
-                           "canonical pair sort"},
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": table,

@@ -1,0 +1,867 @@
+// Device narrowphase for pairs the analytic generators do not cover: support maps of every convex
+// kind (hulls by a linear scan of their vertices), GJK closest points, EPA penetration, the
+// direction probe fallback, and manifolds from clipped feature rings. Semantics:
+// crates/dynamis-shader/shaders/convex.wgsl:1-1071 and crates/dynamis-rigid/shaders/
+// narrowphase.wgsl:479-512. One thread handles one pair; the EPA polytope lives in the thread's
+// private memory.
+#pragma once
+
+#include "narrow.h"
+
+namespace dyn {
+
+constexpr uint32_t kFeatureMax = 8;
+constexpr float kCapAlignment = 0.93f;
+constexpr uint32_t kCylinderCapPoints = 8;
+constexpr uint32_t kCylinderSidePoints = 16;
+constexpr uint32_t kCylinderSideFace = 2;
+constexpr uint32_t kEmptyRingFace = 1u << 12;  // FEATURE_INDEX_LIMIT
+constexpr float kBig = 3.402823466e38f;
+
+// cos / sin of f32(i) * 0.78539816 for the eight cylinder cap points, correctly rounded
+__constant__ float kCapCos[8] = {0x1.0p+0f, 0x1.6a09e6p-1f, -0x1.777a5cp-25f, -0x1.6a09e6p-1f,
+                                 -0x1.0p+0f, -0x1.6a09e2p-1f, 0x1.99bc5cp-27f, 0x1.6a09eep-1f};
+__constant__ float kCapSin[8] = {0x0.0p+0f, 0x1.6a09e6p-1f, 0x1.0p+0f, 0x1.6a09e6p-1f,
+                                 -0x1.777a5cp-24f, -0x1.6a09eap-1f, -0x1.0p+0f, -0x1.6a09dep-1f};
+
+__device__ __forceinline__ v3 hull_vertex(const ShapeSet& ss, const dyn_shape_source& src, uint32_t i) {
+    const float4 v = ss.vertices[src.vertex_offset + i];
+    return mk(v.x, v.y, v.z);
+}
+
+// convex.wgsl:1-85 (hulls of at most DYN_LINEAR_SUPPORT_VERTICES vertices)
+__device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
+    const v3 n = norm3(d);
+    if (w.kind == DYN_SHAPE_SPHERE) return mul(n, w.radius);
+    if (w.kind == DYN_SHAPE_CUBOID) {
+        const v3 s = mk(d.x > 0.0f ? 1.0f : -1.0f, d.y > 0.0f ? 1.0f : -1.0f, d.z > 0.0f ? 1.0f : -1.0f);
+        return mulv(s, w.half_extents);
+    }
+    if (w.kind == DYN_SHAPE_CAPSULE) {
+        const v3 tip = mul(mul(mk(0.0f, 1.0f, 0.0f), w.half_height), d.y > 0.0f ? 1.0f : -1.0f);
+        return add(tip, mul(n, w.radius));
+    }
+    if (w.kind == DYN_SHAPE_CYLINDER) {
+        const v3 base = mul(mul(mk(0.0f, 1.0f, 0.0f), w.half_height), d.y > 0.0f ? 1.0f : -1.0f);
+        const v3 radial = sub(d, mk(0.0f, d.y, 0.0f));
+        if (len3(radial) < 1e-6f) return base;
+        return add(base, mul(norm3(radial), w.radius));
+    }
+    if (w.kind != DYN_SHAPE_HULL) return zero3v();
+    const dyn_shape_source src = ss.sources[w.source];
+    float best_dot = -kBig;
+    v3 best = zero3v();
+    for (uint32_t i = 0; i < src.vertex_count; ++i) {
+        const v3 v = hull_vertex(ss, src, i);
+        const float s = dot3(v, d);
+        if (s > best_dot) {
+            best_dot = s;
+            best = v;
+        }
+    }
+    return best;
+}
+
+// convex.wgsl:86-91
+__device__ inline v3 support(const Shape& w, const ShapeSet& ss, v3 direction) {
+    const v3 sd = mulv(qrot(qconj(w.rotation), direction), w.scale);
+    return add(w.center, qrot(w.rotation, mulv(support_local(w, ss, sd), w.scale)));
+}
+
+// ---- GJK (convex.wgsl:93-282, 439-521) -------------------------------------------------------
+
+struct SimplexPoint {
+    v3 w, a, b;
+};
+
+struct SimplexResult {
+    v3 v;
+    float lambdas[4];
+    bool penetrating;
+    v3 point_a, point_b;
+};
+
+struct Closest {
+    float distance;
+    v3 point_a, point_b, normal;
+    bool penetrating;
+};
+
+struct Hit {
+    float distance;
+    v3 point, normal;
+};
+
+struct Epa {
+    bool valid;
+    v3 normal;
+    float depth;
+    v3 point;
+};
+
+__device__ inline v3 closest_on_triangle(v3 o, v3 a, v3 b, v3 c, v3* weights) {
+    const v3 ab = sub(b, a), ac = sub(c, a), ap = sub(o, a);
+    const float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        *weights = mk(1.0f, 0.0f, 0.0f);
+        return a;
+    }
+    const v3 bp = sub(o, b);
+    const float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
+        *weights = mk(0.0f, 1.0f, 0.0f);
+        return b;
+    }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float t = d1 / (d1 - d3);
+        *weights = mk(1.0f - t, t, 0.0f);
+        return add(a, mul(ab, t));
+    }
+    const v3 cp = sub(o, c);
+    const float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
+        *weights = mk(0.0f, 0.0f, 1.0f);
+        return c;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float t = d2 / (d2 - d6);
+        *weights = mk(1.0f - t, 0.0f, t);
+        return add(a, mul(ac, t));
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *weights = mk(0.0f, 1.0f - t, t);
+        return add(b, mul(sub(c, b), t));
+    }
+    const float denom = 1.0f / (va + vb + vc);
+    const float v = vb * denom, wt = vc * denom;
+    *weights = mk(1.0f - v - wt, v, wt);
+    return add(add(a, mul(ab, v)), mul(ac, wt));
+}
+
+__device__ __forceinline__ void tet_face(uint32_t f, uint32_t* a, uint32_t* b, uint32_t* c) {
+    // the four faces of a tetrahedron in the reference's order
+    *a = f == 3u ? 1u : 0u;
+    *b = f == 0u ? 1u : (f == 1u ? 2u : 3u);
+    *c = f == 0u ? 2u : (f == 1u ? 3u : (f == 2u ? 1u : 2u));
+}
+
+__device__ inline bool tetra_contains(const SimplexPoint* t, v3 o) {
+    for (uint32_t f = 0; f < 4; ++f) {
+        uint32_t ia, ib, ic;
+        tet_face(f, &ia, &ib, &ic);
+        const v3 a = t[ia].w, b = t[ib].w, c = t[ic].w;
+        const v3 normal = cross3(sub(b, a), sub(c, a));
+        const float span = fmx(fmx(len3(sub(b, a)), len3(sub(c, a))), len3(sub(c, b)));
+        if (len3(normal) <= 1e-6f * span * span) return false;
+        if (dot3(normal, sub(o, a)) > 0.0f) return false;
+    }
+    return true;
+}
+
+__device__ inline bool simplex_holds(const SimplexPoint* s, uint32_t count, v3 point, float span) {
+    const float tolerance = (1e-5f * span) * (1e-5f * span);
+    for (uint32_t i = 0; i < count; ++i) {
+        const v3 offset = sub(s[i].w, point);
+        if (dot3(offset, offset) <= tolerance) return true;
+    }
+    return false;
+}
+
+__device__ inline uint32_t simplex_reduce(SimplexPoint* s, const float* lambdas) {
+    SimplexPoint kept[4];
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (lambdas[i] > 1e-6f) kept[count++] = s[i];
+    }
+    if (count == 0u) {
+        kept[0] = s[0];
+        count = 1;
+    }
+    for (uint32_t i = 0; i < count; ++i) s[i] = kept[i];
+    return count;
+}
+
+__device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t count) {
+    SimplexResult r;
+    r.v = mk(kBig, kBig, kBig);
+    for (int i = 0; i < 4; ++i) r.lambdas[i] = 0.0f;
+    r.penetrating = count == 4u && tetra_contains(s, zero3v());
+    r.point_a = zero3v();
+    r.point_b = zero3v();
+    if (r.penetrating) {
+        r.v = zero3v();
+        return r;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        if (len3(s[i].w) < len3(r.v)) {
+            r.v = s[i].w;
+            for (int k = 0; k < 4; ++k) r.lambdas[k] = 0.0f;
+            r.lambdas[i] = 1.0f;
+        }
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        for (uint32_t j = i + 1; j < count; ++j) {
+            const v3 a = s[i].w, b = s[j].w;
+            const v3 ab = sub(b, a);
+            const float denom = dot3(ab, ab);
+            if (denom < 1e-12f) continue;
+            const float t = clampf(-dot3(a, ab) / denom, 0.0f, 1.0f);
+            const v3 point = add(a, mul(ab, t));
+            if (len3(point) < len3(r.v)) {
+                r.v = point;
+                for (int k = 0; k < 4; ++k) r.lambdas[k] = 0.0f;
+                r.lambdas[i] = 1.0f - t;
+                r.lambdas[j] = t;
+            }
+        }
+    }
+    if (count >= 3u) {
+        for (uint32_t i = 0; i < count; ++i) {
+            for (uint32_t j = i + 1; j < count; ++j) {
+                for (uint32_t k = j + 1; k < count; ++k) {
+                    v3 weights;
+                    const v3 point = closest_on_triangle(zero3v(), s[i].w, s[j].w, s[k].w, &weights);
+                    if (len3(point) < len3(r.v)) {
+                        r.v = point;
+                        for (int q = 0; q < 4; ++q) r.lambdas[q] = 0.0f;
+                        r.lambdas[i] = weights.x;
+                        r.lambdas[j] = weights.y;
+                        r.lambdas[k] = weights.z;
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        r.point_a = add(r.point_a, smul(r.lambdas[i], s[i].a));
+        r.point_b = add(r.point_b, smul(r.lambdas[i], s[i].b));
+    }
+    return r;
+}
+
+__device__ inline SimplexPoint support_pair(const Shape& a, const Shape& b, const ShapeSet& ss, v3 dir) {
+    SimplexPoint p;
+    p.a = support(a, ss, dir);
+    p.b = support(b, ss, neg(dir));
+    p.w = sub(p.a, p.b);
+    return p;
+}
+
+__device__ __forceinline__ float shape_scale(const Shape& w) {
+    float scale = w.radius + w.half_height;
+    scale = fmx(scale, fmx(fmx(w.half_extents.x, w.half_extents.y), w.half_extents.z));
+    return fmx(scale, 1e-4f) * fmx(fmx(w.scale.x, w.scale.y), w.scale.z);
+}
+
+__device__ inline v3 simplex_expand_dir(const SimplexPoint* s, uint32_t count) {
+    if (count == 1u) return mk(1.0f, 0.0f, 0.0f);
+    if (count == 2u) {
+        const v3 edge = sub(s[1].w, s[0].w);
+        v3 d = cross3(edge, mk(1.0f, 0.0f, 0.0f));
+        if (len3(d) < 1e-6f) d = cross3(edge, mk(0.0f, 1.0f, 0.0f));
+        return norm3(d);
+    }
+    const v3 normal = cross3(sub(s[1].w, s[0].w), sub(s[2].w, s[0].w));
+    if (len3(normal) > 1e-8f) return norm3(normal);
+    return mk(0.0f, 1.0f, 0.0f);
+}
+
+__device__ inline Closest convex_closest(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                         SimplexPoint* simplex, uint32_t* out_count) {
+    uint32_t count = 0;
+    Closest r;
+    r.distance = 0.0f;
+    r.point_a = first.center;
+    r.point_b = second.center;
+    r.normal = sign_normalize(sub(second.center, first.center));
+    r.penetrating = false;
+    v3 dir = sub(second.center, first.center);
+    if (len3(dir) < 1e-8f) dir = mk(1.0f, 0.0f, 0.0f);
+    simplex[0] = support_pair(first, second, ss, dir);
+    count = 1;
+    SimplexResult closest = simplex_closest(simplex, count);
+    const float tiny = 1e-4f * fmn(shape_scale(first), shape_scale(second));
+    for (uint32_t iter = 0; iter < 24u; ++iter) {
+        if (closest.penetrating || len3(closest.v) < tiny) {
+            while (count < 4u) {
+                const v3 d = simplex_expand_dir(simplex, count);
+                simplex[count] = support_pair(first, second, ss, d);
+                ++count;
+            }
+            r.penetrating = true;
+            r.distance = 0.0f;
+            r.point_a = closest.point_a;
+            r.point_b = closest.point_b;
+            break;
+        }
+        const v3 nv = neg(closest.v);
+        const SimplexPoint sp = support_pair(first, second, ss, nv);
+        if (dot3(sp.w, nv) <= dot3(closest.v, nv) + 1e-4f * fmx(1.0f, dot3(closest.v, closest.v))) {
+            r.distance = len3(closest.v);
+            r.point_a = closest.point_a;
+            r.point_b = closest.point_b;
+            if (r.distance > 1e-6f) r.normal = sign_normalize(sub(closest.point_b, closest.point_a));
+            break;
+        }
+        count = simplex_reduce(simplex, closest.lambdas);
+        if (simplex_holds(simplex, count, sp.w, shape_scale(second))) {
+            r.distance = len3(closest.v);
+            r.point_a = closest.point_a;
+            r.point_b = closest.point_b;
+            if (r.distance > 1e-6f) r.normal = sign_normalize(sub(closest.point_b, closest.point_a));
+            break;
+        }
+        simplex[count] = sp;
+        ++count;
+        closest = simplex_closest(simplex, count);
+    }
+    *out_count = count;
+    return r;
+}
+
+// ---- EPA (convex.wgsl:286-437) ---------------------------------------------------------------
+
+struct EpaFace {
+    uint32_t a, b, c;
+    v3 normal;
+};
+
+__device__ __forceinline__ Epa epa_invalid() {
+    Epa r;
+    r.valid = false;
+    r.normal = mk(0.0f, 1.0f, 0.0f);
+    r.depth = 0.0f;
+    r.point = zero3v();
+    return r;
+}
+
+__device__ __forceinline__ EpaFace epa_face(uint32_t a, uint32_t b, uint32_t c, v3 unit, v3 centroid) {
+    EpaFace f;
+    f.a = a;
+    f.b = b;
+    f.c = c;
+    f.normal = dot3(unit, centroid) > 0.0f ? neg(unit) : unit;
+    return f;
+}
+
+__device__ __noinline__ Epa epa_tetrahedron(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                            const SimplexPoint* simplex, uint32_t count) {
+    SimplexPoint polytope[64];
+    EpaFace new_faces[64];
+    EpaFace faces[128];
+    uint32_t visible[128];
+    uint32_t vertex_count = 0;
+    for (uint32_t i = 0; i < count; ++i) polytope[vertex_count++] = simplex[i];
+    uint32_t face_count = 0;
+    if (vertex_count != 4u) return epa_invalid();
+    for (uint32_t i = 0; i < 4; ++i) {
+        uint32_t a, b, c;
+        tet_face(i, &a, &b, &c);
+        const v3 normal = cross3(sub(polytope[b].w, polytope[a].w), sub(polytope[c].w, polytope[a].w));
+        const float length_n = len3(normal);
+        if (length_n < 1e-10f) return epa_invalid();
+        const v3 unit = dv(normal, length_n);
+        const v3 centroid = mul(add(add(polytope[a].w, polytope[b].w), polytope[c].w), 1.0f / 3.0f);
+        faces[face_count++] = epa_face(a, b, c, unit, centroid);
+    }
+    float best_dist = 0.0f;
+    uint32_t best_face = 0;
+    for (uint32_t iter = 0; iter < 32u; ++iter) {
+        best_dist = kBig;
+        best_face = 0;
+        for (uint32_t i = 0; i < face_count; ++i) {
+            const float dist = dot3(faces[i].normal, polytope[faces[i].a].w);
+            if (dist < best_dist && len3(faces[i].normal) > 0.5f) {
+                best_dist = dist;
+                best_face = i;
+            }
+        }
+        const v3 bn = faces[best_face].normal;
+        const v3 probe = support_pair(first, second, ss, bn).w;
+        const float gain = dot3(probe, bn) - best_dist;
+        if (gain < 1e-4f) break;
+        if (vertex_count >= 64u) break;
+        polytope[vertex_count].w = probe;
+        polytope[vertex_count].a = support(first, ss, bn);
+        polytope[vertex_count].b = support(second, ss, neg(bn));
+        ++vertex_count;
+        uint32_t visible_count = 0;
+        for (uint32_t i = 0; i < face_count; ++i) {
+            if (dot3(faces[i].normal, sub(probe, polytope[faces[i].a].w)) > 1e-6f) visible[visible_count++] = i;
+        }
+        uint32_t new_count = 0;
+        for (uint32_t i = 0; i < visible_count; ++i) {
+            const EpaFace face = faces[visible[i]];
+            for (uint32_t e = 0; e < 3; ++e) {
+                const uint32_t start = e == 0u ? face.a : (e == 1u ? face.b : face.c);
+                const uint32_t end = e == 0u ? face.b : (e == 1u ? face.c : face.a);
+                bool shared = false;
+                for (uint32_t j = 0; j < visible_count; ++j) {
+                    if (j == i) continue;
+                    const EpaFace& o = faces[visible[j]];
+                    if ((o.a == end && o.b == start) || (o.b == end && o.c == start) || (o.c == end && o.a == start)) {
+                        shared = true;
+                    }
+                }
+                if (!shared && new_count < 64u) {
+                    const uint32_t apex = vertex_count - 1u;
+                    const v3 normal = cross3(sub(polytope[end].w, polytope[apex].w),
+                                             sub(polytope[start].w, polytope[apex].w));
+                    const float length_n = len3(normal);
+                    if (length_n > 1e-10f) {
+                        const v3 unit = dv(normal, length_n);
+                        const v3 centroid =
+                            mul(add(add(polytope[apex].w, polytope[start].w), polytope[end].w), 1.0f / 3.0f);
+                        new_faces[new_count++] = epa_face(apex, start, end, unit, centroid);
+                    }
+                }
+            }
+        }
+        uint32_t write = 0, seen = 0;
+        for (uint32_t i = 0; i < face_count; ++i) {
+            if (seen < visible_count && visible[seen] == i) {
+                ++seen;
+                continue;
+            }
+            faces[write++] = faces[i];
+        }
+        face_count = write;
+        if (face_count + new_count > 128u) break;
+        for (uint32_t i = 0; i < new_count; ++i) faces[face_count++] = new_faces[i];
+    }
+    best_dist = kBig;
+    for (uint32_t i = 0; i < face_count; ++i) {
+        const float dist = dot3(faces[i].normal, polytope[faces[i].a].w);
+        if (dist < best_dist && len3(faces[i].normal) > 0.5f) {
+            best_dist = dist;
+            best_face = i;
+        }
+    }
+    Epa r;
+    r.valid = best_dist < kBig;
+    r.normal = faces[best_face].normal;
+    r.depth = best_dist;
+    r.point = polytope[faces[best_face].a].a;
+    return r;
+}
+
+// ---- penetration probe and convex_hit (convex.wgsl:523-618) ----------------------------------
+
+__device__ __forceinline__ float support_projection_depth(const Shape& a, const Shape& b, const ShapeSet& ss, v3 dir) {
+    const v3 n = norm3(dir);
+    return dot3(sub(support(a, ss, n), support(b, ss, neg(n))), n);
+}
+
+__device__ inline Epa penetration_probe(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                        const SimplexPoint* s, uint32_t count) {
+    Epa best;
+    best.valid = false;
+    best.normal = mk(0.0f, 1.0f, 0.0f);
+    best.depth = kBig;
+    best.point = mul(add(first.center, second.center), 0.5f);
+    v3 probed[16];
+    uint32_t probe_count = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        for (uint32_t j = i + 1; j < count; ++j) {
+            for (uint32_t k = j + 1; k < count; ++k) {
+                const v3 normal = cross3(sub(s[j].w, s[i].w), sub(s[k].w, s[i].w));
+                if (len3(normal) > 1e-8f && probe_count < 16u) probed[probe_count++] = normal;
+            }
+        }
+    }
+    const v3 center_dir = sub(second.center, first.center);
+    if (len3(center_dir) > 1e-8f && probe_count < 16u) probed[probe_count++] = center_dir;
+    for (uint32_t i = 0; i < 6; ++i) {
+        const float sgn = (i & 1u) ? -1.0f : 1.0f;
+        const v3 axis = i < 2u ? mk(sgn, 0.0f, 0.0f) : (i < 4u ? mk(0.0f, sgn, 0.0f) : mk(0.0f, 0.0f, sgn));
+        if (probe_count < 16u) probed[probe_count++] = axis;
+    }
+    for (uint32_t i = 0; i < probe_count; ++i) {
+        const float depth = support_projection_depth(first, second, ss, probed[i]);
+        if (depth < best.depth) {
+            best.depth = depth;
+            best.normal = norm3(probed[i]);
+        }
+    }
+    best.valid = best.depth < kBig;
+    best.point = mul(add(support(first, ss, best.normal), support(second, ss, neg(best.normal))), 0.5f);
+    return best;
+}
+
+__device__ inline Hit make_hit(float distance, v3 point, v3 normal) {
+    Hit h;
+    h.distance = distance;
+    h.point = point;
+    h.normal = normal;
+    return h;
+}
+
+__device__ inline Hit convex_hit(const Shape& first, const Shape& second, const ShapeSet& ss) {
+    SimplexPoint simplex[4];
+    uint32_t count = 0;
+    const Closest closest = convex_closest(first, second, ss, simplex, &count);
+    const v3 centers = sub(second.center, first.center);
+    if (!closest.penetrating) {
+        const Epa probe = penetration_probe(first, second, ss, simplex, count);
+        if (probe.depth > 0.0f) {
+            const v3 normal = dot3(probe.normal, centers) < 0.0f ? neg(probe.normal) : probe.normal;
+            return make_hit(-probe.depth, mul(add(closest.point_a, closest.point_b), 0.5f), normal);
+        }
+        if (closest.distance > 0.0f) {
+            return make_hit(closest.distance, mul(add(closest.point_a, closest.point_b), 0.5f), closest.normal);
+        }
+        return make_hit(kBig, zero3v(), zero3v());  // NO_HIT
+    }
+    const Epa epa = epa_tetrahedron(first, second, ss, simplex, count);
+    Hit result;
+    if (epa.valid && epa.depth > 0.0f && epa.depth < kBig && len3(epa.normal) > 0.5f &&
+        epa.depth < support_projection_depth(first, second, ss, epa.normal) +
+                        0.1f * fmn(shape_scale(first), shape_scale(second))) {
+        result = make_hit(-epa.depth,
+                          mul(add(support(first, ss, epa.normal), support(second, ss, neg(epa.normal))), 0.5f),
+                          epa.normal);
+    } else {
+        const Epa probe = penetration_probe(first, second, ss, simplex, count);
+        result = make_hit(-probe.depth,
+                          mul(add(support(first, ss, probe.normal), support(second, ss, neg(probe.normal))), 0.5f),
+                          probe.normal);
+    }
+    if (dot3(result.normal, centers) < 0.0f) result.normal = neg(result.normal);
+    return result;
+}
+
+// ---- feature rings (convex.wgsl:625-902) -----------------------------------------------------
+
+struct Feature {
+    v3 p[kFeatureMax];
+    uint32_t id[kFeatureMax];
+    uint32_t count;
+    uint32_t face;
+    bool ring;
+};
+
+__device__ inline void box_feature_ring(const Shape& w, v3 direction, Feature* f) {
+    const v3 axes[3] = {qrot(w.rotation, mk(1.0f, 0.0f, 0.0f)), qrot(w.rotation, mk(0.0f, 1.0f, 0.0f)),
+                        qrot(w.rotation, mk(0.0f, 0.0f, 1.0f))};
+    uint32_t best = 0;
+    for (uint32_t i = 1; i < 3; ++i) {
+        if (fabsf(dot3(axes[i], direction)) > fabsf(dot3(axes[best], direction))) best = i;
+    }
+    const float sign = dot3(axes[best], direction) < 0.0f ? -1.0f : 1.0f;
+    const v3 face = mul(axes[best], sign);
+    const v3 center = add(w.center, mul(face, comp(w.half_extents, best)));
+    const v3 u1 = axes[(best + 1u) % 3u], u2 = axes[(best + 2u) % 3u];
+    const float e1 = comp(w.half_extents, (best + 1u) % 3u);
+    const float e2 = comp(w.half_extents, (best + 2u) % 3u);
+    f->p[0] = add(add(center, mul(u1, e1)), mul(u2, e2));
+    f->id[0] = box_corner_id(best, sign, 1.0f, 1.0f);
+    f->p[1] = add(sub(center, mul(u1, e1)), mul(u2, e2));
+    f->id[1] = box_corner_id(best, sign, -1.0f, 1.0f);
+    f->p[2] = sub(sub(center, mul(u1, e1)), mul(u2, e2));
+    f->id[2] = box_corner_id(best, sign, -1.0f, -1.0f);
+    f->p[3] = sub(add(center, mul(u1, e1)), mul(u2, e2));
+    f->id[3] = box_corner_id(best, sign, 1.0f, -1.0f);
+    f->face = kFeatureFaceBit | (best * 2u + (sign > 0.0f ? 1u : 0u));
+    f->count = 4;
+}
+
+// The order of atan2(y, x) of two points decided exactly from the half plane and the sign of a
+// cross product (ring_of's angular sort, convex.wgsl:696-711). Groups in increasing angle:
+// -pi, (-pi, 0), [0, pi), pi.
+__device__ __forceinline__ int angle_group(float x, float y) {
+    if (y == 0.0f && x < 0.0f) return signbit(y) ? 0 : 3;
+    if (y < 0.0f) return 1;
+    return 2;
+}
+
+__device__ __forceinline__ bool angle_before(float bx, float by, float ax, float ay) {
+    if (bx == 0.0f && by == 0.0f) bx = 1.0f;
+    if (ax == 0.0f && ay == 0.0f) ax = 1.0f;
+    const int gb = angle_group(bx, by), ga = angle_group(ax, ay);
+    if (gb != ga) return gb < ga;
+    if (gb == 0 || gb == 3) return false;
+    return bx * ay - by * ax > 0.0f;
+}
+
+// convex.wgsl:672-721 over a ring of up to 16 gathered points
+__device__ inline void ring_of(const Shape& w, const ShapeSet& ss, v3 direction, v3* points, uint32_t* ids,
+                               uint32_t count, Feature* f) {
+    if (count <= 1u) {
+        f->p[0] = support(w, ss, direction);
+        f->id[0] = ids[0];
+        f->count = 1;
+        return;
+    }
+    const v3 n = sign_normalize(direction);
+    v3 center = zero3v();
+    for (uint32_t i = 0; i < count; ++i) center = add(center, points[i]);
+    center = dv(center, static_cast<float>(count));
+    const v3 u = fabsf(n.x) > 0.9f ? norm3(cross3(n, mk(0.0f, 1.0f, 0.0f))) : norm3(cross3(n, mk(1.0f, 0.0f, 0.0f)));
+    const v3 v = cross3(n, u);
+    for (uint32_t i = 0; i < count; ++i) {
+        for (uint32_t j = i + 1; j < count; ++j) {
+            const v3 pa = sub(points[i], center), pb = sub(points[j], center);
+            if (angle_before(dot3(pb, u), dot3(pb, v), dot3(pa, u), dot3(pa, v))) {
+                const v3 tp = points[i];
+                points[i] = points[j];
+                points[j] = tp;
+                const uint32_t ti = ids[i];
+                ids[i] = ids[j];
+                ids[j] = ti;
+            }
+        }
+    }
+    const uint32_t keep = count < kFeatureMax ? count : kFeatureMax;
+    for (uint32_t i = 0; i < keep; ++i) {
+        f->p[i] = points[i];
+        f->id[i] = ids[i];
+    }
+    f->count = keep;
+}
+
+// convex.wgsl:723-793, the linear path
+__device__ inline void hull_feature_ring(const Shape& w, const ShapeSet& ss, v3 direction, Feature* f) {
+    const dyn_shape_source src = ss.sources[w.source];
+    const v3 d = mulv(qrot(qconj(w.rotation), direction), w.scale);
+    const float best_dot = dot3(support_local(w, ss, d), d);
+    const v3 extent = sub(ld3(src.local_max), ld3(src.local_min));
+    const float eps = 1e-3f * fmx(fmx(extent.x, extent.y), extent.z) * fmx(fmx(w.scale.x, w.scale.y), w.scale.z);
+    v3 points[16];
+    uint32_t ids[16];
+    ids[0] = 0u;
+    uint32_t count = 0, first_id = kEmptyRingFace;
+    for (uint32_t i = 0; i < src.vertex_count && count < 16u; ++i) {
+        const v3 v = hull_vertex(ss, src, i);
+        if (dot3(v, d) >= best_dot - eps) {
+            points[count] = add(w.center, qrot(w.rotation, mulv(v, w.scale)));
+            ids[count] = i;
+            first_id = first_id < i ? first_id : i;
+            ++count;
+        }
+    }
+    f->face = kFeatureFaceBit | first_id;
+    ring_of(w, ss, direction, points, ids, count, f);
+}
+
+// convex.wgsl:795-819
+__device__ inline void capsule_feature(const Shape& w, v3 direction, Feature* f) {
+    const v3 ld = mulv(qrot(qconj(w.rotation), direction), w.scale);
+    const float cap = ld.y > 0.0f ? 1.0f : -1.0f;
+    const v3 radial = mul(sign_normalize(ld), w.radius);
+    f->face = kFeatureFaceBit | 0u;
+    if (fabsf(ld.y) > kCapAlignment * len3(ld)) {
+        const v3 local = add(mk(0.0f, cap * w.half_height, 0.0f), radial);
+        f->p[0] = add(w.center, qrot(w.rotation, mulv(local, w.scale)));
+        f->id[0] = cap > 0.0f ? 1u : 0u;
+        f->count = 1;
+        return;
+    }
+    const v3 upper = add(mk(0.0f, w.half_height, 0.0f), radial);
+    const v3 lower = add(mk(0.0f, -w.half_height, 0.0f), radial);
+    f->p[0] = add(w.center, qrot(w.rotation, mulv(upper, w.scale)));
+    f->p[1] = add(w.center, qrot(w.rotation, mulv(lower, w.scale)));
+    f->id[0] = 0u;
+    f->id[1] = 1u;
+    f->count = 2;
+}
+
+// convex.wgsl:821-854
+__device__ inline void cylinder_feature(const Shape& w, v3 direction, Feature* f) {
+    const v3 axis = qrot(w.rotation, mk(0.0f, 1.0f, 0.0f));
+    const float alignment = dot3(axis, norm3(direction));
+    if (fabsf(alignment) > kCapAlignment) {
+        const float sign = alignment < 0.0f ? -1.0f : 1.0f;
+        const uint32_t cap = sign > 0.0f ? 1u : 0u;
+        const v3 cap_center = add(w.center, mul(mul(axis, w.half_height), sign));
+        v3 u = norm3(cross3(axis, mk(1.0f, 0.0f, 0.0f)));
+        if (len3(cross3(axis, mk(1.0f, 0.0f, 0.0f))) < 1e-6f) u = norm3(cross3(axis, mk(0.0f, 0.0f, 1.0f)));
+        const v3 v = cross3(axis, u);
+        for (uint32_t i = 0; i < kCylinderCapPoints; ++i) {
+            f->p[i] = add(cap_center, mul(add(mul(u, kCapCos[i]), mul(v, kCapSin[i])), w.radius));
+            f->id[i] = cap * kCylinderCapPoints + i;
+        }
+        f->face = kFeatureFaceBit | cap;
+        f->count = kCylinderCapPoints;
+        return;
+    }
+    const v3 side = sign_normalize(sub(direction, mul(axis, dot3(direction, axis))));
+    f->p[0] = add(add(w.center, mul(axis, w.half_height)), mul(side, w.radius));
+    f->id[0] = kCylinderSidePoints;
+    f->p[1] = add(sub(w.center, mul(axis, w.half_height)), mul(side, w.radius));
+    f->id[1] = kCylinderSidePoints + 1u;
+    f->face = kFeatureFaceBit | kCylinderSideFace;
+    f->count = 2;
+}
+
+// convex.wgsl:856-889
+__device__ inline void shape_feature(const Shape& w, const ShapeSet& ss, v3 direction, Feature* f) {
+    if (w.kind == DYN_SHAPE_CUBOID) {
+        box_feature_ring(w, direction, f);
+        f->ring = true;
+    } else if (w.kind == DYN_SHAPE_HULL) {
+        hull_feature_ring(w, ss, direction, f);
+        f->ring = f->count > 2u;
+    } else if (w.kind == DYN_SHAPE_CAPSULE) {
+        capsule_feature(w, direction, f);
+        f->ring = false;
+    } else if (w.kind == DYN_SHAPE_CYLINDER) {
+        cylinder_feature(w, direction, f);
+        f->ring = f->count > 2u;
+    } else {
+        f->p[0] = support(w, ss, direction);
+        f->id[0] = 0u;
+        f->face = kFeatureFaceBit | 0u;
+        f->count = 1;
+        f->ring = false;
+    }
+}
+
+__device__ inline float feature_radius(const Feature& f) {
+    v3 center = zero3v();
+    for (uint32_t i = 0; i < f.count; ++i) center = add(center, f.p[i]);
+    center = dv(center, static_cast<float>(f.count));
+    float radius = 0.0f;
+    for (uint32_t i = 0; i < f.count; ++i) radius = fmx(radius, len3(sub(f.p[i], center)));
+    return radius;
+}
+
+// convex.wgsl:910-982: clip a feature against the side planes of a reference ring
+__device__ inline uint32_t clip_feature_to_ring(const Feature& inc, const Feature& ref, v3 ref_dir, v3* held,
+                                                uint32_t* held_ids) {
+    v3 kept[kCandidates];
+    uint32_t kept_ids[kCandidates];
+    for (uint32_t i = 0; i < inc.count; ++i) {
+        held[i] = inc.p[i];
+        held_ids[i] = inc.id[i];
+    }
+    uint32_t live = inc.count;
+    for (uint32_t side = 0; side < ref.count; ++side) {
+        const v3 current = ref.p[side], next = ref.p[(side + 1u) % ref.count];
+        const v3 plane_normal = norm3(cross3(sub(next, current), ref_dir));
+        uint32_t written = 0;
+        if (live == 2u) {
+            const v3 p0 = held[0], p1 = held[1];
+            const float s0 = dot3(sub(p0, current), plane_normal) - kClipMargin;
+            const float s1 = dot3(sub(p1, current), plane_normal) - kClipMargin;
+            if (s0 <= 0.0f) {
+                kept[written] = p0;
+                kept_ids[written] = held_ids[0];
+                ++written;
+            }
+            if (s0 * s1 < 0.0f && written < kCandidates) {
+                kept[written] = add(p0, mul(sub(p1, p0), s0 / (s0 - s1)));
+                kept_ids[written] = kFeatureClip | (side * kCandidates);
+                ++written;
+            }
+            if (s1 <= 0.0f && written < kCandidates) {
+                kept[written] = p1;
+                kept_ids[written] = held_ids[1];
+                ++written;
+            }
+        } else {
+            for (uint32_t i = 0; i < live; ++i) {
+                const v3 cp = held[i], np = held[(i + 1u) % live];
+                const float cs = dot3(sub(cp, current), plane_normal) - kClipMargin;
+                const float ns = dot3(sub(np, current), plane_normal) - kClipMargin;
+                if (cs <= 0.0f && written < kCandidates) {
+                    kept[written] = cp;
+                    kept_ids[written] = held_ids[i];
+                    ++written;
+                }
+                if (cs * ns < 0.0f && written < kCandidates) {
+                    kept[written] = add(cp, mul(sub(np, cp), cs / (cs - ns)));
+                    kept_ids[written] = kFeatureClip | (side * kCandidates + i);
+                    ++written;
+                }
+            }
+        }
+        live = written;
+        for (uint32_t i = 0; i < written; ++i) {
+            held[i] = kept[i];
+            held_ids[i] = kept_ids[i];
+        }
+        if (live == 0u) break;
+    }
+    return live;
+}
+
+// convex.wgsl:984-1071
+__device__ inline bool convex_pair_manifold(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                            v3 direction, Manifold& m) {
+    Feature fa, fb;
+    shape_feature(first, ss, direction, &fa);
+    shape_feature(second, ss, neg(direction), &fb);
+    if (!fa.ring && !fb.ring) return false;
+    bool first_reference = fa.ring;
+    if (fa.ring && fb.ring) first_reference = feature_radius(fa) >= feature_radius(fb);
+    const Feature& ref = first_reference ? fa : fb;
+    const Feature& inc = first_reference ? fb : fa;
+    const v3 ref_dir = first_reference ? direction : neg(direction);
+    v3 center = zero3v();
+    for (uint32_t i = 0; i < ref.count; ++i) center = add(center, ref.p[i]);
+    center = dv(center, static_cast<float>(ref.count));
+    v3 deepest = zero3v();
+    float deepest_depth = -1e30f;
+    for (uint32_t i = 0; i < inc.count; ++i) {
+        const float depth = dot3(sub(center, inc.p[i]), ref_dir);
+        if (depth > deepest_depth) {
+            deepest_depth = depth;
+            deepest = inc.p[i];
+        }
+    }
+    v3 clipped[kCandidates];
+    uint32_t clipped_ids[kCandidates];
+    const uint32_t clipped_count = clip_feature_to_ring(inc, ref, ref_dir, clipped, clipped_ids);
+    Candidates cand;
+    cand.count = 0;
+    for (uint32_t i = 0; i < clipped_count && cand.count < kCandidates; ++i) {
+        const float depth = dot3(sub(center, clipped[i]), ref_dir);
+        if (depth > 0.0f) {
+            cand.pos[cand.count] = sub(clipped[i], mul(ref_dir, depth * 0.5f));
+            cand.depth[cand.count] = depth;
+            cand.feature[cand.count] = first_reference ? feature_pair(kFeatureVertex, ref.face, clipped_ids[i])
+                                                       : feature_pair(kFeatureVertex, clipped_ids[i], ref.face);
+            ++cand.count;
+        }
+    }
+    if (cand.count == 0u) {
+        if (deepest_depth <= 0.0f) return false;
+        push(m, sub(deepest, mul(ref_dir, deepest_depth * 0.5f)), deepest_depth, kFeaturePoint);
+        return true;
+    }
+    keep(m, cand);
+    return true;
+}
+
+// narrowphase.wgsl:479-483
+__device__ __forceinline__ void manifold_from_hit(Manifold& m, const Hit& hit, float margin) {
+    if (hit.distance <= margin) push(m, hit.point, -hit.distance, kFeaturePoint);
+}
+
+// narrowphase.wgsl:485-512
+__device__ inline void plane_convex(Manifold& m, const Shape& plane, const Shape& convex, const ShapeSet& ss,
+                                    float margin) {
+    const v3 n = qrot(plane.rotation, mk(0.0f, 1.0f, 0.0f));
+    const float center_side = dot3(sub(convex.center, plane.center), n);
+    const v3 facing = center_side < 0.0f ? neg(n) : n;
+    emit(m, facing);
+    Feature f;
+    shape_feature(convex, ss, neg(facing), &f);
+    Candidates cand;
+    cand.count = 0;
+    for (uint32_t i = 0; i < f.count; ++i) {
+        const float depth = dot3(sub(plane.center, f.p[i]), facing);
+        if (depth <= -margin) continue;
+        cand.pos[cand.count] = add(f.p[i], mul(facing, depth * 0.5f));
+        cand.depth[cand.count] = depth;
+        cand.feature[cand.count] = feature_pair(kFeatureVertex, kFeatureFaceBit | 0u, f.id[i]);
+        ++cand.count;
+    }
+    keep(m, cand);
+}
+
+}  // namespace dyn

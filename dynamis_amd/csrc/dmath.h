@@ -56,13 +56,19 @@ __device__ __forceinline__ q4 qnorm(q4 q) {
     const float n = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     return make_float4(q.x / n, q.y / n, q.z / n, q.w / n);
 }
+// Sine and cosine of an FP32 argument, correctly rounded: evaluated in FP64 and rounded once.
+// (WGSL leaves their precision to the implementation; this choice makes every build of the
+// step agree bit for bit.)
+__device__ __forceinline__ float sin_rn(float x) { return static_cast<float>(sin(static_cast<double>(x))); }
+__device__ __forceinline__ float cos_rn(float x) { return static_cast<float>(cos(static_cast<double>(x))); }
+
 // core.wgsl:170-178
 __device__ __forceinline__ v3 rotate_about(v3 rotation, v3 v) {
     const float angle = len3(rotation);
     if (angle < 1e-8f) return v;
     const v3 axis = dv(rotation, angle);
     const v3 offset = cross3(axis, v);
-    return add(add(v, smul(sinf(angle), offset)), smul(1.0f - cosf(angle), cross3(axis, offset)));
+    return add(add(v, smul(sin_rn(angle), offset)), smul(1.0f - cos_rn(angle), cross3(axis, offset)));
 }
 // core.wgsl:180-186
 __device__ __forceinline__ v3 sign_normalize(v3 v) {

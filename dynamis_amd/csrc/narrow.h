@@ -63,6 +63,13 @@ struct Shape {
     float radius, half_height;
     v3 center, half_extents, scale;
     q4 rotation;
+    uint32_t source;
+};
+
+// Hull shape sources visible to the narrowphase (records.wgsl:256-273).
+struct ShapeSet {
+    const dyn_shape_source* sources;
+    const float4* vertices;
 };
 
 // core.wgsl:473-484
@@ -76,14 +83,33 @@ __device__ __forceinline__ Shape world_collider(const dyn_body_state& s, const d
     w.half_extents = ld3(c.half_extents);
     w.rotation = qmul(q, ld4(c.local_rotation));
     w.scale = ld3(c.scale);
+    w.source = c.source;
     return w;
 }
 
-// core.wgsl:505-536 (shapes without a shape source)
-__device__ __forceinline__ void world_aabb_of(const Shape& w, v3* mn, v3* mx) {
+// core.wgsl:490-503: bounds of a shape that reads a shape source
+__device__ __forceinline__ void world_shape_bounds(const Shape& w, const ShapeSet& ss, v3* mn, v3* mx) {
+    const dyn_shape_source& src = ss.sources[w.source];
+    const v3 smin = mulv(ld3(src.local_min), w.scale);
+    const v3 smax = mulv(ld3(src.local_max), w.scale);
+    const v3 center = add(w.center, qrot(w.rotation, mul(add(smin, smax), 0.5f)));
+    const v3 half = mul(sub(smax, smin), 0.5f);
+    const v3 extent = add(add(absv(qrot(w.rotation, mk(half.x, 0.0f, 0.0f))),
+                              absv(qrot(w.rotation, mk(0.0f, half.y, 0.0f)))),
+                          absv(qrot(w.rotation, mk(0.0f, 0.0f, half.z))));
+    *mn = sub(center, extent);
+    *mx = add(center, extent);
+}
+
+// core.wgsl:505-536
+__device__ __forceinline__ void world_aabb_of(const Shape& w, const ShapeSet& ss, v3* mn, v3* mx) {
     if (w.kind == DYN_SHAPE_PLANE) {
         *mn = sub(w.center, mk(1e6f, 1e6f, 1e6f));
         *mx = add(w.center, mk(1e6f, 1e6f, 1e6f));
+        return;
+    }
+    if (w.kind == DYN_SHAPE_HULL) {
+        world_shape_bounds(w, ss, mn, mx);
         return;
     }
     v3 extent = zero3v();
@@ -539,97 +565,6 @@ __device__ inline void box_box_sat(Manifold& m, const Body& a, const dyn_collide
     if (cand.count == 0u) {
         push(m, sub(ref_center, mul(ref_normal, face_depth * 0.5f)), face_depth, kFeaturePoint);
         return;
-    }
-    keep(m, cand);
-}
-
-// convex.wgsl:1-91 (support of the shapes without a shape source) and the support map itself
-__device__ __forceinline__ v3 support_local(const Shape& w, v3 d) {
-    const v3 n = norm3(d);
-    if (w.kind == DYN_SHAPE_SPHERE) return mul(n, w.radius);
-    if (w.kind == DYN_SHAPE_CUBOID) {
-        const v3 s = mk(d.x > 0.0f ? 1.0f : -1.0f, d.y > 0.0f ? 1.0f : -1.0f, d.z > 0.0f ? 1.0f : -1.0f);
-        return mulv(s, w.half_extents);
-    }
-    if (w.kind == DYN_SHAPE_CAPSULE) {
-        const v3 tip = mul(mul(mk(0.0f, 1.0f, 0.0f), w.half_height), d.y > 0.0f ? 1.0f : -1.0f);
-        return add(tip, mul(n, w.radius));
-    }
-    return zero3v();
-}
-
-__device__ __forceinline__ v3 support(const Shape& w, v3 direction) {
-    const v3 sd = mulv(qrot(qconj(w.rotation), direction), w.scale);
-    return add(w.center, qrot(w.rotation, mulv(support_local(w, sd), w.scale)));
-}
-
-// shape_feature for cuboids, capsules and support-point shapes (convex.wgsl:639-670, 795-819,
-// 856-889)
-__device__ inline uint32_t shape_feature(const Shape& w, v3 direction, v3 out[8], uint32_t ids[8]) {
-    if (w.kind == DYN_SHAPE_CUBOID) {
-        const v3 axes[3] = {qrot(w.rotation, mk(1.0f, 0.0f, 0.0f)), qrot(w.rotation, mk(0.0f, 1.0f, 0.0f)),
-                            qrot(w.rotation, mk(0.0f, 0.0f, 1.0f))};
-        uint32_t best = 0;
-        for (uint32_t i = 1; i < 3; ++i) {
-            if (fabsf(dot3(axes[i], direction)) > fabsf(dot3(axes[best], direction))) best = i;
-        }
-        const float sign = dot3(axes[best], direction) < 0.0f ? -1.0f : 1.0f;
-        const v3 face = mul(axes[best], sign);
-        const v3 center = add(w.center, mul(face, comp(w.half_extents, best)));
-        const v3 u1 = axes[(best + 1u) % 3u], u2 = axes[(best + 2u) % 3u];
-        const float e1 = comp(w.half_extents, (best + 1u) % 3u);
-        const float e2 = comp(w.half_extents, (best + 2u) % 3u);
-        out[0] = add(add(center, mul(u1, e1)), mul(u2, e2));
-        ids[0] = box_corner_id(best, sign, 1.0f, 1.0f);
-        out[1] = add(sub(center, mul(u1, e1)), mul(u2, e2));
-        ids[1] = box_corner_id(best, sign, -1.0f, 1.0f);
-        out[2] = sub(sub(center, mul(u1, e1)), mul(u2, e2));
-        ids[2] = box_corner_id(best, sign, -1.0f, -1.0f);
-        out[3] = sub(add(center, mul(u1, e1)), mul(u2, e2));
-        ids[3] = box_corner_id(best, sign, 1.0f, -1.0f);
-        return 4;
-    }
-    if (w.kind == DYN_SHAPE_CAPSULE) {
-        const v3 ld = mulv(qrot(qconj(w.rotation), direction), w.scale);
-        const float cap = ld.y > 0.0f ? 1.0f : -1.0f;
-        const v3 radial = mul(sign_normalize(ld), w.radius);
-        if (fabsf(ld.y) > 0.93f * len3(ld)) {
-            const v3 local = add(mk(0.0f, cap * w.half_height, 0.0f), radial);
-            out[0] = add(w.center, qrot(w.rotation, mulv(local, w.scale)));
-            ids[0] = cap > 0.0f ? 1u : 0u;
-            return 1;
-        }
-        const v3 upper = add(mk(0.0f, w.half_height, 0.0f), radial);
-        const v3 lower = add(mk(0.0f, -w.half_height, 0.0f), radial);
-        out[0] = add(w.center, qrot(w.rotation, mulv(upper, w.scale)));
-        out[1] = add(w.center, qrot(w.rotation, mulv(lower, w.scale)));
-        ids[0] = 0u;
-        ids[1] = 1u;
-        return 2;
-    }
-    out[0] = support(w, direction);
-    ids[0] = 0u;
-    return 1;
-}
-
-// narrowphase.wgsl:485-512
-__device__ inline void plane_convex(Manifold& m, const Shape& plane, const Shape& convex, float margin) {
-    const v3 n = qrot(plane.rotation, mk(0.0f, 1.0f, 0.0f));
-    const float center_side = dot3(sub(convex.center, plane.center), n);
-    const v3 facing = center_side < 0.0f ? neg(n) : n;
-    emit(m, facing);
-    v3 pts[8];
-    uint32_t ids[8];
-    const uint32_t count = shape_feature(convex, neg(facing), pts, ids);
-    Candidates cand;
-    cand.count = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const float depth = dot3(sub(plane.center, pts[i]), facing);
-        if (depth <= -margin) continue;
-        cand.pos[cand.count] = add(pts[i], mul(facing, depth * 0.5f));
-        cand.depth[cand.count] = depth;
-        cand.feature[cand.count] = feature_pair(kFeatureVertex, kFeatureFaceBit | 0u, ids[i]);
-        ++cand.count;
     }
     keep(m, cand);
 }

@@ -17,7 +17,7 @@
 //    lookup (the reference sorts an index over a free-list archive, commit.rs:278-295).
 #include <vector>
 
-#include "narrow.h"
+#include "convex.h"
 #include "sort.h"
 
 namespace dyn {
@@ -160,12 +160,13 @@ __global__ void __launch_bounds__(kThreads) k_aabb(W w) {
     const dyn_body_desc d = w.de[ow];
     const dyn_collider c = w.co[i];
     const Shape ws = world_collider(s, c);
+    const ShapeSet ss{w.sources, w.vertices};
     v3 tmn, tmx;
-    world_aabb_of(ws, &tmn, &tmx);
+    world_aabb_of(ws, ss, &tmn, &tmx);
     Shape up = ws;
     up.rotation = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     v3 rmn, rmx;
-    world_aabb_of(up, &rmn, &rmx);
+    world_aabb_of(up, ss, &rmn, &rmx);
     v3 smn = tmn, smx = tmx;
     if (kind_moves(d)) {
         const v3 travel = absv(mul(ld3(s.velocity), w.p.dt));
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(kThreads) k_aabb(W w) {
         const v3 com = add(ld3(s.position), qrot(ld4(s.orientation), ld3(d.com)));
         const v3 center = add(ld3(s.position), qrot(ld4(s.orientation), ld3(c.local_offset)));
         const float radius = len3(mul(sub(tmx, tmn), 0.5f)) + len3(sub(center, com));
-        const float reach_r = spin >= kHalfTurn ? 2.0f * radius : 2.0f * radius * sinf(0.5f * spin);
+        const float reach_r = spin >= kHalfTurn ? 2.0f * radius : 2.0f * radius * sin_rn(0.5f * spin);
         const float r = w.p.contact_margin + reach_r;
         const v3 reach = mk(r, r, r);
         smn = sub(sub(tmn, travel), reach);
@@ -212,67 +213,56 @@ __device__ __forceinline__ void mirror(Manifold& m) {
     m.normal = neg(m.normal);
 }
 
-__global__ void __launch_bounds__(kThreads) k_narrow(W w) {
-    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live) return;
-    w.valid[index] = 0u;
-    const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
-    const uint32_t fb = w.owner[fs], sb = w.owner[ss];
-    if (fb == DYN_NO_BODY || sb == DYN_NO_BODY || fb == sb) return;
-    const Body first = load_body(w, fb), second = load_body(w, sb);
-    if (!kind_moves(first.d) && !kind_moves(second.d)) return;
-    const dyn_collider fc = w.co[fs], sc = w.co[ss];
-    if (fc.kind == DYN_SHAPE_NONE || sc.kind == DYN_SHAPE_NONE) return;
-    const uint32_t g1 = fc.collision_group != DYN_NO_COLLISION_FILTER ? fc.collision_group : first.d.collision_group;
-    const uint32_t m1 = fc.collision_mask != DYN_NO_COLLISION_FILTER ? fc.collision_mask : first.d.collision_mask;
-    const uint32_t g2 = sc.collision_group != DYN_NO_COLLISION_FILTER ? sc.collision_group : second.d.collision_group;
-    const uint32_t m2 = sc.collision_mask != DYN_NO_COLLISION_FILTER ? sc.collision_mask : second.d.collision_mask;
-    if (!((g1 & m2) != 0u && (g2 & m1) != 0u)) return;
-    const bool sensor = (fc.flags & DYN_COLLIDER_SENSOR) || (sc.flags & DYN_COLLIDER_SENSOR);
+// Whether a pair leaves the analytic generators for GJK/EPA and clipped features
+// (narrowphase.wgsl:551-664): planes against hulls and cylinders, scaled shapes, any hull or
+// cylinder, and capsule-box pairs.
+__device__ __forceinline__ bool convex_path(const dyn_collider& fc, const dyn_collider& sc) {
     const bool fw = fc.kind == DYN_SHAPE_PLANE, sw = sc.kind == DYN_SHAPE_PLANE;
-    if (fw && sw) return;
-    float margin = w.p.contact_margin;
+    if (fw) return !analytic(sc.kind);
+    if (sw) return !analytic(fc.kind);
+    if (scaled(fc) || scaled(sc) || !analytic(fc.kind) || !analytic(sc.kind)) return true;
+    return (fc.kind == DYN_SHAPE_CAPSULE && sc.kind == DYN_SHAPE_CUBOID) ||
+           (fc.kind == DYN_SHAPE_CUBOID && sc.kind == DYN_SHAPE_CAPSULE);
+}
+
+// The pair filter of narrowphase.wgsl:518-550; false when the pair makes no contact.
+__device__ __forceinline__ bool pair_admitted(const W& w, uint32_t fs, uint32_t ss, uint32_t* fb_out,
+                                              uint32_t* sb_out) {
+    const uint32_t fb = w.owner[fs], sb = w.owner[ss];
+    if (fb == DYN_NO_BODY || sb == DYN_NO_BODY || fb == sb) return false;
+    const dyn_body_desc& fd = w.de[fb];
+    const dyn_body_desc& sd = w.de[sb];
+    if (!kind_moves(fd) && !kind_moves(sd)) return false;
+    const dyn_collider& fc = w.co[fs];
+    const dyn_collider& sc = w.co[ss];
+    if (fc.kind == DYN_SHAPE_NONE || sc.kind == DYN_SHAPE_NONE) return false;
+    const uint32_t g1 = fc.collision_group != DYN_NO_COLLISION_FILTER ? fc.collision_group : fd.collision_group;
+    const uint32_t m1 = fc.collision_mask != DYN_NO_COLLISION_FILTER ? fc.collision_mask : fd.collision_mask;
+    const uint32_t g2 = sc.collision_group != DYN_NO_COLLISION_FILTER ? sc.collision_group : sd.collision_group;
+    const uint32_t m2 = sc.collision_mask != DYN_NO_COLLISION_FILTER ? sc.collision_mask : sd.collision_mask;
+    if (!((g1 & m2) != 0u && (g2 & m1) != 0u)) return false;
+    if (fc.kind == DYN_SHAPE_PLANE && sc.kind == DYN_SHAPE_PLANE) return false;
+    *fb_out = fb;
+    *sb_out = sb;
+    return true;
+}
+
+__device__ __forceinline__ float pair_margin(const W& w, const Body& first, const Body& second) {
     if ((first.d.flags & DYN_BODY_CCD) || (second.d.flags & DYN_BODY_CCD)) {
-        margin = w.p.contact_margin + len3(sub(ld3(second.s.velocity), ld3(first.s.velocity))) * w.p.dt;
+        return w.p.contact_margin + len3(sub(ld3(second.s.velocity), ld3(first.s.velocity))) * w.p.dt;
     }
-    Manifold m;
-    m.count = 0;
-    m.normal = zero3v();
-    bool mirrored_features = false;
-    if (fw) {
-        const Shape scene = world_collider(first.s, fc), other = world_collider(second.s, sc);
-        if (!analytic(other.kind)) return;  // hull / cylinder planes take the GJK path
-        plane_convex(m, scene, other, margin);
-    } else if (sw) {
-        const Shape scene = world_collider(second.s, sc), other = world_collider(first.s, fc);
-        if (!analytic(other.kind)) return;
-        plane_convex(m, scene, other, margin);
-        mirror(m);
-        mirrored_features = true;
-    } else if (scaled(fc) || scaled(sc) || !analytic(fc.kind) || !analytic(sc.kind)) {
-        return;  // GJK/EPA pairs
-    } else {
-        const uint32_t a = fc.kind, b = sc.kind;
-        if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_SPHERE) sphere_sphere(m, first, fc, second, sc, margin);
-        else if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_CUBOID) sphere_box(m, first, fc, second, sc, margin);
-        else if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_CAPSULE) sphere_capsule(m, first, fc, second, sc, margin);
-        else if (a == DYN_SHAPE_CUBOID && b == DYN_SHAPE_SPHERE) {
-            sphere_box(m, second, sc, first, fc, margin);
-            mirror(m);
-        } else if (a == DYN_SHAPE_CUBOID && b == DYN_SHAPE_CUBOID) box_box_sat(m, first, fc, second, sc, margin);
-        else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_SPHERE) {
-            sphere_capsule(m, second, sc, first, fc, margin);
-            mirror(m);
-        } else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_CAPSULE) capsule_capsule(m, first, fc, second, sc, margin);
-        else return;  // capsule-box takes the GJK path
-    }
-    if (m.count == 0u) return;
+    return w.p.contact_margin;
+}
+
+// Material, identity and anchors of a generated manifold (narrowphase.wgsl:665-684).
+__device__ inline void write_contact(const W& w, uint32_t index, uint32_t fs, uint32_t ss, const Body& first,
+                                     const Body& second, const dyn_collider& fc, const dyn_collider& sc,
+                                     const Manifold& m, bool mirrored_features) {
     dyn_contact c;
     c.a = fs;
     c.b = ss;
     c.point_count = m.count;
-    c.sensor = sensor ? 1u : 0u;
+    c.sensor = ((fc.flags & DYN_COLLIDER_SENSOR) || (sc.flags & DYN_COLLIDER_SENSOR)) ? 1u : 0u;
     c.first_body_id = first.s.body_id;
     c.second_body_id = second.s.body_id;
     c.first_generation = first.s.generation;
@@ -315,6 +305,85 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     }
     w.raw[index] = c;
     w.valid[index] = 1u;
+}
+
+// Analytic pairs; pairs for the convex path are marked 2 for k_narrow_convex.
+__global__ void __launch_bounds__(kThreads) k_narrow(W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    if (index >= live) return;
+    w.valid[index] = 0u;
+    const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+    uint32_t fb, sb;
+    if (!pair_admitted(w, fs, ss, &fb, &sb)) return;
+    const dyn_collider fc = w.co[fs], sc = w.co[ss];
+    if (convex_path(fc, sc)) {
+        w.valid[index] = 2u;
+        return;
+    }
+    const Body first = load_body(w, fb), second = load_body(w, sb);
+    const float margin = pair_margin(w, first, second);
+    const ShapeSet shapes{w.sources, w.vertices};
+    Manifold m;
+    m.count = 0;
+    m.normal = zero3v();
+    bool mirrored_features = false;
+    if (fc.kind == DYN_SHAPE_PLANE) {
+        plane_convex(m, world_collider(first.s, fc), world_collider(second.s, sc), shapes, margin);
+    } else if (sc.kind == DYN_SHAPE_PLANE) {
+        plane_convex(m, world_collider(second.s, sc), world_collider(first.s, fc), shapes, margin);
+        mirror(m);
+        mirrored_features = true;
+    } else {
+        const uint32_t a = fc.kind, b = sc.kind;
+        if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_SPHERE) sphere_sphere(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_CUBOID) sphere_box(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_SPHERE && b == DYN_SHAPE_CAPSULE) sphere_capsule(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_CUBOID && b == DYN_SHAPE_SPHERE) {
+            sphere_box(m, second, sc, first, fc, margin);
+            mirror(m);
+        } else if (a == DYN_SHAPE_CUBOID && b == DYN_SHAPE_CUBOID) box_box_sat(m, first, fc, second, sc, margin);
+        else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_SPHERE) {
+            sphere_capsule(m, second, sc, first, fc, margin);
+            mirror(m);
+        } else if (a == DYN_SHAPE_CAPSULE && b == DYN_SHAPE_CAPSULE) capsule_capsule(m, first, fc, second, sc, margin);
+    }
+    if (m.count == 0u) return;
+    write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
+}
+
+// Pairs marked by k_narrow: planes against hulls and cylinders, and GJK/EPA with clipped
+// feature manifolds (narrowphase.wgsl:551-664).
+__global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    if (index >= live || w.valid[index] != 2u) return;
+    w.valid[index] = 0u;
+    const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+    const uint32_t fb = w.owner[fs], sb = w.owner[ss];
+    const dyn_collider fc = w.co[fs], sc = w.co[ss];
+    const Body first = load_body(w, fb), second = load_body(w, sb);
+    const float margin = pair_margin(w, first, second);
+    const ShapeSet shapes{w.sources, w.vertices};
+    const Shape wf = world_collider(first.s, fc), wsd = world_collider(second.s, sc);
+    Manifold m;
+    m.count = 0;
+    m.normal = zero3v();
+    bool mirrored_features = false;
+    if (fc.kind == DYN_SHAPE_PLANE) {
+        plane_convex(m, wf, wsd, shapes, margin);
+    } else if (sc.kind == DYN_SHAPE_PLANE) {
+        plane_convex(m, wsd, wf, shapes, margin);
+        mirror(m);
+        mirrored_features = true;
+    } else {
+        const Hit hit = convex_hit(wf, wsd, shapes);
+        if (!(hit.distance <= margin)) return;
+        emit(m, hit.normal);
+        if (!convex_pair_manifold(wf, wsd, shapes, hit.normal, m)) manifold_from_hit(m, hit, margin);
+    }
+    if (m.count == 0u) return;
+    write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
 }
 
 // ---- stable flag compaction: per-block counts, one-workgroup scan, destination map ---------
@@ -1179,6 +1248,7 @@ struct dyn_world {
     std::vector<uint32_t> host_kinds;    // collider kinds and sources, for upload validation
     std::vector<uint32_t> host_sources;
     uint32_t hull_colliders = 0;
+    uint32_t convex_colliders = 0;  // colliders some pair may send to the convex path
 };
 
 namespace {
@@ -1261,6 +1331,7 @@ int world_step(dyn_world* world, float dt) {
     io.scratch_bytes = world->bp_scratch_bytes;
     DYN_TRY(dyn_broadphase(st, &io));
     LAUNCH(k_narrow, pb, w);
+    if (world->convex_colliders) LAUNCH(k_narrow_convex, pb, w);
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
     DYN_TRY(scan(world, w.valid, pair_count, w.pair_cap, w.dest, &w.scan_total[0]));
     LAUNCH(k_contacts_scatter, pb, w);
@@ -1376,6 +1447,12 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
         world->host_kinds[i] = colliders[i].kind;
         world->host_sources[i] = colliders[i].source;
         if (colliders[i].kind == DYN_SHAPE_HULL) ++world->hull_colliders;
+        const uint32_t k = colliders[i].kind;
+        const bool unit_scale = colliders[i].scale[0] == 1.0f && colliders[i].scale[1] == 1.0f &&
+                                colliders[i].scale[2] == 1.0f;
+        if (k == DYN_SHAPE_HULL || k == DYN_SHAPE_CYLINDER || k == DYN_SHAPE_CAPSULE || !unit_scale) {
+            ++world->convex_colliders;
+        }
     }
     W& w = world->w;
     w.id_cap = id_cap;

@@ -16,3 +16,79 @@ def uniform_boxes(k: int, extent: float, half_lo: float, half_hi: float,
     centres = rng.uniform(0.0, extent, size=(k, 3)).astype(np.float32)
     halves = np.exp(rng.uniform(np.log(half_lo), np.log(half_hi), size=(k, 3))).astype(np.float32)
     return abi.aabbs(centres - halves, centres + halves)
+
+
+def ellipsoid_hull_sources(count: int, seed: int):
+    """`count` hull shape sources, each the hull of 16-64 points drawn uniformly on an
+    ellipsoid with semi-axes ~ U[0.3, 0.9]."""
+    from .world import build_hull
+    rng = np.random.default_rng(seed)
+    hulls = []
+    for _ in range(count):
+        n = int(rng.integers(16, 65))
+        p = rng.normal(size=(n, 3))
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        hulls.append(build_hull((p * rng.uniform(0.3, 0.9, 3)).astype(np.float32)))
+    return hulls
+
+
+def convex_pile(count: int = 100_000, seed: int = 3, hull_sources: int = 512, bin_size: float = 60.0,
+                spacing: float = 2.0, layer_height: float = 2.6):
+    """The mixed convex pile: `count` bodies, a quarter each of spheres (r ~ U[0.25, 0.75]),
+    boxes (half-extents ~ U[0.2, 0.8] per axis), capsules (r ~ U[0.15, 0.4], half-height
+    ~ U[0.2, 0.8]) and hulls (drawn from `hull_sources` ellipsoid hulls of 16-64 points), density
+    1000, on a jittered lattice inside a bin_size x bin_size bin of four static cuboid walls
+    standing on a static plane. Returns (bodies, hulls)."""
+    from .world import BodyDesc
+    rng = np.random.default_rng(seed)
+    hulls = ellipsoid_hull_sources(hull_sources, seed + 1)
+    half = bin_size / 2.0
+    wall_h = 25.0
+    bodies = [BodyDesc.plane()]
+    for sx, sz, hx, hz in ((half + 0.5, 0.0, 0.5, half + 1.0), (-half - 0.5, 0.0, 0.5, half + 1.0),
+                           (0.0, half + 0.5, half + 1.0, 0.5), (0.0, -half - 0.5, half + 1.0, 0.5)):
+        bodies.append(BodyDesc.cuboid((hx, wall_h, hz), mass=0.0, position=(sx, wall_h, sz)))
+    per_row = int(bin_size // spacing) - 1
+    per_layer = per_row * per_row
+    first = -0.5 * (per_row - 1) * spacing
+    kinds = rng.permutation(np.arange(count) % 4)
+    jitter = rng.uniform(-0.1, 0.1, size=(count, 3))
+    for i in range(count):
+        layer, cell = divmod(i, per_layer)
+        gx, gz = divmod(cell, per_row)
+        pos = (float(first + gx * spacing + jitter[i, 0]), float(1.5 + layer * layer_height + jitter[i, 1]),
+               float(first + gz * spacing + jitter[i, 2]))
+        kind = int(kinds[i])
+        if kind == 0:
+            bodies.append(BodyDesc.sphere(float(rng.uniform(0.25, 0.75)), position=pos, density=1000.0))
+        elif kind == 1:
+            bodies.append(BodyDesc.cuboid(tuple(float(v) for v in rng.uniform(0.2, 0.8, 3)), position=pos,
+                                          density=1000.0))
+        elif kind == 2:
+            bodies.append(BodyDesc.capsule(float(rng.uniform(0.15, 0.4)), float(rng.uniform(0.2, 0.8)),
+                                           position=pos, density=1000.0))
+        else:
+            bodies.append(BodyDesc.hull(int(rng.integers(0, hull_sources)), position=pos, density=1000.0))
+    return bodies, hulls
+
+
+def cube_drop(count: int = 1000, seed: int = 1, footprint: float = 20.0, low: float = 1.0,
+              high: float = 40.0):
+    """The small plain workload: cubes of half-extent 0.5 m, density 1000 kg/m^3, friction 0.5,
+    restitution 0, centres uniform in a footprint x footprint square at heights [low, high],
+    uniform random orientations (normalised 4-D Gaussians), above a static plane at y = 0."""
+    from .world import BodyDesc
+    rng = np.random.default_rng(seed)
+    half = footprint / 2.0
+    xs = rng.uniform(-half, half, count)
+    zs = rng.uniform(-half, half, count)
+    ys = rng.uniform(low, high, count)
+    q = rng.normal(size=(count, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q = q.astype(np.float32)
+    bodies = [BodyDesc.plane()]
+    for i in range(count):
+        bodies.append(BodyDesc.cuboid((0.5, 0.5, 0.5), density=1000.0, friction=0.5, restitution=0.0,
+                                      position=(float(xs[i]), float(ys[i]), float(zs[i])),
+                                      orientation=tuple(float(v) for v in q[i])))
+    return bodies

@@ -62,13 +62,17 @@ static inline q4 qnorm(q4 q) {
     q4 r = {q.x / n, q.y / n, q.z / n, q.w / n};
     return r;
 }
+/* sine and cosine of an FP32 argument, correctly rounded (FP64 evaluation, one rounding) */
+static inline float sin_rn(float x) { return (float)sin((double)x); }
+static inline float cos_rn(float x) { return (float)cos((double)x); }
+
 /* core.wgsl:170-178 */
 static inline v3 rotate_about(v3 rotation, v3 v) {
     float angle = vlen(rotation);
     if (angle < 1e-8f) return v;
     v3 axis = vdiv(rotation, angle);
     v3 offset = vcross(axis, v);
-    return vadd(vadd(v, smul(sinf(angle), offset)), smul(1.0f - cosf(angle), vcross(axis, offset)));
+    return vadd(vadd(v, smul(sin_rn(angle), offset)), smul(1.0f - cos_rn(angle), vcross(axis, offset)));
 }
 /* core.wgsl:180-186 */
 static inline v3 sign_normalize(v3 v) {

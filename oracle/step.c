@@ -1341,6 +1341,7 @@ static uint32_t hull_feature_ring(const oracle_world* w, const world_shape* ws, 
                 fmaxo(fmaxo(ws->scale.x, ws->scale.y), ws->scale.z);
     v3 points[16];
     uint32_t ids[16];
+    ids[0] = 0u;
     uint32_t count = 0, first_id = EMPTY_RING_FACE;
     for (uint32_t i = 0; i < src->vertex_count && count < 16u; i++) {
         v3 v = hull_vertex(w, src, i);
@@ -2374,7 +2375,7 @@ static void prepare_and_index(oracle_world* w) {
             v3 com = vadd(vload(s->position), qrot(qload(s->orientation), vload(d->com)));
             v3 center = vadd(vload(s->position), qrot(qload(s->orientation), vload(c->local_offset)));
             float radius = vlen(vmul(vsub(tmx, tmn), 0.5f)) + vlen(vsub(center, com));
-            float reach_r = spin >= PI_F ? 2.0f * radius : 2.0f * radius * sinf(0.5f * spin);
+            float reach_r = spin >= PI_F ? 2.0f * radius : 2.0f * radius * sin_rn(0.5f * spin);
             float r = w->p.contact_margin + reach_r;
             v3 reach = V(r, r, r);
             smn = vsub(vsub(tmn, travel), reach);

@@ -14,8 +14,8 @@ import numpy as np
 import pytest
 
 from dynamis_amd.world import World
-from tests.worlds import (DT, BodyDesc, config, cube_drop, encode, gravity_config, mixed_pile,
-                          qrot, tower)
+from tests.worlds import (DT, BodyDesc, config, cube_drop, ellipsoid_hulls, encode, gravity_config,
+                          mixed_pile, qrot, tower)
 
 pytestmark = pytest.mark.gpu
 
@@ -84,19 +84,22 @@ def start_from(scene, states):
 
 
 def single_step_cases():
+    hulls = ellipsoid_hulls(12, seed=4)
     return {
-        "tower16": (tower(16), config(), 0),
+        "tower16": (tower(16), config(), 0, None),
         "cube_drop_300_mid_fall": (cube_drop(300, seed=5, footprint=12.0, low=1.0, high=12.0),
-                                   config(solve_iterations=4), 90),
-        "mixed_pile_240": (mixed_pile(240, seed=2), config(), 60),
+                                   config(solve_iterations=4), 90, None),
+        "mixed_pile_240": (mixed_pile(240, seed=2), config(), 60, None),
+        "convex_pile_300": (mixed_pile(300, seed=6, spacing=1.6, kinds=5, hull_sources=12),
+                            config(), 75, hulls),
     }
 
 
 @pytest.mark.parametrize("name", list(single_step_cases()))
 def test_single_step_from_identical_state(device, oracle, name):
-    bodies, cfg, warm = single_step_cases()[name]
+    bodies, cfg, warm, hulls = single_step_cases()[name]
     print(f"[world parity] single step {name} after {warm} oracle steps", flush=True)
-    scene = encode(bodies, cfg)
+    scene = encode(bodies, cfg, hulls)
     pre = oracle.world(scene)
     for _ in range(warm):
         pre.step(DT)
@@ -112,7 +115,7 @@ def test_single_step_from_identical_state(device, oracle, name):
 
 
 @pytest.mark.parametrize("name,frames", [("tower16", 120), ("box_stack_plane", 120),
-                                         ("spheres_capsules", 90)])
+                                         ("spheres_capsules", 90), ("hulls_cylinders", 90)])
 def test_short_rollout_tracks_oracle(device, oracle, name, frames):
     if name == "tower16":
         bodies, cfg = tower(16), config()
@@ -120,15 +123,23 @@ def test_short_rollout_tracks_oracle(device, oracle, name, frames):
         bodies = [BodyDesc.plane()] + [BodyDesc.cuboid((0.5, 0.5, 0.5), position=(0.02 * i, 0.55 + i * 1.1, 0.0))
                                        for i in range(6)]
         cfg = config()
-    else:
+    elif name == "spheres_capsules":
         bodies = [BodyDesc.plane()]
         for i in range(12):
             pos = (0.3 * (i % 3), 1.0 + 1.3 * i, 0.2 * (i % 2))
             bodies.append(BodyDesc.sphere(0.4, position=pos) if i % 2 == 0
                           else BodyDesc.capsule(0.25, 0.5, position=pos))
         cfg = gravity_config()
+    else:
+        hulls = ellipsoid_hulls(4, seed=8)
+        bodies = [BodyDesc.plane()]
+        for i in range(12):
+            pos = (0.25 * (i % 3), 1.0 + 1.4 * i, 0.15 * (i % 2))
+            bodies.append(BodyDesc.hull(i % 4, position=pos) if i % 2 == 0
+                          else BodyDesc.cylinder(0.3, 0.4, position=pos))
+        cfg = config()
     print(f"[world parity] rollout {name} {frames} frames", flush=True)
-    scene = encode(bodies, cfg)
+    scene = encode(bodies, cfg, hulls if name == "hulls_cylinders" else None)
     ref, dev = oracle.world(scene), device_world(scene)
     for frame in range(frames):
         ref.step(DT)

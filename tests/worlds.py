@@ -83,10 +83,26 @@ def cube_drop(count: int, seed: int = 0, footprint: float = 20.0, low: float = 1
     return bodies
 
 
-def mixed_pile(count: int, seed: int = 0, spacing: float = 2.0) -> list[BodyDesc]:
-    """Spheres, boxes and capsules on a jittered lattice above a plane (sizes of the mixed
-    convex pile: r ~ U[0.25, 0.75]; half-extents ~ U[0.2, 0.8]; capsule r ~ U[0.15, 0.4],
-    half-height ~ U[0.2, 0.8])."""
+def ellipsoid_hulls(count: int, seed: int = 0):
+    """Hull shape sources of 16-64 points drawn uniformly on ellipsoids with semi-axes
+    ~ U[0.3, 0.9] (the convex pile's hulls)."""
+    from dynamis_amd.world import build_hull
+    rng = np.random.default_rng(seed + 1000)
+    hulls = []
+    for _ in range(count):
+        n = int(rng.integers(16, 65))
+        p = rng.normal(size=(n, 3))
+        p /= np.linalg.norm(p, axis=1, keepdims=True)
+        hulls.append(build_hull((p * rng.uniform(0.3, 0.9, 3)).astype(np.float32)))
+    return hulls
+
+
+def mixed_pile(count: int, seed: int = 0, spacing: float = 2.0, kinds: int = 3,
+               hull_sources: int = 0) -> list[BodyDesc]:
+    """Convex bodies on a jittered lattice above a plane, cycling through `kinds` shape kinds:
+    spheres r ~ U[0.25, 0.75], boxes half-extents ~ U[0.2, 0.8], capsules r ~ U[0.15, 0.4] with
+    half-height ~ U[0.2, 0.8], then hulls (`hull_sources` shape sources, drawn in turn) and
+    cylinders of the capsule sizes."""
     rng = np.random.default_rng(seed)
     side = int(np.ceil(np.sqrt(count / 4.0)))
     bodies = [BodyDesc.plane()]
@@ -96,7 +112,14 @@ def mixed_pile(count: int, seed: int = 0, spacing: float = 2.0) -> list[BodyDesc
         jitter = rng.uniform(-0.1, 0.1, 3)
         pos = (float((gx - side / 2) * spacing + jitter[0]), float(1.5 + layer * spacing + jitter[1]),
                float((gz - side / 2) * spacing + jitter[2]))
-        kind = i % 3
+        kind = i % kinds
+        if kind == 3:
+            bodies.append(BodyDesc.hull(i % max(hull_sources, 1), position=pos, density=1000.0))
+            continue
+        if kind == 4:
+            bodies.append(BodyDesc.cylinder(float(rng.uniform(0.15, 0.4)), float(rng.uniform(0.2, 0.8)),
+                                            position=pos, density=1000.0))
+            continue
         if kind == 0:
             bodies.append(BodyDesc.sphere(float(rng.uniform(0.25, 0.75)), position=pos,
                                           density=1000.0))
