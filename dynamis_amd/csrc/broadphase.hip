@@ -31,14 +31,13 @@ __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
 
 
 
-__global__ void bp_reset_kernel(uint32_t* counters, uint32_t* overflow_n) {
+__global__ void bp_reset_kernel(uint32_t* counters) {
     const uint32_t slots[] = {DYN_COUNTER_ENTRIES,     DYN_COUNTER_PAIRS,
                               DYN_COUNTER_REFUSED_PAIRS, DYN_COUNTER_ENTRY_FAULTS,
                               DYN_COUNTER_GRID_LEVELS, DYN_COUNTER_COARSE_ACTIVE,
                               DYN_COUNTER_GRID_SCALE,  DYN_COUNTER_GRID_EXTENT};
     const uint32_t t = threadIdx.x;
     if (t < sizeof(slots) / sizeof(slots[0])) *ctr(counters, slots[t]) = 0;
-    if (t == 0) *overflow_n = 0;
 }
 
 // broadphase_aabb.wgsl:21-24: the base cell is the smallest resolution box's largest side.
@@ -325,62 +324,41 @@ __device__ __forceinline__ void key_run(const uint32_t* __restrict__ keys,
     *end = e;
 }
 
-// Candidate pairs are staged per workgroup in LDS (pair_emit.wgsl:10-31 appends each pair with a
-// device atomic on one counter; here the only atomics are on LDS). A workgroup's staged pairs
-// go to its own region of a staging array at the end of the kernel; bp_pairs_scan_kernel and
-// bp_pairs_copy_kernel then lay the regions out back to back. Pairs beyond a workgroup's LDS
-// room take a global overflow list.
-constexpr uint32_t kPairGroups = 1024;        // persistent workgroups of the cell-mate kernel
-constexpr uint32_t kLinkGroups = 4096;        // most workgroups of the level-link kernel
-constexpr uint32_t kStageCapacity = 512;      // pairs staged per workgroup
-
-struct PairStage {
-    uint32_t n;
-    uint32_t major[kStageCapacity];
-    uint32_t minor[kStageCapacity];
+// Candidate pairs are written in two passes (pair_emit.wgsl:10-31 appends each pair with a device
+// atomic on one counter, which serializes): a counting pass records how many pairs every
+// collider's level links and every sorted entry's cell mates produce, one scan turns the counts
+// into offsets, and an emitting pass repeats the same walk and writes each pair at its offset.
+// Link pairs come first, then cell-mate pairs; the canonical sort orders them afterwards.
+struct PairCounter {
+    uint32_t n = 0;
+    __device__ __forceinline__ void operator()(uint32_t, uint32_t) { ++n; }
 };
 
-struct PairSink {
-    uint32_t* staged;      // (kPairGroups + kLinkGroups) * kStageCapacity * 2 words
-    uint32_t* counts;      // staged count of every region
-    uint32_t* overflow;    // overflow pairs, 2 words each
-    uint32_t* overflow_n;  // overflow count
-    uint32_t overflow_capacity;
+struct PairWriter {
+    uint32_t at;
+    uint32_t capacity;
+    uint32_t* major;
+    uint32_t* minor;
+    __device__ __forceinline__ void operator()(uint32_t a, uint32_t b) {
+        if (at < capacity) {
+            major[at] = a;
+            minor[at] = b;
+        }
+        ++at;
+    }
 };
 
-__device__ __forceinline__ void stage_pair(PairStage& stage, const PairSink& sink,
-                                           uint32_t group_a, uint32_t info_a, uint32_t group_b,
-                                           uint32_t info_b) {
-    // pair_emit.wgsl:10-31 (node identity is excluded by the callers).
+// pair_emit.wgsl:10-31 (node identity is excluded by the callers).
+template <typename Sink>
+__device__ __forceinline__ void emit_pair(Sink& sink, uint32_t group_a, uint32_t info_a, uint32_t group_b,
+                                          uint32_t info_b) {
     if (((info_a & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (((info_b & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (group_a == group_b) return;
     if (!(info_a & DYN_ENTRY_MOBILE) && !(info_b & DYN_ENTRY_MOBILE)) return;
     const uint32_t a = info_a & DYN_ENTRY_INDEX_MASK;
     const uint32_t b = info_b & DYN_ENTRY_INDEX_MASK;
-    const uint32_t slot = atomicAdd(&stage.n, 1u);
-    if (slot < kStageCapacity) {
-        stage.major[slot] = min(a, b);
-        stage.minor[slot] = max(a, b);
-        return;
-    }
-    const uint32_t o = atomicAdd(sink.overflow_n, 1u);
-    if (o < sink.overflow_capacity) {
-        sink.overflow[2 * o] = min(a, b);
-        sink.overflow[2 * o + 1] = max(a, b);
-    }
-}
-
-// Writes a workgroup's staged pairs to its region (called by every thread after a barrier).
-__device__ __forceinline__ void flush_stage(const PairStage& stage, const PairSink& sink,
-                                            uint32_t region) {
-    const uint32_t n = min(stage.n, kStageCapacity);
-    uint32_t* out = sink.staged + static_cast<size_t>(region) * kStageCapacity * 2;
-    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
-        out[2 * j] = stage.major[j];
-        out[2 * j + 1] = stage.minor[j];
-    }
-    if (threadIdx.x == 0) sink.counts[region] = n;
+    sink(min(a, b), max(a, b));
 }
 
 struct EntryView {
@@ -422,11 +400,12 @@ __device__ __forceinline__ bool mate_in_cell(const EntryView& a, const EntryView
 // Same-level cell mates (cell_pairs.wgsl:20-50) of one sorted entry. An entry's own key is the
 // key of the cell it searches, so its run is the block of equal keys around it; most entries have
 // no equal neighbour and finish after reading three coalesced keys.
+template <typename Sink>
 __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
                                            const uint32_t* __restrict__ keys,
                                            const uint32_t* __restrict__ order,
                                            const dyn_grid_entry* __restrict__ entries, float grid,
-                                           PairStage& stage, const PairSink& sink) {
+                                           Sink& sink) {
     const uint32_t key = keys[index];
     const bool before = index > 0 && keys[index - 1] == key;
     const bool after = index + 1 < len && keys[index + 1] == key;
@@ -442,38 +421,56 @@ __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
     for (uint32_t e = index; e-- > 0 && keys[e] == key;) {
         const EntryView o = load_entry(entries, order[e]);
         if (o.info & DYN_ENTRY_AWAKE) continue;
-        if (mate_in_cell(self, o, cs, cell))
-            stage_pair(stage, sink, self.group, self.info, o.group, o.info);
+        if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info);
     }
     for (uint32_t e = index + 1; e < len && keys[e] == key; ++e) {
         const EntryView o = load_entry(entries, order[e]);
-        if (mate_in_cell(self, o, cs, cell))
-            stage_pair(stage, sink, self.group, self.info, o.group, o.info);
+        if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info);
     }
 }
 
-__global__ void __launch_bounds__(256) bp_mates_kernel(
+// Cell mates, counting pass: one thread per sorted entry.
+__global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
     const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
-    const uint32_t* sort_ctl, PairSink sink) {
-    __shared__ PairStage stage;
-    if (threadIdx.x == 0) stage.n = 0;
-    __syncthreads();
+    const uint32_t* sort_ctl, uint32_t* mate_counts, uint32_t* mate_blocks) {
+    __shared__ uint32_t s_scan[kBpThreads / kWave];
     const uint32_t len = sort_ctl[kCtlLen];
     const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
-    for (uint32_t index = blockIdx.x * blockDim.x + threadIdx.x; index < len;
-         index += gridDim.x * blockDim.x) {
-        cell_mates(index, len, keys, order, entries, grid, stage, sink);
-    }
-    __syncthreads();
-    flush_stage(stage, sink, blockIdx.x);
+    const uint32_t index = blockIdx.x * kBpThreads + threadIdx.x;
+    PairCounter counter;
+    if (index < len) cell_mates(index, len, keys, order, entries, grid, counter);
+    if (index < len) mate_counts[index] = counter.n;
+    uint32_t total = 0;
+    block_exclusive_sum<kBpThreads>(counter.n, s_scan, &total);
+    if (threadIdx.x == 0) mate_blocks[blockIdx.x] = total;
+}
+
+// Cell mates, emitting pass: the same walk, pairs written at their scanned offsets.
+__global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+    const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
+    const uint32_t* sort_ctl, const uint32_t* __restrict__ mate_counts,
+    const uint32_t* __restrict__ mate_offsets, uint32_t* pair_major, uint32_t* pair_minor,
+    uint32_t pair_capacity) {
+    __shared__ uint32_t s_scan[kBpThreads / kWave];
+    const uint32_t len = sort_ctl[kCtlLen];
+    const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
+                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
+    const uint32_t index = blockIdx.x * kBpThreads + threadIdx.x;
+    const uint32_t mine = index < len ? mate_counts[index] : 0u;
+    const uint32_t at = mate_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
+    if (mine == 0u || at >= pair_capacity) return;
+    PairWriter writer{at, pair_capacity, pair_major, pair_minor};
+    cell_mates(index, len, keys, order, entries, grid, writer);
 }
 
 // Coarser-level links (level_links.wgsl:43-65) of one collider. Every collider's primary entry
 // (cell offset 0) is its first slot, so the probe starts from the collider's own box, flags and
 // level instead of from the sorted array; a primary beyond capacity probes nothing, as in the
 // reference.
+template <typename Sink>
 __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_t slot,
                                             const dyn_aabb* __restrict__ aabbs,
                                             const uint32_t* __restrict__ groups,
@@ -483,7 +480,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                                             const dyn_grid_entry* __restrict__ entries,
                                             const uint32_t* __restrict__ dir, bool coarse_active,
                                             uint32_t occupied, float grid, uint32_t capacity,
-                                            PairStage& stage, const PairSink& sink) {
+                                            Sink& sink) {
     const uint32_t emitted = packed & 0xFFu;
     if (emitted == 0u || slot >= capacity) return;
     const uint32_t f = flags ? flags[i] : (DYN_BP_MOBILE | DYN_BP_AWAKE);
@@ -517,99 +514,93 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                 if (other == slot) continue;
                 const EntryView o = load_entry(entries, other);
                 if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
-                if (mate_in_cell(self, o, cs, cell))
-                    stage_pair(stage, sink, self.group, self.info, o.group, o.info);
+                if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info);
             }
         }
     }
 }
 
-__global__ void __launch_bounds__(kBpThreads) bp_links_kernel(
+// Level links, counting pass: one thread per collider; blocks of 256 colliders are the count
+// kernel's blocks, so block_offsets gives each collider's first entry slot.
+__global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
-    PairSink sink) {
-    __shared__ PairStage stage;
+    uint32_t* link_counts, uint32_t* link_blocks) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
-    if (threadIdx.x == 0) stage.n = 0;
-    __syncthreads();
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
     const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
-    const uint32_t chunks = ceil_div(n, kBpThreads);
-    // Chunks of 256 colliders are the count kernel's blocks, so block_offsets applies per chunk.
-    for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
-        const uint32_t i = chunk * kBpThreads + threadIdx.x;
-        const uint32_t packed = i < n ? cnt_level[i] : 0u;
-        const uint32_t slot = block_offsets[chunk] +
-                              block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
-        if (i < n) {
-            level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir,
-                        coarse_active, occupied, grid, capacity, stage, sink);
-        }
+    const uint32_t i = blockIdx.x * kBpThreads + threadIdx.x;
+    const uint32_t packed = i < n ? cnt_level[i] : 0u;
+    const uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
+    PairCounter counter;
+    if (i < n) {
+        level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir, coarse_active,
+                    occupied, grid, capacity, counter);
+        link_counts[i] = counter.n;
     }
-    __syncthreads();
-    flush_stage(stage, sink, kPairGroups + blockIdx.x);
+    uint32_t total = 0;
+    block_exclusive_sum<kBpThreads>(counter.n, s_scan, &total);
+    if (threadIdx.x == 0) link_blocks[blockIdx.x] = total;
 }
 
-// One workgroup: offsets of the staged regions, the pair count and the refused count.
-__global__ void __launch_bounds__(1024) bp_pairs_scan_kernel(const uint32_t* __restrict__ counts,
-                                                             uint32_t regions,
-                                                             uint32_t* region_offsets,
-                                                             const uint32_t* overflow_n,
-                                                             uint32_t* counters,
-                                                             uint32_t pair_capacity) {
+// Level links, emitting pass.
+__global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
+    const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
+    const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
+    const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
+    const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
+    const uint32_t* __restrict__ link_counts, const uint32_t* __restrict__ link_offsets,
+    uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
+    __shared__ uint32_t s_scan[kBpThreads / kWave];
+    const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
+    const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
+    const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
+                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
+    const uint32_t i = blockIdx.x * kBpThreads + threadIdx.x;
+    const uint32_t packed = i < n ? cnt_level[i] : 0u;
+    const uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
+    const uint32_t mine = i < n ? link_counts[i] : 0u;
+    const uint32_t at = link_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
+    if (mine == 0u || at >= pair_capacity) return;
+    PairWriter writer{at, pair_capacity, pair_major, pair_minor};
+    level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir, coarse_active, occupied,
+                grid, capacity, writer);
+}
+
+// One workgroup: exclusive offsets of the link blocks, then of the mate blocks after them, the
+// pair count and the refused count.
+__global__ void __launch_bounds__(1024) bp_pair_offsets_kernel(uint32_t* link_blocks, uint32_t link_n,
+                                                               uint32_t* mate_blocks, uint32_t mate_n,
+                                                               uint32_t* counters, uint32_t pair_capacity) {
     __shared__ uint32_t s_scan[1024 / kWave];
     __shared__ uint32_t s_carry;
     if (threadIdx.x == 0) s_carry = 0;
-    for (uint32_t first = 0; first < regions; first += 1024) {
-        const uint32_t r = first + threadIdx.x;
-        const uint32_t v = r < regions ? counts[r] : 0u;
-        uint32_t total = 0;
-        const uint32_t excl = block_exclusive_sum<1024>(v, s_scan, &total);
-        if (r < regions) region_offsets[r] = s_carry + excl;
-        __syncthreads();
-        if (threadIdx.x == 0) s_carry += total;
-        __syncthreads();
+    for (int part = 0; part < 2; ++part) {
+        uint32_t* blocks = part == 0 ? link_blocks : mate_blocks;
+        const uint32_t count = part == 0 ? link_n : mate_n;
+        for (uint32_t first = 0; first < count; first += 1024) {
+            __syncthreads();
+            const uint32_t r = first + threadIdx.x;
+            const uint32_t v = r < count ? blocks[r] : 0u;
+            uint32_t total = 0;
+            const uint32_t excl = block_exclusive_sum<1024>(v, s_scan, &total);
+            if (r < count) blocks[r] = s_carry + excl;
+            __syncthreads();
+            if (threadIdx.x == 0) s_carry += total;
+        }
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t staged = s_carry;
-        const uint32_t total = staged + overflow_n[0];
-        region_offsets[regions] = staged;
+        const uint32_t total = s_carry;
         counters[DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS] = total;
         counters[DYN_COUNTER_REFUSED_PAIRS * DYN_COUNTER_STRIDE_WORDS] =
             total > pair_capacity ? total - pair_capacity : 0u;
-    }
-}
-
-// Lays the staged regions (then the overflow list) out back to back in the pair stream.
-__global__ void __launch_bounds__(256) bp_pairs_copy_kernel(PairSink sink,
-                                                            const uint32_t* __restrict__ region_offsets,
-                                                            uint32_t regions, uint32_t* pair_major,
-                                                            uint32_t* pair_minor,
-                                                            uint32_t pair_capacity) {
-    for (uint32_t r = blockIdx.x; r < regions; r += gridDim.x) {
-        const uint32_t n = sink.counts[r];
-        const uint32_t at = region_offsets[r];
-        const uint32_t* in = sink.staged + static_cast<size_t>(r) * kStageCapacity * 2;
-        for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
-            if (at + j < pair_capacity) {
-                pair_major[at + j] = in[2 * j];
-                pair_minor[at + j] = in[2 * j + 1];
-            }
-        }
-    }
-    const uint32_t staged = region_offsets[regions];
-    const uint32_t extra = min(sink.overflow_n[0], sink.overflow_capacity);
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < extra;
-         j += gridDim.x * blockDim.x) {
-        if (staged + j < pair_capacity) {
-            pair_major[staged + j] = sink.overflow[2 * j];
-            pair_minor[staged + j] = sink.overflow[2 * j + 1];
-        }
     }
 }
 
@@ -619,11 +610,10 @@ struct BpLayout {
     uint32_t* block_stats;
     uint32_t* block_offsets;
     uint32_t* res_partials;
-    uint32_t* staged;
-    uint32_t* stage_counts;
-    uint32_t* region_offsets;
-    uint32_t* overflow;
-    uint32_t* overflow_n;
+    uint32_t* link_counts;
+    uint32_t* link_blocks;
+    uint32_t* mate_counts;
+    uint32_t* mate_blocks;
     uint32_t* dir;
     uint32_t* pair_payload;
     SortScratch entry_sort;
@@ -641,12 +631,11 @@ BpLayout bp_layout(uint32_t colliders, uint32_t entry_capacity, uint32_t pair_ca
     l.block_stats = carve.take<uint32_t>(3 * static_cast<size_t>(blocks));
     l.block_offsets = carve.take<uint32_t>(blocks);
     l.res_partials = carve.take<uint32_t>(2 * kResolutionBlocks);
-    l.staged = carve.take<uint32_t>(static_cast<size_t>(kPairGroups + kLinkGroups) *
-                                    kStageCapacity * 2);
-    l.stage_counts = carve.take<uint32_t>(kPairGroups + kLinkGroups);
-    l.region_offsets = carve.take<uint32_t>(kPairGroups + kLinkGroups + 1);
-    l.overflow = carve.take<uint32_t>(2 * static_cast<size_t>(pair_capacity ? pair_capacity : 1u));
-    l.overflow_n = carve.take<uint32_t>(1);
+    const uint32_t entry_blocks = ceil_div(entry_capacity ? entry_capacity : 1u, kBpThreads);
+    l.link_counts = carve.take<uint32_t>(colliders ? colliders : 1u);
+    l.link_blocks = carve.take<uint32_t>(blocks);
+    l.mate_counts = carve.take<uint32_t>(entry_capacity ? entry_capacity : 1u);
+    l.mate_blocks = carve.take<uint32_t>(entry_blocks);
     l.dir = carve.take<uint32_t>(static_cast<size_t>(kGridLevels) * kDirStride);
     l.pair_payload = carve.take<uint32_t>(pair_capacity ? pair_capacity : 1u);
     const uint32_t pair_passes = 2u * dyn_sort_key_words(colliders ? colliders : 1u);
@@ -694,7 +683,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     uint32_t* counters = io->d_counters;
     {
         ProfileScope scope("bp_reset_kernel", st);
-        bp_reset_kernel<<<1, 64, 0, st>>>(counters, l.overflow_n);
+        bp_reset_kernel<<<1, 64, 0, st>>>(counters);
     }
     DYN_LAUNCHED("bp_reset_kernel");
     const uint32_t blocks = ceil_div(n ? n : 1u, kBpThreads);
@@ -748,38 +737,46 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
                                                                         l.entry_sort.ctl, l.dir);
     }
     DYN_LAUNCHED("bp_directory_kernel");
-    const PairSink sink{l.staged, l.stage_counts, l.overflow, l.overflow_n, io->pair_capacity};
-    const uint32_t link_groups = min(blocks, kLinkGroups);
-    const uint32_t regions = kPairGroups + link_groups;
+    const uint32_t entry_blocks = ceil_div(io->entry_capacity, kBpThreads);
     {
-        ProfileScope scope("bp_links_kernel", st);
-        bp_links_kernel<<<link_groups, kBpThreads, 0, st>>>(
+        ProfileScope scope("bp_links_count_kernel", st);
+        bp_links_count_kernel<<<blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
             io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
-            io->entry_capacity, sink);
+            io->entry_capacity, l.link_counts, l.link_blocks);
     }
-    DYN_LAUNCHED("bp_links_kernel");
+    DYN_LAUNCHED("bp_links_count_kernel");
     {
-        ProfileScope scope("bp_mates_kernel", st);
-        bp_mates_kernel<<<kPairGroups, 256, 0, st>>>(io->d_entry_keys, io->d_entry_order,
-                                                     io->d_entries, counters, l.entry_sort.ctl,
-                                                     sink);
+        ProfileScope scope("bp_mates_count_kernel", st);
+        bp_mates_count_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
+            io->d_entry_keys, io->d_entry_order, io->d_entries, counters, l.entry_sort.ctl,
+            l.mate_counts, l.mate_blocks);
     }
-    DYN_LAUNCHED("bp_mates_kernel");
+    DYN_LAUNCHED("bp_mates_count_kernel");
     {
-        ProfileScope scope("bp_pairs_scan_kernel", st);
-        bp_pairs_scan_kernel<<<1, 1024, 0, st>>>(l.stage_counts, regions,
-                                                 l.region_offsets, l.overflow_n, counters,
-                                                 io->pair_capacity);
+        ProfileScope scope("bp_pair_offsets_kernel", st);
+        bp_pair_offsets_kernel<<<1, 1024, 0, st>>>(l.link_blocks, blocks, l.mate_blocks, entry_blocks,
+                                                   counters, io->pair_capacity);
     }
-    DYN_LAUNCHED("bp_pairs_scan_kernel");
-    {
-        ProfileScope scope("bp_pairs_copy_kernel", st);
-        bp_pairs_copy_kernel<<<kPairGroups, 256, 0, st>>>(sink, l.region_offsets,
-                                                         regions, io->d_pair_major,
-                                                         io->d_pair_minor, io->pair_capacity);
+    DYN_LAUNCHED("bp_pair_offsets_kernel");
+    if (io->pair_capacity) {
+        {
+            ProfileScope scope("bp_links_emit_kernel", st);
+            bp_links_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
+                io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
+                io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
+                io->entry_capacity, l.link_counts, l.link_blocks, io->d_pair_major, io->d_pair_minor,
+                io->pair_capacity);
+        }
+        DYN_LAUNCHED("bp_links_emit_kernel");
+        {
+            ProfileScope scope("bp_mates_emit_kernel", st);
+            bp_mates_emit_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
+                io->d_entry_keys, io->d_entry_order, io->d_entries, counters, l.entry_sort.ctl,
+                l.mate_counts, l.mate_blocks, io->d_pair_major, io->d_pair_minor, io->pair_capacity);
+        }
+        DYN_LAUNCHED("bp_mates_emit_kernel");
     }
-    DYN_LAUNCHED("bp_pairs_copy_kernel");
     if (io->sort_pairs && io->pair_capacity) {
         const uint32_t words = dyn_sort_key_words(n);
         sort_digits(words, words, digits);

@@ -2,8 +2,10 @@
 // kind (hulls by a linear scan of their vertices), GJK closest points, EPA penetration, the
 // direction probe fallback, and manifolds from clipped feature rings. Semantics:
 // crates/dynamis-shader/shaders/convex.wgsl:1-1071 and crates/dynamis-rigid/shaders/
-// narrowphase.wgsl:479-512. One thread handles one pair; the EPA polytope lives in the thread's
-// private memory.
+// narrowphase.wgsl:479-512. GJK, the probe and the manifolds run one thread per pair; EPA runs one
+// wavefront per penetrating pair, with the polytope in the wave's LDS and the lanes scanning a
+// hull's vertices and EPA's faces in parallel, reduced to the element the reference's sequential
+// scan picks (the first maximum or minimum).
 #pragma once
 
 #include "narrow.h"
@@ -29,7 +31,57 @@ __device__ __forceinline__ v3 hull_vertex(const ShapeSet& ss, const dyn_shape_so
     return mk(v.x, v.y, v.z);
 }
 
-// convex.wgsl:1-85 (hulls of at most DYN_LINEAR_SUPPORT_VERTICES vertices)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The hull vertex of largest dot(v, d), first index on ties (the sequential scan's `s >
+// best` keeps the first maximum), the lanes of a wave scanning every 64th vertex.
+__device__ inline v3 hull_support_wave(const Shape& w, const ShapeSet& ss, v3 d) {
+    const dyn_shape_source src = ss.sources[w.source];
+    float best = -kBig;
+    uint32_t index = 0xFFFFFFFFu;
+    for (uint32_t i = lane_id(); i < src.vertex_count; i += kWave) {
+        const float s = dot3(hull_vertex(ss, src, i), d);
+        if (s > best) {
+            best = s;
+            index = i;
+        }
+    }
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) {
+        const float ob = __shfl_xor(best, offset, kWave);
+        const uint32_t oi = __shfl_xor(index, offset, kWave);
+        if (ob > best || (ob == best && oi < index)) {
+            best = ob;
+            index = oi;
+        }
+    }
+    return index == 0xFFFFFFFFu ? zero3v() : hull_vertex(ss, src, index);
+}
+
+// The same vertex by one thread's sequential scan.
+__device__ inline v3 hull_support_thread(const Shape& w, const ShapeSet& ss, v3 d) {
+    const dyn_shape_source src = ss.sources[w.source];
+    float best_dot = -kBig;
+    v3 best = zero3v();
+#pragma unroll 8
+    for (uint32_t i = 0; i < src.vertex_count; ++i) {
+        const v3 v = hull_vertex(ss, src, i);
+        const float s = dot3(v, d);
+        if (s > best_dot) {
+            best_dot = s;
+            best = v;
+        }
+    }
+    return best;
+}
+
+// convex.wgsl:1-85 (hulls of at most DYN_LINEAR_SUPPORT_VERTICES vertices). With kWaveHull the
+// lanes of a wave call it together and share the hull scan.
+template <bool kWaveHull = false>
 __device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
     const v3 n = norm3(d);
     if (w.kind == DYN_SHAPE_SPHERE) return mul(n, w.radius);
@@ -48,24 +100,14 @@ __device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
         return add(base, mul(norm3(radial), w.radius));
     }
     if (w.kind != DYN_SHAPE_HULL) return zero3v();
-    const dyn_shape_source src = ss.sources[w.source];
-    float best_dot = -kBig;
-    v3 best = zero3v();
-    for (uint32_t i = 0; i < src.vertex_count; ++i) {
-        const v3 v = hull_vertex(ss, src, i);
-        const float s = dot3(v, d);
-        if (s > best_dot) {
-            best_dot = s;
-            best = v;
-        }
-    }
-    return best;
+    return kWaveHull ? hull_support_wave(w, ss, d) : hull_support_thread(w, ss, d);
 }
 
 // convex.wgsl:86-91
+template <bool kWaveHull = false>
 __device__ inline v3 support(const Shape& w, const ShapeSet& ss, v3 direction) {
     const v3 sd = mulv(qrot(qconj(w.rotation), direction), w.scale);
-    return add(w.center, qrot(w.rotation, mulv(support_local(w, ss, sd), w.scale)));
+    return add(w.center, qrot(w.rotation, mulv(support_local<kWaveHull>(w, ss, sd), w.scale)));
 }
 
 // ---- GJK (convex.wgsl:93-282, 439-521) -------------------------------------------------------
@@ -323,11 +365,20 @@ __device__ inline Closest convex_closest(const Shape& first, const Shape& second
     return r;
 }
 
-// ---- EPA (convex.wgsl:286-437) ---------------------------------------------------------------
+// ---- EPA (convex.wgsl:286-437), one wave per polytope ------------------------------------------
 
 struct EpaFace {
-    uint32_t a, b, c;
-    v3 normal;
+    uint32_t a, b, c, pad;
+    float4 normal;
+};
+
+// The EPA polytope of one wave in LDS. Only the Minkowski points are kept: the reference also
+// stores each point's two support points, but its callers never read them.
+struct EpaShared {
+    float4 w[64];
+    EpaFace faces[128];
+    EpaFace fresh[64];
+    uint32_t visible[128];
 };
 
 __device__ __forceinline__ Epa epa_invalid() {
@@ -339,113 +390,172 @@ __device__ __forceinline__ Epa epa_invalid() {
     return r;
 }
 
+__device__ __forceinline__ v3 lw(const EpaShared& e, uint32_t i) {
+    const float4 p = e.w[i];
+    return mk(p.x, p.y, p.z);
+}
+
 __device__ __forceinline__ EpaFace epa_face(uint32_t a, uint32_t b, uint32_t c, v3 unit, v3 centroid) {
     EpaFace f;
     f.a = a;
     f.b = b;
     f.c = c;
-    f.normal = dot3(unit, centroid) > 0.0f ? neg(unit) : unit;
+    f.pad = 0u;
+    const v3 n = dot3(unit, centroid) > 0.0f ? neg(unit) : unit;
+    f.normal = make_float4(n.x, n.y, n.z, 0.0f);
     return f;
 }
 
-__device__ __noinline__ Epa epa_tetrahedron(const Shape& first, const Shape& second, const ShapeSet& ss,
-                                            const SimplexPoint* simplex, uint32_t count) {
-    SimplexPoint polytope[64];
-    EpaFace new_faces[64];
-    EpaFace faces[128];
-    uint32_t visible[128];
-    uint32_t vertex_count = 0;
-    for (uint32_t i = 0; i < count; ++i) polytope[vertex_count++] = simplex[i];
+__device__ __forceinline__ v3 face_normal(const EpaFace& f) { return mk(f.normal.x, f.normal.y, f.normal.z); }
+
+// The face of smallest dot(normal, w[a]) among faces with a unit normal, first index on ties
+// (the sequential `dist < best` keeps the first minimum); kBig and face 0 when none qualifies.
+__device__ inline void epa_best_face(const EpaShared& e, uint32_t face_count, float* best_dist,
+                                     uint32_t* best_face) {
+    float best = kBig;
+    uint32_t index = 0xFFFFFFFFu;
+    for (uint32_t i = lane_id(); i < face_count; i += kWave) {
+        const EpaFace f = e.faces[i];
+        const float dist = dot3(face_normal(f), lw(e, f.a));
+        if (dist < best && len3(face_normal(f)) > 0.5f) {
+            best = dist;
+            index = i;
+        }
+    }
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1) {
+        const float ob = __shfl_xor(best, offset, kWave);
+        const uint32_t oi = __shfl_xor(index, offset, kWave);
+        if (ob < best || (ob == best && oi < index)) {
+            best = ob;
+            index = oi;
+        }
+    }
+    *best_dist = best;
+    *best_face = index == 0xFFFFFFFFu ? 0u : index;
+}
+
+__device__ inline Epa epa_tetrahedron(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                      const SimplexPoint* simplex, uint32_t count, EpaShared& e) {
+    const uint32_t lane = lane_id();
+    if (count != 4u) return epa_invalid();
+    if (lane < 4u) e.w[lane] = make_float4(simplex[lane].w.x, simplex[lane].w.y, simplex[lane].w.z, 0.0f);
+    wave_sync();
+    uint32_t vertex_count = 4;
     uint32_t face_count = 0;
-    if (vertex_count != 4u) return epa_invalid();
     for (uint32_t i = 0; i < 4; ++i) {
         uint32_t a, b, c;
         tet_face(i, &a, &b, &c);
-        const v3 normal = cross3(sub(polytope[b].w, polytope[a].w), sub(polytope[c].w, polytope[a].w));
+        const v3 normal = cross3(sub(lw(e, b), lw(e, a)), sub(lw(e, c), lw(e, a)));
         const float length_n = len3(normal);
         if (length_n < 1e-10f) return epa_invalid();
         const v3 unit = dv(normal, length_n);
-        const v3 centroid = mul(add(add(polytope[a].w, polytope[b].w), polytope[c].w), 1.0f / 3.0f);
-        faces[face_count++] = epa_face(a, b, c, unit, centroid);
+        const v3 centroid = mul(add(add(lw(e, a), lw(e, b)), lw(e, c)), 1.0f / 3.0f);
+        if (lane == 0u) e.faces[face_count] = epa_face(a, b, c, unit, centroid);
+        ++face_count;
     }
+    wave_sync();
     float best_dist = 0.0f;
     uint32_t best_face = 0;
     for (uint32_t iter = 0; iter < 32u; ++iter) {
-        best_dist = kBig;
-        best_face = 0;
-        for (uint32_t i = 0; i < face_count; ++i) {
-            const float dist = dot3(faces[i].normal, polytope[faces[i].a].w);
-            if (dist < best_dist && len3(faces[i].normal) > 0.5f) {
-                best_dist = dist;
-                best_face = i;
-            }
-        }
-        const v3 bn = faces[best_face].normal;
-        const v3 probe = support_pair(first, second, ss, bn).w;
+        epa_best_face(e, face_count, &best_dist, &best_face);
+        const v3 bn = face_normal(e.faces[best_face]);
+        const v3 probe = sub(support<true>(first, ss, bn), support<true>(second, ss, neg(bn)));
         const float gain = dot3(probe, bn) - best_dist;
         if (gain < 1e-4f) break;
         if (vertex_count >= 64u) break;
-        polytope[vertex_count].w = probe;
-        polytope[vertex_count].a = support(first, ss, bn);
-        polytope[vertex_count].b = support(second, ss, neg(bn));
+        if (lane == 0u) e.w[vertex_count] = make_float4(probe.x, probe.y, probe.z, 0.0f);
         ++vertex_count;
+        wave_sync();
+        // faces that see the new point, in face order
         uint32_t visible_count = 0;
-        for (uint32_t i = 0; i < face_count; ++i) {
-            if (dot3(faces[i].normal, sub(probe, polytope[faces[i].a].w)) > 1e-6f) visible[visible_count++] = i;
+        uint64_t seen_masks[2] = {0u, 0u};
+        for (uint32_t base = 0; base < face_count; base += kWave) {
+            const uint32_t i = base + lane;
+            bool vis = false;
+            if (i < face_count) {
+                const EpaFace f = e.faces[i];
+                vis = dot3(face_normal(f), sub(probe, lw(e, f.a))) > 1e-6f;
+            }
+            const uint64_t mask = __ballot(vis);
+            seen_masks[base / kWave] = mask;
+            if (vis) e.visible[visible_count + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+            visible_count += __popcll(mask);
         }
+        wave_sync();
+        // horizon edges of the visible faces, in (face, edge) order; the first 64 that make a
+        // proper face become new faces
+        const uint32_t apex = vertex_count - 1u;
         uint32_t new_count = 0;
-        for (uint32_t i = 0; i < visible_count; ++i) {
-            const EpaFace face = faces[visible[i]];
-            for (uint32_t e = 0; e < 3; ++e) {
-                const uint32_t start = e == 0u ? face.a : (e == 1u ? face.b : face.c);
-                const uint32_t end = e == 0u ? face.b : (e == 1u ? face.c : face.a);
+        const uint32_t items = visible_count * 3u;
+        for (uint32_t base = 0; base < items && new_count < 64u; base += kWave) {
+            const uint32_t t = base + lane;
+            bool ok = false;
+            EpaFace made;
+            if (t < items) {
+                const uint32_t i = t / 3u, edge = t % 3u;
+                const EpaFace face = e.faces[e.visible[i]];
+                const uint32_t start = edge == 0u ? face.a : (edge == 1u ? face.b : face.c);
+                const uint32_t end = edge == 0u ? face.b : (edge == 1u ? face.c : face.a);
                 bool shared = false;
                 for (uint32_t j = 0; j < visible_count; ++j) {
                     if (j == i) continue;
-                    const EpaFace& o = faces[visible[j]];
+                    const EpaFace o = e.faces[e.visible[j]];
                     if ((o.a == end && o.b == start) || (o.b == end && o.c == start) || (o.c == end && o.a == start)) {
                         shared = true;
                     }
                 }
-                if (!shared && new_count < 64u) {
-                    const uint32_t apex = vertex_count - 1u;
-                    const v3 normal = cross3(sub(polytope[end].w, polytope[apex].w),
-                                             sub(polytope[start].w, polytope[apex].w));
+                if (!shared) {
+                    const v3 normal = cross3(sub(lw(e, end), lw(e, apex)), sub(lw(e, start), lw(e, apex)));
                     const float length_n = len3(normal);
                     if (length_n > 1e-10f) {
                         const v3 unit = dv(normal, length_n);
-                        const v3 centroid =
-                            mul(add(add(polytope[apex].w, polytope[start].w), polytope[end].w), 1.0f / 3.0f);
-                        new_faces[new_count++] = epa_face(apex, start, end, unit, centroid);
+                        const v3 centroid = mul(add(add(lw(e, apex), lw(e, start)), lw(e, end)), 1.0f / 3.0f);
+                        made = epa_face(apex, start, end, unit, centroid);
+                        ok = true;
                     }
                 }
             }
+            const uint64_t mask = __ballot(ok);
+            const uint32_t rank = new_count + __popcll(mask & ((1ull << lane) - 1ull));
+            if (ok && rank < 64u) e.fresh[rank] = made;
+            new_count = min(new_count + static_cast<uint32_t>(__popcll(mask)), 64u);
         }
-        uint32_t write = 0, seen = 0;
-        for (uint32_t i = 0; i < face_count; ++i) {
-            if (seen < visible_count && visible[seen] == i) {
-                ++seen;
-                continue;
+        wave_sync();
+        // drop the visible faces, keeping the order of the rest
+        EpaFace kept[2];
+        uint32_t kept_at[2];
+        bool keep[2] = {false, false};
+        uint32_t write = 0;
+        for (uint32_t k = 0; k < 2; ++k) {
+            const uint32_t i = k * kWave + lane;
+            if (k * kWave >= face_count) break;
+            const bool mine = i < face_count && !((seen_masks[k] >> lane) & 1ull);
+            const uint64_t mask = __ballot(mine);
+            if (mine) {
+                kept[k] = e.faces[i];
+                kept_at[k] = write + __popcll(mask & ((1ull << lane) - 1ull));
+                keep[k] = true;
             }
-            faces[write++] = faces[i];
+            write += __popcll(mask);
+        }
+        wave_sync();
+        for (uint32_t k = 0; k < 2; ++k) {
+            if (keep[k]) e.faces[kept_at[k]] = kept[k];
         }
         face_count = write;
+        wave_sync();
         if (face_count + new_count > 128u) break;
-        for (uint32_t i = 0; i < new_count; ++i) faces[face_count++] = new_faces[i];
+        for (uint32_t i = lane; i < new_count; i += kWave) e.faces[face_count + i] = e.fresh[i];
+        face_count += new_count;
+        wave_sync();
     }
-    best_dist = kBig;
-    for (uint32_t i = 0; i < face_count; ++i) {
-        const float dist = dot3(faces[i].normal, polytope[faces[i].a].w);
-        if (dist < best_dist && len3(faces[i].normal) > 0.5f) {
-            best_dist = dist;
-            best_face = i;
-        }
-    }
+    epa_best_face(e, face_count, &best_dist, &best_face);
     Epa r;
     r.valid = best_dist < kBig;
-    r.normal = faces[best_face].normal;
+    r.normal = face_normal(e.faces[best_face]);
     r.depth = best_dist;
-    r.point = polytope[faces[best_face].a].a;
+    r.point = zero3v();  // the support points of the polytope are not kept (never read)
     return r;
 }
 
@@ -500,23 +610,30 @@ __device__ inline Hit make_hit(float distance, v3 point, v3 normal) {
     return h;
 }
 
-__device__ inline Hit convex_hit(const Shape& first, const Shape& second, const ShapeSet& ss) {
-    SimplexPoint simplex[4];
-    uint32_t count = 0;
-    const Closest closest = convex_closest(first, second, ss, simplex, &count);
+// convex_hit (convex.wgsl:584-618), GJK half: answers every pair GJK does not find penetrating
+// (returns true with the hit); for a penetrating pair returns false with GJK's final simplex,
+// which EPA (one wave) and convex_hit_finish take up.
+__device__ inline bool convex_hit_gjk(const Shape& first, const Shape& second, const ShapeSet& ss, Hit* hit,
+                                      SimplexPoint* simplex, uint32_t* count) {
+    const Closest closest = convex_closest(first, second, ss, simplex, count);
+    if (closest.penetrating) return false;
     const v3 centers = sub(second.center, first.center);
-    if (!closest.penetrating) {
-        const Epa probe = penetration_probe(first, second, ss, simplex, count);
-        if (probe.depth > 0.0f) {
-            const v3 normal = dot3(probe.normal, centers) < 0.0f ? neg(probe.normal) : probe.normal;
-            return make_hit(-probe.depth, mul(add(closest.point_a, closest.point_b), 0.5f), normal);
-        }
-        if (closest.distance > 0.0f) {
-            return make_hit(closest.distance, mul(add(closest.point_a, closest.point_b), 0.5f), closest.normal);
-        }
-        return make_hit(kBig, zero3v(), zero3v());  // NO_HIT
+    const Epa probe = penetration_probe(first, second, ss, simplex, *count);
+    if (probe.depth > 0.0f) {
+        const v3 normal = dot3(probe.normal, centers) < 0.0f ? neg(probe.normal) : probe.normal;
+        *hit = make_hit(-probe.depth, mul(add(closest.point_a, closest.point_b), 0.5f), normal);
+    } else if (closest.distance > 0.0f) {
+        *hit = make_hit(closest.distance, mul(add(closest.point_a, closest.point_b), 0.5f), closest.normal);
+    } else {
+        *hit = make_hit(kBig, zero3v(), zero3v());  // NO_HIT
     }
-    const Epa epa = epa_tetrahedron(first, second, ss, simplex, count);
+    return true;
+}
+
+// convex_hit, the rest of a penetrating pair once EPA has run on the simplex.
+__device__ inline Hit convex_hit_finish(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                        const SimplexPoint* simplex, uint32_t count, const Epa& epa) {
+    const v3 centers = sub(second.center, first.center);
     Hit result;
     if (epa.valid && epa.depth > 0.0f && epa.depth < kBig && len3(epa.normal) > 0.5f &&
         epa.depth < support_projection_depth(first, second, ss, epa.normal) +

@@ -352,8 +352,29 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
 }
 
-// Pairs marked by k_narrow: planes against hulls and cylinders, and GJK/EPA with clipped
-// feature manifolds (narrowphase.wgsl:551-664).
+// Whether two shapes' unswept bounds are apart by more than the margin along some axis. Then
+// the reference makes no contact: the axis probe of convex_hit sees the gap, and GJK's distance
+// is at least the gap (convex.wgsl:584-604); the 1 mm slack covers the rounding of the bounds.
+__device__ __forceinline__ bool bounds_apart(const Shape& a, const Shape& b, const ShapeSet& ss, float margin) {
+    v3 amn, amx, bmn, bmx;
+    world_aabb_of(a, ss, &amn, &amx);
+    world_aabb_of(b, ss, &bmn, &bmx);
+    const float reach = margin + 1e-3f;
+    return bmn.x - amx.x > reach || amn.x - bmx.x > reach || bmn.y - amx.y > reach ||
+           amn.y - bmx.y > reach || bmn.z - amx.z > reach || amn.z - bmx.z > reach;
+}
+
+// A penetrating convex pair's GJK simplex and EPA result, parked in the pair's raw contact slot
+// (which only its final contact overwrites) between k_narrow_convex, k_narrow_epa and
+// k_narrow_finish: floats 0-11 the four Minkowski points, 12 the EPA validity, 13-15 its normal,
+// 16 its depth.
+__device__ __forceinline__ float* parked(const W& w, uint32_t index) {
+    return reinterpret_cast<float*>(w.raw + index);
+}
+
+// GJK pairs marked 2 by k_narrow (narrowphase.wgsl:551-664), one thread per pair: planes against
+// hulls and cylinders, separated pairs, and touching pairs GJK does not find penetrating.
+// Penetrating pairs park their simplex and are marked 3 for k_narrow_epa.
 __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
@@ -377,13 +398,94 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
         mirror(m);
         mirrored_features = true;
     } else {
-        const Hit hit = convex_hit(wf, wsd, shapes);
+        if (bounds_apart(wf, wsd, shapes, margin)) return;
+        Hit hit;
+        SimplexPoint simplex[4];
+        uint32_t count = 0;
+        if (!convex_hit_gjk(wf, wsd, shapes, &hit, simplex, &count)) {
+            float* park = parked(w, index);
+            for (uint32_t i = 0; i < 4; ++i) {
+                park[3 * i] = simplex[i].w.x;
+                park[3 * i + 1] = simplex[i].w.y;
+                park[3 * i + 2] = simplex[i].w.z;
+            }
+            w.valid[index] = 3u;
+            return;
+        }
         if (!(hit.distance <= margin)) return;
         emit(m, hit.normal);
         if (!convex_pair_manifold(wf, wsd, shapes, hit.normal, m)) manifold_from_hit(m, hit, margin);
     }
     if (m.count == 0u) return;
     write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
+}
+
+constexpr uint32_t kEpaWaves = 4;  // waves per workgroup of k_narrow_epa
+
+// EPA of the pairs marked 3, one wave per pair: tasks of 64 candidate pairs, a wave working
+// through the marked pairs of its task one after another. Lane 0 parks the result.
+__global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
+    __shared__ EpaShared epa[kEpaWaves];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    const uint32_t waves = gridDim.x * kEpaWaves;
+    const ShapeSet shapes{w.sources, w.vertices};
+    for (uint32_t task = blockIdx.x * kEpaWaves + wave; task * kWave < live; task += waves) {
+        const uint32_t mine = task * kWave + lane;
+        uint64_t todo = __ballot(mine < live && w.valid[mine] == 3u);
+        while (todo != 0u) {
+            const uint32_t bit = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(todo))) - 1u;
+            todo &= todo - 1u;
+            const uint32_t index = task * kWave + bit;
+            const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+            const Shape wf = world_collider(w.st[w.owner[fs]], w.co[fs]);
+            const Shape wsd = world_collider(w.st[w.owner[ss]], w.co[ss]);
+            float* park = parked(w, index);
+            SimplexPoint simplex[4];
+            for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
+            const Epa r = epa_tetrahedron(wf, wsd, shapes, simplex, 4u, epa[wave]);
+            if (lane == 0u) {
+                park[12] = r.valid ? 1.0f : 0.0f;
+                park[13] = r.normal.x;
+                park[14] = r.normal.y;
+                park[15] = r.normal.z;
+                park[16] = r.depth;
+                w.valid[index] = 4u;
+            }
+        }
+    }
+}
+
+// The penetrating pairs after EPA (marked 4): the rest of convex_hit and the clipped manifold.
+__global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    if (index >= live || w.valid[index] != 4u) return;
+    w.valid[index] = 0u;
+    const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+    const uint32_t fb = w.owner[fs], sb = w.owner[ss];
+    const dyn_collider fc = w.co[fs], sc = w.co[ss];
+    const Body first = load_body(w, fb), second = load_body(w, sb);
+    const float margin = pair_margin(w, first, second);
+    const ShapeSet shapes{w.sources, w.vertices};
+    const Shape wf = world_collider(first.s, fc), wsd = world_collider(second.s, sc);
+    const float* park = parked(w, index);
+    SimplexPoint simplex[4];
+    for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
+    Epa epa;
+    epa.valid = park[12] != 0.0f;
+    epa.normal = mk(park[13], park[14], park[15]);
+    epa.depth = park[16];
+    epa.point = zero3v();
+    const Hit hit = convex_hit_finish(wf, wsd, shapes, simplex, 4u, epa);
+    if (!(hit.distance <= margin)) return;
+    Manifold m;
+    m.count = 0;
+    emit(m, hit.normal);
+    if (!convex_pair_manifold(wf, wsd, shapes, hit.normal, m)) manifold_from_hit(m, hit, margin);
+    if (m.count == 0u) return;
+    write_contact(w, index, fs, ss, first, second, fc, sc, m, false);
 }
 
 // ---- stable flag compaction: per-block counts, one-workgroup scan, destination map ---------
@@ -617,12 +719,18 @@ __global__ void __launch_bounds__(kThreads) k_island_init(W w) {
     w.istate[i] = 0u;
 }
 
+// The root of a node, halving the path on the way (each visited node is pointed at its
+// grandparent). Parents only ever move to an ancestor and a root only to a lower root, so the
+// components, their minimum-row roots and the flattened parents after the pointer-jumping
+// rounds are those of the plain walk.
 __device__ __forceinline__ uint32_t island_find(const W& w, uint32_t node) {
     uint32_t cur = node;
     uint32_t parent = atomic_load_agent(&w.parents[cur]);
     while (parent != cur) {
+        const uint32_t grand = atomic_load_agent(&w.parents[parent]);
+        if (grand != parent) atomic_store_agent(&w.parents[cur], grand);
         cur = parent;
-        parent = atomic_load_agent(&w.parents[cur]);
+        parent = grand;
     }
     return cur;
 }
@@ -678,7 +786,10 @@ __global__ void __launch_bounds__(kThreads) k_island_aggregate(W w) {
     uint32_t roots = 0;
     if (i < w.d) {
         const uint32_t root = w.parents[i];
-        if (w.wake[i] != 0u || w.motion[i] != 0u) atomicMax(&w.istate[root], kIslandWake);
+        // one island holds many rows: skip the atomic once the flag is there
+        if ((w.wake[i] != 0u || w.motion[i] != 0u) && !(atomic_load_agent(&w.istate[root]) & kIslandWake)) {
+            atomicMax(&w.istate[root], kIslandWake);
+        }
         roots = root == i ? 1u : 0u;
     }
     roots = wave_sum(roots);
@@ -1331,7 +1442,17 @@ int world_step(dyn_world* world, float dt) {
     io.scratch_bytes = world->bp_scratch_bytes;
     DYN_TRY(dyn_broadphase(st, &io));
     LAUNCH(k_narrow, pb, w);
-    if (world->convex_colliders) LAUNCH(k_narrow_convex, pb, w);
+    if (world->convex_colliders) {
+        LAUNCH(k_narrow_convex, pb, w);
+        const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
+        const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
+        {
+            ProfileScope scope("k_narrow_epa", st);
+            k_narrow_epa<<<groups, kEpaWaves * kWave, 0, st>>>(w);
+        }
+        DYN_LAUNCHED("k_narrow_epa");
+        LAUNCH(k_narrow_finish, pb, w);
+    }
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
     DYN_TRY(scan(world, w.valid, pair_count, w.pair_cap, w.dest, &w.scan_total[0]));
     LAUNCH(k_contacts_scatter, pb, w);
@@ -1400,7 +1521,7 @@ extern "C" {
 int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* params,
                      const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
                      uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
-                     uint32_t collider_count, uint32_t contact_capacity) {
+                     uint32_t collider_count, uint32_t contact_capacity, uint32_t pair_capacity) {
     using namespace dyn;
     if (!out || !params || (body_count && (!states || !descs)) ||
         (collider_count && (!colliders || !owners))) {
@@ -1460,7 +1581,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     w.n = body_count;
     w.d = dynamic_count;
     w.k = collider_count;
-    w.pair_cap = contact_capacity;
+    w.pair_cap = pair_capacity;
     w.contact_cap = contact_capacity;
     world->entry_cap = DYN_MAX_CELLS_PER_COLLIDER * (collider_count ? collider_count : 1u);
     int rc = DYN_OK;
@@ -1477,11 +1598,11 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.aabbs, collider_count);
     A(w.res, collider_count);
     A(w.flags, collider_count);
-    A(w.pmaj, contact_capacity);
-    A(w.pmin, contact_capacity);
-    A(w.raw, contact_capacity);
-    A(w.valid, contact_capacity);
-    A(w.dest, contact_capacity);
+    A(w.pmaj, pair_capacity);
+    A(w.pmin, pair_capacity);
+    A(w.raw, pair_capacity);
+    A(w.valid, pair_capacity);
+    A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
     A(w.matched, contact_capacity);
     A(w.archive, contact_capacity);
@@ -1500,7 +1621,8 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.vdeltas, 8u * static_cast<size_t>(body_count));
     A(w.pdeltas, 8u * static_cast<size_t>(body_count));
     A(w.contrib, body_count);
-    A(w.scan_blocks, ceil_div(contact_capacity ? contact_capacity : 1u, kScanBlock) + 1u);
+    const uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
+    A(w.scan_blocks, ceil_div(scan_limit ? scan_limit : 1u, kScanBlock) + 1u);
     A(w.scan_total, 4);
     A(world->words, 8);
     A(world->entry_keys, world->entry_cap);
@@ -1514,7 +1636,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     w.ncontacts = world->words + 1;
     w.narchive = world->words + 2;
     w.nresting = world->words + 3;
-    world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(collider_count, world->entry_cap, contact_capacity);
+    world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(collider_count, world->entry_cap, pair_capacity);
     world->sort_scratch_bytes = dyn_sort_scratch_bytes(contact_capacity, 2u * dyn_sort_key_words(id_cap ? id_cap : 1u));
     if (hipMalloc(&world->bp_scratch, world->bp_scratch_bytes) != hipSuccess ||
         hipMalloc(&world->sort_scratch, world->sort_scratch_bytes) != hipSuccess) {

@@ -222,13 +222,23 @@ def encode(bodies: list[BodyDesc], config: np.ndarray | None = None,
                  handles=np.asarray(order, dtype=np.uint32), sources=sources, vertices=vertices)
 
 
+def default_capacities(bodies: int, contacts: int | None = None,
+                       pairs: int | None = None) -> tuple[int, int]:
+    """Per-step stream capacities: 8 contacts and 32 candidate pairs per body unless given."""
+    c = int(contacts if contacts is not None else max(64, 8 * bodies))
+    p = int(pairs if pairs is not None else max(256, 32 * bodies))
+    return c, p
+
+
 class World:
     """A device world. Bodies are spawned before the first step; `step(dt)` enqueues one step
     on the world's stream, reads wait for it."""
 
-    def __init__(self, config: np.ndarray | None = None, contact_capacity: int | None = None):
+    def __init__(self, config: np.ndarray | None = None, contact_capacity: int | None = None,
+                 pair_capacity: int | None = None):
         self.config = default_config() if config is None else config.copy()
         self.contact_capacity = contact_capacity
+        self.pair_capacity = pair_capacity
         self._bodies: list[BodyDesc] = []
         self._hulls: list[HullSource] = []
         self._handle = ctypes.c_void_p()
@@ -257,17 +267,19 @@ class World:
         if not self._bodies:
             raise RuntimeError("a world needs at least one body")
         self.scene = encode(self._bodies, self.config, self._hulls)
-        self.create(self.scene, self.contact_capacity)
+        self.create(self.scene, self.contact_capacity, self.pair_capacity)
 
-    def create(self, scene: Scene, contact_capacity: int | None = None) -> None:
+    def create(self, scene: Scene, contact_capacity: int | None = None,
+               pair_capacity: int | None = None) -> None:
         """Creates the device world from already-encoded records."""
         self.scene = scene
         n = len(scene.states)
-        capacity = int(contact_capacity if contact_capacity is not None else max(64, 8 * n))
+        capacity, pairs = default_capacities(n, contact_capacity, pair_capacity)
         call("dyn_world_create", ctypes.byref(self._handle), None, scene.params.ctypes.data,
              scene.states.ctypes.data, scene.descs.ctypes.data, n, scene.dynamic_count,
-             scene.colliders.ctypes.data, scene.owners.ctypes.data, n, capacity)
+             scene.colliders.ctypes.data, scene.owners.ctypes.data, n, capacity, pairs)
         self.capacity = capacity
+        self.pair_capacity = pairs
         if scene.sources is not None and len(scene.sources):
             call("dyn_world_upload_shapes", self._handle, scene.sources.ctypes.data,
                  len(scene.sources), scene.vertices.ctypes.data, len(scene.vertices))
@@ -319,11 +331,11 @@ class World:
     def pairs(self) -> np.ndarray:
         """Sorted candidate pairs of the last step as (P, 2) collider indices."""
         self._ensure()
-        major = np.zeros(self.capacity, dtype=np.uint32)
-        minor = np.zeros(self.capacity, dtype=np.uint32)
+        major = np.zeros(self.pair_capacity, dtype=np.uint32)
+        minor = np.zeros(self.pair_capacity, dtype=np.uint32)
         count = ctypes.c_uint32()
         call("dyn_world_read_pairs", self._handle, major.ctypes.data, minor.ctypes.data,
-             self.capacity, ctypes.byref(count))
+             self.pair_capacity, ctypes.byref(count))
         return np.stack([major[:count.value], minor[:count.value]], axis=1)
 
     def stream(self) -> int:

@@ -233,12 +233,14 @@ typedef struct dyn_world dyn_world;
  * handle id; the world keeps the id -> row table of body_row.wgsl), colliders in collider-slot
  * order with the rows of their owners (upload.rs:133-175). `params` is the StepParams record (dt and substep_dt are set by each step).
  * `stream` (hipStream_t) orders all of the world's work; NULL creates a private stream.
- * contact_capacity bounds candidate pairs, contacts and resting manifolds.
+ * contact_capacity bounds contacts and resting manifolds per step, pair_capacity the candidate
+ * pairs (the reference's stream capacities; overflow is counted in REFUSED_CONTACTS,
+ * REFUSED_RESTING and REFUSED_PAIRS).
  */
 int dyn_world_create(dyn_world** world, void* stream, const dyn_step_params* params,
                      const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
                      uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
-                     uint32_t collider_count, uint32_t contact_capacity);
+                     uint32_t collider_count, uint32_t contact_capacity, uint32_t pair_capacity);
 int dyn_world_destroy(dyn_world* world);
 /* Uploads the hull shape sources colliders name in their `source` field, with their vertices as
  * float4 rows (x, y, z, 0) (the reference's shape pool flush, crates/dynamis-world/src/

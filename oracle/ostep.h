@@ -23,7 +23,7 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
                                   const dyn_body_desc* descs, uint32_t body_count,
                                   uint32_t dynamic_count, const dyn_collider* colliders,
                                   const uint32_t* owners, uint32_t collider_count,
-                                  uint32_t contact_capacity);
+                                  uint32_t contact_capacity, uint32_t pair_capacity);
 void oracle_world_destroy(oracle_world* w);
 /* Hull shape sources (records.wgsl ShapeSource) and their vertices as float4 rows; 0 on success,
  * -1 when a source exceeds the linear support bound or its vertex range. */

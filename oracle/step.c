@@ -2518,7 +2518,7 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
                                   const dyn_body_desc* descs, uint32_t body_count,
                                   uint32_t dynamic_count, const dyn_collider* colliders,
                                   const uint32_t* owners, uint32_t collider_count,
-                                  uint32_t contact_capacity) {
+                                  uint32_t contact_capacity, uint32_t pair_capacity) {
     oracle_world* w = calloc(1, sizeof *w);
     w->p = *params;
     w->n = body_count;
@@ -2545,7 +2545,7 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
     ALLOC(w->activity, body_count);
     ALLOC(w->motion, body_count);
     w->entry_cap = 8u * (collider_count ? collider_count : 1u);
-    w->pair_cap = contact_capacity;
+    w->pair_cap = pair_capacity;
     w->contact_cap = contact_capacity;
     ALLOC(w->aabbs, collider_count);
     ALLOC(w->res, collider_count);

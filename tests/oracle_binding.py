@@ -49,7 +49,7 @@ class Oracle:
         h.oracle_broadphase.argtypes = [_p, _p, _p, _p, _u32, _p, _p, _p, _u32, _p, _p, _u32,
                                         _p, _u32]
         h.oracle_world_create.restype = _p
-        h.oracle_world_create.argtypes = [_p, _p, _p, _u32, _u32, _p, _p, _u32, _u32]
+        h.oracle_world_create.argtypes = [_p, _p, _p, _u32, _u32, _p, _p, _u32, _u32, _u32]
         h.oracle_world_destroy.restype = None
         h.oracle_world_destroy.argtypes = [_p]
         h.oracle_world_step.restype = None
@@ -147,23 +147,24 @@ class Oracle:
                 "normal": out[7:10].copy(), "penetrating": bool(out[10]),
                 "hit_distance": float(out[11]), "hit_normal": out[12:15].copy()}
 
-    def world(self, scene, contact_capacity=None) -> "OracleWorld":
-        return OracleWorld(self, scene, contact_capacity)
+    def world(self, scene, contact_capacity=None, pair_capacity=None) -> "OracleWorld":
+        return OracleWorld(self, scene, contact_capacity, pair_capacity)
 
 
 class OracleWorld:
     """The oracle's serial restatement of World::step over the same device records."""
 
-    def __init__(self, oracle: Oracle, scene, contact_capacity=None):
+    def __init__(self, oracle: Oracle, scene, contact_capacity=None, pair_capacity=None):
+        from dynamis_amd.world import default_capacities
         self.o = oracle
         n = len(scene.states)
         self.n = n
-        self.capacity = int(contact_capacity if contact_capacity is not None else max(64, 8 * n))
+        self.capacity, self.pair_capacity = default_capacities(n, contact_capacity, pair_capacity)
         self.scene = scene
         self.w = oracle.h.oracle_world_create(
             scene.params.ctypes.data, scene.states.ctypes.data, scene.descs.ctypes.data, n,
             scene.dynamic_count, scene.colliders.ctypes.data, scene.owners.ctypes.data, n,
-            self.capacity)
+            self.capacity, self.pair_capacity)
         if not self.w:
             raise MemoryError("oracle_world_create failed")
         if scene.sources is not None and len(scene.sources):
@@ -191,11 +192,11 @@ class OracleWorld:
         return out[:min(count, self.capacity)].copy()
 
     def pairs(self) -> np.ndarray:
-        major = np.zeros(self.capacity, np.uint32)
-        minor = np.zeros(self.capacity, np.uint32)
+        major = np.zeros(self.pair_capacity, np.uint32)
+        minor = np.zeros(self.pair_capacity, np.uint32)
         count = self.o.h.oracle_world_pairs(self.w, major.ctypes.data, minor.ctypes.data,
-                                            self.capacity)
-        count = min(count, self.capacity)
+                                            self.pair_capacity)
+        count = min(count, self.pair_capacity)
         return np.stack([major[:count], minor[:count]], axis=1)
 
     def close(self) -> None:
