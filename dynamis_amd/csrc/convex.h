@@ -192,6 +192,7 @@ __device__ __forceinline__ void tet_face(uint32_t f, uint32_t* a, uint32_t* b, u
 }
 
 __device__ inline bool tetra_contains(const SimplexPoint* t, v3 o) {
+#pragma unroll
     for (uint32_t f = 0; f < 4; ++f) {
         uint32_t ia, ib, ic;
         tet_face(f, &ia, &ib, &ic);
@@ -204,32 +205,63 @@ __device__ inline bool tetra_contains(const SimplexPoint* t, v3 o) {
     return true;
 }
 
+// The simplex lives in four fixed slots that every loop below addresses with indices known at
+// compile time (the loops are unrolled over the four slots and guarded by the live count), so
+// it stays in registers; the iterations run in the sequential order of the reference.
+__device__ __forceinline__ v3 pick3(bool take, v3 a, v3 b) {
+    return mk(take ? a.x : b.x, take ? a.y : b.y, take ? a.z : b.z);
+}
+
+__device__ __forceinline__ SimplexPoint pick_point(bool take, const SimplexPoint& a, const SimplexPoint& b) {
+    SimplexPoint r;
+    r.w = pick3(take, a.w, b.w);
+    r.a = pick3(take, a.a, b.a);
+    r.b = pick3(take, a.b, b.b);
+    return r;
+}
+
+// s[at] = p, as a select in every slot (a store under a branch is merged into a store through a
+// selected pointer, which would keep the simplex in memory).
+__device__ __forceinline__ void set_slot(SimplexPoint* s, uint32_t at, const SimplexPoint& p) {
+#pragma unroll
+    for (uint32_t t = 0; t < 4; ++t) s[t] = pick_point(t == at, p, s[t]);
+}
+
 __device__ inline bool simplex_holds(const SimplexPoint* s, uint32_t count, v3 point, float span) {
     const float tolerance = (1e-5f * span) * (1e-5f * span);
-    for (uint32_t i = 0; i < count; ++i) {
-        const v3 offset = sub(s[i].w, point);
-        if (dot3(offset, offset) <= tolerance) return true;
+    bool held = false;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (i < count) {
+            const v3 offset = sub(s[i].w, point);
+            held = held || dot3(offset, offset) <= tolerance;
+        }
     }
-    return false;
+    return held;
 }
 
 __device__ inline uint32_t simplex_reduce(SimplexPoint* s, const float* lambdas) {
-    SimplexPoint kept[4];
+    SimplexPoint kept[4] = {s[0], s[1], s[2], s[3]};
     uint32_t count = 0;
+#pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
-        if (lambdas[i] > 1e-6f) kept[count++] = s[i];
+        const bool take = lambdas[i] > 1e-6f;
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) kept[t] = pick_point(take && t == count, s[i], kept[t]);
+        count += take ? 1u : 0u;
     }
-    if (count == 0u) {
-        kept[0] = s[0];
+    if (count == 0u) {  // kept[0] is still s[0]
         count = 1;
     }
-    for (uint32_t i = 0; i < count; ++i) s[i] = kept[i];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) s[i] = pick_point(i < count, kept[i], s[i]);
     return count;
 }
 
 __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t count) {
     SimplexResult r;
     r.v = mk(kBig, kBig, kBig);
+#pragma unroll
     for (int i = 0; i < 4; ++i) r.lambdas[i] = 0.0f;
     r.penetrating = count == 4u && tetra_contains(s, zero3v());
     r.point_a = zero3v();
@@ -238,15 +270,20 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
         r.v = zero3v();
         return r;
     }
-    for (uint32_t i = 0; i < count; ++i) {
-        if (len3(s[i].w) < len3(r.v)) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (i < count && len3(s[i].w) < len3(r.v)) {
             r.v = s[i].w;
+#pragma unroll
             for (int k = 0; k < 4; ++k) r.lambdas[k] = 0.0f;
             r.lambdas[i] = 1.0f;
         }
     }
-    for (uint32_t i = 0; i < count; ++i) {
-        for (uint32_t j = i + 1; j < count; ++j) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+#pragma unroll
+        for (uint32_t j = i + 1; j < 4; ++j) {
+            if (j >= count) continue;
             const v3 a = s[i].w, b = s[j].w;
             const v3 ab = sub(b, a);
             const float denom = dot3(ab, ab);
@@ -255,6 +292,7 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
             const v3 point = add(a, mul(ab, t));
             if (len3(point) < len3(r.v)) {
                 r.v = point;
+#pragma unroll
                 for (int k = 0; k < 4; ++k) r.lambdas[k] = 0.0f;
                 r.lambdas[i] = 1.0f - t;
                 r.lambdas[j] = t;
@@ -262,13 +300,18 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
         }
     }
     if (count >= 3u) {
-        for (uint32_t i = 0; i < count; ++i) {
-            for (uint32_t j = i + 1; j < count; ++j) {
-                for (uint32_t k = j + 1; k < count; ++k) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+#pragma unroll
+            for (uint32_t j = i + 1; j < 4; ++j) {
+#pragma unroll
+                for (uint32_t k = j + 1; k < 4; ++k) {
+                    if (k >= count) continue;
                     v3 weights;
                     const v3 point = closest_on_triangle(zero3v(), s[i].w, s[j].w, s[k].w, &weights);
                     if (len3(point) < len3(r.v)) {
                         r.v = point;
+#pragma unroll
                         for (int q = 0; q < 4; ++q) r.lambdas[q] = 0.0f;
                         r.lambdas[i] = weights.x;
                         r.lambdas[j] = weights.y;
@@ -278,9 +321,12 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
             }
         }
     }
-    for (uint32_t i = 0; i < count; ++i) {
-        r.point_a = add(r.point_a, smul(r.lambdas[i], s[i].a));
-        r.point_b = add(r.point_b, smul(r.lambdas[i], s[i].b));
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (i < count) {
+            r.point_a = add(r.point_a, smul(r.lambdas[i], s[i].a));
+            r.point_b = add(r.point_b, smul(r.lambdas[i], s[i].b));
+        }
     }
     return r;
 }
@@ -331,7 +377,7 @@ __device__ inline Closest convex_closest(const Shape& first, const Shape& second
         if (closest.penetrating || len3(closest.v) < tiny) {
             while (count < 4u) {
                 const v3 d = simplex_expand_dir(simplex, count);
-                simplex[count] = support_pair(first, second, ss, d);
+                set_slot(simplex, count, support_pair(first, second, ss, d));
                 ++count;
             }
             r.penetrating = true;
@@ -357,7 +403,7 @@ __device__ inline Closest convex_closest(const Shape& first, const Shape& second
             if (r.distance > 1e-6f) r.normal = sign_normalize(sub(closest.point_b, closest.point_a));
             break;
         }
-        simplex[count] = sp;
+        set_slot(simplex, count, sp);
         ++count;
         closest = simplex_closest(simplex, count);
     }
@@ -566,39 +612,54 @@ __device__ __forceinline__ float support_projection_depth(const Shape& a, const 
     return dot3(sub(support(a, ss, n), support(b, ss, neg(n))), n);
 }
 
-__device__ inline Epa penetration_probe(const Shape& first, const Shape& second, const ShapeSet& ss,
-                                        const SimplexPoint* s, uint32_t count) {
-    Epa best;
-    best.valid = false;
-    best.normal = mk(0.0f, 1.0f, 0.0f);
+struct Probe {
+    float depth;
+    v3 normal;
+};
+
+__device__ __forceinline__ void probe_direction(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                                v3 direction, Probe* best) {
+    const float depth = support_projection_depth(first, second, ss, direction);
+    if (depth < best->depth) {
+        best->depth = depth;
+        best->normal = norm3(direction);
+    }
+}
+
+// The smallest support-projection overlap over the probe directions, in the reference's order:
+// the normals of the simplex's triangles, the centre line, then +-x, +-y, +-z (at most 11 of
+// the 16 the reference allows, so its cap never binds). The directions are evaluated as they
+// are generated. With `until_separated` the scan stops at the first direction whose overlap is
+// not positive: the depth is then not positive either, which is all the GJK half of convex_hit
+// reads of it (the normal of such a probe is never used).
+__device__ inline Probe penetration_probe(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                          const SimplexPoint* s, uint32_t count, bool until_separated) {
+    Probe best;
     best.depth = kBig;
-    best.point = mul(add(first.center, second.center), 0.5f);
-    v3 probed[16];
-    uint32_t probe_count = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        for (uint32_t j = i + 1; j < count; ++j) {
-            for (uint32_t k = j + 1; k < count; ++k) {
-                const v3 normal = cross3(sub(s[j].w, s[i].w), sub(s[k].w, s[i].w));
-                if (len3(normal) > 1e-8f && probe_count < 16u) probed[probe_count++] = normal;
-            }
+    best.normal = mk(0.0f, 1.0f, 0.0f);
+    // triangles (i, j, k), i < j < k < count, in loop order: 012, 013, 023, 123
+    for (uint32_t t = 0; t < 4; ++t) {
+        if ((t == 0u ? 2u : 3u) >= count) break;
+        const v3 a = t == 3u ? s[1].w : s[0].w;
+        const v3 b = t < 2u ? s[1].w : s[2].w;
+        const v3 c = t == 0u ? s[2].w : s[3].w;
+        const v3 normal = cross3(sub(b, a), sub(c, a));
+        if (len3(normal) > 1e-8f) {
+            probe_direction(first, second, ss, normal, &best);
+            if (until_separated && !(best.depth > 0.0f)) return best;
         }
     }
     const v3 center_dir = sub(second.center, first.center);
-    if (len3(center_dir) > 1e-8f && probe_count < 16u) probed[probe_count++] = center_dir;
+    if (len3(center_dir) > 1e-8f) {
+        probe_direction(first, second, ss, center_dir, &best);
+        if (until_separated && !(best.depth > 0.0f)) return best;
+    }
     for (uint32_t i = 0; i < 6; ++i) {
         const float sgn = (i & 1u) ? -1.0f : 1.0f;
         const v3 axis = i < 2u ? mk(sgn, 0.0f, 0.0f) : (i < 4u ? mk(0.0f, sgn, 0.0f) : mk(0.0f, 0.0f, sgn));
-        if (probe_count < 16u) probed[probe_count++] = axis;
+        probe_direction(first, second, ss, axis, &best);
+        if (until_separated && !(best.depth > 0.0f)) return best;
     }
-    for (uint32_t i = 0; i < probe_count; ++i) {
-        const float depth = support_projection_depth(first, second, ss, probed[i]);
-        if (depth < best.depth) {
-            best.depth = depth;
-            best.normal = norm3(probed[i]);
-        }
-    }
-    best.valid = best.depth < kBig;
-    best.point = mul(add(support(first, ss, best.normal), support(second, ss, neg(best.normal))), 0.5f);
     return best;
 }
 
@@ -618,7 +679,7 @@ __device__ inline bool convex_hit_gjk(const Shape& first, const Shape& second, c
     const Closest closest = convex_closest(first, second, ss, simplex, count);
     if (closest.penetrating) return false;
     const v3 centers = sub(second.center, first.center);
-    const Epa probe = penetration_probe(first, second, ss, simplex, *count);
+    const Probe probe = penetration_probe(first, second, ss, simplex, *count, true);
     if (probe.depth > 0.0f) {
         const v3 normal = dot3(probe.normal, centers) < 0.0f ? neg(probe.normal) : probe.normal;
         *hit = make_hit(-probe.depth, mul(add(closest.point_a, closest.point_b), 0.5f), normal);
@@ -642,7 +703,7 @@ __device__ inline Hit convex_hit_finish(const Shape& first, const Shape& second,
                           mul(add(support(first, ss, epa.normal), support(second, ss, neg(epa.normal))), 0.5f),
                           epa.normal);
     } else {
-        const Epa probe = penetration_probe(first, second, ss, simplex, count);
+        const Probe probe = penetration_probe(first, second, ss, simplex, count, false);
         result = make_hit(-probe.depth,
                           mul(add(support(first, ss, probe.normal), support(second, ss, neg(probe.normal))), 0.5f),
                           probe.normal);

@@ -254,6 +254,16 @@ __device__ __forceinline__ float pair_margin(const W& w, const Body& first, cons
     return w.p.contact_margin;
 }
 
+// What a candidate pair's `valid` word says between the narrowphase kernels. Only kPairContact
+// survives them: every other mark is cleared by the kernel that takes the pair up.
+constexpr uint32_t kPairNone = 0;
+constexpr uint32_t kPairContact = 1;  // its contact is written in the pair's raw slot
+constexpr uint32_t kPairGjk = 2;      // convex pair for k_narrow_convex
+constexpr uint32_t kPairEpa = 3;      // GJK found it penetrating; simplex parked for k_narrow_epa
+constexpr uint32_t kPairEpaDone = 4;  // EPA result parked for k_narrow_finish
+constexpr uint32_t kPairHit = 5;      // GJK hit within the margin parked for k_narrow_finish
+constexpr uint32_t kPairPlane = 6;    // plane against a hull or cylinder, for k_narrow_finish
+
 // Material, identity and anchors of a generated manifold (narrowphase.wgsl:665-684).
 __device__ inline void write_contact(const W& w, uint32_t index, uint32_t fs, uint32_t ss, const Body& first,
                                      const Body& second, const dyn_collider& fc, const dyn_collider& sc,
@@ -304,21 +314,22 @@ __device__ inline void write_contact(const W& w, uint32_t index, uint32_t fs, ui
         mp.accumulated_tangent_2 = 0.0f;
     }
     w.raw[index] = c;
-    w.valid[index] = 1u;
+    w.valid[index] = kPairContact;
 }
 
-// Analytic pairs; pairs for the convex path are marked 2 for k_narrow_convex.
+// Analytic pairs; convex-path pairs are marked for the kernels below.
 __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
     if (index >= live) return;
-    w.valid[index] = 0u;
+    w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     uint32_t fb, sb;
     if (!pair_admitted(w, fs, ss, &fb, &sb)) return;
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
     if (convex_path(fc, sc)) {
-        w.valid[index] = 2u;
+        const bool plane = fc.kind == DYN_SHAPE_PLANE || sc.kind == DYN_SHAPE_PLANE;
+        w.valid[index] = plane ? kPairPlane : kPairGjk;
         return;
     }
     const Body first = load_body(w, fb), second = load_body(w, sb);
@@ -364,66 +375,64 @@ __device__ __forceinline__ bool bounds_apart(const Shape& a, const Shape& b, con
            amn.y - bmx.y > reach || bmn.z - amx.z > reach || amn.z - bmx.z > reach;
 }
 
-// A penetrating convex pair's GJK simplex and EPA result, parked in the pair's raw contact slot
-// (which only its final contact overwrites) between k_narrow_convex, k_narrow_epa and
-// k_narrow_finish: floats 0-11 the four Minkowski points, 12 the EPA validity, 13-15 its normal,
-// 16 its depth.
+// Intermediate results of a convex pair, parked in the pair's raw contact slot (which only its
+// final contact overwrites) between k_narrow_convex, k_narrow_epa and k_narrow_finish. A
+// penetrating pair: floats 0-11 GJK's four Minkowski points, 12 EPA's validity, 13-15 its
+// normal, 16 its depth. A hit within the margin: 0 its distance, 1-3 its point, 4-6 its normal.
 __device__ __forceinline__ float* parked(const W& w, uint32_t index) {
     return reinterpret_cast<float*>(w.raw + index);
 }
 
-// GJK pairs marked 2 by k_narrow (narrowphase.wgsl:551-664), one thread per pair: planes against
-// hulls and cylinders, separated pairs, and touching pairs GJK does not find penetrating.
-// Penetrating pairs park their simplex and are marked 3 for k_narrow_epa.
+// GJK of the pairs marked kPairGjk (convex_hit, convex.wgsl:584-618), one thread per pair.
+// Separated pairs end here; a hit within the margin is parked for k_narrow_finish and a
+// penetrating pair's simplex for k_narrow_epa.
 __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live || w.valid[index] != 2u) return;
-    w.valid[index] = 0u;
+    if (index >= live || w.valid[index] != kPairGjk) return;
+    w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
-    const Body first = load_body(w, fb), second = load_body(w, sb);
-    const float margin = pair_margin(w, first, second);
-    const ShapeSet shapes{w.sources, w.vertices};
-    const Shape wf = world_collider(first.s, fc), wsd = world_collider(second.s, sc);
-    Manifold m;
-    m.count = 0;
-    m.normal = zero3v();
-    bool mirrored_features = false;
-    if (fc.kind == DYN_SHAPE_PLANE) {
-        plane_convex(m, wf, wsd, shapes, margin);
-    } else if (sc.kind == DYN_SHAPE_PLANE) {
-        plane_convex(m, wsd, wf, shapes, margin);
-        mirror(m);
-        mirrored_features = true;
-    } else {
-        if (bounds_apart(wf, wsd, shapes, margin)) return;
-        Hit hit;
-        SimplexPoint simplex[4];
-        uint32_t count = 0;
-        if (!convex_hit_gjk(wf, wsd, shapes, &hit, simplex, &count)) {
-            float* park = parked(w, index);
-            for (uint32_t i = 0; i < 4; ++i) {
-                park[3 * i] = simplex[i].w.x;
-                park[3 * i + 1] = simplex[i].w.y;
-                park[3 * i + 2] = simplex[i].w.z;
-            }
-            w.valid[index] = 3u;
-            return;
-        }
-        if (!(hit.distance <= margin)) return;
-        emit(m, hit.normal);
-        if (!convex_pair_manifold(wf, wsd, shapes, hit.normal, m)) manifold_from_hit(m, hit, margin);
+    const dyn_body_state fst = w.st[fb], sst = w.st[sb];
+    float margin = w.p.contact_margin;
+    if ((w.de[fb].flags & DYN_BODY_CCD) || (w.de[sb].flags & DYN_BODY_CCD)) {
+        margin += len3(sub(ld3(sst.velocity), ld3(fst.velocity))) * w.p.dt;  // pair_margin
     }
-    if (m.count == 0u) return;
-    write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
+    const ShapeSet shapes{w.sources, w.vertices};
+    const Shape wf = world_collider(fst, fc), wsd = world_collider(sst, sc);
+    if (bounds_apart(wf, wsd, shapes, margin)) return;
+    Hit hit;
+    SimplexPoint simplex[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) simplex[i].w = simplex[i].a = simplex[i].b = zero3v();
+    uint32_t count = 0;
+    float* park = parked(w, index);
+    if (!convex_hit_gjk(wf, wsd, shapes, &hit, simplex, &count)) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            park[3 * i] = simplex[i].w.x;
+            park[3 * i + 1] = simplex[i].w.y;
+            park[3 * i + 2] = simplex[i].w.z;
+        }
+        w.valid[index] = kPairEpa;
+        return;
+    }
+    if (!(hit.distance <= margin)) return;
+    park[0] = hit.distance;
+    park[1] = hit.point.x;
+    park[2] = hit.point.y;
+    park[3] = hit.point.z;
+    park[4] = hit.normal.x;
+    park[5] = hit.normal.y;
+    park[6] = hit.normal.z;
+    w.valid[index] = kPairHit;
 }
 
 constexpr uint32_t kEpaWaves = 4;  // waves per workgroup of k_narrow_epa
 
-// EPA of the pairs marked 3, one wave per pair: tasks of 64 candidate pairs, a wave working
-// through the marked pairs of its task one after another. Lane 0 parks the result.
+// EPA of the pairs marked kPairEpa, one wave per pair: tasks of 64 candidate pairs, a wave
+// working through the marked pairs of its task one after another. Lane 0 parks the result.
 __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
     __shared__ EpaShared epa[kEpaWaves];
     const uint32_t lane = lane_id();
@@ -433,7 +442,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
     const ShapeSet shapes{w.sources, w.vertices};
     for (uint32_t task = blockIdx.x * kEpaWaves + wave; task * kWave < live; task += waves) {
         const uint32_t mine = task * kWave + lane;
-        uint64_t todo = __ballot(mine < live && w.valid[mine] == 3u);
+        uint64_t todo = __ballot(mine < live && w.valid[mine] == kPairEpa);
         while (todo != 0u) {
             const uint32_t bit = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(todo))) - 1u;
             todo &= todo - 1u;
@@ -443,6 +452,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
             const Shape wsd = world_collider(w.st[w.owner[ss]], w.co[ss]);
             float* park = parked(w, index);
             SimplexPoint simplex[4];
+#pragma unroll
             for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
             const Epa r = epa_tetrahedron(wf, wsd, shapes, simplex, 4u, epa[wave]);
             if (lane == 0u) {
@@ -451,18 +461,22 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
                 park[14] = r.normal.y;
                 park[15] = r.normal.z;
                 park[16] = r.depth;
-                w.valid[index] = 4u;
+                w.valid[index] = kPairEpaDone;
             }
         }
     }
 }
 
-// The penetrating pairs after EPA (marked 4): the rest of convex_hit and the clipped manifold.
+// The manifolds of the convex path (narrowphase.wgsl:551-664): planes against hulls and
+// cylinders, GJK hits within the margin, and penetrating pairs after EPA (the rest of
+// convex_hit first).
 __global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live || w.valid[index] != 4u) return;
-    w.valid[index] = 0u;
+    if (index >= live) return;
+    const uint32_t mark = w.valid[index];
+    if (mark != kPairEpaDone && mark != kPairHit && mark != kPairPlane) return;
+    w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
@@ -471,21 +485,39 @@ __global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
     const ShapeSet shapes{w.sources, w.vertices};
     const Shape wf = world_collider(first.s, fc), wsd = world_collider(second.s, sc);
     const float* park = parked(w, index);
-    SimplexPoint simplex[4];
-    for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
-    Epa epa;
-    epa.valid = park[12] != 0.0f;
-    epa.normal = mk(park[13], park[14], park[15]);
-    epa.depth = park[16];
-    epa.point = zero3v();
-    const Hit hit = convex_hit_finish(wf, wsd, shapes, simplex, 4u, epa);
-    if (!(hit.distance <= margin)) return;
     Manifold m;
     m.count = 0;
-    emit(m, hit.normal);
-    if (!convex_pair_manifold(wf, wsd, shapes, hit.normal, m)) manifold_from_hit(m, hit, margin);
+    m.normal = zero3v();
+    bool mirrored_features = false;
+    if (mark == kPairPlane) {
+        if (fc.kind == DYN_SHAPE_PLANE) {
+            plane_convex(m, wf, wsd, shapes, margin);
+        } else {
+            plane_convex(m, wsd, wf, shapes, margin);
+            mirror(m);
+            mirrored_features = true;
+        }
+    } else {
+        Hit hit;
+        if (mark == kPairHit) {
+            hit = make_hit(park[0], mk(park[1], park[2], park[3]), mk(park[4], park[5], park[6]));
+        } else {
+            SimplexPoint simplex[4];
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
+            Epa epa;
+            epa.valid = park[12] != 0.0f;
+            epa.normal = mk(park[13], park[14], park[15]);
+            epa.depth = park[16];
+            epa.point = zero3v();
+            hit = convex_hit_finish(wf, wsd, shapes, simplex, 4u, epa);
+            if (!(hit.distance <= margin)) return;
+        }
+        emit(m, hit.normal);
+        if (!convex_pair_manifold(wf, wsd, shapes, hit.normal, m)) manifold_from_hit(m, hit, margin);
+    }
     if (m.count == 0u) return;
-    write_contact(w, index, fs, ss, first, second, fc, sc, m, false);
+    write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
 }
 
 // ---- stable flag compaction: per-block counts, one-workgroup scan, destination map ---------
@@ -562,7 +594,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_dest(const uint32_t* __restri
 __global__ void __launch_bounds__(kThreads) k_contacts_scatter(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live || !w.valid[index]) return;
+    if (index >= live || w.valid[index] != kPairContact) return;
     const uint32_t at = w.dest[index];
     if (at >= w.contact_cap) return;
     w.contacts[at] = w.raw[index];
@@ -942,23 +974,29 @@ __global__ void __launch_bounds__(kThreads) k_warm(W w) {
                  sub(ld3(second.s.angular_velocity), ld3(sl.s.angular_velocity)), false);
 }
 
-__device__ __forceinline__ float normal_delta(const dyn_contact& c, float mass, float truth, float step,
-                                              float goal, float depth, float speed, float accumulated) {
-    if (c.relaxation <= 0.0f) return (goal - speed) / mass;
+__device__ __forceinline__ float normal_delta(float relaxation, float damping_ratio, float mass, float truth,
+                                              float step, float goal, float depth, float speed,
+                                              float accumulated) {
+    if (relaxation <= 0.0f) return (goal - speed) / mass;
     const float bias = fmx(depth / step, goal);
-    const float drag = 2.0f * c.damping_ratio * c.relaxation / step;
-    const float softness = c.relaxation * c.relaxation * truth / (step * step);
+    const float drag = 2.0f * damping_ratio * relaxation / step;
+    const float softness = relaxation * relaxation * truth / (step * step);
     const float prior = speed - accumulated * truth;
     const float wanted = (bias - (1.0f + drag) * prior) / (softness + (1.0f + drag) * truth);
     return truth * (wanted - accumulated) / mass;
 }
 
-// solver_contact_block.wgsl:31-193
+// solver_contact_block.wgsl:31-193. The contact is read field by field from its record and its
+// points' impulses are applied as each point is solved: every point reads only the block's
+// bodies as they were before the block (pf, ps and their split copies), so applying point i
+// before solving point i + 1 adds the same impulses in the same order as applying them all
+// after the loop.
 __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
     const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= *w.ncontacts) return;
-    const dyn_contact contact = w.contacts[ci];
-    if (!block_resolves(contact)) return;
+    dyn_contact& contact = w.contacts[ci];
+    const uint32_t point_count = contact.point_count;
+    if (point_count == 0u || contact.sensor != 0u) return;  // block_resolves
     const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
     Body pf = load_body(w, fb), ps = load_body(w, sb);
     Body split_f = pf, split_s = ps;
@@ -982,25 +1020,26 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
     if (fi && si) return;  // both inert: zero words (commit_block of zeros)
     Body first = pf, second = ps;
     const v3 normal = ld3(contact.normal);
+    const float friction = contact.friction, relaxation = contact.relaxation;
+    const float damping_ratio = contact.damping_ratio;
     v3 t1, t2;
     make_tangents(normal, &t1, &t2);
-    dyn_contact updated = contact;
     v3 reach_a = zero3v(), reach_b = zero3v();
     const v3 com_f = body_com(pf), com_s = body_com(ps);
-    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
-        reach_a = add(reach_a, sub(arm(pf, contact.points[pi], true), com_f));
-        reach_b = add(reach_b, sub(arm(ps, contact.points[pi], false), com_s));
-    }
-    const float count = static_cast<float>(max(contact.point_count, 1u));
     float normal_total = 0.0f;
-    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
-        normal_total = normal_total + fmx(contact.points[pi].accumulated_normal, 0.0f);
-    }
-    v3 impulses[4] = {zero3v(), zero3v(), zero3v(), zero3v()};
-    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+    for (uint32_t pi = 0; pi < point_count; ++pi) {
         const dyn_manifold_point& p = contact.points[pi];
+        reach_a = add(reach_a, sub(arm(pf, p, true), com_f));
+        reach_b = add(reach_b, sub(arm(ps, p, false), com_s));
+        normal_total = normal_total + fmx(p.accumulated_normal, 0.0f);
+    }
+    const float count = static_cast<float>(max(point_count, 1u));
+    for (uint32_t pi = 0; pi < point_count; ++pi) {
+        dyn_manifold_point& p = contact.points[pi];
         const v3 aa = arm(pf, p, true), ab = arm(ps, p, false);
         float an = p.accumulated_normal, at1 = p.accumulated_tangent_1, at2 = p.accumulated_tangent_2;
+        const float depth = p.depth;
+        v3 impulse = zero3v();
         const float nspeed = dot3(relative_velocity(pf, ps, aa, ab), normal);
         const v3 arm_a = cross3(sub(aa, body_com(split_f)), normal);
         const v3 arm_b = cross3(sub(ab, body_com(split_s)), normal);
@@ -1008,30 +1047,27 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
                             dot3(apply_inverse_inertia(split_f, cross3(reach_a, normal)), arm_a) +
                             dot3(apply_inverse_inertia(split_s, cross3(reach_b, normal)), arm_b);
         const float truth = momentum_mass(pf, ps, aa, ab, normal);
-        const float delta = normal_delta(contact, nmass, truth, w.p.dt, w.targets[ci * 4u + pi], p.depth,
-                                         nspeed, an);
+        const float delta = normal_delta(relaxation, damping_ratio, nmass, truth, w.p.dt,
+                                         w.targets[ci * 4u + pi], depth, nspeed, an);
         const float next_n = fmx(0.0f, an + delta);
-        impulses[pi] = add(impulses[pi], mul(normal, next_n - an));
+        impulse = add(impulse, mul(normal, next_n - an));
         an = next_n;
-        const float limit = contact.friction * an;
+        const float limit = friction * an;
         const float m1 = momentum_mass(split_f, split_s, aa, ab, t1) * count;
         const float s1 = dot3(relative_velocity(pf, ps, aa, ab), t1);
         const float next_1 = clampf(at1 - s1 / m1, -limit, limit);
-        impulses[pi] = add(impulses[pi], mul(t1, next_1 - at1));
+        impulse = add(impulse, mul(t1, next_1 - at1));
         at1 = next_1;
         const float m2 = momentum_mass(split_f, split_s, aa, ab, t2) * count;
         const float s2 = dot3(relative_velocity(pf, ps, aa, ab), t2);
         const float remaining = sqrtf(fmx(limit * limit - at1 * at1, 0.0f));
         const float next_2 = clampf(at2 - s2 / m2, -remaining, remaining);
-        impulses[pi] = add(impulses[pi], mul(t2, next_2 - at2));
+        impulse = add(impulse, mul(t2, next_2 - at2));
         at2 = next_2;
-        updated.points[pi].accumulated_normal = an;
-        updated.points[pi].accumulated_tangent_1 = at1;
-        updated.points[pi].accumulated_tangent_2 = at2;
-    }
-    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
-        const dyn_manifold_point& p = contact.points[pi];
-        apply_pair_impulse(first, second, arm(pf, p, true), arm(ps, p, false), impulses[pi]);
+        p.accumulated_normal = an;
+        p.accumulated_tangent_1 = at1;
+        p.accumulated_tangent_2 = at2;
+        apply_pair_impulse(first, second, aa, ab, impulse);
     }
     const v3 rel_spin = sub(ld3(second.s.angular_velocity), ld3(first.s.angular_velocity));
     const v3 planar = sub(rel_spin, mul(normal, dot3(rel_spin, normal)));
@@ -1061,12 +1097,6 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
             st3(second.s.angular_velocity,
                 add(ld3(second.s.angular_velocity), apply_inverse_inertia(second, mul(normal, applied))));
         }
-    }
-    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
-        dyn_manifold_point& q = w.contacts[ci].points[pi];
-        q.accumulated_normal = updated.points[pi].accumulated_normal;
-        q.accumulated_tangent_1 = updated.points[pi].accumulated_tangent_1;
-        q.accumulated_tangent_2 = updated.points[pi].accumulated_tangent_2;
     }
     commit_block(w, fb, sb, fi, si, sub(ld3(first.s.velocity), ld3(pf.s.velocity)),
                  sub(ld3(first.s.angular_velocity), ld3(pf.s.angular_velocity)),
