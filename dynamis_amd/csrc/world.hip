@@ -94,10 +94,18 @@ struct W {
     uint32_t* block_counts;
     uint32_t* blocks;
     float* targets;
+    float4* prows;  // per contact block: the position phase's three float4 (k_blocks)
     float* resolution;
-    uint32_t* vdeltas;
-    uint32_t* pdeltas;
-    uint32_t* contrib;
+    // Solver deltas without atomics: every block writes its fixed-point words to its own slots
+    // and every row sums the words of the blocks on its list. The words are integers, so the
+    // sum does not depend on the order of the list.
+    uint32_t* slots;        // per block: its place in the first and the second row's list
+    uint32_t* list_counts;  // per row: blocks on its list (0 for inert rows)
+    uint32_t* row_start;    // per row: first entry of its list in adj
+    uint32_t* adj;          // list entries: block << 1 | side
+    uint32_t* vwords;       // per block: 12 velocity words (first lin, ang; second lin, ang)
+    uint4* pwords;          // per block: {contributed, first lin}, {second lin, 0}
+    uint32_t* contrib;      // inert rows: marked when a block of theirs contributed
     uint32_t* scan_blocks;   // per-block sums of compaction scans
     uint32_t* scan_total;    // two words: totals of the last scans
 };
@@ -525,7 +533,9 @@ __global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
 constexpr int kScanItems = 4;
 constexpr int kScanBlock = kThreads * kScanItems;  // 1024 flags per block
 
-// flags[i] for i < *count (or < limit when count is null)
+// flags[i] for i < *count (or < limit when count is null); with kCounts the values are counts
+// and the scan sums them instead of the set flags.
+template <bool kCounts>
 __global__ void __launch_bounds__(kThreads) k_scan_count(const uint32_t* __restrict__ flags,
                                                          const uint32_t* count, uint32_t limit,
                                                          uint32_t* block_sums) {
@@ -536,7 +546,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_count(const uint32_t* __restr
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const uint32_t i = first + j;
-        if (i < n && flags[i]) ++sum;
+        if (i < n) sum += kCounts ? flags[i] : (flags[i] ? 1u : 0u);
     }
     uint32_t total = 0;
     block_exclusive_sum<kThreads>(sum, s_scan, &total);
@@ -561,8 +571,10 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* block_sums, uint
     if (threadIdx.x == 0) *total_out = s_carry;
 }
 
-// dest[i] = rank of flag i among set flags (kNoSlot when clear); the same partition as
-// k_scan_count, whose block sums k_scan_blocks turned into offsets.
+// dest[i] = rank of flag i among set flags (kNoSlot when clear), or with kCounts the sum of
+// the counts before i; the same partition as k_scan_count, whose block sums k_scan_blocks
+// turned into offsets.
+template <bool kCounts>
 __global__ void __launch_bounds__(kThreads) k_scan_dest(const uint32_t* __restrict__ flags,
                                                         const uint32_t* count, uint32_t limit,
                                                         const uint32_t* __restrict__ block_offsets,
@@ -575,7 +587,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_dest(const uint32_t* __restri
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const uint32_t i = first + j;
-        f[j] = (i < n && flags[i]) ? 1u : 0u;
+        f[j] = i < n ? (kCounts ? flags[i] : (flags[i] ? 1u : 0u)) : 0u;
         sum += f[j];
     }
     uint32_t at = block_offsets[blockIdx.x] + block_exclusive_sum<kThreads>(sum, s_scan, nullptr);
@@ -583,7 +595,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_dest(const uint32_t* __restri
     for (int j = 0; j < kScanItems; ++j) {
         const uint32_t i = first + j;
         if (i < n) {
-            dest[i] = f[j] ? at : kNoSlot;
+            dest[i] = (kCounts || f[j]) ? at : kNoSlot;
             at += f[j];
         }
     }
@@ -876,9 +888,25 @@ __global__ void __launch_bounds__(kThreads) k_blocks(W w) {
     const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
     w.blocks[2 * i] = fb;
     w.blocks[2 * i + 1] = sb;
-    if (!block_resolves(c)) return;
+    float4* prow = w.prows + 3u * i;
+    if (!block_resolves(c)) {
+        prow[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(0u), 0.0f);
+        w.slots[2 * i] = w.slots[2 * i + 1] = kNoSlot;
+        return;
+    }
     const Body first = load_body(w, fb), second = load_body(w, sb);
     const v3 normal = ld3(c.normal);
+    {
+        // position_correction.wgsl:5-46 reads only what stays fixed through the substeps
+        const bool fi = body_is_inert(first), si = body_is_inert(second);
+        const uint32_t flags = c.relaxation > 0.0f ? 0u : (c.point_count | (fi ? 8u : 0u) | (si ? 16u : 0u));
+        prow[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(flags),
+                              fi ? 0.0f : first.d.inverse_mass);
+        prow[1] = make_float4(si ? 0.0f : second.d.inverse_mass, normal.x, normal.y, normal.z);
+        prow[2] = make_float4(c.points[0].depth, c.point_count > 1u ? c.points[1].depth : 0.0f,
+                              c.point_count > 2u ? c.points[2].depth : 0.0f,
+                              c.point_count > 3u ? c.points[3].depth : 0.0f);
+    }
     for (uint32_t pi = 0; pi < c.point_count; ++pi) {
         const v3 pos = ld3(c.points[pi].position);
         const float speed = dot3(relative_velocity(first, second, pos, pos), normal);
@@ -887,25 +915,42 @@ __global__ void __launch_bounds__(kThreads) k_blocks(W w) {
         else if (speed < -w.p.restitution_threshold) target = -speed * c.restitution;
         w.targets[i * 4u + pi] = target;
     }
-    if (body_is_inert(first)) w.block_counts[fb] = 1u; else atomicAdd(&w.block_counts[fb], 1u);
-    if (body_is_inert(second)) w.block_counts[sb] = 1u; else atomicAdd(&w.block_counts[sb], 1u);
+    // a non-inert row's count is the length of its list; the count before this block's
+    // increment is the block's place on it
+    uint32_t slot_f = kNoSlot, slot_s = kNoSlot;
+    if (body_is_inert(first)) w.block_counts[fb] = 1u; else slot_f = atomicAdd(&w.block_counts[fb], 1u);
+    if (body_is_inert(second)) w.block_counts[sb] = 1u; else slot_s = atomicAdd(&w.block_counts[sb], 1u);
+    w.slots[2 * i] = slot_f;
+    w.slots[2 * i + 1] = slot_s;
 }
 
 __global__ void __launch_bounds__(kThreads) k_rows_count(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t claimed = (i < w.n && w.block_counts[i] != 0u) ? 1u : 0u;
+    if (i < w.n) {
+        const bool listed = claimed && !(!kind_simulates(w.de[i]) || w.st[i].sleeping != 0u);
+        w.list_counts[i] = listed ? w.block_counts[i] : 0u;
+    }
     claimed = wave_sum(claimed);
     if (lane_id() == 0 && claimed) atomicAdd(cw(w.counters, DYN_COUNTER_SOLVER_ROWS), claimed);
 }
 
-__device__ __forceinline__ void accumulate(uint32_t* deltas, uint32_t row, v3 lin, v3 ang, float scale) {
-    uint32_t* d = deltas + row * 8u;
-    atomicAdd(d + 0, solver_word(lin.x, scale));
-    atomicAdd(d + 1, solver_word(lin.y, scale));
-    atomicAdd(d + 2, solver_word(lin.z, scale));
-    atomicAdd(d + 3, solver_word(ang.x, scale));
-    atomicAdd(d + 4, solver_word(ang.y, scale));
-    atomicAdd(d + 5, solver_word(ang.z, scale));
+// The list entries of every block's non-inert sides, at the row's offset plus the block's place.
+__global__ void __launch_bounds__(kThreads) k_fill_lists(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *w.ncontacts) return;
+    const uint2 slot = reinterpret_cast<const uint2*>(w.slots)[i];
+    if (slot.x != kNoSlot) w.adj[w.row_start[w.blocks[2 * i]] + slot.x] = i << 1;
+    if (slot.y != kNoSlot) w.adj[w.row_start[w.blocks[2 * i + 1]] + slot.y] = (i << 1) | 1u;
+}
+
+// One side's six velocity words of block ci.
+__device__ __forceinline__ void store_words(const W& w, uint32_t ci, uint32_t side, v3 lin, v3 ang) {
+    uint2* d = reinterpret_cast<uint2*>(w.vwords + ci * 12u + side * 6u);
+    const float scale = DYN_SOLVER_VELOCITY_SCALE;
+    d[0] = make_uint2(solver_word(lin.x, scale), solver_word(lin.y, scale));
+    d[1] = make_uint2(solver_word(lin.z, scale), solver_word(ang.x, scale));
+    d[2] = make_uint2(solver_word(ang.y, scale), solver_word(ang.z, scale));
 }
 
 __device__ __forceinline__ void make_tangents(v3 n, v3* t1, v3* t2) {
@@ -934,11 +979,10 @@ __device__ __forceinline__ v3 arm(const Body& b, const dyn_manifold_point& p, bo
 }
 
 // solver_block_solve.wgsl:56-83: inert bodies receive exactly zero words (skipped).
-__device__ __forceinline__ void commit_block(const W& w, uint32_t fb, uint32_t sb, bool fa_inert,
-                                             bool sb_inert, v3 da, v3 sa, v3 db, v3 sbv,
-                                             bool residual) {
-    if (!fa_inert) accumulate(w.vdeltas, fb, da, sa, DYN_SOLVER_VELOCITY_SCALE);
-    if (!sb_inert) accumulate(w.vdeltas, sb, db, sbv, DYN_SOLVER_VELOCITY_SCALE);
+__device__ __forceinline__ void commit_block(const W& w, uint32_t ci, bool fa_inert, bool sb_inert, v3 da,
+                                             v3 sa, v3 db, v3 sbv, bool residual) {
+    if (!fa_inert) store_words(w, ci, 0u, da, sa);
+    if (!sb_inert) store_words(w, ci, 1u, db, sbv);
     if (!residual) return;
     atomicMax(cw(w.counters, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL),
               solver_word(fmx(len3(da), len3(db)), DYN_SOLVER_VELOCITY_SCALE));
@@ -968,7 +1012,7 @@ __global__ void __launch_bounds__(kThreads) k_warm(W w) {
                                mul(t2, p.accumulated_tangent_2));
         apply_pair_impulse(first, second, aa, ab, impulse);
     }
-    commit_block(w, fb, sb, fi, si, sub(ld3(first.s.velocity), ld3(fl.s.velocity)),
+    commit_block(w, ci, fi, si, sub(ld3(first.s.velocity), ld3(fl.s.velocity)),
                  sub(ld3(first.s.angular_velocity), ld3(fl.s.angular_velocity)),
                  sub(ld3(second.s.velocity), ld3(sl.s.velocity)),
                  sub(ld3(second.s.angular_velocity), ld3(sl.s.angular_velocity)), false);
@@ -1098,7 +1142,7 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
                 add(ld3(second.s.angular_velocity), apply_inverse_inertia(second, mul(normal, applied))));
         }
     }
-    commit_block(w, fb, sb, fi, si, sub(ld3(first.s.velocity), ld3(pf.s.velocity)),
+    commit_block(w, ci, fi, si, sub(ld3(first.s.velocity), ld3(pf.s.velocity)),
                  sub(ld3(first.s.angular_velocity), ld3(pf.s.angular_velocity)),
                  sub(ld3(second.s.velocity), ld3(ps.s.velocity)),
                  sub(ld3(second.s.angular_velocity), ld3(ps.s.angular_velocity)), residual != 0u);
@@ -1108,16 +1152,26 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
 __global__ void __launch_bounds__(kThreads) k_apply(W w) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= w.n || w.block_counts[row] == 0u) return;
-    uint32_t* d = w.vdeltas + row * 8u;
-    const uint4 lo = *reinterpret_cast<const uint4*>(d);
-    const uint2 hi = *reinterpret_cast<const uint2*>(d + 4);
-    if ((lo.x | lo.y | lo.z | lo.w | hi.x | hi.y) == 0u) return;
-    *reinterpret_cast<uint4*>(d) = make_uint4(0u, 0u, 0u, 0u);
-    *reinterpret_cast<uint2*>(d + 4) = make_uint2(0u, 0u);
-    const v3 lin = mk(solver_value(lo.x, DYN_SOLVER_VELOCITY_SCALE), solver_value(lo.y, DYN_SOLVER_VELOCITY_SCALE),
-                      solver_value(lo.z, DYN_SOLVER_VELOCITY_SCALE));
-    const v3 ang = mk(solver_value(lo.w, DYN_SOLVER_VELOCITY_SCALE), solver_value(hi.x, DYN_SOLVER_VELOCITY_SCALE),
-                      solver_value(hi.y, DYN_SOLVER_VELOCITY_SCALE));
+    const uint32_t count = w.list_counts[row];
+    if (count == 0u) return;  // inert rows receive no words
+    const uint32_t* list = w.adj + w.row_start[row];
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+    for (uint32_t e = 0; e < count; ++e) {
+        const uint32_t entry = list[e];
+        const uint2* d = reinterpret_cast<const uint2*>(w.vwords + (entry >> 1) * 12u + (entry & 1u) * 6u);
+        const uint2 a = d[0], b = d[1], c = d[2];
+        s0 += a.x;
+        s1 += a.y;
+        s2 += b.x;
+        s3 += b.y;
+        s4 += c.x;
+        s5 += c.y;
+    }
+    if ((s0 | s1 | s2 | s3 | s4 | s5) == 0u) return;
+    const v3 lin = mk(solver_value(s0, DYN_SOLVER_VELOCITY_SCALE), solver_value(s1, DYN_SOLVER_VELOCITY_SCALE),
+                      solver_value(s2, DYN_SOLVER_VELOCITY_SCALE));
+    const v3 ang = mk(solver_value(s3, DYN_SOLVER_VELOCITY_SCALE), solver_value(s4, DYN_SOLVER_VELOCITY_SCALE),
+                      solver_value(s5, DYN_SOLVER_VELOCITY_SCALE));
     if (zero3(lin) && zero3(ang)) return;
     dyn_body_state& s = w.st[row];
     st3(s.velocity, add(ld3(s.velocity), lin));
@@ -1180,26 +1234,34 @@ __global__ void __launch_bounds__(kThreads) k_advance(W w) {
     w.st[row] = s;
 }
 
-// position_correction.wgsl:5-46, position_block.wgsl
+// position_correction.wgsl:5-46, position_block.wgsl, from the block's position row
 __global__ void __launch_bounds__(kThreads) k_position_block(W w) {
     const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= *w.ncontacts) return;
-    const dyn_contact& c = w.contacts[ci];
-    if (!block_resolves(c) || c.relaxation > 0.0f) return;
-    const uint32_t fr = w.blocks[2 * ci], sr = w.blocks[2 * ci + 1];
-    const dyn_body_desc& df = w.de[fr];
-    const dyn_body_desc& ds = w.de[sr];
-    const bool fi = !kind_simulates(df) || w.st[fr].sleeping != 0u;
-    const bool si = !kind_simulates(ds) || w.st[sr].sleeping != 0u;
-    const float imf = fi ? 0.0f : df.inverse_mass, ims = si ? 0.0f : ds.inverse_mass;
-    const v3 normal = ld3(c.normal);
+    const float4* prow = w.prows + 3u * ci;
+    const float4 h0 = prow[0];
+    const uint32_t flags = __float_as_uint(h0.z);
+    const uint32_t count = flags & 7u;
+    uint4* out = w.pwords + 2u * ci;
+    out[0] = make_uint4(0u, 0u, 0u, 0u);  // no contribution unless the end below is reached
+    if (count == 0u) return;  // not resolved, or relaxed by compliance
+    const uint32_t fr = __float_as_uint(h0.x), sr = __float_as_uint(h0.y);
+    const bool fi = (flags & 8u) != 0u, si = (flags & 16u) != 0u;
+    const float imf = h0.w;
+    const float4 h1 = prow[1];
+    const float ims = h1.x;
+    const v3 normal = mk(h1.y, h1.z, h1.w);
+    const float4 depths = prow[2];
     const float4 rs = reinterpret_cast<const float4*>(w.resolution)[sr];
     const float4 rf = reinterpret_cast<const float4*>(w.resolution)[fr];
     const float resolved = dot3(normal, sub(mk(rs.x, rs.y, rs.z), mk(rf.x, rf.y, rf.z)));
     v3 total = zero3v();
     uint32_t contributing = 0;
-    for (uint32_t pi = 0; pi < c.point_count; ++pi) {
-        const float error = fmx(c.points[pi].depth - w.p.slop - resolved, 0.0f);
+#pragma unroll
+    for (uint32_t pi = 0; pi < 4; ++pi) {
+        if (pi >= count) continue;
+        const float depth = pi == 0 ? depths.x : (pi == 1 ? depths.y : (pi == 2 ? depths.z : depths.w));
+        const float error = fmx(depth - w.p.slop - resolved, 0.0f);
         if (error <= 0.0f) continue;
         const float k = imf + ims;
         if (k == 0.0f) continue;
@@ -1211,24 +1273,46 @@ __global__ void __launch_bounds__(kThreads) k_position_block(W w) {
     const v3 la = mul(neg(correction), imf), lb = mul(correction, ims);
     if (!(dot3(la, la) > 0.0f || dot3(lb, lb) > 0.0f)) return;
     // Inert rows get exactly zero words; only their "contributed" mark matters.
-    if (fi) w.contrib[fr] = 1u; else atomicAdd(&w.contrib[fr], 1u);
-    if (si) w.contrib[sr] = 1u; else atomicAdd(&w.contrib[sr], 1u);
-    if (!fi) accumulate(w.pdeltas, fr, la, zero3v(), DYN_SOLVER_POSITION_SCALE);
-    if (!si) accumulate(w.pdeltas, sr, lb, zero3v(), DYN_SOLVER_POSITION_SCALE);
+    if (fi) w.contrib[fr] = 1u;
+    if (si) w.contrib[sr] = 1u;
+    const float scale = DYN_SOLVER_POSITION_SCALE;
+    out[1] = make_uint4(solver_word(lb.x, scale), solver_word(lb.y, scale), solver_word(lb.z, scale), 0u);
+    out[0] = make_uint4(1u, solver_word(la.x, scale), solver_word(la.y, scale), solver_word(la.z, scale));
 }
 
 // position_apply.wgsl over every claimed row
 __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= w.n || w.block_counts[row] == 0u) return;
-    const uint32_t contributing = w.contrib[row];
-    if (contributing == 0u) return;
-    w.contrib[row] = 0u;
-    uint32_t* d = w.pdeltas + row * 8u;
-    const uint4 lo = *reinterpret_cast<const uint4*>(d);
-    const uint2 hi = *reinterpret_cast<const uint2*>(d + 4);
-    *reinterpret_cast<uint4*>(d) = make_uint4(0u, 0u, 0u, 0u);
-    *reinterpret_cast<uint2*>(d + 4) = make_uint2(0u, 0u);
+    const uint32_t count = w.list_counts[row];
+    uint32_t contributing = 0;
+    uint4 lo = make_uint4(0u, 0u, 0u, 0u);  // linear words, then the (zero) angular x
+    uint2 hi = make_uint2(0u, 0u);          // angular y, z: the position blocks send none
+    if (count == 0u) {
+        contributing = w.contrib[row];  // inert row: only the mark
+        if (contributing == 0u) return;
+        w.contrib[row] = 0u;
+    } else {
+        const uint32_t* list = w.adj + w.row_start[row];
+        for (uint32_t e = 0; e < count; ++e) {
+            const uint32_t entry = list[e];
+            const uint4* d = w.pwords + 2u * (entry >> 1);
+            const uint4 head = d[0];
+            if (head.x == 0u) continue;
+            ++contributing;
+            if ((entry & 1u) == 0u) {
+                lo.x += head.y;
+                lo.y += head.z;
+                lo.z += head.w;
+            } else {
+                const uint4 tail = d[1];
+                lo.x += tail.x;
+                lo.y += tail.y;
+                lo.z += tail.z;
+            }
+        }
+        if (contributing == 0u) return;
+    }
     const v3 lin = mk(solver_value(lo.x, DYN_SOLVER_POSITION_SCALE), solver_value(lo.y, DYN_SOLVER_POSITION_SCALE),
                       solver_value(lo.z, DYN_SOLVER_POSITION_SCALE));
     const v3 ang = mk(solver_value(lo.w, DYN_SOLVER_POSITION_SCALE), solver_value(hi.x, DYN_SOLVER_POSITION_SCALE),
@@ -1407,13 +1491,14 @@ int alloc(dyn_world* world, T** ptr, size_t count) {
 uint32_t blocks_for(uint32_t n) { return dyn::ceil_div(n ? n : 1u, dyn::kThreads); }
 
 int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_t limit,
-         uint32_t* dest, uint32_t* total) {
+         uint32_t* dest, uint32_t* total, bool counts = false) {
     using namespace dyn;
     hipStream_t st = world->stream;
     const uint32_t blocks = ceil_div(limit ? limit : 1u, kScanBlock);
     {
         ProfileScope scope("k_scan_count", st);
-        k_scan_count<<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks);
+        if (counts) k_scan_count<true><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks);
+        else k_scan_count<false><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks);
     }
     DYN_LAUNCHED("k_scan_count");
     {
@@ -1423,7 +1508,8 @@ int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_
     DYN_LAUNCHED("k_scan_blocks");
     {
         ProfileScope scope("k_scan_dest", st);
-        k_scan_dest<<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks, dest);
+        if (counts) k_scan_dest<true><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks, dest);
+        else k_scan_dest<false><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks, dest);
     }
     DYN_LAUNCHED("k_scan_dest");
     return DYN_OK;
@@ -1505,6 +1591,8 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_solver_reset, nb, w);
     LAUNCH(k_blocks, cb, w);
     LAUNCH(k_rows_count, nb, w);
+    DYN_TRY(scan(world, w.list_counts, nullptr, w.n, w.row_start, &w.scan_total[3], true));
+    LAUNCH(k_fill_lists, cb, w);
     for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
         LAUNCH(k_integrate, db, w);
         if (sub == 0) {
@@ -1558,6 +1646,10 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
         return fail(DYN_E_INVALID, "dyn_world_create: null argument");
     }
     if (dynamic_count > body_count) return fail(DYN_E_INVALID, "dyn_world_create: dynamic_count > body_count");
+    if (contact_capacity > (1u << 30) || pair_capacity > (1u << 30)) {
+        // list entries hold a contact index shifted by one bit
+        return fail(DYN_E_INVALID, "dyn_world_create: capacities above 2^30");
+    }
     if (params->substeps == 0 || params->solve_iterations == 0 || params->position_iterations == 0) {
         return fail(DYN_E_INVALID, "dyn_world_create: substeps and iterations must be positive");
     }
@@ -1647,11 +1739,17 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.block_counts, body_count);
     A(w.blocks, 2u * static_cast<size_t>(contact_capacity));
     A(w.targets, 4u * static_cast<size_t>(contact_capacity));
+    A(w.prows, 3u * static_cast<size_t>(contact_capacity));
     A(w.resolution, 4u * static_cast<size_t>(body_count));
-    A(w.vdeltas, 8u * static_cast<size_t>(body_count));
-    A(w.pdeltas, 8u * static_cast<size_t>(body_count));
+    A(w.slots, 2u * static_cast<size_t>(contact_capacity));
+    A(w.list_counts, body_count);
+    A(w.row_start, body_count);
+    A(w.adj, 2u * static_cast<size_t>(contact_capacity));
+    A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
+    A(w.pwords, 2u * static_cast<size_t>(contact_capacity));
     A(w.contrib, body_count);
-    const uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
+    uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
+    if (body_count > scan_limit) scan_limit = body_count;  // the rows' list offsets
     A(w.scan_blocks, ceil_div(scan_limit ? scan_limit : 1u, kScanBlock) + 1u);
     A(w.scan_total, 4);
     A(world->words, 8);
