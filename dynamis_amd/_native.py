@@ -59,7 +59,10 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_world_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_u32, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_u32]),
     "dyn_world_destroy": (c_int, [c_void_p]),
+    "dyn_batch_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p, c_void_p, c_u32,
+                                 c_u32, c_void_p, c_void_p, c_void_p, c_u32, c_u32, c_u32, c_u32]),
     "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
+    "dyn_world_set_graphs": (c_int, [c_void_p, c_int]),
     "dyn_world_wait": (c_int, [c_void_p]),
     "dyn_world_stream": (c_void_p, [c_void_p]),
     "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),

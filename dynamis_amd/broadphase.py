@@ -36,6 +36,7 @@ class BroadphaseIO(ctypes.Structure):
         ("_reserved", ctypes.c_uint32),
         ("d_scratch", ctypes.c_void_p),
         ("scratch_bytes", ctypes.c_size_t),
+        ("d_worlds", ctypes.c_void_p),
     ]
 
 

@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
     const uint32_t* __restrict__ cnt_level, const uint32_t* __restrict__ block_offsets,
     uint32_t* __restrict__ entry_keys, uint32_t* __restrict__ entry_order,
     dyn_grid_entry* __restrict__ entries, uint32_t capacity, uint32_t* sort_hist,
-    const uint32_t* sort_ctl, uint32_t* flags0) {
+    const uint32_t* sort_ctl, uint32_t* flags0, const uint32_t* __restrict__ worlds) {
     __shared__ uint32_t s_hist[kEntryPasses][kSortBins];
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const uint32_t t = threadIdx.x;
@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
         const uint32_t f = flags ? flags[i] : (DYN_BP_MOBILE | DYN_BP_AWAKE);
         const uint32_t group = groups ? groups[i] : i;
         const Cells cells = grid_cells(mn, mx, level_cell_size(level, base));
+        const uint32_t salt = worlds ? world_salt(worlds[i]) : 0u;
         uint32_t info_base = i | (DYN_ENTRY_KIND_COLLIDER << DYN_ENTRY_KIND_SHIFT);
         if (f & DYN_BP_AWAKE) info_base |= DYN_ENTRY_AWAKE;
         if (f & DYN_BP_MOBILE) info_base |= DYN_ENTRY_MOBILE;
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
             const uint32_t offset = cell_offset(cells, coord);
             uint32_t info = info_base | (offset << DYN_ENTRY_CELL_SHIFT);
             if (offset == 0u) info |= DYN_ENTRY_PRIMARY;
-            const uint32_t key = cell_key(level, coord[0], coord[1], coord[2]);
+            const uint32_t key = cell_key(level, coord[0], coord[1], coord[2]) ^ salt;
             entry_keys[slot] = key;
             entry_order[slot] = slot;
             float4* rec = reinterpret_cast<float4*>(entries + slot);
@@ -351,13 +352,14 @@ struct PairWriter {
 // pair_emit.wgsl:10-31 (node identity is excluded by the callers).
 template <typename Sink>
 __device__ __forceinline__ void emit_pair(Sink& sink, uint32_t group_a, uint32_t info_a, uint32_t group_b,
-                                          uint32_t info_b) {
+                                          uint32_t info_b, const uint32_t* __restrict__ worlds) {
     if (((info_a & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (((info_b & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (group_a == group_b) return;
     if (!(info_a & DYN_ENTRY_MOBILE) && !(info_b & DYN_ENTRY_MOBILE)) return;
     const uint32_t a = info_a & DYN_ENTRY_INDEX_MASK;
     const uint32_t b = info_b & DYN_ENTRY_INDEX_MASK;
+    if (worlds && worlds[a] != worlds[b]) return;  // a batch's worlds never meet
     sink(min(a, b), max(a, b));
 }
 
@@ -405,7 +407,7 @@ __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
                                            const uint32_t* __restrict__ keys,
                                            const uint32_t* __restrict__ order,
                                            const dyn_grid_entry* __restrict__ entries, float grid,
-                                           Sink& sink) {
+                                           const uint32_t* __restrict__ worlds, Sink& sink) {
     const uint32_t key = keys[index];
     const bool before = index > 0 && keys[index - 1] == key;
     const bool after = index + 1 < len && keys[index + 1] == key;
@@ -421,11 +423,11 @@ __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
     for (uint32_t e = index; e-- > 0 && keys[e] == key;) {
         const EntryView o = load_entry(entries, order[e]);
         if (o.info & DYN_ENTRY_AWAKE) continue;
-        if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info);
+        if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
     }
     for (uint32_t e = index + 1; e < len && keys[e] == key; ++e) {
         const EntryView o = load_entry(entries, order[e]);
-        if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info);
+        if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
     }
 }
 
@@ -433,14 +435,15 @@ __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
 __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
     const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
-    const uint32_t* sort_ctl, uint32_t* mate_counts, uint32_t* mate_blocks) {
+    const uint32_t* sort_ctl, const uint32_t* __restrict__ worlds, uint32_t* mate_counts,
+    uint32_t* mate_blocks) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const uint32_t len = sort_ctl[kCtlLen];
     const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
     const uint32_t index = blockIdx.x * kBpThreads + threadIdx.x;
     PairCounter counter;
-    if (index < len) cell_mates(index, len, keys, order, entries, grid, counter);
+    if (index < len) cell_mates(index, len, keys, order, entries, grid, worlds, counter);
     if (index < len) mate_counts[index] = counter.n;
     uint32_t total = 0;
     block_exclusive_sum<kBpThreads>(counter.n, s_scan, &total);
@@ -451,9 +454,9 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
 __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
     const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
-    const uint32_t* sort_ctl, const uint32_t* __restrict__ mate_counts,
-    const uint32_t* __restrict__ mate_offsets, uint32_t* pair_major, uint32_t* pair_minor,
-    uint32_t pair_capacity) {
+    const uint32_t* sort_ctl, const uint32_t* __restrict__ worlds,
+    const uint32_t* __restrict__ mate_counts, const uint32_t* __restrict__ mate_offsets,
+    uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const uint32_t len = sort_ctl[kCtlLen];
     const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
@@ -463,7 +466,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
     const uint32_t at = mate_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
     if (mine == 0u || at >= pair_capacity) return;
     PairWriter writer{at, pair_capacity, pair_major, pair_minor};
-    cell_mates(index, len, keys, order, entries, grid, writer);
+    cell_mates(index, len, keys, order, entries, grid, worlds, writer);
 }
 
 // Coarser-level links (level_links.wgsl:43-65) of one collider. Every collider's primary entry
@@ -480,7 +483,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                                             const dyn_grid_entry* __restrict__ entries,
                                             const uint32_t* __restrict__ dir, bool coarse_active,
                                             uint32_t occupied, float grid, uint32_t capacity,
-                                            Sink& sink) {
+                                            const uint32_t* __restrict__ worlds, Sink& sink) {
     const uint32_t emitted = packed & 0xFFu;
     if (emitted == 0u || slot >= capacity) return;
     const uint32_t f = flags ? flags[i] : (DYN_BP_MOBILE | DYN_BP_AWAKE);
@@ -498,6 +501,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
     self.info = i | (DYN_ENTRY_KIND_COLLIDER << DYN_ENTRY_KIND_SHIFT) | DYN_ENTRY_PRIMARY;
     if (awake) self.info |= DYN_ENTRY_AWAKE;
     if (f & DYN_BP_MOBILE) self.info |= DYN_ENTRY_MOBILE;
+    const uint32_t salt = worlds ? world_salt(worlds[i]) : 0u;
     while (coarser != 0u) {
         const uint32_t link = 31u - static_cast<uint32_t>(__clz(coarser));
         coarser &= ~(1u << link);
@@ -508,13 +512,13 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             int32_t cell[3];
             cell_at(cells, ordinal, cell);
             uint32_t first, end;
-            key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]), &first, &end);
+            key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]) ^ salt, &first, &end);
             for (uint32_t e = first; e < end; ++e) {
                 const uint32_t other = order[e];
                 if (other == slot) continue;
                 const EntryView o = load_entry(entries, other);
                 if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
-                if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info);
+                if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
             }
         }
     }
@@ -528,7 +532,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
-    uint32_t* link_counts, uint32_t* link_blocks) {
+    const uint32_t* __restrict__ worlds, uint32_t* link_counts, uint32_t* link_blocks) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
@@ -540,7 +544,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     PairCounter counter;
     if (i < n) {
         level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir, coarse_active,
-                    occupied, grid, capacity, counter);
+                    occupied, grid, capacity, worlds, counter);
         link_counts[i] = counter.n;
     }
     uint32_t total = 0;
@@ -555,8 +559,9 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
-    const uint32_t* __restrict__ link_counts, const uint32_t* __restrict__ link_offsets,
-    uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
+    const uint32_t* __restrict__ worlds, const uint32_t* __restrict__ link_counts,
+    const uint32_t* __restrict__ link_offsets, uint32_t* pair_major, uint32_t* pair_minor,
+    uint32_t pair_capacity) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
@@ -570,7 +575,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     if (mine == 0u || at >= pair_capacity) return;
     PairWriter writer{at, pair_capacity, pair_major, pair_minor};
     level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir, coarse_active, occupied,
-                grid, capacity, writer);
+                grid, capacity, worlds, writer);
 }
 
 // One workgroup: exclusive offsets of the link blocks, then of the mate blocks after them, the
@@ -723,7 +728,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
         bp_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, counters, l.cnt_level, l.block_offsets,
             io->d_entry_keys, io->d_entry_order, io->d_entries, io->entry_capacity, l.entry_sort.hist,
-            l.entry_sort.ctl, l.entry_sort.flags);
+            l.entry_sort.ctl, l.entry_sort.flags, io->d_worlds);
     }
     DYN_LAUNCHED("bp_emit_kernel");
     SortDigit digits[kSortMaxPasses];
@@ -743,14 +748,14 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
         bp_links_count_kernel<<<blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
             io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
-            io->entry_capacity, l.link_counts, l.link_blocks);
+            io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks);
     }
     DYN_LAUNCHED("bp_links_count_kernel");
     {
         ProfileScope scope("bp_mates_count_kernel", st);
         bp_mates_count_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
             io->d_entry_keys, io->d_entry_order, io->d_entries, counters, l.entry_sort.ctl,
-            l.mate_counts, l.mate_blocks);
+            io->d_worlds, l.mate_counts, l.mate_blocks);
     }
     DYN_LAUNCHED("bp_mates_count_kernel");
     {
@@ -765,15 +770,16 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
             bp_links_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
                 io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
                 io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
-                io->entry_capacity, l.link_counts, l.link_blocks, io->d_pair_major, io->d_pair_minor,
-                io->pair_capacity);
+                io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, io->d_pair_major,
+                io->d_pair_minor, io->pair_capacity);
         }
         DYN_LAUNCHED("bp_links_emit_kernel");
         {
             ProfileScope scope("bp_mates_emit_kernel", st);
             bp_mates_emit_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
                 io->d_entry_keys, io->d_entry_order, io->d_entries, counters, l.entry_sort.ctl,
-                l.mate_counts, l.mate_blocks, io->d_pair_major, io->d_pair_minor, io->pair_capacity);
+                io->d_worlds, l.mate_counts, l.mate_blocks, io->d_pair_major, io->d_pair_minor,
+                io->pair_capacity);
         }
         DYN_LAUNCHED("bp_mates_emit_kernel");
     }

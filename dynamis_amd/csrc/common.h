@@ -141,6 +141,9 @@ __device__ inline uint32_t block_exclusive_sum(uint32_t v, uint32_t* scratch, ui
 
 namespace dyn {
 
+// Whether per-kernel profiling (dyn_profile_enable) is on.
+bool profiling();
+
 // Brackets one kernel launch with events on its stream while profiling is enabled
 // (dyn_profile_enable); a no-op otherwise.
 class ProfileScope {

@@ -33,6 +33,14 @@ __device__ __forceinline__ uint32_t cell_key(uint32_t level, int32_t x, int32_t 
     return (level << DYN_LEVEL_KEY_SHIFT) | cell_hash(x, y, z);
 }
 
+// The hash salt of a world of a batch: its cells' keys are the single-world keys with the hash
+// bits XORed by it, so worlds sharing space fall in different hash buckets. World 0 (and a
+// single world) has salt 0, the reference's keys.
+__device__ __forceinline__ uint32_t world_salt(uint32_t world) {
+    const uint32_t h = world * 0x27D4EB2Fu;
+    return (h ^ (h >> 15)) & DYN_CELL_HASH_MASK;
+}
+
 // grid_index.wgsl:39-41
 __device__ __forceinline__ float level_cell_size(uint32_t level, float base) {
     return base * static_cast<float>(1u << level);

@@ -49,6 +49,12 @@ Profile& profile() {
 
 }  // namespace
 
+bool profiling() {
+    Profile& p = profile();
+    std::lock_guard<std::mutex> guard(p.lock);
+    return p.enabled;
+}
+
 ProfileScope::ProfileScope(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
     Profile& p = profile();
     std::lock_guard<std::mutex> guard(p.lock);

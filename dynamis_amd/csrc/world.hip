@@ -58,6 +58,7 @@ struct W {
     dyn_body_desc* de;
     dyn_collider* co;
     uint32_t* owner;
+    const uint32_t* worlds;  // world of each collider in a batch, or null for one world
     uint32_t* row_of_body;  // body id -> row, ids < id_cap
     uint32_t id_cap;
     const dyn_shape_source* sources;  // hull shape sources and their float4 vertex rows
@@ -75,6 +76,10 @@ struct W {
     uint32_t pair_cap;
     dyn_contact* raw;
     uint32_t* valid;
+    uint8_t* pkind;        // per convex pair: the unordered class of its two shape kinds
+    uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
+    uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
+    uint32_t* work_total;  // device word: entries on the worklist
     uint32_t* dest;
     dyn_contact* contacts;
     uint32_t* matched;
@@ -325,6 +330,90 @@ __device__ inline void write_contact(const W& w, uint32_t index, uint32_t fs, ui
     w.valid[index] = kPairContact;
 }
 
+// Whether two shapes' unswept bounds are apart by more than the margin along some axis. Then
+// the reference makes no contact: the axis probe of convex_hit sees the gap, and GJK's distance
+// is at least the gap (convex.wgsl:584-604); the 1 mm slack covers the rounding of the bounds.
+__device__ __forceinline__ bool bounds_apart(const Shape& a, const Shape& b, const ShapeSet& ss, float margin) {
+    v3 amn, amx, bmn, bmx;
+    world_aabb_of(a, ss, &amn, &amx);
+    world_aabb_of(b, ss, &bmn, &bmx);
+    const float reach = margin + 1e-3f;
+    return bmn.x - amx.x > reach || amn.x - bmx.x > reach || bmn.y - amx.y > reach ||
+           amn.y - bmx.y > reach || bmn.z - amx.z > reach || amn.z - bmx.z > reach;
+}
+
+// ---- convex worklists: pairs grouped by the kinds of their shapes ---------------------------
+
+constexpr uint32_t kClasses = 32;  // unordered pairs of 7 kind slots (28) and spare
+constexpr uint32_t kClassItems = 16;
+constexpr uint32_t kClassBlock = kThreads * kClassItems;  // 4096 pairs per block
+constexpr uint32_t kNoClass = 0xFFu;
+
+__device__ __forceinline__ uint32_t kind_slot(uint32_t kind) {
+    switch (kind) {
+        case DYN_SHAPE_SPHERE: return 0u;
+        case DYN_SHAPE_CUBOID: return 1u;
+        case DYN_SHAPE_CAPSULE: return 2u;
+        case DYN_SHAPE_CYLINDER: return 3u;
+        case DYN_SHAPE_HULL: return 4u;
+        case DYN_SHAPE_PLANE: return 5u;
+        default: return 6u;
+    }
+}
+
+__device__ __forceinline__ uint8_t kind_class(uint32_t first_kind, uint32_t second_kind) {
+    uint32_t a = kind_slot(first_kind), b = kind_slot(second_kind);
+    if (a > b) {
+        const uint32_t t = a;
+        a = b;
+        b = t;
+    }
+    return static_cast<uint8_t>(a * 7u - (a * (a - 1u)) / 2u + (b - a));  // row-major upper triangle
+}
+
+// The pairs a convex kernel takes: kPairGjk (mode 0), the pairs k_narrow_finish completes
+// (mode 1), kPairEpa (mode 2).
+__device__ __forceinline__ uint32_t work_class(const W& w, uint32_t index, uint32_t mode) {
+    const uint32_t v = w.valid[index];
+    const bool take = mode == 0u   ? v == kPairGjk
+                      : mode == 1u ? (v == kPairEpaDone || v == kPairHit || v == kPairPlane)
+                                   : v == kPairEpa;
+    return take ? static_cast<uint32_t>(w.pkind[index]) : kNoClass;
+}
+
+__global__ void __launch_bounds__(kThreads) k_class_count(W w, uint32_t mode) {
+    __shared__ uint32_t hist[kClasses];
+    if (threadIdx.x < kClasses) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    const uint32_t base = blockIdx.x * kClassBlock;
+    for (uint32_t j = 0; j < kClassItems; ++j) {
+        const uint32_t index = base + j * kThreads + threadIdx.x;
+        if (index >= live) break;
+        const uint32_t c = work_class(w, index, mode);
+        if (c != kNoClass) atomicAdd(&hist[c], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses) w.class_hist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+}
+
+// After k_scan_blocks turned the class-major counts into offsets: each block places its pairs
+// of every class at that class's offset (in any order within the block: every pair's result
+// goes to its own slot).
+__global__ void __launch_bounds__(kThreads) k_class_scatter(W w, uint32_t mode) {
+    __shared__ uint32_t cursor[kClasses];
+    if (threadIdx.x < kClasses) cursor[threadIdx.x] = w.class_hist[threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    const uint32_t base = blockIdx.x * kClassBlock;
+    for (uint32_t j = 0; j < kClassItems; ++j) {
+        const uint32_t index = base + j * kThreads + threadIdx.x;
+        if (index >= live) break;
+        const uint32_t c = work_class(w, index, mode);
+        if (c != kNoClass) w.worklist[atomicAdd(&cursor[c], 1u)] = index;
+    }
+}
+
 // Analytic pairs; convex-path pairs are marked for the kernels below.
 __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
@@ -335,14 +424,18 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     uint32_t fb, sb;
     if (!pair_admitted(w, fs, ss, &fb, &sb)) return;
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
-    if (convex_path(fc, sc)) {
-        const bool plane = fc.kind == DYN_SHAPE_PLANE || sc.kind == DYN_SHAPE_PLANE;
-        w.valid[index] = plane ? kPairPlane : kPairGjk;
-        return;
-    }
     const Body first = load_body(w, fb), second = load_body(w, sb);
     const float margin = pair_margin(w, first, second);
     const ShapeSet shapes{w.sources, w.vertices};
+    if (convex_path(fc, sc)) {
+        w.pkind[index] = kind_class(fc.kind, sc.kind);
+        if (fc.kind == DYN_SHAPE_PLANE || sc.kind == DYN_SHAPE_PLANE) {
+            w.valid[index] = kPairPlane;
+        } else if (!bounds_apart(world_collider(first.s, fc), world_collider(second.s, sc), shapes, margin)) {
+            w.valid[index] = kPairGjk;
+        }
+        return;
+    }
     Manifold m;
     m.count = 0;
     m.normal = zero3v();
@@ -371,18 +464,6 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
 }
 
-// Whether two shapes' unswept bounds are apart by more than the margin along some axis. Then
-// the reference makes no contact: the axis probe of convex_hit sees the gap, and GJK's distance
-// is at least the gap (convex.wgsl:584-604); the 1 mm slack covers the rounding of the bounds.
-__device__ __forceinline__ bool bounds_apart(const Shape& a, const Shape& b, const ShapeSet& ss, float margin) {
-    v3 amn, amx, bmn, bmx;
-    world_aabb_of(a, ss, &amn, &amx);
-    world_aabb_of(b, ss, &bmn, &bmx);
-    const float reach = margin + 1e-3f;
-    return bmn.x - amx.x > reach || amn.x - bmx.x > reach || bmn.y - amx.y > reach ||
-           amn.y - bmx.y > reach || bmn.z - amx.z > reach || amn.z - bmx.z > reach;
-}
-
 // Intermediate results of a convex pair, parked in the pair's raw contact slot (which only its
 // final contact overwrites) between k_narrow_convex, k_narrow_epa and k_narrow_finish. A
 // penetrating pair: floats 0-11 GJK's four Minkowski points, 12 EPA's validity, 13-15 its
@@ -395,9 +476,9 @@ __device__ __forceinline__ float* parked(const W& w, uint32_t index) {
 // Separated pairs end here; a hit within the margin is parked for k_narrow_finish and a
 // penetrating pair's simplex for k_narrow_epa.
 __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
-    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live || w.valid[index] != kPairGjk) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *w.work_total) return;
+    const uint32_t index = w.worklist[t];
     w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
@@ -409,7 +490,6 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
     }
     const ShapeSet shapes{w.sources, w.vertices};
     const Shape wf = world_collider(fst, fc), wsd = world_collider(sst, sc);
-    if (bounds_apart(wf, wsd, shapes, margin)) return;
     Hit hit;
     SimplexPoint simplex[4];
 #pragma unroll
@@ -439,38 +519,32 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
 
 constexpr uint32_t kEpaWaves = 4;  // waves per workgroup of k_narrow_epa
 
-// EPA of the pairs marked kPairEpa, one wave per pair: tasks of 64 candidate pairs, a wave
-// working through the marked pairs of its task one after another. Lane 0 parks the result.
+// EPA of the pairs marked kPairEpa, one wave per pair, the waves striding over the worklist.
+// Lane 0 parks the result.
 __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
     __shared__ EpaShared epa[kEpaWaves];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
+    const uint32_t total = *w.work_total;
     const uint32_t waves = gridDim.x * kEpaWaves;
     const ShapeSet shapes{w.sources, w.vertices};
-    for (uint32_t task = blockIdx.x * kEpaWaves + wave; task * kWave < live; task += waves) {
-        const uint32_t mine = task * kWave + lane;
-        uint64_t todo = __ballot(mine < live && w.valid[mine] == kPairEpa);
-        while (todo != 0u) {
-            const uint32_t bit = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(todo))) - 1u;
-            todo &= todo - 1u;
-            const uint32_t index = task * kWave + bit;
-            const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
-            const Shape wf = world_collider(w.st[w.owner[fs]], w.co[fs]);
-            const Shape wsd = world_collider(w.st[w.owner[ss]], w.co[ss]);
-            float* park = parked(w, index);
-            SimplexPoint simplex[4];
+    for (uint32_t t = blockIdx.x * kEpaWaves + wave; t < total; t += waves) {
+        const uint32_t index = w.worklist[t];
+        const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+        const Shape wf = world_collider(w.st[w.owner[fs]], w.co[fs]);
+        const Shape wsd = world_collider(w.st[w.owner[ss]], w.co[ss]);
+        float* park = parked(w, index);
+        SimplexPoint simplex[4];
 #pragma unroll
-            for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
-            const Epa r = epa_tetrahedron(wf, wsd, shapes, simplex, 4u, epa[wave]);
-            if (lane == 0u) {
-                park[12] = r.valid ? 1.0f : 0.0f;
-                park[13] = r.normal.x;
-                park[14] = r.normal.y;
-                park[15] = r.normal.z;
-                park[16] = r.depth;
-                w.valid[index] = kPairEpaDone;
-            }
+        for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
+        const Epa r = epa_tetrahedron(wf, wsd, shapes, simplex, 4u, epa[wave]);
+        if (lane == 0u) {
+            park[12] = r.valid ? 1.0f : 0.0f;
+            park[13] = r.normal.x;
+            park[14] = r.normal.y;
+            park[15] = r.normal.z;
+            park[16] = r.depth;
+            w.valid[index] = kPairEpaDone;
         }
     }
 }
@@ -479,9 +553,9 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
 // cylinders, GJK hits within the margin, and penetrating pairs after EPA (the rest of
 // convex_hit first).
 __global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
-    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *w.work_total) return;
+    const uint32_t index = w.worklist[t];
     const uint32_t mark = w.valid[index];
     if (mark != kPairEpaDone && mark != kPairHit && mark != kPairPlane) return;
     w.valid[index] = kPairNone;
@@ -1474,6 +1548,12 @@ struct dyn_world {
     std::vector<uint32_t> host_sources;
     uint32_t hull_colliders = 0;
     uint32_t convex_colliders = 0;  // colliders some pair may send to the convex path
+    // The whole step captured once as a HIP graph and replayed: every launch of a step has
+    // its grid from the capacities and its arguments from the world, so only dt (and the
+    // shape pointers, replaced by an upload) can change what a step launches.
+    bool graphs = true;
+    hipGraphExec_t graph = nullptr;
+    float graph_dt = 0.0f;
 };
 
 namespace {
@@ -1515,6 +1595,31 @@ int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_
     return DYN_OK;
 }
 
+// The worklist of convex pairs of one mode, grouped by kind class: block histograms, their
+// class-major scan (total in work_total), the scatter.
+int worklist(dyn_world* world, uint32_t mode) {
+    using namespace dyn;
+    W& w = world->w;
+    hipStream_t st = world->stream;
+    const uint32_t blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
+    {
+        ProfileScope scope("k_class_count", st);
+        k_class_count<<<blocks, kThreads, 0, st>>>(w, mode);
+    }
+    DYN_LAUNCHED("k_class_count");
+    {
+        ProfileScope scope("k_class_scan", st);
+        k_scan_blocks<<<1, 1024, 0, st>>>(w.class_hist, kClasses * blocks, w.work_total);
+    }
+    DYN_LAUNCHED("k_class_scan");
+    {
+        ProfileScope scope("k_class_scatter", st);
+        k_class_scatter<<<blocks, kThreads, 0, st>>>(w, mode);
+    }
+    DYN_LAUNCHED("k_class_scatter");
+    return DYN_OK;
+}
+
 #define LAUNCH(name, grid, ...)                                            \
     do {                                                                   \
         {                                                                  \
@@ -1523,6 +1628,34 @@ int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_
         }                                                                  \
         DYN_LAUNCHED(#name);                                               \
     } while (0)
+
+int world_step(dyn_world* world, float dt);
+
+void drop_graph(dyn_world* world) {
+    if (world->graph) (void)hipGraphExecDestroy(world->graph);
+    world->graph = nullptr;
+}
+
+// Records one step on the world's stream into a graph and instantiates it for dt.
+int capture_step(dyn_world* world, float dt) {
+    drop_graph(world);
+    DYN_HIP(hipStreamBeginCapture(world->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = world_step(world, dt);
+    hipGraph_t graph = nullptr;
+    const hipError_t ended = hipStreamEndCapture(world->stream, &graph);
+    if (rc != DYN_OK || ended != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != DYN_OK ? rc : dyn::hip_fail(ended, "dyn_world_step: graph capture");
+    }
+    const hipError_t made = hipGraphInstantiate(&world->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (made != hipSuccess) {
+        world->graph = nullptr;
+        return dyn::hip_fail(made, "dyn_world_step: graph instantiation");
+    }
+    world->graph_dt = dt;
+    return DYN_OK;
+}
 
 int world_step(dyn_world* world, float dt) {
     using namespace dyn;
@@ -1556,10 +1689,13 @@ int world_step(dyn_world* world, float dt) {
     io.sort_pairs = 1;
     io.d_scratch = world->bp_scratch;
     io.scratch_bytes = world->bp_scratch_bytes;
+    io.d_worlds = w.worlds;
     DYN_TRY(dyn_broadphase(st, &io));
     LAUNCH(k_narrow, pb, w);
     if (world->convex_colliders) {
+        DYN_TRY(worklist(world, 0u));
         LAUNCH(k_narrow_convex, pb, w);
+        DYN_TRY(worklist(world, 2u));
         const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
         const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
         {
@@ -1567,6 +1703,7 @@ int world_step(dyn_world* world, float dt) {
             k_narrow_epa<<<groups, kEpaWaves * kWave, 0, st>>>(w);
         }
         DYN_LAUNCHED("k_narrow_epa");
+        DYN_TRY(worklist(world, 1u));
         LAUNCH(k_narrow_finish, pb, w);
     }
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
@@ -1724,6 +1861,10 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.pmin, pair_capacity);
     A(w.raw, pair_capacity);
     A(w.valid, pair_capacity);
+    A(w.pkind, pair_capacity);
+    A(w.worklist, pair_capacity);
+    A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
+    A(w.work_total, 1);
     A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
     A(w.matched, contact_capacity);
@@ -1785,6 +1926,40 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     return DYN_OK;
 }
 
+int dyn_batch_create(dyn_world** out, void* stream, const dyn_step_params* params,
+                     const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
+                     uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
+                     const uint32_t* collider_worlds, uint32_t collider_count, uint32_t world_count,
+                     uint32_t contact_capacity, uint32_t pair_capacity) {
+    using namespace dyn;
+    if (!out || (collider_count && !collider_worlds)) return fail(DYN_E_INVALID, "dyn_batch_create: null argument");
+    if (collider_count && !owners) return fail(DYN_E_INVALID, "dyn_batch_create: null argument");
+    // every collider names a world of the batch, and all colliders of one body the same one
+    std::vector<uint32_t> world_of_row(body_count, DYN_NO_BODY);
+    for (uint32_t i = 0; i < collider_count; ++i) {
+        if (collider_worlds[i] >= world_count) return fail(DYN_E_INVALID, "dyn_batch_create: world id out of range");
+        if (owners[i] >= body_count) return fail(DYN_E_INVALID, "dyn_batch_create: collider owner out of range");
+        uint32_t& seen = world_of_row[owners[i]];
+        if (seen != DYN_NO_BODY && seen != collider_worlds[i]) {
+            return fail(DYN_E_INVALID, "dyn_batch_create: a body's colliders lie in different worlds");
+        }
+        seen = collider_worlds[i];
+    }
+    dyn_world* world = nullptr;
+    DYN_TRY(dyn_world_create(&world, stream, params, states, descs, body_count, dynamic_count, colliders,
+                             owners, collider_count, contact_capacity, pair_capacity));
+    uint32_t* worlds = nullptr;
+    if (alloc(world, &worlds, collider_count) != DYN_OK ||
+        (collider_count &&
+         hipMemcpy(worlds, collider_worlds, 4u * collider_count, hipMemcpyHostToDevice) != hipSuccess)) {
+        dyn_world_destroy(world);
+        return fail(DYN_E_HIP, "dyn_batch_create: world ids");
+    }
+    world->w.worlds = worlds;
+    *out = world;
+    return DYN_OK;
+}
+
 int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
                             uint32_t source_count, const float* vertices, uint32_t vertex_count) {
     using namespace dyn;
@@ -1806,6 +1981,7 @@ int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
         }
     }
     DYN_HIP(hipStreamSynchronize(world->stream));
+    drop_graph(world);  // the captured step holds the old shape pointers
     if (world->shape_sources) DYN_HIP(hipFree(world->shape_sources));
     if (world->shape_vertices) DYN_HIP(hipFree(world->shape_vertices));
     world->shape_sources = nullptr;
@@ -1825,6 +2001,7 @@ int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
 int dyn_world_destroy(dyn_world* world) {
     if (!world) return DYN_OK;
     (void)hipStreamSynchronize(world->stream);
+    drop_graph(world);
     for (void* p : world->owned) (void)hipFree(p);
     if (world->shape_sources) (void)hipFree(world->shape_sources);
     if (world->shape_vertices) (void)hipFree(world->shape_vertices);
@@ -1839,7 +2016,18 @@ int dyn_world_step(dyn_world* world, float dt) {
     if (world->hull_colliders && world->shape_sources == nullptr) {
         return dyn::fail(DYN_E_INVALID, "dyn_world_step: hull colliders need dyn_world_upload_shapes first");
     }
-    return world_step(world, dt);
+    if (!world->graphs || dyn::profiling()) return world_step(world, dt);  // profiled steps run eagerly
+    if (!world->graph || world->graph_dt != dt) DYN_TRY(capture_step(world, dt));
+    DYN_HIP(hipGraphLaunch(world->graph, world->stream));
+    return DYN_OK;
+}
+
+int dyn_world_set_graphs(dyn_world* world, int enabled) {
+    if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_set_graphs: null world");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    world->graphs = enabled != 0;
+    drop_graph(world);
+    return DYN_OK;
 }
 
 int dyn_world_wait(dyn_world* world) {

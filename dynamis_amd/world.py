@@ -293,6 +293,11 @@ class World:
         if self.scene is not None:
             call("dyn_world_wait", self._handle)
 
+    def set_graphs(self, enabled: bool) -> None:
+        """Replays each step as a captured HIP graph (the default) or launches it eagerly."""
+        self._ensure()
+        call("dyn_world_set_graphs", self._handle, 1 if enabled else 0)
+
     # -- reads -------------------------------------------------------------------------------
     def states(self) -> np.ndarray:
         """Body states in row order."""
@@ -358,3 +363,119 @@ class World:
 
     def __exit__(self, *exc) -> None:
         self.close()
+
+
+@dataclass
+class BatchLayout:
+    """Where each world of a batch lives in the merged records."""
+
+    dynamic_base: np.ndarray   # first merged row of each world's simulated rows
+    static_base: np.ndarray    # first merged row of each world's other rows
+    dynamic_count: np.ndarray
+    body_count: np.ndarray
+    collider_base: np.ndarray  # first merged collider slot of each world
+    collider_count: np.ndarray
+    id_base: np.ndarray        # offset added to each world's body ids
+    collider_worlds: np.ndarray  # world of each merged collider
+
+    def rows_of(self, world: int) -> np.ndarray:
+        """Merged row of each of the world's own rows, in its row order."""
+        d, n = int(self.dynamic_count[world]), int(self.body_count[world])
+        dyn = self.dynamic_base[world] + np.arange(d, dtype=np.int64)
+        rest = self.static_base[world] + np.arange(n - d, dtype=np.int64)
+        return np.concatenate([dyn, rest])
+
+
+def merge_scenes(scenes: list[Scene], config: np.ndarray | None = None) -> tuple[Scene, BatchLayout]:
+    """The records of a batch of independent worlds (dyn_batch_create): all worlds' simulated
+    rows first, world by world, then their other rows; colliders world by world in each world's
+    arena order; body ids offset so they are unique; hull sources concatenated. Every world
+    keeps the relative order of its rows, colliders and ids, so it steps as it does alone."""
+    config = default_config() if config is None else config
+    w = len(scenes)
+    d = np.array([s.dynamic_count for s in scenes], dtype=np.int64)
+    n = np.array([len(s.states) for s in scenes], dtype=np.int64)
+    k = np.array([len(s.colliders) for s in scenes], dtype=np.int64)
+    caps = np.array([int(s.states["body_id"].max()) + 1 if len(s.states) else 0 for s in scenes], dtype=np.int64)
+    dyn_base = np.concatenate([[0], np.cumsum(d)[:-1]]) if w else d
+    total_dynamic = int(d.sum())
+    static_base = total_dynamic + (np.concatenate([[0], np.cumsum(n - d)[:-1]]) if w else d)
+    collider_base = np.concatenate([[0], np.cumsum(k)[:-1]]) if w else k
+    id_base = np.concatenate([[0], np.cumsum(caps)[:-1]]) if w else caps
+    total = int(n.sum())
+    states = np.zeros(total, dtype=abi.BODY_STATE)
+    descs = np.zeros(total, dtype=abi.BODY_DESC)
+    colliders = np.zeros(int(k.sum()), dtype=abi.COLLIDER)
+    owners = np.zeros(int(k.sum()), dtype=np.uint32)
+    collider_worlds = np.zeros(int(k.sum()), dtype=np.uint32)
+    source_parts, vertex_parts = [], []
+    source_base = vertex_base = 0
+    layout = BatchLayout(dyn_base, static_base, d, n, collider_base, k, id_base, collider_worlds)
+    for i, s in enumerate(scenes):
+        rows = layout.rows_of(i)
+        states[rows] = s.states
+        states["body_id"][rows] += np.uint32(id_base[i])
+        descs[rows] = s.descs
+        c0, c1 = int(collider_base[i]), int(collider_base[i] + k[i])
+        colliders[c0:c1] = s.colliders
+        owners[c0:c1] = rows[s.owners.astype(np.int64)]
+        collider_worlds[c0:c1] = i
+        if s.sources is not None and len(s.sources):
+            src = s.sources.copy()
+            src["vertex_offset"] += np.uint32(vertex_base)
+            source_parts.append(src)
+            vertex_parts.append(s.vertices)
+            hull = c0 + np.flatnonzero(colliders["kind"][c0:c1] == abi.SHAPE_HULL)
+            field_view = colliders["source"]
+            field_view[hull] += np.uint32(source_base)
+            source_base += len(src)
+            vertex_base += len(s.vertices)
+    params = np.zeros(1, dtype=abi.STEP_PARAMS)
+    call("dyn_step_params_build", config.ctypes.data, total, total_dynamic, int(k.sum()), params.ctypes.data)
+    sources = np.concatenate(source_parts) if source_parts else None
+    vertices = np.concatenate(vertex_parts) if vertex_parts else None
+    merged = Scene(params=params, states=states, descs=descs, colliders=colliders, owners=owners,
+                   dynamic_count=total_dynamic, rows=np.arange(total, dtype=np.uint32),
+                   handles=np.arange(total, dtype=np.uint32), sources=sources, vertices=vertices)
+    return merged, layout
+
+
+class Batch(World):
+    """Independent worlds stepped together as one device world (dyn_batch_create)."""
+
+    def __init__(self, scenes: list[Scene], config: np.ndarray | None = None,
+                 contact_capacity: int | None = None, pair_capacity: int | None = None):
+        super().__init__(config, contact_capacity, pair_capacity)
+        self.worlds = len(scenes)
+        merged, self.layout = merge_scenes(scenes, self.config)
+        self.scene = merged
+        n = len(merged.states)
+        capacity, pairs = default_capacities(n, contact_capacity, pair_capacity)
+        call("dyn_batch_create", ctypes.byref(self._handle), None, merged.params.ctypes.data,
+             merged.states.ctypes.data, merged.descs.ctypes.data, n, merged.dynamic_count,
+             merged.colliders.ctypes.data, merged.owners.ctypes.data,
+             self.layout.collider_worlds.ctypes.data, len(merged.colliders), self.worlds, capacity, pairs)
+        self.capacity = capacity
+        self.pair_capacity = pairs
+        if merged.sources is not None and len(merged.sources):
+            call("dyn_world_upload_shapes", self._handle, merged.sources.ctypes.data,
+                 len(merged.sources), merged.vertices.ctypes.data, len(merged.vertices))
+
+    def world_states(self, world: int, states: np.ndarray | None = None) -> np.ndarray:
+        """One world's body states in its own row order, with its own body ids."""
+        states = self.states() if states is None else states
+        out = states[self.layout.rows_of(world)].copy()
+        out["body_id"] -= np.uint32(self.layout.id_base[world])
+        return out
+
+    def world_contacts(self, world: int, contacts: np.ndarray | None = None) -> np.ndarray:
+        """One world's contacts in step order, with its own collider slots and body ids."""
+        contacts = self.inspect_contacts() if contacts is None else contacts
+        c0 = int(self.layout.collider_base[world])
+        mine = (contacts["a"] >= c0) & (contacts["a"] < c0 + int(self.layout.collider_count[world]))
+        out = contacts[mine].copy()
+        out["a"] -= np.uint32(c0)
+        out["b"] -= np.uint32(c0)
+        out["first_body_id"] -= np.uint32(self.layout.id_base[world])
+        out["second_body_id"] -= np.uint32(self.layout.id_base[world])
+        return out

@@ -122,6 +122,9 @@ typedef struct dyn_broadphase_io {
     uint32_t _reserved;
     void* d_scratch;
     size_t scratch_bytes;
+    const uint32_t* d_worlds;      /* K world ids of a batch of independent worlds, or NULL for
+                                      one world: colliders of different worlds never pair and
+                                      their cells hash apart */
 } dyn_broadphase_io;
 
 /* Scratch bytes dyn_broadphase needs. */
@@ -242,6 +245,17 @@ int dyn_world_create(dyn_world** world, void* stream, const dyn_step_params* par
                      uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
                      uint32_t collider_count, uint32_t contact_capacity, uint32_t pair_capacity);
 int dyn_world_destroy(dyn_world* world);
+/* A batch of independent worlds stepped as one: the worlds' rows, colliders and contacts share
+ * the buffers of one dyn_world, and collider_worlds[k] names the world of collider k (all
+ * colliders of a body in one world). Colliders of different worlds never pair, so every world
+ * steps exactly as it would alone. Rows: all worlds' simulated rows first, then the others;
+ * body ids unique across the batch. Step, wait, reads and destroy are the dyn_world_* calls.
+ * No counterpart in the reference, which steps one world per World. */
+int dyn_batch_create(dyn_world** out, void* stream, const dyn_step_params* params,
+                     const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
+                     uint32_t dynamic_count, const dyn_collider* colliders, const uint32_t* owners,
+                     const uint32_t* collider_worlds, uint32_t collider_count, uint32_t world_count,
+                     uint32_t contact_capacity, uint32_t pair_capacity);
 /* Uploads the hull shape sources colliders name in their `source` field, with their vertices as
  * float4 rows (x, y, z, 0) (the reference's shape pool flush, crates/dynamis-world/src/
  * shape_pool.rs:243-279). Replaces any earlier upload; call before a step that needs them.
@@ -249,8 +263,13 @@ int dyn_world_destroy(dyn_world* world);
  * the vertex array, or when a hull collider names a source that is not there. */
 int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
                             uint32_t source_count, const float* vertices, uint32_t vertex_count);
-/* Enqueues one step of `dt` seconds on the world's stream (non-blocking). */
+/* Enqueues one step of `dt` seconds on the world's stream (non-blocking). The step is captured
+ * as a HIP graph on its first call for a given dt and replayed afterwards; steps run eagerly
+ * while dyn_profile_enable is on or after dyn_world_set_graphs(world, 0). */
 int dyn_world_step(dyn_world* world, float dt);
+/* Turns the graph capture of dyn_world_step on (default) or off. No counterpart in the
+ * reference, whose steps are recorded command buffers (crates/dynamis-gpu/src/recorder.rs:29). */
+int dyn_world_set_graphs(dyn_world* world, int enabled);
 /* Blocks until every enqueued step has finished. */
 int dyn_world_wait(dyn_world* world);
 /* The world's hipStream_t. */
