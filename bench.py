@@ -9,6 +9,11 @@ Workloads (`--workload`):
               first `--warmup` are untimed.
   cubes       1,000 cubes (half-extent 0.5 m, density 1000, friction 0.5) dropped from 1-40 m over
               a 20 x 20 m footprint onto a plane; 4 solver iterations; 300 steps.
+  batch       4,096 independent worlds stepped as one batch (dyn_batch_create): each a plane and
+              N_w bodies, N_w log-uniform in [8, 2048] (1,550,732 bodies in all), half boxes
+              (half-extents ~ U[0.2, 0.8]) and half spheres (r ~ U[0.25, 0.75]), density 1000;
+              8 solver iterations, 240 steps. Also times the same worlds stepped one at a time on
+              one stream (a sample of every 16th world, scaled to all 4,096).
   broadphase  the isolated sort + broadphase pass over 1,048,576 boxes.
 One step = one World::step of every body (broadphase: one pass over every box); body-steps/s =
 bodies x timed steps / seconds. Inputs are resident in HBM before the timed region; the timed
@@ -105,6 +110,15 @@ def cubes_scene():
     return encode(cube_drop(1000, seed=1), cfg)
 
 
+BATCH_WORLDS = 4096
+BATCH_SEED = 4
+
+
+def batch_scenes():
+    from dynamis_amd.scenes import batch_worlds
+    return batch_worlds(BATCH_WORLDS, seed=BATCH_SEED)
+
+
 WORKLOADS = {
     "pile": {"scene": pile_scene, "steps": 540, "warmup": 60, "run": 600,
              "describe": "mixed convex pile: 100,000 bodies (25% each spheres r~U[0.25,0.75], boxes "
@@ -115,6 +129,11 @@ WORKLOADS = {
     "cubes": {"scene": cubes_scene, "steps": 290, "warmup": 10, "run": 300,
               "describe": "1,000 cubes (h=0.5 m, density 1000, friction 0.5) dropped from 1-40 m "
                           "over 20x20 m onto a plane, dt=1/60, 4 solver iterations"},
+    "batch": {"scene": None, "steps": 220, "warmup": 20, "run": 240,
+              "describe": "4,096 independent worlds in one batch, each a plane and N_w bodies with "
+                          "N_w log-uniform in [8, 2048] (1,550,732 bodies), 50/50 boxes "
+                          "h~U[0.2,0.8]^3 and spheres r~U[0.25,0.75], density 1000, dt=1/60, "
+                          "8 solver iterations"},
 }
 
 
@@ -163,6 +182,108 @@ def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     final_states = world.states()
     world.close()
     return ms.value, len(scene.states), stages, counts, m, scene, final_states
+
+
+def timed_steps(worlds, stream, steps: int) -> float:
+    """Milliseconds of `steps` rounds of one step of every world, by events on `stream`."""
+    import ctypes
+
+    from dynamis_amd._native import call
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    call("dyn_event_create", ctypes.byref(e0))
+    call("dyn_event_create", ctypes.byref(e1))
+    call("dyn_event_record", e0, stream)
+    for _ in range(steps):
+        for w in worlds:
+            w.step(DT)
+    call("dyn_event_record", e1, stream)
+    call("dyn_stream_synchronize", stream)
+    ms = ctypes.c_float()
+    call("dyn_event_elapsed_ms", ctypes.byref(ms), e0, e1)
+    call("dyn_event_destroy", e0)
+    call("dyn_event_destroy", e1)
+    return ms.value
+
+
+def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, serial_steps: int):
+    import ctypes
+
+    import dynamis_amd
+    from dynamis_amd._native import call
+    from dynamis_amd.device import profile_collect, profile_enable
+    from dynamis_amd.world import Batch, World
+
+    if dynamis_amd.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    call("dyn_set_device", 0)
+    scenes = batch_scenes()
+    batch = Batch(scenes)
+    for _ in range(warmup):
+        batch.step(DT)
+    batch.wait()
+    stream = ctypes.c_void_p(batch.stream())
+    ms = timed_steps([batch], stream, steps)
+    m = batch.measured()
+    contacts = batch.inspect_contacts()
+    points = float(np.mean(contacts["point_count"])) if len(contacts) else 0.0
+    profile_enable(True)
+    for _ in range(profile_steps):
+        batch.step(DT)
+    batch.wait()
+    profile_enable(False)
+    stages = profile_collect()
+    m2 = batch.measured()
+    counts = {"n": len(batch.scene.states), "k": len(batch.scene.colliders), "entries": m2["ENTRIES"],
+              "pairs": m2["PAIRS"], "contacts": m2["CONTACTS"], "points": points,
+              "rows": m2["SOLVER_ROWS"]}
+    bodies = len(batch.scene.states)
+    final = batch.states()
+    ends = [batch.world_states(i, final) for i in range(len(scenes))]
+    batch.close()
+    # the same worlds one at a time: a sample of every serial_every-th world, all on one stream,
+    # from their first steps (the batch's worlds start from the same states)
+    serial = None
+    if serial_every > 0:
+        shared = ctypes.c_void_p()
+        call("dyn_stream_create", ctypes.byref(shared))
+        sampled = scenes[::serial_every]
+        worlds = []
+        for s in sampled:
+            w = World()
+            w.create(s, stream=shared.value)
+            worlds.append(w)
+        timed_steps(worlds, shared, warmup)  # the batch's untimed steps (and graph capture)
+        sms = timed_steps(worlds, shared, serial_steps)
+        for w in worlds:
+            w.close()
+        call("dyn_stream_destroy", shared)
+        per_step = sms / serial_steps * (len(scenes) / len(sampled))
+        serial = {"serial_ms_per_step": per_step, "batched_ms_per_step": ms / steps,
+                  "batched_vs_serial": per_step / (ms / steps),
+                  "sample": f"{len(sampled)} of the {len(scenes)} worlds (every {serial_every}th) "
+                            f"stepped one at a time on one stream for {serial_steps} timed steps "
+                            f"after the batch's {warmup} untimed ones (earlier, lighter steps than "
+                            f"the batch's timed ones), scaled by {len(scenes) / len(sampled):g}"}
+    return ms, bodies, stages, counts, m, scenes, serial, ends
+
+
+def cpu_batch(scenes, ends, every: int, sample_steps: int):
+    import copy
+
+    from tests.oracle_binding import load
+    oracle = load()
+    bodies, secs = 0, 0.0
+    for s, end in list(zip(scenes, ends))[::every]:
+        start = copy.copy(s)
+        start.states = end.copy()
+        world = oracle.world(start)
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            world.step(DT)
+        secs += time.perf_counter() - t0
+        bodies += len(s.states)
+        world.close()
+    return bodies * sample_steps / secs, secs
 
 
 def run_broadphase(steps: int, warmup: int):
@@ -258,7 +379,7 @@ def stage_table(stages, counts, bytes_of):
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--workload", choices=["pile", "cubes", "broadphase"], default="pile")
+    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase"], default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--profile-steps", type=int, default=5)
@@ -288,6 +409,28 @@ def main() -> int:
                   "boxes": BOXES, "entries": counts["entries"], "pairs": counts["pairs"], "seed": BOX_SEED}
         dtype = "u32"
         measured = {}
+    elif wl == "batch":
+        spec = WORKLOADS[wl]
+        steps = max(args.steps if args.steps is not None else spec["steps"], 1)
+        warmup = max(args.warmup if args.warmup is not None else spec["warmup"], 0)
+        ms, bodies, stages, counts, measured, scenes, serial, ends = run_batch(
+            steps, warmup, args.profile_steps, 16, 3)
+        table, roofline = stage_table(stages, counts, lambda nm: step_bytes(
+            nm, counts["n"], counts["k"], counts["entries"], counts["pairs"], counts["contacts"],
+            counts["points"], counts["rows"]))
+        sample = 10 if args.cpu_sample is None else args.cpu_sample
+        cpu = None
+        if sample > 0:
+            v, secs = cpu_batch(scenes, ends, 16, sample)
+            cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                   "sample": f"{sample} steps of every 16th world of the batch through the "
+                             f"single-threaded C oracle, each started from the device's body "
+                             f"states at the end of the timed and profiled steps ({secs:.1f} s)"}
+        config = {"workload": spec["describe"], "bodies": bodies, "worlds": BATCH_WORLDS,
+                  "steps_in_run": spec["run"], "seed": BATCH_SEED, "serial": serial,
+                  "at_end": {k: measured[k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
+                                                       "REFUSED_PAIRS", "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
+        dtype = "f32"
     else:
         spec = WORKLOADS[wl]
         steps = max(args.steps if args.steps is not None else spec["steps"], 1)

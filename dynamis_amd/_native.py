@@ -56,6 +56,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_hull_solid": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
     "dyn_world_upload_shapes": (c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_u32]),
     "dyn_encode_body": (c_int, [c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_void_p]),
+    "dyn_encode_bodies": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
     "dyn_world_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_u32, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_u32]),
     "dyn_world_destroy": (c_int, [c_void_p]),

@@ -282,4 +282,20 @@ void dyn_config_default(dyn_config* c) {
     c->restitution_combine = 2;  // MaterialCombine::Max
 }
 
+int dyn_encode_bodies(const dyn_body_spec* specs, uint32_t count, const dyn_config* config,
+                      uint32_t first_body_id, dyn_body_state* states, dyn_body_desc* descs,
+                      dyn_collider* colliders) {
+    if (count && (!specs || !states || !descs || !colliders)) {
+        return dyn::fail(DYN_E_INVALID, "dyn_encode_bodies: null argument");
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        const int rc = dyn_encode_body(specs + i, config, first_body_id + i, 0u, states + i, descs + i,
+                                       colliders + i);
+        if (rc != DYN_OK) {
+            return dyn::fail(rc, "dyn_encode_bodies: spec " + std::to_string(i) + ": " + dyn_last_error());
+        }
+    }
+    return DYN_OK;
+}
+
 }  // extern "C"

@@ -92,3 +92,39 @@ def cube_drop(count: int = 1000, seed: int = 1, footprint: float = 20.0, low: fl
                                       position=(float(xs[i]), float(ys[i]), float(zs[i])),
                                       orientation=tuple(float(v) for v in q[i])))
     return bodies
+
+
+def batch_worlds(worlds: int = 4096, seed: int = 4, low: int = 8, high: int = 2048,
+                 spacing: float = 2.0, per_row: int = 16):
+    """The batch of independent worlds: each a static plane and N_w bodies, N_w log-uniform in
+    [low, high], half boxes (half-extents ~ U[0.2, 0.8] per axis) and half spheres
+    (r ~ U[0.25, 0.75]) at random, density 1000, on a jittered lattice of per_row x per_row
+    bodies a layer (spacing apart, layers spacing apart from y = 1). Returns the worlds' scenes
+    (each encoded as the reference would spawn it: the plane first)."""
+    from .world import BodyDesc, encode_specs
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(np.log(low), np.log(high + 1), worlds))).astype(np.int64)
+    counts = np.clip(counts, low, high)
+    box = BodyDesc.cuboid((0.5, 0.5, 0.5), density=1000.0).spec()[0]
+    ball = BodyDesc.sphere(0.5, density=1000.0).spec()[0]
+    plane = BodyDesc.plane().spec()
+    scenes = []
+    for n in counts:
+        kinds = rng.integers(0, 2, n)
+        specs = np.zeros(n, dtype=abi.BODY_SPEC)
+        specs[kinds == 0] = box
+        specs[kinds == 1] = ball
+        halves = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+        radii = rng.uniform(0.25, 0.75, n).astype(np.float32)
+        specs["half_extents"][kinds == 0] = halves[kinds == 0]
+        specs["radius"][kinds == 1] = radii[kinds == 1]
+        i = np.arange(n)
+        layer, cell = np.divmod(i, per_row * per_row)
+        gx, gz = np.divmod(cell, per_row)
+        first = -0.5 * (per_row - 1) * spacing
+        jitter = rng.uniform(-0.1, 0.1, (n, 3))
+        specs["position"][:, 0] = first + gx * spacing + jitter[:, 0]
+        specs["position"][:, 1] = 1.0 + layer * spacing + jitter[:, 1]
+        specs["position"][:, 2] = first + gz * spacing + jitter[:, 2]
+        scenes.append(encode_specs(np.concatenate([plane, specs])))
+    return scenes

@@ -194,26 +194,30 @@ def encode(bodies: list[BodyDesc], config: np.ndarray | None = None,
     """Encodes bodies spawned in this order into device records with the library's host
     encoder. Body handle ids are the spawn indices; collider slots follow spawn order (the
     collider arena, collider.rs:180-214); rows follow the simulated-first partition."""
+    specs = (np.concatenate([b.spec(hulls) for b in bodies]) if bodies
+             else np.zeros(0, dtype=abi.BODY_SPEC))
+    return encode_specs(specs, config, hulls)
+
+
+def encode_specs(specs: np.ndarray, config: np.ndarray | None = None,
+                 hulls: list[HullSource] | None = None) -> Scene:
+    """`encode` of an array of BODY_SPEC records (spawn order), one library call for all."""
     config = default_config() if config is None else config
-    specs = [b.spec(hulls) for b in bodies]
+    specs = np.ascontiguousarray(specs, dtype=abi.BODY_SPEC)
     n = len(specs)
     enc_states = np.zeros(n, dtype=abi.BODY_STATE)
     enc_descs = np.zeros(n, dtype=abi.BODY_DESC)
     colliders = np.zeros(n, dtype=abi.COLLIDER)
-    simulates = []
-    for handle, spec in enumerate(specs):
-        call("dyn_encode_body", spec.ctypes.data, config.ctypes.data, handle, 0,
-             enc_states[handle:handle + 1].ctypes.data, enc_descs[handle:handle + 1].ctypes.data,
-             colliders[handle:handle + 1].ctypes.data)
-        d = enc_descs[handle]
-        simulates.append(bool(d["inverse_mass"] > 0.0 and not (d["flags"] & abi.BODY_KINEMATIC)))
-    order = spawn_rows(simulates)
+    if n:
+        call("dyn_encode_bodies", specs.ctypes.data, n, config.ctypes.data, 0, enc_states.ctypes.data,
+             enc_descs.ctypes.data, colliders.ctypes.data)
+    simulates = (enc_descs["inverse_mass"] > 0.0) & ((enc_descs["flags"] & abi.BODY_KINEMATIC) == 0)
+    order = spawn_rows(simulates.tolist())
     rows = np.zeros(n, dtype=np.uint32)
-    for row, handle in enumerate(order):
-        rows[handle] = row
+    rows[np.asarray(order, dtype=np.int64)] = np.arange(n, dtype=np.uint32)
     states = enc_states[order].copy() if n else enc_states
     descs = enc_descs[order].copy() if n else enc_descs
-    dynamic_count = int(sum(simulates))
+    dynamic_count = int(simulates.sum())
     params = np.zeros(1, dtype=abi.STEP_PARAMS)
     call("dyn_step_params_build", config.ctypes.data, n, dynamic_count, n, params.ctypes.data)
     sources, vertices = shape_records(hulls or [])
@@ -270,12 +274,13 @@ class World:
         self.create(self.scene, self.contact_capacity, self.pair_capacity)
 
     def create(self, scene: Scene, contact_capacity: int | None = None,
-               pair_capacity: int | None = None) -> None:
-        """Creates the device world from already-encoded records."""
+               pair_capacity: int | None = None, stream: int | None = None) -> None:
+        """Creates the device world from already-encoded records, on its own stream or on the
+        given one (worlds sharing a stream step one after another)."""
         self.scene = scene
         n = len(scene.states)
         capacity, pairs = default_capacities(n, contact_capacity, pair_capacity)
-        call("dyn_world_create", ctypes.byref(self._handle), None, scene.params.ctypes.data,
+        call("dyn_world_create", ctypes.byref(self._handle), stream, scene.params.ctypes.data,
              scene.states.ctypes.data, scene.descs.ctypes.data, n, scene.dynamic_count,
              scene.colliders.ctypes.data, scene.owners.ctypes.data, n, capacity, pairs)
         self.capacity = capacity

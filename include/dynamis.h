@@ -222,6 +222,13 @@ int dyn_inertia_inverse(const float* inertia, float* inverse);
 int dyn_encode_body(const dyn_body_spec* spec, const dyn_config* config, uint32_t body_id,
                     uint32_t collider_slot, dyn_body_state* state, dyn_body_desc* desc,
                     dyn_collider* collider);
+/* dyn_encode_body over `count` specs with body ids first_body_id, first_body_id + 1, ... and
+ * collider slot 0: one call for a spawn list (the reference builds the records body by body in
+ * World::spawn, crates/dynamis-world/src/body.rs:34-58). Stops at the first failing spec and
+ * reports its index in the error text. */
+int dyn_encode_bodies(const dyn_body_spec* specs, uint32_t count, const dyn_config* config,
+                      uint32_t first_body_id, dyn_body_state* states, dyn_body_desc* descs,
+                      dyn_collider* colliders);
 
 /* ---- world step (crates/dynamis-world/src/lib.rs:86 World::new, src/step.rs:34 World::step,
  *      src/observation.rs:484 wait, src/body.rs:326 read_state, src/readback.rs:95
