@@ -19,9 +19,13 @@ One step = one World::step of every body (broadphase: one pass over every box); 
 bodies x timed steps / seconds. Inputs are resident in HBM before the timed region; the timed
 region is bracketed by events on the world's own stream.
 
-Prints ONE JSON line. `roofline` covers the kernel with the largest share of device time in a
-separately profiled window of steps (every launch bracketed by events on its stream);
-`cpu_baseline` times the single-threaded C oracle on a bounded sample of the same workload.
+Prints ONE JSON line. `roofline` covers the kernel with the largest device time in a window of
+warm-up steps (every launch bracketed by events): its average launch duration is measured over
+the timed steps themselves by events on the world's stream around each of its launches (the timed
+steps then launch eagerly instead of replaying the step's graph), and its algorithmic bytes per
+launch come from the step's counts at the start and end of the timed steps. `stages` is a window
+of further steps with every launch timed. `cpu_baseline` times the single-threaded C oracle on a
+bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -137,12 +141,69 @@ WORKLOADS = {
 }
 
 
-def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
+def world_counts(world) -> dict:
+    m = world.measured()
+    contacts = world.inspect_contacts()
+    points = float(np.mean(contacts["point_count"])) if len(contacts) else 0.0
+    return {"n": len(world.scene.states), "k": len(world.scene.colliders), "entries": m["ENTRIES"],
+            "pairs": m["PAIRS"], "contacts": m["CONTACTS"], "points": points, "rows": m["SOLVER_ROWS"]}
+
+
+def kernel_bytes(name: str, c: dict) -> float:
+    return step_bytes(name, c["n"], c["k"], c["entries"], c["pairs"], c["contacts"], c["points"], c["rows"])
+
+
+def timed_run(world, steps: int, warmup: int, profile_steps: int):
+    """Warm-up steps (the last `profile_steps` of them with every launch timed, to pick the
+    kernel with the largest device time), the timed steps (events on the world's stream around
+    them and around every launch of that kernel), then a window of steps with every launch timed
+    for the stage table. Steps with launches timed run eagerly; the others replay the graph."""
     import ctypes
 
+    from dynamis_amd.device import profile_collect, profile_enable, profile_watch
+
+    probe_steps = min(profile_steps, warmup)
+    for _ in range(warmup - probe_steps):
+        world.step(DT)
+    world.wait()
+    profile_watch(None)
+    profile_enable(True)
+    for _ in range(probe_steps):
+        world.step(DT)
+    world.wait()
+    profile_enable(False)
+    probe = profile_collect()
+    before = world_counts(world)
+    dominant = max((n for n in probe if kernel_bytes(n, before) > 0), key=lambda n: probe[n][1], default=None)
+    stream = ctypes.c_void_p(world.stream())
+    if dominant:
+        profile_watch(dominant)
+        profile_enable(True)
+    ms = timed_steps([world], stream, steps)
+    profile_enable(False)
+    profile_watch(None)
+    watched = profile_collect()
+    measured = world.measured()
+    after = world_counts(world)
+    profile_enable(True)
+    for _ in range(profile_steps):
+        world.step(DT)
+    world.wait()
+    profile_enable(False)
+    stages = profile_collect()
+    timed = None
+    if dominant and dominant in watched:
+        launches, total_ms = watched[dominant]
+        # the timed steps' bytes per launch: the mean of the counts at their start and end
+        mean = {k: 0.5 * (before[k] + after[k]) for k in before}
+        timed = {"kernel": dominant, "launches": launches, "avg_us": 1000.0 * total_ms / launches,
+                 "bytes_per_launch": kernel_bytes(dominant, mean)}
+    return ms, stages, after, measured, timed
+
+
+def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     import dynamis_amd
     from dynamis_amd._native import call
-    from dynamis_amd.device import profile_collect, profile_enable
     from dynamis_amd.world import World
 
     if dynamis_amd.device_count() < 1:
@@ -151,37 +212,10 @@ def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     scene = WORKLOADS[workload]["scene"]()
     world = World()
     world.create(scene)
-    for _ in range(warmup):
-        world.step(DT)
-    world.wait()
-    stream = ctypes.c_void_p(world.stream())
-    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-    call("dyn_event_create", ctypes.byref(e0))
-    call("dyn_event_create", ctypes.byref(e1))
-    call("dyn_event_record", e0, stream)
-    for _ in range(steps):
-        world.step(DT)
-    call("dyn_event_record", e1, stream)
-    world.wait()
-    ms = ctypes.c_float()
-    call("dyn_event_elapsed_ms", ctypes.byref(ms), e0, e1)
-    m = world.measured()
-    contacts = world.inspect_contacts()
-    points = float(np.mean(contacts["point_count"])) if len(contacts) else 0.0
-    # a profiled window of further steps
-    profile_enable(True)
-    for _ in range(profile_steps):
-        world.step(DT)
-    world.wait()
-    profile_enable(False)
-    stages = profile_collect()
-    m2 = world.measured()
-    counts = {"n": len(scene.states), "k": len(scene.colliders), "entries": m2["ENTRIES"],
-              "pairs": m2["PAIRS"], "contacts": m2["CONTACTS"], "points": points,
-              "rows": m2["SOLVER_ROWS"]}
+    ms, stages, counts, measured, timed = timed_run(world, steps, warmup, profile_steps)
     final_states = world.states()
     world.close()
-    return ms.value, len(scene.states), stages, counts, m, scene, final_states
+    return ms, len(scene.states), stages, counts, measured, scene, final_states, timed
 
 
 def timed_steps(worlds, stream, steps: int) -> float:
@@ -210,7 +244,6 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
 
     import dynamis_amd
     from dynamis_amd._native import call
-    from dynamis_amd.device import profile_collect, profile_enable
     from dynamis_amd.world import Batch, World
 
     if dynamis_amd.device_count() < 1:
@@ -218,24 +251,7 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
     call("dyn_set_device", 0)
     scenes = batch_scenes()
     batch = Batch(scenes)
-    for _ in range(warmup):
-        batch.step(DT)
-    batch.wait()
-    stream = ctypes.c_void_p(batch.stream())
-    ms = timed_steps([batch], stream, steps)
-    m = batch.measured()
-    contacts = batch.inspect_contacts()
-    points = float(np.mean(contacts["point_count"])) if len(contacts) else 0.0
-    profile_enable(True)
-    for _ in range(profile_steps):
-        batch.step(DT)
-    batch.wait()
-    profile_enable(False)
-    stages = profile_collect()
-    m2 = batch.measured()
-    counts = {"n": len(batch.scene.states), "k": len(batch.scene.colliders), "entries": m2["ENTRIES"],
-              "pairs": m2["PAIRS"], "contacts": m2["CONTACTS"], "points": points,
-              "rows": m2["SOLVER_ROWS"]}
+    ms, stages, counts, m, timed = timed_run(batch, steps, warmup, profile_steps)
     bodies = len(batch.scene.states)
     final = batch.states()
     ends = [batch.world_states(i, final) for i in range(len(scenes))]
@@ -264,7 +280,7 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
                             f"stepped one at a time on one stream for {serial_steps} timed steps "
                             f"after the batch's {warmup} untimed ones (earlier, lighter steps than "
                             f"the batch's timed ones), scaled by {len(scenes) / len(sampled):g}"}
-    return ms, bodies, stages, counts, m, scenes, serial, ends
+    return ms, bodies, stages, counts, m, scenes, serial, ends, timed
 
 
 def cpu_batch(scenes, ends, every: int, sample_steps: int):
@@ -356,7 +372,7 @@ def cpu_broadphase(passes: int):
 
 # ---- report ------------------------------------------------------------------------------------------
 
-def stage_table(stages, counts, bytes_of):
+def stage_table(stages, counts, bytes_of, timed=None):
     total = max(sum(t for _, t in stages.values()), 1e-9)
     table = {}
     for name, (launches, total_ms) in stages.items():
@@ -367,7 +383,14 @@ def stage_table(stages, counts, bytes_of):
                        "gbs": round(nbytes / (avg_us * 1e-6) / 1e9, 1) if avg_us > 0 and nbytes else 0.0}
     dominant = max((n for n in table if table[n]["bytes"] > 0), key=lambda n: stages[n][1], default=None)
     roofline = None
-    if dominant:
+    if timed:
+        gbs = timed["bytes_per_launch"] / (timed["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "kernel": timed["kernel"], "achieved": round(gbs, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "traffic": None, "avg_us": round(timed["avg_us"], 3),
+                    "bytes_per_launch": timed["bytes_per_launch"], "launches_timed": timed["launches"],
+                    "share_of_device_time": table.get(timed["kernel"], {}).get("share")}
+    elif dominant:
         d = table[dominant]
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": d["gbs"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
@@ -413,11 +436,9 @@ def main() -> int:
         spec = WORKLOADS[wl]
         steps = max(args.steps if args.steps is not None else spec["steps"], 1)
         warmup = max(args.warmup if args.warmup is not None else spec["warmup"], 0)
-        ms, bodies, stages, counts, measured, scenes, serial, ends = run_batch(
+        ms, bodies, stages, counts, measured, scenes, serial, ends, timed = run_batch(
             steps, warmup, args.profile_steps, 16, 3)
-        table, roofline = stage_table(stages, counts, lambda nm: step_bytes(
-            nm, counts["n"], counts["k"], counts["entries"], counts["pairs"], counts["contacts"],
-            counts["points"], counts["rows"]))
+        table, roofline = stage_table(stages, counts, lambda nm: kernel_bytes(nm, counts), timed)
         sample = 10 if args.cpu_sample is None else args.cpu_sample
         cpu = None
         if sample > 0:
@@ -435,11 +456,10 @@ def main() -> int:
         spec = WORKLOADS[wl]
         steps = max(args.steps if args.steps is not None else spec["steps"], 1)
         warmup = max(args.warmup if args.warmup is not None else spec["warmup"], 0)
-        ms, bodies, stages, counts, measured, scene, final = run_world(wl, steps, warmup, args.profile_steps)
-        table, roofline = stage_table(stages, counts, lambda nm: step_bytes(
-            nm, counts["n"], counts["k"], counts["entries"], counts["pairs"], counts["contacts"],
-            counts["points"], counts["rows"]))
-        sample = (8 if wl == "pile" else 100) if args.cpu_sample is None else args.cpu_sample
+        ms, bodies, stages, counts, measured, scene, final, timed = run_world(wl, steps, warmup,
+                                                                             args.profile_steps)
+        table, roofline = stage_table(stages, counts, lambda nm: kernel_bytes(nm, counts), timed)
+        sample = (2 if wl == "pile" else 100) if args.cpu_sample is None else args.cpu_sample
         cpu = None
         if sample > 0:
             v, secs = cpu_world(scene, final, sample)

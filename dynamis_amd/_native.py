@@ -38,6 +38,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_event_record": (c_int, [c_void_p, c_void_p]),
     "dyn_event_elapsed_ms": (c_int, [ctypes.POINTER(ctypes.c_float), c_void_p, c_void_p]),
     "dyn_profile_enable": (c_int, [c_int]),
+    "dyn_profile_watch": (c_int, [ctypes.c_char_p]),
     "dyn_profile_collect": (c_int, [ctypes.POINTER(c_int)]),
     "dyn_profile_entry": (c_int, [c_int, ctypes.c_char_p, c_size_t, ctypes.POINTER(c_u32),
                                   ctypes.POINTER(ctypes.c_double)]),

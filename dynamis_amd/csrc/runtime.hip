@@ -26,6 +26,7 @@ struct Timed {
 struct Profile {
     std::mutex lock;
     bool enabled = false;
+    std::string watch;  // when set, only launches of this kernel are bracketed
     std::vector<Timed> pending;
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<std::string, std::pair<uint32_t, double>>> collected;
@@ -59,6 +60,7 @@ ProfileScope::ProfileScope(const char* name, hipStream_t stream) : name_(name), 
     Profile& p = profile();
     std::lock_guard<std::mutex> guard(p.lock);
     if (!p.enabled) return;
+    if (!p.watch.empty() && p.watch != name_) return;
     hipEvent_t e = p.take();
     if (e && hipEventRecord(e, stream_) == hipSuccess) start_ = e;
 }
@@ -172,6 +174,13 @@ int dyn_profile_enable(int on) {
     dyn::Profile& p = dyn::profile();
     std::lock_guard<std::mutex> guard(p.lock);
     p.enabled = on != 0;
+    return DYN_OK;
+}
+
+int dyn_profile_watch(const char* kernel) {
+    dyn::Profile& p = dyn::profile();
+    std::lock_guard<std::mutex> guard(p.lock);
+    p.watch = kernel ? kernel : "";
     return DYN_OK;
 }
 

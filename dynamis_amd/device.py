@@ -126,6 +126,11 @@ def profile_enable(on: bool) -> None:
     call("dyn_profile_enable", 1 if on else 0)
 
 
+def profile_watch(kernel: str | None) -> None:
+    """Restricts the bracketing to the launches of one kernel (None: every kernel)."""
+    call("dyn_profile_watch", (kernel or "").encode())
+
+
 def profile_collect() -> dict[str, tuple[int, float]]:
     """Kernel name -> (launches, total ms) of the launches timed since the last collect."""
     n = ctypes.c_int()

@@ -62,6 +62,9 @@ int dyn_event_elapsed_ms(float* ms, void* start, void* stop);
  * dyn_profile_read synchronizes those events and reports, per kernel name, the launch count and
  * the summed device time; it then forgets them. Disabled by default. */
 int dyn_profile_enable(int on);
+/* Restricts the bracketing of dyn_profile_enable to the launches of one kernel (NULL or "" for
+ * all), so a timed run can measure one kernel's launches at the cost of two events each. */
+int dyn_profile_watch(const char* kernel);
 /* Number of distinct kernel names recorded since the last read. */
 int dyn_profile_collect(int* kernels);
 /* Entry i (0 <= i < kernels) of the last collect: name (NUL-terminated, truncated to
