@@ -24,6 +24,8 @@ namespace {
 constexpr int kBpThreads = 256;
 constexpr uint32_t kEntryPasses = 4;  // 4 key bytes, passes.rs:88
 constexpr uint32_t kResolutionBlocks = 1024;
+constexpr uint32_t kSparseLevel = 32;  // coarser levels of at most this many entries are read whole
+constexpr uint32_t kLinkStage = 16;    // link pairs a collider's counting pass keeps for the emitting pass
 
 __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
     return counters + slot * DYN_COUNTER_STRIDE_WORDS;
@@ -278,6 +280,19 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
     for (uint32_t k = blockIdx.x * blockDim.x + t; k < words; k += gridDim.x * blockDim.x) flags0[k] = 0;
 }
 
+// The entry records in sorted-key order, so the pair walks read each cell's run contiguously.
+__global__ void __launch_bounds__(kBpThreads) bp_gather_kernel(const uint32_t* __restrict__ order,
+                                                               const dyn_grid_entry* __restrict__ entries,
+                                                               const uint32_t* sort_ctl,
+                                                               dyn_grid_entry* __restrict__ sorted) {
+    const uint32_t e = blockIdx.x * kBpThreads + threadIdx.x;
+    if (e >= sort_ctl[kCtlLen]) return;
+    const float4* src = reinterpret_cast<const float4*>(entries + order[e]);
+    float4* dst = reinterpret_cast<float4*>(sorted + e);
+    dst[0] = src[0];
+    dst[1] = src[1];
+}
+
 // First sorted index of each 2^16-bucket of every occupied level (lower bound of the bucket's
 // smallest key); bucket kDirBuckets of a level holds the level's end.
 __global__ void __launch_bounds__(256) bp_directory_kernel(const uint32_t* __restrict__ keys,
@@ -333,6 +348,18 @@ __device__ __forceinline__ void key_run(const uint32_t* __restrict__ keys,
 struct PairCounter {
     uint32_t n = 0;
     __device__ __forceinline__ void operator()(uint32_t, uint32_t) { ++n; }
+};
+
+// Counts and keeps the first `limit` pairs, which the emitting pass then copies instead of
+// walking again.
+struct StagedCounter {
+    uint32_t n = 0;
+    uint2* stage;
+    uint32_t limit;
+    __device__ __forceinline__ void operator()(uint32_t a, uint32_t b) {
+        if (n < limit) stage[n] = make_uint2(a, b);
+        ++n;
+    }
 };
 
 struct PairWriter {
@@ -405,14 +432,13 @@ __device__ __forceinline__ bool mate_in_cell(const EntryView& a, const EntryView
 template <typename Sink>
 __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
                                            const uint32_t* __restrict__ keys,
-                                           const uint32_t* __restrict__ order,
-                                           const dyn_grid_entry* __restrict__ entries, float grid,
+                                           const dyn_grid_entry* __restrict__ sorted, float grid,
                                            const uint32_t* __restrict__ worlds, Sink& sink) {
     const uint32_t key = keys[index];
     const bool before = index > 0 && keys[index - 1] == key;
     const bool after = index + 1 < len && keys[index + 1] == key;
     if (!before && !after) return;
-    const EntryView self = load_entry(entries, order[index]);
+    const EntryView self = load_entry(sorted, index);
     if (((self.info & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (!(self.info & DYN_ENTRY_AWAKE)) return;
     // The entry's level is the level bits of its own key (emitted from shape_levels of this box).
@@ -421,20 +447,19 @@ __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
     entry_cell(self.mn, self.info, cs, cell);
     // Earlier mates in the run: only entries that are not awake pair from this side.
     for (uint32_t e = index; e-- > 0 && keys[e] == key;) {
-        const EntryView o = load_entry(entries, order[e]);
+        const EntryView o = load_entry(sorted, e);
         if (o.info & DYN_ENTRY_AWAKE) continue;
         if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
     }
     for (uint32_t e = index + 1; e < len && keys[e] == key; ++e) {
-        const EntryView o = load_entry(entries, order[e]);
+        const EntryView o = load_entry(sorted, e);
         if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
     }
 }
 
 // Cell mates, counting pass: one thread per sorted entry.
 __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
-    const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
+    const uint32_t* __restrict__ keys, const dyn_grid_entry* __restrict__ sorted, const uint32_t* counters,
     const uint32_t* sort_ctl, const uint32_t* __restrict__ worlds, uint32_t* mate_counts,
     uint32_t* mate_blocks) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
@@ -443,7 +468,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
     const uint32_t index = blockIdx.x * kBpThreads + threadIdx.x;
     PairCounter counter;
-    if (index < len) cell_mates(index, len, keys, order, entries, grid, worlds, counter);
+    if (index < len) cell_mates(index, len, keys, sorted, grid, worlds, counter);
     if (index < len) mate_counts[index] = counter.n;
     uint32_t total = 0;
     block_exclusive_sum<kBpThreads>(counter.n, s_scan, &total);
@@ -452,8 +477,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
 
 // Cell mates, emitting pass: the same walk, pairs written at their scanned offsets.
 __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
-    const dyn_grid_entry* __restrict__ entries, const uint32_t* counters,
+    const uint32_t* __restrict__ keys, const dyn_grid_entry* __restrict__ sorted, const uint32_t* counters,
     const uint32_t* sort_ctl, const uint32_t* __restrict__ worlds,
     const uint32_t* __restrict__ mate_counts, const uint32_t* __restrict__ mate_offsets,
     uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
@@ -466,21 +490,22 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
     const uint32_t at = mate_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
     if (mine == 0u || at >= pair_capacity) return;
     PairWriter writer{at, pair_capacity, pair_major, pair_minor};
-    cell_mates(index, len, keys, order, entries, grid, worlds, writer);
+    cell_mates(index, len, keys, sorted, grid, worlds, writer);
 }
 
 // Coarser-level links (level_links.wgsl:43-65) of one collider. Every collider's primary entry
 // (cell offset 0) is its first slot, so the probe starts from the collider's own box, flags and
 // level instead of from the sorted array; a primary beyond capacity probes nothing, as in the
-// reference.
+// reference. Only strictly coarser levels are probed, which hold none of the collider's own
+// entries, so the reference's skip of the probing entry itself never applies. Entry records are
+// read in sorted order (bp_gather_kernel), so a run is read contiguously.
 template <typename Sink>
 __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_t slot,
                                             const dyn_aabb* __restrict__ aabbs,
                                             const uint32_t* __restrict__ groups,
                                             const uint32_t* __restrict__ flags,
                                             const uint32_t* __restrict__ keys,
-                                            const uint32_t* __restrict__ order,
-                                            const dyn_grid_entry* __restrict__ entries,
+                                            const dyn_grid_entry* __restrict__ sorted,
                                             const uint32_t* __restrict__ dir, bool coarse_active,
                                             uint32_t occupied, float grid, uint32_t capacity,
                                             const uint32_t* __restrict__ worlds, Sink& sink) {
@@ -507,6 +532,25 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
         coarser &= ~(1u << link);
         const float cs = level_cell_size(link, grid);
         const Cells cells = grid_cells(self.mn, self.mx, cs);
+        const uint32_t level_first = dir[link * kDirStride], level_end = dir[link * kDirStride + kDirBuckets];
+        if (level_end - level_first <= kSparseLevel) {
+            // A sparse level is read whole: an entry is in the run of one of this box's cells c
+            // exactly when its own cell is c and its key is c's key in this world, which
+            // mate_in_cell and the key test below check, so the same pairs come out.
+            for (uint32_t e = level_first; e < level_end; ++e) {
+                const EntryView o = load_entry(sorted, e);
+                if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
+                int32_t oc[3];
+                entry_cell(o.mn, o.info, cs, oc);
+                if (oc[0] < cells.mn[0] || oc[0] > cells.mx[0] || oc[1] < cells.mn[1] || oc[1] > cells.mx[1] ||
+                    oc[2] < cells.mn[2] || oc[2] > cells.mx[2]) {
+                    continue;
+                }
+                if (keys[e] != (cell_key(link, oc[0], oc[1], oc[2]) ^ salt)) continue;
+                if (mate_in_cell(self, o, cs, oc)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
+            }
+            continue;
+        }
         const uint32_t count = cell_count(cells);
         for (uint32_t ordinal = 0; ordinal < count; ++ordinal) {
             int32_t cell[3];
@@ -514,9 +558,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             uint32_t first, end;
             key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]) ^ salt, &first, &end);
             for (uint32_t e = first; e < end; ++e) {
-                const uint32_t other = order[e];
-                if (other == slot) continue;
-                const EntryView o = load_entry(entries, other);
+                const EntryView o = load_entry(sorted, e);
                 if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
                 if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
             }
@@ -530,9 +572,9 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
-    const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
+    const dyn_grid_entry* __restrict__ sorted,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
-    const uint32_t* __restrict__ worlds, uint32_t* link_counts, uint32_t* link_blocks) {
+    const uint32_t* __restrict__ worlds, uint32_t* link_counts, uint32_t* link_blocks, uint2* link_stage) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
@@ -541,9 +583,9 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const uint32_t i = blockIdx.x * kBpThreads + threadIdx.x;
     const uint32_t packed = i < n ? cnt_level[i] : 0u;
     const uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
-    PairCounter counter;
+    StagedCounter counter{0u, link_stage + static_cast<size_t>(i) * kLinkStage, kLinkStage};
     if (i < n) {
-        level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir, coarse_active,
+        level_links(i, packed, slot, aabbs, groups, flags, keys, sorted, dir, coarse_active,
                     occupied, grid, capacity, worlds, counter);
         link_counts[i] = counter.n;
     }
@@ -557,11 +599,11 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
-    const uint32_t* __restrict__ order, const dyn_grid_entry* __restrict__ entries,
+    const dyn_grid_entry* __restrict__ sorted,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
     const uint32_t* __restrict__ worlds, const uint32_t* __restrict__ link_counts,
-    const uint32_t* __restrict__ link_offsets, uint32_t* pair_major, uint32_t* pair_minor,
-    uint32_t pair_capacity) {
+    const uint32_t* __restrict__ link_offsets, const uint2* __restrict__ link_stage,
+    uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
@@ -573,8 +615,17 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     const uint32_t mine = i < n ? link_counts[i] : 0u;
     const uint32_t at = link_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
     if (mine == 0u || at >= pair_capacity) return;
+    if (mine <= kLinkStage) {  // the counting pass kept them all
+        const uint2* kept = link_stage + static_cast<size_t>(i) * kLinkStage;
+        for (uint32_t j = 0; j < mine && at + j < pair_capacity; ++j) {
+            const uint2 p = kept[j];
+            pair_major[at + j] = p.x;
+            pair_minor[at + j] = p.y;
+        }
+        return;
+    }
     PairWriter writer{at, pair_capacity, pair_major, pair_minor};
-    level_links(i, packed, slot, aabbs, groups, flags, keys, order, entries, dir, coarse_active, occupied,
+    level_links(i, packed, slot, aabbs, groups, flags, keys, sorted, dir, coarse_active, occupied,
                 grid, capacity, worlds, writer);
 }
 
@@ -621,6 +672,8 @@ struct BpLayout {
     uint32_t* mate_blocks;
     uint32_t* dir;
     uint32_t* pair_payload;
+    dyn_grid_entry* sorted;
+    uint2* link_stage;
     SortScratch entry_sort;
     SortScratch pair_sort;
     size_t bytes;
@@ -643,6 +696,8 @@ BpLayout bp_layout(uint32_t colliders, uint32_t entry_capacity, uint32_t pair_ca
     l.mate_blocks = carve.take<uint32_t>(entry_blocks);
     l.dir = carve.take<uint32_t>(static_cast<size_t>(kGridLevels) * kDirStride);
     l.pair_payload = carve.take<uint32_t>(pair_capacity ? pair_capacity : 1u);
+    l.sorted = carve.take<dyn_grid_entry>(entry_capacity ? entry_capacity : 1u);
+    l.link_stage = carve.take<uint2>(static_cast<size_t>(colliders ? colliders : 1u) * kLinkStage);
     const uint32_t pair_passes = 2u * dyn_sort_key_words(colliders ? colliders : 1u);
     char* sort_base = base ? static_cast<char*>(base) + carve.used : nullptr;
     const size_t entry_bytes = sort_scratch_layout(entry_capacity, kEntryPasses, false, sort_base,
@@ -735,6 +790,13 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     sort_digits(kEntryPasses, 0u, digits);
     DYN_TRY(sort_passes(st, io->d_entry_keys, nullptr, io->d_entry_order, l.entry_sort, digits,
                         kEntryPasses, io->entry_capacity, kEntrySortNames));
+    const uint32_t entry_blocks = ceil_div(io->entry_capacity, kBpThreads);
+    {
+        ProfileScope scope("bp_gather_kernel", st);
+        bp_gather_kernel<<<entry_blocks, kBpThreads, 0, st>>>(io->d_entry_order, io->d_entries,
+                                                              l.entry_sort.ctl, l.sorted);
+    }
+    DYN_LAUNCHED("bp_gather_kernel");
     const uint32_t dir_threads = kGridLevels * kDirStride;
     {
         ProfileScope scope("bp_directory_kernel", st);
@@ -742,19 +804,18 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
                                                                         l.entry_sort.ctl, l.dir);
     }
     DYN_LAUNCHED("bp_directory_kernel");
-    const uint32_t entry_blocks = ceil_div(io->entry_capacity, kBpThreads);
     {
         ProfileScope scope("bp_links_count_kernel", st);
         bp_links_count_kernel<<<blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
-            io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
-            io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks);
+            io->d_entry_keys, l.sorted, l.dir, counters,
+            io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage);
     }
     DYN_LAUNCHED("bp_links_count_kernel");
     {
         ProfileScope scope("bp_mates_count_kernel", st);
         bp_mates_count_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
-            io->d_entry_keys, io->d_entry_order, io->d_entries, counters, l.entry_sort.ctl,
+            io->d_entry_keys, l.sorted, counters, l.entry_sort.ctl,
             io->d_worlds, l.mate_counts, l.mate_blocks);
     }
     DYN_LAUNCHED("bp_mates_count_kernel");
@@ -769,15 +830,15 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
             ProfileScope scope("bp_links_emit_kernel", st);
             bp_links_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
                 io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
-                io->d_entry_keys, io->d_entry_order, io->d_entries, l.dir, counters,
-                io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, io->d_pair_major,
-                io->d_pair_minor, io->pair_capacity);
+                io->d_entry_keys, l.sorted, l.dir, counters,
+                io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage,
+                io->d_pair_major, io->d_pair_minor, io->pair_capacity);
         }
         DYN_LAUNCHED("bp_links_emit_kernel");
         {
             ProfileScope scope("bp_mates_emit_kernel", st);
             bp_mates_emit_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
-                io->d_entry_keys, io->d_entry_order, io->d_entries, counters, l.entry_sort.ctl,
+                io->d_entry_keys, l.sorted, counters, l.entry_sort.ctl,
                 io->d_worlds, l.mate_counts, l.mate_blocks, io->d_pair_major, io->d_pair_minor,
                 io->pair_capacity);
         }

@@ -853,20 +853,25 @@ __device__ __forceinline__ uint32_t island_find(const W& w, uint32_t node) {
     return cur;
 }
 
-// island_link.wgsl:13-45
+// island_link.wgsl:13-45: the higher root is hooked under the lower one. The hook is an
+// atomic minimum on the higher root's parent word, which never fails: when that node is no
+// longer a root, its parent has become min(its parent, lower) and the node it pointed to is
+// linked with `lower` next, so no connection is lost. Parents only ever point to lower rows,
+// so every component ends as one tree under its minimum row, as with the reference's
+// compare-and-swap retries (which count a fault after 256 failures; this loop counts one if
+// 256 hooks do not finish it).
 __device__ inline void island_couple(const W& w, uint32_t a, uint32_t b) {
     if (b >= w.d && atomic_load_agent(&w.wake[b]) != 0u) atomicOr(&w.wake[a], 1u);
     if (a >= w.d && atomic_load_agent(&w.wake[a]) != 0u) atomicOr(&w.wake[b], 1u);
     if (a >= w.d || b >= w.d || a == b) return;
+    uint32_t ra = island_find(w, a), rb = island_find(w, b);
     for (uint32_t attempt = 0; attempt < 256u; ++attempt) {
-        const uint32_t ra = island_find(w, a), rb = island_find(w, b);
         if (ra == rb) return;
         const uint32_t lower = min(ra, rb), higher = max(ra, rb);
-        uint32_t expected = higher;
-        if (__hip_atomic_compare_exchange_strong(&w.parents[higher], &expected, lower, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            return;
-        }
+        const uint32_t was = atomicMin(&w.parents[higher], lower);
+        if (was == higher) return;  // it was a root
+        ra = island_find(w, lower);
+        rb = island_find(w, was);
     }
     atomicAdd(cw(w.counters, DYN_COUNTER_ISLAND_FAULTS), 1u);
 }
@@ -902,13 +907,24 @@ __global__ void __launch_bounds__(kThreads) k_island_flatten(W w) {
 __global__ void __launch_bounds__(kThreads) k_island_aggregate(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t roots = 0;
+    uint32_t root = kNoSlot;
+    bool raise = false;
     if (i < w.d) {
-        const uint32_t root = w.parents[i];
-        // one island holds many rows: skip the atomic once the flag is there
-        if ((w.wake[i] != 0u || w.motion[i] != 0u) && !(atomic_load_agent(&w.istate[root]) & kIslandWake)) {
-            atomicMax(&w.istate[root], kIslandWake);
-        }
+        root = w.parents[i];
+        raise = w.wake[i] != 0u || w.motion[i] != 0u;
         roots = root == i ? 1u : 0u;
+    }
+    // One island holds many neighbouring rows: one lane per distinct root of the wave raises
+    // its flag, and not at all once the flag is there.
+    uint64_t pending = __ballot(raise);
+    while (pending != 0u) {
+        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(pending))) - 1u;
+        const uint32_t target = __shfl(root, leader, kWave);
+        const uint64_t same = __ballot(raise && root == target);
+        if (lane_id() == leader && !(atomic_load_agent(&w.istate[target]) & kIslandWake)) {
+            atomicMax(&w.istate[target], kIslandWake);
+        }
+        pending &= ~same;
     }
     roots = wave_sum(roots);
     if (lane_id() == 0 && roots) atomicAdd(cw(w.counters, DYN_COUNTER_ISLANDS), roots);

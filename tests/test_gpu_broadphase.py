@@ -67,6 +67,24 @@ def test_mixed_flags_groups_and_sizes(device, oracle):
     assert np.array_equal(gpu.pairs, brute_force_pairs(boxes, groups, flags))
 
 
+def test_sparse_coarse_levels(device, oracle):
+    """A few large boxes over many small ones: the coarse levels hold a handful of entries each
+    and are read whole by the links pass."""
+    rng = np.random.default_rng(23)
+    small = uniform_boxes(3000, 40.0, 0.1, 0.6, 23)
+    centres = rng.uniform(-5.0, 45.0, size=(14, 3)).astype(np.float32)
+    halves = rng.uniform(2.0, 30.0, size=(14, 3)).astype(np.float32)
+    halves[0] = (1.0e4, 0.5, 1.0e4)  # a floor slab far larger than the rest
+    big = abi.aabbs(centres - halves, centres + halves)
+    boxes = np.concatenate([big[:7], small, big[7:]])
+    groups = rng.integers(0, 2500, size=len(boxes)).astype(np.uint32)
+    flags = rng.choice([abi.BP_MOBILE, abi.BP_AWAKE, abi.BP_MOBILE | abi.BP_AWAKE],
+                       size=len(boxes)).astype(np.uint32)
+    gpu = gpu_broadphase(boxes, groups, flags)
+    assert_matches_oracle(gpu, oracle.broadphase(boxes, groups, flags))
+    assert np.array_equal(gpu.pairs, brute_force_pairs(boxes, groups, flags))
+
+
 def test_empty_and_single(device, oracle):
     for k in (0, 1):
         boxes = uniform_boxes(k, 10.0, 0.2, 2.0, 1)
