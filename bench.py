@@ -74,12 +74,13 @@ def broadphase_bytes(name: str, k: int, entries: int, pairs: int) -> float:
 
 
 def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: int, points: float,
-               rows: int) -> float:
+               rows: int, gjk: int = 0, epa: int = 0, convex_manifolds: int = 0) -> float:
     """Minimal HBM payload of one launch of each step kernel, from the step's own counts:
     N bodies, K colliders, E entries, P pairs, C contacts of m points on average, R solver rows.
     A manifold is M(m) = 48 + 56 m bytes (normal, pair and material words plus per point anchor,
     depth, impulses and feature); pose + velocity 52 B and body constants 44 B per body."""
     manifold = 48.0 + 56.0 * points
+    manifold_pairs = convex_manifolds
     if name == "k_solve":               # manifold, two bodies' pose/velocity/constants, targets,
         return contacts * (manifold + 16.0 * points + 240.0)  # delta words
     if name == "k_position_block":
@@ -88,8 +89,12 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
         return rows * 72.0
     if name == "k_narrow":              # two bodies and colliders per pair, manifold out
         return pairs * 316.0 + contacts * manifold
-    if name == "k_narrow_convex":
-        return pairs * 316.0 + contacts * manifold
+    if name == "k_narrow_convex":       # the GJK worklist: pair record + index, parked result
+        return gjk * 320.0 + manifold_pairs * 68.0
+    if name == "k_narrow_epa":          # pair record, parked simplex in, result out
+        return epa * (316.0 + 48.0 + 20.0)
+    if name == "k_narrow_finish":       # pair record, parked result, contact out
+        return manifold_pairs * (316.0 + 68.0 + 352.0)
     if name in ("k_integrate", "k_advance"):
         return n * 126.0
     if name in ("k_relay", "k_contact_begin", "k_contacts_scatter", "k_archive"):
@@ -145,12 +150,15 @@ def world_counts(world) -> dict:
     m = world.measured()
     contacts = world.inspect_contacts()
     points = float(np.mean(contacts["point_count"])) if len(contacts) else 0.0
+    work = world.convex_work()
     return {"n": len(world.scene.states), "k": len(world.scene.colliders), "entries": m["ENTRIES"],
-            "pairs": m["PAIRS"], "contacts": m["CONTACTS"], "points": points, "rows": m["SOLVER_ROWS"]}
+            "pairs": m["PAIRS"], "contacts": m["CONTACTS"], "points": points, "rows": m["SOLVER_ROWS"],
+            "gjk": work["gjk"], "epa": work["epa"], "manifold": work["manifold"]}
 
 
 def kernel_bytes(name: str, c: dict) -> float:
-    return step_bytes(name, c["n"], c["k"], c["entries"], c["pairs"], c["contacts"], c["points"], c["rows"])
+    return step_bytes(name, c["n"], c["k"], c["entries"], c["pairs"], c["contacts"], c["points"], c["rows"],
+                      c.get("gjk", 0), c.get("epa", 0), c.get("manifold", 0))
 
 
 def timed_run(world, steps: int, warmup: int, profile_steps: int):

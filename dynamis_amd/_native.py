@@ -65,6 +65,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
                                  c_u32, c_void_p, c_void_p, c_void_p, c_u32, c_u32, c_u32, c_u32]),
     "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
     "dyn_world_set_graphs": (c_int, [c_void_p, c_int]),
+    "dyn_world_read_work": (c_int, [c_void_p, ctypes.POINTER(c_u32)]),
     "dyn_world_wait": (c_int, [c_void_p]),
     "dyn_world_stream": (c_void_p, [c_void_p]),
     "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),

@@ -25,7 +25,8 @@ constexpr int kBpThreads = 256;
 constexpr uint32_t kEntryPasses = 4;  // 4 key bytes, passes.rs:88
 constexpr uint32_t kResolutionBlocks = 1024;
 constexpr uint32_t kSparseLevel = 32;  // coarser levels of at most this many entries are read whole
-constexpr uint32_t kLinkStage = 16;    // link pairs a collider's counting pass keeps for the emitting pass
+constexpr uint32_t kLinkStage = 64;    // link pairs a collider's counting pass keeps for the emitting pass
+constexpr uint32_t kGatherRun = 8;     // mean run length of a coarser level from which records are gathered
 
 __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
     return counters + slot * DYN_COUNTER_STRIDE_WORDS;
@@ -33,13 +34,14 @@ __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
 
 
 
-__global__ void bp_reset_kernel(uint32_t* counters) {
+__global__ void bp_reset_kernel(uint32_t* counters, uint32_t* level_keys) {
     const uint32_t slots[] = {DYN_COUNTER_ENTRIES,     DYN_COUNTER_PAIRS,
                               DYN_COUNTER_REFUSED_PAIRS, DYN_COUNTER_ENTRY_FAULTS,
                               DYN_COUNTER_GRID_LEVELS, DYN_COUNTER_COARSE_ACTIVE,
                               DYN_COUNTER_GRID_SCALE,  DYN_COUNTER_GRID_EXTENT};
     const uint32_t t = threadIdx.x;
     if (t < sizeof(slots) / sizeof(slots[0])) *ctr(counters, slots[t]) = 0;
+    if (t < kGridLevels) level_keys[t] = 0u;
 }
 
 // broadphase_aabb.wgsl:21-24: the base cell is the smallest resolution box's largest side.
@@ -280,17 +282,64 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
     for (uint32_t k = blockIdx.x * blockDim.x + t; k < words; k += gridDim.x * blockDim.x) flags0[k] = 0;
 }
 
+// Whether to read entry records in sorted-key order (gathered once after the sort) or through the
+// sorted slots. Gathering pays when some coarser level holds long runs of equal keys, which every
+// finer collider probing that level scans; with short runs the gather's random reads cost more
+// than they save. Either way the same records are read.
+__device__ __forceinline__ bool gather_pays(const uint32_t* __restrict__ dir, uint32_t occupied,
+                                            const uint32_t* __restrict__ level_keys) {
+    const uint32_t coarser = occupied & (occupied - 1u);  // every occupied level but the finest
+    for (uint32_t m = coarser; m != 0u; m &= m - 1u) {
+        const uint32_t level = static_cast<uint32_t>(__ffs(m)) - 1u;
+        const uint32_t entries = dir[level * kDirStride + kDirBuckets] - dir[level * kDirStride];
+        if (entries > kSparseLevel && entries > kGatherRun * level_keys[level]) return true;
+    }
+    return false;
+}
+
+// Distinct keys of every level, one count per run start.
+__global__ void __launch_bounds__(kBpThreads) bp_level_keys_kernel(const uint32_t* __restrict__ keys,
+                                                                   const uint32_t* sort_ctl,
+                                                                   uint32_t* level_keys) {
+    __shared__ uint32_t s_levels[kGridLevels];
+    if (threadIdx.x < kGridLevels) s_levels[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t len = sort_ctl[kCtlLen];
+    // A few blocks stride over the keys, so few adds reach each level's global word.
+    for (uint32_t first = blockIdx.x * kBpThreads; first < len; first += gridDim.x * kBpThreads) {
+        const uint32_t e = first + threadIdx.x;
+        const bool start = e < len && (e == 0u || keys[e - 1] != keys[e]);
+        const uint32_t level = start ? keys[e] >> DYN_LEVEL_KEY_SHIFT : 0u;
+        // one shared add per level present in the wave
+        uint64_t pending = __ballot(start);
+        while (pending != 0u) {
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(pending))) - 1u;
+            const uint32_t target = __shfl(level, leader, kWave);
+            const uint64_t same = __ballot(start && level == target);
+            if (lane_id() == leader) atomicAdd(&s_levels[target], static_cast<uint32_t>(__popcll(same)));
+            pending &= ~same;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kGridLevels && s_levels[threadIdx.x]) atomicAdd(&level_keys[threadIdx.x], s_levels[threadIdx.x]);
+}
+
 // The entry records in sorted-key order, so the pair walks read each cell's run contiguously.
 __global__ void __launch_bounds__(kBpThreads) bp_gather_kernel(const uint32_t* __restrict__ order,
                                                                const dyn_grid_entry* __restrict__ entries,
                                                                const uint32_t* sort_ctl,
+                                                               const uint32_t* __restrict__ dir,
+                                                               const uint32_t* counters,
+                                                               const uint32_t* __restrict__ level_keys,
                                                                dyn_grid_entry* __restrict__ sorted) {
-    const uint32_t e = blockIdx.x * kBpThreads + threadIdx.x;
-    if (e >= sort_ctl[kCtlLen]) return;
-    const float4* src = reinterpret_cast<const float4*>(entries + order[e]);
-    float4* dst = reinterpret_cast<float4*>(sorted + e);
-    dst[0] = src[0];
-    dst[1] = src[1];
+    if (!gather_pays(dir, counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS], level_keys)) return;
+    const uint32_t len = sort_ctl[kCtlLen];
+    for (uint32_t e = blockIdx.x * kBpThreads + threadIdx.x; e < len; e += gridDim.x * kBpThreads) {
+        const float4* src = reinterpret_cast<const float4*>(entries + order[e]);
+        float4* dst = reinterpret_cast<float4*>(sorted + e);
+        dst[0] = src[0];
+        dst[1] = src[1];
+    }
 }
 
 // First sorted index of each 2^16-bucket of every occupied level (lower bound of the bucket's
@@ -408,6 +457,17 @@ __device__ __forceinline__ EntryView load_entry(const dyn_grid_entry* __restrict
     return e;
 }
 
+struct EntryRecords {
+    const dyn_grid_entry* sorted;   // records in key order, when gathered
+    const dyn_grid_entry* entries;  // records by slot
+    const uint32_t* order;          // slot of each sorted key
+    const uint32_t* level_keys;     // distinct keys of every level
+    bool gathered;
+    __device__ __forceinline__ EntryView at(uint32_t e) const {
+        return gathered ? load_entry(sorted, e) : load_entry(entries, order[e]);
+    }
+};
+
 __device__ __forceinline__ bool same3(const int32_t a[3], const int32_t b[3]) {
     return a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
 }
@@ -432,13 +492,13 @@ __device__ __forceinline__ bool mate_in_cell(const EntryView& a, const EntryView
 template <typename Sink>
 __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
                                            const uint32_t* __restrict__ keys,
-                                           const dyn_grid_entry* __restrict__ sorted, float grid,
+                                           const EntryRecords& rec, float grid,
                                            const uint32_t* __restrict__ worlds, Sink& sink) {
     const uint32_t key = keys[index];
     const bool before = index > 0 && keys[index - 1] == key;
     const bool after = index + 1 < len && keys[index + 1] == key;
     if (!before && !after) return;
-    const EntryView self = load_entry(sorted, index);
+    const EntryView self = rec.at(index);
     if (((self.info & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
     if (!(self.info & DYN_ENTRY_AWAKE)) return;
     // The entry's level is the level bits of its own key (emitted from shape_levels of this box).
@@ -447,19 +507,19 @@ __device__ __forceinline__ void cell_mates(uint32_t index, uint32_t len,
     entry_cell(self.mn, self.info, cs, cell);
     // Earlier mates in the run: only entries that are not awake pair from this side.
     for (uint32_t e = index; e-- > 0 && keys[e] == key;) {
-        const EntryView o = load_entry(sorted, e);
+        const EntryView o = rec.at(e);
         if (o.info & DYN_ENTRY_AWAKE) continue;
         if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
     }
     for (uint32_t e = index + 1; e < len && keys[e] == key; ++e) {
-        const EntryView o = load_entry(sorted, e);
+        const EntryView o = rec.at(e);
         if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
     }
 }
 
 // Cell mates, counting pass: one thread per sorted entry.
 __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
-    const uint32_t* __restrict__ keys, const dyn_grid_entry* __restrict__ sorted, const uint32_t* counters,
+    const uint32_t* __restrict__ keys, EntryRecords rec, const uint32_t* __restrict__ dir, const uint32_t* counters,
     const uint32_t* sort_ctl, const uint32_t* __restrict__ worlds, uint32_t* mate_counts,
     uint32_t* mate_blocks) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
@@ -468,7 +528,8 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
     const uint32_t index = blockIdx.x * kBpThreads + threadIdx.x;
     PairCounter counter;
-    if (index < len) cell_mates(index, len, keys, sorted, grid, worlds, counter);
+    rec.gathered = gather_pays(dir, counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS], rec.level_keys);
+    if (index < len) cell_mates(index, len, keys, rec, grid, worlds, counter);
     if (index < len) mate_counts[index] = counter.n;
     uint32_t total = 0;
     block_exclusive_sum<kBpThreads>(counter.n, s_scan, &total);
@@ -477,7 +538,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_count_kernel(
 
 // Cell mates, emitting pass: the same walk, pairs written at their scanned offsets.
 __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
-    const uint32_t* __restrict__ keys, const dyn_grid_entry* __restrict__ sorted, const uint32_t* counters,
+    const uint32_t* __restrict__ keys, EntryRecords rec, const uint32_t* __restrict__ dir, const uint32_t* counters,
     const uint32_t* sort_ctl, const uint32_t* __restrict__ worlds,
     const uint32_t* __restrict__ mate_counts, const uint32_t* __restrict__ mate_offsets,
     uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
@@ -490,7 +551,8 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
     const uint32_t at = mate_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
     if (mine == 0u || at >= pair_capacity) return;
     PairWriter writer{at, pair_capacity, pair_major, pair_minor};
-    cell_mates(index, len, keys, sorted, grid, worlds, writer);
+    rec.gathered = gather_pays(dir, counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS], rec.level_keys);
+    cell_mates(index, len, keys, rec, grid, worlds, writer);
 }
 
 // Coarser-level links (level_links.wgsl:43-65) of one collider. Every collider's primary entry
@@ -505,7 +567,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                                             const uint32_t* __restrict__ groups,
                                             const uint32_t* __restrict__ flags,
                                             const uint32_t* __restrict__ keys,
-                                            const dyn_grid_entry* __restrict__ sorted,
+                                            const EntryRecords& rec,
                                             const uint32_t* __restrict__ dir, bool coarse_active,
                                             uint32_t occupied, float grid, uint32_t capacity,
                                             const uint32_t* __restrict__ worlds, Sink& sink) {
@@ -538,7 +600,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             // exactly when its own cell is c and its key is c's key in this world, which
             // mate_in_cell and the key test below check, so the same pairs come out.
             for (uint32_t e = level_first; e < level_end; ++e) {
-                const EntryView o = load_entry(sorted, e);
+                const EntryView o = rec.at(e);
                 if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
                 int32_t oc[3];
                 entry_cell(o.mn, o.info, cs, oc);
@@ -558,7 +620,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             uint32_t first, end;
             key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]) ^ salt, &first, &end);
             for (uint32_t e = first; e < end; ++e) {
-                const EntryView o = load_entry(sorted, e);
+                const EntryView o = rec.at(e);
                 if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
                 if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
             }
@@ -572,7 +634,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
-    const dyn_grid_entry* __restrict__ sorted,
+    EntryRecords rec,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
     const uint32_t* __restrict__ worlds, uint32_t* link_counts, uint32_t* link_blocks, uint2* link_stage) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
@@ -583,9 +645,10 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const uint32_t i = blockIdx.x * kBpThreads + threadIdx.x;
     const uint32_t packed = i < n ? cnt_level[i] : 0u;
     const uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
+    rec.gathered = gather_pays(dir, occupied, rec.level_keys);
     StagedCounter counter{0u, link_stage + static_cast<size_t>(i) * kLinkStage, kLinkStage};
     if (i < n) {
-        level_links(i, packed, slot, aabbs, groups, flags, keys, sorted, dir, coarse_active,
+        level_links(i, packed, slot, aabbs, groups, flags, keys, rec, dir, coarse_active,
                     occupied, grid, capacity, worlds, counter);
         link_counts[i] = counter.n;
     }
@@ -599,7 +662,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
     const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
-    const dyn_grid_entry* __restrict__ sorted,
+    EntryRecords rec,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
     const uint32_t* __restrict__ worlds, const uint32_t* __restrict__ link_counts,
     const uint32_t* __restrict__ link_offsets, const uint2* __restrict__ link_stage,
@@ -624,8 +687,9 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
         }
         return;
     }
+    rec.gathered = gather_pays(dir, occupied, rec.level_keys);
     PairWriter writer{at, pair_capacity, pair_major, pair_minor};
-    level_links(i, packed, slot, aabbs, groups, flags, keys, sorted, dir, coarse_active, occupied,
+    level_links(i, packed, slot, aabbs, groups, flags, keys, rec, dir, coarse_active, occupied,
                 grid, capacity, worlds, writer);
 }
 
@@ -674,6 +738,7 @@ struct BpLayout {
     uint32_t* pair_payload;
     dyn_grid_entry* sorted;
     uint2* link_stage;
+    uint32_t* level_keys;
     SortScratch entry_sort;
     SortScratch pair_sort;
     size_t bytes;
@@ -698,6 +763,7 @@ BpLayout bp_layout(uint32_t colliders, uint32_t entry_capacity, uint32_t pair_ca
     l.pair_payload = carve.take<uint32_t>(pair_capacity ? pair_capacity : 1u);
     l.sorted = carve.take<dyn_grid_entry>(entry_capacity ? entry_capacity : 1u);
     l.link_stage = carve.take<uint2>(static_cast<size_t>(colliders ? colliders : 1u) * kLinkStage);
+    l.level_keys = carve.take<uint32_t>(kGridLevels);
     const uint32_t pair_passes = 2u * dyn_sort_key_words(colliders ? colliders : 1u);
     char* sort_base = base ? static_cast<char*>(base) + carve.used : nullptr;
     const size_t entry_bytes = sort_scratch_layout(entry_capacity, kEntryPasses, false, sort_base,
@@ -743,7 +809,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     uint32_t* counters = io->d_counters;
     {
         ProfileScope scope("bp_reset_kernel", st);
-        bp_reset_kernel<<<1, 64, 0, st>>>(counters);
+        bp_reset_kernel<<<1, 64, 0, st>>>(counters, l.level_keys);
     }
     DYN_LAUNCHED("bp_reset_kernel");
     const uint32_t blocks = ceil_div(n ? n : 1u, kBpThreads);
@@ -791,12 +857,6 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     DYN_TRY(sort_passes(st, io->d_entry_keys, nullptr, io->d_entry_order, l.entry_sort, digits,
                         kEntryPasses, io->entry_capacity, kEntrySortNames));
     const uint32_t entry_blocks = ceil_div(io->entry_capacity, kBpThreads);
-    {
-        ProfileScope scope("bp_gather_kernel", st);
-        bp_gather_kernel<<<entry_blocks, kBpThreads, 0, st>>>(io->d_entry_order, io->d_entries,
-                                                              l.entry_sort.ctl, l.sorted);
-    }
-    DYN_LAUNCHED("bp_gather_kernel");
     const uint32_t dir_threads = kGridLevels * kDirStride;
     {
         ProfileScope scope("bp_directory_kernel", st);
@@ -805,17 +865,31 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     }
     DYN_LAUNCHED("bp_directory_kernel");
     {
+        ProfileScope scope("bp_level_keys_kernel", st);
+        bp_level_keys_kernel<<<min(entry_blocks, 1024u), kBpThreads, 0, st>>>(io->d_entry_keys, l.entry_sort.ctl,
+                                                                  l.level_keys);
+    }
+    DYN_LAUNCHED("bp_level_keys_kernel");
+    {
+        ProfileScope scope("bp_gather_kernel", st);
+        bp_gather_kernel<<<min(entry_blocks, 2048u), kBpThreads, 0, st>>>(io->d_entry_order, io->d_entries,
+                                                              l.entry_sort.ctl, l.dir, counters,
+                                                              l.level_keys, l.sorted);
+    }
+    DYN_LAUNCHED("bp_gather_kernel");
+    const EntryRecords rec{l.sorted, io->d_entries, io->d_entry_order, l.level_keys, false};
+    {
         ProfileScope scope("bp_links_count_kernel", st);
         bp_links_count_kernel<<<blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
-            io->d_entry_keys, l.sorted, l.dir, counters,
+            io->d_entry_keys, rec, l.dir, counters,
             io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage);
     }
     DYN_LAUNCHED("bp_links_count_kernel");
     {
         ProfileScope scope("bp_mates_count_kernel", st);
         bp_mates_count_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
-            io->d_entry_keys, l.sorted, counters, l.entry_sort.ctl,
+            io->d_entry_keys, rec, l.dir, counters, l.entry_sort.ctl,
             io->d_worlds, l.mate_counts, l.mate_blocks);
     }
     DYN_LAUNCHED("bp_mates_count_kernel");
@@ -830,7 +904,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
             ProfileScope scope("bp_links_emit_kernel", st);
             bp_links_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
                 io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
-                io->d_entry_keys, l.sorted, l.dir, counters,
+                io->d_entry_keys, rec, l.dir, counters,
                 io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage,
                 io->d_pair_major, io->d_pair_minor, io->pair_capacity);
         }
@@ -838,7 +912,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
         {
             ProfileScope scope("bp_mates_emit_kernel", st);
             bp_mates_emit_kernel<<<entry_blocks, kBpThreads, 0, st>>>(
-                io->d_entry_keys, l.sorted, counters, l.entry_sort.ctl,
+                io->d_entry_keys, rec, l.dir, counters, l.entry_sort.ctl,
                 io->d_worlds, l.mate_counts, l.mate_blocks, io->d_pair_major, io->d_pair_minor,
                 io->pair_capacity);
         }

@@ -79,7 +79,7 @@ struct W {
     uint8_t* pkind;        // per convex pair: the unordered class of its two shape kinds
     uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
     uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
-    uint32_t* work_total;  // device word: entries on the worklist
+    uint32_t* work_total;  // device words, one per worklist mode: entries on that worklist
     uint32_t* dest;
     dyn_contact* contacts;
     uint32_t* matched;
@@ -477,7 +477,7 @@ __device__ __forceinline__ float* parked(const W& w, uint32_t index) {
 // penetrating pair's simplex for k_narrow_epa.
 __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *w.work_total) return;
+    if (t >= w.work_total[0]) return;
     const uint32_t index = w.worklist[t];
     w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
@@ -525,7 +525,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
     __shared__ EpaShared epa[kEpaWaves];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t total = *w.work_total;
+    const uint32_t total = w.work_total[2];
     const uint32_t waves = gridDim.x * kEpaWaves;
     const ShapeSet shapes{w.sources, w.vertices};
     for (uint32_t t = blockIdx.x * kEpaWaves + wave; t < total; t += waves) {
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
 // convex_hit first).
 __global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *w.work_total) return;
+    if (t >= w.work_total[1]) return;
     const uint32_t index = w.worklist[t];
     const uint32_t mark = w.valid[index];
     if (mark != kPairEpaDone && mark != kPairHit && mark != kPairPlane) return;
@@ -1612,7 +1612,7 @@ int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_
 }
 
 // The worklist of convex pairs of one mode, grouped by kind class: block histograms, their
-// class-major scan (total in work_total), the scatter.
+// class-major scan (total in work_total[mode]), the scatter.
 int worklist(dyn_world* world, uint32_t mode) {
     using namespace dyn;
     W& w = world->w;
@@ -1625,7 +1625,7 @@ int worklist(dyn_world* world, uint32_t mode) {
     DYN_LAUNCHED("k_class_count");
     {
         ProfileScope scope("k_class_scan", st);
-        k_scan_blocks<<<1, 1024, 0, st>>>(w.class_hist, kClasses * blocks, w.work_total);
+        k_scan_blocks<<<1, 1024, 0, st>>>(w.class_hist, kClasses * blocks, w.work_total + mode);
     }
     DYN_LAUNCHED("k_class_scan");
     {
@@ -1880,7 +1880,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.pkind, pair_capacity);
     A(w.worklist, pair_capacity);
     A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
-    A(w.work_total, 1);
+    A(w.work_total, 3);
     A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
     A(w.matched, contact_capacity);
@@ -2035,6 +2035,13 @@ int dyn_world_step(dyn_world* world, float dt) {
     if (!world->graphs || dyn::profiling()) return world_step(world, dt);  // profiled steps run eagerly
     if (!world->graph || world->graph_dt != dt) DYN_TRY(capture_step(world, dt));
     DYN_HIP(hipGraphLaunch(world->graph, world->stream));
+    return DYN_OK;
+}
+
+int dyn_world_read_work(dyn_world* world, uint32_t* out) {
+    if (!world || !out) return dyn::fail(DYN_E_INVALID, "dyn_world_read_work: null argument");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    DYN_HIP(hipMemcpy(out, world->w.work_total, 3u * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return DYN_OK;
 }
 

@@ -298,6 +298,13 @@ class World:
         if self.scene is not None:
             call("dyn_world_wait", self._handle)
 
+    def convex_work(self) -> dict[str, int]:
+        """Pairs the last step gave to GJK, to EPA and to the convex manifold kernel."""
+        self._ensure()
+        out = (ctypes.c_uint32 * 3)()
+        call("dyn_world_read_work", self._handle, out)
+        return {"gjk": int(out[0]), "manifold": int(out[1]), "epa": int(out[2])}
+
     def set_graphs(self, enabled: bool) -> None:
         """Replays each step as a captured HIP graph (the default) or launches it eagerly."""
         self._ensure()
