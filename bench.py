@@ -15,6 +15,8 @@ Workloads (`--workload`):
               8 solver iterations, 240 steps. Also times the same worlds stepped one at a time on
               one stream (a sample of every 16th world, scaled to all 4,096).
   broadphase  the isolated sort + broadphase pass over 1,048,576 boxes.
+  copy        calibration only: device-to-device copies of 1 GiB, whose HBM counter readings
+              (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) scale the counters of the other kernels.
 One step = one World::step of every body (broadphase: one pass over every box); body-steps/s =
 bodies x timed steps / seconds. Inputs are resident in HBM before the timed region; the timed
 region is bracketed by events on the world's own stream.
@@ -170,6 +172,12 @@ def timed_run(world, steps: int, warmup: int, profile_steps: int):
 
     from dynamis_amd.device import profile_collect, profile_enable, profile_watch
 
+    from dynamis_amd._native import call
+
+    stream = ctypes.c_void_p(world.stream())
+    first = ctypes.c_void_p()
+    call("dyn_event_create", ctypes.byref(first))
+    call("dyn_event_record", first, stream)
     probe_steps = min(profile_steps, warmup)
     for _ in range(warmup - probe_steps):
         world.step(DT)
@@ -183,7 +191,6 @@ def timed_run(world, steps: int, warmup: int, profile_steps: int):
     probe = profile_collect()
     before = world_counts(world)
     dominant = max((n for n in probe if kernel_bytes(n, before) > 0), key=lambda n: probe[n][1], default=None)
-    stream = ctypes.c_void_p(world.stream())
     if dominant:
         profile_watch(dominant)
         profile_enable(True)
@@ -191,6 +198,14 @@ def timed_run(world, steps: int, warmup: int, profile_steps: int):
     profile_enable(False)
     profile_watch(None)
     watched = profile_collect()
+    # the reference's benchmark.rs figure: its wall time includes the warm-up steps
+    last = ctypes.c_void_p()
+    call("dyn_event_create", ctypes.byref(last))
+    call("dyn_event_record", last, stream)
+    all_ms = ctypes.c_float()
+    call("dyn_event_elapsed_ms", ctypes.byref(all_ms), first, last)
+    call("dyn_event_destroy", first)
+    call("dyn_event_destroy", last)
     measured = world.measured()
     after = world_counts(world)
     profile_enable(True)
@@ -206,7 +221,35 @@ def timed_run(world, steps: int, warmup: int, profile_steps: int):
         mean = {k: 0.5 * (before[k] + after[k]) for k in before}
         timed = {"kernel": dominant, "launches": launches, "avg_us": 1000.0 * total_ms / launches,
                  "bytes_per_launch": kernel_bytes(dominant, mean)}
+    if timed is not None:
+        timed["with_warmup_ms"] = float(all_ms.value)
     return ms, stages, after, measured, timed
+
+
+def sleep_off(scene):
+    """The same scene with sleeping switched off (both velocity thresholds zero)."""
+    import copy
+    s = copy.copy(scene)
+    s.params = scene.params.copy()
+    s.descs = scene.descs.copy()
+    for field in ("sleep_velocity", "sleep_angular_velocity"):
+        s.params[field] = 0.0
+        s.descs[field] = 0.0
+    return s
+
+
+def run_sleep_off(scene, steps: int, warmup: int) -> float:
+    import ctypes
+
+    from dynamis_amd.world import World
+    world = World()
+    world.create(sleep_off(scene))
+    for _ in range(warmup):
+        world.step(DT)
+    world.wait()
+    ms = timed_steps([world], ctypes.c_void_p(world.stream()), steps)
+    world.close()
+    return ms
 
 
 def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
@@ -348,6 +391,31 @@ def run_broadphase(steps: int, warmup: int):
     return ms, BOXES, stages, counts
 
 
+def run_copy(count: int) -> int:
+    """Copies of a known size for the HBM counters' calibration; prints one JSON line."""
+    import ctypes
+
+    import dynamis_amd
+    from dynamis_amd._native import call
+    if dynamis_amd.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    call("dyn_set_device", 0)
+    nbytes = 1 << 30
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    call("dyn_malloc", ctypes.byref(src), ctypes.c_size_t(nbytes))
+    call("dyn_malloc", ctypes.byref(dst), ctypes.c_size_t(nbytes))
+    call("dyn_memset", src, 1, ctypes.c_size_t(nbytes))
+    t0 = time.perf_counter()
+    for _ in range(count):
+        call("dyn_memcpy_dtod", dst, src, ctypes.c_size_t(nbytes))
+    secs = time.perf_counter() - t0
+    call("dyn_free", src)
+    call("dyn_free", dst)
+    print(json.dumps({"workload": "copy", "bytes_per_copy": nbytes, "copies": count,
+                      "gbs": 2.0 * nbytes * count / secs / 1e9}))
+    return 0
+
+
 # ---- CPU baseline ------------------------------------------------------------------------------
 
 def cpu_world(scene, states, sample_steps: int):
@@ -380,6 +448,15 @@ def cpu_broadphase(passes: int):
 
 # ---- report ------------------------------------------------------------------------------------------
 
+def step_roofline(table: dict, profile_steps: int, ms_per_step: float) -> dict:
+    """All kernels' algorithmic bytes of one step (the profiled window's launches) over the
+    timed step time: the whole step against the HBM roof."""
+    per_step = sum(v["bytes"] * v["launches"] for v in table.values()) / max(profile_steps, 1)
+    gbs = per_step / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_step": per_step, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "unit": "GB/s"}
+
+
 def stage_table(stages, counts, bytes_of, timed=None):
     total = max(sum(t for _, t in stages.values()), 1e-9)
     table = {}
@@ -410,7 +487,7 @@ def stage_table(stages, counts, bytes_of, timed=None):
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase"], default="pile")
+    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "copy"], default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--profile-steps", type=int, default=5)
@@ -422,6 +499,8 @@ def main() -> int:
               "(independent worlds would shard by world range)", file=sys.stderr)
         return 2
     wl = args.workload
+    if wl == "copy":
+        return run_copy(max(args.steps if args.steps is not None else 5, 1))
     if wl == "broadphase":
         steps = max(args.steps if args.steps is not None else 50, 1)
         warmup = max(args.warmup if args.warmup is not None else 10, 0)
@@ -447,6 +526,8 @@ def main() -> int:
         ms, bodies, stages, counts, measured, scenes, serial, ends, timed = run_batch(
             steps, warmup, args.profile_steps, 16, 3)
         table, roofline = stage_table(stages, counts, lambda nm: kernel_bytes(nm, counts), timed)
+        if roofline is not None:
+            roofline["step"] = step_roofline(table, args.profile_steps, ms / steps)
         sample = 10 if args.cpu_sample is None else args.cpu_sample
         cpu = None
         if sample > 0:
@@ -467,6 +548,14 @@ def main() -> int:
         ms, bodies, stages, counts, measured, scene, final, timed = run_world(wl, steps, warmup,
                                                                              args.profile_steps)
         table, roofline = stage_table(stages, counts, lambda nm: kernel_bytes(nm, counts), timed)
+        if roofline is not None:
+            roofline["step"] = step_roofline(table, args.profile_steps, ms / steps)
+        sleep = None
+        if wl == "pile":
+            off_ms = run_sleep_off(scene, steps, warmup)
+            sleep = {"sleep_off_value": bodies * steps / (off_ms / 1000.0), "sleep_off_ms_per_step": off_ms / steps,
+                     "note": "the same run with sleep_velocity = sleep_angular_velocity = 0; the main value "
+                             "keeps the reference's default sleeping"}
         sample = (2 if wl == "pile" else 100) if args.cpu_sample is None else args.cpu_sample
         cpu = None
         if sample > 0:
@@ -476,7 +565,9 @@ def main() -> int:
                              f"oracle, started from the device's body states at the end of the "
                              f"timed and profiled steps ({secs:.1f} s)"}
         config = {"workload": spec["describe"], "bodies": bodies, "steps_in_run": spec["run"],
-                  "seed": 3 if wl == "pile" else 1,
+                  "seed": 3 if wl == "pile" else 1, "sleep": sleep,
+                  "reference_style_value": (bodies * (steps + warmup) / (timed["with_warmup_ms"] / 1000.0)
+                                            if timed else None),
                   "at_end": {k: measured[k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
                                                        "RESTING", "ACTIVE", "REFUSED_PAIRS",
                                                        "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
