@@ -76,7 +76,7 @@ def broadphase_bytes(name: str, k: int, entries: int, pairs: int) -> float:
 
 
 def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: int, points: float,
-               rows: int, gjk: int = 0, epa: int = 0, convex_manifolds: int = 0) -> float:
+               rows: int, gjk: int = 0, epa: int = 0, convex_manifolds: int = 0, gjk_hull: int = 0) -> float:
     """Minimal HBM payload of one launch of each step kernel, from the step's own counts:
     N bodies, K colliders, E entries, P pairs, C contacts of m points on average, R solver rows.
     A manifold is M(m) = 48 + 56 m bytes (normal, pair and material words plus per point anchor,
@@ -91,7 +91,7 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
         return rows * 72.0
     if name == "k_narrow":              # two bodies and colliders per pair, manifold out
         return pairs * 316.0 + contacts * manifold
-    if name == "k_narrow_convex":       # the GJK worklist: pair record + index, parked result
+    if name == "k_narrow_convex":       # the GJK worklist: pair record + index, parked results
         return gjk * 320.0 + manifold_pairs * 68.0
     if name == "k_narrow_epa":          # pair record, parked simplex in, result out
         return epa * (316.0 + 48.0 + 20.0)
@@ -155,12 +155,12 @@ def world_counts(world) -> dict:
     work = world.convex_work()
     return {"n": len(world.scene.states), "k": len(world.scene.colliders), "entries": m["ENTRIES"],
             "pairs": m["PAIRS"], "contacts": m["CONTACTS"], "points": points, "rows": m["SOLVER_ROWS"],
-            "gjk": work["gjk"], "epa": work["epa"], "manifold": work["manifold"]}
+            "gjk": work["gjk"], "epa": work["epa"], "manifold": work["manifold"], "gjk_hull": work["gjk_hull"]}
 
 
 def kernel_bytes(name: str, c: dict) -> float:
     return step_bytes(name, c["n"], c["k"], c["entries"], c["pairs"], c["contacts"], c["points"], c["rows"],
-                      c.get("gjk", 0), c.get("epa", 0), c.get("manifold", 0))
+                      c.get("gjk", 0), c.get("epa", 0), c.get("manifold", 0), c.get("gjk_hull", 0))
 
 
 def timed_run(world, steps: int, warmup: int, profile_steps: int):

@@ -38,12 +38,15 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // The hull vertex of largest dot(v, d), first index on ties (the sequential scan's `s >
-// best` keeps the first maximum), the lanes of a wave scanning every 64th vertex.
-__device__ inline v3 hull_support_wave(const Shape& w, const ShapeSet& ss, v3 d) {
+// best` keeps the first maximum), the kLanes lanes of an aligned lane group scanning every
+// kLanes-th vertex and reducing (value, index) pairs. All lanes of the group call it together
+// with the same arguments.
+template <int kLanes>
+__device__ inline v3 hull_support_lanes(const Shape& w, const ShapeSet& ss, v3 d) {
     const dyn_shape_source src = ss.sources[w.source];
     float best = -kBig;
     uint32_t index = 0xFFFFFFFFu;
-    for (uint32_t i = lane_id(); i < src.vertex_count; i += kWave) {
+    for (uint32_t i = lane_id() % kLanes; i < src.vertex_count; i += kLanes) {
         const float s = dot3(hull_vertex(ss, src, i), d);
         if (s > best) {
             best = s;
@@ -51,9 +54,9 @@ __device__ inline v3 hull_support_wave(const Shape& w, const ShapeSet& ss, v3 d)
         }
     }
 #pragma unroll
-    for (int offset = kWave / 2; offset > 0; offset >>= 1) {
-        const float ob = __shfl_xor(best, offset, kWave);
-        const uint32_t oi = __shfl_xor(index, offset, kWave);
+    for (int offset = kLanes / 2; offset > 0; offset >>= 1) {
+        const float ob = __shfl_xor(best, offset, kLanes);
+        const uint32_t oi = __shfl_xor(index, offset, kLanes);
         if (ob > best || (ob == best && oi < index)) {
             best = ob;
             index = oi;
@@ -79,9 +82,9 @@ __device__ inline v3 hull_support_thread(const Shape& w, const ShapeSet& ss, v3 
     return best;
 }
 
-// convex.wgsl:1-85 (hulls of at most DYN_LINEAR_SUPPORT_VERTICES vertices). With kWaveHull the
-// lanes of a wave call it together and share the hull scan.
-template <bool kWaveHull = false>
+// convex.wgsl:1-85 (hulls of at most DYN_LINEAR_SUPPORT_VERTICES vertices). With kLanes > 1 an
+// aligned group of kLanes lanes calls it together and shares the hull scan.
+template <int kLanes = 1>
 __device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
     const v3 n = norm3(d);
     if (w.kind == DYN_SHAPE_SPHERE) return mul(n, w.radius);
@@ -100,14 +103,14 @@ __device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
         return add(base, mul(norm3(radial), w.radius));
     }
     if (w.kind != DYN_SHAPE_HULL) return zero3v();
-    return kWaveHull ? hull_support_wave(w, ss, d) : hull_support_thread(w, ss, d);
+    return kLanes > 1 ? hull_support_lanes<kLanes>(w, ss, d) : hull_support_thread(w, ss, d);
 }
 
 // convex.wgsl:86-91
-template <bool kWaveHull = false>
+template <int kLanes = 1>
 __device__ inline v3 support(const Shape& w, const ShapeSet& ss, v3 direction) {
     const v3 sd = mulv(qrot(qconj(w.rotation), direction), w.scale);
-    return add(w.center, qrot(w.rotation, mulv(support_local<kWaveHull>(w, ss, sd), w.scale)));
+    return add(w.center, qrot(w.rotation, mulv(support_local<kLanes>(w, ss, sd), w.scale)));
 }
 
 // ---- GJK (convex.wgsl:93-282, 439-521) -------------------------------------------------------
@@ -331,10 +334,11 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
     return r;
 }
 
+template <int kLanes = 1>
 __device__ inline SimplexPoint support_pair(const Shape& a, const Shape& b, const ShapeSet& ss, v3 dir) {
     SimplexPoint p;
-    p.a = support(a, ss, dir);
-    p.b = support(b, ss, neg(dir));
+    p.a = support<kLanes>(a, ss, dir);
+    p.b = support<kLanes>(b, ss, neg(dir));
     p.w = sub(p.a, p.b);
     return p;
 }
@@ -358,6 +362,7 @@ __device__ inline v3 simplex_expand_dir(const SimplexPoint* s, uint32_t count) {
     return mk(0.0f, 1.0f, 0.0f);
 }
 
+template <int kLanes = 1>
 __device__ inline Closest convex_closest(const Shape& first, const Shape& second, const ShapeSet& ss,
                                          SimplexPoint* simplex, uint32_t* out_count) {
     uint32_t count = 0;
@@ -369,7 +374,7 @@ __device__ inline Closest convex_closest(const Shape& first, const Shape& second
     r.penetrating = false;
     v3 dir = sub(second.center, first.center);
     if (len3(dir) < 1e-8f) dir = mk(1.0f, 0.0f, 0.0f);
-    simplex[0] = support_pair(first, second, ss, dir);
+    simplex[0] = support_pair<kLanes>(first, second, ss, dir);
     count = 1;
     SimplexResult closest = simplex_closest(simplex, count);
     const float tiny = 1e-4f * fmn(shape_scale(first), shape_scale(second));
@@ -377,7 +382,7 @@ __device__ inline Closest convex_closest(const Shape& first, const Shape& second
         if (closest.penetrating || len3(closest.v) < tiny) {
             while (count < 4u) {
                 const v3 d = simplex_expand_dir(simplex, count);
-                set_slot(simplex, count, support_pair(first, second, ss, d));
+                set_slot(simplex, count, support_pair<kLanes>(first, second, ss, d));
                 ++count;
             }
             r.penetrating = true;
@@ -387,7 +392,7 @@ __device__ inline Closest convex_closest(const Shape& first, const Shape& second
             break;
         }
         const v3 nv = neg(closest.v);
-        const SimplexPoint sp = support_pair(first, second, ss, nv);
+        const SimplexPoint sp = support_pair<kLanes>(first, second, ss, nv);
         if (dot3(sp.w, nv) <= dot3(closest.v, nv) + 1e-4f * fmx(1.0f, dot3(closest.v, closest.v))) {
             r.distance = len3(closest.v);
             r.point_a = closest.point_a;
@@ -506,7 +511,7 @@ __device__ inline Epa epa_tetrahedron(const Shape& first, const Shape& second, c
     for (uint32_t iter = 0; iter < 32u; ++iter) {
         epa_best_face(e, face_count, &best_dist, &best_face);
         const v3 bn = face_normal(e.faces[best_face]);
-        const v3 probe = sub(support<true>(first, ss, bn), support<true>(second, ss, neg(bn)));
+        const v3 probe = sub(support<kWave>(first, ss, bn), support<kWave>(second, ss, neg(bn)));
         const float gain = dot3(probe, bn) - best_dist;
         if (gain < 1e-4f) break;
         if (vertex_count >= 64u) break;
@@ -607,9 +612,10 @@ __device__ inline Epa epa_tetrahedron(const Shape& first, const Shape& second, c
 
 // ---- penetration probe and convex_hit (convex.wgsl:523-618) ----------------------------------
 
+template <int kLanes = 1>
 __device__ __forceinline__ float support_projection_depth(const Shape& a, const Shape& b, const ShapeSet& ss, v3 dir) {
     const v3 n = norm3(dir);
-    return dot3(sub(support(a, ss, n), support(b, ss, neg(n))), n);
+    return dot3(sub(support<kLanes>(a, ss, n), support<kLanes>(b, ss, neg(n))), n);
 }
 
 struct Probe {
@@ -617,9 +623,10 @@ struct Probe {
     v3 normal;
 };
 
+template <int kLanes = 1>
 __device__ __forceinline__ void probe_direction(const Shape& first, const Shape& second, const ShapeSet& ss,
                                                 v3 direction, Probe* best) {
-    const float depth = support_projection_depth(first, second, ss, direction);
+    const float depth = support_projection_depth<kLanes>(first, second, ss, direction);
     if (depth < best->depth) {
         best->depth = depth;
         best->normal = norm3(direction);
@@ -632,6 +639,7 @@ __device__ __forceinline__ void probe_direction(const Shape& first, const Shape&
 // are generated. With `until_separated` the scan stops at the first direction whose overlap is
 // not positive: the depth is then not positive either, which is all the GJK half of convex_hit
 // reads of it (the normal of such a probe is never used).
+template <int kLanes = 1>
 __device__ inline Probe penetration_probe(const Shape& first, const Shape& second, const ShapeSet& ss,
                                           const SimplexPoint* s, uint32_t count, bool until_separated) {
     Probe best;
@@ -645,19 +653,19 @@ __device__ inline Probe penetration_probe(const Shape& first, const Shape& secon
         const v3 c = t == 0u ? s[2].w : s[3].w;
         const v3 normal = cross3(sub(b, a), sub(c, a));
         if (len3(normal) > 1e-8f) {
-            probe_direction(first, second, ss, normal, &best);
+            probe_direction<kLanes>(first, second, ss, normal, &best);
             if (until_separated && !(best.depth > 0.0f)) return best;
         }
     }
     const v3 center_dir = sub(second.center, first.center);
     if (len3(center_dir) > 1e-8f) {
-        probe_direction(first, second, ss, center_dir, &best);
+        probe_direction<kLanes>(first, second, ss, center_dir, &best);
         if (until_separated && !(best.depth > 0.0f)) return best;
     }
     for (uint32_t i = 0; i < 6; ++i) {
         const float sgn = (i & 1u) ? -1.0f : 1.0f;
         const v3 axis = i < 2u ? mk(sgn, 0.0f, 0.0f) : (i < 4u ? mk(0.0f, sgn, 0.0f) : mk(0.0f, 0.0f, sgn));
-        probe_direction(first, second, ss, axis, &best);
+        probe_direction<kLanes>(first, second, ss, axis, &best);
         if (until_separated && !(best.depth > 0.0f)) return best;
     }
     return best;
@@ -674,12 +682,13 @@ __device__ inline Hit make_hit(float distance, v3 point, v3 normal) {
 // convex_hit (convex.wgsl:584-618), GJK half: answers every pair GJK does not find penetrating
 // (returns true with the hit); for a penetrating pair returns false with GJK's final simplex,
 // which EPA (one wave) and convex_hit_finish take up.
+template <int kLanes = 1>
 __device__ inline bool convex_hit_gjk(const Shape& first, const Shape& second, const ShapeSet& ss, Hit* hit,
                                       SimplexPoint* simplex, uint32_t* count) {
-    const Closest closest = convex_closest(first, second, ss, simplex, count);
+    const Closest closest = convex_closest<kLanes>(first, second, ss, simplex, count);
     if (closest.penetrating) return false;
     const v3 centers = sub(second.center, first.center);
-    const Probe probe = penetration_probe(first, second, ss, simplex, *count, true);
+    const Probe probe = penetration_probe<kLanes>(first, second, ss, simplex, *count, true);
     if (probe.depth > 0.0f) {
         const v3 normal = dot3(probe.normal, centers) < 0.0f ? neg(probe.normal) : probe.normal;
         *hit = make_hit(-probe.depth, mul(add(closest.point_a, closest.point_b), 0.5f), normal);

@@ -79,7 +79,7 @@ struct W {
     uint8_t* pkind;        // per convex pair: the unordered class of its two shape kinds
     uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
     uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
-    uint32_t* work_total;  // device words, one per worklist mode: entries on that worklist
+    uint32_t* work_total;  // device words: entries on the worklist of each mode, then the hull-class GJK pairs
     uint32_t* dest;
     dyn_contact* contacts;
     uint32_t* matched;
@@ -344,7 +344,8 @@ __device__ __forceinline__ bool bounds_apart(const Shape& a, const Shape& b, con
 
 // ---- convex worklists: pairs grouped by the kinds of their shapes ---------------------------
 
-constexpr uint32_t kClasses = 32;  // unordered pairs of 7 kind slots (28) and spare
+constexpr uint32_t kClasses = 32;       // unordered pairs of 7 kind slots (28) and spare
+constexpr uint32_t kHullClasses = 21;   // the classes of pairs with a hull come last, from here
 constexpr uint32_t kClassItems = 16;
 constexpr uint32_t kClassBlock = kThreads * kClassItems;  // 4096 pairs per block
 constexpr uint32_t kNoClass = 0xFFu;
@@ -355,12 +356,14 @@ __device__ __forceinline__ uint32_t kind_slot(uint32_t kind) {
         case DYN_SHAPE_CUBOID: return 1u;
         case DYN_SHAPE_CAPSULE: return 2u;
         case DYN_SHAPE_CYLINDER: return 3u;
-        case DYN_SHAPE_HULL: return 4u;
-        case DYN_SHAPE_PLANE: return 5u;
-        default: return 6u;
+        case DYN_SHAPE_PLANE: return 4u;
+        case DYN_SHAPE_HULL: return 6u;
+        default: return 5u;
     }
 }
 
+// Pairs without a hull: the row-major upper triangle of slots 0-5 (classes 0-20); pairs with a
+// hull: kHullClasses + the other shape's slot (classes 21-27).
 __device__ __forceinline__ uint8_t kind_class(uint32_t first_kind, uint32_t second_kind) {
     uint32_t a = kind_slot(first_kind), b = kind_slot(second_kind);
     if (a > b) {
@@ -368,7 +371,8 @@ __device__ __forceinline__ uint8_t kind_class(uint32_t first_kind, uint32_t seco
         a = b;
         b = t;
     }
-    return static_cast<uint8_t>(a * 7u - (a * (a - 1u)) / 2u + (b - a));  // row-major upper triangle
+    if (b == 6u) return static_cast<uint8_t>(kHullClasses + a);
+    return static_cast<uint8_t>(a * 6u - (a * (a - 1u)) / 2u + (b - a));
 }
 
 // The pairs a convex kernel takes: kPairGjk (mode 0), the pairs k_narrow_finish completes
@@ -472,14 +476,18 @@ __device__ __forceinline__ float* parked(const W& w, uint32_t index) {
     return reinterpret_cast<float*>(w.raw + index);
 }
 
-// GJK of the pairs marked kPairGjk (convex_hit, convex.wgsl:584-618), one thread per pair.
-// Separated pairs end here; a hit within the margin is parked for k_narrow_finish and a
-// penetrating pair's simplex for k_narrow_epa.
-__global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
+// GJK of the pairs marked kPairGjk (convex_hit, convex.wgsl:584-618), one thread per pair of the
+// worklist. Separated pairs end here; a hit within the margin is parked for k_narrow_finish and
+// a penetrating pair's simplex for k_narrow_epa. (Sharing a hull pair's support scans among
+// four lanes measured slower: the simplex arithmetic, not the scan, sets the time.)
+__global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_blocks) {
+    constexpr int kLanes = 1;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0u) w.work_total[3] = w.work_total[0] - w.class_hist[kHullClasses * class_blocks];  // hull pairs
     if (t >= w.work_total[0]) return;
+    const bool writer = true;
     const uint32_t index = w.worklist[t];
-    w.valid[index] = kPairNone;
+    if (writer) w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
@@ -496,7 +504,8 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
     for (uint32_t i = 0; i < 4; ++i) simplex[i].w = simplex[i].a = simplex[i].b = zero3v();
     uint32_t count = 0;
     float* park = parked(w, index);
-    if (!convex_hit_gjk(wf, wsd, shapes, &hit, simplex, &count)) {
+    if (!convex_hit_gjk<kLanes>(wf, wsd, shapes, &hit, simplex, &count)) {
+        if (!writer) return;
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
             park[3 * i] = simplex[i].w.x;
@@ -506,7 +515,7 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w) {
         w.valid[index] = kPairEpa;
         return;
     }
-    if (!(hit.distance <= margin)) return;
+    if (!(hit.distance <= margin) || !writer) return;
     park[0] = hit.distance;
     park[1] = hit.point.x;
     park[2] = hit.point.y;
@@ -1246,16 +1255,29 @@ __global__ void __launch_bounds__(kThreads) k_apply(W w) {
     if (count == 0u) return;  // inert rows receive no words
     const uint32_t* list = w.adj + w.row_start[row];
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    for (uint32_t e = 0; e < count; ++e) {
-        const uint32_t entry = list[e];
-        const uint2* d = reinterpret_cast<const uint2*>(w.vwords + (entry >> 1) * 12u + (entry & 1u) * 6u);
-        const uint2 a = d[0], b = d[1], c = d[2];
-        s0 += a.x;
-        s1 += a.y;
-        s2 += b.x;
-        s3 += b.y;
-        s4 += c.x;
-        s5 += c.y;
+    // four entries' loads in flight at a time (integer sums: any order gives the same words)
+    for (uint32_t e = 0; e < count; e += 4) {
+        uint2 a[4], b[4], c[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            a[j] = b[j] = c[j] = make_uint2(0u, 0u);
+            if (e + j < count) {
+                const uint32_t entry = list[e + j];
+                const uint2* d = reinterpret_cast<const uint2*>(w.vwords + (entry >> 1) * 12u + (entry & 1u) * 6u);
+                a[j] = d[0];
+                b[j] = d[1];
+                c[j] = d[2];
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            s0 += a[j].x;
+            s1 += a[j].y;
+            s2 += b[j].x;
+            s3 += b[j].y;
+            s4 += c[j].x;
+            s5 += c[j].y;
+        }
     }
     if ((s0 | s1 | s2 | s3 | s4 | s5) == 0u) return;
     const v3 lin = mk(solver_value(s0, DYN_SOLVER_VELOCITY_SCALE), solver_value(s1, DYN_SOLVER_VELOCITY_SCALE),
@@ -1384,21 +1406,30 @@ __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
         w.contrib[row] = 0u;
     } else {
         const uint32_t* list = w.adj + w.row_start[row];
-        for (uint32_t e = 0; e < count; ++e) {
-            const uint32_t entry = list[e];
-            const uint4* d = w.pwords + 2u * (entry >> 1);
-            const uint4 head = d[0];
-            if (head.x == 0u) continue;
-            ++contributing;
-            if ((entry & 1u) == 0u) {
-                lo.x += head.y;
-                lo.y += head.z;
-                lo.z += head.w;
-            } else {
-                const uint4 tail = d[1];
-                lo.x += tail.x;
-                lo.y += tail.y;
-                lo.z += tail.z;
+        // four entries' loads in flight at a time (integer sums: any order gives the same words)
+        for (uint32_t e = 0; e < count; e += 4) {
+            uint4 word[4];
+            bool take[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                word[j] = make_uint4(0u, 0u, 0u, 0u);
+                take[j] = false;
+                if (e + j < count) {
+                    const uint32_t entry = list[e + j];
+                    const uint4* d = w.pwords + 2u * (entry >> 1);
+                    const uint4 head = d[0];
+                    take[j] = head.x != 0u;
+                    // the block's words for this side: first side after the flag, second in the tail
+                    word[j] = (entry & 1u) == 0u ? make_uint4(head.y, head.z, head.w, 0u) : d[1];
+                }
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                if (!take[j]) continue;
+                ++contributing;
+                lo.x += word[j].x;
+                lo.y += word[j].y;
+                lo.z += word[j].z;
             }
         }
         if (contributing == 0u) return;
@@ -1710,7 +1741,8 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_narrow, pb, w);
     if (world->convex_colliders) {
         DYN_TRY(worklist(world, 0u));
-        LAUNCH(k_narrow_convex, pb, w);
+        const uint32_t class_blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
+        LAUNCH(k_narrow_convex, pb, w, class_blocks);
         DYN_TRY(worklist(world, 2u));
         const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
         const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
@@ -1880,7 +1912,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.pkind, pair_capacity);
     A(w.worklist, pair_capacity);
     A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
-    A(w.work_total, 3);
+    A(w.work_total, 4);
     A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
     A(w.matched, contact_capacity);
@@ -2041,7 +2073,7 @@ int dyn_world_step(dyn_world* world, float dt) {
 int dyn_world_read_work(dyn_world* world, uint32_t* out) {
     if (!world || !out) return dyn::fail(DYN_E_INVALID, "dyn_world_read_work: null argument");
     DYN_HIP(hipStreamSynchronize(world->stream));
-    DYN_HIP(hipMemcpy(out, world->w.work_total, 3u * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    DYN_HIP(hipMemcpy(out, world->w.work_total, 4u * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return DYN_OK;
 }
 

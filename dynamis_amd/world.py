@@ -301,9 +301,9 @@ class World:
     def convex_work(self) -> dict[str, int]:
         """Pairs the last step gave to GJK, to EPA and to the convex manifold kernel."""
         self._ensure()
-        out = (ctypes.c_uint32 * 3)()
+        out = (ctypes.c_uint32 * 4)()
         call("dyn_world_read_work", self._handle, out)
-        return {"gjk": int(out[0]), "manifold": int(out[1]), "epa": int(out[2])}
+        return {"gjk": int(out[0]), "manifold": int(out[1]), "epa": int(out[2]), "gjk_hull": int(out[3])}
 
     def set_graphs(self, enabled: bool) -> None:
         """Replays each step as a captured HIP graph (the default) or launches it eagerly."""
