@@ -100,6 +100,9 @@ struct W {
     uint32_t* blocks;
     float* targets;
     float4* prows;  // per contact block: the position phase's three float4 (k_blocks)
+    float4* srows;    // per contact block, per substep: kSolveRow float4 for the velocity iterations
+    float4* sbodies;  // per row, per substep: orientation, frozen inverse inertia and mass, split count, velocities
+    float4* simp;     // per contact block, for the step's iterations: accumulated normal, tangent 1, tangent 2
     float* resolution;
     // Solver deltas without atomics: every block writes its fixed-point words to its own slots
     // and every row sums the words of the blocks on its list. The words are integers, so the
@@ -1129,18 +1132,92 @@ __device__ __forceinline__ float normal_delta(float relaxation, float damping_ra
     return truth * (wanted - accumulated) / mass;
 }
 
-// solver_contact_block.wgsl:31-193. The contact is read field by field from its record and its
-// points' impulses are applied as each point is solved: every point reads only the block's
-// bodies as they were before the block (pf, ps and their split copies), so applying point i
-// before solving point i + 1 adds the same impulses in the same order as applying them all
-// after the loop.
-__global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
+// ---- velocity iterations over per-substep rows -------------------------------------------------
+//
+// Within one substep's iterations (solver_block_solve.wgsl:56-101) only the velocities and the
+// accumulated impulses change: the bodies' poses, the masses, the split counts and every
+// quantity of a block built from them stay put. k_solver_bodies and k_solve_prep evaluate those
+// once per substep with exactly the expressions solver_contact_block.wgsl evaluates in every
+// iteration, and k_solve reads them back, so the iterations produce the same bits.
+
+constexpr uint32_t kSolveRow = 15;    // float4 per block: 3 of header, 3 per manifold point
+constexpr uint32_t kSolverBody = 5;   // float4 per row: q, inertia 0-3, inertia 4-5 mass count, v, w
+constexpr uint32_t kRowActive = 1u << 5;
+
+__device__ __forceinline__ v3 inverse_inertia_at(float4 q, const float* inertia, v3 v) {
+    return qrot(q, sym_apply(inertia, qrot(qconj(q), v)));  // apply_inverse_inertia
+}
+
+struct SolverBody {
+    float4 q;
+    float inertia[6];  // frozen (zero) for inert bodies
+    float inverse_mass;
+    float count;       // max(block count, 1), the mass split
+    v3 v, w;           // the row's velocities (k_apply keeps them equal to the body state's)
+};
+
+__device__ __forceinline__ SolverBody load_solver_body(const W& w, uint32_t row) {
+    const float4* p = w.sbodies + kSolverBody * row;
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    SolverBody r;
+    r.q = a;
+    r.inertia[0] = b.x;
+    r.inertia[1] = b.y;
+    r.inertia[2] = b.z;
+    r.inertia[3] = b.w;
+    r.inertia[4] = c.x;
+    r.inertia[5] = c.y;
+    r.inverse_mass = c.z;
+    r.count = c.w;
+    r.v = mk(d.x, d.y, d.z);
+    r.w = mk(e.x, e.y, e.z);
+    return r;
+}
+
+// solver_block_solve.wgsl:24-54 for every claimed row: the frozen or unsplit body of a block.
+__global__ void __launch_bounds__(kThreads) k_solver_bodies(W w) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= w.n || w.block_counts[row] == 0u) return;
+    Body b = load_body(w, row);
+    if (body_is_inert(b)) body_frozen(b);
+    float4* p = w.sbodies + kSolverBody * row;
+    p[0] = ld4(b.s.orientation);
+    p[1] = make_float4(b.d.inverse_inertia[0], b.d.inverse_inertia[1], b.d.inverse_inertia[2],
+                       b.d.inverse_inertia[3]);
+    p[2] = make_float4(b.d.inverse_inertia[4], b.d.inverse_inertia[5], b.d.inverse_mass,
+                       static_cast<float>(max(w.block_counts[row], 1u)));
+    p[3] = make_float4(b.s.velocity[0], b.s.velocity[1], b.s.velocity[2], 0.0f);
+    p[4] = make_float4(b.s.angular_velocity[0], b.s.angular_velocity[1], b.s.angular_velocity[2], 0.0f);
+}
+
+// solver_contact_block.wgsl:31-110, the parts of a block that depend on poses and masses only:
+// each point's arms from the centres of mass, its row masses, target speed and depth.
+__global__ void __launch_bounds__(kThreads) k_solve_prep(W w, uint32_t load_impulses) {
     const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
     if (ci >= *w.ncontacts) return;
-    dyn_contact& contact = w.contacts[ci];
+    float4* row = w.srows + static_cast<size_t>(kSolveRow) * ci;
+    const dyn_contact& contact = w.contacts[ci];
     const uint32_t point_count = contact.point_count;
-    if (point_count == 0u || contact.sensor != 0u) return;  // block_resolves
     const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
+    if (point_count == 0u || contact.sensor != 0u) {  // block_resolves
+        row[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(0u), 0.0f);
+        return;
+    }
+    if (load_impulses) {  // the step's iterations keep the impulses in simp (k_impulses_store)
+        float an[4] = {0.0f, 0.0f, 0.0f, 0.0f}, a1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, a2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t pi = 0; pi < 4; ++pi) {
+            if (pi < point_count) {
+                an[pi] = contact.points[pi].accumulated_normal;
+                a1[pi] = contact.points[pi].accumulated_tangent_1;
+                a2[pi] = contact.points[pi].accumulated_tangent_2;
+            }
+        }
+        float4* imp = w.simp + 3u * ci;
+        imp[0] = make_float4(an[0], an[1], an[2], an[3]);
+        imp[1] = make_float4(a1[0], a1[1], a1[2], a1[3]);
+        imp[2] = make_float4(a2[0], a2[1], a2[2], a2[3]);
+    }
     Body pf = load_body(w, fb), ps = load_body(w, sb);
     Body split_f = pf, split_s = ps;
     const float c1 = static_cast<float>(max(w.block_counts[fb], 1u));
@@ -1160,91 +1237,164 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
         body_frozen(ps);
         body_frozen(split_s);
     }
-    if (fi && si) return;  // both inert: zero words (commit_block of zeros)
-    Body first = pf, second = ps;
+    const uint32_t flags = point_count | (fi ? 8u : 0u) | (si ? 16u : 0u) | ((fi && si) ? 0u : kRowActive);
     const v3 normal = ld3(contact.normal);
-    const float friction = contact.friction, relaxation = contact.relaxation;
-    const float damping_ratio = contact.damping_ratio;
+    row[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(flags), contact.friction);
+    row[1] = make_float4(normal.x, normal.y, normal.z, contact.relaxation);
+    row[2] = make_float4(contact.damping_ratio, contact.rolling_friction, contact.spin_friction, 0.0f);
+    if (fi && si) return;  // both inert: zero words (commit_block of zeros)
     v3 t1, t2;
     make_tangents(normal, &t1, &t2);
     v3 reach_a = zero3v(), reach_b = zero3v();
     const v3 com_f = body_com(pf), com_s = body_com(ps);
-    float normal_total = 0.0f;
     for (uint32_t pi = 0; pi < point_count; ++pi) {
         const dyn_manifold_point& p = contact.points[pi];
         reach_a = add(reach_a, sub(arm(pf, p, true), com_f));
         reach_b = add(reach_b, sub(arm(ps, p, false), com_s));
-        normal_total = normal_total + fmx(p.accumulated_normal, 0.0f);
     }
     const float count = static_cast<float>(max(point_count, 1u));
     for (uint32_t pi = 0; pi < point_count; ++pi) {
-        dyn_manifold_point& p = contact.points[pi];
+        const dyn_manifold_point& p = contact.points[pi];
         const v3 aa = arm(pf, p, true), ab = arm(ps, p, false);
-        float an = p.accumulated_normal, at1 = p.accumulated_tangent_1, at2 = p.accumulated_tangent_2;
-        const float depth = p.depth;
-        v3 impulse = zero3v();
-        const float nspeed = dot3(relative_velocity(pf, ps, aa, ab), normal);
+        const v3 ra = sub(aa, body_com(pf)), rb = sub(ab, body_com(ps));  // point_velocity's arms
         const v3 arm_a = cross3(sub(aa, body_com(split_f)), normal);
         const v3 arm_b = cross3(sub(ab, body_com(split_s)), normal);
         const float nmass = count * (split_f.d.inverse_mass + split_s.d.inverse_mass) +
                             dot3(apply_inverse_inertia(split_f, cross3(reach_a, normal)), arm_a) +
                             dot3(apply_inverse_inertia(split_s, cross3(reach_b, normal)), arm_b);
         const float truth = momentum_mass(pf, ps, aa, ab, normal);
-        const float delta = normal_delta(relaxation, damping_ratio, nmass, truth, w.p.dt,
-                                         w.targets[ci * 4u + pi], depth, nspeed, an);
+        const float m1 = momentum_mass(split_f, split_s, aa, ab, t1) * count;
+        const float m2 = momentum_mass(split_f, split_s, aa, ab, t2) * count;
+        float4* q = row + 3u + 3u * pi;
+        q[0] = make_float4(ra.x, ra.y, ra.z, nmass);
+        q[1] = make_float4(rb.x, rb.y, rb.z, truth);
+        q[2] = make_float4(m1, m2, w.targets[ci * 4u + pi], p.depth);
+    }
+}
+
+// solver_contact_block.wgsl:31-193 over the block's row: the points' impulses are applied as
+// each point is solved (every point reads only the velocities the block started from, so this
+// adds the same impulses in the same order as applying them all after the loop).
+__global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= *w.ncontacts) return;
+    const float4* row = w.srows + static_cast<size_t>(kSolveRow) * ci;
+    const float4 h0 = row[0];
+    const uint32_t flags = __float_as_uint(h0.z);
+    if (!(flags & kRowActive)) return;  // not resolved, or both bodies inert
+    const uint32_t fb = __float_as_uint(h0.x), sb = __float_as_uint(h0.y);
+    const bool fi = (flags & 8u) != 0u, si = (flags & 16u) != 0u;
+    const uint32_t point_count = flags & 7u;
+    const float friction = h0.w;
+    const float4 h1 = row[1], h2 = row[2];
+    const v3 normal = mk(h1.x, h1.y, h1.z);
+    const float relaxation = h1.w, damping_ratio = h2.x;
+    const SolverBody bf = load_solver_body(w, fb), bs = load_solver_body(w, sb);
+    float split_f[6], split_s[6];
+    for (int k = 0; k < 6; ++k) {
+        split_f[k] = bf.inertia[k] * bf.count;
+        split_s[k] = bs.inertia[k] * bs.count;
+    }
+    const v3 vf = bf.v, wf = bf.w, vs = bs.v, ws = bs.w;
+    v3 first_v = vf, first_w = wf, second_v = vs, second_w = ws;
+    v3 t1, t2;
+    make_tangents(normal, &t1, &t2);
+    float4* imp = w.simp + 3u * ci;
+    const float4 n4 = imp[0], t14 = imp[1], t24 = imp[2];
+    float acc_n[4] = {n4.x, n4.y, n4.z, n4.w}, acc_1[4] = {t14.x, t14.y, t14.z, t14.w};
+    float acc_2[4] = {t24.x, t24.y, t24.z, t24.w};
+    float normal_total = 0.0f;
+#pragma unroll
+    for (uint32_t pi = 0; pi < 4; ++pi) {
+        if (pi < point_count) normal_total = normal_total + fmx(acc_n[pi], 0.0f);
+    }
+#pragma unroll
+    for (uint32_t pi = 0; pi < 4; ++pi) {
+        if (pi >= point_count) continue;
+        const float4* q = row + 3u + 3u * pi;
+        const float4 qa = q[0], qb = q[1], qc = q[2];
+        const v3 ra = mk(qa.x, qa.y, qa.z), rb = mk(qb.x, qb.y, qb.z);
+        const float nmass = qa.w, truth = qb.w, m1 = qc.x, m2 = qc.y, target = qc.z, depth = qc.w;
+        float an = acc_n[pi], at1 = acc_1[pi], at2 = acc_2[pi];
+        // relative_velocity of the velocities the block started from
+        const v3 rel = sub(add(vs, cross3(ws, rb)), add(vf, cross3(wf, ra)));
+        v3 impulse = zero3v();
+        const float nspeed = dot3(rel, normal);
+        const float delta = normal_delta(relaxation, damping_ratio, nmass, truth, w.p.dt, target, depth,
+                                         nspeed, an);
         const float next_n = fmx(0.0f, an + delta);
         impulse = add(impulse, mul(normal, next_n - an));
         an = next_n;
         const float limit = friction * an;
-        const float m1 = momentum_mass(split_f, split_s, aa, ab, t1) * count;
-        const float s1 = dot3(relative_velocity(pf, ps, aa, ab), t1);
+        const float s1 = dot3(rel, t1);
         const float next_1 = clampf(at1 - s1 / m1, -limit, limit);
         impulse = add(impulse, mul(t1, next_1 - at1));
         at1 = next_1;
-        const float m2 = momentum_mass(split_f, split_s, aa, ab, t2) * count;
-        const float s2 = dot3(relative_velocity(pf, ps, aa, ab), t2);
+        const float s2 = dot3(rel, t2);
         const float remaining = sqrtf(fmx(limit * limit - at1 * at1, 0.0f));
         const float next_2 = clampf(at2 - s2 / m2, -remaining, remaining);
         impulse = add(impulse, mul(t2, next_2 - at2));
         at2 = next_2;
-        p.accumulated_normal = an;
-        p.accumulated_tangent_1 = at1;
-        p.accumulated_tangent_2 = at2;
-        apply_pair_impulse(first, second, aa, ab, impulse);
+        acc_n[pi] = an;
+        acc_1[pi] = at1;
+        acc_2[pi] = at2;
+        // apply_pair_impulse on the unsplit (frozen when inert) bodies
+        first_v = sub(first_v, mul(impulse, bf.inverse_mass));
+        first_w = sub(first_w, inverse_inertia_at(bf.q, bf.inertia, cross3(ra, impulse)));
+        second_v = add(second_v, mul(impulse, bs.inverse_mass));
+        second_w = add(second_w, inverse_inertia_at(bs.q, bs.inertia, cross3(rb, impulse)));
     }
-    const v3 rel_spin = sub(ld3(second.s.angular_velocity), ld3(first.s.angular_velocity));
+    const v3 rel_spin = sub(second_w, first_w);
     const v3 planar = sub(rel_spin, mul(normal, dot3(rel_spin, normal)));
     const float planar_len = len3(planar);
     if (planar_len > 1e-6f) {
         const v3 axis = dv(planar, planar_len);
-        const float mass = dot3(axis, apply_inverse_inertia(split_f, axis)) +
-                           dot3(axis, apply_inverse_inertia(split_s, axis));
+        const float mass = dot3(axis, inverse_inertia_at(bf.q, split_f, axis)) +
+                           dot3(axis, inverse_inertia_at(bs.q, split_s, axis));
         if (mass > 0.0f) {
-            const float lim = contact.rolling_friction * normal_total;
+            const float lim = h2.y * normal_total;
             const float applied = clampf(-planar_len / mass, -lim, lim);
-            st3(first.s.angular_velocity,
-                sub(ld3(first.s.angular_velocity), apply_inverse_inertia(first, mul(axis, applied))));
-            st3(second.s.angular_velocity,
-                add(ld3(second.s.angular_velocity), apply_inverse_inertia(second, mul(axis, applied))));
+            first_w = sub(first_w, inverse_inertia_at(bf.q, bf.inertia, mul(axis, applied)));
+            second_w = add(second_w, inverse_inertia_at(bs.q, bs.inertia, mul(axis, applied)));
         }
     }
     const float twist = dot3(rel_spin, normal);
     if (fabsf(twist) > 1e-6f) {
-        const float mass = dot3(normal, apply_inverse_inertia(split_f, normal)) +
-                           dot3(normal, apply_inverse_inertia(split_s, normal));
+        const float mass = dot3(normal, inverse_inertia_at(bf.q, split_f, normal)) +
+                           dot3(normal, inverse_inertia_at(bs.q, split_s, normal));
         if (mass > 0.0f) {
-            const float lim = contact.spin_friction * normal_total;
+            const float lim = h2.z * normal_total;
             const float applied = clampf(-twist / mass, -lim, lim);
-            st3(first.s.angular_velocity,
-                sub(ld3(first.s.angular_velocity), apply_inverse_inertia(first, mul(normal, applied))));
-            st3(second.s.angular_velocity,
-                add(ld3(second.s.angular_velocity), apply_inverse_inertia(second, mul(normal, applied))));
+            first_w = sub(first_w, inverse_inertia_at(bf.q, bf.inertia, mul(normal, applied)));
+            second_w = add(second_w, inverse_inertia_at(bs.q, bs.inertia, mul(normal, applied)));
         }
     }
-    commit_block(w, ci, fi, si, sub(ld3(first.s.velocity), ld3(pf.s.velocity)),
-                 sub(ld3(first.s.angular_velocity), ld3(pf.s.angular_velocity)),
-                 sub(ld3(second.s.velocity), ld3(ps.s.velocity)),
-                 sub(ld3(second.s.angular_velocity), ld3(ps.s.angular_velocity)), residual != 0u);
+    imp[0] = make_float4(acc_n[0], acc_n[1], acc_n[2], acc_n[3]);
+    imp[1] = make_float4(acc_1[0], acc_1[1], acc_1[2], acc_1[3]);
+    imp[2] = make_float4(acc_2[0], acc_2[1], acc_2[2], acc_2[3]);
+    commit_block(w, ci, fi, si, sub(first_v, vf), sub(first_w, wf), sub(second_v, vs), sub(second_w, ws),
+                 residual != 0u);
+}
+
+// The iterations' impulses back into the contact records, for the archive and the next step.
+__global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= *w.ncontacts) return;
+    dyn_contact& contact = w.contacts[ci];
+    const uint32_t point_count = contact.point_count;
+    if (point_count == 0u || contact.sensor != 0u) return;
+    const float4* imp = w.simp + 3u * ci;
+    const float4 n4 = imp[0], t14 = imp[1], t24 = imp[2];
+    const float an[4] = {n4.x, n4.y, n4.z, n4.w}, a1[4] = {t14.x, t14.y, t14.z, t14.w};
+    const float a2[4] = {t24.x, t24.y, t24.z, t24.w};
+#pragma unroll
+    for (uint32_t pi = 0; pi < 4; ++pi) {
+        if (pi < point_count) {
+            contact.points[pi].accumulated_normal = an[pi];
+            contact.points[pi].accumulated_tangent_1 = a1[pi];
+            contact.points[pi].accumulated_tangent_2 = a2[pi];
+        }
+    }
 }
 
 // solver_block_apply.wgsl over every claimed row
@@ -1286,8 +1436,12 @@ __global__ void __launch_bounds__(kThreads) k_apply(W w) {
                       solver_value(s5, DYN_SOLVER_VELOCITY_SCALE));
     if (zero3(lin) && zero3(ang)) return;
     dyn_body_state& s = w.st[row];
-    st3(s.velocity, add(ld3(s.velocity), lin));
-    st3(s.angular_velocity, add(ld3(s.angular_velocity), ang));
+    const v3 v = add(ld3(s.velocity), lin), av = add(ld3(s.angular_velocity), ang);
+    st3(s.velocity, v);
+    st3(s.angular_velocity, av);
+    float4* p = w.sbodies + kSolverBody * row;  // the solver body's copy
+    p[3] = make_float4(v.x, v.y, v.z, 0.0f);
+    p[4] = make_float4(av.x, av.y, av.z, 0.0f);
 }
 
 // substep_integrate.wgsl:15-76 over the awake dynamic rows (no fields)
@@ -1780,6 +1934,8 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_fill_lists, cb, w);
     for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
         LAUNCH(k_integrate, db, w);
+        LAUNCH(k_solver_bodies, nb, w);
+        LAUNCH(k_solve_prep, cb, w, sub == 0 ? 1u : 0u);
         if (sub == 0) {
             LAUNCH(k_warm, cb, w);
             LAUNCH(k_apply, nb, w);
@@ -1797,6 +1953,7 @@ int world_step(dyn_world* world, float dt) {
             LAUNCH(k_position_apply, nb, w);
         }
     }
+    LAUNCH(k_impulses_store, cb, w);
     LAUNCH(k_sleep, db, w);
     LAUNCH(k_thaw, cb, w);
     LAUNCH(k_archive, cb, w, world->freeze_flags);
@@ -1929,6 +2086,9 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.blocks, 2u * static_cast<size_t>(contact_capacity));
     A(w.targets, 4u * static_cast<size_t>(contact_capacity));
     A(w.prows, 3u * static_cast<size_t>(contact_capacity));
+    A(w.srows, static_cast<size_t>(kSolveRow) * contact_capacity);
+    A(w.sbodies, static_cast<size_t>(kSolverBody) * body_count);
+    A(w.simp, 3u * static_cast<size_t>(contact_capacity));
     A(w.resolution, 4u * static_cast<size_t>(body_count));
     A(w.slots, 2u * static_cast<size_t>(contact_capacity));
     A(w.list_counts, body_count);
