@@ -80,6 +80,8 @@ struct W {
     uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
     uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
     uint32_t* work_total;  // device words: entries on the worklist of each mode, then the hull-class GJK pairs
+    uint32_t* work_keys;   // GJK worklist sort keys: class, then the hull sources
+    uint32_t* work_minor;  // the sort's unused minor lane
     uint32_t* dest;
     dyn_contact* contacts;
     uint32_t* matched;
@@ -419,6 +421,25 @@ __global__ void __launch_bounds__(kThreads) k_class_scatter(W w, uint32_t mode) 
         const uint32_t c = work_class(w, index, mode);
         if (c != kNoClass) w.worklist[atomicAdd(&cursor[c], 1u)] = index;
     }
+}
+
+// Within a worklist's classes (GJK's and the manifolds'), pairs that share hull sources run side
+// by side, so a wave's
+// lanes mostly scan the same hull's vertices (one cache line per load instruction): the key is
+// the class, then the first and the second shape's hull source (sources from 511 on, and shapes
+// that are not hulls, share the last value). Every pair's result goes to its own slot, so the
+// order changes no result.
+constexpr uint32_t kWorkKeyBytes = 3;
+__global__ void __launch_bounds__(kThreads) k_work_keys(W w, uint32_t mode) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w.work_total[mode]) return;
+    const uint32_t index = w.worklist[t];
+    const dyn_collider& fc = w.co[w.pmaj[index]];
+    const dyn_collider& sc = w.co[w.pmin[index]];
+    const uint32_t fa = fc.kind == DYN_SHAPE_HULL ? min(fc.source, 511u) : 511u;
+    const uint32_t fb = sc.kind == DYN_SHAPE_HULL ? min(sc.source, 511u) : 511u;
+    w.work_keys[t] = (static_cast<uint32_t>(w.pkind[index]) << 18) | (fa << 9) | fb;
+    w.work_minor[t] = 0u;
 }
 
 // Analytic pairs; convex-path pairs are marked for the kernels below.
@@ -1741,6 +1762,8 @@ struct dyn_world {
     size_t bp_scratch_bytes = 0;
     void* sort_scratch = nullptr;
     size_t sort_scratch_bytes = 0;
+    void* work_scratch = nullptr;  // the GJK worklist sort
+    size_t work_scratch_bytes = 0;
     std::vector<void*> owned;
     void* shape_sources = nullptr;   // owned separately: replaced by each shape upload
     void* shape_vertices = nullptr;
@@ -1896,6 +1919,9 @@ int world_step(dyn_world* world, float dt) {
     if (world->convex_colliders) {
         DYN_TRY(worklist(world, 0u));
         const uint32_t class_blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
+        LAUNCH(k_work_keys, pb, w, 0u);
+        DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[0], nullptr,
+                               1u, kWorkKeyBytes, 0u, world->work_scratch, world->work_scratch_bytes));
         LAUNCH(k_narrow_convex, pb, w, class_blocks);
         DYN_TRY(worklist(world, 2u));
         const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
@@ -1906,6 +1932,9 @@ int world_step(dyn_world* world, float dt) {
         }
         DYN_LAUNCHED("k_narrow_epa");
         DYN_TRY(worklist(world, 1u));
+        LAUNCH(k_work_keys, pb, w, 1u);
+        DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[1], nullptr,
+                               1u, kWorkKeyBytes, 0u, world->work_scratch, world->work_scratch_bytes));
         LAUNCH(k_narrow_finish, pb, w);
     }
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
@@ -2070,6 +2099,8 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.worklist, pair_capacity);
     A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
     A(w.work_total, 4);
+    A(w.work_keys, pair_capacity);
+    A(w.work_minor, pair_capacity);
     A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
     A(w.matched, contact_capacity);
@@ -2121,6 +2152,11 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     }
     world->owned.push_back(world->bp_scratch);
     world->owned.push_back(world->sort_scratch);
+    world->work_scratch_bytes = dyn_sort_scratch_bytes(pair_capacity, kWorkKeyBytes);
+    if (hipMalloc(&world->work_scratch, world->work_scratch_bytes) != hipSuccess) {
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: scratch"));
+    }
+    world->owned.push_back(world->work_scratch);
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (bytes && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = DYN_E_HIP;
     };
