@@ -47,6 +47,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 METRIC = "body-steps/s (100k convex pile) + batched vs plain; % HBM peak; bwd/fwd ratio"
 DT = np.float32(1.0 / 60.0)
+GRAPHS = True  # steps replay the captured graph; --no-graphs launches them eagerly
 BOXES = 1 << 20
 BOX_SEED = 2
 
@@ -244,6 +245,7 @@ def run_sleep_off(scene, steps: int, warmup: int) -> float:
     from dynamis_amd.world import World
     world = World()
     world.create(sleep_off(scene))
+    world.set_graphs(GRAPHS)
     for _ in range(warmup):
         world.step(DT)
     world.wait()
@@ -263,6 +265,7 @@ def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     scene = WORKLOADS[workload]["scene"]()
     world = World()
     world.create(scene)
+    world.set_graphs(GRAPHS)
     ms, stages, counts, measured, timed = timed_run(world, steps, warmup, profile_steps)
     final_states = world.states()
     world.close()
@@ -302,6 +305,7 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
     call("dyn_set_device", 0)
     scenes = batch_scenes()
     batch = Batch(scenes)
+    batch.set_graphs(GRAPHS)
     ms, stages, counts, m, timed = timed_run(batch, steps, warmup, profile_steps)
     bodies = len(batch.scene.states)
     final = batch.states()
@@ -318,6 +322,7 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
         for s in sampled:
             w = World()
             w.create(s, stream=shared.value)
+            w.set_graphs(GRAPHS)
             worlds.append(w)
         timed_steps(worlds, shared, warmup)  # the batch's untimed steps (and graph capture)
         sms = timed_steps(worlds, shared, serial_steps)
@@ -491,9 +496,14 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--profile-steps", type=int, default=5)
+    parser.add_argument("--no-graphs", action="store_true",
+                        help="launch every step eagerly (for external profilers, which trace the "
+                             "replayed graph of a captured step badly)")
     parser.add_argument("--cpu-sample", type=int, default=None,
                         help="oracle steps (broadphase: passes) timed for the CPU baseline; 0 skips it")
     args = parser.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent worlds would shard by world range)", file=sys.stderr)
