@@ -66,6 +66,15 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
     "dyn_world_set_graphs": (c_int, [c_void_p, c_int]),
     "dyn_world_read_work": (c_int, [c_void_p, ctypes.POINTER(c_u32)]),
+    "dyn_world_snapshot_bytes": (c_int, [c_void_p, ctypes.POINTER(c_size_t)]),
+    "dyn_world_snapshot": (c_int, [c_void_p, c_void_p]),
+    "dyn_world_restore": (c_int, [c_void_p, c_void_p]),
+    "dyn_adjoint_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_u32, c_u32, c_void_p, c_u32]),
+    "dyn_adjoint_forward": (c_int, [c_void_p, ctypes.c_float]),
+    "dyn_adjoint_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dyn_adjoint_read_tape": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    "dyn_adjoint_read_bodies": (c_int, [c_void_p, c_void_p]),
+    "dyn_adjoint_destroy": (c_int, [c_void_p]),
     "dyn_world_wait": (c_int, [c_void_p]),
     "dyn_world_stream": (c_void_p, [c_void_p]),
     "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),

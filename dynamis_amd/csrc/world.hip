@@ -15,6 +15,7 @@
 //    never read: the mass split of an inert body is frozen to zero);
 //  - the resting archive is kept sorted by (min body id, max body id) for the contact_begin
 //    lookup (the reference sorts an index over a free-list archive, commit.rs:278-295).
+#include <algorithm>
 #include <vector>
 
 #include "convex.h"
@@ -118,6 +119,7 @@ struct W {
     uint32_t* contrib;      // inert rows: marked when a block of theirs contributed
     uint32_t* scan_blocks;   // per-block sums of compaction scans
     uint32_t* scan_total;    // two words: totals of the last scans
+    uint32_t* relay_src;     // while a tape is recorded: per contact point, the archive point relayed into it
 };
 
 __device__ __forceinline__ Body load_body(const W& w, uint32_t row) {
@@ -762,7 +764,7 @@ __device__ __forceinline__ void carry(dyn_contact& c) {
 }
 
 // identity.wgsl:85-117
-__device__ inline void relay_impulses(dyn_contact& r, const dyn_contact& held) {
+__device__ inline void relay_impulses(dyn_contact& r, const dyn_contact& held, uint32_t* picks = nullptr) {
     bool taken[4] = {false, false, false, false};
     for (uint32_t pi = 0; pi < r.point_count; ++pi) {
         const dyn_manifold_point& point = r.points[pi];
@@ -781,6 +783,7 @@ __device__ inline void relay_impulses(dyn_contact& r, const dyn_contact& held) {
         }
         if (picked == kNoSlot) continue;
         taken[picked] = true;
+        if (picks) picks[pi] = picked;
         r.points[pi].accumulated_normal = held.points[picked].accumulated_normal;
         r.points[pi].accumulated_tangent_1 = held.points[picked].accumulated_tangent_1;
         r.points[pi].accumulated_tangent_2 = held.points[picked].accumulated_tangent_2;
@@ -816,7 +819,12 @@ __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     if (!same_pair(held, cur)) return;
     if (!(held.events & kContactAnnounced)) return;
     w.matched[lo] = 1u;
-    relay_impulses(cur, held);
+    uint32_t picks[4] = {kNoSlot, kNoSlot, kNoSlot, kNoSlot};
+    relay_impulses(cur, held, picks);
+    if (w.relay_src) {
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) w.relay_src[4u * lo + q] = picks[q] == kNoSlot ? kNoSlot : 4u * index + picks[q];
+    }
     carry(cur);
     cur.events |= kContactAnnounced;
     w.contacts[lo] = cur;
@@ -858,6 +866,76 @@ __global__ void __launch_bounds__(kThreads) k_contact_begin(W w, float slop) {
     }
     if (touches(c, slop)) {
         w.contacts[index].events = c.events | kContactAnnounced;
+    }
+}
+
+// ---- the differentiable rollout's tape ---------------------------------------------------------
+//
+// While a tape is recorded, each step writes the states it starts from and its contacts as the
+// solver receives them (after the warm-start relay) into the layouts of dynamis_amd/csrc/adjoint.h:
+// per row 16 floats (position, orientation, velocity, spin), per contact 76 floats (rows, flags,
+// colliders, normal, combined friction and its partials, restitution; per point the anchors,
+// depth, world position, relay source and warm-start impulses).
+
+__global__ void __launch_bounds__(kThreads) k_tape_sources(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4u * w.contact_cap) w.relay_src[i] = kNoSlot;
+}
+
+__device__ __forceinline__ void friction_partials(float a, float b, uint32_t mode, float* da, float* db) {
+    if (mode == 1u) {  // min
+        *da = a < b ? 1.0f : 0.0f;
+        *db = a < b ? 0.0f : 1.0f;
+    } else if (mode == 2u) {  // max
+        *da = a > b ? 1.0f : 0.0f;
+        *db = a > b ? 0.0f : 1.0f;
+    } else if (mode == 3u) {  // average
+        *da = *db = 0.5f;
+    } else {  // geometric mean
+        const float c = sqrtf(a * b);
+        *da = c > 0.0f ? b / (2.0f * c) : 0.0f;
+        *db = c > 0.0f ? a / (2.0f * c) : 0.0f;
+    }
+}
+
+// error bits: 1 a resolving contact is soft or has rolling or spin friction, 2 a body sleeps
+__global__ void __launch_bounds__(kThreads) k_tape_record(W w, float* states, float* records, uint32_t* count,
+                                                          uint32_t* error) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = *w.ncontacts;
+    if (i < w.n) {
+        const dyn_body_state& s = w.st[i];
+        float4* o = reinterpret_cast<float4*>(states + 16u * static_cast<size_t>(i));
+        o[0] = make_float4(s.position[0], s.position[1], s.position[2], 0.0f);
+        o[1] = make_float4(s.orientation[0], s.orientation[1], s.orientation[2], s.orientation[3]);
+        o[2] = make_float4(s.velocity[0], s.velocity[1], s.velocity[2], 0.0f);
+        o[3] = make_float4(s.angular_velocity[0], s.angular_velocity[1], s.angular_velocity[2], 0.0f);
+        if (s.sleeping != 0u && i < w.d) atomicOr(error, 2u);
+    }
+    if (i >= *w.ncontacts) return;
+    const dyn_contact& c = w.contacts[i];
+    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
+    const bool fi = body_is_inert(load_body(w, fb)), si = body_is_inert(load_body(w, sb));
+    const bool resolves = c.point_count > 0u && c.sensor == 0u;  // block_resolves
+    if (resolves && (c.relaxation > 0.0f || c.rolling_friction != 0.0f || c.spin_friction != 0.0f)) atomicOr(error, 1u);
+    const uint32_t flags = c.point_count | (fi ? 8u : 0u) | (si ? 16u : 0u) | (resolves ? 32u : 0u);
+    float da, db;
+    friction_partials(w.co[c.a].friction, w.co[c.b].friction, w.p.friction_combine, &da, &db);
+    float4* o = reinterpret_cast<float4*>(records + 76u * static_cast<size_t>(i));
+    o[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(flags), __uint_as_float(c.a));
+    o[1] = make_float4(c.normal[0], c.normal[1], c.normal[2], c.friction);
+    o[2] = make_float4(da, db, c.restitution, __uint_as_float(c.b));
+    for (uint32_t p = 0; p < 4; ++p) {
+        const dyn_manifold_point& m = c.points[p];
+        const bool has = p < c.point_count;
+        const uint32_t src = has ? w.relay_src[4u * i + p] : kNoSlot;
+        float4* q = o + 3u + 4u * p;
+        q[0] = has ? make_float4(m.local_a[0], m.local_a[1], m.local_a[2], m.depth) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        q[1] = has ? make_float4(m.local_b[0], m.local_b[1], m.local_b[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        q[2] = make_float4(has ? m.position[0] : 0.0f, has ? m.position[1] : 0.0f, has ? m.position[2] : 0.0f,
+                           __uint_as_float(src));
+        q[3] = has ? make_float4(m.accumulated_normal, m.accumulated_tangent_1, m.accumulated_tangent_2, 0.0f)
+                   : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -1778,6 +1856,11 @@ struct dyn_world {
     bool graphs = true;
     hipGraphExec_t graph = nullptr;
     float graph_dt = 0.0f;
+    // the tape a recorded step writes (dyn::world_record); null when not recording
+    float* tape_states = nullptr;
+    float* tape_contacts = nullptr;
+    uint32_t* tape_count = nullptr;
+    uint32_t* tape_error = nullptr;
 };
 
 namespace {
@@ -1945,6 +2028,7 @@ int world_step(dyn_world* world, float dt) {
         k_contacts_total<<<1, 64, 0, st>>>(w);
     }
     DYN_LAUNCHED("k_contacts_total");
+    if (world->tape_states) LAUNCH(k_tape_sources, blocks_for(4u * w.contact_cap), w);
     LAUNCH(k_relay, cb, w);
     LAUNCH(k_contact_begin, cb, w, w.p.slop);
     LAUNCH(k_island_init, db, w);
@@ -1961,6 +2045,10 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_rows_count, nb, w);
     DYN_TRY(scan(world, w.list_counts, nullptr, w.n, w.row_start, &w.scan_total[3], true));
     LAUNCH(k_fill_lists, cb, w);
+    if (world->tape_states) {
+        LAUNCH(k_tape_record, blocks_for(std::max(w.n, w.contact_cap)), w, world->tape_states, world->tape_contacts,
+               world->tape_count, world->tape_error);
+    }
     for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
         LAUNCH(k_integrate, db, w);
         LAUNCH(k_solver_bodies, nb, w);
@@ -2260,7 +2348,8 @@ int dyn_world_step(dyn_world* world, float dt) {
     if (world->hull_colliders && world->shape_sources == nullptr) {
         return dyn::fail(DYN_E_INVALID, "dyn_world_step: hull colliders need dyn_world_upload_shapes first");
     }
-    if (!world->graphs || dyn::profiling()) return world_step(world, dt);  // profiled steps run eagerly
+    // profiled and recorded steps run eagerly
+    if (!world->graphs || dyn::profiling() || world->tape_states) return world_step(world, dt);
     if (!world->graph || world->graph_dt != dt) DYN_TRY(capture_step(world, dt));
     DYN_HIP(hipGraphLaunch(world->graph, world->stream));
     return DYN_OK;
@@ -2330,3 +2419,93 @@ int dyn_world_read_pairs(dyn_world* world, uint32_t* major, uint32_t* minor, uin
 }
 
 }  // extern "C"
+
+namespace {
+
+// What a step reads that an earlier step wrote (Retention::Durable streams,
+// crates/dynamis-world/src/snapshot.rs:64-74): the body states, the contact archive (warm start),
+// the resting archive and its sorted index, the static wake marks, the counters and the count
+// words. Everything else is rebuilt by every step.
+struct SnapPart {
+    void* ptr;
+    size_t bytes;
+};
+
+std::vector<SnapPart> snapshot_parts(dyn_world* world) {
+    dyn::W& w = world->w;
+    const size_t cap = w.contact_cap;
+    return {{w.st, sizeof(dyn_body_state) * w.n},
+            {w.archive, sizeof(dyn_contact) * cap},
+            {w.resting, sizeof(dyn_contact) * cap},
+            {w.rest_major, 4u * cap},
+            {w.rest_minor, 4u * cap},
+            {w.rest_slot, 4u * cap},
+            {w.wake, 4u * static_cast<size_t>(w.n)},
+            {w.counters, 4u * DYN_COUNTER_BLOCK_WORDS},
+            {world->words, 4u * 8u}};
+}
+
+size_t snap_align(size_t bytes) { return (bytes + 255u) & ~static_cast<size_t>(255u); }
+
+}  // namespace
+
+extern "C" {
+
+int dyn_world_snapshot_bytes(dyn_world* world, size_t* bytes) {
+    if (!world || !bytes) return dyn::fail(DYN_E_INVALID, "dyn_world_snapshot_bytes: null argument");
+    size_t total = 0;
+    for (const SnapPart& part : snapshot_parts(world)) total += snap_align(part.bytes);
+    *bytes = total;
+    return DYN_OK;
+}
+
+int dyn_world_snapshot(dyn_world* world, void* d_dst) {
+    if (!world || !d_dst) return dyn::fail(DYN_E_INVALID, "dyn_world_snapshot: null argument");
+    char* at = static_cast<char*>(d_dst);
+    for (const SnapPart& part : snapshot_parts(world)) {
+        DYN_HIP(hipMemcpyAsync(at, part.ptr, part.bytes, hipMemcpyDeviceToDevice, world->stream));
+        at += snap_align(part.bytes);
+    }
+    return DYN_OK;
+}
+
+int dyn_world_restore(dyn_world* world, const void* d_src) {
+    if (!world || !d_src) return dyn::fail(DYN_E_INVALID, "dyn_world_restore: null argument");
+    const char* at = static_cast<const char*>(d_src);
+    for (const SnapPart& part : snapshot_parts(world)) {
+        DYN_HIP(hipMemcpyAsync(part.ptr, at, part.bytes, hipMemcpyDeviceToDevice, world->stream));
+        at += snap_align(part.bytes);
+    }
+    return DYN_OK;
+}
+
+}  // extern "C"
+
+namespace dyn {
+
+// The differentiable rollout's access to a world (adjoint.hip).
+hipStream_t world_stream(dyn_world* world) { return world->stream; }
+
+void world_sizes(dyn_world* world, uint32_t* n, uint32_t* d, uint32_t* k, uint32_t* cap) {
+    *n = world->w.n;
+    *d = world->w.d;
+    *k = world->w.k;
+    *cap = world->w.contact_cap;
+}
+
+const dyn_step_params& world_params(dyn_world* world) { return world->w.p; }
+const dyn_body_state* world_states(dyn_world* world) { return world->w.st; }
+const dyn_body_desc* world_descs(dyn_world* world) { return world->w.de; }
+
+// Starts (tape non-null) or ends recording: every following step writes its start states and
+// contacts into the given device buffers (relay_src: 4 words per contact of capacity).
+void world_record(dyn_world* world, float* states, float* contacts, uint32_t* count, uint32_t* error,
+                  uint32_t* relay_src) {
+    world->tape_states = states;
+    world->tape_contacts = contacts;
+    world->tape_count = count;
+    world->tape_error = error;
+    world->w.relay_src = states ? relay_src : nullptr;
+}
+
+}  // namespace dyn

@@ -128,3 +128,40 @@ def batch_worlds(worlds: int = 4096, seed: int = 4, low: int = 8, high: int = 20
         specs["position"][:, 2] = first + gz * spacing + jitter[:, 2]
         scenes.append(encode_specs(np.concatenate([plane, specs])))
     return scenes
+
+
+def adjoint_worlds(worlds: int = 256, boxes: int = 32, seed: int = 5, region: float = 4.0,
+                   speed: float = 2.0, friction=(0.2, 0.8)):
+    """The differentiation workload: per world a static plane and `boxes` boxes (half-extents
+    ~ U[0.2, 0.5] per axis, density 1000) on a jittered 4 x 2 x 4 lattice filling a region^3 box
+    whose floor is 0.5 above the plane, initial velocities ~ N(0, speed) per axis, friction
+    ~ U[friction]. Returns the worlds' scenes (plane first, as spawned), the configuration
+    (8 iterations, sleeping off: the differentiable step has no sleep) and a target point per
+    world for the centre-of-mass loss, uniform in the region."""
+    from .world import BodyDesc, default_config, encode_specs
+    rng = np.random.default_rng(seed)
+    config = default_config()
+    config["solve_iterations"] = 8
+    config["sleep_velocity"] = 0.0
+    config["sleep_angular_velocity"] = 0.0
+    box = BodyDesc.cuboid((0.5, 0.5, 0.5), density=1000.0).spec()[0]
+    plane = BodyDesc.plane().spec()
+    nx, ny, nz = 4, 2, 4
+    cell = np.array([region / nx, region / ny, region / nz])
+    scenes, targets = [], np.zeros((worlds, 3), np.float32)
+    for w in range(worlds):
+        specs = np.zeros(boxes, dtype=abi.BODY_SPEC)
+        specs[:] = box
+        specs["half_extents"] = rng.uniform(0.2, 0.5, (boxes, 3)).astype(np.float32)
+        i = np.arange(boxes) % (nx * ny * nz)
+        gy, rest = np.divmod(i, nx * nz)
+        gx, gz = np.divmod(rest, nz)
+        jitter = rng.uniform(-0.05, 0.05, (boxes, 3))
+        specs["position"][:, 0] = (gx + 0.5) * cell[0] - 0.5 * region + jitter[:, 0]
+        specs["position"][:, 1] = 0.5 + (gy + 0.5) * cell[1] + jitter[:, 1]
+        specs["position"][:, 2] = (gz + 0.5) * cell[2] - 0.5 * region + jitter[:, 2]
+        specs["velocity"] = rng.normal(0.0, speed, (boxes, 3)).astype(np.float32)
+        specs["friction"] = rng.uniform(friction[0], friction[1], boxes).astype(np.float32)
+        scenes.append(encode_specs(np.concatenate([plane, specs]), config))
+        targets[w] = rng.uniform([-0.5 * region, 0.5, -0.5 * region], [0.5 * region, 0.5 + region, 0.5 * region])
+    return scenes, config, targets

@@ -286,6 +286,44 @@ int dyn_world_read_work(dyn_world* world, uint32_t* out);
 int dyn_world_set_graphs(dyn_world* world, int enabled);
 /* Blocks until every enqueued step has finished. */
 int dyn_world_wait(dyn_world* world);
+
+/* Checkpoint / resume on the device (World::snapshot / restore, crates/dynamis-world/src/
+ * snapshot.rs:56-123): the durable state a step reads from earlier steps (body states, contact
+ * and resting archives with the resting index, static wake marks, counters) copied to or from a
+ * caller's device buffer of dyn_world_snapshot_bytes bytes, enqueued on the world's stream.
+ * A world restored from a snapshot steps bit for bit as it did after the snapshot was taken. */
+int dyn_world_snapshot_bytes(dyn_world* world, size_t* bytes);
+int dyn_world_snapshot(dyn_world* world, void* d_dst);
+int dyn_world_restore(dyn_world* world, const void* d_src);
+
+/* ---- the differentiable rollout (no reference counterpart: the reference has no adjoint) ----
+ *
+ * dyn_adjoint_forward steps the world `steps` times with dt (HIP graphs as usual), keeping a
+ * snapshot every `checkpoint_every` steps. dyn_adjoint_backward takes the adjoint of the final
+ * states (device, 16 floats per row: position xyz _, orientation xyzw, velocity xyz _, spin xyz _)
+ * and returns, on the device, the adjoint of the initial states (same layout), of each row's mass
+ * (mass and inertia scaled together at fixed shape; zero for rows that do not simulate) and of each
+ * collider's friction coefficient. It re-steps each segment from its snapshot while recording the
+ * tape (the contacts as the solver received them) and differentiates the step with that contact
+ * set frozen: contact pairs, normals, features, body-local anchors, warm-start relay and split
+ * counts stay at their forward values; poses, velocities, the contact depths along the frozen
+ * normals, the impulses and the mass and friction parameters are differentiated, branches taking
+ * the derivative of the branch the forward selected. Soft contacts, rolling or spin friction and
+ * sleeping bodies are refused (DYN_E_INVALID). world_ranges: per world of a batch 4 words (first
+ * simulated row, simulated rows, first collider, colliders); one world: {0, dynamic, 0, colliders}.
+ * The world is left as the forward rollout ended. */
+typedef struct dyn_adjoint dyn_adjoint;
+int dyn_adjoint_create(dyn_adjoint** out, dyn_world* world, uint32_t steps, uint32_t checkpoint_every,
+                       const uint32_t* world_ranges, uint32_t world_count);
+int dyn_adjoint_forward(dyn_adjoint* adjoint, float dt);
+int dyn_adjoint_backward(dyn_adjoint* adjoint, const float* d_state_grad, float* d_state0_grad,
+                         float* d_mass_grad, float* d_friction_grad);
+/* Host copies of the last recomputed segment's tape (slot < checkpoint_every; with
+ * checkpoint_every >= steps, slot t is step t) and of the rollout's body records. */
+int dyn_adjoint_read_tape(dyn_adjoint* adjoint, uint32_t slot, float* states, float* contacts,
+                          uint32_t capacity, uint32_t* count);
+int dyn_adjoint_read_bodies(dyn_adjoint* adjoint, float* out);
+int dyn_adjoint_destroy(dyn_adjoint* adjoint);
 /* The world's hipStream_t. */
 void* dyn_world_stream(dyn_world* world);
 /* Synchronous reads (they wait for the stream first). */

@@ -32,6 +32,11 @@ int oracle_world_shapes(oracle_world* w, const dyn_shape_source* sources, uint32
 /* One step with the given dt (params.dt / substep_dt are set from it). */
 void oracle_world_step(oracle_world* w, float dt);
 void oracle_world_states(const oracle_world* w, dyn_body_state* out);
+/* The last step's tape for the differentiable-step checks: the states it started from, its
+ * contacts as the solver received them (after the warm-start relay) and, per manifold point,
+ * the archive point its impulses were relayed from (4 * archived contact + point, or ~0). */
+uint32_t oracle_world_tape(const oracle_world* w, dyn_body_state* start, dyn_contact* pre, uint32_t* relay_src,
+                           uint32_t capacity);
 /* Contacts of the last step (after the solve), in canonical pair order; returns the count. */
 uint32_t oracle_world_contacts(const oracle_world* w, dyn_contact* out, uint32_t capacity);
 void oracle_world_counters(const oracle_world* w, uint32_t* out);

@@ -97,6 +97,11 @@ struct oracle_world {
     uint32_t* vdeltas;
     uint32_t* pdeltas;
     uint32_t* contrib;
+    /* the last step's tape: start states, contacts as the solver received them, relay sources */
+    dyn_body_state* start;
+    dyn_contact* pre;
+    uint32_t npre;
+    uint32_t* relay_src;
 };
 
 /* ---- counters --------------------------------------------------------------------------- */
@@ -1758,7 +1763,7 @@ static float carried_distance(const dyn_manifold_point* h, const dyn_manifold_po
     return vlen(vsub(vload(h->local_a), vload(c->local_a))) + vlen(vsub(vload(h->local_b), vload(c->local_b)));
 }
 /* identity.wgsl:85-117 */
-static dyn_contact relay_impulses(const dyn_contact* current, const dyn_contact* held) {
+static dyn_contact relay_impulses(const dyn_contact* current, const dyn_contact* held, uint32_t* picks) {
     dyn_contact r = *current;
     int taken[4] = {0, 0, 0, 0};
     for (uint32_t pi = 0; pi < current->point_count; pi++) {
@@ -1777,6 +1782,7 @@ static dyn_contact relay_impulses(const dyn_contact* current, const dyn_contact*
         }
         if (picked == NO_SLOT) continue;
         taken[picked] = 1;
+        if (picks) picks[pi] = picked;
         r.points[pi].accumulated_normal = held->points[picked].accumulated_normal;
         r.points[pi].accumulated_tangent_1 = held->points[picked].accumulated_tangent_1;
         r.points[pi].accumulated_tangent_2 = held->points[picked].accumulated_tangent_2;
@@ -1818,7 +1824,12 @@ static void contact_relay(oracle_world* w) {
             dyn_contact relayed = *cur;
             relayed.events |= CONTACT_ANNOUNCED;
             /* contact_carries_over == contact_same_pair, which holds here */
-            relayed = contact_carried(relay_impulses(cur, held));
+            uint32_t* picks = w->relay_src + 4u * slot;
+            for (int q = 0; q < 4; q++) picks[q] = NO_SLOT;
+            relayed = contact_carried(relay_impulses(cur, held, picks));
+            for (int q = 0; q < 4; q++) {
+                if (picks[q] != NO_SLOT) picks[q] = 4u * i + picks[q];
+            }
             relayed.events |= CONTACT_ANNOUNCED;
             *cur = relayed;
         }
@@ -1841,7 +1852,7 @@ static void contact_begin(oracle_world* w) {
             dyn_contact revived = *c;
             if (w->resting[slot].events & CONTACT_ANNOUNCED) {
                 revived.events |= CONTACT_ANNOUNCED;
-                revived = contact_carried(relay_impulses(&revived, &w->resting[slot]));
+                revived = contact_carried(relay_impulses(&revived, &w->resting[slot], NULL));
                 revived.events |= CONTACT_ANNOUNCED;
             } else if (contact_touches(c, w->p.slop)) {
                 revived.events |= CONTACT_ANNOUNCED;
@@ -2413,6 +2424,7 @@ void oracle_world_step(oracle_world* w, float dt) {
         memset(w->st[i].force, 0, 12);
         memset(w->st[i].torque, 0, 12);
     }
+    memcpy(w->start, w->st, (size_t)w->n * sizeof *w->st);
     for (uint32_t i = 0; i < w->n; i++) {
         int awake = body_is_active(&w->st[i], &w->de[i]);
         w->activity[i] = awake ? 1u : 0u;
@@ -2442,8 +2454,11 @@ void oracle_world_step(oracle_world* w, float dt) {
     *ctr(w, DYN_COUNTER_CONTACTS) = total_valid;
 
     /* build_islands */
+    for (uint32_t i = 0; i < 4u * w->ncontacts; i++) w->relay_src[i] = NO_SLOT;
     contact_relay(w);
     contact_begin(w);
+    memcpy(w->pre, w->contacts, (size_t)w->ncontacts * sizeof *w->contacts);
+    w->npre = w->ncontacts;
     build_islands(w);
     /* wake, live */
     island_wake(w);
@@ -2572,6 +2587,9 @@ oracle_world* oracle_world_create(const dyn_step_params* params, const dyn_body_
     ALLOC(w->vdeltas, 8u * body_count);
     ALLOC(w->pdeltas, 8u * body_count);
     ALLOC(w->contrib, body_count);
+    ALLOC(w->start, body_count);
+    ALLOC(w->pre, w->contact_cap);
+    ALLOC(w->relay_src, 4u * w->contact_cap);
     return w;
 }
 
@@ -2627,9 +2645,19 @@ void oracle_world_destroy(oracle_world* w) {
                     w->aabbs, w->res, w->flags, w->keys, w->order, w->entries, w->pmaj, w->pmin,
                     w->raw, w->valid, w->contacts, w->matched, w->archive, w->resting,
                     w->parents, w->istate, w->live, w->block_counts, w->blocks, w->rows,
-                    w->targets, w->resolution, w->vdeltas, w->pdeltas, w->contrib};
+                    w->targets, w->resolution, w->vdeltas, w->pdeltas, w->contrib,
+                    w->start, w->pre, w->relay_src};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(w);
+}
+
+uint32_t oracle_world_tape(const oracle_world* w, dyn_body_state* start, dyn_contact* pre, uint32_t* relay_src,
+                           uint32_t capacity) {
+    memcpy(start, w->start, (size_t)w->n * sizeof *start);
+    uint32_t count = w->npre < capacity ? w->npre : capacity;
+    memcpy(pre, w->pre, (size_t)count * sizeof *pre);
+    memcpy(relay_src, w->relay_src, (size_t)count * 16u);
+    return w->npre;
 }
 
 void oracle_world_states(const oracle_world* w, dyn_body_state* out) {
