@@ -17,6 +17,10 @@ Workloads (`--workload`):
   broadphase  the isolated sort + broadphase pass over 1,048,576 boxes.
   copy        calibration only: device-to-device copies of 1 GiB, whose HBM counter readings
               (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) scale the counters of the other kernels.
+  adjoint     the differentiable rollout alone: 256 worlds of 32 boxes (half-extents
+              ~ U[0.2, 0.5]) on a plane, v0 ~ N(0, 2 m/s), friction ~ U[0.2, 0.8], dt = 1/120,
+              8 iterations, 64 steps with a checkpoint every 8; forward and backward times and
+              their ratio (also reported as `adjoint` in the default line).
 One step = one World::step of every body (broadphase: one pass over every box); body-steps/s =
 bodies x timed steps / seconds. Inputs are resident in HBM before the timed region; the timed
 region is bracketed by events on the world's own stream.
@@ -489,10 +493,58 @@ def stage_table(stages, counts, bytes_of, timed=None):
     return table, roofline
 
 
+def run_adjoint(reps: int = 3) -> dict:
+    """Forward (64 steps, graphs) and backward (segment recompute with recording + the per-world
+    reverse) of the differentiation workload; wall-clock seconds around each call on an idle
+    stream, the minimum over `reps` repetitions after one untimed round."""
+    from dynamis_amd import adjoint as adj
+    from dynamis_amd import scenes as scene_lib
+    from dynamis_amd.world import Batch
+    worlds, boxes, steps, every, dt = 256, 32, 64, 8, 1.0 / 120.0
+    sc, cfg, targets = scene_lib.adjoint_worlds(worlds, boxes, seed=5)
+    batch = Batch(sc, cfg)
+    a = adj.Adjoint(batch, steps, every)
+    L = batch.layout
+
+    def grad(states):
+        g = np.zeros((len(states), adj.STATE_FLOATS), np.float32)
+        for w in range(worlds):
+            rows = L.dynamic_base[w] + np.arange(L.dynamic_count[w])
+            m = 1.0 / batch.scene.descs["inverse_mass"][rows].astype(np.float64)
+            p = states["position"][rows].astype(np.float64)
+            r = (m[:, None] * p).sum(0) / m.sum() - targets[w]
+            g[rows, 0:3] = 2 * r[None, :] * (m / m.sum())[:, None]
+        return g
+
+    fwd, bwd = [], []
+    for rep in range(reps + 1):
+        batch.wait()
+        t0 = time.perf_counter()
+        a.forward(dt)
+        batch.wait()
+        t1 = time.perf_counter()
+        g = grad(batch.states())
+        batch.wait()
+        t2 = time.perf_counter()
+        a.backward(g)
+        t3 = time.perf_counter()
+        if rep:
+            fwd.append(t1 - t0)
+            bwd.append(t3 - t2)
+    a.close()
+    f, b = min(fwd), min(bwd)
+    return {"workload": f"{worlds} worlds x {boxes} boxes on a plane, dt = 1/120, 8 iterations, {steps} steps, "
+                        f"checkpoint every {every}; loss = squared distance of each world's centre of mass "
+                        f"to a target",
+            "bodies": worlds * boxes, "forward_ms": f * 1e3, "backward_ms": b * 1e3, "bwd_fwd": b / f,
+            "forward_body_steps_per_s": worlds * boxes * steps / f}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "copy"], default="pile")
+    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "copy", "adjoint"],
+                        default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--profile-steps", type=int, default=5)
@@ -511,6 +563,13 @@ def main() -> int:
     wl = args.workload
     if wl == "copy":
         return run_copy(max(args.steps if args.steps is not None else 5, 1))
+    if wl == "adjoint":
+        r = run_adjoint()
+        print(json.dumps({"metric": "adjoint backward / forward time", "value": r["bwd_fwd"], "unit": "ratio",
+                          "n_gpus": 1, "steps": 64, "warmup": 0, "ms_per_step": r["forward_ms"] / 64,
+                          "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                          "data": "synthetic", "config": r}))
+        return 0
     if wl == "broadphase":
         steps = max(args.steps if args.steps is not None else 50, 1)
         warmup = max(args.warmup if args.warmup is not None else 10, 0)
@@ -600,6 +659,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "stages": table,
     }
+    if wl == "pile":
+        line["adjoint"] = run_adjoint()
     print(json.dumps(line))
     return 0
 
