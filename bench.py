@@ -503,6 +503,7 @@ def run_adjoint(reps: int = 3) -> dict:
     worlds, boxes, steps, every, dt = 256, 32, 64, 8, 1.0 / 120.0
     sc, cfg, targets = scene_lib.adjoint_worlds(worlds, boxes, seed=5)
     batch = Batch(sc, cfg)
+    batch.set_graphs(GRAPHS)
     a = adj.Adjoint(batch, steps, every)
     L = batch.layout
 
