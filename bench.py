@@ -258,6 +258,22 @@ def run_sleep_off(scene, steps: int, warmup: int) -> float:
     return ms
 
 
+TUNE = True
+TUNING = None
+
+
+def tune_world(world) -> None:
+    """The sorts' tilings from the autotuner (timed on this GPU, or loaded from its cache),
+    before any timed step."""
+    global TUNING
+    if not TUNE:
+        return
+    from dynamis_amd import tune
+    info = tune.autotune_world(world)
+    TUNING = {"sort_tilings": {str(c): {"keys_per_lane": t[0], "groups": t[1]} for c, t in info["tilings"].items()},
+              "from_cache": info["from_cache"]}
+
+
 def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     import dynamis_amd
     from dynamis_amd._native import call
@@ -270,6 +286,7 @@ def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     world = World()
     world.create(scene)
     world.set_graphs(GRAPHS)
+    tune_world(world)
     ms, stages, counts, measured, timed = timed_run(world, steps, warmup, profile_steps)
     final_states = world.states()
     world.close()
@@ -310,6 +327,7 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
     scenes = batch_scenes()
     batch = Batch(scenes)
     batch.set_graphs(GRAPHS)
+    tune_world(batch)
     ms, stages, counts, m, timed = timed_run(batch, steps, warmup, profile_steps)
     bodies = len(batch.scene.states)
     final = batch.states()
@@ -552,11 +570,14 @@ def main() -> int:
     parser.add_argument("--no-graphs", action="store_true",
                         help="launch every step eagerly (for external profilers, which trace the "
                              "replayed graph of a captured step badly)")
+    parser.add_argument("--no-tune", action="store_true",
+                        help="keep the sorts' default tilings instead of autotuning them first")
     parser.add_argument("--cpu-sample", type=int, default=None,
                         help="oracle steps (broadphase: passes) timed for the CPU baseline; 0 skips it")
     args = parser.parse_args()
-    global GRAPHS
+    global GRAPHS, TUNE
     GRAPHS = not args.no_graphs
+    TUNE = not args.no_tune
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent worlds would shard by world range)", file=sys.stderr)
@@ -660,6 +681,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "stages": table,
     }
+    if isinstance(config, dict) and TUNING is not None:
+        config["autotune"] = TUNING
     if wl == "pile":
         line["adjoint"] = run_adjoint()
     print(json.dumps(line))

@@ -66,6 +66,11 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
     "dyn_world_set_graphs": (c_int, [c_void_p, c_int]),
     "dyn_world_read_work": (c_int, [c_void_p, ctypes.POINTER(c_u32)]),
+    "dyn_sort_autotune": (c_int, [c_void_p, c_u32, c_u32, c_u32_p, c_u32_p]),
+    "dyn_sort_tuning_export": (c_int, [c_void_p, c_u32, c_u32_p]),
+    "dyn_sort_tuning_import": (c_int, [c_void_p, c_u32]),
+    "dyn_world_autotune": (c_int, [c_void_p, c_void_p]),
+    "dyn_device_name": (c_int, [ctypes.c_char_p, c_u32]),
     "dyn_world_snapshot_bytes": (c_int, [c_void_p, ctypes.POINTER(c_size_t)]),
     "dyn_world_snapshot": (c_int, [c_void_p, c_void_p]),
     "dyn_world_restore": (c_int, [c_void_p, c_void_p]),

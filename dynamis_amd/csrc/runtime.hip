@@ -101,6 +101,19 @@ int dyn_set_device(int ordinal) {
     return DYN_OK;
 }
 
+int dyn_device_name(char* out, uint32_t capacity) {
+    if (!out || capacity == 0) return dyn::fail(DYN_E_INVALID, "dyn_device_name: null output");
+    int device = 0;
+    DYN_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    DYN_HIP(hipGetDeviceProperties(&prop, device));
+    std::string name = std::string(prop.name) + "|" + prop.gcnArchName;
+    const size_t n = name.size() < capacity - 1 ? name.size() : capacity - 1;
+    std::memcpy(out, name.data(), n);
+    out[n] = '\0';
+    return DYN_OK;
+}
+
 int dyn_malloc(void** d_ptr, size_t bytes) {
     if (!d_ptr) return dyn::fail(DYN_E_INVALID, "dyn_malloc: null output");
     *d_ptr = nullptr;

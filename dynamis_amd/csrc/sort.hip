@@ -12,6 +12,11 @@
 // produced is the same stable order.
 #include "sort.h"
 
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
 namespace dyn {
 
 namespace {
@@ -310,11 +315,12 @@ int sort_histogram(hipStream_t stream, const uint32_t* major, const uint32_t* mi
 template <int ITEMS>
 static int launch_passes(hipStream_t stream, uint32_t* major, uint32_t* minor, uint32_t* payload,
                          const SortScratch& s, const SortDigit* digits, uint32_t passes,
-                         uint32_t capacity, const SortNames& names) {
+                         uint32_t capacity, const SortNames& names, uint32_t group_cap) {
     constexpr int TILE = kSortThreads * ITEMS;
     const uint32_t tiles = ceil_div(capacity ? capacity : 1u, TILE);
-    // Persistent workgroups take tiles in order; about two per CU keep every CU busy.
-    const uint32_t groups = tiles < 1024u ? tiles : 1024u;
+    // Persistent workgroups take tiles in order (any count makes progress: a tile's look-back
+    // only waits on earlier tiles, which running groups hold).
+    const uint32_t groups = tiles < group_cap ? tiles : group_cap;
     // The look-back words are laid out for the narrowest tiling, so every tiling fits.
     const size_t flag_stride = static_cast<size_t>(s.max_tiles) * kSortBins;
     if (static_cast<size_t>(tiles) * kSortBins > flag_stride) {
@@ -365,15 +371,75 @@ static int launch_passes(hipStream_t stream, uint32_t* major, uint32_t* minor, u
     return DYN_OK;
 }
 
+// ---- tiling: the autotuner's table ---------------------------------------------------------------
+//
+// A pass's tiling is keys per lane (8: 2048-key tiles, 16: 4096-key tiles) and the cap on its
+// persistent workgroups. The tuner times every candidate on the device for a capacity bucket
+// (ceil(log2(capacity))) and keeps the fastest per (device, bucket); untuned buckets use the
+// default: narrow tiles up to 2^21 keys (more of the chip busy on shorter streams), 1024 groups
+// (about four per CU).
+
+struct SortTiling {
+    uint32_t items, groups;
+};
+
+constexpr SortTiling kTilings[] = {{8u, 512u}, {8u, 1024u}, {8u, 2048u}, {16u, 512u}, {16u, 1024u}, {16u, 2048u}};
+constexpr uint32_t kTilingCount = sizeof(kTilings) / sizeof(kTilings[0]);
+
+std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::map<std::pair<int, uint32_t>, SortTiling>& tune_table() {
+    static std::map<std::pair<int, uint32_t>, SortTiling> t;
+    return t;
+}
+
+uint32_t tune_bucket(uint32_t capacity) {
+    return capacity <= 1u ? 0u : 32u - static_cast<uint32_t>(__builtin_clz(capacity - 1u));
+}
+
+SortTiling default_tiling(uint32_t capacity) {
+    return SortTiling{capacity <= (1u << 21) ? 8u : 16u, 1024u};
+}
+
+SortTiling tiling_for(uint32_t capacity) {
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return default_tiling(capacity);
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    const auto it = tune_table().find({device, tune_bucket(capacity)});
+    return it == tune_table().end() ? default_tiling(capacity) : it->second;
+}
+
+int passes_with(hipStream_t stream, uint32_t* major, uint32_t* minor, uint32_t* payload, const SortScratch& s,
+                const SortDigit* digits, uint32_t passes, uint32_t capacity, const SortNames& names,
+                SortTiling t) {
+    if (t.items == 8u) return launch_passes<8>(stream, major, minor, payload, s, digits, passes, capacity, names, t.groups);
+    return launch_passes<16>(stream, major, minor, payload, s, digits, passes, capacity, names, t.groups);
+}
+
 int sort_passes(hipStream_t stream, uint32_t* major, uint32_t* minor, uint32_t* payload,
                 const SortScratch& s, const SortDigit* digits, uint32_t passes,
                 uint32_t capacity, const SortNames& names) {
-    // Narrow tiles keep more of the chip busy on shorter streams.
-    if (capacity <= (1u << 21)) {
-        return launch_passes<8>(stream, major, minor, payload, s, digits, passes, capacity, names);
-    }
-    return launch_passes<16>(stream, major, minor, payload, s, digits, passes, capacity, names);
+    return passes_with(stream, major, minor, payload, s, digits, passes, capacity, names, tiling_for(capacity));
 }
+
+namespace {
+
+__global__ void __launch_bounds__(256) sort_tune_fill(uint32_t* major, uint32_t* minor, uint32_t* payload,
+                                                      uint32_t count, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t h = (i + seed) * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    major[i] = h;
+    minor[i] = h * 0xC2B2AE3Du;
+    payload[i] = i;
+}
+
+}  // namespace
 
 }  // namespace dyn
 
@@ -397,6 +463,106 @@ uint32_t dyn_sort_key_words(uint32_t elements) {
 
 size_t dyn_sort_scratch_bytes(uint32_t capacity, uint32_t digit_passes) {
     return dyn::sort_scratch_layout(capacity, digit_passes, true, nullptr, nullptr);
+}
+
+int dyn_sort_autotune(void* stream, uint32_t capacity, uint32_t digit_passes, uint32_t* items_out,
+                      uint32_t* groups_out) {
+    using namespace dyn;
+    if (capacity == 0u || capacity > kSortMaxCapacity) return fail(DYN_E_INVALID, "dyn_sort_autotune: capacity out of range");
+    if (digit_passes == 0u || digit_passes > kSortMaxPasses) {
+        return fail(DYN_E_INVALID, "dyn_sort_autotune: digit_passes must be 1..8");
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t scratch_bytes = sort_scratch_layout(capacity, digit_passes, true, nullptr, nullptr);
+    std::vector<void*> owned;
+    auto cleanup = [&](int rc) {
+        (void)hipStreamSynchronize(st);
+        for (void* p : owned) (void)hipFree(p);
+        return rc;
+    };
+    uint32_t* lanes[3] = {nullptr, nullptr, nullptr};
+    void* scratch = nullptr;
+    for (auto*& lane : lanes) {
+        if (hipMalloc(&lane, 4u * static_cast<size_t>(capacity)) != hipSuccess) return cleanup(fail(DYN_E_HIP, "dyn_sort_autotune: lanes"));
+        owned.push_back(lane);
+    }
+    if (hipMalloc(&scratch, scratch_bytes) != hipSuccess) return cleanup(fail(DYN_E_HIP, "dyn_sort_autotune: scratch"));
+    owned.push_back(scratch);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        return cleanup(fail(DYN_E_HIP, "dyn_sort_autotune: events"));
+    }
+    SortScratch s;
+    sort_scratch_layout(capacity, digit_passes, true, scratch, &s);
+    SortDigit digits[kSortMaxPasses];
+    sort_digits(digit_passes > 4u ? 4u : digit_passes, digit_passes > 4u ? digit_passes - 4u : 0u, digits);
+    const SortNames& names = kGenericSortNames;
+    float best_ms = 0.0f;
+    uint32_t best = kTilingCount;
+    int rc = DYN_OK;
+    for (uint32_t round = 0; round < 4 && rc == DYN_OK; ++round) {  // round 0 warms every variant
+        for (uint32_t c = 0; c < kTilingCount && rc == DYN_OK; ++c) {
+            sort_tune_fill<<<ceil_div(capacity, 256u), 256, 0, st>>>(lanes[0], lanes[1], lanes[2], capacity, round);
+            rc = sort_prepare(st, s, nullptr, nullptr, capacity, names);
+            if (rc == DYN_OK) rc = sort_histogram(st, lanes[0], lanes[1], s, digits, digit_passes, capacity, names);
+            if (rc != DYN_OK || hipEventRecord(e0, st) != hipSuccess) break;
+            rc = passes_with(st, lanes[0], lanes[1], lanes[2], s, digits, digit_passes, capacity, names, kTilings[c]);
+            if (rc != DYN_OK || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (round > 0 && (best == kTilingCount || ms < best_ms)) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != DYN_OK) return cleanup(rc);
+    if (best == kTilingCount) return cleanup(fail(DYN_E_HIP, "dyn_sort_autotune: timing failed"));
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return cleanup(fail(DYN_E_HIP, "dyn_sort_autotune: device"));
+    {
+        std::lock_guard<std::mutex> lock(tune_mutex());
+        tune_table()[{device, tune_bucket(capacity)}] = kTilings[best];
+    }
+    if (items_out) *items_out = kTilings[best].items;
+    if (groups_out) *groups_out = kTilings[best].groups;
+    return cleanup(DYN_OK);
+}
+
+int dyn_sort_tuning_export(uint32_t* out, uint32_t capacity, uint32_t* count) {
+    using namespace dyn;
+    if (!count) return fail(DYN_E_INVALID, "dyn_sort_tuning_export: null count");
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    *count = static_cast<uint32_t>(tune_table().size());
+    uint32_t i = 0;
+    for (const auto& kv : tune_table()) {
+        if (i >= capacity || !out) break;
+        out[4 * i + 0] = static_cast<uint32_t>(kv.first.first);
+        out[4 * i + 1] = kv.first.second;
+        out[4 * i + 2] = kv.second.items;
+        out[4 * i + 3] = kv.second.groups;
+        ++i;
+    }
+    return DYN_OK;
+}
+
+int dyn_sort_tuning_import(const uint32_t* entries, uint32_t count) {
+    using namespace dyn;
+    if (count && !entries) return fail(DYN_E_INVALID, "dyn_sort_tuning_import: null entries");
+    for (uint32_t i = 0; i < count; ++i) {
+        const SortTiling t{entries[4 * i + 2], entries[4 * i + 3]};
+        bool known = false;
+        for (const SortTiling& k : kTilings) known |= k.items == t.items && k.groups == t.groups;
+        if (!known || entries[4 * i + 1] > 31u) return fail(DYN_E_INVALID, "dyn_sort_tuning_import: not a tiling of this build");
+    }
+    std::lock_guard<std::mutex> lock(tune_mutex());
+    tune_table().clear();
+    for (uint32_t i = 0; i < count; ++i) {
+        tune_table()[{static_cast<int>(entries[4 * i]), entries[4 * i + 1]}] = SortTiling{entries[4 * i + 2], entries[4 * i + 3]};
+    }
+    return DYN_OK;
 }
 
 int dyn_sort_pairs(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t* d_payload,

@@ -2451,6 +2451,28 @@ size_t snap_align(size_t bytes) { return (bytes + 255u) & ~static_cast<size_t>(2
 
 extern "C" {
 
+int dyn_world_autotune(dyn_world* world, uint32_t* tilings) {
+    using namespace dyn;
+    if (!world) return fail(DYN_E_INVALID, "dyn_world_autotune: null world");
+    const W& w = world->w;
+    // the step's sorts: entry regions (4 key digits), candidate pairs and GJK worklists (pair
+    // capacity), the resting index (contact capacity)
+    const uint32_t caps[3] = {world->entry_cap, w.pair_cap, w.contact_cap};
+    const uint32_t passes[3] = {4u, 2u * dyn_sort_key_words(w.k ? w.k : 1u), 2u * dyn_sort_key_words(w.id_cap ? w.id_cap : 1u)};
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    for (int i = 0; i < 3; ++i) {
+        uint32_t items = 0, groups = 0;
+        DYN_TRY(dyn_sort_autotune(world->stream, caps[i] ? caps[i] : 1u, passes[i] < 8u ? passes[i] : 8u, &items, &groups));
+        if (tilings) {
+            tilings[3 * i] = caps[i];
+            tilings[3 * i + 1] = items;
+            tilings[3 * i + 2] = groups;
+        }
+    }
+    drop_graph(world);  // the captured step holds the old launch shapes
+    return DYN_OK;
+}
+
 int dyn_world_snapshot_bytes(dyn_world* world, size_t* bytes) {
     if (!world || !bytes) return dyn::fail(DYN_E_INVALID, "dyn_world_snapshot_bytes: null argument");
     size_t total = 0;

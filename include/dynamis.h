@@ -287,6 +287,24 @@ int dyn_world_set_graphs(dyn_world* world, int enabled);
 /* Blocks until every enqueued step has finished. */
 int dyn_world_wait(dyn_world* world);
 
+/* ---- autotuning (no reference counterpart: the reference's sort has one fixed tiling) ----
+ * A sort pass's tiling (keys per lane 8 or 16, cap on persistent workgroups 512/1024/2048) is
+ * chosen per (device, ceil(log2(capacity))) from a process-wide table; untuned buckets use the
+ * default. dyn_sort_autotune times every tiling on the device for `capacity` keys and
+ * `digit_passes` digits (synthetic keys, minimum of three rounds after a warm round) and keeps the
+ * fastest; dyn_world_autotune does so for the capacities of a world's sorts (entries, pairs and
+ * worklists, the resting index) and drops its captured step; tilings (optional, 9 words) receives
+ * capacity, keys per lane, groups for each. The table is exported and imported as 4 words per
+ * entry (device, bucket, keys per lane, groups) so a caller can keep it per GPU. Every tiling
+ * sorts to the same stable order. */
+int dyn_sort_autotune(void* stream, uint32_t capacity, uint32_t digit_passes, uint32_t* items_out,
+                      uint32_t* groups_out);
+int dyn_sort_tuning_export(uint32_t* out, uint32_t capacity, uint32_t* count);
+int dyn_sort_tuning_import(const uint32_t* entries, uint32_t count);
+int dyn_world_autotune(dyn_world* world, uint32_t* tilings);
+/* The current device's marketing name and gfx target, "name|gfx950", into out (NUL-terminated). */
+int dyn_device_name(char* out, uint32_t capacity);
+
 /* Checkpoint / resume on the device (World::snapshot / restore, crates/dynamis-world/src/
  * snapshot.rs:56-123): the durable state a step reads from earlier steps (body states, contact
  * and resting archives with the resting index, static wake marks, counters) copied to or from a
