@@ -64,21 +64,19 @@ __global__ void __launch_bounds__(kAdjThreads) k_adj_zero(float* p, size_t count
     if (i < count) p[i] = 0.0f;
 }
 
-// One step of every world, backwards: its replay from the recorded start, then the reverse, then
-// the impulses' adjoint sent to the previous step's contacts (accb_prev, zero beforehand).
-__global__ void __launch_bounds__(kAdjThreads) k_adj_step(AdjParams P, const AdjRange* ranges, const float* bodies,
-                                                          const float* contacts, const uint32_t* count,
-                                                          const float* states, Work<float> work,
-                                                          const float* accb_end, float* accb_prev) {
-    const AdjRange wr = ranges[blockIdx.x];
-    Work<float> W = work;
+// One world's step backwards over the batch-wide arrays (global row and contact indices): its
+// replay from the recorded start, then the reverse, then the impulses' adjoint sent to the
+// previous step's contacts (accb_prev, zero beforehand).
+__device__ void step_world_global(const AdjParams& P, const AdjRange& wr, const float* bodies, const float* contacts,
+                                  uint32_t c0, uint32_t c1, const float* states, Work<float> W,
+                                  const float* accb_end, float* accb_prev) {
     const DevExec ex;
     StepIn in;
     in.bodies = bodies;
     in.contacts = contacts;
     in.fwd = states;
-    in.c0 = lower_collider(contacts, *count, wr.collider_base);
-    in.c1 = lower_collider(contacts, *count, wr.collider_base + wr.collider_count);
+    in.c0 = c0;
+    in.c1 = c1;
     ex.each(wr.row_count, [&](uint32_t i) {
         const size_t row = wr.row_base + i;
         for (uint32_t j = 0; j < kStateFloats; ++j) W.x[kStateFloats * row + j] = states[kStateFloats * row + j];
@@ -88,6 +86,160 @@ __global__ void __launch_bounds__(kAdjThreads) k_adj_step(AdjParams P, const Adj
                 static_cast<const float*>(nullptr));
     reverse_step(ex, P, wr, in, W, accb_end, static_cast<float*>(nullptr));
     if (accb_prev) relay_back(ex, in, W.accb, accb_prev);
+}
+
+// ---- the world-local step ------------------------------------------------------------------------
+//
+// A world's rows, contacts and colliders renumbered from zero (its simulated rows, then the static
+// rows its contacts touch), so that the step's per-row and per-contact arrays are dense and small:
+// the ones every phase reads (states, velocity and pose stores, resolutions, block words, depths,
+// lists, the state adjoint, and the contact records when they fit) sit in LDS, the per-sweep
+// impulse and row stores in a per-world stretch of the batch-wide arrays. Phases then wait on LDS
+// instead of on L2 round trips through arrays strided by the whole batch.
+
+constexpr uint32_t kMaxStaticRows = 8;
+
+__host__ __device__ inline size_t lds_align(size_t b) { return (b + 15u) & ~static_cast<size_t>(15u); }
+
+// LDS bytes of a world with n local rows and c contacts (contacts in LDS or not)
+__host__ __device__ inline size_t local_bytes(const AdjParams& P, uint32_t n, uint32_t c, bool contacts_in_lds) {
+    const size_t S = P.substeps, I = P.iterations, Q = P.position_iterations;
+    size_t b = 0;
+    if (contacts_in_lds) b += lds_align(4u * kContactFloats * static_cast<size_t>(c));
+    b += lds_align(4u * kBodyFloats * n) + lds_align(4u * kStateFloats * n);  // bodies, recorded start
+    b += lds_align(4u * kStateFloats * n);                                     // x
+    b += lds_align(4u * S * (3u + I) * n * 8u) + lds_align(4u * S * 2u * n * 8u) + lds_align(4u * S * Q * n * 4u);
+    b += 2u * lds_align(4u * 4u * static_cast<size_t>(c));                    // depth, target
+    b += lds_align(4u * 2u * kSlotFloats * static_cast<size_t>(c));           // slot
+    b += 2u * lds_align(4u * n) + lds_align(4u * 2u * static_cast<size_t>(c)); // list start, count, list
+    b += lds_align(4u * kStateFloats * n) + 2u * lds_align(4u * 4u * n);       // xb, resb, alb
+    b += lds_align(4u * n) + lds_align(4u * static_cast<size_t>(c));           // sbar, mucb
+    return b;
+}
+
+__global__ void __launch_bounds__(kAdjThreads) k_adj_step(AdjParams P, const AdjRange* ranges, const float* bodies,
+                                                          const float* contacts, const uint32_t* count,
+                                                          const float* states, Work<float> work, float* local_contacts,
+                                                          const float* accb_end, float* accb_prev, uint32_t lds_bytes) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ uint32_t s_static[kMaxStaticRows];
+    __shared__ uint32_t s_nstatic, s_mode;
+    const AdjRange wr = ranges[blockIdx.x];
+    const uint32_t total = *count;
+    const uint32_t c0 = lower_collider(contacts, total, wr.collider_base);
+    const uint32_t c1 = lower_collider(contacts, total, wr.collider_base + wr.collider_count);
+    const uint32_t C = c1 - c0, D = wr.row_count, r0 = wr.row_base, r1 = wr.row_base + wr.row_count;
+    if (threadIdx.x == 0) {
+        uint32_t ns = 0;
+        bool ok = true;
+        for (uint32_t ci = c0; ci < c1 && ok; ++ci) {
+            const float* c = contacts + static_cast<size_t>(kContactFloats) * ci;
+            for (uint32_t side = 0; side < 2; ++side) {
+                const uint32_t row = contact_u(c, side);
+                if (row >= r0 && row < r1) continue;
+                bool seen = false;
+                for (uint32_t j = 0; j < ns; ++j) seen |= s_static[j] == row;
+                if (seen) continue;
+                if (ns == kMaxStaticRows) {
+                    ok = false;
+                    break;
+                }
+                s_static[ns++] = row;
+            }
+        }
+        s_nstatic = ns;
+        // 0: batch-wide arrays; 1: local, contacts in LDS; 2: local, contacts in the global stretch
+        uint32_t mode = 0;
+        if (ok && local_bytes(P, D + ns, C, true) <= lds_bytes) mode = 1;
+        else if (ok && local_bytes(P, D + ns, C, false) <= lds_bytes) mode = 2;
+        s_mode = mode;
+    }
+    __syncthreads();
+    if (s_mode == 0) {
+        step_world_global(P, wr, bodies, contacts, c0, c1, states, work, accb_end, accb_prev);
+        return;
+    }
+    const DevExec ex;
+    const uint32_t NS = s_nstatic, n = D + NS;
+    const size_t S = P.substeps, I = P.iterations, Q = P.position_iterations;
+    char* at = lds;
+    auto take = [&](size_t bytes) -> float* {
+        float* p = reinterpret_cast<float*>(at);
+        at += lds_align(bytes);
+        return p;
+    };
+    float* lc = s_mode == 1 ? take(4u * kContactFloats * static_cast<size_t>(C))
+                            : local_contacts + static_cast<size_t>(kContactFloats) * c0;
+    float* lb = take(4u * kBodyFloats * n);
+    float* lf = take(4u * kStateFloats * n);
+    Work<float> L;
+    L.n = n;
+    L.cap = C;
+    L.x = take(4u * kStateFloats * n);
+    L.vel = take(4u * S * (3u + I) * n * 8u);
+    L.pose = take(4u * S * 2u * n * 8u);
+    L.res = take(4u * S * Q * n * 4u);
+    L.depth = take(4u * 4u * C);
+    L.target = take(4u * 4u * C);
+    L.slot = take(4u * 2u * kSlotFloats * C);
+    L.list_start = reinterpret_cast<uint32_t*>(take(4u * n));
+    L.list_count = reinterpret_cast<uint32_t*>(take(4u * n));
+    L.list = reinterpret_cast<uint32_t*>(take(4u * 2u * C));
+    L.xb = take(4u * kStateFloats * n);
+    L.resb = take(4u * 4u * n);
+    L.alb = take(4u * 4u * n);
+    L.sbar = take(4u * n);
+    L.mucb = take(4u * C);
+    L.acc = work.acc + S * (I + 1u) * kAccFloats * static_cast<size_t>(c0);
+    L.rows = work.rows + S * kRowFloats * static_cast<size_t>(c0);
+    L.pointb = work.pointb + 4u * kPointAdj * static_cast<size_t>(c0);
+    L.sideb = work.sideb + 2u * kSideFloats * static_cast<size_t>(c0);
+    L.accb = work.accb + kAccFloats * static_cast<size_t>(c0);
+    L.mubar = work.mubar + wr.collider_base;
+    auto global_row = [&](uint32_t local) { return local < D ? r0 + local : s_static[local - D]; };
+    auto local_row = [&](uint32_t row) {
+        if (row >= r0 && row < r1) return row - r0;
+        uint32_t j = 0;
+        while (j + 1u < NS && s_static[j] != row) ++j;
+        return D + j;
+    };
+    ex.each(C, [&](uint32_t k) {
+        const float* src = contacts + static_cast<size_t>(kContactFloats) * (c0 + k);
+        float* dst = lc + static_cast<size_t>(kContactFloats) * k;
+        for (uint32_t j = 0; j < kContactFloats; ++j) dst[j] = src[j];
+        dst[0] = bitsf(local_row(contact_u(src, 0)));
+        dst[1] = bitsf(local_row(contact_u(src, 1)));
+        dst[3] = bitsf(contact_u(src, 3) - wr.collider_base);
+        dst[11] = bitsf(contact_u(src, 11) - wr.collider_base);  // relay sources stay batch-wide
+    });
+    ex.each(n, [&](uint32_t i) {
+        const size_t g = global_row(i);
+        for (uint32_t j = 0; j < kBodyFloats; ++j) lb[kBodyFloats * i + j] = bodies[kBodyFloats * g + j];
+        for (uint32_t j = 0; j < kStateFloats; ++j) {
+            lf[kStateFloats * i + j] = states[kStateFloats * g + j];
+            L.x[kStateFloats * i + j] = states[kStateFloats * g + j];
+            L.xb[kStateFloats * i + j] = i < D ? work.xb[kStateFloats * g + j] : 0.0f;
+        }
+        L.sbar[i] = 0.0f;
+    });
+    ex.sync();
+    const AdjRange lr = {0u, D, 0u, wr.collider_count};
+    StepIn in;
+    in.bodies = lb;
+    in.contacts = lc;
+    in.fwd = lf;
+    in.c0 = 0;
+    in.c1 = C;
+    replay_step(ex, P, lr, in, L, static_cast<const float*>(nullptr), static_cast<const float*>(nullptr),
+                static_cast<const float*>(nullptr));
+    reverse_step(ex, P, lr, in, L, accb_end ? accb_end + kAccFloats * static_cast<size_t>(c0) : nullptr,
+                 static_cast<float*>(nullptr));
+    if (accb_prev) relay_back(ex, in, L.accb, accb_prev);
+    ex.each(D, [&](uint32_t i) {
+        const size_t g = r0 + i;
+        for (uint32_t j = 0; j < kStateFloats; ++j) work.xb[kStateFloats * g + j] = L.xb[kStateFloats * i + j];
+        work.sbar[g] += L.sbar[i];
+    });
 }
 
 // d loss / d mass = (d loss / d scale) / mass at fixed shape
@@ -121,6 +273,8 @@ struct dyn_adjoint {
     uint32_t* d_error = nullptr;
     uint32_t* d_relay_src = nullptr;
     float* d_accb[2] = {nullptr, nullptr};
+    float* d_local_contacts = nullptr;  // cap x 76: renumbered contact records of worlds whose records stay out of LDS
+    uint32_t lds_bytes = 0;             // dynamic LDS of k_adj_step
     Work<float> W{};
     std::vector<void*> owned;
 };
@@ -197,6 +351,7 @@ int dyn_adjoint_create(dyn_adjoint** out, dyn_world* world, uint32_t steps, uint
     ADJ_A(a->d_relay_src, 4u * cap);
     ADJ_A(a->d_accb[0], cap * kAccFloats);
     ADJ_A(a->d_accb[1], cap * kAccFloats);
+    ADJ_A(a->d_local_contacts, cap * kContactFloats);
     Work<float>& W = a->W;
     W.n = a->n;
     W.cap = a->cap;
@@ -224,6 +379,19 @@ int dyn_adjoint_create(dyn_adjoint** out, dyn_world* world, uint32_t steps, uint
 #undef ADJ_A
     if (rc == DYN_OK && hipMemcpy(a->d_ranges, ranges.data(), sizeof(AdjRange) * world_count, hipMemcpyHostToDevice) != hipSuccess) {
         rc = fail(DYN_E_HIP, "dyn_adjoint_create: world ranges");
+    }
+    if (rc == DYN_OK) {
+        int device = 0, lds = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+            rc = fail(DYN_E_HIP, "dyn_adjoint_create: LDS size");
+        } else {
+            // the kernel's static LDS (static rows, mode) is a few words
+            a->lds_bytes = lds > 1024 ? static_cast<uint32_t>(lds - 1024) : 0u;
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_adj_step),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(a->lds_bytes));
+            (void)hipGetLastError();
+        }
     }
     if (rc != DYN_OK) {
         destroy(a);
@@ -301,10 +469,10 @@ int dyn_adjoint_backward(dyn_adjoint* a, const float* d_state_grad, float* d_sta
             const uint32_t t = first + j;
             float* prev = t > 0 ? a->d_accb[(t + 1u) & 1u] : nullptr;
             if (prev) DYN_TRY(zero(a, prev, cap * kAccFloats));
-            k_adj_step<<<a->worlds, kAdjThreads, 0, st>>>(a->P, a->d_ranges, a->d_bodies,
-                                                          a->d_contacts + static_cast<size_t>(j) * cap * kContactFloats,
-                                                          a->d_counts + j, a->d_states + static_cast<size_t>(j) * n * kStateFloats,
-                                                          W, a->d_accb[t & 1u], prev);
+            k_adj_step<<<a->worlds, kAdjThreads, a->lds_bytes, st>>>(
+                a->P, a->d_ranges, a->d_bodies, a->d_contacts + static_cast<size_t>(j) * cap * kContactFloats,
+                a->d_counts + j, a->d_states + static_cast<size_t>(j) * n * kStateFloats, W, a->d_local_contacts,
+                a->d_accb[t & 1u], prev, a->lds_bytes);
             DYN_LAUNCHED("k_adj_step");
         }
     }
