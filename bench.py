@@ -15,6 +15,9 @@ Workloads (`--workload`):
               8 solver iterations, 240 steps. Also times the same worlds stepped one at a time on
               one stream (a sample of every 16th world, scaled to all 4,096).
   broadphase  the isolated sort + broadphase pass over 1,048,576 boxes.
+  fixed_grid  the same boxes through the literal fixed-cell mode: a uniform 4 m grid, keys
+              (cell hash, box index), LSD radix sort, cell runs, canonical overlapping pairs
+              (dyn_fixed_grid; the same pair set as the hierarchical grid).
   copy        calibration only: device-to-device copies of 1 GiB, whose HBM counter readings
               (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) scale the counters of the other kernels.
   adjoint     the differentiable rollout alone: 256 worlds of 32 boxes (half-extents
@@ -418,6 +421,57 @@ def run_broadphase(steps: int, warmup: int):
     return ms, BOXES, stages, counts
 
 
+def fixed_grid_bytes(name: str, k: int, entries: int, pairs: int) -> float:
+    """Minimal HBM payload per launch of the fixed-cell pass: AABB 24 B, entry 12 B (hash, box,
+    ordinal), pair 8 B; a sort pass moves its three lanes in and out (24 B per record)."""
+    words = max(1, ((k - 1).bit_length() + 7) // 8)
+    if name == "k_fixed_count":
+        return k * 28.0
+    if name == "k_fixed_emit":
+        return k * 28.0 + entries * 12.0
+    if name == "k_fixed_pairs":
+        return entries * (12.0 + 24.0) + pairs * 8.0
+    if name == "sort_pass_kernel":  # the entry sort's 4 + words passes and the pair sort's 2 words
+        return (entries * 24.0 * (4 + words) + pairs * 24.0 * (2 * words)) / (4 + 3 * words)
+    if name == "sort_histogram_kernel":
+        return (entries * 8.0 + pairs * 8.0) / 2
+    return 0.0
+
+
+def run_fixed_grid(steps: int, warmup: int):
+    import dynamis_amd
+    from dynamis_amd.broadphase import FixedGrid
+    from dynamis_amd.device import Event, Stream, profile_collect, profile_enable
+    from dynamis_amd.scenes import uniform_boxes
+
+    if dynamis_amd.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    dynamis_amd._native.call("dyn_set_device", 0)
+    boxes = uniform_boxes(BOXES, 1000.0, 0.2, 2.0, BOX_SEED)
+    stream = Stream()
+    grid = FixedGrid(BOXES, 4.0, stream=stream)
+    grid.upload(boxes)
+    for _ in range(warmup):
+        grid.run()
+    res = grid.read()
+    start, stop = Event(), Event()
+    start.record(stream)
+    for _ in range(steps):
+        grid.run()
+    stop.record(stream)
+    ms = start.elapsed_ms(stop)
+    stream.synchronize()
+    profile_enable(True)
+    for _ in range(min(steps, 10)):
+        grid.run()
+    stream.synchronize()
+    profile_enable(False)
+    stages = profile_collect()
+    counts = {"k": BOXES, "entries": len(res.entry_hash), "pairs": len(res.pairs),
+              "refused": res.refused_boxes + res.refused_pairs}
+    return ms, stages, counts
+
+
 def run_copy(count: int) -> int:
     """Copies of a known size for the HBM counters' calibration; prints one JSON line."""
     import ctypes
@@ -562,8 +616,8 @@ def run_adjoint(reps: int = 3) -> dict:
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "copy", "adjoint"],
-                        default="pile")
+    parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "fixed_grid", "copy",
+                                               "adjoint"], default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--profile-steps", type=int, default=5)
@@ -592,7 +646,21 @@ def main() -> int:
                           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                           "data": "synthetic", "config": r}))
         return 0
-    if wl == "broadphase":
+    if wl == "fixed_grid":
+        steps = max(args.steps if args.steps is not None else 50, 1)
+        warmup = max(args.warmup if args.warmup is not None else 10, 0)
+        ms, stages, counts = run_fixed_grid(steps, warmup)
+        table, roofline = stage_table(stages, counts, lambda nm: fixed_grid_bytes(
+            nm, counts["k"], counts["entries"], counts["pairs"]))
+        bodies = BOXES
+        cpu = None
+        config = {"workload": "fixed-cell sort + broadphase pass over 1,048,576 boxes (centres U[0,1000)^3 m, "
+                              "half-extents log-U[0.2,2] m), 4 m cells, keys (cell hash, box index)",
+                  "boxes": BOXES, "entries": counts["entries"], "pairs": counts["pairs"], "refused": counts["refused"],
+                  "seed": BOX_SEED, "cell_size": 4.0}
+        dtype = "u32"
+        measured = {}
+    elif wl == "broadphase":
         steps = max(args.steps if args.steps is not None else 50, 1)
         warmup = max(args.warmup if args.warmup is not None else 10, 0)
         ms, bodies, stages, counts = run_broadphase(steps, warmup)

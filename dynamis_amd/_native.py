@@ -66,6 +66,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
     "dyn_world_set_graphs": (c_int, [c_void_p, c_int]),
     "dyn_world_read_work": (c_int, [c_void_p, ctypes.POINTER(c_u32)]),
+    "dyn_fixed_grid_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),
+    "dyn_fixed_grid": (c_int, [c_void_p, c_void_p, c_u32, ctypes.c_float, c_void_p, c_void_p, c_void_p, c_u32,
+                               c_void_p, c_void_p, c_u32, c_void_p, c_void_p, c_size_t]),
     "dyn_sort_autotune": (c_int, [c_void_p, c_u32, c_u32, c_u32_p, c_u32_p]),
     "dyn_sort_tuning_export": (c_int, [c_void_p, c_u32, c_u32_p]),
     "dyn_sort_tuning_import": (c_int, [c_void_p, c_u32]),

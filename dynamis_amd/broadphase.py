@@ -121,3 +121,55 @@ class Broadphase:
             self.entries.download(abi.GRID_ENTRY, entries),
             np.stack([major, minor], axis=1) if pairs else np.zeros((0, 2), np.uint32),
             counters)
+
+
+@dataclass
+class FixedGridResult:
+    entry_hash: np.ndarray    # sorted entries: cell hash, box index, cell ordinal
+    entry_box: np.ndarray
+    entry_ordinal: np.ndarray
+    pairs: np.ndarray         # (P, 2) canonical overlapping pairs, first < second
+    refused_boxes: int
+    refused_pairs: int
+
+
+class FixedGrid:
+    """The literal fixed-cell broadphase (dyn_fixed_grid): a uniform grid of side `cell_size`,
+    keys (cell hash, box index), radix sort, cell runs, canonical overlapping pairs."""
+
+    def __init__(self, boxes: int, cell_size: float = 4.0, entry_capacity: int | None = None,
+                 pair_capacity: int | None = None, stream: Stream | None = None):
+        self.k = int(boxes)
+        self.cell_size = float(cell_size)
+        self.entry_capacity = int(entry_capacity if entry_capacity is not None else 8 * max(self.k, 1))
+        self.pair_capacity = int(pair_capacity if pair_capacity is not None else 4 * max(self.k, 1))
+        self.stream = stream or Stream()
+        self.boxes = DeviceBuffer(max(self.k, 1) * abi.AABB.itemsize)
+        self.entry_hash = DeviceBuffer(max(self.entry_capacity, 1) * 4)
+        self.entry_box = DeviceBuffer(max(self.entry_capacity, 1) * 4)
+        self.entry_ordinal = DeviceBuffer(max(self.entry_capacity, 1) * 4)
+        self.pair_first = DeviceBuffer(max(self.pair_capacity, 1) * 4)
+        self.pair_second = DeviceBuffer(max(self.pair_capacity, 1) * 4)
+        self.counts = DeviceBuffer.zeros(16)
+        need = lib().dyn_fixed_grid_scratch_bytes(self.k, self.entry_capacity, self.pair_capacity)
+        self.scratch = DeviceBuffer(need)
+
+    def upload(self, boxes: np.ndarray) -> None:
+        self.boxes.upload(np.ascontiguousarray(boxes, dtype=abi.AABB))
+
+    def run(self) -> None:
+        call("dyn_fixed_grid", self.stream.handle, self.boxes.ptr, self.k, ctypes.c_float(self.cell_size),
+             self.entry_hash.ptr, self.entry_box.ptr, self.entry_ordinal.ptr, self.entry_capacity,
+             self.pair_first.ptr, self.pair_second.ptr, self.pair_capacity, self.counts.ptr,
+             self.scratch.ptr, self.scratch.nbytes)
+
+    def read(self) -> FixedGridResult:
+        self.stream.synchronize()
+        counts = self.counts.download(np.uint32, 4)
+        e = min(int(counts[0]), self.entry_capacity)
+        p = min(int(counts[1]), self.pair_capacity)
+        first = self.pair_first.download(np.uint32, p)
+        second = self.pair_second.download(np.uint32, p)
+        return FixedGridResult(self.entry_hash.download(np.uint32, e), self.entry_box.download(np.uint32, e),
+                               self.entry_ordinal.download(np.uint32, e), np.stack([first, second], axis=1),
+                               int(counts[2]), int(counts[3]))

@@ -287,6 +287,21 @@ int dyn_world_set_graphs(dyn_world* world, int enabled);
 /* Blocks until every enqueued step has finished. */
 int dyn_world_wait(dyn_world* world);
 
+/* ---- the literal fixed-cell broadphase (no reference counterpart: the reference's grid is
+ * hierarchical, crates/dynamis-broadphase/src/passes.rs:76-160) ----
+ * Every box enters each cell of a uniform grid of side cell_size that its AABB touches, with the
+ * key (cell hash of grid_index.wgsl:28-33, box index); the keys are radix-sorted (hash major, box
+ * minor, cell ordinal payload: entries out in d_entry_*), and every pair of overlapping boxes is
+ * emitted once (in the cell holding the overlap's min corner) and canonicalised (first < second,
+ * sorted lexicographically). The pair set equals the hierarchical grid's on the same boxes.
+ * d_counts receives 4 words: entries, pairs, boxes refused (touching more than 64 cells), pairs
+ * refused (past pair_capacity); entries past entry_capacity are dropped. Asynchronous on stream. */
+size_t dyn_fixed_grid_scratch_bytes(uint32_t boxes, uint32_t entry_capacity, uint32_t pair_capacity);
+int dyn_fixed_grid(void* stream, const dyn_aabb* d_boxes, uint32_t boxes, float cell_size, uint32_t* d_entry_hash,
+                   uint32_t* d_entry_box, uint32_t* d_entry_ordinal, uint32_t entry_capacity, uint32_t* d_pair_first,
+                   uint32_t* d_pair_second, uint32_t pair_capacity, uint32_t* d_counts, void* d_scratch,
+                   size_t scratch_bytes);
+
 /* ---- autotuning (no reference counterpart: the reference's sort has one fixed tiling) ----
  * A sort pass's tiling (keys per lane 8 or 16, cap on persistent workgroups 512/1024/2048) is
  * chosen per (device, ceil(log2(capacity))) from a process-wide table; untuned buckets use the
