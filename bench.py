@@ -93,8 +93,8 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
     manifold_pairs = convex_manifolds
     if name == "k_solve":               # manifold, two bodies' pose/velocity/constants, targets,
         return contacts * (manifold + 16.0 * points + 240.0)  # delta words
-    if name == "k_position_block":
-        return contacts * (48.0 + 20.0 * points + 272.0)
+    if name == "k_position_block":      # position row, both sides' resolution, both sides' words
+        return contacts * (48.0 + 32.0 + 32.0)
     if name in ("k_apply", "k_position_apply"):
         return rows * 72.0
     if name == "k_narrow":              # two bodies and colliders per pair, manifold out

@@ -85,6 +85,7 @@ struct W {
     uint32_t* work_minor;  // the sort's unused minor lane
     uint32_t* dest;
     dyn_contact* contacts;
+    uint2* ckeys;          // per contact: its collider pair (a, b), the key the relay searches
     uint32_t* matched;
     uint32_t contact_cap;
     uint32_t* ncontacts;  // device word: contacts this step
@@ -718,7 +719,9 @@ __global__ void __launch_bounds__(kThreads) k_contacts_scatter(W w) {
     if (index >= live || w.valid[index] != kPairContact) return;
     const uint32_t at = w.dest[index];
     if (at >= w.contact_cap) return;
-    w.contacts[at] = w.raw[index];
+    const dyn_contact& c = w.raw[index];
+    w.contacts[at] = c;
+    w.ckeys[at] = make_uint2(c.a, c.b);
     w.matched[at] = 0u;
 }
 
@@ -804,18 +807,23 @@ __device__ __forceinline__ uint32_t resolve_row(const W& w, uint32_t id, uint32_
 __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     if (index >= *w.narchive) return;
-    const dyn_contact held = w.archive[index];
+    // records are read in place and only what the relay changes is written back: the points'
+    // impulses, the carried totals and the event bits (not the whole 352-byte record)
+    const dyn_contact& held = w.archive[index];
     if (resolve_row(w, held.first_body_id, held.first_generation) == DYN_NO_BODY ||
         resolve_row(w, held.second_body_id, held.second_generation) == DYN_NO_BODY) return;
     const uint32_t count = *w.ncontacts;
     uint32_t lo = 0, hi = count;
+    // lower bound of (a, b) over the dense keys (8 bytes a contact, not its 352-byte record)
     while (lo < hi) {
         const uint32_t mid = (lo + hi) / 2u;
-        const uint32_t ma = w.contacts[mid].a, mb = w.contacts[mid].b;
-        if (ma < held.a || (ma == held.a && mb < held.b)) lo = mid + 1u; else hi = mid;
+        const uint2 m = w.ckeys[mid];
+        if (m.x < held.a || (m.x == held.a && m.y < held.b)) lo = mid + 1u; else hi = mid;
     }
-    if (lo >= count || w.contacts[lo].a != held.a || w.contacts[lo].b != held.b) return;
-    dyn_contact cur = w.contacts[lo];
+    if (lo >= count) return;
+    const uint2 at = w.ckeys[lo];
+    if (at.x != held.a || at.y != held.b) return;
+    dyn_contact& cur = w.contacts[lo];
     if (!same_pair(held, cur)) return;
     if (!(held.events & kContactAnnounced)) return;
     w.matched[lo] = 1u;
@@ -827,7 +835,6 @@ __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     }
     carry(cur);
     cur.events |= kContactAnnounced;
-    w.contacts[lo] = cur;
 }
 
 // contact_begin.wgsl:41-71 over the resting archive sorted by (min id, max id)
@@ -1749,12 +1756,14 @@ __global__ void __launch_bounds__(kThreads) k_thaw(W w) {
 }
 
 // contact_archive.wgsl + freeze_contacts.wgsl flags (freeze flags after the resting flags)
+// contact_archive.wgsl keeps the step's contacts for the next step's relay: here the contact and
+// archive buffers trade places at the end of the step (swap_contact_buffers) instead of copying,
+// so this pass only marks the contacts freeze_contacts.wgsl:13-45 moves to the resting archive.
 __global__ void __launch_bounds__(kThreads) k_archive(W w, uint32_t* freeze_flags) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *w.ncontacts) return;
-    const dyn_contact c = w.contacts[i];
-    w.archive[i] = c;
-    const uint32_t f = w.owner[c.a], s = w.owner[c.b];
+    const uint2 ab = w.ckeys[i];
+    const uint32_t f = w.owner[ab.x], s = w.owner[ab.y];
     const bool frozen = !body_is_active(w.st[f], w.de[f]) && !body_is_active(w.st[s], w.de[s]);
     freeze_flags[i] = frozen ? 1u : 0u;
 }
@@ -1854,8 +1863,11 @@ struct dyn_world {
     // its grid from the capacities and its arguments from the world, so only dt (and the
     // shape pointers, replaced by an upload) can change what a step launches.
     bool graphs = true;
-    hipGraphExec_t graph = nullptr;
-    float graph_dt = 0.0f;
+    // The step captured once per parity: the contact and archive buffers trade places every step,
+    // so even and odd steps launch the same kernels over swapped pointers.
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    float graph_dt[2] = {0.0f, 0.0f};
+    uint32_t parity = 0;
     // the tape a recorded step writes (dyn::world_record); null when not recording
     float* tape_states = nullptr;
     float* tape_contacts = nullptr;
@@ -1939,13 +1951,25 @@ int worklist(dyn_world* world, uint32_t mode) {
 int world_step(dyn_world* world, float dt);
 
 void drop_graph(dyn_world* world) {
-    if (world->graph) (void)hipGraphExecDestroy(world->graph);
-    world->graph = nullptr;
+    for (int p = 0; p < 2; ++p) {
+        if (world->graph[p]) (void)hipGraphExecDestroy(world->graph[p]);
+        world->graph[p] = nullptr;
+    }
 }
 
-// Records one step on the world's stream into a graph and instantiates it for dt.
+// The step's contacts become the next step's archive (contact_archive.wgsl) and the old archive
+// the buffer the next step compacts into.
+void swap_contact_buffers(dyn_world* world) {
+    std::swap(world->w.contacts, world->w.archive);
+    world->parity ^= 1u;
+}
+
+// Records one step on the world's stream into the graph of the current parity and instantiates
+// it for dt (the recording swaps the buffers, as a launched step does).
 int capture_step(dyn_world* world, float dt) {
-    drop_graph(world);
+    const uint32_t p = world->parity;
+    if (world->graph[p]) (void)hipGraphExecDestroy(world->graph[p]);
+    world->graph[p] = nullptr;
     DYN_HIP(hipStreamBeginCapture(world->stream, hipStreamCaptureModeThreadLocal));
     const int rc = world_step(world, dt);
     hipGraph_t graph = nullptr;
@@ -1954,13 +1978,13 @@ int capture_step(dyn_world* world, float dt) {
         if (graph) (void)hipGraphDestroy(graph);
         return rc != DYN_OK ? rc : dyn::hip_fail(ended, "dyn_world_step: graph capture");
     }
-    const hipError_t made = hipGraphInstantiate(&world->graph, graph, nullptr, nullptr, 0);
+    const hipError_t made = hipGraphInstantiate(&world->graph[p], graph, nullptr, nullptr, 0);
+    world->graph_dt[p] = dt;
     (void)hipGraphDestroy(graph);
     if (made != hipSuccess) {
-        world->graph = nullptr;
+        world->graph[p] = nullptr;
         return dyn::hip_fail(made, "dyn_world_step: graph instantiation");
     }
-    world->graph_dt = dt;
     return DYN_OK;
 }
 
@@ -2088,6 +2112,7 @@ int world_step(dyn_world* world, float dt) {
         k_consume<<<1, 64, 0, st>>>(w);
     }
     DYN_LAUNCHED("k_consume");
+    swap_contact_buffers(world);
     return DYN_OK;
 }
 
@@ -2191,6 +2216,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.work_minor, pair_capacity);
     A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
+    A(w.ckeys, contact_capacity);
     A(w.matched, contact_capacity);
     A(w.archive, contact_capacity);
     A(w.resting, contact_capacity);
@@ -2350,8 +2376,13 @@ int dyn_world_step(dyn_world* world, float dt) {
     }
     // profiled and recorded steps run eagerly
     if (!world->graphs || dyn::profiling() || world->tape_states) return world_step(world, dt);
-    if (!world->graph || world->graph_dt != dt) DYN_TRY(capture_step(world, dt));
-    DYN_HIP(hipGraphLaunch(world->graph, world->stream));
+    const uint32_t p = world->parity;
+    if (!world->graph[p] || world->graph_dt[p] != dt) {
+        DYN_TRY(capture_step(world, dt));  // recording the step swapped the buffers
+    } else {
+        swap_contact_buffers(world);
+    }
+    DYN_HIP(hipGraphLaunch(world->graph[p], world->stream));
     return DYN_OK;
 }
 
@@ -2399,7 +2430,8 @@ int dyn_world_read_contacts(dyn_world* world, dyn_contact* out, uint32_t capacit
     DYN_HIP(hipMemcpy(&n, world->w.ncontacts, 4u, hipMemcpyDeviceToHost));
     *count = n;
     const uint32_t copy = n < capacity ? n : capacity;
-    if (copy && out) DYN_HIP(hipMemcpy(out, world->w.contacts, sizeof(dyn_contact) * copy, hipMemcpyDeviceToHost));
+    // the last step's contacts: since its end they are the archive
+    if (copy && out) DYN_HIP(hipMemcpy(out, world->w.archive, sizeof(dyn_contact) * copy, hipMemcpyDeviceToHost));
     return DYN_OK;
 }
 
