@@ -107,8 +107,10 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
         return manifold_pairs * (316.0 + 68.0 + 352.0)
     if name in ("k_integrate", "k_advance"):
         return n * 126.0
-    if name in ("k_relay", "k_contact_begin", "k_contacts_scatter", "k_archive"):
+    if name in ("k_relay", "k_contact_begin", "k_contacts_scatter"):
         return 2.0 * contacts * manifold
+    if name == "k_archive":             # the archive is the swapped contact buffer: freeze marks only
+        return contacts * 40.0
     return broadphase_bytes(name, k, entries, pairs)
 
 
