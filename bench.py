@@ -607,8 +607,29 @@ def run_adjoint(reps: int = 3) -> dict:
             fwd.append(t1 - t0)
             bwd.append(t3 - t2)
     a.close()
+    # the whole rollout as one segment: every step's tape stays in HBM (no recompute)
+    a1 = adj.Adjoint(batch, steps, steps)
+    f1, b1 = [], []
+    for rep in range(reps + 1):
+        batch.wait()
+        t0 = time.perf_counter()
+        a1.forward(dt)
+        batch.wait()
+        t1 = time.perf_counter()
+        g = grad(batch.states())
+        batch.wait()
+        t2 = time.perf_counter()
+        a1.backward(g)
+        t3 = time.perf_counter()
+        if rep:
+            f1.append(t1 - t0)
+            b1.append(t3 - t2)
+    a1.close()
+    one = {"forward_ms": min(f1) * 1e3, "backward_ms": min(b1) * 1e3, "bwd_fwd": min(b1) / min(f1),
+           "note": "checkpoint_every = steps: the forward records every step's tape and the backward recomputes none"}
     f, b = min(fwd), min(bwd)
-    return {"workload": f"{worlds} worlds x {boxes} boxes on a plane, dt = 1/120, 8 iterations, {steps} steps, "
+    return {"single_segment": one,
+            "workload": f"{worlds} worlds x {boxes} boxes on a plane, dt = 1/120, 8 iterations, {steps} steps, "
                         f"checkpoint every {every}; loss = squared distance of each world's centre of mass "
                         f"to a target",
             "bodies": worlds * boxes, "forward_ms": f * 1e3, "backward_ms": b * 1e3, "bwd_fwd": b / f,

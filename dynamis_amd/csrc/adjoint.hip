@@ -262,6 +262,7 @@ struct dyn_adjoint {
     uint32_t n = 0, d = 0, k = 0, cap = 0;
     float dt = 0.0f;
     bool forwarded = false;
+    bool last_recorded = false;  // the tape slots hold the last segment, recorded by the forward rollout
     AdjParams P{};
     AdjRange* d_ranges = nullptr;
     size_t snap_bytes = 0;
@@ -423,12 +424,25 @@ int dyn_adjoint_forward(dyn_adjoint* a, float dt) {
     k_adj_bodies<<<grid_for(a->n), kAdjThreads, 0, a->stream>>>(world_states(a->world), world_descs(a->world), a->n,
                                                                  a->d_bodies);
     DYN_LAUNCHED("k_adj_bodies");
+    // the last segment records its tape as it steps, so the backward pass starts without a recompute
+    const uint32_t last = (a->segments - 1u) * a->every;
+    const size_t n = a->n, cap = a->cap;
+    DYN_TRY(zero(a, reinterpret_cast<float*>(a->d_error), 1));
+    a->last_recorded = false;
     for (uint32_t t = 0; t < a->steps; ++t) {
         if (t % a->every == 0) DYN_TRY(dyn_world_snapshot(a->world, a->d_snaps + a->snap_bytes * (t / a->every)));
-        DYN_TRY(dyn_world_step(a->world, dt));
+        if (t >= last) {
+            const size_t j = t - last;
+            world_record(a->world, a->d_states + j * n * kStateFloats, a->d_contacts + j * cap * kContactFloats,
+                         a->d_counts + j, a->d_error, a->d_relay_src);
+        }
+        const int rc = dyn_world_step(a->world, dt);
+        world_record(a->world, nullptr, nullptr, nullptr, nullptr, nullptr);
+        DYN_TRY(rc);
     }
     DYN_TRY(dyn_world_snapshot(a->world, a->d_snaps + a->snap_bytes * a->segments));
     a->forwarded = true;
+    a->last_recorded = true;
     return DYN_OK;
 }
 
@@ -445,18 +459,21 @@ int dyn_adjoint_backward(dyn_adjoint* a, const float* d_state_grad, float* d_sta
     DYN_TRY(zero(a, W.mubar, a->k));
     DYN_TRY(zero(a, a->d_accb[0], cap * kAccFloats));
     DYN_TRY(zero(a, a->d_accb[1], cap * kAccFloats));
-    DYN_TRY(zero(a, reinterpret_cast<float*>(a->d_error), 1));
     for (uint32_t seg = a->segments; seg-- > 0;) {
         const uint32_t first = seg * a->every;
         const uint32_t len = (first + a->every <= a->steps) ? a->every : a->steps - first;
-        DYN_TRY(dyn_world_restore(a->world, a->d_snaps + a->snap_bytes * seg));
-        for (uint32_t j = 0; j < len; ++j) {
-            world_record(a->world, a->d_states + static_cast<size_t>(j) * n * kStateFloats,
-                         a->d_contacts + static_cast<size_t>(j) * cap * kContactFloats, a->d_counts + j, a->d_error,
-                         a->d_relay_src);
-            const int rc = dyn_world_step(a->world, a->dt);
-            world_record(a->world, nullptr, nullptr, nullptr, nullptr, nullptr);
-            DYN_TRY(rc);
+        if (!(seg + 1u == a->segments && a->last_recorded)) {
+            if (seg + 1u == a->segments) DYN_TRY(zero(a, reinterpret_cast<float*>(a->d_error), 1));
+            DYN_TRY(dyn_world_restore(a->world, a->d_snaps + a->snap_bytes * seg));
+            for (uint32_t j = 0; j < len; ++j) {
+                world_record(a->world, a->d_states + static_cast<size_t>(j) * n * kStateFloats,
+                             a->d_contacts + static_cast<size_t>(j) * cap * kContactFloats, a->d_counts + j,
+                             a->d_error, a->d_relay_src);
+                const int rc = dyn_world_step(a->world, a->dt);
+                world_record(a->world, nullptr, nullptr, nullptr, nullptr, nullptr);
+                DYN_TRY(rc);
+            }
+            if (seg + 1u != a->segments) a->last_recorded = false;  // the slots now hold this segment
         }
         if (seg + 1u == a->segments) {  // the model's limits hold for the whole rollout or it is refused
             uint32_t error = 0;
@@ -476,8 +493,8 @@ int dyn_adjoint_backward(dyn_adjoint* a, const float* d_state_grad, float* d_sta
             DYN_LAUNCHED("k_adj_step");
         }
     }
-    // leave the world where the forward rollout ended
-    DYN_TRY(dyn_world_restore(a->world, a->d_snaps + a->snap_bytes * a->segments));
+    // leave the world where the forward rollout ended (already so when nothing was recomputed)
+    if (!a->last_recorded) DYN_TRY(dyn_world_restore(a->world, a->d_snaps + a->snap_bytes * a->segments));
     if (d_state0_grad) {
         DYN_HIP(hipMemcpyAsync(d_state0_grad, W.xb, sizeof(float) * n * kStateFloats, hipMemcpyDeviceToDevice, st));
     }

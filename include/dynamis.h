@@ -332,7 +332,8 @@ int dyn_world_restore(dyn_world* world, const void* d_src);
 /* ---- the differentiable rollout (no reference counterpart: the reference has no adjoint) ----
  *
  * dyn_adjoint_forward steps the world `steps` times with dt (HIP graphs as usual), keeping a
- * snapshot every `checkpoint_every` steps. dyn_adjoint_backward takes the adjoint of the final
+ * snapshot every `checkpoint_every` steps, recording the tape of the last segment as it goes.
+ * dyn_adjoint_backward takes the adjoint of the final
  * states (device, 16 floats per row: position xyz _, orientation xyzw, velocity xyz _, spin xyz _)
  * and returns, on the device, the adjoint of the initial states (same layout), of each row's mass
  * (mass and inertia scaled together at fixed shape; zero for rows that do not simulate) and of each
@@ -351,8 +352,10 @@ int dyn_adjoint_create(dyn_adjoint** out, dyn_world* world, uint32_t steps, uint
 int dyn_adjoint_forward(dyn_adjoint* adjoint, float dt);
 int dyn_adjoint_backward(dyn_adjoint* adjoint, const float* d_state_grad, float* d_state0_grad,
                          float* d_mass_grad, float* d_friction_grad);
-/* Host copies of the last recomputed segment's tape (slot < checkpoint_every; with
- * checkpoint_every >= steps, slot t is step t) and of the rollout's body records. */
+/* Host copies of the tape slots (slot < checkpoint_every): the segment last recorded, by the
+ * forward rollout (its last segment) or by the backward pass's recompute; with checkpoint_every >=
+ * steps, slot t is step t and the backward pass recomputes nothing. Also the rollout's body
+ * records. */
 int dyn_adjoint_read_tape(dyn_adjoint* adjoint, uint32_t slot, float* states, float* contacts,
                           uint32_t capacity, uint32_t* count);
 int dyn_adjoint_read_bodies(dyn_adjoint* adjoint, float* out);
