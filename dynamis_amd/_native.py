@@ -49,6 +49,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
                                c_void_p, c_u32, c_u32, c_u32, c_void_p, c_size_t]),
     "dyn_broadphase_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),
     "dyn_broadphase": (c_int, [c_void_p, c_void_p]),
+    "dyn_narrowphase_scratch_bytes": (c_size_t, [c_u32]),
+    "dyn_narrowphase": (c_int, [c_void_p, c_void_p]),
     "dyn_config_default": (None, [c_void_p]),
     "dyn_step_params_build": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p]),
     "dyn_inertia_inverse": (c_int, [c_void_p, c_void_p]),

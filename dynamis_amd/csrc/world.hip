@@ -249,14 +249,21 @@ __device__ __forceinline__ bool convex_path(const dyn_collider& fc, const dyn_co
 // The pair filter of narrowphase.wgsl:518-550; false when the pair makes no contact.
 __device__ __forceinline__ bool pair_admitted(const W& w, uint32_t fs, uint32_t ss, uint32_t* fb_out,
                                               uint32_t* sb_out) {
+    // indices outside the records (possible only for pairs handed to dyn_narrowphase) make no
+    // contact
+    if (fs >= w.k || ss >= w.k) return false;
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
-    if (fb == DYN_NO_BODY || sb == DYN_NO_BODY || fb == sb) return false;
+    if (fb == DYN_NO_BODY || sb == DYN_NO_BODY || fb == sb || fb >= w.n || sb >= w.n) return false;
     const dyn_body_desc& fd = w.de[fb];
     const dyn_body_desc& sd = w.de[sb];
     if (!kind_moves(fd) && !kind_moves(sd)) return false;
     const dyn_collider& fc = w.co[fs];
     const dyn_collider& sc = w.co[ss];
     if (fc.kind == DYN_SHAPE_NONE || sc.kind == DYN_SHAPE_NONE) return false;
+    if ((fc.kind == DYN_SHAPE_HULL && fc.source >= w.source_count) ||
+        (sc.kind == DYN_SHAPE_HULL && sc.source >= w.source_count)) {
+        return false;
+    }
     const uint32_t g1 = fc.collision_group != DYN_NO_COLLISION_FILTER ? fc.collision_group : fd.collision_group;
     const uint32_t m1 = fc.collision_mask != DYN_NO_COLLISION_FILTER ? fc.collision_mask : fd.collision_mask;
     const uint32_t g2 = sc.collision_group != DYN_NO_COLLISION_FILTER ? sc.collision_group : sd.collision_group;
@@ -1916,10 +1923,8 @@ int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_
 
 // The worklist of convex pairs of one mode, grouped by kind class: block histograms, their
 // class-major scan (total in work_total[mode]), the scatter.
-int worklist(dyn_world* world, uint32_t mode) {
+int worklist(const dyn::W& w, hipStream_t st, uint32_t mode) {
     using namespace dyn;
-    W& w = world->w;
-    hipStream_t st = world->stream;
     const uint32_t blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
     {
         ProfileScope scope("k_class_count", st);
@@ -1939,14 +1944,47 @@ int worklist(dyn_world* world, uint32_t mode) {
     return DYN_OK;
 }
 
-#define LAUNCH(name, grid, ...)                                            \
+#define LAUNCH_ON(stream, name, grid, ...)                                 \
     do {                                                                   \
         {                                                                  \
-            ::dyn::ProfileScope scope_(#name, world->stream);              \
-            name<<<(grid), ::dyn::kThreads, 0, world->stream>>>(__VA_ARGS__); \
+            ::dyn::ProfileScope scope_(#name, (stream));                   \
+            name<<<(grid), ::dyn::kThreads, 0, (stream)>>>(__VA_ARGS__);   \
         }                                                                  \
         DYN_LAUNCHED(#name);                                               \
     } while (0)
+#define LAUNCH(name, grid, ...) LAUNCH_ON(world->stream, name, grid, __VA_ARGS__)
+
+// The narrowphase of the candidate pairs on pmaj/pmin (narrowphase.wgsl:518-685): k_narrow
+// writes the analytic manifolds and marks the convex-path pairs, which (when `convex`) GJK, EPA
+// and the convex manifolds take up over class-grouped worklists. Afterwards valid[i] is
+// kPairContact where raw[i] holds pair i's contact and kPairNone elsewhere, for every pair below
+// the PAIRS counter (and the capacity).
+int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch, size_t work_scratch_bytes) {
+    using namespace dyn;
+    const uint32_t pb = blocks_for(w.pair_cap);
+    LAUNCH_ON(st, k_narrow, pb, w);
+    if (!convex) return DYN_OK;
+    DYN_TRY(worklist(w, st, 0u));
+    const uint32_t class_blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
+    LAUNCH_ON(st, k_work_keys, pb, w, 0u);
+    DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[0], nullptr, 1u,
+                           kWorkKeyBytes, 0u, work_scratch, work_scratch_bytes));
+    LAUNCH_ON(st, k_narrow_convex, pb, w, class_blocks);
+    DYN_TRY(worklist(w, st, 2u));
+    const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
+    const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
+    {
+        ProfileScope scope("k_narrow_epa", st);
+        k_narrow_epa<<<groups, kEpaWaves * kWave, 0, st>>>(w);
+    }
+    DYN_LAUNCHED("k_narrow_epa");
+    DYN_TRY(worklist(w, st, 1u));
+    LAUNCH_ON(st, k_work_keys, pb, w, 1u);
+    DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[1], nullptr, 1u,
+                           kWorkKeyBytes, 0u, work_scratch, work_scratch_bytes));
+    LAUNCH_ON(st, k_narrow_finish, pb, w);
+    return DYN_OK;
+}
 
 int world_step(dyn_world* world, float dt);
 
@@ -2022,28 +2060,7 @@ int world_step(dyn_world* world, float dt) {
     io.scratch_bytes = world->bp_scratch_bytes;
     io.d_worlds = w.worlds;
     DYN_TRY(dyn_broadphase(st, &io));
-    LAUNCH(k_narrow, pb, w);
-    if (world->convex_colliders) {
-        DYN_TRY(worklist(world, 0u));
-        const uint32_t class_blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
-        LAUNCH(k_work_keys, pb, w, 0u);
-        DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[0], nullptr,
-                               1u, kWorkKeyBytes, 0u, world->work_scratch, world->work_scratch_bytes));
-        LAUNCH(k_narrow_convex, pb, w, class_blocks);
-        DYN_TRY(worklist(world, 2u));
-        const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
-        const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
-        {
-            ProfileScope scope("k_narrow_epa", st);
-            k_narrow_epa<<<groups, kEpaWaves * kWave, 0, st>>>(w);
-        }
-        DYN_LAUNCHED("k_narrow_epa");
-        DYN_TRY(worklist(world, 1u));
-        LAUNCH(k_work_keys, pb, w, 1u);
-        DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[1], nullptr,
-                               1u, kWorkKeyBytes, 0u, world->work_scratch, world->work_scratch_bytes));
-        LAUNCH(k_narrow_finish, pb, w);
-    }
+    DYN_TRY(narrowphase(w, st, world->convex_colliders, world->work_scratch, world->work_scratch_bytes));
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
     DYN_TRY(scan(world, w.valid, pair_count, w.pair_cap, w.dest, &w.scan_total[0]));
     LAUNCH(k_contacts_scatter, pb, w);
@@ -2433,6 +2450,91 @@ int dyn_world_read_contacts(dyn_world* world, dyn_contact* out, uint32_t capacit
     // the last step's contacts: since its end they are the archive
     if (copy && out) DYN_HIP(hipMemcpy(out, world->w.archive, sizeof(dyn_contact) * copy, hipMemcpyDeviceToHost));
     return DYN_OK;
+}
+
+namespace {
+
+// dyn_narrowphase's scratch: the convex worklists and their sort.
+struct NarrowLayout {
+    uint8_t* pkind;
+    uint32_t *worklist, *class_hist, *work_total, *work_keys, *work_minor;
+    void* sort;
+    size_t sort_bytes, bytes;
+};
+
+NarrowLayout narrow_layout(uint32_t pair_capacity, void* base) {
+    using namespace dyn;
+    NarrowLayout l{};
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t here = at;
+        at = (at + bytes + 255u) & ~static_cast<size_t>(255u);
+        return static_cast<char*>(base) + here;
+    };
+    const size_t cap = pair_capacity ? pair_capacity : 1u;
+    l.pkind = reinterpret_cast<uint8_t*>(take(cap));
+    l.worklist = reinterpret_cast<uint32_t*>(take(4u * cap));
+    l.class_hist = reinterpret_cast<uint32_t*>(take(4u * kClasses * static_cast<size_t>(ceil_div(
+                                                                         static_cast<uint32_t>(cap), kClassBlock))));
+    l.work_total = reinterpret_cast<uint32_t*>(take(4u * 4u));
+    l.work_keys = reinterpret_cast<uint32_t*>(take(4u * cap));
+    l.work_minor = reinterpret_cast<uint32_t*>(take(4u * cap));
+    l.sort_bytes = dyn_sort_scratch_bytes(static_cast<uint32_t>(cap), kWorkKeyBytes);
+    l.sort = take(l.sort_bytes);
+    l.bytes = at;
+    return l;
+}
+
+}  // namespace
+
+size_t dyn_narrowphase_scratch_bytes(uint32_t pair_capacity) {
+    return narrow_layout(pair_capacity, nullptr).bytes;
+}
+
+int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io) {
+    using namespace dyn;
+    if (!io || !io->params) return fail(DYN_E_INVALID, "dyn_narrowphase: null io or params");
+    if (io->pair_capacity > kSortMaxCapacity) return fail(DYN_E_INVALID, "dyn_narrowphase: capacity above 2^30-1");
+    if (!io->d_counters || !io->d_scratch || (io->pair_capacity && (!io->d_pair_major || !io->d_pair_minor ||
+                                                                    !io->d_contacts || !io->d_valid))) {
+        return fail(DYN_E_INVALID, "dyn_narrowphase: null buffer");
+    }
+    if ((io->bodies && (!io->d_states || !io->d_descs)) || (io->colliders && (!io->d_colliders || !io->d_owners))) {
+        return fail(DYN_E_INVALID, "dyn_narrowphase: null records");
+    }
+    if (io->source_count && (!io->d_sources || !io->d_vertices)) {
+        return fail(DYN_E_INVALID, "dyn_narrowphase: null shape sources");
+    }
+    const NarrowLayout l = narrow_layout(io->pair_capacity, io->d_scratch);
+    if (io->scratch_bytes < l.bytes) {
+        return fail(DYN_E_CAPACITY, "dyn_narrowphase: scratch smaller than dyn_narrowphase_scratch_bytes");
+    }
+    if (io->pair_capacity == 0u) return DYN_OK;
+    W w{};
+    w.p = *io->params;
+    w.n = io->bodies;
+    w.k = io->colliders;
+    // the kernels take the records through the world's non-const view and only read them
+    w.st = const_cast<dyn_body_state*>(io->d_states);
+    w.de = const_cast<dyn_body_desc*>(io->d_descs);
+    w.co = const_cast<dyn_collider*>(io->d_colliders);
+    w.owner = const_cast<uint32_t*>(io->d_owners);
+    w.sources = io->d_sources;
+    w.vertices = reinterpret_cast<const float4*>(io->d_vertices);
+    w.source_count = io->source_count;
+    w.counters = const_cast<uint32_t*>(io->d_counters);
+    w.pmaj = const_cast<uint32_t*>(io->d_pair_major);
+    w.pmin = const_cast<uint32_t*>(io->d_pair_minor);
+    w.pair_cap = io->pair_capacity;
+    w.raw = io->d_contacts;
+    w.valid = io->d_valid;
+    w.pkind = l.pkind;
+    w.worklist = l.worklist;
+    w.class_hist = l.class_hist;
+    w.work_total = l.work_total;
+    w.work_keys = l.work_keys;
+    w.work_minor = l.work_minor;
+    return narrowphase(w, static_cast<hipStream_t>(stream), true, l.sort, l.sort_bytes);
 }
 
 int dyn_world_read_pairs(dyn_world* world, uint32_t* major, uint32_t* minor, uint32_t capacity,

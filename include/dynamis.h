@@ -144,6 +144,48 @@ size_t dyn_broadphase_scratch_bytes(uint32_t colliders, uint32_t entry_capacity,
  */
 int dyn_broadphase(void* stream, const dyn_broadphase_io* io);
 
+/* ---- narrowphase (crates/dynamis-rigid/src/narrowphase.rs, crates/dynamis-rigid/shaders/
+ *      narrowphase.wgsl:518-685, crates/dynamis-shader/shaders/convex.wgsl) ------------------- */
+
+typedef struct dyn_narrowphase_io {
+    /* inputs */
+    const dyn_step_params* params;       /* host; contact_margin, dt (CCD margins) and the
+                                            friction and restitution combine modes are read */
+    const dyn_body_state* d_states;      /* bodies by row (device) */
+    const dyn_body_desc* d_descs;
+    uint32_t bodies;
+    uint32_t colliders;
+    const dyn_collider* d_colliders;     /* K collider records (device) */
+    const uint32_t* d_owners;            /* K body rows, DYN_NO_BODY for none */
+    const dyn_shape_source* d_sources;   /* hull shape sources as dyn_world_upload_shapes takes
+                                            them, or NULL without hulls */
+    const float* d_vertices;             /* their float4 vertex rows */
+    uint32_t source_count;
+    uint32_t pair_capacity;
+    const uint32_t* d_pair_major;        /* candidate pairs (first, second collider), as
+                                            dyn_broadphase leaves them */
+    const uint32_t* d_pair_minor;
+    const uint32_t* d_counters;          /* counter block; PAIRS is read (device) */
+    /* outputs (device) */
+    dyn_contact* d_contacts;             /* pair_capacity raw contact slots (contacts_raw) */
+    uint32_t* d_valid;                   /* pair_capacity words (contact_valid): 1 where the
+                                            pair's slot holds its contact, 0 elsewhere */
+    void* d_scratch;
+    size_t scratch_bytes;
+} dyn_narrowphase_io;
+
+/* Scratch bytes dyn_narrowphase needs. */
+size_t dyn_narrowphase_scratch_bytes(uint32_t pair_capacity);
+/*
+ * The narrowphase pass over the first min(PAIRS, pair_capacity) candidate pairs: the reference's
+ * per-pair `work` (narrowphase.wgsl:518-685) with its pair filter, the analytic manifolds, GJK,
+ * EPA, the direction probe and the clipped convex manifolds, material combination and anchors,
+ * writing contact_raw / contact_valid as the reference's pass does (narrowphase.rs:126);
+ * compaction is the caller's. Pairs naming colliders, bodies or hull sources outside the records
+ * make no contact. The same kernels run inside dyn_world_step.
+ */
+int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io);
+
 /* ---- host record encoding (crates/dynamis-model/src/config.rs:12-100 PhysicsConfig,
  *      body.rs:89 BodyDesc; crates/dynamis-abi/src/body.rs:96-139, collider.rs:47-102,
  *      param.rs:110-137; crates/dynamis-model/src/mass.rs) -------------------------------------- */

@@ -41,6 +41,10 @@ uint32_t oracle_world_tape(const oracle_world* w, dyn_body_state* start, dyn_con
 uint32_t oracle_world_contacts(const oracle_world* w, dyn_contact* out, uint32_t capacity);
 void oracle_world_counters(const oracle_world* w, uint32_t* out);
 /* Sorted candidate pairs of the last step; returns the count. */
+/* The narrowphase alone over the given pairs and the current records (contacts_raw,
+ * contact_valid); -1 when count exceeds the pair capacity or a pair names no collider. */
+int oracle_world_narrowphase(oracle_world* w, const uint32_t* major, const uint32_t* minor, uint32_t count,
+                             dyn_contact* raw, uint32_t* valid);
 uint32_t oracle_world_pairs(const oracle_world* w, uint32_t* major, uint32_t* minor,
                             uint32_t capacity);
 

@@ -2674,6 +2674,26 @@ void oracle_world_counters(const oracle_world* w, uint32_t* out) {
     memcpy(out, w->counters, sizeof w->counters);
 }
 
+/* The narrowphase pass alone (narrowphase.wgsl:518-685, dispatched at narrowphase.rs:126) over
+ * the given candidate pairs and the world's current records: contacts_raw and contact_valid for
+ * pairs [0, count), count <= the world's pair capacity. The world's own pair list is replaced. */
+int oracle_world_narrowphase(oracle_world* w, const uint32_t* major, const uint32_t* minor, uint32_t count,
+                             dyn_contact* raw, uint32_t* valid) {
+    if (count > w->pair_cap) return -1;
+    for (uint32_t i = 0; i < count; i++) {
+        if (major[i] >= w->k || minor[i] >= w->k) return -1;
+    }
+    memcpy(w->pmaj, major, (size_t)count * 4u);
+    memcpy(w->pmin, minor, (size_t)count * 4u);
+    w->npairs = count;
+    for (uint32_t i = 0; i < count; i++) narrow_work(w, i);
+    for (uint32_t i = 0; i < count; i++) {
+        valid[i] = w->valid[i];
+        if (w->valid[i]) raw[i] = w->raw[i];
+    }
+    return 0;
+}
+
 uint32_t oracle_world_pairs(const oracle_world* w, uint32_t* major, uint32_t* minor,
                             uint32_t capacity) {
     uint32_t n = w->npairs < capacity ? w->npairs : capacity;

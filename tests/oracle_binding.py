@@ -62,6 +62,8 @@ class Oracle:
         h.oracle_world_counters.argtypes = [_p, _p]
         h.oracle_world_pairs.restype = _u32
         h.oracle_world_pairs.argtypes = [_p, _p, _p, _u32]
+        h.oracle_world_narrowphase.restype = ctypes.c_int
+        h.oracle_world_narrowphase.argtypes = [_p, _p, _p, _u32, _p, _p]
         h.oracle_world_shapes.restype = ctypes.c_int
         h.oracle_world_shapes.argtypes = [_p, _p, _u32, _p, _u32]
         h.oracle_convex_query.restype = ctypes.c_int
@@ -198,6 +200,19 @@ class OracleWorld:
                                             self.pair_capacity)
         count = min(count, self.pair_capacity)
         return np.stack([major[:count], minor[:count]], axis=1)
+
+    def narrowphase(self, pairs: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        """The narrowphase pass alone over (P, 2) collider pairs and the current records:
+        (contacts_raw, contact_valid), one per pair."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        major, minor = np.ascontiguousarray(pairs[:, 0]), np.ascontiguousarray(pairs[:, 1])
+        raw = np.zeros(len(pairs), dtype=abi.CONTACT)
+        valid = np.zeros(len(pairs), np.uint32)
+        rc = self.o.h.oracle_world_narrowphase(self.w, major.ctypes.data, minor.ctypes.data, len(pairs),
+                                               raw.ctypes.data, valid.ctypes.data)
+        if rc != 0:
+            raise ValueError("oracle_world_narrowphase refused the pairs")
+        return raw, valid
 
     def close(self) -> None:
         if self.w:
