@@ -1003,10 +1003,26 @@ __device__ inline void island_couple(const W& w, uint32_t a, uint32_t b) {
 
 __global__ void __launch_bounds__(kThreads) k_island_link(W w, float slop) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *w.ncontacts) return;
+    if (i >= min(*w.ncontacts, w.contact_cap)) return;
     const dyn_contact& c = w.contacts[i];
     if (c.point_count == 0u || c.sensor == 1u || !touches(c, slop)) return;
-    island_couple(w, w.owner[c.a], w.owner[c.b]);
+    // colliders or bodies outside the records (possible only for contacts handed to dyn_islands)
+    // link nothing
+    if (c.a >= w.k || c.b >= w.k) return;
+    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
+    if (fb >= w.n || sb >= w.n) return;
+    island_couple(w, fb, sb);
+}
+
+// dyn_islands' set-up: no wake or motion flags, the island counters at zero.
+__global__ void __launch_bounds__(kThreads) k_islands_stage_reset(W w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.n) w.wake[i] = 0u;
+    if (i < w.d) w.motion[i] = 0u;
+    if (i == 0u) {
+        *cw(w.counters, DYN_COUNTER_ISLANDS) = 0u;
+        *cw(w.counters, DYN_COUNTER_ISLAND_FAULTS) = 0u;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) k_island_link_resting(W w) {
@@ -2535,6 +2551,51 @@ int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io) {
     w.work_keys = l.work_keys;
     w.work_minor = l.work_minor;
     return narrowphase(w, static_cast<hipStream_t>(stream), true, l.sort, l.sort_bytes);
+}
+
+size_t dyn_islands_scratch_bytes(uint32_t bodies) {
+    const size_t words = bodies ? bodies : 1u;
+    return 3u * ((4u * words + 255u) & ~static_cast<size_t>(255u));
+}
+
+int dyn_islands(void* stream, const dyn_islands_io* io) {
+    using namespace dyn;
+    if (!io) return fail(DYN_E_INVALID, "dyn_islands: null io");
+    if (io->dynamic_count > io->bodies) return fail(DYN_E_INVALID, "dyn_islands: dynamic_count above bodies");
+    if (!io->d_counters || !io->d_scratch || !io->d_contact_count || (io->dynamic_count && !io->d_parents) ||
+        (io->contact_capacity && !io->d_contacts) || (io->colliders && !io->d_owners)) {
+        return fail(DYN_E_INVALID, "dyn_islands: null buffer");
+    }
+    if (io->scratch_bytes < dyn_islands_scratch_bytes(io->bodies)) {
+        return fail(DYN_E_CAPACITY, "dyn_islands: scratch smaller than dyn_islands_scratch_bytes");
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t part = (4u * static_cast<size_t>(io->bodies ? io->bodies : 1u) + 255u) & ~static_cast<size_t>(255u);
+    char* base = static_cast<char*>(io->d_scratch);
+    W w{};
+    w.n = io->bodies;
+    w.d = io->dynamic_count;
+    w.k = io->colliders;
+    w.owner = const_cast<uint32_t*>(io->d_owners);
+    w.contacts = const_cast<dyn_contact*>(io->d_contacts);
+    w.ncontacts = const_cast<uint32_t*>(io->d_contact_count);
+    w.contact_cap = io->contact_capacity;
+    w.parents = io->d_parents;
+    w.counters = io->d_counters;
+    w.wake = reinterpret_cast<uint32_t*>(base);
+    w.motion = reinterpret_cast<uint32_t*>(base + part);
+    w.istate = reinterpret_cast<uint32_t*>(base + 2u * part);
+    const uint32_t nb = blocks_for(w.n), db = blocks_for(w.d), cb = blocks_for(w.contact_cap);
+    LAUNCH_ON(st, k_islands_stage_reset, nb, w);
+    if (w.d == 0u) return DYN_OK;
+    LAUNCH_ON(st, k_island_init, db, w);
+    if (w.contact_cap) LAUNCH_ON(st, k_island_link, cb, w, io->slop);
+    uint32_t rounds = 0;
+    for (uint32_t v = w.d < 2u ? 2u : w.d; v > 1u; v >>= 1) ++rounds;
+    rounds += 1;  // ilog2(max(dynamic, 2)) + 1, crates/dynamis-rigid/src/lib.rs:79-81
+    for (uint32_t r = 0; r < rounds; ++r) LAUNCH_ON(st, k_island_flatten, db, w);
+    LAUNCH_ON(st, k_island_aggregate, db, w);
+    return DYN_OK;
 }
 
 int dyn_world_read_pairs(dyn_world* world, uint32_t* major, uint32_t* minor, uint32_t capacity,

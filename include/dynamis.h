@@ -186,6 +186,38 @@ size_t dyn_narrowphase_scratch_bytes(uint32_t pair_capacity);
  */
 int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io);
 
+/* ---- islands (crates/dynamis-rigid/src/islands.rs:237-258, crates/dynamis-rigid/shaders/
+ *      island_*.wgsl) ----------------------------------------------------------------------- */
+
+typedef struct dyn_islands_io {
+    const dyn_contact* d_contacts;     /* compacted contacts (device) */
+    const uint32_t* d_contact_count;   /* device word; min(*count, contact_capacity) are read */
+    uint32_t contact_capacity;
+    uint32_t colliders;
+    const uint32_t* d_owners;          /* K body rows of the colliders (device) */
+    uint32_t bodies;                   /* rows; rows [0, dynamic_count) are simulated */
+    uint32_t dynamic_count;
+    float slop;                        /* a contact links when a point is deeper than -slop */
+    uint32_t* d_parents;               /* out: dynamic_count words, each row's island root (the
+                                          island's minimum row) */
+    uint32_t* d_counters;              /* counter block; ISLANDS (islands among the simulated
+                                          rows) and ISLAND_FAULTS are written */
+    void* d_scratch;
+    size_t scratch_bytes;
+} dyn_islands_io;
+
+/* Scratch bytes dyn_islands needs. */
+size_t dyn_islands_scratch_bytes(uint32_t bodies);
+/*
+ * The island pass over contacts alone (islands.rs:237-258 without the resting-archive and
+ * constraint links, wake and sleep): island_init, island_link_contacts (a touching, non-sensor
+ * contact between two simulated rows joins their islands; the higher root is hooked under the
+ * lower), ilog2(max(dynamic, 2)) + 1 island_flatten rounds (lib.rs:79-81), island_count. Every
+ * row's parent is then its island's minimum row. Contacts naming colliders or bodies outside the
+ * records link nothing. The same kernels run inside dyn_world_step.
+ */
+int dyn_islands(void* stream, const dyn_islands_io* io);
+
 /* ---- host record encoding (crates/dynamis-model/src/config.rs:12-100 PhysicsConfig,
  *      body.rs:89 BodyDesc; crates/dynamis-abi/src/body.rs:96-139, collider.rs:47-102,
  *      param.rs:110-137; crates/dynamis-model/src/mass.rs) -------------------------------------- */
