@@ -53,6 +53,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_narrowphase": (c_int, [c_void_p, c_void_p]),
     "dyn_islands_scratch_bytes": (c_size_t, [c_u32]),
     "dyn_islands": (c_int, [c_void_p, c_void_p]),
+    "dyn_solve_scratch_bytes": (c_size_t, [c_u32, c_u32]),
+    "dyn_solve": (c_int, [c_void_p, c_void_p, ctypes.c_float]),
     "dyn_config_default": (None, [c_void_p]),
     "dyn_step_params_build": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p]),
     "dyn_inertia_inverse": (c_int, [c_void_p, c_void_p]),

@@ -1014,6 +1014,26 @@ __global__ void __launch_bounds__(kThreads) k_island_link(W w, float slop) {
     island_couple(w, fb, sb);
 }
 
+// dyn_solve's set-up: the solver's counters and the check word at zero, the contact count
+// clamped to the capacity (into the stage's own count word, words[0]).
+__global__ void __launch_bounds__(kThreads) k_solve_stage_reset(W w, const uint32_t* count, uint32_t* words) {
+    if (threadIdx.x != 0u) return;
+    *cw(w.counters, DYN_COUNTER_LIVE) = 0u;
+    *cw(w.counters, DYN_COUNTER_SOLVER_ROWS) = 0u;
+    *cw(w.counters, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL) = 0u;
+    *cw(w.counters, DYN_COUNTER_SOLVE_ANGULAR_RESIDUAL) = 0u;
+    words[0] = min(*count, w.contact_cap);
+    words[1] = 0u;
+}
+
+// dyn_solve's check: words[1] set when a contact names a collider or a body outside the records.
+__global__ void __launch_bounds__(kThreads) k_solve_stage_check(W w, uint32_t* words) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= words[0]) return;
+    const dyn_contact& c = w.contacts[i];
+    if (c.a >= w.k || c.b >= w.k || w.owner[c.a] >= w.n || w.owner[c.b] >= w.n) atomicOr(&words[1], 1u);
+}
+
 // dyn_islands' set-up: no wake or motion flags, the island counters at zero.
 __global__ void __launch_bounds__(kThreads) k_islands_stage_reset(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1912,29 +1932,33 @@ int alloc(dyn_world* world, T** ptr, size_t count) {
 
 uint32_t blocks_for(uint32_t n) { return dyn::ceil_div(n ? n : 1u, dyn::kThreads); }
 
-int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_t limit,
+int scan(const dyn::W& w, hipStream_t st, const uint32_t* flags, const uint32_t* count, uint32_t limit,
          uint32_t* dest, uint32_t* total, bool counts = false) {
     using namespace dyn;
-    hipStream_t st = world->stream;
     const uint32_t blocks = ceil_div(limit ? limit : 1u, kScanBlock);
     {
         ProfileScope scope("k_scan_count", st);
-        if (counts) k_scan_count<true><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks);
-        else k_scan_count<false><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks);
+        if (counts) k_scan_count<true><<<blocks, kThreads, 0, st>>>(flags, count, limit, w.scan_blocks);
+        else k_scan_count<false><<<blocks, kThreads, 0, st>>>(flags, count, limit, w.scan_blocks);
     }
     DYN_LAUNCHED("k_scan_count");
     {
         ProfileScope scope("k_scan_blocks", st);
-        k_scan_blocks<<<1, 1024, 0, st>>>(world->w.scan_blocks, blocks, total);
+        k_scan_blocks<<<1, 1024, 0, st>>>(w.scan_blocks, blocks, total);
     }
     DYN_LAUNCHED("k_scan_blocks");
     {
         ProfileScope scope("k_scan_dest", st);
-        if (counts) k_scan_dest<true><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks, dest);
-        else k_scan_dest<false><<<blocks, kThreads, 0, st>>>(flags, count, limit, world->w.scan_blocks, dest);
+        if (counts) k_scan_dest<true><<<blocks, kThreads, 0, st>>>(flags, count, limit, w.scan_blocks, dest);
+        else k_scan_dest<false><<<blocks, kThreads, 0, st>>>(flags, count, limit, w.scan_blocks, dest);
     }
     DYN_LAUNCHED("k_scan_dest");
     return DYN_OK;
+}
+
+int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_t limit, uint32_t* dest,
+         uint32_t* total, bool counts = false) {
+    return scan(world->w, world->stream, flags, count, limit, dest, total, counts);
 }
 
 // The worklist of convex pairs of one mode, grouped by kind class: block histograms, their
@@ -2003,6 +2027,49 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
 }
 
 int world_step(dyn_world* world, float dt);
+
+// SolverReset (solver.rs:89-101) and the block set-up of SolveSubsteps (solver.rs:310): block
+// counts, the contact blocks, the rows' block lists.
+int solver_prepare(const dyn::W& w, hipStream_t st) {
+    using namespace dyn;
+    const uint32_t nb = blocks_for(w.n), cb = blocks_for(w.contact_cap);
+    LAUNCH_ON(st, k_solver_reset, nb, w);
+    LAUNCH_ON(st, k_blocks, cb, w);
+    LAUNCH_ON(st, k_rows_count, nb, w);
+    DYN_TRY(scan(w, st, w.list_counts, nullptr, w.n, w.row_start, &w.scan_total[3], true));
+    LAUNCH_ON(st, k_fill_lists, cb, w);
+    return DYN_OK;
+}
+
+// The substeps of SolveSubsteps (solver.rs:311-332), then the accumulated impulses back into the
+// contacts.
+int solver_substeps(const dyn::W& w, hipStream_t st) {
+    using namespace dyn;
+    const uint32_t nb = blocks_for(w.n), db = blocks_for(w.d), cb = blocks_for(w.contact_cap);
+    for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
+        LAUNCH_ON(st, k_integrate, db, w);
+        LAUNCH_ON(st, k_solver_bodies, nb, w);
+        LAUNCH_ON(st, k_solve_prep, cb, w, sub == 0 ? 1u : 0u);
+        if (sub == 0) {
+            LAUNCH_ON(st, k_warm, cb, w);
+            LAUNCH_ON(st, k_apply, nb, w);
+        }
+        for (uint32_t it = 0; it < w.p.solve_iterations; ++it) {
+            // solver_block_solve.wgsl:85-87: the residual round is the one where the round
+            // counter (reset by integrate, bumped by every apply) equals the iteration count,
+            // which only the warm-started first substep reaches.
+            LAUNCH_ON(st, k_solve, cb, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
+            LAUNCH_ON(st, k_apply, nb, w);
+        }
+        LAUNCH_ON(st, k_advance, db, w);
+        for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
+            LAUNCH_ON(st, k_position_block, cb, w);
+            LAUNCH_ON(st, k_position_apply, nb, w);
+        }
+    }
+    LAUNCH_ON(st, k_impulses_store, cb, w);
+    return DYN_OK;
+}
 
 void drop_graph(dyn_world* world) {
     for (int p = 0; p < 2; ++p) {
@@ -2097,37 +2164,12 @@ int world_step(dyn_world* world, float dt) {
     for (uint32_t r = 0; r < rounds; ++r) LAUNCH(k_island_flatten, db, w);
     LAUNCH(k_island_aggregate, db, w);
     LAUNCH(k_island_wake, db, w);
-    LAUNCH(k_solver_reset, nb, w);
-    LAUNCH(k_blocks, cb, w);
-    LAUNCH(k_rows_count, nb, w);
-    DYN_TRY(scan(world, w.list_counts, nullptr, w.n, w.row_start, &w.scan_total[3], true));
-    LAUNCH(k_fill_lists, cb, w);
+    DYN_TRY(solver_prepare(w, st));
     if (world->tape_states) {
         LAUNCH(k_tape_record, blocks_for(std::max(w.n, w.contact_cap)), w, world->tape_states, world->tape_contacts,
                world->tape_count, world->tape_error);
     }
-    for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
-        LAUNCH(k_integrate, db, w);
-        LAUNCH(k_solver_bodies, nb, w);
-        LAUNCH(k_solve_prep, cb, w, sub == 0 ? 1u : 0u);
-        if (sub == 0) {
-            LAUNCH(k_warm, cb, w);
-            LAUNCH(k_apply, nb, w);
-        }
-        for (uint32_t it = 0; it < w.p.solve_iterations; ++it) {
-            // solver_block_solve.wgsl:85-87: the residual round is the one where the round
-            // counter (reset by integrate, bumped by every apply) equals the iteration count,
-            // which only the warm-started first substep reaches.
-            LAUNCH(k_solve, cb, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
-            LAUNCH(k_apply, nb, w);
-        }
-        LAUNCH(k_advance, db, w);
-        for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
-            LAUNCH(k_position_block, cb, w);
-            LAUNCH(k_position_apply, nb, w);
-        }
-    }
-    LAUNCH(k_impulses_store, cb, w);
+    DYN_TRY(solver_substeps(w, st));
     LAUNCH(k_sleep, db, w);
     LAUNCH(k_thaw, cb, w);
     LAUNCH(k_archive, cb, w, world->freeze_flags);
@@ -2551,6 +2593,93 @@ int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io) {
     w.work_keys = l.work_keys;
     w.work_minor = l.work_minor;
     return narrowphase(w, static_cast<hipStream_t>(stream), true, l.sort, l.sort_bytes);
+}
+
+namespace {
+
+// dyn_solve's scratch: the solver's per-block and per-row buffers.
+struct SolveLayout {
+    dyn::W parts;  // only the scratch pointers are set
+    size_t bytes;
+};
+
+SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base) {
+    using namespace dyn;
+    SolveLayout l{};
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t here = at;
+        at = (at + (bytes ? bytes : 1u) + 255u) & ~static_cast<size_t>(255u);
+        return static_cast<char*>(base) + here;
+    };
+    const size_t n = bodies ? bodies : 1u, c = contact_capacity ? contact_capacity : 1u;
+    W& w = l.parts;
+    w.block_counts = reinterpret_cast<uint32_t*>(take(4u * n));
+    w.blocks = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
+    w.targets = reinterpret_cast<float*>(take(4u * 4u * c));
+    w.prows = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
+    w.srows = reinterpret_cast<float4*>(take(sizeof(float4) * kSolveRow * c));
+    w.sbodies = reinterpret_cast<float4*>(take(sizeof(float4) * kSolverBody * n));
+    w.simp = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
+    w.resolution = reinterpret_cast<float*>(take(4u * 4u * n));
+    w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
+    w.list_counts = reinterpret_cast<uint32_t*>(take(4u * n));
+    w.row_start = reinterpret_cast<uint32_t*>(take(4u * n));
+    w.adj = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
+    w.vwords = reinterpret_cast<uint32_t*>(take(4u * 12u * c));
+    w.pwords = reinterpret_cast<uint4*>(take(sizeof(uint4) * 2u * c));
+    w.contrib = reinterpret_cast<uint32_t*>(take(4u * n));
+    const size_t limit = n > c ? n : c;
+    w.scan_blocks = reinterpret_cast<uint32_t*>(take(4u * (ceil_div(static_cast<uint32_t>(limit), kScanBlock) + 1u)));
+    w.scan_total = reinterpret_cast<uint32_t*>(take(4u * 4u));
+    w.ncontacts = reinterpret_cast<uint32_t*>(take(4u * 2u));  // the clamped count, the check word
+    l.bytes = at;
+    return l;
+}
+
+}  // namespace
+
+size_t dyn_solve_scratch_bytes(uint32_t bodies, uint32_t contact_capacity) {
+    return solve_layout(bodies, contact_capacity, nullptr).bytes;
+}
+
+int dyn_solve(void* stream, const dyn_solve_io* io, float dt) {
+    using namespace dyn;
+    if (!io || !io->params) return fail(DYN_E_INVALID, "dyn_solve: null io or params");
+    if (!(dt > 0.0f)) return fail(DYN_E_INVALID, "dyn_solve: dt must be positive");
+    if (io->params->substeps == 0u) return fail(DYN_E_INVALID, "dyn_solve: substeps must be at least 1");
+    if (io->dynamic_count > io->bodies) return fail(DYN_E_INVALID, "dyn_solve: dynamic_count above bodies");
+    if (!io->d_counters || !io->d_scratch || !io->d_contact_count || (io->bodies && (!io->d_states || !io->d_descs)) ||
+        (io->contact_capacity && !io->d_contacts) || (io->colliders && !io->d_owners)) {
+        return fail(DYN_E_INVALID, "dyn_solve: null buffer");
+    }
+    SolveLayout l = solve_layout(io->bodies, io->contact_capacity, io->d_scratch);
+    if (io->scratch_bytes < l.bytes) {
+        return fail(DYN_E_CAPACITY, "dyn_solve: scratch smaller than dyn_solve_scratch_bytes");
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    W w = l.parts;
+    w.p = *io->params;
+    w.p.dt = dt;
+    w.p.substep_dt = dt / static_cast<float>(w.p.substeps);
+    w.n = io->bodies;
+    w.d = io->dynamic_count;
+    w.k = io->colliders;
+    w.st = io->d_states;
+    w.de = const_cast<dyn_body_desc*>(io->d_descs);
+    w.owner = const_cast<uint32_t*>(io->d_owners);
+    w.contacts = io->d_contacts;
+    w.contact_cap = io->contact_capacity;
+    w.counters = io->d_counters;
+    uint32_t* words = w.ncontacts;  // scratch: [0] the count the kernels read, [1] the check
+    LAUNCH_ON(st, k_solve_stage_reset, 1u, w, io->d_contact_count, words);
+    if (w.contact_cap) LAUNCH_ON(st, k_solve_stage_check, blocks_for(w.contact_cap), w, words);
+    uint32_t bad = 0;
+    DYN_HIP(hipMemcpyAsync(&bad, words + 1, 4u, hipMemcpyDeviceToHost, st));
+    DYN_HIP(hipStreamSynchronize(st));
+    if (bad) return fail(DYN_E_INVALID, "dyn_solve: a contact names a collider or body outside the records");
+    DYN_TRY(solver_prepare(w, st));
+    return solver_substeps(w, st);
 }
 
 size_t dyn_islands_scratch_bytes(uint32_t bodies) {

@@ -218,6 +218,42 @@ size_t dyn_islands_scratch_bytes(uint32_t bodies);
  */
 int dyn_islands(void* stream, const dyn_islands_io* io);
 
+/* ---- solver (crates/dynamis-rigid/src/solver.rs, crates/dynamis-rigid/shaders/solver_*.wgsl,
+ *      substep_*.wgsl, position_*.wgsl) --------------------------------------------------------- */
+
+typedef struct dyn_solve_io {
+    const dyn_step_params* params;     /* host; substeps, iterations, gravity, damping, clamps,
+                                          restitution threshold are read (dt is an argument) */
+    dyn_body_state* d_states;          /* bodies by row, read and written (device) */
+    const dyn_body_desc* d_descs;
+    uint32_t bodies;
+    uint32_t dynamic_count;            /* rows [0, dynamic_count) are simulated */
+    uint32_t colliders;
+    uint32_t contact_capacity;
+    const uint32_t* d_owners;          /* K body rows of the colliders (device) */
+    dyn_contact* d_contacts;           /* compacted contacts (device): their accumulated impulses
+                                          warm-start the first substep and are replaced by the
+                                          new ones */
+    const uint32_t* d_contact_count;   /* device word; min(*count, contact_capacity) are solved */
+    uint32_t* d_counters;              /* counter block; LIVE, SOLVER_ROWS and the residuals are
+                                          written */
+    void* d_scratch;
+    size_t scratch_bytes;
+} dyn_solve_io;
+
+/* Scratch bytes dyn_solve needs. */
+size_t dyn_solve_scratch_bytes(uint32_t bodies, uint32_t contact_capacity);
+/*
+ * The solver alone: SolverReset (solver.rs:89-101) and SolveSubsteps (solver.rs:304-334) —
+ * the contact blocks and their mass splitting, then per substep integration, the warm start
+ * (first substep), the Jacobi velocity iterations, the pose advance and the position
+ * iterations — and the new accumulated impulses into the contacts. Rows that are awake move;
+ * sleeping rows stay. The contacts are checked first (one synchronization of the stream): a
+ * contact naming a collider or body outside the records is refused with DYN_E_INVALID and
+ * nothing is solved. The same kernels run inside dyn_world_step.
+ */
+int dyn_solve(void* stream, const dyn_solve_io* io, float dt);
+
 /* ---- host record encoding (crates/dynamis-model/src/config.rs:12-100 PhysicsConfig,
  *      body.rs:89 BodyDesc; crates/dynamis-abi/src/body.rs:96-139, collider.rs:47-102,
  *      param.rs:110-137; crates/dynamis-model/src/mass.rs) -------------------------------------- */

@@ -41,6 +41,10 @@ uint32_t oracle_world_tape(const oracle_world* w, dyn_body_state* start, dyn_con
 uint32_t oracle_world_contacts(const oracle_world* w, dyn_contact* out, uint32_t capacity);
 void oracle_world_counters(const oracle_world* w, uint32_t* out);
 /* Sorted candidate pairs of the last step; returns the count. */
+/* The solver alone over the given contacts (read, then written back with the new impulses) and
+ * the current records; -1 when count exceeds the contact capacity or a contact names a collider
+ * or body outside the records. */
+int oracle_world_solve(oracle_world* w, dyn_contact* contacts, uint32_t count, float dt);
 /* The narrowphase alone over the given pairs and the current records (contacts_raw,
  * contact_valid); -1 when count exceeds the pair capacity or a pair names no collider. */
 int oracle_world_narrowphase(oracle_world* w, const uint32_t* major, const uint32_t* minor, uint32_t count,

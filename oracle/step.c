@@ -2674,6 +2674,54 @@ void oracle_world_counters(const oracle_world* w, uint32_t* out) {
     memcpy(out, w->counters, sizeof w->counters);
 }
 
+/* The solver alone (solver.rs:89-101 SolverReset, solver.rs:304-334 SolveSubsteps) over the
+ * given contacts and the world's current records, with the live rows of the step (the simulated
+ * rows that are awake): the same statements as the step between its island pass and its sleep
+ * pass. The contacts are read (their accumulated impulses warm-start the first substep) and
+ * written back with the new impulses; count <= the world's contact capacity. */
+int oracle_world_solve(oracle_world* w, dyn_contact* contacts, uint32_t count, float dt) {
+    if (count > w->contact_cap) return -1;
+    for (uint32_t i = 0; i < count; i++) {
+        if (contacts[i].a >= w->k || contacts[i].b >= w->k) return -1;
+        if (w->owner[contacts[i].a] >= w->n || w->owner[contacts[i].b] >= w->n) return -1;
+    }
+    w->p.dt = dt;
+    w->p.substep_dt = dt / (float)w->p.substeps;
+    memcpy(w->contacts, contacts, (size_t)count * sizeof *contacts);
+    w->ncontacts = count;
+    w->nlive = 0;
+    for (uint32_t i = 0; i < w->d; i++) {
+        if (body_is_active(&w->st[i], &w->de[i])) w->live[w->nlive++] = i;
+    }
+    *ctr(w, DYN_COUNTER_LIVE) = w->nlive;
+    for (uint32_t i = 0; i < w->n; i++) {
+        w->block_counts[i] = 0;
+        memset(&w->resolution[4 * i], 0, 16);
+    }
+    *ctr(w, DYN_COUNTER_BLOCKS) = w->ncontacts;
+    w->nrows = 0;
+    solver_blocks(w);
+    for (uint32_t sub = 0; sub < w->p.substeps; sub++) {
+        integrate(w);
+        if (sub == 0) {
+            for (uint32_t ci = 0; ci < w->ncontacts; ci++) warm_block(w, ci);
+            block_apply(w);
+        }
+        for (uint32_t it = 0; it < w->p.solve_iterations; it++) {
+            int residual = sub == 0 && it + 1u == w->p.solve_iterations;
+            for (uint32_t ci = 0; ci < w->ncontacts; ci++) solve_block(w, ci, residual);
+            block_apply(w);
+        }
+        advance(w);
+        for (uint32_t it = 0; it < w->p.position_iterations; it++) {
+            position_block(w);
+            position_apply(w);
+        }
+    }
+    memcpy(contacts, w->contacts, (size_t)count * sizeof *contacts);
+    return 0;
+}
+
 /* The narrowphase pass alone (narrowphase.wgsl:518-685, dispatched at narrowphase.rs:126) over
  * the given candidate pairs and the world's current records: contacts_raw and contact_valid for
  * pairs [0, count), count <= the world's pair capacity. The world's own pair list is replaced. */

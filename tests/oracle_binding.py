@@ -64,6 +64,10 @@ class Oracle:
         h.oracle_world_pairs.argtypes = [_p, _p, _p, _u32]
         h.oracle_world_narrowphase.restype = ctypes.c_int
         h.oracle_world_narrowphase.argtypes = [_p, _p, _p, _u32, _p, _p]
+        h.oracle_world_solve.restype = ctypes.c_int
+        h.oracle_world_solve.argtypes = [_p, _p, _u32, ctypes.c_float]
+        h.oracle_world_tape.restype = _u32
+        h.oracle_world_tape.argtypes = [_p, _p, _p, _p, _u32]
         h.oracle_world_shapes.restype = ctypes.c_int
         h.oracle_world_shapes.argtypes = [_p, _p, _u32, _p, _u32]
         h.oracle_convex_query.restype = ctypes.c_int
@@ -213,6 +217,25 @@ class OracleWorld:
         if rc != 0:
             raise ValueError("oracle_world_narrowphase refused the pairs")
         return raw, valid
+
+    def solve(self, contacts: np.ndarray, dt: float) -> np.ndarray:
+        """The solver alone over the contacts and the current records; returns the contacts with
+        their new accumulated impulses (the states move in place: read them with states())."""
+        contacts = np.ascontiguousarray(contacts, dtype=abi.CONTACT).copy()
+        rc = self.o.h.oracle_world_solve(self.w, contacts.ctypes.data, len(contacts), ctypes.c_float(dt))
+        if rc != 0:
+            raise ValueError("oracle_world_solve refused the contacts")
+        return contacts
+
+    def tape(self) -> tuple[np.ndarray, np.ndarray]:
+        """The last step's states after its input clearing and its contacts after the warm-start
+        relay and contact_begin, as the solver received them."""
+        start = np.zeros(self.n, abi.BODY_STATE)
+        pre = np.zeros(self.capacity, abi.CONTACT)
+        src = np.zeros((self.capacity, 4), np.uint32)
+        count = self.o.h.oracle_world_tape(self.w, start.ctypes.data, pre.ctypes.data, src.ctypes.data,
+                                           self.capacity)
+        return start, pre[:min(count, self.capacity)].copy()
 
     def close(self) -> None:
         if self.w:
