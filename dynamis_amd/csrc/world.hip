@@ -111,12 +111,16 @@ struct W {
     // Solver deltas without atomics: every block writes its fixed-point words to its own slots
     // and every row sums the words of the blocks on its list. The words are integers, so the
     // sum does not depend on the order of the list.
-    uint32_t* slots;        // per block: its place in the first and the second row's list
+    // A row's list is a run of entries from row_start, one per block on it; each block writes
+    // its words for a side straight into that side's entry, so a row reads its run in order.
+    uint32_t* slots;        // per block and side: k_blocks' place in the row's list, then
+                            // (k_fill_lists) the entry's index; kNoSlot for an inert side
     uint32_t* list_counts;  // per row: blocks on its list (0 for inert rows)
-    uint32_t* row_start;    // per row: first entry of its list in adj
-    uint32_t* adj;          // list entries: block << 1 | side
-    uint32_t* vwords;       // per block: 12 velocity words (first lin, ang; second lin, ang)
-    uint4* pwords;          // per block: {contributed, first lin}, {second lin, 0}
+    uint32_t* row_start;    // per row: its first list entry
+    uint32_t* vwords;       // per list entry: 6 velocity words (lin, ang) its block last sent, in
+                            // kVelocityEntry-word entries
+    uint4* pwords;          // per list entry: {contributed, lin words} of its block's last
+                            // position pass
     uint32_t* contrib;      // inert rows: marked when a block of theirs contributed
     uint32_t* scan_blocks;   // per-block sums of compaction scans
     uint32_t* scan_total;    // two words: totals of the last scans
@@ -1014,10 +1018,14 @@ __global__ void __launch_bounds__(kThreads) k_island_link(W w, float slop) {
     island_couple(w, fb, sb);
 }
 
-// dyn_solve's set-up: the solver's counters and the check word at zero, the contact count
-// clamped to the capacity (into the stage's own count word, words[0]).
+// dyn_solve's set-up: the inert rows' contribution marks at zero (a world keeps them zero
+// between position passes: zeroed at allocation, cleared by k_position_apply), the solver's
+// counters and the check word at zero, the contact count clamped to the capacity (into the
+// stage's own count word, words[0]).
 __global__ void __launch_bounds__(kThreads) k_solve_stage_reset(W w, const uint32_t* count, uint32_t* words) {
-    if (threadIdx.x != 0u) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.n) w.contrib[i] = 0u;
+    if (i != 0u) return;
     *cw(w.counters, DYN_COUNTER_LIVE) = 0u;
     *cw(w.counters, DYN_COUNTER_SOLVER_ROWS) = 0u;
     *cw(w.counters, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL) = 0u;
@@ -1186,22 +1194,27 @@ __global__ void __launch_bounds__(kThreads) k_rows_count(W w) {
     if (lane_id() == 0 && claimed) atomicAdd(cw(w.counters, DYN_COUNTER_SOLVER_ROWS), claimed);
 }
 
-// The list entries of every block's non-inert sides, at the row's offset plus the block's place.
+// The list entry of every block's non-inert sides: the row's first entry plus the block's place.
 __global__ void __launch_bounds__(kThreads) k_fill_lists(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *w.ncontacts) return;
-    const uint2 slot = reinterpret_cast<const uint2*>(w.slots)[i];
-    if (slot.x != kNoSlot) w.adj[w.row_start[w.blocks[2 * i]] + slot.x] = i << 1;
-    if (slot.y != kNoSlot) w.adj[w.row_start[w.blocks[2 * i + 1]] + slot.y] = (i << 1) | 1u;
+    uint2& slot = reinterpret_cast<uint2*>(w.slots)[i];
+    const uint2 place = slot;
+    slot = make_uint2(place.x != kNoSlot ? w.row_start[w.blocks[2 * i]] + place.x : kNoSlot,
+                      place.y != kNoSlot ? w.row_start[w.blocks[2 * i + 1]] + place.y : kNoSlot);
 }
 
 // One side's six velocity words of block ci.
+constexpr uint32_t kVelocityEntry = 8;  // words per velocity list entry: lin, ang, 2 spare (aligned)
+
 __device__ __forceinline__ void store_words(const W& w, uint32_t ci, uint32_t side, v3 lin, v3 ang) {
-    uint2* d = reinterpret_cast<uint2*>(w.vwords + ci * 12u + side * 6u);
+    const uint32_t entry = w.slots[2u * ci + side];
+    if (entry == kNoSlot) return;
+    uint32_t* d = w.vwords + kVelocityEntry * entry;
     const float scale = DYN_SOLVER_VELOCITY_SCALE;
-    d[0] = make_uint2(solver_word(lin.x, scale), solver_word(lin.y, scale));
-    d[1] = make_uint2(solver_word(lin.z, scale), solver_word(ang.x, scale));
-    d[2] = make_uint2(solver_word(ang.y, scale), solver_word(ang.z, scale));
+    *reinterpret_cast<uint4*>(d) = make_uint4(solver_word(lin.x, scale), solver_word(lin.y, scale),
+                                              solver_word(lin.z, scale), solver_word(ang.x, scale));
+    *reinterpret_cast<uint2*>(d + 4) = make_uint2(solver_word(ang.y, scale), solver_word(ang.z, scale));
 }
 
 __device__ __forceinline__ void make_tangents(v3 n, v3* t1, v3* t2) {
@@ -1547,35 +1560,43 @@ __global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
 }
 
 // solver_block_apply.wgsl over every claimed row
+// List entries whose loads k_apply and k_position_apply keep in flight together: a row of up to
+// kApplyBatch blocks reads its run of entries in one round (integer sums: any order gives the
+// same words).
+constexpr uint32_t kApplyBatch = 8;
+
 __global__ void __launch_bounds__(kThreads) k_apply(W w) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= w.n || w.block_counts[row] == 0u) return;
+    if (row >= w.n) return;
+    // list_counts is 0 for rows without blocks and for inert rows, which receive no words
     const uint32_t count = w.list_counts[row];
-    if (count == 0u) return;  // inert rows receive no words
-    const uint32_t* list = w.adj + w.row_start[row];
+    const uint32_t start = w.row_start[row];
+    if (count == 0u) return;
+    dyn_body_state& s = w.st[row];
+    const v3 v0 = ld3(s.velocity), av0 = ld3(s.angular_velocity);  // in flight with the list
+    const uint32_t* run = w.vwords + kVelocityEntry * start;
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    // four entries' loads in flight at a time (integer sums: any order gives the same words)
-    for (uint32_t e = 0; e < count; e += 4) {
-        uint2 a[4], b[4], c[4];
+    for (uint32_t e = 0; e < count; e += kApplyBatch) {
+        uint4 a[kApplyBatch];
+        uint2 b[kApplyBatch];
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-            a[j] = b[j] = c[j] = make_uint2(0u, 0u);
+        for (uint32_t j = 0; j < kApplyBatch; ++j) {
+            a[j] = make_uint4(0u, 0u, 0u, 0u);
+            b[j] = make_uint2(0u, 0u);
             if (e + j < count) {
-                const uint32_t entry = list[e + j];
-                const uint2* d = reinterpret_cast<const uint2*>(w.vwords + (entry >> 1) * 12u + (entry & 1u) * 6u);
-                a[j] = d[0];
-                b[j] = d[1];
-                c[j] = d[2];
+                const uint32_t* d = run + kVelocityEntry * (e + j);
+                a[j] = *reinterpret_cast<const uint4*>(d);
+                b[j] = *reinterpret_cast<const uint2*>(d + 4);
             }
         }
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
+        for (uint32_t j = 0; j < kApplyBatch; ++j) {
             s0 += a[j].x;
             s1 += a[j].y;
-            s2 += b[j].x;
-            s3 += b[j].y;
-            s4 += c[j].x;
-            s5 += c[j].y;
+            s2 += a[j].z;
+            s3 += a[j].w;
+            s4 += b[j].x;
+            s5 += b[j].y;
         }
     }
     if ((s0 | s1 | s2 | s3 | s4 | s5) == 0u) return;
@@ -1584,8 +1605,7 @@ __global__ void __launch_bounds__(kThreads) k_apply(W w) {
     const v3 ang = mk(solver_value(s3, DYN_SOLVER_VELOCITY_SCALE), solver_value(s4, DYN_SOLVER_VELOCITY_SCALE),
                       solver_value(s5, DYN_SOLVER_VELOCITY_SCALE));
     if (zero3(lin) && zero3(ang)) return;
-    dyn_body_state& s = w.st[row];
-    const v3 v = add(ld3(s.velocity), lin), av = add(ld3(s.angular_velocity), ang);
+    const v3 v = add(v0, lin), av = add(av0, ang);
     st3(s.velocity, v);
     st3(s.angular_velocity, av);
     float4* p = w.sbodies + kSolverBody * row;  // the solver body's copy
@@ -1650,16 +1670,10 @@ __global__ void __launch_bounds__(kThreads) k_advance(W w) {
 }
 
 // position_correction.wgsl:5-46, position_block.wgsl, from the block's position row
-__global__ void __launch_bounds__(kThreads) k_position_block(W w) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= *w.ncontacts) return;
-    const float4* prow = w.prows + 3u * ci;
-    const float4 h0 = prow[0];
-    const uint32_t flags = __float_as_uint(h0.z);
-    const uint32_t count = flags & 7u;
-    uint4* out = w.pwords + 2u * ci;
-    out[0] = make_uint4(0u, 0u, 0u, 0u);  // no contribution unless the end below is reached
-    if (count == 0u) return;  // not resolved, or relaxed by compliance
+// position_correction.wgsl:5-46 for one block of `count` points: false when it contributes
+// nothing, else its two sides' corrections (inert rows only get their "contributed" mark).
+__device__ __forceinline__ bool position_correction(const W& w, float4 h0, const float4* prow, uint32_t flags,
+                                                    uint32_t count, v3* la_out, v3* lb_out) {
     const uint32_t fr = __float_as_uint(h0.x), sr = __float_as_uint(h0.y);
     const bool fi = (flags & 8u) != 0u, si = (flags & 16u) != 0u;
     const float imf = h0.w;
@@ -1683,23 +1697,50 @@ __global__ void __launch_bounds__(kThreads) k_position_block(W w) {
         total = add(total, mul(normal, w.p.relaxation * error / k));
         ++contributing;
     }
-    if (contributing == 0u) return;
+    if (contributing == 0u) return false;
     const v3 correction = dv(total, static_cast<float>(contributing));
     const v3 la = mul(neg(correction), imf), lb = mul(correction, ims);
-    if (!(dot3(la, la) > 0.0f || dot3(lb, lb) > 0.0f)) return;
+    if (!(dot3(la, la) > 0.0f || dot3(lb, lb) > 0.0f)) return false;
     // Inert rows get exactly zero words; only their "contributed" mark matters.
     if (fi) w.contrib[fr] = 1u;
     if (si) w.contrib[sr] = 1u;
-    const float scale = DYN_SOLVER_POSITION_SCALE;
-    out[1] = make_uint4(solver_word(lb.x, scale), solver_word(lb.y, scale), solver_word(lb.z, scale), 0u);
-    out[0] = make_uint4(1u, solver_word(la.x, scale), solver_word(la.y, scale), solver_word(la.z, scale));
+    *la_out = la;
+    *lb_out = lb;
+    return true;
+}
+
+__global__ void __launch_bounds__(kThreads) k_position_block(W w) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= *w.ncontacts) return;
+    const float4* prow = w.prows + 3u * ci;
+    const float4 h0 = prow[0];
+    const uint32_t flags = __float_as_uint(h0.z);
+    const uint32_t count = flags & 7u;  // 0: not resolved, or relaxed by compliance
+    const uint2 entries = reinterpret_cast<const uint2*>(w.slots)[ci];
+    v3 la, lb;
+    // every entry of the block is written once per pass: "no contribution" unless corrected
+    uint4 wa = make_uint4(0u, 0u, 0u, 0u), wb = wa;
+    if (count != 0u && position_correction(w, h0, prow, flags, count, &la, &lb)) {
+        const float scale = DYN_SOLVER_POSITION_SCALE;
+        wa = make_uint4(1u, solver_word(la.x, scale), solver_word(la.y, scale), solver_word(la.z, scale));
+        wb = make_uint4(1u, solver_word(lb.x, scale), solver_word(lb.y, scale), solver_word(lb.z, scale));
+    }
+    if (entries.x != kNoSlot) w.pwords[entries.x] = wa;
+    if (entries.y != kNoSlot) w.pwords[entries.y] = wb;
 }
 
 // position_apply.wgsl over every claimed row
 __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= w.n || w.block_counts[row] == 0u) return;
+    if (row >= w.n) return;
+    // list_counts is 0 for rows without blocks and for inert rows; contrib, zero outside a
+    // position pass, is marked only for inert rows with a contributing block
     const uint32_t count = w.list_counts[row];
+    const uint32_t start = w.row_start[row];
+    dyn_body_state& s = w.st[row];
+    float4& r = reinterpret_cast<float4*>(w.resolution)[row];
+    const v3 p0 = ld3(s.position);  // in flight with the list
+    const float4 q0 = ld4(s.orientation), r0 = r;
     uint32_t contributing = 0;
     uint4 lo = make_uint4(0u, 0u, 0u, 0u);  // linear words, then the (zero) angular x
     uint2 hi = make_uint2(0u, 0u);          // angular y, z: the position blocks send none
@@ -1708,31 +1749,18 @@ __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
         if (contributing == 0u) return;
         w.contrib[row] = 0u;
     } else {
-        const uint32_t* list = w.adj + w.row_start[row];
-        // four entries' loads in flight at a time (integer sums: any order gives the same words)
-        for (uint32_t e = 0; e < count; e += 4) {
-            uint4 word[4];
-            bool take[4];
+        const uint4* run = w.pwords + start;
+        for (uint32_t e = 0; e < count; e += kApplyBatch) {
+            uint4 word[kApplyBatch];
 #pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) {
-                word[j] = make_uint4(0u, 0u, 0u, 0u);
-                take[j] = false;
-                if (e + j < count) {
-                    const uint32_t entry = list[e + j];
-                    const uint4* d = w.pwords + 2u * (entry >> 1);
-                    const uint4 head = d[0];
-                    take[j] = head.x != 0u;
-                    // the block's words for this side: first side after the flag, second in the tail
-                    word[j] = (entry & 1u) == 0u ? make_uint4(head.y, head.z, head.w, 0u) : d[1];
-                }
-            }
+            for (uint32_t j = 0; j < kApplyBatch; ++j) word[j] = e + j < count ? run[e + j] : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) {
-                if (!take[j]) continue;
+            for (uint32_t j = 0; j < kApplyBatch; ++j) {
+                if (word[j].x == 0u) continue;  // the block did not contribute
                 ++contributing;
-                lo.x += word[j].x;
-                lo.y += word[j].y;
-                lo.z += word[j].z;
+                lo.x += word[j].y;
+                lo.y += word[j].z;
+                lo.z += word[j].w;
             }
         }
         if (contributing == 0u) return;
@@ -1743,11 +1771,9 @@ __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
                       solver_value(hi.y, DYN_SOLVER_POSITION_SCALE));
     const float cf = static_cast<float>(contributing);
     const v3 al = dv(lin, cf), aa = dv(ang, cf);
-    dyn_body_state& s = w.st[row];
-    st3(s.position, add(ld3(s.position), al));
-    st4(s.orientation, qnorm(qmul(make_float4(aa.x * 0.5f, aa.y * 0.5f, aa.z * 0.5f, 1.0f), ld4(s.orientation))));
-    float4& r = reinterpret_cast<float4*>(w.resolution)[row];
-    const v3 nr = add(mk(r.x, r.y, r.z), al);
+    st3(s.position, add(p0, al));
+    st4(s.orientation, qnorm(qmul(make_float4(aa.x * 0.5f, aa.y * 0.5f, aa.z * 0.5f, 1.0f), q0)));
+    const v3 nr = add(mk(r0.x, r0.y, r0.z), al);
     r = make_float4(nr.x, nr.y, nr.z, 0.0f);
 }
 
@@ -2313,8 +2339,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.slots, 2u * static_cast<size_t>(contact_capacity));
     A(w.list_counts, body_count);
     A(w.row_start, body_count);
-    A(w.adj, 2u * static_cast<size_t>(contact_capacity));
-    A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
+    A(w.vwords, 2u * kVelocityEntry * static_cast<size_t>(contact_capacity));
     A(w.pwords, 2u * static_cast<size_t>(contact_capacity));
     A(w.contrib, body_count);
     uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
@@ -2625,8 +2650,7 @@ SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base)
     w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
     w.list_counts = reinterpret_cast<uint32_t*>(take(4u * n));
     w.row_start = reinterpret_cast<uint32_t*>(take(4u * n));
-    w.adj = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
-    w.vwords = reinterpret_cast<uint32_t*>(take(4u * 12u * c));
+    w.vwords = reinterpret_cast<uint32_t*>(take(4u * 2u * kVelocityEntry * c));
     w.pwords = reinterpret_cast<uint4*>(take(sizeof(uint4) * 2u * c));
     w.contrib = reinterpret_cast<uint32_t*>(take(4u * n));
     const size_t limit = n > c ? n : c;
@@ -2672,7 +2696,7 @@ int dyn_solve(void* stream, const dyn_solve_io* io, float dt) {
     w.contact_cap = io->contact_capacity;
     w.counters = io->d_counters;
     uint32_t* words = w.ncontacts;  // scratch: [0] the count the kernels read, [1] the check
-    LAUNCH_ON(st, k_solve_stage_reset, 1u, w, io->d_contact_count, words);
+    LAUNCH_ON(st, k_solve_stage_reset, blocks_for(w.n), w, io->d_contact_count, words);
     if (w.contact_cap) LAUNCH_ON(st, k_solve_stage_check, blocks_for(w.contact_cap), w, words);
     uint32_t bad = 0;
     DYN_HIP(hipMemcpyAsync(&bad, words + 1, 4u, hipMemcpyDeviceToHost, st));
