@@ -25,7 +25,6 @@ constexpr int kBpThreads = 256;
 constexpr uint32_t kEntryPasses = 4;  // 4 key bytes, passes.rs:88
 constexpr uint32_t kResolutionBlocks = 1024;
 constexpr uint32_t kSparseLevel = 32;  // coarser levels of at most this many entries are read whole
-constexpr uint32_t kLinkBatch = 4;     // run entries a level-link probe loads together
 constexpr uint32_t kLinkStage = 64;    // link pairs a collider's counting pass keeps for the emitting pass
 constexpr uint32_t kGatherRun = 8;     // mean run length of a coarser level from which records are gathered
 
@@ -620,22 +619,10 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             cell_at(cells, ordinal, cell);
             uint32_t first, end;
             key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]) ^ salt, &first, &end);
-            // A settled pile's coarse cells hold runs of tens of entries: kLinkBatch records are
-            // loaded together, then tested in run order (the pairs come out in the same order).
-            for (uint32_t e = first; e < end; e += kLinkBatch) {
-                EntryView o[kLinkBatch];
-#pragma unroll
-                for (uint32_t j = 0; j < kLinkBatch; ++j) {
-                    if (e + j < end) o[j] = rec.at(e + j);
-                }
-#pragma unroll
-                for (uint32_t j = 0; j < kLinkBatch; ++j) {
-                    if (e + j >= end) break;
-                    if (!awake && !(o[j].info & DYN_ENTRY_AWAKE)) continue;
-                    if (mate_in_cell(self, o[j], cs, cell)) {
-                        emit_pair(sink, self.group, self.info, o[j].group, o[j].info, worlds);
-                    }
-                }
+            for (uint32_t e = first; e < end; ++e) {
+                const EntryView o = rec.at(e);
+                if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
+                if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
             }
         }
     }

@@ -111,14 +111,16 @@ struct W {
     // Solver deltas without atomics: every block writes its fixed-point words to its own slots
     // and every row sums the words of the blocks on its list. The words are integers, so the
     // sum does not depend on the order of the list.
-    // A row's list is a run of entries from row_start, one per block on it; each block writes
-    // its words for a side straight into that side's entry, so a row reads its run in order.
+    // A row's list is a run of entries from row_start, one per block on it. The velocity words
+    // stay with their block (k_solve writes a block's 12 words together) and the row gathers
+    // them through adj; a block writes its position words straight into its sides' entries, so
+    // the position pass reads a row's run in order.
     uint32_t* slots;        // per block and side: k_blocks' place in the row's list, then
                             // (k_fill_lists) the entry's index; kNoSlot for an inert side
     uint32_t* list_counts;  // per row: blocks on its list (0 for inert rows)
     uint32_t* row_start;    // per row: its first list entry
-    uint32_t* vwords;       // per list entry: 6 velocity words (lin, ang) its block last sent, in
-                            // kVelocityEntry-word entries
+    uint32_t* adj;          // per list entry: block << 1 | side
+    uint32_t* vwords;       // per block: 12 velocity words (first lin, ang; second lin, ang)
     uint4* pwords;          // per list entry: {contributed, lin words} of its block's last
                             // position pass
     uint32_t* contrib;      // inert rows: marked when a block of theirs contributed
@@ -1200,21 +1202,20 @@ __global__ void __launch_bounds__(kThreads) k_fill_lists(W w) {
     if (i >= *w.ncontacts) return;
     uint2& slot = reinterpret_cast<uint2*>(w.slots)[i];
     const uint2 place = slot;
-    slot = make_uint2(place.x != kNoSlot ? w.row_start[w.blocks[2 * i]] + place.x : kNoSlot,
-                      place.y != kNoSlot ? w.row_start[w.blocks[2 * i + 1]] + place.y : kNoSlot);
+    const uint2 entry = make_uint2(place.x != kNoSlot ? w.row_start[w.blocks[2 * i]] + place.x : kNoSlot,
+                                   place.y != kNoSlot ? w.row_start[w.blocks[2 * i + 1]] + place.y : kNoSlot);
+    slot = entry;
+    if (entry.x != kNoSlot) w.adj[entry.x] = i << 1;
+    if (entry.y != kNoSlot) w.adj[entry.y] = (i << 1) | 1u;
 }
 
 // One side's six velocity words of block ci.
-constexpr uint32_t kVelocityEntry = 8;  // words per velocity list entry: lin, ang, 2 spare (aligned)
-
 __device__ __forceinline__ void store_words(const W& w, uint32_t ci, uint32_t side, v3 lin, v3 ang) {
-    const uint32_t entry = w.slots[2u * ci + side];
-    if (entry == kNoSlot) return;
-    uint32_t* d = w.vwords + kVelocityEntry * entry;
+    uint2* d = reinterpret_cast<uint2*>(w.vwords + ci * 12u + side * 6u);
     const float scale = DYN_SOLVER_VELOCITY_SCALE;
-    *reinterpret_cast<uint4*>(d) = make_uint4(solver_word(lin.x, scale), solver_word(lin.y, scale),
-                                              solver_word(lin.z, scale), solver_word(ang.x, scale));
-    *reinterpret_cast<uint2*>(d + 4) = make_uint2(solver_word(ang.y, scale), solver_word(ang.z, scale));
+    d[0] = make_uint2(solver_word(lin.x, scale), solver_word(lin.y, scale));
+    d[1] = make_uint2(solver_word(lin.z, scale), solver_word(ang.x, scale));
+    d[2] = make_uint2(solver_word(ang.y, scale), solver_word(ang.z, scale));
 }
 
 __device__ __forceinline__ void make_tangents(v3 n, v3* t1, v3* t2) {
@@ -1561,8 +1562,8 @@ __global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
 
 // solver_block_apply.wgsl over every claimed row
 // List entries whose loads k_apply and k_position_apply keep in flight together: a row of up to
-// kApplyBatch blocks reads its run of entries in one round (integer sums: any order gives the
-// same words).
+// kApplyBatch blocks reads its entries (and, in k_apply, then their blocks' words) in one round
+// (integer sums: any order gives the same words).
 constexpr uint32_t kApplyBatch = 8;
 
 __global__ void __launch_bounds__(kThreads) k_apply(W w) {
@@ -1574,29 +1575,32 @@ __global__ void __launch_bounds__(kThreads) k_apply(W w) {
     if (count == 0u) return;
     dyn_body_state& s = w.st[row];
     const v3 v0 = ld3(s.velocity), av0 = ld3(s.angular_velocity);  // in flight with the list
-    const uint32_t* run = w.vwords + kVelocityEntry * start;
+    const uint32_t* list = w.adj + start;
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     for (uint32_t e = 0; e < count; e += kApplyBatch) {
-        uint4 a[kApplyBatch];
-        uint2 b[kApplyBatch];
+        uint32_t entry[kApplyBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kApplyBatch; ++j) entry[j] = e + j < count ? list[e + j] : kNoSlot;
+        uint2 a[kApplyBatch], b[kApplyBatch], c[kApplyBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kApplyBatch; ++j) {
-            a[j] = make_uint4(0u, 0u, 0u, 0u);
-            b[j] = make_uint2(0u, 0u);
-            if (e + j < count) {
-                const uint32_t* d = run + kVelocityEntry * (e + j);
-                a[j] = *reinterpret_cast<const uint4*>(d);
-                b[j] = *reinterpret_cast<const uint2*>(d + 4);
+            a[j] = b[j] = c[j] = make_uint2(0u, 0u);
+            if (entry[j] != kNoSlot) {
+                const uint2* d =
+                    reinterpret_cast<const uint2*>(w.vwords + (entry[j] >> 1) * 12u + (entry[j] & 1u) * 6u);
+                a[j] = d[0];
+                b[j] = d[1];
+                c[j] = d[2];
             }
         }
 #pragma unroll
         for (uint32_t j = 0; j < kApplyBatch; ++j) {
             s0 += a[j].x;
             s1 += a[j].y;
-            s2 += a[j].z;
-            s3 += a[j].w;
-            s4 += b[j].x;
-            s5 += b[j].y;
+            s2 += b[j].x;
+            s3 += b[j].y;
+            s4 += c[j].x;
+            s5 += c[j].y;
         }
     }
     if ((s0 | s1 | s2 | s3 | s4 | s5) == 0u) return;
@@ -2339,7 +2343,8 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.slots, 2u * static_cast<size_t>(contact_capacity));
     A(w.list_counts, body_count);
     A(w.row_start, body_count);
-    A(w.vwords, 2u * kVelocityEntry * static_cast<size_t>(contact_capacity));
+    A(w.adj, 2u * static_cast<size_t>(contact_capacity));
+    A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
     A(w.pwords, 2u * static_cast<size_t>(contact_capacity));
     A(w.contrib, body_count);
     uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
@@ -2650,7 +2655,8 @@ SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base)
     w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
     w.list_counts = reinterpret_cast<uint32_t*>(take(4u * n));
     w.row_start = reinterpret_cast<uint32_t*>(take(4u * n));
-    w.vwords = reinterpret_cast<uint32_t*>(take(4u * 2u * kVelocityEntry * c));
+    w.adj = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
+    w.vwords = reinterpret_cast<uint32_t*>(take(4u * 12u * c));
     w.pwords = reinterpret_cast<uint4*>(take(sizeof(uint4) * 2u * c));
     w.contrib = reinterpret_cast<uint32_t*>(take(4u * n));
     const size_t limit = n > c ? n : c;
