@@ -1044,6 +1044,22 @@ __global__ void __launch_bounds__(kThreads) k_solve_stage_check(W w, uint32_t* w
     if (c.a >= w.k || c.b >= w.k || w.owner[c.a] >= w.n || w.owner[c.b] >= w.n) atomicOr(&words[1], 1u);
 }
 
+// Before the hooking pass: every linking contact between two simulated rows points the higher
+// row's parent at the lower row (an atomic minimum, so each row ends at its lowest linked
+// neighbour below it, or itself). Parents still point only at lower rows of the same
+// component, so k_island_link's hooks and the flatten rounds end at the same minimum-row roots;
+// most rows then start one short hop from their root, which spares the hooks their long finds
+// on a pile that is one island.
+__global__ void __launch_bounds__(kThreads) k_island_seed(W w, float slop) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= min(*w.ncontacts, w.contact_cap)) return;
+    const dyn_contact& c = w.contacts[i];
+    if (c.point_count == 0u || c.sensor == 1u || c.a >= w.k || c.b >= w.k) return;
+    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
+    if (fb >= w.d || sb >= w.d || fb == sb || !touches(c, slop)) return;
+    atomicMin(&w.parents[max(fb, sb)], min(fb, sb));
+}
+
 // dyn_islands' set-up: no wake or motion flags, the island counters at zero.
 __global__ void __launch_bounds__(kThreads) k_islands_stage_reset(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2186,6 +2202,7 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_relay, cb, w);
     LAUNCH(k_contact_begin, cb, w, w.p.slop);
     LAUNCH(k_island_init, db, w);
+    LAUNCH(k_island_seed, cb, w, w.p.slop);
     LAUNCH(k_island_link, cb, w, w.p.slop);
     LAUNCH(k_island_link_resting, cb, w);
     uint32_t rounds = 0;
@@ -2748,7 +2765,10 @@ int dyn_islands(void* stream, const dyn_islands_io* io) {
     LAUNCH_ON(st, k_islands_stage_reset, nb, w);
     if (w.d == 0u) return DYN_OK;
     LAUNCH_ON(st, k_island_init, db, w);
-    if (w.contact_cap) LAUNCH_ON(st, k_island_link, cb, w, io->slop);
+    if (w.contact_cap) {
+        LAUNCH_ON(st, k_island_seed, cb, w, io->slop);
+        LAUNCH_ON(st, k_island_link, cb, w, io->slop);
+    }
     uint32_t rounds = 0;
     for (uint32_t v = w.d < 2u ? 2u : w.d; v > 1u; v >>= 1) ++rounds;
     rounds += 1;  // ilog2(max(dynamic, 2)) + 1, crates/dynamis-rigid/src/lib.rs:79-81
