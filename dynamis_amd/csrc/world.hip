@@ -2202,7 +2202,9 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_relay, cb, w);
     LAUNCH(k_contact_begin, cb, w, w.p.slop);
     LAUNCH(k_island_init, db, w);
-    LAUNCH(k_island_seed, cb, w, w.p.slop);
+    // A batch's islands are bounded by its worlds, whose finds are short already: the seeding
+    // pass pays only in a single world, where one island can span the whole pile.
+    if (!w.worlds) LAUNCH(k_island_seed, cb, w, w.p.slop);
     LAUNCH(k_island_link, cb, w, w.p.slop);
     LAUNCH(k_island_link_resting, cb, w);
     uint32_t rounds = 0;
