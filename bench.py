@@ -5,15 +5,15 @@ Workloads (`--workload`):
   pile        (default) the mixed convex pile: 100,000 bodies, a quarter each of spheres, boxes,
               capsules and ellipsoid hulls of 16-64 points, density 1000, on a jittered lattice in
               a 60 x 60 m bin of four static walls on a plane; FP32, dt = 1/60, 8 solver
-              iterations (2 substeps, 10 position iterations); the run is 600 steps, of which the
-              first `--warmup` are untimed.
+              iterations (2 substeps, 10 position iterations); the run is 600 steps.
   cubes       1,000 cubes (half-extent 0.5 m, density 1000, friction 0.5) dropped from 1-40 m over
               a 20 x 20 m footprint onto a plane; 4 solver iterations; 300 steps.
   batch       4,096 independent worlds stepped as one batch (dyn_batch_create): each a plane and
               N_w bodies, N_w log-uniform in [8, 2048] (1,550,732 bodies in all), half boxes
               (half-extents ~ U[0.2, 0.8]) and half spheres (r ~ U[0.25, 0.75]), density 1000;
               8 solver iterations, 240 steps. Also times the same worlds stepped one at a time on
-              one stream (a sample of every 16th world, scaled to all 4,096).
+              one stream (every 32nd world through the same run, the same window timed, scaled
+              to all 4,096).
   broadphase  the isolated sort + broadphase pass over 1,048,576 boxes.
   fixed_grid  the same boxes through the literal fixed-cell mode: a uniform 4 m grid, keys
               (cell hash, box index), LSD radix sort, cell runs, canonical overlapping pairs
@@ -25,16 +25,24 @@ Workloads (`--workload`):
               8 iterations, 64 steps with a checkpoint every 8; forward and backward times and
               their ratio (also reported as `adjoint` in the default line).
 One step = one World::step of every body (broadphase: one pass over every box); body-steps/s =
-bodies x timed steps / seconds. Inputs are resident in HBM before the timed region; the timed
-region is bracketed by events on the world's own stream.
+bodies x timed steps / seconds. Inputs are resident in HBM before the timed region.
 
-Prints ONE JSON line. `roofline` covers the kernel with the largest device time in a window of
-warm-up steps (every launch bracketed by events): its average launch duration is measured over
-the timed steps themselves by events on the world's stream around each of its launches (the timed
-steps then launch eagerly instead of replaying the step's graph), and its algorithmic bytes per
-launch come from the step's counts at the start and end of the timed steps. `stages` is a window
-of further steps with every launch timed. `cpu_baseline` times the single-threaded C oracle on a
-bounded sample of the same workload.
+The world workloads time a settled window: of the workload's run (pile 600 steps, batch 240,
+cubes 300) the first run - steps - warmup steps settle the scene untimed, `--warmup` more steps
+are untimed, and the last `--steps` steps are timed by events on the world's own stream as
+replays of the captured step graph, with no launch bracketed. The default line (pile) also carries
+the batch's settled window with its ratio to the same worlds stepped one at a time (`batch`), the
+isolated sort + broadphase pass (`broadphase`) and the adjoint's backward/forward ratio
+(`adjoint`).
+
+`roofline` covers the kernel with the largest device time among those with an algorithmic byte
+count: the timed steps are re-run eagerly from a snapshot taken at their start with every launch
+bracketed by events (the `stages` table), then re-run again with only that kernel's launches
+bracketed, which gives its average launch duration (both re-runs end in the timed run's states
+bit for bit: `config.reruns_identical`). The second re-run is marked with roctxProfilerResume /
+roctxProfilerPause, so `rocprofv3 --selected-regions` traces exactly those launches. Bytes per
+launch come from the step's counts at the window's start and end. `cpu_baseline` times the C
+oracle on a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -57,6 +65,11 @@ DT = np.float32(1.0 / 60.0)
 GRAPHS = True  # steps replay the captured graph; --no-graphs launches them eagerly
 BOXES = 1 << 20
 BOX_SEED = 2
+
+
+def progress(message: str) -> None:
+    """One line on stderr per finished phase (the JSON line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {message}", file=sys.stderr, flush=True)
 
 
 # ---- algorithmic bytes per launch ---------------------------------------------------------------
@@ -141,16 +154,16 @@ def batch_scenes():
 
 
 WORKLOADS = {
-    "pile": {"scene": pile_scene, "steps": 540, "warmup": 60, "run": 600,
+    "pile": {"scene": pile_scene, "steps": 50, "warmup": 10, "run": 600,
              "describe": "mixed convex pile: 100,000 bodies (25% each spheres r~U[0.25,0.75], boxes "
                          "h~U[0.2,0.8]^3, capsules r~U[0.15,0.4] hh~U[0.2,0.8], hulls of 16-64 points "
                          "on ellipsoids with semi-axes ~U[0.3,0.9] from 512 shape sources) on a "
                          "jittered lattice in a 60x60 m walled bin on a plane, density 1000, "
                          "dt=1/60, 8 solver iterations, 2 substeps, 10 position iterations"},
-    "cubes": {"scene": cubes_scene, "steps": 290, "warmup": 10, "run": 300,
+    "cubes": {"scene": cubes_scene, "steps": 50, "warmup": 10, "run": 300,
               "describe": "1,000 cubes (h=0.5 m, density 1000, friction 0.5) dropped from 1-40 m "
                           "over 20x20 m onto a plane, dt=1/60, 4 solver iterations"},
-    "batch": {"scene": None, "steps": 220, "warmup": 20, "run": 240,
+    "batch": {"scene": None, "steps": 40, "warmup": 10, "run": 240,
               "describe": "4,096 independent worlds in one batch, each a plane and N_w bodies with "
                           "N_w log-uniform in [8, 2048] (1,550,732 bodies), 50/50 boxes "
                           "h~U[0.2,0.8]^3 and spheres r~U[0.25,0.75], density 1000, dt=1/60, "
@@ -173,41 +186,70 @@ def kernel_bytes(name: str, c: dict) -> float:
                       c.get("gjk", 0), c.get("epa", 0), c.get("manifold", 0), c.get("gjk_hull", 0))
 
 
-def timed_run(world, steps: int, warmup: int, profile_steps: int):
-    """Warm-up steps (the last `profile_steps` of them with every launch timed, to pick the
-    kernel with the largest device time), the timed steps (events on the world's stream around
-    them and around every launch of that kernel), then a window of steps with every launch timed
-    for the stage table. Steps with launches timed run eagerly; the others replay the graph."""
+class Roctx:
+    """roctxProfilerResume / roctxProfilerPause around the eager window whose kernel launches the
+    roofline figure times, so that `rocprofv3 --selected-regions` traces exactly those launches
+    (no-ops when no profiler is attached; absent library: nothing is marked)."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            for fn in (self.lib.roctxProfilerResume, self.lib.roctxProfilerPause):
+                fn.argtypes = [ctypes.c_uint64]
+                fn.restype = ctypes.c_int
+
+    def resume(self) -> None:
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+    def pause(self) -> None:
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+
+ROCTX = None
+
+
+def settled_run(world, run: int, steps: int, warmup: int):
+    """The last `steps` steps of a `run`-step run, timed as graph replays.
+
+    The first run - steps - warmup steps settle the scene untimed, `warmup` more steps are
+    untimed, then the durable state is snapshotted and `steps` steps are timed by events on the
+    world's stream (graph replays unless --no-graphs; no launch is bracketed). The same steps are
+    then re-run twice from the snapshot, eagerly: once with every launch bracketed by events (the
+    stage table; its kernel with the largest device time is the roofline kernel), once with only
+    the roofline kernel's launches bracketed (its average launch duration). Each re-run ends in
+    the timed run's states bit for bit, so all three windows do the same work."""
     import ctypes
 
+    from dynamis_amd._native import call
     from dynamis_amd.device import profile_collect, profile_enable, profile_watch
 
-    from dynamis_amd._native import call
-
+    global ROCTX
+    if ROCTX is None:
+        ROCTX = Roctx()
     stream = ctypes.c_void_p(world.stream())
+    settle = max(run - steps - warmup, 0)
+    for _ in range(settle):
+        world.step(DT)
+    world.wait()
+    progress(f"{settle} settling steps done")
     first = ctypes.c_void_p()
     call("dyn_event_create", ctypes.byref(first))
     call("dyn_event_record", first, stream)
-    probe_steps = min(profile_steps, warmup)
-    for _ in range(warmup - probe_steps):
+    for _ in range(warmup):
         world.step(DT)
     world.wait()
-    profile_watch(None)
-    profile_enable(True)
-    for _ in range(probe_steps):
-        world.step(DT)
-    world.wait()
-    profile_enable(False)
-    probe = profile_collect()
+    snap = world.snapshot()
     before = world_counts(world)
-    dominant = max((n for n in probe if kernel_bytes(n, before) > 0), key=lambda n: probe[n][1], default=None)
-    if dominant:
-        profile_watch(dominant)
-        profile_enable(True)
     ms = timed_steps([world], stream, steps)
-    profile_enable(False)
-    profile_watch(None)
-    watched = profile_collect()
     # the reference's benchmark.rs figure: its wall time includes the warm-up steps
     last = ctypes.c_void_p()
     call("dyn_event_create", ctypes.byref(last))
@@ -216,24 +258,48 @@ def timed_run(world, steps: int, warmup: int, profile_steps: int):
     call("dyn_event_elapsed_ms", ctypes.byref(all_ms), first, last)
     call("dyn_event_destroy", first)
     call("dyn_event_destroy", last)
-    measured = world.measured()
     after = world_counts(world)
+    measured = world.measured()
+    final_states = world.states()
+    progress(f"{steps} timed steps: {ms / steps:.3f} ms per step")
+    # every launch of the same steps timed, from the snapshot
+    world.restore(snap)
+    profile_watch(None)
     profile_enable(True)
-    for _ in range(profile_steps):
+    for _ in range(steps):
         world.step(DT)
     world.wait()
     profile_enable(False)
     stages = profile_collect()
+    same = world.states().tobytes() == final_states.tobytes()
+    mean = {k: 0.5 * (before[k] + after[k]) for k in before}
+    dominant = max((n for n in stages if kernel_bytes(n, mean) > 0), key=lambda n: stages[n][1], default=None)
     timed = None
-    if dominant and dominant in watched:
-        launches, total_ms = watched[dominant]
-        # the timed steps' bytes per launch: the mean of the counts at their start and end
-        mean = {k: 0.5 * (before[k] + after[k]) for k in before}
-        timed = {"kernel": dominant, "launches": launches, "avg_us": 1000.0 * total_ms / launches,
-                 "bytes_per_launch": kernel_bytes(dominant, mean)}
-    if timed is not None:
-        timed["with_warmup_ms"] = float(all_ms.value)
-    return ms, stages, after, measured, timed
+    if dominant:
+        world.restore(snap)
+        profile_watch(dominant)
+        profile_enable(True)
+        ROCTX.resume()
+        for _ in range(steps):
+            world.step(DT)
+        world.wait()
+        ROCTX.pause()
+        profile_enable(False)
+        profile_watch(None)
+        watched = profile_collect()
+        same = same and world.states().tobytes() == final_states.tobytes()
+        if dominant in watched:
+            launches, total_ms = watched[dominant]
+            # bytes per launch: the mean of the counts at the window's start and end
+            timed = {"kernel": dominant, "launches": launches, "avg_us": 1000.0 * total_ms / launches,
+                     "bytes_per_launch": kernel_bytes(dominant, mean),
+                     "window": f"the {steps} timed steps re-run eagerly from a snapshot taken at "
+                               f"their start, only this kernel's launches bracketed by events"}
+    snap.free()
+    progress(f"eager re-runs done (roofline kernel {dominant})")
+    return {"ms": ms, "stages": stages, "counts": mean, "measured": measured, "timed": timed,
+            "final_states": final_states, "with_warmup_ms": float(all_ms.value), "settle": settle,
+            "reruns_identical": same}
 
 
 def sleep_off(scene):
@@ -248,14 +314,15 @@ def sleep_off(scene):
     return s
 
 
-def run_sleep_off(scene, steps: int, warmup: int) -> float:
+def run_sleep_off(scene, run: int, steps: int, warmup: int) -> float:
+    """Milliseconds of the same settled window (the last `steps` of `run`) with sleeping off."""
     import ctypes
 
     from dynamis_amd.world import World
     world = World()
     world.create(sleep_off(scene))
     world.set_graphs(GRAPHS)
-    for _ in range(warmup):
+    for _ in range(max(run - steps, 0)):
         world.step(DT)
     world.wait()
     ms = timed_steps([world], ctypes.c_void_p(world.stream()), steps)
@@ -279,7 +346,7 @@ def tune_world(world) -> None:
               "from_cache": info["from_cache"]}
 
 
-def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
+def run_world(workload: str, steps: int, warmup: int):
     import dynamis_amd
     from dynamis_amd._native import call
     from dynamis_amd.world import World
@@ -292,10 +359,11 @@ def run_world(workload: str, steps: int, warmup: int, profile_steps: int):
     world.create(scene)
     world.set_graphs(GRAPHS)
     tune_world(world)
-    ms, stages, counts, measured, timed = timed_run(world, steps, warmup, profile_steps)
-    final_states = world.states()
+    r = settled_run(world, WORKLOADS[workload]["run"], steps, warmup)
     world.close()
-    return ms, len(scene.states), stages, counts, measured, scene, final_states, timed
+    r["scene"] = scene
+    r["bodies"] = len(scene.states)
+    return r
 
 
 def timed_steps(worlds, stream, steps: int) -> float:
@@ -319,7 +387,9 @@ def timed_steps(worlds, stream, steps: int) -> float:
     return ms.value
 
 
-def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, serial_steps: int):
+def run_batch(steps: int, warmup: int, serial_every: int):
+    """The batch's settled window (the last `steps` of its run) and the same window of every
+    `serial_every`-th world stepped one at a time on one stream."""
     import ctypes
 
     import dynamis_amd
@@ -329,17 +399,15 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
     if dynamis_amd.device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
     call("dyn_set_device", 0)
+    run = WORKLOADS["batch"]["run"]
     scenes = batch_scenes()
     batch = Batch(scenes)
     batch.set_graphs(GRAPHS)
     tune_world(batch)
-    ms, stages, counts, m, timed = timed_run(batch, steps, warmup, profile_steps)
-    bodies = len(batch.scene.states)
-    final = batch.states()
-    ends = [batch.world_states(i, final) for i in range(len(scenes))]
+    r = settled_run(batch, run, steps, warmup)
+    r["bodies"] = len(batch.scene.states)
+    r["ends"] = [batch.world_states(i, r["final_states"]) for i in range(len(scenes))]
     batch.close()
-    # the same worlds one at a time: a sample of every serial_every-th world, all on one stream,
-    # from their first steps (the batch's worlds start from the same states)
     serial = None
     if serial_every > 0:
         shared = ctypes.c_void_p()
@@ -351,28 +419,37 @@ def run_batch(steps: int, warmup: int, profile_steps: int, serial_every: int, se
             w.create(s, stream=shared.value)
             w.set_graphs(GRAPHS)
             worlds.append(w)
-        timed_steps(worlds, shared, warmup)  # the batch's untimed steps (and graph capture)
-        sms = timed_steps(worlds, shared, serial_steps)
+        progress(f"serial: {len(worlds)} worlds created")
+        timed_steps(worlds, shared, max(run - steps, 0))  # the batch's untimed steps
+        sms = timed_steps(worlds, shared, steps)
+        same = all(w.states().tobytes() == end.tobytes()
+                   for w, end in zip(worlds, r["ends"][::serial_every]))
         for w in worlds:
             w.close()
         call("dyn_stream_destroy", shared)
-        per_step = sms / serial_steps * (len(scenes) / len(sampled))
-        serial = {"serial_ms_per_step": per_step, "batched_ms_per_step": ms / steps,
-                  "batched_vs_serial": per_step / (ms / steps),
-                  "sample": f"{len(sampled)} of the {len(scenes)} worlds (every {serial_every}th) "
-                            f"stepped one at a time on one stream for {serial_steps} timed steps "
-                            f"after the batch's {warmup} untimed ones (earlier, lighter steps than "
-                            f"the batch's timed ones), scaled by {len(scenes) / len(sampled):g}"}
-    return ms, bodies, stages, counts, m, scenes, serial, ends, timed
+        batched = r["ms"] / steps
+        per_step = sms / steps * (len(scenes) / len(sampled))
+        serial = {"serial_ms_per_step": per_step, "batched_ms_per_step": batched,
+                  "batched_vs_serial": per_step / batched, "sampled_worlds_match_batch": same,
+                  "sample": f"{len(sampled)} of the {len(scenes)} worlds (every {serial_every}th), each "
+                            f"its own world, stepped one at a time on one stream through the same "
+                            f"{run}-step run; the same {steps} steps timed, scaled by "
+                            f"{len(scenes) / len(sampled):g}"}
+    r["serial"] = serial
+    r["scenes"] = scenes
+    return r
 
 
-def cpu_batch(scenes, ends, every: int, sample_steps: int):
+def _cpu_worlds(args):
+    """Worker: `sample_steps` oracle steps of each (scene, start states) pair; body-steps and
+    seconds of stepping."""
     import copy
 
     from tests.oracle_binding import load
+    pairs, sample_steps = args
     oracle = load()
     bodies, secs = 0, 0.0
-    for s, end in list(zip(scenes, ends))[::every]:
+    for s, end in pairs:
         start = copy.copy(s)
         start.states = end.copy()
         world = oracle.world(start)
@@ -380,9 +457,30 @@ def cpu_batch(scenes, ends, every: int, sample_steps: int):
         for _ in range(sample_steps):
             world.step(DT)
         secs += time.perf_counter() - t0
-        bodies += len(s.states)
+        bodies += len(s.states) * sample_steps
         world.close()
-    return bodies * sample_steps / secs, secs
+    return bodies, secs
+
+
+def cpu_batch(scenes, ends, every: int, sample_steps: int, workers: int):
+    """The oracle on every `every`-th world from the device's end states: the worlds are
+    independent, so `workers` processes step disjoint sets of them. Returns body-steps/s over
+    the wall time of the pool, the wall time, and the single-process rate (the sum of the
+    workers' stepping seconds)."""
+    import concurrent.futures
+    pairs = list(zip(scenes, ends))[::every]
+    chunks = [(pairs[i::workers], sample_steps) for i in range(workers)]
+    t0 = time.perf_counter()
+    if workers > 1:
+        import multiprocessing
+        with concurrent.futures.ProcessPoolExecutor(workers, mp_context=multiprocessing.get_context("fork")) as pool:
+            results = list(pool.map(_cpu_worlds, chunks))
+    else:
+        results = [_cpu_worlds(chunks[0])]
+    wall = time.perf_counter() - t0
+    bodies = sum(r[0] for r in results)
+    busy = sum(r[1] for r in results)
+    return bodies / wall, wall, bodies / busy
 
 
 def run_broadphase(steps: int, warmup: int):
@@ -531,10 +629,10 @@ def cpu_broadphase(passes: int):
 
 # ---- report ------------------------------------------------------------------------------------------
 
-def step_roofline(table: dict, profile_steps: int, ms_per_step: float) -> dict:
-    """All kernels' algorithmic bytes of one step (the profiled window's launches) over the
-    timed step time: the whole step against the HBM roof."""
-    per_step = sum(v["bytes"] * v["launches"] for v in table.values()) / max(profile_steps, 1)
+def step_roofline(table: dict, steps: int, ms_per_step: float) -> dict:
+    """All kernels' algorithmic bytes of one step (the stage window's launches over its `steps`
+    steps) over the timed step time: the whole step against the HBM roof."""
+    per_step = sum(v["bytes"] * v["launches"] for v in table.values()) / max(steps, 1)
     gbs = per_step / (ms_per_step * 1e-3) / 1e9
     return {"bytes_per_step": per_step, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
             "unit": "GB/s"}
@@ -636,6 +734,97 @@ def run_adjoint(reps: int = 3) -> dict:
             "forward_body_steps_per_s": worlds * boxes * steps / f}
 
 
+CPU_WORKERS = 16  # host cores a GPU box grants one GPU's job
+
+
+def world_report(wl: str, r: dict, steps: int, warmup: int, cpu_sample: int | None) -> dict:
+    """The line of a single-world workload's settled window (pile, cubes)."""
+    spec = WORKLOADS[wl]
+    counts = r["counts"]
+    table, roofline = stage_table(r["stages"], counts, lambda nm: kernel_bytes(nm, counts), r["timed"])
+    ms = r["ms"]
+    if roofline is not None:
+        roofline["step"] = step_roofline(table, steps, ms / steps)
+    bodies = r["bodies"]
+    sleep = None
+    if wl == "pile":
+        off_ms = run_sleep_off(r["scene"], spec["run"], steps, warmup)
+        sleep = {"sleep_off_value": bodies * steps / (off_ms / 1000.0), "sleep_off_ms_per_step": off_ms / steps,
+                 "note": "the same window with sleep_velocity = sleep_angular_velocity = 0; the main value "
+                         "keeps the reference's default sleeping"}
+    sample = (2 if wl == "pile" else 100) if cpu_sample is None else cpu_sample
+    cpu = None
+    if sample > 0:
+        progress(f"CPU baseline: {sample} oracle steps")
+        v, secs = cpu_world(r["scene"], r["final_states"], sample)
+        cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} steps of the same scene through the single-threaded C oracle, started "
+                         f"from the device's body states at the end of the timed steps ({secs:.1f} s)"}
+    run = r["settle"] + warmup + steps
+    config = {"workload": spec["describe"], "bodies": bodies, "steps_in_run": run,
+              "timed_window": f"steps {run - steps + 1}-{run} of the {run}-step run: {r['settle']} settling "
+                              f"steps and {warmup} warm-up steps untimed, then {steps} timed "
+                              f"{'graph replays' if GRAPHS else 'eager steps'}",
+              "graphs": GRAPHS, "seed": 3 if wl == "pile" else 1, "sleep": sleep,
+              "reference_style_value": bodies * (steps + warmup) / ((r["with_warmup_ms"]) / 1000.0),
+              "reference_style_note": "benchmark.rs divides by a wall time that includes its warm-up "
+                                      "steps: here the warm-up and timed steps together",
+              "reruns_identical": r["reruns_identical"],
+              "at_end": {k: r["measured"][k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
+                                                      "RESTING", "ACTIVE", "REFUSED_PAIRS",
+                                                      "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
+    return {"value": bodies * steps / (ms / 1000.0), "ms_per_step": ms / steps, "config": config,
+            "roofline": roofline, "cpu_baseline": cpu, "stages": table}
+
+
+def batch_report(r: dict, steps: int, warmup: int, cpu_sample: int | None) -> dict:
+    spec = WORKLOADS["batch"]
+    counts = r["counts"]
+    table, roofline = stage_table(r["stages"], counts, lambda nm: kernel_bytes(nm, counts), r["timed"])
+    ms = r["ms"]
+    if roofline is not None:
+        roofline["step"] = step_roofline(table, steps, ms / steps)
+    sample = 10 if cpu_sample is None else cpu_sample
+    cpu = None
+    if sample > 0:
+        workers = max(1, min(CPU_WORKERS, os.cpu_count() or 1))
+        progress(f"CPU baseline: oracle over sampled worlds in {workers} processes")
+        v, wall, one = cpu_batch(r["scenes"], r["ends"], 16, sample, workers)
+        cpu = {"value": v, "unit": "body-steps/s", "cores": workers, "kind": "port",
+               "single_core_value": one,
+               "sample": f"{sample} steps of every 16th world through the C oracle in {workers} worker "
+                         f"processes (independent worlds), each world started from the device's body "
+                         f"states at the end of the timed steps ({wall:.1f} s wall)"}
+    run = r["settle"] + warmup + steps
+    config = {"workload": spec["describe"], "bodies": r["bodies"], "worlds": BATCH_WORLDS,
+              "steps_in_run": run, "seed": BATCH_SEED, "graphs": GRAPHS,
+              "timed_window": f"steps {run - steps + 1}-{run} of the {run}-step run",
+              "serial": r["serial"], "reruns_identical": r["reruns_identical"],
+              "at_end": {k: r["measured"][k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
+                                                      "REFUSED_PAIRS", "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
+    return {"value": r["bodies"] * steps / (ms / 1000.0), "ms_per_step": ms / steps, "config": config,
+            "roofline": roofline, "cpu_baseline": cpu, "stages": table}
+
+
+def broadphase_report(steps: int, warmup: int, cpu_sample: int | None) -> dict:
+    ms, bodies, stages, counts = run_broadphase(steps, warmup)
+    table, roofline = stage_table(stages, counts, lambda nm: broadphase_bytes(
+        nm, counts["k"], counts["entries"], counts["pairs"]))
+    sample = 3 if cpu_sample is None else cpu_sample
+    cpu = None
+    if sample > 0:
+        v, secs = cpu_broadphase(sample)
+        cpu = {"value": v, "unit": "boxes/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} full passes of the same 2^20-box workload through the "
+                         f"single-threaded C oracle ({secs:.1f} s)"}
+    config = {"workload": "isolated sort + broadphase pass over 1,048,576 boxes (centres "
+                          "U[0,1000)^3 m, half-extents log-U[0.2,2] m)",
+              "boxes": BOXES, "entries": counts["entries"], "pairs": counts["pairs"], "seed": BOX_SEED}
+    return {"value": bodies * steps / (ms / 1000.0), "unit": "boxes/s", "ms_per_pass": ms / steps,
+            "passes": steps, "warmup": warmup, "config": config, "roofline": roofline, "cpu_baseline": cpu,
+            "stages": table}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -643,7 +832,6 @@ def main() -> int:
                                                "adjoint"], default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
-    parser.add_argument("--profile-steps", type=int, default=5)
     parser.add_argument("--no-graphs", action="store_true",
                         help="launch every step eagerly (for external profilers, which trace the "
                              "replayed graph of a captured step badly)")
@@ -651,6 +839,8 @@ def main() -> int:
                         help="keep the sorts' default tilings instead of autotuning them first")
     parser.add_argument("--cpu-sample", type=int, default=None,
                         help="oracle steps (broadphase: passes) timed for the CPU baseline; 0 skips it")
+    parser.add_argument("--only", action="store_true",
+                        help="pile: leave out the batch, broadphase and adjoint parts of the line")
     args = parser.parse_args()
     global GRAPHS, TUNE
     GRAPHS = not args.no_graphs
@@ -675,107 +865,65 @@ def main() -> int:
         ms, stages, counts = run_fixed_grid(steps, warmup)
         table, roofline = stage_table(stages, counts, lambda nm: fixed_grid_bytes(
             nm, counts["k"], counts["entries"], counts["pairs"]))
-        bodies = BOXES
-        cpu = None
         config = {"workload": "fixed-cell sort + broadphase pass over 1,048,576 boxes (centres U[0,1000)^3 m, "
                               "half-extents log-U[0.2,2] m), 4 m cells, keys (cell hash, box index)",
                   "boxes": BOXES, "entries": counts["entries"], "pairs": counts["pairs"], "refused": counts["refused"],
                   "seed": BOX_SEED, "cell_size": 4.0}
-        dtype = "u32"
-        measured = {}
+        part = {"value": BOXES * steps / (ms / 1000.0), "ms_per_step": ms / steps, "config": config,
+                "roofline": roofline, "cpu_baseline": None, "stages": table}
+        dtype, unit = "u32", "boxes/s"
     elif wl == "broadphase":
         steps = max(args.steps if args.steps is not None else 50, 1)
         warmup = max(args.warmup if args.warmup is not None else 10, 0)
-        ms, bodies, stages, counts = run_broadphase(steps, warmup)
-        table, roofline = stage_table(stages, counts, lambda nm: broadphase_bytes(
-            nm, counts["k"], counts["entries"], counts["pairs"]))
-        sample = 3 if args.cpu_sample is None else args.cpu_sample
-        cpu = None
-        if sample > 0:
-            v, secs = cpu_broadphase(sample)
-            cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
-                   "sample": f"{sample} full passes of the same 2^20-box workload through the "
-                             f"single-threaded C oracle ({secs:.1f} s)"}
-        config = {"workload": "isolated sort + broadphase pass over 1,048,576 boxes (centres "
-                              "U[0,1000)^3 m, half-extents log-U[0.2,2] m)",
-                  "boxes": BOXES, "entries": counts["entries"], "pairs": counts["pairs"], "seed": BOX_SEED}
-        dtype = "u32"
-        measured = {}
+        part = broadphase_report(steps, warmup, args.cpu_sample)
+        part["ms_per_step"] = part["ms_per_pass"]
+        dtype, unit = "u32", "boxes/s"
     elif wl == "batch":
         spec = WORKLOADS[wl]
         steps = max(args.steps if args.steps is not None else spec["steps"], 1)
         warmup = max(args.warmup if args.warmup is not None else spec["warmup"], 0)
-        ms, bodies, stages, counts, measured, scenes, serial, ends, timed = run_batch(
-            steps, warmup, args.profile_steps, 16, 3)
-        table, roofline = stage_table(stages, counts, lambda nm: kernel_bytes(nm, counts), timed)
-        if roofline is not None:
-            roofline["step"] = step_roofline(table, args.profile_steps, ms / steps)
-        sample = 10 if args.cpu_sample is None else args.cpu_sample
-        cpu = None
-        if sample > 0:
-            v, secs = cpu_batch(scenes, ends, 16, sample)
-            cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
-                   "sample": f"{sample} steps of every 16th world of the batch through the "
-                             f"single-threaded C oracle, each started from the device's body "
-                             f"states at the end of the timed and profiled steps ({secs:.1f} s)"}
-        config = {"workload": spec["describe"], "bodies": bodies, "worlds": BATCH_WORLDS,
-                  "steps_in_run": spec["run"], "seed": BATCH_SEED, "serial": serial,
-                  "at_end": {k: measured[k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
-                                                       "REFUSED_PAIRS", "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
-        dtype = "f32"
+        part = batch_report(run_batch(steps, warmup, 32), steps, warmup, args.cpu_sample)
+        dtype, unit = "f32", "body-steps/s"
     else:
         spec = WORKLOADS[wl]
         steps = max(args.steps if args.steps is not None else spec["steps"], 1)
         warmup = max(args.warmup if args.warmup is not None else spec["warmup"], 0)
-        ms, bodies, stages, counts, measured, scene, final, timed = run_world(wl, steps, warmup,
-                                                                             args.profile_steps)
-        table, roofline = stage_table(stages, counts, lambda nm: kernel_bytes(nm, counts), timed)
-        if roofline is not None:
-            roofline["step"] = step_roofline(table, args.profile_steps, ms / steps)
-        sleep = None
-        if wl == "pile":
-            off_ms = run_sleep_off(scene, steps, warmup)
-            sleep = {"sleep_off_value": bodies * steps / (off_ms / 1000.0), "sleep_off_ms_per_step": off_ms / steps,
-                     "note": "the same run with sleep_velocity = sleep_angular_velocity = 0; the main value "
-                             "keeps the reference's default sleeping"}
-        sample = (2 if wl == "pile" else 100) if args.cpu_sample is None else args.cpu_sample
-        cpu = None
-        if sample > 0:
-            v, secs = cpu_world(scene, final, sample)
-            cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
-                   "sample": f"{sample} steps of the same scene through the single-threaded C "
-                             f"oracle, started from the device's body states at the end of the "
-                             f"timed and profiled steps ({secs:.1f} s)"}
-        config = {"workload": spec["describe"], "bodies": bodies, "steps_in_run": spec["run"],
-                  "seed": 3 if wl == "pile" else 1, "sleep": sleep,
-                  "reference_style_value": (bodies * (steps + warmup) / (timed["with_warmup_ms"] / 1000.0)
-                                            if timed else None),
-                  "at_end": {k: measured[k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
-                                                       "RESTING", "ACTIVE", "REFUSED_PAIRS",
-                                                       "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
-        dtype = "f32"
+        part = world_report(wl, run_world(wl, steps, warmup), steps, warmup, args.cpu_sample)
+        dtype, unit = "f32", "body-steps/s"
     line = {
         "metric": METRIC,
-        "value": bodies * steps / (ms / 1000.0),
-        "unit": "body-steps/s",
+        "value": part["value"],
+        "unit": unit,
         "n_gpus": 1,
         "steps": steps,
         "warmup": warmup,
-        "ms_per_step": ms / steps,
+        "ms_per_step": part["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": config,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "stages": table,
+        "config": part["config"],
+        "roofline": part["roofline"],
+        "cpu_baseline": part["cpu_baseline"],
     }
-    if isinstance(config, dict) and TUNING is not None:
-        config["autotune"] = TUNING
-    if wl == "pile":
+    if TUNING is not None:
+        line["config"]["autotune"] = TUNING
+    if wl == "pile" and not args.only:
+        # the rest of the metric: the batch's rate and its ratio to serial stepping over the
+        # same window, the isolated sort + broadphase pass, the adjoint's backward/forward ratio
+        progress("batch")
+        b = batch_report(run_batch(steps, warmup, 32), steps, warmup, args.cpu_sample)
+        line["batch"] = {"value": b["value"], "unit": "body-steps/s", "ms_per_step": b["ms_per_step"],
+                         "batched_vs_serial": (b["config"]["serial"] or {}).get("batched_vs_serial"),
+                         "config": b["config"], "roofline": b["roofline"], "cpu_baseline": b["cpu_baseline"]}
+        progress("broadphase")
+        bp = broadphase_report(max(steps, 20), max(warmup, 5), args.cpu_sample)
+        line["broadphase"] = {k: bp[k] for k in ("value", "unit", "ms_per_pass", "passes", "warmup", "config",
+                                                 "roofline", "cpu_baseline")}
+        progress("adjoint")
         line["adjoint"] = run_adjoint()
+    line["stages"] = part["stages"]
     print(json.dumps(line))
     return 0
 

@@ -47,6 +47,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_sort_scratch_bytes": (c_size_t, [c_u32, c_u32]),
     "dyn_sort_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_void_p,
                                c_void_p, c_u32, c_u32, c_u32, c_void_p, c_size_t]),
+    "dyn_sort_faults": (c_int, [ctypes.POINTER(c_u32), c_int]),
     "dyn_broadphase_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),
     "dyn_broadphase": (c_int, [c_void_p, c_void_p]),
     "dyn_narrowphase_scratch_bytes": (c_size_t, [c_u32]),
@@ -94,6 +95,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),
     "dyn_world_read_counters": (c_int, [c_void_p, c_void_p]),
     "dyn_world_read_contacts": (c_int, [c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
+    "dyn_world_read_resting": (c_int, [c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
     "dyn_world_read_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_u32, ctypes.POINTER(c_u32)]),
 }
 
@@ -143,3 +145,11 @@ def device_count() -> int:
     n = c_int(0)
     call("dyn_device_count", ctypes.byref(n))
     return n.value
+
+
+def sort_faults(clear: bool = False) -> int:
+    """Look-back waits of any sort on the current device that gave up since the last clear
+    (dyn_sort_faults); 0 unless some sort's order is wrong."""
+    out = c_u32(0)
+    call("dyn_sort_faults", ctypes.byref(out), 1 if clear else 0)
+    return int(out.value)

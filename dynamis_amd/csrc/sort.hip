@@ -26,6 +26,16 @@ constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kFlagValue = (1u << 30) - 1;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
+}  // namespace
+
+// Look-back waits that gave up since the count was last cleared, over every sort this process ran
+// on the device (never expected: a tile's predecessors always publish). A timed-out tile scatters
+// with a partial prefix, so a non-zero count means some sort's order is wrong; dyn_sort_faults
+// reads it.
+__device__ uint32_t g_sort_faults = 0u;
+
+namespace {
+
 struct DigitList {
     uint32_t major[kSortMaxPasses];
     uint32_t shift[kSortMaxPasses];
@@ -186,6 +196,7 @@ __global__ void __launch_bounds__(256) sort_pass_kernel(
                 if (status == 0) {
                     if (++spins > kSpinLimit) {
                         atomicOr(&ctl[kCtlError], 1u);
+                        atomicAdd(&g_sort_faults, 1u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -561,6 +572,17 @@ int dyn_sort_tuning_import(const uint32_t* entries, uint32_t count) {
     tune_table().clear();
     for (uint32_t i = 0; i < count; ++i) {
         tune_table()[{static_cast<int>(entries[4 * i]), entries[4 * i + 1]}] = SortTiling{entries[4 * i + 2], entries[4 * i + 3]};
+    }
+    return DYN_OK;
+}
+
+int dyn_sort_faults(uint32_t* out, int clear) {
+    if (!out) return dyn::fail(DYN_E_INVALID, "dyn_sort_faults: null output");
+    DYN_HIP(hipDeviceSynchronize());
+    DYN_HIP(hipMemcpyFromSymbol(out, HIP_SYMBOL(dyn::g_sort_faults), sizeof(uint32_t), 0, hipMemcpyDeviceToHost));
+    if (clear) {
+        const uint32_t zero = 0u;
+        DYN_HIP(hipMemcpyToSymbol(HIP_SYMBOL(dyn::g_sort_faults), &zero, sizeof(uint32_t), 0, hipMemcpyHostToDevice));
     }
     return DYN_OK;
 }

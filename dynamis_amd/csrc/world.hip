@@ -2559,6 +2559,19 @@ int dyn_world_read_contacts(dyn_world* world, dyn_contact* out, uint32_t capacit
     return DYN_OK;
 }
 
+// The resting archive (frozen manifolds of sleeping pairs), sorted by (min body id, max body id):
+// the second half of the reference's contact readback (crates/dynamis-world/src/readback.rs:95-139).
+int dyn_world_read_resting(dyn_world* world, dyn_contact* out, uint32_t capacity, uint32_t* count) {
+    if (!world || !count) return dyn::fail(DYN_E_INVALID, "dyn_world_read_resting: null argument");
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    uint32_t n = 0;
+    DYN_HIP(hipMemcpy(&n, world->w.nresting, 4u, hipMemcpyDeviceToHost));
+    *count = n;
+    const uint32_t copy = n < capacity ? n : capacity;
+    if (copy && out) DYN_HIP(hipMemcpy(out, world->w.resting, sizeof(dyn_contact) * copy, hipMemcpyDeviceToHost));
+    return DYN_OK;
+}
+
 namespace {
 
 // dyn_narrowphase's scratch: the convex worklists and their sort.

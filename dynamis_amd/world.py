@@ -345,6 +345,14 @@ class World:
              ctypes.byref(count))
         return out[:count.value].copy()
 
+    def resting_contacts(self) -> np.ndarray:
+        """Manifolds frozen with their sleeping pairs (the resting archive)."""
+        self._ensure()
+        out = np.zeros(self.capacity, dtype=abi.CONTACT)
+        count = ctypes.c_uint32()
+        call("dyn_world_read_resting", self._handle, out.ctypes.data, self.capacity, ctypes.byref(count))
+        return out[:min(count.value, self.capacity)].copy()
+
     def pairs(self) -> np.ndarray:
         """Sorted candidate pairs of the last step as (P, 2) collider indices."""
         self._ensure()
@@ -358,6 +366,22 @@ class World:
     def stream(self) -> int:
         self._ensure()
         return lib().dyn_world_stream(self._handle)
+
+    # -- checkpoints (World::snapshot / restore, crates/dynamis-world/src/snapshot.rs:64-123) ---
+    def snapshot(self):
+        """The durable state copied into a new device buffer (on the world's stream)."""
+        from .device import DeviceBuffer
+        self._ensure()
+        nbytes = ctypes.c_size_t()
+        call("dyn_world_snapshot_bytes", self._handle, ctypes.byref(nbytes))
+        buf = DeviceBuffer(nbytes.value)
+        call("dyn_world_snapshot", self._handle, ctypes.c_void_p(buf.ptr))
+        return buf
+
+    def restore(self, buf) -> None:
+        """Puts a snapshot of this world back (on the world's stream)."""
+        self._ensure()
+        call("dyn_world_restore", self._handle, ctypes.c_void_p(buf.ptr))
 
     def close(self) -> None:
         if self._handle.value:

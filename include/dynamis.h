@@ -96,6 +96,12 @@ int dyn_sort_pairs(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t*
                    uint32_t capacity, const uint32_t* d_count, const uint32_t* d_base,
                    uint32_t plan, uint32_t major_words, uint32_t minor_words,
                    void* d_scratch, size_t scratch_bytes);
+/* Sort faults on the current device since the last clear: look-back waits of any sort (this
+ * entry point's, the broadphase's, a world step's) that gave up and left that sort's order wrong.
+ * Never expected; a caller that must trust an order checks it is 0. Waits for the device first;
+ * clears the count when `clear` is non-zero. No counterpart in the reference, whose sort has no
+ * cross-workgroup wait (crates/dynamis-sort/src/shaders/sort_scatter.wgsl:45-104). */
+int dyn_sort_faults(uint32_t* out, int clear);
 
 /* ---- broadphase (crates/dynamis-broadphase/src/passes.rs, crates/dynamis-rigid/shaders/
  *      collider_entries.wgsl, crates/dynamis-shader/shaders/grid_index.wgsl) -------------------- */
@@ -476,6 +482,9 @@ void* dyn_world_stream(dyn_world* world);
 int dyn_world_read_states(dyn_world* world, dyn_body_state* out);
 int dyn_world_read_counters(dyn_world* world, uint32_t* out);
 int dyn_world_read_contacts(dyn_world* world, dyn_contact* out, uint32_t capacity, uint32_t* count);
+/* The resting archive: manifolds frozen with their sleeping pairs, which the reference's
+ * World::inspect_contacts appends to the active ones (crates/dynamis-world/src/readback.rs:95-139). */
+int dyn_world_read_resting(dyn_world* world, dyn_contact* out, uint32_t capacity, uint32_t* count);
 int dyn_world_read_pairs(dyn_world* world, uint32_t* major, uint32_t* minor, uint32_t capacity,
                          uint32_t* count);
 

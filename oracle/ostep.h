@@ -39,6 +39,8 @@ uint32_t oracle_world_tape(const oracle_world* w, dyn_body_state* start, dyn_con
                            uint32_t capacity);
 /* Contacts of the last step (after the solve), in canonical pair order; returns the count. */
 uint32_t oracle_world_contacts(const oracle_world* w, dyn_contact* out, uint32_t capacity);
+/* The resting archive (frozen manifolds of sleeping pairs), in list order; returns the count. */
+uint32_t oracle_world_resting(const oracle_world* w, dyn_contact* out, uint32_t capacity);
 void oracle_world_counters(const oracle_world* w, uint32_t* out);
 /* Sorted candidate pairs of the last step; returns the count. */
 /* The solver alone over the given contacts (read, then written back with the new impulses) and
@@ -51,6 +53,11 @@ int oracle_world_narrowphase(oracle_world* w, const uint32_t* major, const uint3
                              dyn_contact* raw, uint32_t* valid);
 uint32_t oracle_world_pairs(const oracle_world* w, uint32_t* major, uint32_t* minor,
                             uint32_t capacity);
+
+/* The warm-start relay of identity.wgsl:85-117 alone (test hook): out = current with the
+ * impulses of held relayed; picks (4 words, may be null) = the held point relayed into each
+ * point, or ~0. */
+void oracle_relay_impulses(const dyn_contact* current, const dyn_contact* held, dyn_contact* out, uint32_t* picks);
 
 /* GJK / convex_hit of two colliders at given body states (test hook, no world). */
 int oracle_convex_query(const dyn_shape_source* sources, uint32_t source_count, const float* vertices,

@@ -1789,6 +1789,13 @@ static dyn_contact relay_impulses(const dyn_contact* current, const dyn_contact*
     }
     return r;
 }
+/* The relay of identity.wgsl:85-117 alone (test hook): `current` with the impulses of `held`
+ * relayed point by point; picks[i] = the held point relayed into point i, or ~0. */
+void oracle_relay_impulses(const dyn_contact* current, const dyn_contact* held, dyn_contact* out, uint32_t* picks) {
+    uint32_t local[4] = {NO_SLOT, NO_SLOT, NO_SLOT, NO_SLOT};
+    *out = relay_impulses(current, held, local);
+    if (picks) memcpy(picks, local, sizeof(local));
+}
 /* body_row.wgsl:1-13 */
 static uint32_t resolve_row(const oracle_world* w, uint32_t id, uint32_t gen) {
     if (id >= w->id_capacity) return DYN_NO_BODY;
@@ -2668,6 +2675,12 @@ uint32_t oracle_world_contacts(const oracle_world* w, dyn_contact* out, uint32_t
     uint32_t n = w->ncontacts < capacity ? w->ncontacts : capacity;
     memcpy(out, w->contacts, (size_t)n * sizeof *out);
     return w->ncontacts;
+}
+
+uint32_t oracle_world_resting(const oracle_world* w, dyn_contact* out, uint32_t capacity) {
+    uint32_t n = w->nresting < capacity ? w->nresting : capacity;
+    memcpy(out, w->resting, (size_t)n * sizeof *out);
+    return w->nresting;
 }
 
 void oracle_world_counters(const oracle_world* w, uint32_t* out) {

@@ -58,6 +58,8 @@ class Oracle:
         h.oracle_world_states.argtypes = [_p, _p]
         h.oracle_world_contacts.restype = _u32
         h.oracle_world_contacts.argtypes = [_p, _p, _u32]
+        h.oracle_world_resting.restype = _u32
+        h.oracle_world_resting.argtypes = [_p, _p, _u32]
         h.oracle_world_counters.restype = None
         h.oracle_world_counters.argtypes = [_p, _p]
         h.oracle_world_pairs.restype = _u32
@@ -74,6 +76,8 @@ class Oracle:
         h.oracle_convex_query.argtypes = [_p, _u32, _p, _u32, _p, _p, _p, _p, _p]
         h.oracle_inertia_inverse.restype = None
         h.oracle_inertia_inverse.argtypes = [_p, _p]
+        h.oracle_relay_impulses.restype = None
+        h.oracle_relay_impulses.argtypes = [_p, _p, _p, _p]
 
     def units_for(self, live: int) -> int:
         return int(self.h.oracle_units_for(live))
@@ -153,6 +157,16 @@ class Oracle:
                 "normal": out[7:10].copy(), "penetrating": bool(out[10]),
                 "hit_distance": float(out[11]), "hit_normal": out[12:15].copy()}
 
+    def relay(self, current, held):
+        """identity.wgsl's relay alone: (current with held's impulses relayed, the held point
+        relayed into each point or -1)."""
+        cur = np.ascontiguousarray(current, dtype=abi.CONTACT).reshape(1)
+        hld = np.ascontiguousarray(held, dtype=abi.CONTACT).reshape(1)
+        out = np.zeros(1, abi.CONTACT)
+        picks = np.zeros(4, np.uint32)
+        self.h.oracle_relay_impulses(cur.ctypes.data, hld.ctypes.data, out.ctypes.data, picks.ctypes.data)
+        return out[0], np.where(picks == 0xFFFFFFFF, -1, picks.astype(np.int64))
+
     def world(self, scene, contact_capacity=None, pair_capacity=None) -> "OracleWorld":
         return OracleWorld(self, scene, contact_capacity, pair_capacity)
 
@@ -195,6 +209,11 @@ class OracleWorld:
     def inspect_contacts(self) -> np.ndarray:
         out = np.zeros(self.capacity, dtype=abi.CONTACT)
         count = self.o.h.oracle_world_contacts(self.w, out.ctypes.data, self.capacity)
+        return out[:min(count, self.capacity)].copy()
+
+    def resting_contacts(self) -> np.ndarray:
+        out = np.zeros(self.capacity, dtype=abi.CONTACT)
+        count = self.o.h.oracle_world_resting(self.w, out.ctypes.data, self.capacity)
         return out[:min(count, self.capacity)].copy()
 
     def pairs(self) -> np.ndarray:

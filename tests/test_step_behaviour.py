@@ -18,22 +18,6 @@ from tests.worlds import (DT, BodyDesc, angle_between, body_momentum, body_spin,
                           static_config, symplectic_fall, tower)
 
 
-@pytest.fixture(params=["oracle", pytest.param("device", marks=pytest.mark.gpu)])
-def engine(request):
-    """A factory of worlds over encoded scenes: the oracle's or the device's."""
-    if request.param == "oracle":
-        oracle = request.getfixturevalue("oracle")
-        return lambda scene, capacity=None: oracle.world(scene, capacity)
-    request.getfixturevalue("device")
-    from dynamis_amd.world import World
-
-    def make(scene, capacity=None):
-        world = World()
-        world.create(scene, capacity)
-        return world
-    return make
-
-
 def run(engine, bodies, cfg, frames, capacity=None):
     scene = encode(bodies, cfg)
     world = engine(scene, capacity)
