@@ -71,6 +71,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_batch_create": (c_int, [ctypes.POINTER(c_void_p), c_void_p, c_void_p, c_void_p, c_void_p, c_u32,
                                  c_u32, c_void_p, c_void_p, c_void_p, c_u32, c_u32, c_u32, c_u32]),
     "dyn_world_step": (c_int, [c_void_p, ctypes.c_float]),
+    "dyn_world_edit": (c_int, [c_void_p, c_void_p, c_void_p, c_u32]),
     "dyn_world_set_graphs": (c_int, [c_void_p, c_int]),
     "dyn_world_read_work": (c_int, [c_void_p, ctypes.POINTER(c_u32)]),
     "dyn_fixed_grid_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),

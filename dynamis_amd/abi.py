@@ -158,6 +158,25 @@ assert STEP_PARAMS.itemsize == 176 and BODY_STATE.itemsize == 144
 assert BODY_DESC.itemsize == 128 and COLLIDER.itemsize == 112
 assert MANIFOLD_POINT.itemsize == 64 and CONTACT.itemsize == 352
 
+# records.wgsl:193-199 BodyEdit (160 bytes)
+BODY_EDIT = np.dtype([("kind", "<u4"), ("mask", "<u4"), ("_pad", "<u4", 2), ("state", BODY_STATE)])
+assert BODY_EDIT.itemsize == 160
+
+# body edit kinds and patch fields (crates/dynamis-abi/src/constant.rs:6-18)
+EDIT_PATCH = 0
+EDIT_FORCE = 1
+EDIT_TORQUE = 2
+EDIT_IMPULSE = 3
+EDIT_SLEEP = 4
+EDIT_WAKE = 5
+EDIT_FORCE_AT_POINT = 6
+EDIT_IMPULSE_AT_POINT = 7
+EDIT_ANGULAR_IMPULSE = 8
+PATCH_POSITION = 1
+PATCH_VELOCITY = 2
+PATCH_ORIENTATION = 4
+PATCH_ANGULAR_VELOCITY = 8
+
 # shape kinds and body flags, the values of include/dyn_abi.h
 SHAPE_SPHERE = 1
 SHAPE_CUBOID = 2

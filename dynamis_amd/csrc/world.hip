@@ -127,6 +127,11 @@ struct W {
     uint32_t* scan_blocks;   // per-block sums of compaction scans
     uint32_t* scan_total;    // two words: totals of the last scans
     uint32_t* relay_src;     // while a tape is recorded: per contact point, the archive point relayed into it
+    // Body edits queued by dyn_world_edit for the next step: runs (row, first edit, edit count)
+    // sorted by row over the edits in queue order; *edit_runs_count runs (0: no edits).
+    const dyn_body_edit* edits;
+    const uint4* edit_runs;
+    uint32_t* edit_runs_count;
 };
 
 __device__ __forceinline__ Body load_body(const W& w, uint32_t row) {
@@ -143,7 +148,97 @@ __global__ void k_reset_counters(W w) {
     if (t < kPerStepCount) *cw(w.counters, kPerStepSlots[t]) = 0u;
 }
 
-// clear_inputs.wgsl, activity.wgsl (all rows), begin_step.wgsl (dynamic rows)
+// body_edits.wgsl:10-18 mark_disturbed
+__device__ __forceinline__ void mark_disturbed(const W& w, dyn_body_state& s, uint32_t row, bool disturbed) {
+    if (s.sleeping != 0u) {
+        atomicAdd(cw(w.counters, DYN_COUNTER_WOKE), 1u);
+        atomicOr(&w.wake[row], 1u);
+    } else if (disturbed) {
+        atomicOr(&w.wake[row], 1u);
+    }
+    s.sleeping = 0u;
+    s.sleep_timer = 0.0f;
+}
+
+// body_edits.wgsl:34-101: the row's queued edits, in queue order, on its state. (The reference also
+// clears the row's admission to the resting grid region, body_admitted; this build emits every
+// collider into one region each step, so there is no admission to clear.)
+__device__ inline void apply_edits(const W& w, uint32_t row, dyn_body_state& s, const dyn_body_desc& d) {
+    const uint32_t runs = *w.edit_runs_count;
+    if (runs == 0u) return;
+    uint32_t lo = 0u, hi = runs;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (w.edit_runs[mid].x < row) lo = mid + 1u;
+        else hi = mid;
+    }
+    if (lo >= runs || w.edit_runs[lo].x != row) return;
+    const uint4 run = w.edit_runs[lo];
+    Body b;
+    b.d = d;
+    for (uint32_t e = 0; e < run.z; ++e) {
+        const dyn_body_edit& edit = w.edits[run.y + e];
+        const dyn_body_state& es = edit.state;
+        if (edit.kind == DYN_EDIT_PATCH) {
+            if (edit.mask & DYN_PATCH_POSITION) {
+                for (int c = 0; c < 3; ++c) s.position[c] = s.prev_position[c] = es.position[c];
+            }
+            if (edit.mask & DYN_PATCH_ORIENTATION) {
+                for (int c = 0; c < 4; ++c) s.orientation[c] = s.prev_orientation[c] = es.orientation[c];
+            }
+            if (edit.mask & DYN_PATCH_VELOCITY) {
+                for (int c = 0; c < 3; ++c) s.velocity[c] = es.velocity[c];
+            }
+            if (edit.mask & DYN_PATCH_ANGULAR_VELOCITY) {
+                for (int c = 0; c < 3; ++c) s.angular_velocity[c] = es.angular_velocity[c];
+            }
+            const bool teleport = (edit.mask & (DYN_PATCH_POSITION | DYN_PATCH_ORIENTATION)) != 0u;
+            mark_disturbed(w, s, row, teleport || !kind_simulates(d));
+        } else if (edit.kind == DYN_EDIT_FORCE) {
+            st3(s.force, add(ld3(s.force), ld3(es.force)));
+            mark_disturbed(w, s, row, false);
+        } else if (edit.kind == DYN_EDIT_FORCE_AT_POINT) {
+            b.s = s;
+            st3(s.force, add(ld3(s.force), ld3(es.force)));
+            st3(s.torque, add(ld3(s.torque), cross3(sub(ld3(es.position), body_com(b)), ld3(es.force))));
+            mark_disturbed(w, s, row, false);
+        } else if (edit.kind == DYN_EDIT_TORQUE) {
+            st3(s.torque, add(ld3(s.torque), ld3(es.torque)));
+            mark_disturbed(w, s, row, false);
+        } else if (edit.kind == DYN_EDIT_IMPULSE) {
+            st3(s.velocity, add(ld3(s.velocity), mul(ld3(es.velocity), d.inverse_mass)));
+            mark_disturbed(w, s, row, false);
+        } else if (edit.kind == DYN_EDIT_IMPULSE_AT_POINT) {
+            const v3 impulse = ld3(es.velocity);
+            st3(s.velocity, add(ld3(s.velocity), mul(impulse, d.inverse_mass)));
+            b.s = s;
+            const v3 lever = sub(ld3(es.position), body_com(b));
+            st3(s.angular_velocity, add(ld3(s.angular_velocity), apply_inverse_inertia(b, cross3(lever, impulse))));
+            mark_disturbed(w, s, row, false);
+        } else if (edit.kind == DYN_EDIT_ANGULAR_IMPULSE) {
+            b.s = s;
+            st3(s.angular_velocity, add(ld3(s.angular_velocity), apply_inverse_inertia(b, ld3(es.angular_velocity))));
+            mark_disturbed(w, s, row, false);
+        } else if (edit.kind == DYN_EDIT_SLEEP) {
+            if (s.sleeping == 0u) atomicAdd(cw(w.counters, DYN_COUNTER_SLEPT), 1u);
+            for (int c = 0; c < 3; ++c) {  // core.wgsl:218-227 freeze_body
+                s.velocity[c] = s.angular_velocity[c] = s.force[c] = s.torque[c] = 0.0f;
+                s.prev_position[c] = s.position[c];
+            }
+            for (int c = 0; c < 4; ++c) s.prev_orientation[c] = s.orientation[c];
+            s.sleep_timer = 0.0f;
+            s.sleeping = 1u;
+            atomic_store_agent(&w.wake[row], 0u);
+        } else if (edit.kind == DYN_EDIT_WAKE) {
+            if (s.sleeping != 0u) atomicAdd(cw(w.counters, DYN_COUNTER_WOKE), 1u);
+            s.sleep_timer = 0.0f;
+            s.sleeping = 0u;
+            atomicOr(&w.wake[row], 1u);
+        }
+    }
+}
+
+// clear_inputs.wgsl, body_edits.wgsl, activity.wgsl (all rows), begin_step.wgsl (dynamic rows)
 __global__ void __launch_bounds__(kThreads) k_begin(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool awake = false;
@@ -152,6 +247,7 @@ __global__ void __launch_bounds__(kThreads) k_begin(W w) {
         const dyn_body_desc d = w.de[i];
         s.force[0] = s.force[1] = s.force[2] = 0.0f;
         s.torque[0] = s.torque[1] = s.torque[2] = 0.0f;
+        apply_edits(w, i, s, d);
         awake = body_is_active(s, d);
         w.activity[i] = awake ? 1u : 0u;
         bool moving;
@@ -1913,7 +2009,10 @@ __global__ void __launch_bounds__(kThreads) k_rest_place(W w, const uint32_t* co
 }
 
 __global__ void k_consume(W w) {
-    if (threadIdx.x == 0) *cw(w.counters, DYN_COUNTER_STEP) += 1u;
+    if (threadIdx.x == 0) {
+        *cw(w.counters, DYN_COUNTER_STEP) += 1u;
+        *w.edit_runs_count = 0u;  // the step applied the queued edits
+    }
 }
 
 }  // namespace
@@ -1957,6 +2056,15 @@ struct dyn_world {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     float graph_dt[2] = {0.0f, 0.0f};
     uint32_t parity = 0;
+    // body edits queued since the last step (dyn_world_edit): row and edit in queue order, and the
+    // device buffers they are uploaded to (capacities in edits and runs)
+    std::vector<uint32_t> host_row_of_body;
+    std::vector<std::pair<uint32_t, dyn_body_edit>> pending_edits;
+    std::vector<dyn_body_edit> edit_upload;
+    std::vector<uint4> run_upload;
+    dyn_body_edit* edit_buf = nullptr;
+    uint4* run_buf = nullptr;
+    uint32_t edit_cap = 0, run_cap = 0;
     // the tape a recorded step writes (dyn::world_record); null when not recording
     float* tape_states = nullptr;
     float* tape_contacts = nullptr;
@@ -2382,6 +2490,17 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     w.ncontacts = world->words + 1;
     w.narchive = world->words + 2;
     w.nresting = world->words + 3;
+    w.edit_runs_count = world->words + 5;
+    world->host_row_of_body = row_of_body;
+    world->edit_cap = 64u;
+    world->run_cap = 64u;
+    if (hipMalloc(&world->edit_buf, sizeof(dyn_body_edit) * world->edit_cap) != hipSuccess ||
+        hipMalloc(&world->run_buf, sizeof(uint4) * world->run_cap) != hipSuccess) {
+        if (world->edit_buf) (void)hipFree(world->edit_buf);
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: edit buffers"));
+    }
+    w.edits = world->edit_buf;
+    w.edit_runs = world->run_buf;
     world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(collider_count, world->entry_cap, pair_capacity);
     world->sort_scratch_bytes = dyn_sort_scratch_bytes(contact_capacity, 2u * dyn_sort_key_words(id_cap ? id_cap : 1u));
     if (hipMalloc(&world->bp_scratch, world->bp_scratch_bytes) != hipSuccess ||
@@ -2487,13 +2606,73 @@ int dyn_world_destroy(dyn_world* world) {
     for (void* p : world->owned) (void)hipFree(p);
     if (world->shape_sources) (void)hipFree(world->shape_sources);
     if (world->shape_vertices) (void)hipFree(world->shape_vertices);
+    if (world->edit_buf) (void)hipFree(world->edit_buf);
+    if (world->run_buf) (void)hipFree(world->run_buf);
     if (world->own_stream) (void)hipStreamDestroy(world->stream);
     delete world;
     return DYN_OK;
 }
 
+int dyn_world_edit(dyn_world* world, const uint32_t* body_ids, const dyn_body_edit* edits, uint32_t count) {
+    using namespace dyn;
+    if (!world) return fail(DYN_E_INVALID, "dyn_world_edit: null world");
+    if (count == 0u) return DYN_OK;
+    if (!body_ids || !edits) return fail(DYN_E_INVALID, "dyn_world_edit: null argument");
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t id = body_ids[i];
+        if (id >= world->host_row_of_body.size() || world->host_row_of_body[id] == DYN_NO_BODY) {
+            return fail(DYN_E_INVALID, "dyn_world_edit: body id names no body");
+        }
+        if (edits[i].kind > DYN_EDIT_ANGULAR_IMPULSE) return fail(DYN_E_INVALID, "dyn_world_edit: unknown edit kind");
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        world->pending_edits.emplace_back(world->host_row_of_body[body_ids[i]], edits[i]);
+    }
+    // runs in row order, each row's edits in queue order (command.rs compiles one run per body)
+    std::vector<std::pair<uint32_t, dyn_body_edit>> ordered = world->pending_edits;
+    std::stable_sort(ordered.begin(), ordered.end(),
+                     [](const std::pair<uint32_t, dyn_body_edit>& a, const std::pair<uint32_t, dyn_body_edit>& b) {
+                         return a.first < b.first;
+                     });
+    world->edit_upload.clear();
+    world->run_upload.clear();
+    for (uint32_t i = 0; i < ordered.size(); ++i) {
+        if (world->run_upload.empty() || world->run_upload.back().x != ordered[i].first) {
+            world->run_upload.push_back(make_uint4(ordered[i].first, i, 0u, 0u));
+        }
+        world->run_upload.back().z += 1u;
+        world->edit_upload.push_back(ordered[i].second);
+    }
+    const uint32_t edits_n = static_cast<uint32_t>(world->edit_upload.size());
+    const uint32_t runs_n = static_cast<uint32_t>(world->run_upload.size());
+    if (edits_n > world->edit_cap || runs_n > world->run_cap) {
+        // the captured step holds the old buffers' addresses
+        DYN_HIP(hipStreamSynchronize(world->stream));
+        drop_graph(world);
+        while (world->edit_cap < edits_n) world->edit_cap *= 2u;
+        while (world->run_cap < runs_n) world->run_cap *= 2u;
+        DYN_HIP(hipFree(world->edit_buf));
+        DYN_HIP(hipFree(world->run_buf));
+        world->edit_buf = nullptr;
+        world->run_buf = nullptr;
+        DYN_HIP(hipMalloc(&world->edit_buf, sizeof(dyn_body_edit) * world->edit_cap));
+        DYN_HIP(hipMalloc(&world->run_buf, sizeof(uint4) * world->run_cap));
+        world->w.edits = world->edit_buf;
+        world->w.edit_runs = world->run_buf;
+    }
+    hipStream_t st = world->stream;
+    DYN_HIP(hipMemcpyAsync(world->edit_buf, world->edit_upload.data(), sizeof(dyn_body_edit) * edits_n,
+                           hipMemcpyHostToDevice, st));
+    DYN_HIP(hipMemcpyAsync(world->run_buf, world->run_upload.data(), sizeof(uint4) * runs_n, hipMemcpyHostToDevice, st));
+    DYN_HIP(hipMemcpyAsync(world->w.edit_runs_count, &runs_n, 4u, hipMemcpyHostToDevice, st));
+    // the host copies above are read when the copies run; keep them until the step is enqueued
+    DYN_HIP(hipStreamSynchronize(st));
+    return DYN_OK;
+}
+
 int dyn_world_step(dyn_world* world, float dt) {
     if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_step: null world");
+    world->pending_edits.clear();  // the step applies (and consumes) every queued edit
     if (!(dt > 0.0f)) return dyn::fail(DYN_E_INVALID, "dyn_world_step: dt must be positive");
     if (world->hull_colliders && world->shape_sources == nullptr) {
         return dyn::fail(DYN_E_INVALID, "dyn_world_step: hull colliders need dyn_world_upload_shapes first");

@@ -234,7 +234,70 @@ def default_capacities(bodies: int, contacts: int | None = None,
     return c, p
 
 
-class World:
+def body_edit(kind: int, mask: int = 0, position=None, orientation=None, velocity=None,
+              angular_velocity=None, force=None, torque=None) -> np.ndarray:
+    """One BODY_EDIT record (records.wgsl:193-199): the fields a kind reads (body_edits.wgsl)."""
+    e = np.zeros(1, dtype=abi.BODY_EDIT)
+    e["kind"] = kind
+    e["mask"] = mask
+    for name, value in (("position", position), ("orientation", orientation), ("velocity", velocity),
+                        ("angular_velocity", angular_velocity), ("force", force), ("torque", torque)):
+        if value is not None:
+            e["state"][name] = value
+    return e
+
+
+class BodyCommands:
+    """The body commands of the reference's World (crates/dynamis-world/src/command.rs and
+    body.rs: set_velocity, set_position, apply_force, apply_impulse, sleep, wake, ...), each
+    queued as a body edit for the next step. `_queue_edits(ids, edits)` queues the records;
+    `_body_id(handle)` maps a handle to its body id."""
+
+    def _body_id(self, handle: int) -> int:
+        return int(handle)
+
+    def edit(self, handle: int, record: np.ndarray) -> None:
+        ids = np.array([self._body_id(handle)], dtype=np.uint32)
+        self._queue_edits(ids, np.ascontiguousarray(record, dtype=abi.BODY_EDIT).reshape(1))
+
+    def set_position(self, handle, position) -> None:
+        self.edit(handle, body_edit(abi.EDIT_PATCH, abi.PATCH_POSITION, position=position))
+
+    def set_orientation(self, handle, orientation) -> None:
+        self.edit(handle, body_edit(abi.EDIT_PATCH, abi.PATCH_ORIENTATION, orientation=orientation))
+
+    def set_velocity(self, handle, velocity) -> None:
+        self.edit(handle, body_edit(abi.EDIT_PATCH, abi.PATCH_VELOCITY, velocity=velocity))
+
+    def set_angular_velocity(self, handle, angular_velocity) -> None:
+        self.edit(handle, body_edit(abi.EDIT_PATCH, abi.PATCH_ANGULAR_VELOCITY, angular_velocity=angular_velocity))
+
+    def apply_force(self, handle, force) -> None:
+        self.edit(handle, body_edit(abi.EDIT_FORCE, force=force))
+
+    def apply_force_at_point(self, handle, force, point) -> None:
+        self.edit(handle, body_edit(abi.EDIT_FORCE_AT_POINT, force=force, position=point))
+
+    def apply_torque(self, handle, torque) -> None:
+        self.edit(handle, body_edit(abi.EDIT_TORQUE, torque=torque))
+
+    def apply_impulse(self, handle, impulse) -> None:
+        self.edit(handle, body_edit(abi.EDIT_IMPULSE, velocity=impulse))
+
+    def apply_impulse_at_point(self, handle, impulse, point) -> None:
+        self.edit(handle, body_edit(abi.EDIT_IMPULSE_AT_POINT, velocity=impulse, position=point))
+
+    def apply_angular_impulse(self, handle, impulse) -> None:
+        self.edit(handle, body_edit(abi.EDIT_ANGULAR_IMPULSE, angular_velocity=impulse))
+
+    def sleep(self, handle) -> None:
+        self.edit(handle, body_edit(abi.EDIT_SLEEP))
+
+    def wake(self, handle) -> None:
+        self.edit(handle, body_edit(abi.EDIT_WAKE))
+
+
+class World(BodyCommands):
     """A device world. Bodies are spawned before the first step; `step(dt)` enqueues one step
     on the world's stream, reads wait for it."""
 
@@ -304,6 +367,10 @@ class World:
         out = (ctypes.c_uint32 * 4)()
         call("dyn_world_read_work", self._handle, out)
         return {"gjk": int(out[0]), "manifold": int(out[1]), "epa": int(out[2]), "gjk_hull": int(out[3])}
+
+    def _queue_edits(self, ids: np.ndarray, edits: np.ndarray) -> None:
+        self._ensure()
+        call("dyn_world_edit", self._handle, ids.ctypes.data, edits.ctypes.data, len(ids))
 
     def set_graphs(self, enabled: bool) -> None:
         """Replays each step as a captured HIP graph (the default) or launches it eagerly."""

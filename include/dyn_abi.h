@@ -212,7 +212,21 @@ typedef struct dyn_contact {
     dyn_manifold_point points[4];
 } dyn_contact;
 
+/* BodyEdit, records.wgsl:193-199 (160 bytes): one host command on one body, applied at the start of
+ * the next step (body_edits.wgsl). `kind` is a DYN_EDIT_* value; a DYN_EDIT_PATCH writes the fields
+ * of `state` its `mask` (DYN_PATCH_*) names; the force, torque and impulse kinds read
+ * state.force / state.torque / state.velocity (linear impulse) / state.angular_velocity (angular
+ * impulse) and, for the _AT_POINT kinds, state.position (the world point). */
+typedef struct dyn_body_edit {
+    uint32_t kind;
+    uint32_t mask;
+    uint32_t _pad0;
+    uint32_t _pad1;
+    dyn_body_state state;
+} dyn_body_edit;
+
 #if defined(__cplusplus)
+static_assert(sizeof(dyn_body_edit) == 160, "BodyEdit is 160 bytes");
 static_assert(sizeof(dyn_step_params) == 176, "StepParams is 176 bytes");
 static_assert(sizeof(dyn_body_state) == 144, "BodyState is 144 bytes");
 static_assert(sizeof(dyn_body_desc) == 128, "BodyDescriptor is 128 bytes");
@@ -223,6 +237,7 @@ static_assert(sizeof(dyn_grid_entry) == 32, "GridEntry is 32 bytes");
 static_assert(sizeof(dyn_manifold_point) == 64, "ManifoldPoint is 64 bytes");
 static_assert(sizeof(dyn_contact) == 352, "Contact is 352 bytes");
 #else
+_Static_assert(sizeof(dyn_body_edit) == 160, "BodyEdit is 160 bytes");
 _Static_assert(sizeof(dyn_step_params) == 176, "StepParams is 176 bytes");
 _Static_assert(sizeof(dyn_body_state) == 144, "BodyState is 144 bytes");
 _Static_assert(sizeof(dyn_body_desc) == 128, "BodyDescriptor is 128 bytes");
@@ -235,6 +250,21 @@ _Static_assert(sizeof(dyn_contact) == 352, "Contact is 352 bytes");
 #endif
 
 /* ---- constants (crates/dynamis-abi/src/constant.rs) ---------------------------------------- */
+
+/* body edit kinds and patch fields (constant.rs:6-18) */
+#define DYN_EDIT_PATCH 0u
+#define DYN_EDIT_FORCE 1u
+#define DYN_EDIT_TORQUE 2u
+#define DYN_EDIT_IMPULSE 3u
+#define DYN_EDIT_SLEEP 4u
+#define DYN_EDIT_WAKE 5u
+#define DYN_EDIT_FORCE_AT_POINT 6u
+#define DYN_EDIT_IMPULSE_AT_POINT 7u
+#define DYN_EDIT_ANGULAR_IMPULSE 8u
+#define DYN_PATCH_POSITION 1u
+#define DYN_PATCH_VELOCITY 2u
+#define DYN_PATCH_ORIENTATION 4u
+#define DYN_PATCH_ANGULAR_VELOCITY 8u
 
 #define DYN_BODY_KINEMATIC 1u
 #define DYN_BODY_CCD 2u

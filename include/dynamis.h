@@ -389,6 +389,16 @@ int dyn_batch_create(dyn_world** out, void* stream, const dyn_step_params* param
  * the vertex array, or when a hull collider names a source that is not there. */
 int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
                             uint32_t source_count, const float* vertices, uint32_t vertex_count);
+/* Queues body edits for the next step (replaces the body-edit half of the reference's command
+ * path: World::set_velocity / set_position / apply_force / apply_impulse / sleep / wake compile
+ * into BodyEdit runs, crates/dynamis-world/src/command.rs, applied by body_edits.wgsl in
+ * ApplyCommands, crates/dynamis-rigid/src/apply_commands.rs:273-365). body_ids[i] names the body
+ * edits[i] acts on; the next dyn_world_step applies every queued edit after clearing the bodies'
+ * inputs and before their activity flags, a body's edits in the order they were queued (over any
+ * number of calls), exactly as body_edits.wgsl does. The edits reach the device on the world's
+ * stream after the steps already enqueued; the call waits for those steps. Errors: an id naming
+ * no body, an unknown kind. */
+int dyn_world_edit(dyn_world* world, const uint32_t* body_ids, const dyn_body_edit* edits, uint32_t count);
 /* Enqueues one step of `dt` seconds on the world's stream (non-blocking). The step is captured
  * as a HIP graph on its first call for a given dt and replayed afterwards; steps run eagerly
  * while dyn_profile_enable is on or after dyn_world_set_graphs(world, 0). */

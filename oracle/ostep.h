@@ -29,6 +29,9 @@ void oracle_world_destroy(oracle_world* w);
  * -1 when a source exceeds the linear support bound or its vertex range. */
 int oracle_world_shapes(oracle_world* w, const dyn_shape_source* sources, uint32_t source_count,
                         const float* vertices, uint32_t vertex_count);
+/* Queues body edits for the next step (body_edits.wgsl semantics); -1 when an id names no body or
+ * a kind is unknown. */
+int oracle_world_edit(oracle_world* w, const uint32_t* body_ids, const dyn_body_edit* edits, uint32_t count);
 /* One step with the given dt (params.dt / substep_dt are set from it). */
 void oracle_world_step(oracle_world* w, float dt);
 void oracle_world_states(const oracle_world* w, dyn_body_state* out);

@@ -97,6 +97,10 @@ struct oracle_world {
     uint32_t* vdeltas;
     uint32_t* pdeltas;
     uint32_t* contrib;
+    /* body edits queued for the next step (row, edit), in queue order */
+    uint32_t* edit_rows;
+    dyn_body_edit* edits;
+    uint32_t nedits, edit_cap;
     /* the last step's tape: start states, contacts as the solver received them, relay sources */
     dyn_body_state* start;
     dyn_contact* pre;
@@ -2422,6 +2426,119 @@ static void prepare_and_index(oracle_world* w) {
     w->npairs = pairs < w->pair_cap ? pairs : w->pair_cap;
 }
 
+/* body_edits.wgsl:10-18 */
+static void mark_disturbed(oracle_world* w, dyn_body_state* s, uint32_t row, int disturbed) {
+    if (s->sleeping != 0u) {
+        (*ctr(w, DYN_COUNTER_WOKE))++;
+        w->wake[row] |= 1u;
+        w->admitted[row] = 0;
+    } else if (disturbed) {
+        w->wake[row] |= 1u;
+    }
+    s->sleeping = 0u;
+    s->sleep_timer = 0.0f;
+}
+
+/* body_edits.wgsl:34-101, one edit on its row */
+static void apply_edit(oracle_world* w, uint32_t row, const dyn_body_edit* e) {
+    dyn_body_state* s = &w->st[row];
+    const dyn_body_desc* d = &w->de[row];
+    const dyn_body_state* es = &e->state;
+    body_t b;
+    b.d = *d;
+    switch (e->kind) {
+        case DYN_EDIT_PATCH: {
+            if (e->mask & DYN_PATCH_POSITION) {
+                memcpy(s->position, es->position, 12);
+                memcpy(s->prev_position, es->position, 12);
+            }
+            if (e->mask & DYN_PATCH_ORIENTATION) {
+                memcpy(s->orientation, es->orientation, 16);
+                memcpy(s->prev_orientation, es->orientation, 16);
+            }
+            if (e->mask & DYN_PATCH_VELOCITY) memcpy(s->velocity, es->velocity, 12);
+            if (e->mask & DYN_PATCH_ANGULAR_VELOCITY) memcpy(s->angular_velocity, es->angular_velocity, 12);
+            int teleport = (e->mask & (DYN_PATCH_POSITION | DYN_PATCH_ORIENTATION)) != 0u;
+            mark_disturbed(w, s, row, teleport || !kind_simulates(d));
+            break;
+        }
+        case DYN_EDIT_FORCE:
+            vstore(s->force, vadd(vload(s->force), vload(es->force)));
+            mark_disturbed(w, s, row, 0);
+            break;
+        case DYN_EDIT_FORCE_AT_POINT:
+            b.s = *s;
+            vstore(s->force, vadd(vload(s->force), vload(es->force)));
+            vstore(s->torque, vadd(vload(s->torque), vcross(vsub(vload(es->position), body_com(&b)), vload(es->force))));
+            mark_disturbed(w, s, row, 0);
+            break;
+        case DYN_EDIT_TORQUE:
+            vstore(s->torque, vadd(vload(s->torque), vload(es->torque)));
+            mark_disturbed(w, s, row, 0);
+            break;
+        case DYN_EDIT_IMPULSE:
+            vstore(s->velocity, vadd(vload(s->velocity), vmul(vload(es->velocity), d->inverse_mass)));
+            mark_disturbed(w, s, row, 0);
+            break;
+        case DYN_EDIT_IMPULSE_AT_POINT: {
+            v3 impulse = vload(es->velocity);
+            vstore(s->velocity, vadd(vload(s->velocity), vmul(impulse, d->inverse_mass)));
+            b.s = *s;
+            v3 lever = vsub(vload(es->position), body_com(&b));
+            vstore(s->angular_velocity, vadd(vload(s->angular_velocity), apply_inverse_inertia(&b, vcross(lever, impulse))));
+            mark_disturbed(w, s, row, 0);
+            break;
+        }
+        case DYN_EDIT_ANGULAR_IMPULSE:
+            b.s = *s;
+            vstore(s->angular_velocity, vadd(vload(s->angular_velocity), apply_inverse_inertia(&b, vload(es->angular_velocity))));
+            mark_disturbed(w, s, row, 0);
+            break;
+        case DYN_EDIT_SLEEP:
+            if (s->sleeping == 0u) {
+                (*ctr(w, DYN_COUNTER_SLEPT))++;
+                w->admitted[row] = 0;
+            }
+            freeze_body(s);
+            w->wake[row] = 0u;
+            break;
+        case DYN_EDIT_WAKE:
+            if (s->sleeping != 0u) {
+                (*ctr(w, DYN_COUNTER_WOKE))++;
+                w->admitted[row] = 0;
+            }
+            s->sleep_timer = 0.0f;
+            s->sleeping = 0u;
+            w->wake[row] |= 1u;
+            break;
+        default:
+            break;
+    }
+}
+
+/* Queues body edits for the next step (dyn_world_edit); -1 when an id names no body or a kind is
+ * unknown (nothing is queued then). */
+int oracle_world_edit(oracle_world* w, const uint32_t* body_ids, const dyn_body_edit* edits, uint32_t count) {
+    for (uint32_t i = 0; i < count; i++) {
+        uint32_t id = body_ids[i];
+        if (id >= w->id_capacity || w->row_of_body[id] >= w->n) return -1;
+        if (edits[i].kind > DYN_EDIT_ANGULAR_IMPULSE) return -1;
+    }
+    if (w->nedits + count > w->edit_cap) {
+        uint32_t cap = w->edit_cap ? w->edit_cap : 16u;
+        while (cap < w->nedits + count) cap *= 2u;
+        w->edits = realloc(w->edits, (size_t)cap * sizeof *w->edits);
+        w->edit_rows = realloc(w->edit_rows, (size_t)cap * sizeof *w->edit_rows);
+        w->edit_cap = cap;
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        w->edit_rows[w->nedits] = w->row_of_body[body_ids[i]];
+        w->edits[w->nedits] = edits[i];
+        w->nedits++;
+    }
+    return 0;
+}
+
 void oracle_world_step(oracle_world* w, float dt) {
     w->p.dt = dt;
     w->p.substep_dt = dt / (float)w->p.substeps;
@@ -2431,6 +2548,10 @@ void oracle_world_step(oracle_world* w, float dt) {
         memset(w->st[i].force, 0, 12);
         memset(w->st[i].torque, 0, 12);
     }
+    /* body_edits.wgsl: the queued edits, each body's in queue order (edits of different bodies
+     * commute) */
+    for (uint32_t i = 0; i < w->nedits; i++) apply_edit(w, w->edit_rows[i], &w->edits[i]);
+    w->nedits = 0;
     memcpy(w->start, w->st, (size_t)w->n * sizeof *w->st);
     for (uint32_t i = 0; i < w->n; i++) {
         int awake = body_is_active(&w->st[i], &w->de[i]);
@@ -2653,7 +2774,7 @@ void oracle_world_destroy(oracle_world* w) {
                     w->raw, w->valid, w->contacts, w->matched, w->archive, w->resting,
                     w->parents, w->istate, w->live, w->block_counts, w->blocks, w->rows,
                     w->targets, w->resolution, w->vdeltas, w->pdeltas, w->contrib,
-                    w->start, w->pre, w->relay_src};
+                    w->start, w->pre, w->relay_src, w->edits, w->edit_rows};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(w);
 }

@@ -140,3 +140,28 @@ def oracle_tape(scene, steps: int, dt: float) -> tuple[Tape, list[np.ndarray]]:
         after.append(world.states())
     world.close()
     return Tape(params, ranges, bodies, contacts, starts, len(scene.colliders), cap), after
+
+
+def differences(f, h: float, unit: float = 1.0) -> tuple[float, float, float]:
+    """Central, forward and backward differences of f (the loss at the parameter moved by e) at
+    step h, divided by `unit` (a parameter's scale, for derivatives taken in a scaled variable)."""
+    lp, l0, lm = f(h), f(0.0), f(-h)
+    return (lp - lm) / (2 * h) / unit, (lp - l0) / h / unit, (l0 - lm) / h / unit
+
+
+def classify(dev: float, diffs: tuple[float, float, float], scale: float, tol: float = 1e-3) -> str:
+    """'smooth' when the central difference agrees with `dev` within tol (relative, floored at 1e-2 of
+    the largest derivative, `scale`); 'kink' when the parameter sits at a kink of the piecewise-smooth
+    rollout within h: the forward and backward differences differ by more than tol, and `dev` (the
+    slope of the branch the recording selects) agrees with one of them within tol; else 'off'."""
+    central, forward, backward = diffs
+    floor = 1e-2 * scale
+
+    def rel(a, b):
+        return abs(a - b) / max(abs(b), floor)
+
+    if rel(dev, central) <= tol:
+        return "smooth"
+    if rel(forward, backward) > tol and min(rel(dev, forward), rel(dev, backward)) <= tol:
+        return "kink"
+    return "off"

@@ -13,6 +13,7 @@ import subprocess
 import numpy as np
 
 from dynamis_amd import abi
+from dynamis_amd.world import BodyCommands
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 ORACLE_DIR = ROOT / "oracle"
@@ -58,6 +59,8 @@ class Oracle:
         h.oracle_world_states.argtypes = [_p, _p]
         h.oracle_world_contacts.restype = _u32
         h.oracle_world_contacts.argtypes = [_p, _p, _u32]
+        h.oracle_world_edit.restype = ctypes.c_int
+        h.oracle_world_edit.argtypes = [_p, _p, _p, _u32]
         h.oracle_world_resting.restype = _u32
         h.oracle_world_resting.argtypes = [_p, _p, _u32]
         h.oracle_world_counters.restype = None
@@ -171,7 +174,7 @@ class Oracle:
         return OracleWorld(self, scene, contact_capacity, pair_capacity)
 
 
-class OracleWorld:
+class OracleWorld(BodyCommands):
     """The oracle's serial restatement of World::step over the same device records."""
 
     def __init__(self, oracle: Oracle, scene, contact_capacity=None, pair_capacity=None):
@@ -210,6 +213,10 @@ class OracleWorld:
         out = np.zeros(self.capacity, dtype=abi.CONTACT)
         count = self.o.h.oracle_world_contacts(self.w, out.ctypes.data, self.capacity)
         return out[:min(count, self.capacity)].copy()
+
+    def _queue_edits(self, ids: np.ndarray, edits: np.ndarray) -> None:
+        if self.o.h.oracle_world_edit(self.w, ids.ctypes.data, edits.ctypes.data, len(ids)) != 0:
+            raise ValueError("oracle_world_edit refused the edits")
 
     def resting_contacts(self) -> np.ndarray:
         out = np.zeros(self.capacity, dtype=abi.CONTACT)
