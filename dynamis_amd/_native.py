@@ -92,6 +92,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_adjoint_read_bodies": (c_int, [c_void_p, c_void_p]),
     "dyn_adjoint_destroy": (c_int, [c_void_p]),
     "dyn_world_wait": (c_int, [c_void_p]),
+    "dyn_world_capacity": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dyn_world_reserve": (c_int, [c_void_p, c_u32, c_u32]),
     "dyn_world_stream": (c_void_p, [c_void_p]),
     "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),
     "dyn_world_read_counters": (c_int, [c_void_p, c_void_p]),

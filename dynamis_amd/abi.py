@@ -41,6 +41,21 @@ COUNTER_NAMES = [
     "SOFT_EVENTS", "REFUSED_SOFT_EVENTS",
 ]
 COUNTER = {name: slot for slot, name in enumerate(COUNTER_NAMES)}
+# The counters that refuse work, with their labels and shortfall classes (counter.rs:75-128):
+# "physics" refusals change the step's result, "report" ones only what it reports, "fatal" ones
+# are faults a step cannot lose.
+REFUSING_COUNTERS = {
+    "REFUSED_PAIRS": ("candidate pairs beyond the pair stream", "physics"),
+    "REFUSED_EVENTS": ("contact events beyond the event stream", "report"),
+    "ENTRY_FAULTS": ("grid entries beyond their budget", "fatal"),
+    "REFUSED_RESTING": ("resting manifolds beyond the resting archive", "physics"),
+    "ISLAND_FAULTS": ("island unions beyond their hook budget", "fatal"),
+    "LIVE_FAULTS": ("live body rows beyond the row stream", "fatal"),
+    "ROW_FAULTS": ("row moves that answer no state their row could hold", "fatal"),
+    "REFUSED_CONTACTS": ("contact manifolds beyond the contact stream", "physics"),
+    "REFUSED_IMPACTS": ("impacts beyond the impact stream", "report"),
+    "REFUSED_SOFT_EVENTS": ("soft contact events beyond the soft event stream", "report"),
+}
 COUNTER_DEVICE_COUNT = len(COUNTER_NAMES)
 COUNTER_STRIDE_WORDS = 64
 COUNTER_BLOCK_WORDS = COUNTER_DEVICE_COUNT * COUNTER_STRIDE_WORDS

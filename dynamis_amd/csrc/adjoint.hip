@@ -14,6 +14,7 @@
 namespace dyn {
 hipStream_t world_stream(dyn_world* world);
 void world_sizes(dyn_world* world, uint32_t* n, uint32_t* d, uint32_t* k, uint32_t* cap);
+void world_lock_growth(dyn_world* world, int delta);
 const dyn_step_params& world_params(dyn_world* world);
 const dyn_body_state* world_states(dyn_world* world);
 const dyn_body_desc* world_descs(dyn_world* world);
@@ -276,6 +277,7 @@ struct dyn_adjoint {
     float* d_accb[2] = {nullptr, nullptr};
     float* d_local_contacts = nullptr;  // cap x 76: renumbered contact records of worlds whose records stay out of LDS
     uint32_t lds_bytes = 0;             // dynamic LDS of k_adj_step
+    bool locked = false;                // holds the world's capacity fixed
     Work<float> W{};
     std::vector<void*> owned;
 };
@@ -300,6 +302,7 @@ int zero(dyn_adjoint* a, float* p, size_t count) {
 }
 
 void destroy(dyn_adjoint* a) {
+    if (a->locked) dyn::world_lock_growth(a->world, -1);
     for (void* p : a->owned) (void)hipFree(p);
     delete a;
 }
@@ -318,6 +321,10 @@ int dyn_adjoint_create(dyn_adjoint** out, dyn_world* world, uint32_t steps, uint
     dyn_adjoint* a = new dyn_adjoint();
     a->world = world;
     a->stream = world_stream(world);
+    // the tapes, snapshots and relay sources are sized by the world's contact capacity: the
+    // world's streams keep their size while this rollout exists
+    world_lock_growth(world, 1);
+    a->locked = true;
     a->steps = steps;
     a->every = checkpoint_every < steps ? checkpoint_every : steps;
     a->segments = (steps + a->every - 1) / a->every;

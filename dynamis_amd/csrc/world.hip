@@ -2065,6 +2065,22 @@ struct dyn_world {
     dyn_body_edit* edit_buf = nullptr;
     uint4* run_buf = nullptr;
     uint32_t edit_cap = 0, run_cap = 0;
+    // Capacity planning (crates/dynamis-world/src/backend/mod.rs:194-230 apply_plan;
+    // crates/dynamis-broadphase/src/capacity.rs, crates/dynamis-rigid/src/capacity.rs): after
+    // every step its demand (the uncapped pair, contact and resting totals) is copied to pinned
+    // host memory; before step s the demand of step s - kPlanLag is read (waiting for that step
+    // if it has not ended), and a demand above a stream's capacity grows the stream to
+    // max(demand, 2 x capacity) (dynamis-domain capacity.rs grown). A fixed lag keeps the
+    // trajectory independent of host timing. The per-step streams (pairs, contacts and their
+    // solver and compaction arrays) live in one group that growth reallocates; the durable ones
+    // (archive, resting archive and its index) are copied over.
+    std::vector<void*> streams;       // the group's allocations
+    uint32_t* snap_header = nullptr;  // device: snapshot magic, contact capacity, rows, colliders
+    uint32_t* plan_host = nullptr;    // pinned: kPlanRing demand records of kPlanWords words
+    hipEvent_t plan_events[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t issued = 0;              // dyn_world_step calls since create
+    uint32_t growth_locks = 0;        // holders (differentiable rollouts) whose buffers fix the capacity
+    uint32_t grows = 0;
     // the tape a recorded step writes (dyn::world_record); null when not recording
     float* tape_states = nullptr;
     float* tape_contacts = nullptr;
@@ -2075,13 +2091,149 @@ struct dyn_world {
 namespace {
 
 template <typename T>
-int alloc(dyn_world* world, T** ptr, size_t count) {
+int alloc_in(std::vector<void*>& owned, T** ptr, size_t count) {
     void* p = nullptr;
     DYN_HIP(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
+    owned.push_back(p);
     DYN_HIP(hipMemset(p, 0, (count ? count : 1) * sizeof(T)));
-    world->owned.push_back(p);
     *ptr = static_cast<T*>(p);
     return DYN_OK;
+}
+
+template <typename T>
+int alloc(dyn_world* world, T** ptr, size_t count) {
+    return alloc_in(world->owned, ptr, count);
+}
+
+constexpr uint32_t kPlanLag = 2;    // steps between a step and the plan that reads its demand
+constexpr uint32_t kPlanRing = 4;   // demand records in flight (> kPlanLag)
+// The counter words a demand record copies: slots PAIRS (10) through REFUSED_RESTING (20).
+constexpr uint32_t kPlanFirst = DYN_COUNTER_PAIRS;
+constexpr uint32_t kPlanWords = (DYN_COUNTER_REFUSED_RESTING - kPlanFirst) * DYN_COUNTER_STRIDE_WORDS + 1u;
+static_assert(DYN_COUNTER_CONTACTS > kPlanFirst && DYN_COUNTER_RESTING > kPlanFirst &&
+                  DYN_COUNTER_REFUSED_RESTING > DYN_COUNTER_RESTING,
+              "the demand record spans the pair, contact and resting counters");
+
+// The per-step streams sized by the pair and contact capacities, allocated as one group (growth
+// frees and reallocates the whole group). Durable contents are the caller's to copy.
+int alloc_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capacity) {
+    using namespace dyn;
+    W& w = world->w;
+    std::vector<void*>& g = world->streams;
+    const uint32_t body_count = w.n;
+    int rc = DYN_OK;
+#define A(ptr, count) if (rc == DYN_OK) rc = alloc_in(g, &(ptr), (count))
+    A(w.pmaj, pair_capacity);
+    A(w.pmin, pair_capacity);
+    A(w.raw, pair_capacity);
+    A(w.valid, pair_capacity);
+    A(w.pkind, pair_capacity);
+    A(w.worklist, pair_capacity);
+    A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
+    A(w.work_keys, pair_capacity);
+    A(w.work_minor, pair_capacity);
+    A(w.dest, pair_capacity);
+    A(w.contacts, contact_capacity);
+    A(w.ckeys, contact_capacity);
+    A(w.matched, contact_capacity);
+    A(w.archive, contact_capacity);
+    A(w.resting, contact_capacity);
+    A(w.rest_tmp, contact_capacity);
+    A(w.rest_major, contact_capacity);
+    A(w.rest_minor, contact_capacity);
+    A(w.rest_slot, contact_capacity);
+    A(w.rest_flags, contact_capacity);
+    A(w.blocks, 2u * static_cast<size_t>(contact_capacity));
+    A(w.targets, 4u * static_cast<size_t>(contact_capacity));
+    A(w.prows, 3u * static_cast<size_t>(contact_capacity));
+    A(w.srows, static_cast<size_t>(kSolveRow) * contact_capacity);
+    A(w.simp, 3u * static_cast<size_t>(contact_capacity));
+    A(w.slots, 2u * static_cast<size_t>(contact_capacity));
+    A(w.adj, 2u * static_cast<size_t>(contact_capacity));
+    A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
+    A(w.pwords, 2u * static_cast<size_t>(contact_capacity));
+    uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
+    if (body_count > scan_limit) scan_limit = body_count;  // the rows' list offsets
+    A(w.scan_blocks, ceil_div(scan_limit ? scan_limit : 1u, kScanBlock) + 1u);
+    A(world->freeze_flags, contact_capacity);
+    A(world->keep_dest, contact_capacity);
+    A(world->freeze_dest, contact_capacity);
+#undef A
+    if (rc != DYN_OK) return rc;
+    w.pair_cap = pair_capacity;
+    w.contact_cap = contact_capacity;
+    world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(w.k, world->entry_cap, pair_capacity);
+    world->sort_scratch_bytes = dyn_sort_scratch_bytes(contact_capacity, 2u * dyn_sort_key_words(w.id_cap ? w.id_cap : 1u));
+    world->work_scratch_bytes = dyn_sort_scratch_bytes(pair_capacity, kWorkKeyBytes);
+    for (auto [ptr, bytes] : {std::pair<void**, size_t>{&world->bp_scratch, world->bp_scratch_bytes},
+                              {&world->sort_scratch, world->sort_scratch_bytes},
+                              {&world->work_scratch, world->work_scratch_bytes}}) {
+        *ptr = nullptr;
+        DYN_HIP(hipMalloc(ptr, bytes ? bytes : 1u));
+        g.push_back(*ptr);
+    }
+    return DYN_OK;
+}
+
+void free_streams(std::vector<void*>& owned) {
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+}
+
+void drop_graph(dyn_world* world);
+
+constexpr uint32_t kSnapMagicWord = 0x44594E53u;  // "DYNS", the first word of a snapshot
+
+int write_snap_header(dyn_world* world) {
+    const uint32_t header[4] = {kSnapMagicWord, world->w.contact_cap, world->w.n, world->w.k};
+    DYN_HIP(hipMemcpy(world->snap_header, header, sizeof header, hipMemcpyHostToDevice));
+    return DYN_OK;
+}
+
+// Grows the stream group to the given capacities (never below the current ones): a new group,
+// the durable streams copied into it, the old group freed, the captured graphs dropped (they hold
+// the old pointers and launch shapes).
+int grow_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capacity) {
+    using namespace dyn;
+    W& w = world->w;
+    pair_capacity = std::max(pair_capacity, w.pair_cap);
+    contact_capacity = std::max(contact_capacity, w.contact_cap);
+    if (pair_capacity == w.pair_cap && contact_capacity == w.contact_cap) return DYN_OK;
+    if (contact_capacity > (1u << 30) || pair_capacity > (1u << 30)) {
+        return fail(DYN_E_INVALID, "capacity growth past 2^30 pairs or contacts");
+    }
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    const W old = w;
+    std::vector<void*> old_group;
+    old_group.swap(world->streams);
+    void* old_scratch[3] = {world->bp_scratch, world->sort_scratch, world->work_scratch};
+    (void)old_scratch;
+    const int rc = alloc_streams(world, pair_capacity, contact_capacity);
+    if (rc != DYN_OK) {
+        free_streams(world->streams);
+        world->streams.swap(old_group);  // the world keeps its old streams
+        w = old;
+        world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(w.k, world->entry_cap, w.pair_cap);
+        world->sort_scratch_bytes = dyn_sort_scratch_bytes(w.contact_cap, 2u * dyn_sort_key_words(w.id_cap ? w.id_cap : 1u));
+        world->work_scratch_bytes = dyn_sort_scratch_bytes(w.pair_cap, kWorkKeyBytes);
+        world->bp_scratch = old_scratch[0];
+        world->sort_scratch = old_scratch[1];
+        world->work_scratch = old_scratch[2];
+        return rc;
+    }
+    const size_t cap = old.contact_cap;
+    for (auto [dst, src, bytes] : {std::tuple<void*, const void*, size_t>{w.archive, old.archive, sizeof(dyn_contact) * cap},
+                                   {w.resting, old.resting, sizeof(dyn_contact) * cap},
+                                   {w.rest_major, old.rest_major, 4u * cap},
+                                   {w.rest_minor, old.rest_minor, 4u * cap},
+                                   {w.rest_slot, old.rest_slot, 4u * cap}}) {
+        if (bytes) DYN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, world->stream));
+    }
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    free_streams(old_group);
+    drop_graph(world);
+    ++world->grows;
+    return write_snap_header(world);
 }
 
 uint32_t blocks_for(uint32_t n) { return dyn::ceil_div(n ? n : 1u, dyn::kThreads); }
@@ -2263,6 +2415,42 @@ int capture_step(dyn_world* world, float dt) {
     return DYN_OK;
 }
 
+// After a step: its demand record (the counter words from PAIRS through REFUSED_RESTING) copied
+// to the ring of pinned records, and the event that marks it complete.
+int record_demand(dyn_world* world) {
+    const uint32_t slot = static_cast<uint32_t>(world->issued % kPlanRing);
+    DYN_HIP(hipMemcpyAsync(world->plan_host + static_cast<size_t>(slot) * kPlanWords,
+                           world->w.counters + kPlanFirst * DYN_COUNTER_STRIDE_WORDS, 4u * kPlanWords,
+                           hipMemcpyDeviceToHost, world->stream));
+    DYN_HIP(hipEventRecord(world->plan_events[slot], world->stream));
+    ++world->issued;
+    return DYN_OK;
+}
+
+// Before a step: the demand of the step kPlanLag steps back, and growth of the streams it
+// outgrew (crates/dynamis-domain/src/capacity.rs grown: max(required, 2 x capacity)). The pair
+// demand is the uncapped pair total; the contact demand the larger of the uncapped contact and
+// resting totals (the resting archive shares the contact capacity, rigid/src/capacity.rs:106).
+int plan_capacity(dyn_world* world) {
+    if (world->issued < kPlanLag) return DYN_OK;
+    const uint32_t slot = static_cast<uint32_t>((world->issued - kPlanLag) % kPlanRing);
+    DYN_HIP(hipEventSynchronize(world->plan_events[slot]));
+    const uint32_t* rec = world->plan_host + static_cast<size_t>(slot) * kPlanWords;
+    auto word = [&](uint32_t counter) { return rec[(counter - kPlanFirst) * DYN_COUNTER_STRIDE_WORDS]; };
+    const uint64_t pairs = word(DYN_COUNTER_PAIRS);
+    const uint64_t contacts = std::max<uint64_t>(word(DYN_COUNTER_CONTACTS),
+                                                 static_cast<uint64_t>(word(DYN_COUNTER_RESTING)) +
+                                                     word(DYN_COUNTER_REFUSED_RESTING));
+    const dyn::W& w = world->w;
+    auto grown = [](uint64_t capacity, uint64_t required) -> uint64_t {
+        return required <= capacity ? capacity : std::max(required, 2u * capacity);
+    };
+    const uint64_t np = std::min<uint64_t>(grown(w.pair_cap, pairs), 1u << 30);
+    const uint64_t nc = std::min<uint64_t>(grown(w.contact_cap, contacts), 1u << 30);
+    if ((np == w.pair_cap && nc == w.contact_cap) || world->growth_locks) return DYN_OK;
+    return grow_streams(world, static_cast<uint32_t>(np), static_cast<uint32_t>(nc));
+}
+
 int world_step(dyn_world* world, float dt) {
     using namespace dyn;
     W& w = world->w;
@@ -2387,6 +2575,11 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     dyn_world* world = new dyn_world();
     auto cleanup = [&](int rc) {
         for (void* p : world->owned) (void)hipFree(p);
+        free_streams(world->streams);
+        if (world->plan_host) (void)hipHostFree(world->plan_host);
+        for (hipEvent_t& e : world->plan_events) {
+            if (e) (void)hipEventDestroy(e);
+        }
         if (world->own_stream && world->stream) (void)hipStreamDestroy(world->stream);
         delete world;
         return rc;
@@ -2436,55 +2629,21 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.aabbs, collider_count);
     A(w.res, collider_count);
     A(w.flags, collider_count);
-    A(w.pmaj, pair_capacity);
-    A(w.pmin, pair_capacity);
-    A(w.raw, pair_capacity);
-    A(w.valid, pair_capacity);
-    A(w.pkind, pair_capacity);
-    A(w.worklist, pair_capacity);
-    A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
-    A(w.work_total, 4);
-    A(w.work_keys, pair_capacity);
-    A(w.work_minor, pair_capacity);
-    A(w.dest, pair_capacity);
-    A(w.contacts, contact_capacity);
-    A(w.ckeys, contact_capacity);
-    A(w.matched, contact_capacity);
-    A(w.archive, contact_capacity);
-    A(w.resting, contact_capacity);
-    A(w.rest_tmp, contact_capacity);
-    A(w.rest_major, contact_capacity);
-    A(w.rest_minor, contact_capacity);
-    A(w.rest_slot, contact_capacity);
-    A(w.rest_flags, contact_capacity);
     A(w.parents, body_count);
     A(w.istate, body_count);
     A(w.block_counts, body_count);
-    A(w.blocks, 2u * static_cast<size_t>(contact_capacity));
-    A(w.targets, 4u * static_cast<size_t>(contact_capacity));
-    A(w.prows, 3u * static_cast<size_t>(contact_capacity));
-    A(w.srows, static_cast<size_t>(kSolveRow) * contact_capacity);
     A(w.sbodies, static_cast<size_t>(kSolverBody) * body_count);
-    A(w.simp, 3u * static_cast<size_t>(contact_capacity));
     A(w.resolution, 4u * static_cast<size_t>(body_count));
-    A(w.slots, 2u * static_cast<size_t>(contact_capacity));
     A(w.list_counts, body_count);
     A(w.row_start, body_count);
-    A(w.adj, 2u * static_cast<size_t>(contact_capacity));
-    A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
-    A(w.pwords, 2u * static_cast<size_t>(contact_capacity));
     A(w.contrib, body_count);
-    uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
-    if (body_count > scan_limit) scan_limit = body_count;  // the rows' list offsets
-    A(w.scan_blocks, ceil_div(scan_limit ? scan_limit : 1u, kScanBlock) + 1u);
     A(w.scan_total, 4);
+    A(w.work_total, 4);
+    A(world->snap_header, 4);
     A(world->words, 8);
     A(world->entry_keys, world->entry_cap);
     A(world->entry_order, world->entry_cap);
     A(world->entries, world->entry_cap);
-    A(world->freeze_flags, contact_capacity);
-    A(world->keep_dest, contact_capacity);
-    A(world->freeze_dest, contact_capacity);
 #undef A
     if (rc != DYN_OK) return cleanup(rc);
     w.ncontacts = world->words + 1;
@@ -2501,19 +2660,20 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     }
     w.edits = world->edit_buf;
     w.edit_runs = world->run_buf;
-    world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(collider_count, world->entry_cap, pair_capacity);
-    world->sort_scratch_bytes = dyn_sort_scratch_bytes(contact_capacity, 2u * dyn_sort_key_words(id_cap ? id_cap : 1u));
-    if (hipMalloc(&world->bp_scratch, world->bp_scratch_bytes) != hipSuccess ||
-        hipMalloc(&world->sort_scratch, world->sort_scratch_bytes) != hipSuccess) {
-        return cleanup(fail(DYN_E_HIP, "dyn_world_create: scratch"));
+    if (alloc_streams(world, pair_capacity, contact_capacity) != DYN_OK) {
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: pair and contact streams"));
     }
-    world->owned.push_back(world->bp_scratch);
-    world->owned.push_back(world->sort_scratch);
-    world->work_scratch_bytes = dyn_sort_scratch_bytes(pair_capacity, kWorkKeyBytes);
-    if (hipMalloc(&world->work_scratch, world->work_scratch_bytes) != hipSuccess) {
-        return cleanup(fail(DYN_E_HIP, "dyn_world_create: scratch"));
+    if (write_snap_header(world) != DYN_OK) return cleanup(fail(DYN_E_HIP, "dyn_world_create: snapshot header"));
+    if (hipHostMalloc(reinterpret_cast<void**>(&world->plan_host), 4u * kPlanWords * kPlanRing) != hipSuccess) {
+        world->plan_host = nullptr;
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: demand records"));
     }
-    world->owned.push_back(world->work_scratch);
+    for (uint32_t i = 0; i < kPlanRing; ++i) {
+        if (hipEventCreateWithFlags(&world->plan_events[i], hipEventDisableTiming) != hipSuccess) {
+            world->plan_events[i] = nullptr;
+            return cleanup(fail(DYN_E_HIP, "dyn_world_create: demand events"));
+        }
+    }
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (bytes && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = DYN_E_HIP;
     };
@@ -2604,6 +2764,11 @@ int dyn_world_destroy(dyn_world* world) {
     (void)hipStreamSynchronize(world->stream);
     drop_graph(world);
     for (void* p : world->owned) (void)hipFree(p);
+    free_streams(world->streams);
+    if (world->plan_host) (void)hipHostFree(world->plan_host);
+    for (hipEvent_t& e : world->plan_events) {
+        if (e) (void)hipEventDestroy(e);
+    }
     if (world->shape_sources) (void)hipFree(world->shape_sources);
     if (world->shape_vertices) (void)hipFree(world->shape_vertices);
     if (world->edit_buf) (void)hipFree(world->edit_buf);
@@ -2677,16 +2842,20 @@ int dyn_world_step(dyn_world* world, float dt) {
     if (world->hull_colliders && world->shape_sources == nullptr) {
         return dyn::fail(DYN_E_INVALID, "dyn_world_step: hull colliders need dyn_world_upload_shapes first");
     }
+    DYN_TRY(plan_capacity(world));
     // profiled and recorded steps run eagerly
-    if (!world->graphs || dyn::profiling() || world->tape_states) return world_step(world, dt);
-    const uint32_t p = world->parity;
-    if (!world->graph[p] || world->graph_dt[p] != dt) {
-        DYN_TRY(capture_step(world, dt));  // recording the step swapped the buffers
+    if (!world->graphs || dyn::profiling() || world->tape_states) {
+        DYN_TRY(world_step(world, dt));
     } else {
-        swap_contact_buffers(world);
+        const uint32_t p = world->parity;
+        if (!world->graph[p] || world->graph_dt[p] != dt) {
+            DYN_TRY(capture_step(world, dt));  // recording the step swapped the buffers
+        } else {
+            swap_contact_buffers(world);
+        }
+        DYN_HIP(hipGraphLaunch(world->graph[p], world->stream));
     }
-    DYN_HIP(hipGraphLaunch(world->graph[p], world->stream));
-    return DYN_OK;
+    return record_demand(world);
 }
 
 int dyn_world_read_work(dyn_world* world, uint32_t* out) {
@@ -2702,6 +2871,22 @@ int dyn_world_set_graphs(dyn_world* world, int enabled) {
     world->graphs = enabled != 0;
     drop_graph(world);
     return DYN_OK;
+}
+
+int dyn_world_capacity(dyn_world* world, uint32_t* pair_capacity, uint32_t* contact_capacity, uint32_t* grows) {
+    if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_capacity: null world");
+    if (pair_capacity) *pair_capacity = world->w.pair_cap;
+    if (contact_capacity) *contact_capacity = world->w.contact_cap;
+    if (grows) *grows = world->grows;
+    return DYN_OK;
+}
+
+int dyn_world_reserve(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capacity) {
+    if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_reserve: null world");
+    if (world->growth_locks && (pair_capacity > world->w.pair_cap || contact_capacity > world->w.contact_cap)) {
+        return dyn::fail(DYN_E_INVALID, "dyn_world_reserve: a differentiable rollout holds the world's capacities");
+    }
+    return grow_streams(world, pair_capacity, contact_capacity);
 }
 
 int dyn_world_wait(dyn_world* world) {
@@ -3002,7 +3187,9 @@ struct SnapPart {
 std::vector<SnapPart> snapshot_parts(dyn_world* world) {
     dyn::W& w = world->w;
     const size_t cap = w.contact_cap;
-    return {{w.st, sizeof(dyn_body_state) * w.n},
+    // first the header: the capacities the streams below were sized by
+    return {{world->snap_header, 4u * 4u},
+            {w.st, sizeof(dyn_body_state) * w.n},
             {w.archive, sizeof(dyn_contact) * cap},
             {w.resting, sizeof(dyn_contact) * cap},
             {w.rest_major, 4u * cap},
@@ -3061,6 +3248,13 @@ int dyn_world_snapshot(dyn_world* world, void* d_dst) {
 
 int dyn_world_restore(dyn_world* world, const void* d_src) {
     if (!world || !d_src) return dyn::fail(DYN_E_INVALID, "dyn_world_restore: null argument");
+    uint32_t header[4] = {0, 0, 0, 0};
+    DYN_HIP(hipMemcpyAsync(header, d_src, sizeof header, hipMemcpyDeviceToHost, world->stream));
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    if (header[0] != kSnapMagicWord || header[1] != world->w.contact_cap || header[2] != world->w.n ||
+        header[3] != world->w.k) {
+        return dyn::fail(DYN_E_INVALID, "dyn_world_restore: the snapshot was taken at other capacities or of another world");
+    }
     const char* at = static_cast<const char*>(d_src);
     for (const SnapPart& part : snapshot_parts(world)) {
         DYN_HIP(hipMemcpyAsync(part.ptr, at, part.bytes, hipMemcpyDeviceToDevice, world->stream));
@@ -3075,6 +3269,12 @@ namespace dyn {
 
 // The differentiable rollout's access to a world (adjoint.hip).
 hipStream_t world_stream(dyn_world* world) { return world->stream; }
+
+// A holder of buffers sized by the world's capacities (the differentiable rollout) keeps the
+// world's streams from growing while it exists.
+void world_lock_growth(dyn_world* world, int delta) {
+    world->growth_locks = static_cast<uint32_t>(static_cast<int>(world->growth_locks) + delta);
+}
 
 void world_sizes(dyn_world* world, uint32_t* n, uint32_t* d, uint32_t* k, uint32_t* cap) {
     *n = world->w.n;

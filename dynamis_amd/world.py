@@ -305,7 +305,7 @@ class World(BodyCommands):
                  pair_capacity: int | None = None):
         self.config = default_config() if config is None else config.copy()
         self.contact_capacity = contact_capacity
-        self.pair_capacity = pair_capacity
+        self._pair_request = pair_capacity
         self._bodies: list[BodyDesc] = []
         self._hulls: list[HullSource] = []
         self._handle = ctypes.c_void_p()
@@ -334,7 +334,7 @@ class World(BodyCommands):
         if not self._bodies:
             raise RuntimeError("a world needs at least one body")
         self.scene = encode(self._bodies, self.config, self._hulls)
-        self.create(self.scene, self.contact_capacity, self.pair_capacity)
+        self.create(self.scene, self.contact_capacity, self._pair_request)
 
     def create(self, scene: Scene, contact_capacity: int | None = None,
                pair_capacity: int | None = None, stream: int | None = None) -> None:
@@ -346,8 +346,6 @@ class World(BodyCommands):
         call("dyn_world_create", ctypes.byref(self._handle), stream, scene.params.ctypes.data,
              scene.states.ctypes.data, scene.descs.ctypes.data, n, scene.dynamic_count,
              scene.colliders.ctypes.data, scene.owners.ctypes.data, n, capacity, pairs)
-        self.capacity = capacity
-        self.pair_capacity = pairs
         if scene.sources is not None and len(scene.sources):
             call("dyn_world_upload_shapes", self._handle, scene.sources.ctypes.data,
                  len(scene.sources), scene.vertices.ctypes.data, len(scene.vertices))
@@ -360,6 +358,41 @@ class World(BodyCommands):
     def wait(self) -> None:
         if self.scene is not None:
             call("dyn_world_wait", self._handle)
+
+    # -- capacity planning (World::apply_plan, crates/dynamis-world/src/backend/mod.rs:194-230) --
+    def capacities(self) -> tuple[int, int, int]:
+        """(pair capacity, contact capacity, growths so far): a step's demand grows the streams
+        two steps later (dyn_world_capacity)."""
+        pairs, contacts, grows = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        call("dyn_world_capacity", self._handle, ctypes.byref(pairs), ctypes.byref(contacts), ctypes.byref(grows))
+        return int(pairs.value), int(contacts.value), int(grows.value)
+
+    @property
+    def capacity(self) -> int:
+        """Contacts (and resting manifolds) a step can hold now."""
+        if not self._handle.value:
+            return default_capacities(len(self._bodies), self.contact_capacity, None)[0]
+        return self.capacities()[1]
+
+    @property
+    def pair_capacity(self) -> int:
+        """Candidate pairs a step can hold now."""
+        if not self._handle.value:
+            return default_capacities(len(self._bodies), None, self._pair_request)[1]
+        return self.capacities()[0]
+
+    def reserve(self, pair_capacity: int = 0, contact_capacity: int = 0) -> None:
+        """Grows the pair and contact streams to at least the given capacities now."""
+        self._ensure()
+        call("dyn_world_reserve", self._handle, int(pair_capacity), int(contact_capacity))
+
+    def refusals(self) -> list[dict]:
+        """What the last step could not hold (World::refusals, crates/dynamis-world/src/
+        readback.rs:226): every refusing counter above zero with its label and shortfall class
+        (crates/dynamis-abi/src/counter.rs:75-128)."""
+        m = self.measured()
+        return [{"counter": name, "label": label, "shortfall": shortfall, "count": m[name]}
+                for name, (label, shortfall) in abi.REFUSING_COUNTERS.items() if m[name] > 0]
 
     def convex_work(self) -> dict[str, int]:
         """Pairs the last step gave to GJK, to EPA and to the convex manifold kernel."""
@@ -558,8 +591,6 @@ class Batch(World):
              merged.states.ctypes.data, merged.descs.ctypes.data, n, merged.dynamic_count,
              merged.colliders.ctypes.data, merged.owners.ctypes.data,
              self.layout.collider_worlds.ctypes.data, len(merged.colliders), self.worlds, capacity, pairs)
-        self.capacity = capacity
-        self.pair_capacity = pairs
         if merged.sources is not None and len(merged.sources):
             call("dyn_world_upload_shapes", self._handle, merged.sources.ctypes.data,
                  len(merged.sources), merged.vertices.ctypes.data, len(merged.vertices))

@@ -364,7 +364,8 @@ typedef struct dyn_world dyn_world;
  * `stream` (hipStream_t) orders all of the world's work; NULL creates a private stream.
  * contact_capacity bounds contacts and resting manifolds per step, pair_capacity the candidate
  * pairs (the reference's stream capacities; overflow is counted in REFUSED_CONTACTS,
- * REFUSED_RESTING and REFUSED_PAIRS).
+ * REFUSED_RESTING and REFUSED_PAIRS). The capacities are the starting plan: a step whose demand
+ * outgrew them grows them two steps later (see dyn_world_capacity).
  */
 int dyn_world_create(dyn_world** world, void* stream, const dyn_step_params* params,
                      const dyn_body_state* states, const dyn_body_desc* descs, uint32_t body_count,
@@ -412,6 +413,23 @@ int dyn_world_read_work(dyn_world* world, uint32_t* out);
 int dyn_world_set_graphs(dyn_world* world, int enabled);
 /* Blocks until every enqueued step has finished. */
 int dyn_world_wait(dyn_world* world);
+/* Capacity planning (World::apply_plan, crates/dynamis-world/src/backend/mod.rs:194-230, with the
+ * stream plans of crates/dynamis-broadphase/src/capacity.rs:27-80 and crates/dynamis-rigid/src/
+ * capacity.rs:50-140). Every step's demand (its uncapped pair total, and the larger of its
+ * uncapped contact and resting totals) is read back two steps later; before that later step a
+ * demand above a stream's capacity grows the stream to max(demand, 2 x capacity), keeping the
+ * durable contact and resting archives. The step that overflowed has already refused what did
+ * not fit (REFUSED_PAIRS, REFUSED_CONTACTS, REFUSED_RESTING), as in the reference, whose plan also
+ * follows a lagged readback. A fixed lag makes growth a function of the steps alone, so a run is
+ * deterministic. Streams never shrink (the reference releases capacity once a scene settles).
+ * Growth waits for the world's stream and drops its captured graphs. It is held off while a
+ * differentiable rollout (dyn_adjoint_create) on the world exists, whose buffers are sized by the
+ * contact capacity. Snapshots record the capacities they were taken at; a restore into other
+ * capacities fails.
+ * dyn_world_capacity reads the current capacities and the number of growths so far (any pointer
+ * may be NULL); dyn_world_reserve grows them to at least the given values now. */
+int dyn_world_capacity(dyn_world* world, uint32_t* pair_capacity, uint32_t* contact_capacity, uint32_t* grows);
+int dyn_world_reserve(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capacity);
 
 /* ---- the literal fixed-cell broadphase (no reference counterpart: the reference's grid is
  * hierarchical, crates/dynamis-broadphase/src/passes.rs:76-160) ----

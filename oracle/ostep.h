@@ -34,6 +34,8 @@ int oracle_world_shapes(oracle_world* w, const dyn_shape_source* sources, uint32
 int oracle_world_edit(oracle_world* w, const uint32_t* body_ids, const dyn_body_edit* edits, uint32_t count);
 /* One step with the given dt (params.dt / substep_dt are set from it). */
 void oracle_world_step(oracle_world* w, float dt);
+/* Current pair and contact capacities and the growths so far (3 words); see plan_capacity. */
+void oracle_world_capacity(const oracle_world* w, uint32_t* out);
 void oracle_world_states(const oracle_world* w, dyn_body_state* out);
 /* The last step's tape for the differentiable-step checks: the states it started from, its
  * contacts as the solver received them (after the warm-start relay) and, per manifold point,

@@ -101,6 +101,10 @@ struct oracle_world {
     uint32_t* edit_rows;
     dyn_body_edit* edits;
     uint32_t nedits, edit_cap;
+    /* capacity planning: each step's (pair, contact) demand, read two steps later */
+    uint32_t demand[4][2];
+    uint64_t issued;
+    uint32_t grows;
     /* the last step's tape: start states, contacts as the solver received them, relay sources */
     dyn_body_state* start;
     dyn_contact* pre;
@@ -2539,7 +2543,75 @@ int oracle_world_edit(oracle_world* w, const uint32_t* body_ids, const dyn_body_
     return 0;
 }
 
+/* Capacity planning (World::apply_plan, crates/dynamis-world/src/backend/mod.rs:194-230; stream
+ * plans crates/dynamis-broadphase/src/capacity.rs:27-80, crates/dynamis-rigid/src/capacity.rs:
+ * 50-140): the demand of the step two steps back (its uncapped pair total; the larger of its
+ * uncapped contact and resting totals) grows a stream it outgrew to max(demand, 2 x capacity)
+ * (crates/dynamis-domain/src/capacity.rs:15-23 grown). The library plans with the same lag, so the
+ * two engines refuse and grow at the same steps. Per-step arrays grow zero-filled; the archives
+ * keep their contents. */
+#define PLAN_LAG 2u
+#define PLAN_RING 4u
+
+static void* grow_array(void* ptr, size_t old_count, size_t new_count, size_t size) {
+    char* p = realloc(ptr, (new_count ? new_count : 1u) * size);
+    if (!p) abort();
+    if (new_count > old_count) memset(p + old_count * size, 0, (new_count - old_count) * size);
+    return p;
+}
+
+static uint32_t grown_capacity(uint32_t capacity, uint32_t required) {
+    if (required <= capacity) return capacity;
+    uint64_t twice = 2ull * capacity;
+    uint64_t next = required > twice ? required : twice;
+    return next > (1u << 30) ? (1u << 30) : (uint32_t)next;
+}
+
+static void plan_capacity(oracle_world* w) {
+    if (w->issued < PLAN_LAG) return;
+    const uint32_t* d = w->demand[(w->issued - PLAN_LAG) % PLAN_RING];
+    uint32_t pairs = grown_capacity(w->pair_cap, d[0]);
+    uint32_t contacts = grown_capacity(w->contact_cap, d[1]);
+    if (pairs != w->pair_cap || contacts != w->contact_cap) w->grows++;
+    if (pairs != w->pair_cap) {
+        size_t o = w->pair_cap, n = pairs;
+        w->pmaj = grow_array(w->pmaj, o, n, sizeof *w->pmaj);
+        w->pmin = grow_array(w->pmin, o, n, sizeof *w->pmin);
+        w->raw = grow_array(w->raw, o, n, sizeof *w->raw);
+        w->valid = grow_array(w->valid, o, n, sizeof *w->valid);
+        w->pair_cap = pairs;
+    }
+    if (contacts != w->contact_cap) {
+        size_t o = w->contact_cap, n = contacts;
+        w->contacts = grow_array(w->contacts, o, n, sizeof *w->contacts);
+        w->matched = grow_array(w->matched, o, n, sizeof *w->matched);
+        w->archive = grow_array(w->archive, o, n, sizeof *w->archive);
+        w->resting = grow_array(w->resting, o, n, sizeof *w->resting);
+        w->blocks = grow_array(w->blocks, 2u * o, 2u * n, sizeof *w->blocks);
+        w->targets = grow_array(w->targets, 4u * o, 4u * n, sizeof *w->targets);
+        w->pre = grow_array(w->pre, o, n, sizeof *w->pre);
+        w->relay_src = grow_array(w->relay_src, 4u * o, 4u * n, sizeof *w->relay_src);
+        w->contact_cap = contacts;
+    }
+}
+
+static void record_demand(oracle_world* w) {
+    uint32_t* d = w->demand[w->issued % PLAN_RING];
+    d[0] = *ctr(w, DYN_COUNTER_PAIRS);
+    uint64_t resting = (uint64_t)*ctr(w, DYN_COUNTER_RESTING) + *ctr(w, DYN_COUNTER_REFUSED_RESTING);
+    uint32_t contacts = *ctr(w, DYN_COUNTER_CONTACTS);
+    d[1] = resting > contacts ? (uint32_t)(resting > 0xFFFFFFFFu ? 0xFFFFFFFFu : resting) : contacts;
+    w->issued++;
+}
+
+void oracle_world_capacity(const oracle_world* w, uint32_t* out) {
+    out[0] = w->pair_cap;
+    out[1] = w->contact_cap;
+    out[2] = w->grows;
+}
+
 void oracle_world_step(oracle_world* w, float dt) {
+    plan_capacity(w);
     w->p.dt = dt;
     w->p.substep_dt = dt / (float)w->p.substeps;
     for (size_t i = 0; i < sizeof kPerStep / sizeof kPerStep[0]; i++) *ctr(w, kPerStep[i]) = 0;
@@ -2651,6 +2723,7 @@ void oracle_world_step(oracle_world* w, float dt) {
     }
     *ctr(w, DYN_COUNTER_RESTING) = w->nresting;
     (*ctr(w, DYN_COUNTER_STEP))++;
+    record_demand(w);
 }
 
 /* ---- lifetime and accessors ------------------------------------------------------------- */

@@ -55,6 +55,8 @@ class Oracle:
         h.oracle_world_destroy.argtypes = [_p]
         h.oracle_world_step.restype = None
         h.oracle_world_step.argtypes = [_p, ctypes.c_float]
+        h.oracle_world_capacity.restype = None
+        h.oracle_world_capacity.argtypes = [_p, _p]
         h.oracle_world_states.restype = None
         h.oracle_world_states.argtypes = [_p, _p]
         h.oracle_world_contacts.restype = _u32
@@ -182,12 +184,12 @@ class OracleWorld(BodyCommands):
         self.o = oracle
         n = len(scene.states)
         self.n = n
-        self.capacity, self.pair_capacity = default_capacities(n, contact_capacity, pair_capacity)
+        capacity, pairs = default_capacities(n, contact_capacity, pair_capacity)
         self.scene = scene
         self.w = oracle.h.oracle_world_create(
             scene.params.ctypes.data, scene.states.ctypes.data, scene.descs.ctypes.data, n,
             scene.dynamic_count, scene.colliders.ctypes.data, scene.owners.ctypes.data, n,
-            self.capacity, self.pair_capacity)
+            capacity, pairs)
         if not self.w:
             raise MemoryError("oracle_world_create failed")
         if scene.sources is not None and len(scene.sources):
@@ -198,6 +200,20 @@ class OracleWorld(BodyCommands):
 
     def step(self, dt: float) -> None:
         self.o.h.oracle_world_step(self.w, ctypes.c_float(dt))
+
+    def capacities(self) -> tuple[int, int, int]:
+        """(pair capacity, contact capacity, growths so far), as World.capacities."""
+        out = np.zeros(3, np.uint32)
+        self.o.h.oracle_world_capacity(self.w, out.ctypes.data)
+        return int(out[0]), int(out[1]), int(out[2])
+
+    @property
+    def capacity(self) -> int:
+        return self.capacities()[1]
+
+    @property
+    def pair_capacity(self) -> int:
+        return self.capacities()[0]
 
     def states(self) -> np.ndarray:
         out = np.zeros(self.n, dtype=abi.BODY_STATE)

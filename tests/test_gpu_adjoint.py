@@ -226,7 +226,10 @@ def test_full_workload_sampled_worlds_against_fp64_reverse():
             whole.forward(DT)
             assert again.states().tobytes() == end.tobytes()
             w0, wm, wf = whole.backward(g)
-            assert np.array_equal(w0, g0) and np.array_equal(wm, gm) and np.array_equal(wf, gf)
+            # byte equality: a world whose rollout throws a body (a manifold row whose mass comes
+            # out non-positive, solver_contact_block.wgsl:14-28) overflows its FP32 reverse to
+            # NaN, identically in both passes
+            assert w0.tobytes() == g0.tobytes() and wm.tobytes() == gm.tobytes() and wf.tobytes() == gf.tobytes()
             tapes = [whole.tape(t) for t in range(STEPS)]
             bodies = whole.bodies()
         finally:
