@@ -20,6 +20,9 @@ Workloads (`--workload`):
               (dyn_fixed_grid; the same pair set as the hierarchical grid).
   copy        calibration only: device-to-device copies of 1 GiB, whose HBM counter readings
               (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) scale the counters of the other kernels.
+  narrowphase the narrowphase pass alone (dyn_narrowphase) over the candidate pairs of the pile's
+              600th step, saved by a first run (--stage-file) so that a later run executes the pass
+              and nothing else (for rocprofv3 counter passes).
   adjoint     the differentiable rollout alone: 256 worlds of 32 boxes (half-extents
               ~ U[0.2, 0.5]) on a plane, v0 ~ N(0, 2 m/s), friction ~ U[0.2, 0.8], dt = 1/120,
               8 iterations, 64 steps with a checkpoint every 8; forward and backward times and
@@ -572,6 +575,60 @@ def run_fixed_grid(steps: int, warmup: int):
     return ms, stages, counts
 
 
+def run_narrowphase(steps: int, warmup: int, stage_file: str) -> dict:
+    """The narrowphase pass alone (dyn_narrowphase: the kernels dyn_world_step runs) over the
+    settled pile's candidate pairs. Without `stage_file` on disk, the pile runs its 600 steps
+    first and the states its last step began from, with that step's pairs, are saved there; with
+    it, they are loaded and only the pass runs, so a counter pass (rocprofv3 --pmc) sees no world
+    step at all."""
+    import dynamis_amd
+    from dynamis_amd.device import Event, Stream, profile_collect, profile_enable
+    from dynamis_amd.narrowphase import Narrowphase
+    from dynamis_amd.world import World
+
+    if dynamis_amd.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    dynamis_amd._native.call("dyn_set_device", 0)
+    scene = pile_scene()
+    if not os.path.exists(stage_file):
+        world = World()
+        world.create(scene)
+        world.set_graphs(GRAPHS)
+        run = WORKLOADS["pile"]["run"]
+        for _ in range(run - 1):
+            world.step(DT)
+        states = world.states()
+        world.step(DT)
+        pairs = world.pairs()
+        world.close()
+        np.savez(stage_file, states=states, pairs=pairs)
+        progress(f"saved the pile's step-{run} narrowphase input ({len(pairs)} pairs) to {stage_file}")
+    data = np.load(stage_file)
+    states, pairs = data["states"], data["pairs"]
+    stream = Stream()
+    stage = Narrowphase(scene, len(pairs), stream=stream)
+    stage.set_states(states)
+    for _ in range(warmup):
+        stage.run(pairs)
+    stream.synchronize()
+    start, stop = Event(), Event()
+    start.record(stream)
+    for _ in range(steps):
+        stage.run(pairs)
+    stop.record(stream)
+    ms = start.elapsed_ms(stop)
+    stream.synchronize()
+    profile_enable(True)
+    for _ in range(min(steps, 10)):
+        stage.run(pairs)
+    stream.synchronize()
+    profile_enable(False)
+    stages = profile_collect()
+    _, valid = stage.read()
+    return {"ms": ms, "pairs": int(len(pairs)), "contacts": int((valid != 0).sum()), "stages": stages,
+            "bodies": int(len(scene.states))}
+
+
 def run_copy(count: int) -> int:
     """Copies of a known size for the HBM counters' calibration; prints one JSON line."""
     import ctypes
@@ -829,7 +886,7 @@ def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "fixed_grid", "copy",
-                                               "adjoint"], default="pile")
+                                               "adjoint", "narrowphase"], default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--no-graphs", action="store_true",
@@ -841,6 +898,8 @@ def main() -> int:
                         help="oracle steps (broadphase: passes) timed for the CPU baseline; 0 skips it")
     parser.add_argument("--only", action="store_true",
                         help="pile: leave out the batch, broadphase and adjoint parts of the line")
+    parser.add_argument("--stage-file", default="/tmp/dynamis_pile_narrowphase.npz",
+                        help="narrowphase: where the settled pile's narrowphase input is saved or found")
     args = parser.parse_args()
     global GRAPHS, TUNE
     GRAPHS = not args.no_graphs
@@ -858,6 +917,19 @@ def main() -> int:
                           "n_gpus": 1, "steps": 64, "warmup": 0, "ms_per_step": r["forward_ms"] / 64,
                           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                           "data": "synthetic", "config": r}))
+        return 0
+    if wl == "narrowphase":
+        steps = max(args.steps if args.steps is not None else 20, 1)
+        warmup = max(args.warmup if args.warmup is not None else 3, 0)
+        r = run_narrowphase(steps, warmup, args.stage_file)
+        print(json.dumps({"metric": "narrowphase pass over the settled pile's candidate pairs", "unit": "pairs/s",
+                          "value": r["pairs"] * steps / (r["ms"] / 1000.0), "n_gpus": 1, "steps": steps,
+                          "warmup": warmup, "ms_per_step": r["ms"] / steps, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "dyn_narrowphase over the candidate pairs of step 600 of the "
+                                                 "100,000-body mixed convex pile", "pairs": r["pairs"],
+                                     "contacts": r["contacts"], "bodies": r["bodies"]},
+                          "stages": {k: {"launches": v[0], "ms": v[1]} for k, v in r["stages"].items()}}))
         return 0
     if wl == "fixed_grid":
         steps = max(args.steps if args.steps is not None else 50, 1)
