@@ -47,6 +47,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_sort_scratch_bytes": (c_size_t, [c_u32, c_u32]),
     "dyn_sort_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_void_p,
                                c_void_p, c_u32, c_u32, c_u32, c_void_p, c_size_t]),
+    "dyn_sort_static_scratch_bytes": (c_size_t, []),
+    "dyn_sort_pairs_static": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_void_p,
+                                      c_void_p, c_u32, c_u32, c_u32]),
     "dyn_sort_faults": (c_int, [ctypes.POINTER(c_u32), c_int]),
     "dyn_broadphase_scratch_bytes": (c_size_t, [c_u32, c_u32, c_u32]),
     "dyn_broadphase": (c_int, [c_void_p, c_void_p]),

@@ -622,3 +622,59 @@ int dyn_sort_pairs(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t*
 }
 
 }  // extern "C"
+
+// ---- the sort with no allocation: its scratch is a file-scope device array ----------------------
+
+namespace {
+
+// The scratch of dyn_sort_pairs_static: one array per device (each GPU the library runs on holds
+// its own copy until the process ends), large enough for 2^20 records at any digit count.
+constexpr size_t kStaticScratchBytes = size_t{17} << 20;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kStaticDevices = 64;
+std::mutex static_mutex;
+void* static_scratch[kStaticDevices] = {};
+
+// The current device's address of `scratch`, looked up on the device's first call (the array's
+// host-side name is no device address).
+int static_scratch_of_device(void** out) {
+    int device = 0;
+    DYN_HIP(hipGetDevice(&device));
+    if (device < 0 || device >= kStaticDevices) return dyn::fail(DYN_E_INVALID, "dyn_sort_pairs_static: device ordinal out of range");
+    std::lock_guard<std::mutex> guard(static_mutex);
+    if (!static_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !ptr) return dyn::hip_fail(rc, "dyn_sort_pairs_static: address of the static scratch");
+        static_scratch[device] = ptr;
+    }
+    *out = static_scratch[device];
+    return DYN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t dyn_sort_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int dyn_sort_pairs_static(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t* d_payload,
+                          uint32_t capacity, const uint32_t* d_count, const uint32_t* d_base, uint32_t plan,
+                          uint32_t major_words, uint32_t minor_words) {
+    using namespace dyn;
+    if (major_words > 4u || minor_words > 4u || major_words + minor_words == 0u) {
+        return dyn_sort_pairs(stream, d_major, d_minor, d_payload, capacity, d_count, d_base, plan, major_words,
+                              minor_words, nullptr, 0u);  // the same argument errors
+    }
+    if (capacity > kSortMaxCapacity ||
+        sort_scratch_layout(capacity, major_words + minor_words, true, nullptr, nullptr) > kStaticScratchBytes) {
+        return fail(DYN_E_CAPACITY, "dyn_sort_pairs_static: capacity beyond the static scratch (2^20 records)");
+    }
+    void* base = nullptr;
+    DYN_TRY(static_scratch_of_device(&base));
+    return dyn_sort_pairs(stream, d_major, d_minor, d_payload, capacity, d_count, d_base, plan, major_words,
+                          minor_words, base, kStaticScratchBytes);
+}
+
+}  // extern "C"

@@ -96,6 +96,18 @@ int dyn_sort_pairs(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t*
                    uint32_t capacity, const uint32_t* d_count, const uint32_t* d_base,
                    uint32_t plan, uint32_t major_words, uint32_t minor_words,
                    void* d_scratch, size_t scratch_bytes);
+/* dyn_sort_pairs with no allocation and no caller scratch. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). The sort's scratch is the library's static device array of
+ * dyn_sort_static_scratch_bytes() bytes (17 MiB: at least 2^20 records at any digit count), one
+ * per device, held until the process ends; a capacity whose scratch (dyn_sort_scratch_bytes)
+ * exceeds it fails with DYN_E_CAPACITY. Every call on a device shares that device's array, so
+ * calls that overlap on one device (on different streams, or from several threads) race; calls
+ * on one stream are ordered and safe. Everything else is dyn_sort_pairs's contract. */
+size_t dyn_sort_static_scratch_bytes(void);
+int dyn_sort_pairs_static(void* stream, uint32_t* d_major, uint32_t* d_minor, uint32_t* d_payload,
+                          uint32_t capacity, const uint32_t* d_count, const uint32_t* d_base,
+                          uint32_t plan, uint32_t major_words, uint32_t minor_words);
 /* Sort faults on the current device since the last clear: look-back waits of any sort (this
  * entry point's, the broadphase's, a world step's) that gave up and left that sort's order wrong.
  * Never expected; a caller that must trust an order checks it is 0. Waits for the device first;
