@@ -62,6 +62,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_config_default": (None, [c_void_p]),
     "dyn_step_params_build": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p]),
     "dyn_inertia_inverse": (c_int, [c_void_p, c_void_p]),
+    "dyn_hull_support_cells": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p]),
     "dyn_hull_build": (c_int, [c_void_p, c_u32, c_void_p, ctypes.POINTER(c_u32), c_void_p,
                                ctypes.POINTER(c_u32), c_u32]),
     "dyn_hull_solid": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
@@ -96,6 +97,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_adjoint_destroy": (c_int, [c_void_p]),
     "dyn_world_wait": (c_int, [c_void_p]),
     "dyn_world_capacity": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dyn_world_set_solver_threads": (c_int, [c_void_p, c_u32]),
+    "dyn_world_tune_solver": (c_int, [c_void_p, c_u32, c_void_p, c_void_p]),
     "dyn_world_reserve": (c_int, [c_void_p, c_u32, c_u32]),
     "dyn_world_stream": (c_void_p, [c_void_p]),
     "dyn_world_read_states": (c_int, [c_void_p, c_void_p]),

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "narrow.h"
+#include "support_cells.h"
 
 namespace dyn {
 
@@ -65,11 +66,48 @@ __device__ inline v3 hull_support_lanes(const Shape& w, const ShapeSet& ss, v3 d
     return index == 0xFFFFFFFFu ? zero3v() : hull_vertex(ss, src, index);
 }
 
-// The same vertex by one thread's sequential scan.
+// The same vertex by one thread's sequential scan: of the direction's support-cell candidates
+// when the source has them (the same vertex, support_cells.h), else of every vertex.
 __device__ inline v3 hull_support_thread(const Shape& w, const ShapeSet& ss, v3 d) {
     const dyn_shape_source src = ss.sources[w.source];
     float best_dot = -kBig;
     v3 best = zero3v();
+    uint32_t cell;
+    if (ss.cells && support_cell_of(d.x, d.y, d.z, &cell)) {
+        const uint2 rec = ss.cells[static_cast<size_t>(w.source) * kSupportCells + cell];
+        const uint32_t count = rec.x & 0xFFu;
+        if (count != kSupportNoCell) {
+            if (count <= kSupportInline) {
+                const uint32_t idx[4] = {(rec.x >> 8) & 0xFFFu, rec.x >> 20, rec.y & 0xFFFu, (rec.y >> 12) & 0xFFFu};
+                v3 cand[4];
+#pragma unroll
+                for (uint32_t k = 0; k < kSupportInline; ++k) {
+                    if (k < count) cand[k] = hull_vertex(ss, src, idx[k]);
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kSupportInline; ++k) {
+                    if (k < count) {
+                        const float s = dot3(cand[k], d);
+                        if (s > best_dot) {
+                            best_dot = s;
+                            best = cand[k];
+                        }
+                    }
+                }
+                return best;
+            }
+            const uint16_t* list = ss.cands + rec.y;
+            for (uint32_t k = 0; k < count; ++k) {
+                const v3 v = hull_vertex(ss, src, list[k]);
+                const float s = dot3(v, d);
+                if (s > best_dot) {
+                    best_dot = s;
+                    best = v;
+                }
+            }
+            return best;
+        }
+    }
 #pragma unroll 8
     for (uint32_t i = 0; i < src.vertex_count; ++i) {
         const v3 v = hull_vertex(ss, src, i);
@@ -86,7 +124,10 @@ __device__ inline v3 hull_support_thread(const Shape& w, const ShapeSet& ss, v3 
 // aligned group of kLanes lanes calls it together and shares the hull scan.
 template <int kLanes = 1>
 __device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
-    const v3 n = norm3(d);
+    if (w.kind == DYN_SHAPE_HULL) {
+        return kLanes > 1 ? hull_support_lanes<kLanes>(w, ss, d) : hull_support_thread(w, ss, d);
+    }
+    const v3 n = norm3(d);  // the direction normalized (read by spheres and capsules)
     if (w.kind == DYN_SHAPE_SPHERE) return mul(n, w.radius);
     if (w.kind == DYN_SHAPE_CUBOID) {
         const v3 s = mk(d.x > 0.0f ? 1.0f : -1.0f, d.y > 0.0f ? 1.0f : -1.0f, d.z > 0.0f ? 1.0f : -1.0f);
@@ -102,8 +143,7 @@ __device__ inline v3 support_local(const Shape& w, const ShapeSet& ss, v3 d) {
         if (len3(radial) < 1e-6f) return base;
         return add(base, mul(norm3(radial), w.radius));
     }
-    if (w.kind != DYN_SHAPE_HULL) return zero3v();
-    return kLanes > 1 ? hull_support_lanes<kLanes>(w, ss, d) : hull_support_thread(w, ss, d);
+    return zero3v();
 }
 
 // convex.wgsl:86-91

@@ -70,6 +70,8 @@ struct Shape {
 struct ShapeSet {
     const dyn_shape_source* sources;
     const float4* vertices;
+    const uint2* cells;       // support-cell records per source (support_cells.h), or null
+    const uint16_t* cands;
 };
 
 // core.wgsl:473-484

@@ -64,6 +64,8 @@ struct W {
     uint32_t id_cap;
     const dyn_shape_source* sources;  // hull shape sources and their float4 vertex rows
     const float4* vertices;
+    const uint2* support_cells;      // per source and cube-map cell: a two-word record (support_cells.h)
+    const uint16_t* support_cands;   // candidate vertex indices, ascending within a cell
     uint32_t source_count;
     uint32_t* wake;
     uint32_t* activity;
@@ -283,7 +285,7 @@ __global__ void __launch_bounds__(kThreads) k_aabb(W w) {
     const dyn_body_desc d = w.de[ow];
     const dyn_collider c = w.co[i];
     const Shape ws = world_collider(s, c);
-    const ShapeSet ss{w.sources, w.vertices};
+    const ShapeSet ss{w.sources, w.vertices, w.support_cells, w.support_cands};
     v3 tmn, tmx;
     world_aabb_of(ws, ss, &tmn, &tmx);
     Shape up = ws;
@@ -566,7 +568,7 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
     const Body first = load_body(w, fb), second = load_body(w, sb);
     const float margin = pair_margin(w, first, second);
-    const ShapeSet shapes{w.sources, w.vertices};
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
     if (convex_path(fc, sc)) {
         w.pkind[index] = kind_class(fc.kind, sc.kind);
         if (fc.kind == DYN_SHAPE_PLANE || sc.kind == DYN_SHAPE_PLANE) {
@@ -632,7 +634,7 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_
     if ((w.de[fb].flags & DYN_BODY_CCD) || (w.de[sb].flags & DYN_BODY_CCD)) {
         margin += len3(sub(ld3(sst.velocity), ld3(fst.velocity))) * w.p.dt;  // pair_margin
     }
-    const ShapeSet shapes{w.sources, w.vertices};
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
     const Shape wf = world_collider(fst, fc), wsd = world_collider(sst, sc);
     Hit hit;
     SimplexPoint simplex[4];
@@ -672,7 +674,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t total = w.work_total[2];
     const uint32_t waves = gridDim.x * kEpaWaves;
-    const ShapeSet shapes{w.sources, w.vertices};
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
     for (uint32_t t = blockIdx.x * kEpaWaves + wave; t < total; t += waves) {
         const uint32_t index = w.worklist[t];
         const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
@@ -709,7 +711,7 @@ __global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
     const Body first = load_body(w, fb), second = load_body(w, sb);
     const float margin = pair_margin(w, first, second);
-    const ShapeSet shapes{w.sources, w.vertices};
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
     const Shape wf = world_collider(first.s, fc), wsd = world_collider(second.s, sc);
     const float* park = parked(w, index);
     Manifold m;
@@ -2042,6 +2044,8 @@ struct dyn_world {
     std::vector<void*> owned;
     void* shape_sources = nullptr;   // owned separately: replaced by each shape upload
     void* shape_vertices = nullptr;
+    void* support_cells = nullptr;   // the hull sources' support cells (owned like the sources)
+    void* support_cands = nullptr;
     uint32_t collider_count = 0;
     std::vector<uint32_t> host_kinds;    // collider kinds and sources, for upload validation
     std::vector<uint32_t> host_sources;
@@ -2081,6 +2085,9 @@ struct dyn_world {
     uint64_t issued = 0;              // dyn_world_step calls since create
     uint32_t growth_locks = 0;        // holders (differentiable rollouts) whose buffers fix the capacity
     uint32_t grows = 0;
+    bool tuning = false;              // steps of dyn_world_tune_solver: no demand is recorded
+    float last_dt = 0.0f;             // the dt of the last step
+    uint32_t solver_threads = dyn::kThreads;  // workgroup size of the solver substeps' kernels
     // the tape a recorded step writes (dyn::world_record); null when not recording
     float* tape_states = nullptr;
     float* tape_contacts = nullptr;
@@ -2347,33 +2354,45 @@ int solver_prepare(const dyn::W& w, hipStream_t st) {
     return DYN_OK;
 }
 
+#define LAUNCH_TH(stream, name, items, threads, ...)                                      \
+    do {                                                                                  \
+        {                                                                                 \
+            ::dyn::ProfileScope scope_(#name, (stream));                                  \
+            name<<<::dyn::ceil_div((items) ? (items) : 1u, (threads)), (threads), 0, (stream)>>>(__VA_ARGS__); \
+        }                                                                                 \
+        DYN_LAUNCHED(#name);                                                              \
+    } while (0)
+
 // The substeps of SolveSubsteps (solver.rs:311-332), then the accumulated impulses back into the
-// contacts.
-int solver_substeps(const dyn::W& w, hipStream_t st) {
+// contacts. The per-contact and per-row kernels of the substeps index their element by the global
+// thread index alone, so they run in workgroups of `threads` (64, 128 or 256; the solver's
+// autotuned shape) with the same results.
+int solver_substeps(const dyn::W& w, hipStream_t st, uint32_t threads = dyn::kThreads) {
     using namespace dyn;
-    const uint32_t nb = blocks_for(w.n), db = blocks_for(w.d), cb = blocks_for(w.contact_cap);
+    const uint32_t nb = blocks_for(w.n);
     for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
-        LAUNCH_ON(st, k_integrate, db, w);
-        LAUNCH_ON(st, k_solver_bodies, nb, w);
-        LAUNCH_ON(st, k_solve_prep, cb, w, sub == 0 ? 1u : 0u);
+        LAUNCH_TH(st, k_integrate, w.d, threads, w);
+        LAUNCH_TH(st, k_solver_bodies, w.n, threads, w);
+        LAUNCH_TH(st, k_solve_prep, w.contact_cap, threads, w, sub == 0 ? 1u : 0u);
         if (sub == 0) {
-            LAUNCH_ON(st, k_warm, cb, w);
-            LAUNCH_ON(st, k_apply, nb, w);
+            LAUNCH_TH(st, k_warm, w.contact_cap, threads, w);
+            LAUNCH_TH(st, k_apply, w.n, threads, w);
         }
         for (uint32_t it = 0; it < w.p.solve_iterations; ++it) {
             // solver_block_solve.wgsl:85-87: the residual round is the one where the round
             // counter (reset by integrate, bumped by every apply) equals the iteration count,
             // which only the warm-started first substep reaches.
-            LAUNCH_ON(st, k_solve, cb, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
-            LAUNCH_ON(st, k_apply, nb, w);
+            LAUNCH_TH(st, k_solve, w.contact_cap, threads, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
+            LAUNCH_TH(st, k_apply, w.n, threads, w);
         }
-        LAUNCH_ON(st, k_advance, db, w);
+        LAUNCH_TH(st, k_advance, w.d, threads, w);
         for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
-            LAUNCH_ON(st, k_position_block, cb, w);
-            LAUNCH_ON(st, k_position_apply, nb, w);
+            LAUNCH_TH(st, k_position_block, w.contact_cap, threads, w);
+            LAUNCH_TH(st, k_position_apply, w.n, threads, w);
         }
     }
-    LAUNCH_ON(st, k_impulses_store, cb, w);
+    LAUNCH_TH(st, k_impulses_store, w.contact_cap, threads, w);
+    (void)nb;
     return DYN_OK;
 }
 
@@ -2514,7 +2533,7 @@ int world_step(dyn_world* world, float dt) {
         LAUNCH(k_tape_record, blocks_for(std::max(w.n, w.contact_cap)), w, world->tape_states, world->tape_contacts,
                world->tape_count, world->tape_error);
     }
-    DYN_TRY(solver_substeps(w, st));
+    DYN_TRY(solver_substeps(w, st, world->solver_threads));
     LAUNCH(k_sleep, db, w);
     LAUNCH(k_thaw, cb, w);
     LAUNCH(k_archive, cb, w, world->freeze_flags);
@@ -2756,6 +2775,26 @@ int dyn_world_upload_shapes(dyn_world* world, const dyn_shape_source* sources,
     world->w.sources = static_cast<const dyn_shape_source*>(world->shape_sources);
     world->w.vertices = static_cast<const float4*>(world->shape_vertices);
     world->w.source_count = source_count;
+    // the hull sources' support cells (support_cells.h), built on the host and uploaded
+    if (world->support_cells) DYN_HIP(hipFree(world->support_cells));
+    if (world->support_cands) DYN_HIP(hipFree(world->support_cands));
+    world->support_cells = world->support_cands = nullptr;
+    world->w.support_cells = nullptr;
+    world->w.support_cands = nullptr;
+    if (source_count) {
+        std::vector<uint32_t> cells(2u * static_cast<size_t>(source_count) * kSupportCells);
+        uint32_t total = 0;
+        DYN_TRY(dyn_hull_support_cells(sources, source_count, vertices, vertex_count, cells.data(), nullptr, 0u, &total));
+        std::vector<uint16_t> cands(total ? total : 1u);
+        DYN_TRY(dyn_hull_support_cells(sources, source_count, vertices, vertex_count, cells.data(), cands.data(), total,
+                                       &total));
+        DYN_HIP(hipMalloc(&world->support_cells, 4u * cells.size()));
+        DYN_HIP(hipMalloc(&world->support_cands, 2u * cands.size()));
+        DYN_HIP(hipMemcpy(world->support_cells, cells.data(), 4u * cells.size(), hipMemcpyHostToDevice));
+        DYN_HIP(hipMemcpy(world->support_cands, cands.data(), 2u * cands.size(), hipMemcpyHostToDevice));
+        world->w.support_cells = static_cast<const uint2*>(world->support_cells);
+        world->w.support_cands = static_cast<const uint16_t*>(world->support_cands);
+    }
     return DYN_OK;
 }
 
@@ -2771,6 +2810,8 @@ int dyn_world_destroy(dyn_world* world) {
     }
     if (world->shape_sources) (void)hipFree(world->shape_sources);
     if (world->shape_vertices) (void)hipFree(world->shape_vertices);
+    if (world->support_cells) (void)hipFree(world->support_cells);
+    if (world->support_cands) (void)hipFree(world->support_cands);
     if (world->edit_buf) (void)hipFree(world->edit_buf);
     if (world->run_buf) (void)hipFree(world->run_buf);
     if (world->own_stream) (void)hipStreamDestroy(world->stream);
@@ -2842,7 +2883,8 @@ int dyn_world_step(dyn_world* world, float dt) {
     if (world->hull_colliders && world->shape_sources == nullptr) {
         return dyn::fail(DYN_E_INVALID, "dyn_world_step: hull colliders need dyn_world_upload_shapes first");
     }
-    DYN_TRY(plan_capacity(world));
+    if (!world->tuning) DYN_TRY(plan_capacity(world));
+    world->last_dt = dt;
     // profiled and recorded steps run eagerly
     if (!world->graphs || dyn::profiling() || world->tape_states) {
         DYN_TRY(world_step(world, dt));
@@ -2855,7 +2897,7 @@ int dyn_world_step(dyn_world* world, float dt) {
         }
         DYN_HIP(hipGraphLaunch(world->graph[p], world->stream));
     }
-    return record_demand(world);
+    return world->tuning ? DYN_OK : record_demand(world);
 }
 
 int dyn_world_read_work(dyn_world* world, uint32_t* out) {
@@ -2887,6 +2929,85 @@ int dyn_world_reserve(dyn_world* world, uint32_t pair_capacity, uint32_t contact
         return dyn::fail(DYN_E_INVALID, "dyn_world_reserve: a differentiable rollout holds the world's capacities");
     }
     return grow_streams(world, pair_capacity, contact_capacity);
+}
+
+int dyn_world_set_solver_threads(dyn_world* world, uint32_t threads) {
+    if (!world) return dyn::fail(DYN_E_INVALID, "dyn_world_set_solver_threads: null world");
+    if (threads != 64u && threads != 128u && threads != 256u) {
+        return dyn::fail(DYN_E_INVALID, "dyn_world_set_solver_threads: threads must be 64, 128 or 256");
+    }
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    world->solver_threads = threads;
+    drop_graph(world);
+    return DYN_OK;
+}
+
+int dyn_world_tune_solver(dyn_world* world, uint32_t steps, uint32_t* threads_out, float* ms_out) {
+    using namespace dyn;
+    if (!world) return fail(DYN_E_INVALID, "dyn_world_tune_solver: null world");
+    if (!world->pending_edits.empty()) return fail(DYN_E_INVALID, "dyn_world_tune_solver: body edits are queued");
+    if (world->tape_states) return fail(DYN_E_INVALID, "dyn_world_tune_solver: a tape is being recorded");
+    if (world->hull_colliders && world->shape_sources == nullptr) {
+        return fail(DYN_E_INVALID, "dyn_world_tune_solver: hull colliders need dyn_world_upload_shapes first");
+    }
+    if (steps == 0u) steps = 3u;
+    const float dt = world->last_dt > 0.0f ? world->last_dt : 1.0f / 60.0f;
+    size_t bytes = 0;
+    DYN_TRY(dyn_world_snapshot_bytes(world, &bytes));
+    DYN_HIP(hipStreamSynchronize(world->stream));
+    void* snap = nullptr;
+    DYN_HIP(hipMalloc(&snap, bytes));
+    hipEvent_t start = nullptr, stop = nullptr;
+    const uint32_t parity = world->parity, kept_threads = world->solver_threads;
+    // back to the snapshot: the contact and archive buffers in their order at the snapshot, then
+    // the durable state copied in
+    auto reset = [&]() {
+        if (world->parity != parity) swap_contact_buffers(world);
+        return dyn_world_restore(world, snap);
+    };
+    auto finish = [&](int rc) {
+        (void)hipStreamSynchronize(world->stream);
+        if (rc == DYN_OK) rc = reset();  // the world as it was
+        (void)hipStreamSynchronize(world->stream);
+        world->tuning = false;
+        drop_graph(world);
+        if (start) (void)hipEventDestroy(start);
+        if (stop) (void)hipEventDestroy(stop);
+        (void)hipFree(snap);
+        return rc;
+    };
+    world->tuning = true;
+    int rc = dyn_world_snapshot(world, snap);
+    if (rc == DYN_OK && (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess)) {
+        rc = fail(DYN_E_HIP, "dyn_world_tune_solver: events");
+    }
+    const uint32_t options[3] = {64u, 128u, 256u};
+    float best_ms = 0.0f;
+    uint32_t best = kept_threads;
+    for (uint32_t i = 0; i < 3u && rc == DYN_OK; ++i) {
+        world->solver_threads = options[i];
+        drop_graph(world);
+        if ((rc = reset()) != DYN_OK) break;
+        if ((rc = dyn_world_step(world, dt)) != DYN_OK) break;  // captures the step
+        if (hipEventRecord(start, world->stream) != hipSuccess) rc = fail(DYN_E_HIP, "dyn_world_tune_solver: event");
+        for (uint32_t k = 0; k < steps && rc == DYN_OK; ++k) rc = dyn_world_step(world, dt);
+        if (rc != DYN_OK) break;
+        float ms = 0.0f;
+        if (hipEventRecord(stop, world->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess ||
+            hipEventElapsedTime(&ms, start, stop) != hipSuccess) {
+            rc = fail(DYN_E_HIP, "dyn_world_tune_solver: timing");
+            break;
+        }
+        if (ms_out) ms_out[i] = ms / static_cast<float>(steps);
+        if (i == 0u || ms < best_ms) {
+            best_ms = ms;
+            best = options[i];
+        }
+    }
+    rc = finish(rc);
+    world->solver_threads = rc == DYN_OK ? best : kept_threads;
+    if (threads_out) *threads_out = world->solver_threads;
+    return rc;
 }
 
 int dyn_world_wait(dyn_world* world) {
@@ -3005,6 +3126,8 @@ int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io) {
     w.owner = const_cast<uint32_t*>(io->d_owners);
     w.sources = io->d_sources;
     w.vertices = reinterpret_cast<const float4*>(io->d_vertices);
+    w.support_cells = reinterpret_cast<const uint2*>(io->d_support_cells);
+    w.support_cands = io->d_support_candidates;
     w.source_count = io->source_count;
     w.counters = const_cast<uint32_t*>(io->d_counters);
     w.pmaj = const_cast<uint32_t*>(io->d_pair_major);

@@ -17,6 +17,9 @@ from ._native import call, lib
 from .device import DeviceBuffer, Stream
 
 
+SUPPORT_CELLS = 6 * 8 * 8  # cube-map cells per hull source (support_cells.h)
+
+
 class NarrowphaseIO(ctypes.Structure):
     _fields_ = [
         ("params", ctypes.c_void_p),
@@ -37,14 +40,29 @@ class NarrowphaseIO(ctypes.Structure):
         ("d_valid", ctypes.c_void_p),
         ("d_scratch", ctypes.c_void_p),
         ("scratch_bytes", ctypes.c_size_t),
+        ("d_support_cells", ctypes.c_void_p),
+        ("d_support_candidates", ctypes.c_void_p),
     ]
+
+
+def support_cells(sources: np.ndarray, vertices: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The hull sources' support cells (dyn_hull_support_cells): 2 words per source and cell,
+    and the candidate vertex indices."""
+    cells = np.zeros((len(sources), SUPPORT_CELLS, 2), np.uint32)
+    total = ctypes.c_uint32()
+    call("dyn_hull_support_cells", sources.ctypes.data, len(sources), vertices.ctypes.data, len(vertices),
+         cells.ctypes.data, None, 0, ctypes.byref(total))
+    cands = np.zeros(max(total.value, 1), np.uint16)
+    call("dyn_hull_support_cells", sources.ctypes.data, len(sources), vertices.ctypes.data, len(vertices),
+         cells.ctypes.data, cands.ctypes.data, total.value, ctypes.byref(total))
+    return cells, cands
 
 
 class Narrowphase:
     """Device copies of a scene's records; `run(pairs)` enqueues the pass over the (P, 2)
     collider pairs and `read()` returns (contacts_raw, contact_valid) for them."""
 
-    def __init__(self, scene, pair_capacity: int, stream: Stream | None = None):
+    def __init__(self, scene, pair_capacity: int, stream: Stream | None = None, support_tables: bool = True):
         self.scene = scene
         self.pair_capacity = int(pair_capacity)
         self.stream = stream
@@ -55,10 +73,16 @@ class Narrowphase:
         self.owners = DeviceBuffer.from_array(np.ascontiguousarray(scene.owners, dtype=np.uint32))
         sources = scene.sources if scene.sources is not None else np.zeros(0, abi.SHAPE_SOURCE)
         self.source_count = len(sources)
-        self.sources = self.vertices = None
+        self.sources = self.vertices = self.cells = self.cands = None
         if self.source_count:
-            self.sources = DeviceBuffer.from_array(np.ascontiguousarray(sources, dtype=abi.SHAPE_SOURCE))
-            self.vertices = DeviceBuffer.from_array(np.ascontiguousarray(scene.vertices, dtype=np.float32))
+            src = np.ascontiguousarray(sources, dtype=abi.SHAPE_SOURCE)
+            verts = np.ascontiguousarray(scene.vertices, dtype=np.float32)
+            self.sources = DeviceBuffer.from_array(src)
+            self.vertices = DeviceBuffer.from_array(verts)
+            if support_tables:
+                cells, cands = support_cells(src, verts)
+                self.cells = DeviceBuffer.from_array(cells)
+                self.cands = DeviceBuffer.from_array(cands)
         cap = max(self.pair_capacity, 1)
         self.pair_major = DeviceBuffer(cap * 4)
         self.pair_minor = DeviceBuffer(cap * 4)
@@ -87,7 +111,8 @@ class Narrowphase:
             len(self.scene.colliders), self.colliders.ptr, self.owners.ptr,
             self.sources.ptr if self.sources else None, self.vertices.ptr if self.vertices else None,
             self.source_count, self.pair_capacity, self.pair_major.ptr, self.pair_minor.ptr,
-            self.counters.ptr, self.contacts.ptr, self.valid.ptr, self.scratch.ptr, self.scratch.nbytes)
+            self.counters.ptr, self.contacts.ptr, self.valid.ptr, self.scratch.ptr, self.scratch.nbytes,
+            self.cells.ptr if self.cells else None, self.cands.ptr if self.cands else None)
         call("dyn_narrowphase", ctypes.c_void_p(self.stream.ptr if self.stream else None), ctypes.byref(io))
 
     def read(self) -> tuple[np.ndarray, np.ndarray]:

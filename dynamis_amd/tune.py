@@ -53,6 +53,31 @@ def cache_path(directory: str | os.PathLike | None = None) -> pathlib.Path:
     return base / ("sort-tilings-" + re.sub(r"[^A-Za-z0-9_.-]+", "_", device_name()) + ".json")
 
 
+def autotune_solver(world, directory: str | os.PathLike | None = None, retune: bool = False,
+                    steps: int = 3) -> dict:
+    """Tunes (or loads from the per-GPU cache, keyed by the contact capacity's bucket) the
+    workgroup size of a world's solver substeps; the world's state is left as it was."""
+    if hasattr(world, "_ensure"):
+        world._ensure()
+    path = cache_path(directory).with_name(cache_path(directory).name.replace("sort-tilings-", "solver-threads-"))
+    key = str(bucket(world.capacity))
+    data = {}
+    if path.exists() and not retune:
+        data = json.loads(path.read_text())
+        if data.get("device") != device_name():
+            data = {}
+    sizes = data.get("threads", {})
+    if key in sizes:
+        world.set_solver_threads(int(sizes[key]))
+        return {"threads": int(sizes[key]), "from_cache": True, "cache": str(path)}
+    threads, ms = world.tune_solver(steps)
+    sizes[key] = threads
+    path.parent.mkdir(parents=True, exist_ok=True)
+    path.write_text(json.dumps({"device": device_name(), "threads": sizes}))
+    return {"threads": threads, "ms_per_step": {"64": ms[0], "128": ms[1], "256": ms[2]}, "from_cache": False,
+            "cache": str(path)}
+
+
 def autotune_world(world, directory: str | os.PathLike | None = None, retune: bool = False) -> dict:
     """Tunes (or loads from the per-GPU cache) the tilings of a World's or Batch's sorts and
     makes its next step re-capture. Returns {capacity: (keys per lane, groups)} for its sorts and

@@ -381,6 +381,20 @@ class World(BodyCommands):
             return default_capacities(len(self._bodies), None, self._pair_request)[1]
         return self.capacities()[0]
 
+    def set_solver_threads(self, threads: int) -> None:
+        """Workgroup size (64, 128, 256) of the solver substeps' kernels; results do not change."""
+        self._ensure()
+        call("dyn_world_set_solver_threads", self._handle, int(threads))
+
+    def tune_solver(self, steps: int = 3) -> tuple[int, list[float]]:
+        """Times the solver's workgroup sizes on this world's next steps (from a snapshot, then
+        restored) and keeps the fastest: (size, ms per step of 64, 128, 256)."""
+        self._ensure()
+        threads = ctypes.c_uint32()
+        ms = (ctypes.c_float * 3)()
+        call("dyn_world_tune_solver", self._handle, int(steps), ctypes.byref(threads), ms)
+        return int(threads.value), [float(v) for v in ms]
+
     def reserve(self, pair_capacity: int = 0, contact_capacity: int = 0) -> None:
         """Grows the pair and contact streams to at least the given capacities now."""
         self._ensure()

@@ -190,6 +190,11 @@ typedef struct dyn_narrowphase_io {
                                             pair's slot holds its contact, 0 elsewhere */
     void* d_scratch;
     size_t scratch_bytes;
+    /* optional: the sources' support cells from dyn_hull_support_cells (device copies of its
+       `cells` records and `candidates`), or NULL for the linear support scans; the result is
+       the same */
+    const uint32_t* d_support_cells;
+    const uint16_t* d_support_candidates;
 } dyn_narrowphase_io;
 
 /* Scratch bytes dyn_narrowphase needs. */
@@ -344,6 +349,24 @@ int dyn_hull_build(const float* points, uint32_t count, float* vertices, uint32_
  * closed triangle hull, accumulated in FP64 (crates/dynamis-hull/src/solid.rs:62-105). */
 int dyn_hull_solid(const float* vertices, uint32_t vertex_count, const uint32_t* triangles,
                    uint32_t triangle_count, float* volume, float* centroid, float* second);
+/* Support cells of hull shape sources, for the device's hull support (no counterpart: the
+ * reference scans every vertex, convex.wgsl:1-85, and so does the oracle): per source and cell
+ * of a cube map of 6 x 8 x 8 cells over local directions, the ascending list of the vertices
+ * that can be the direction's support vertex (the first vertex of largest single-precision
+ * dot). A vertex is left out of a cell only when another exceeds it by more than 1e-5 x the
+ * hull's largest |x|+|y|+|z| everywhere in the slightly widened cell, far above the rounding of
+ * a single-precision dot, so scanning a cell's list gives the linear scan's vertex. Sources of
+ * fewer than 12 or more than 4096 vertices, or with coordinates beyond 1e6, keep the linear scan.
+ * Inputs are the sources and float4 vertex rows of dyn_world_upload_shapes (host). `cells`
+ * receives two words per source and cell: word 0 the candidate count in bits 0-7 (0xFF: the cell
+ * keeps the linear scan) and the first two candidates' vertex indices in bits 8-19 and 20-31;
+ * word 1 the third and fourth in bits 0-11 and 12-23 for counts up to 4, else the offset of the
+ * cell's list in `candidates`. `candidates` (optional) receives every cell's ascending index list
+ * (*candidate_count indices); a capacity below that count fails with DYN_E_CAPACITY after setting
+ * it. dyn_world_upload_shapes builds them itself. */
+int dyn_hull_support_cells(const dyn_shape_source* sources, uint32_t source_count, const float* vertices,
+                           uint32_t vertex_count, uint32_t* cells, uint16_t* candidates, uint32_t capacity,
+                           uint32_t* candidate_count);
 /* inertia_inverse (mass.rs:259-280); fails on a tensor that is not positive definite. */
 int dyn_inertia_inverse(const float* inertia, float* inverse);
 /* BodyStateRecord::initial + BodyDescriptorRecord::build + ColliderRecord::build for one body
@@ -425,6 +448,17 @@ int dyn_world_read_work(dyn_world* world, uint32_t* out);
 int dyn_world_set_graphs(dyn_world* world, int enabled);
 /* Blocks until every enqueued step has finished. */
 int dyn_world_wait(dyn_world* world);
+/* The workgroup size of the solver substeps' per-contact and per-row kernels (k_integrate,
+ * k_solver_bodies, k_solve_prep, k_warm, k_solve, k_apply, k_advance, k_position_block,
+ * k_position_apply, k_impulses_store): 64, 128 or 256 (default). Each of them indexes its contact
+ * or row by the global thread index alone, so every size gives the same result. No counterpart in
+ * the reference, whose programs have a fixed workgroup size. dyn_world_tune_solver times `steps`
+ * steps (3 when 0) of the world with each size from a snapshot of its current state (at the last
+ * step's dt), keeps the fastest, restores the state, and returns the size and the three
+ * milliseconds-per-step (either pointer may be NULL); it records no capacity demand, allocates a
+ * snapshot buffer while it runs, and fails while edits are queued or a tape is recorded. */
+int dyn_world_set_solver_threads(dyn_world* world, uint32_t threads);
+int dyn_world_tune_solver(dyn_world* world, uint32_t steps, uint32_t* threads_out, float* ms_out);
 /* Capacity planning (World::apply_plan, crates/dynamis-world/src/backend/mod.rs:194-230, with the
  * stream plans of crates/dynamis-broadphase/src/capacity.rs:27-80 and crates/dynamis-rigid/src/
  * capacity.rs:50-140). Every step's demand (its uncapped pair total, and the larger of its

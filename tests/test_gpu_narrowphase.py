@@ -67,3 +67,41 @@ def test_empty_and_foreign_pairs(device):
     foreign = np.array([[0, k], [k + 5, 1], [1, 2]], np.uint32)
     _, valid = device_stage(scene, foreign)
     assert valid[0] == 0 and valid[1] == 0
+
+
+def hull_pile(count: int, sources: int = 32, spacing: float = 1.4):
+    """Ellipsoid hulls alone on a lattice above a plane."""
+    from tests.worlds import BodyDesc, config, ellipsoid_hulls, encode
+    side = int(np.ceil(np.sqrt(count / 4.0)))
+    bodies = [BodyDesc.plane()]
+    for i in range(count):
+        layer, cell = divmod(i, side * side)
+        gx, gz = divmod(cell, side)
+        bodies.append(BodyDesc.hull(i % sources, position=((gx - side / 2) * spacing, 1.5 + layer * spacing,
+                                                          (gz - side / 2) * spacing), density=1000.0))
+    return encode(bodies, config(), ellipsoid_hulls(sources, seed=11))
+
+
+def test_support_cells_change_no_bit(device):
+    """The hull support over its support cells (dyn_hull_support_cells) against the linear scan
+    over every vertex, on the pass over a settled 3000-body pile with 64 hull sources and on the
+    hull-heavy pairs of a pile of hulls alone: every byte of every contact equal."""
+    for scene, steps in ((pile_scene(3000, seed=5, hull_sources=64, spacing=1.5), 90), (hull_pile(1500), 60)):
+        world = World()
+        world.create(scene)
+        for _ in range(steps):
+            world.step(DT)
+        states = world.states()
+        pairs = world.pairs()
+        world.close()
+        staged = with_states(scene, states)
+        tables = Narrowphase(staged, max(len(pairs), 1))
+        linear = Narrowphase(staged, max(len(pairs), 1), support_tables=False)
+        assert tables.cells is not None and linear.cells is None
+        tables.run(pairs)
+        linear.run(pairs)
+        raw_t, valid_t = tables.read()
+        raw_l, valid_l = linear.read()
+        assert valid_t.tobytes() == valid_l.tobytes()
+        assert raw_t[valid_t == 1].tobytes() == raw_l[valid_l == 1].tobytes()
+        assert (valid_t == 1).sum() > 500

@@ -52,3 +52,46 @@ def test_autotuned_world_steps_identically(device, tmp_path):
         assert again.tobytes() == plain.tobytes()
     finally:
         tune.load(np.zeros((0, 4), np.uint32))
+
+
+@pytest.mark.parametrize("threads", [64, 128])
+def test_solver_workgroup_sizes_step_identically(device, threads):
+    """The solver substeps' kernels in workgroups of 64 or 128 give the 256-thread result bit for
+    bit, over a settling drop with many contacts."""
+    scene = encode(scenes.cube_drop(1500, seed=4))
+
+    def run(size):
+        world = World()
+        world.create(scene)
+        if size:
+            world.set_solver_threads(size)
+        for _ in range(60):
+            world.step(1 / 60)
+        out = world.states(), world.inspect_contacts()
+        world.close()
+        return out
+
+    (s0, c0), (s1, c1) = run(None), run(threads)
+    assert len(c0) > 500
+    assert s1.tobytes() == s0.tobytes() and c1.tobytes() == c0.tobytes()
+
+
+def test_solver_tuning_leaves_the_world_unchanged(device):
+    scene = encode(scenes.cube_drop(1500, seed=5))
+    plain, tuned = World(), World()
+    plain.create(scene)
+    tuned.create(scene)
+    for _ in range(40):
+        plain.step(1 / 60)
+        tuned.step(1 / 60)
+    before = tuned.states().copy()
+    threads, ms = tuned.tune_solver(3)
+    assert threads in (64, 128, 256) and all(m > 0 for m in ms)
+    assert tuned.states().tobytes() == before.tobytes()
+    for _ in range(20):
+        plain.step(1 / 60)
+        tuned.step(1 / 60)
+    assert tuned.states().tobytes() == plain.states().tobytes()
+    assert tuned.inspect_contacts().tobytes() == plain.inspect_contacts().tobytes()
+    plain.close()
+    tuned.close()
