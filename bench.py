@@ -42,9 +42,10 @@ isolated sort + broadphase pass (`broadphase`) and the adjoint's backward/forwar
 count: the timed steps are re-run eagerly from a snapshot taken at their start with every launch
 bracketed by events (the `stages` table), then re-run again with only that kernel's launches
 bracketed, which gives its average launch duration (both re-runs end in the timed run's states
-bit for bit: `config.reruns_identical`). The second re-run is marked with roctxProfilerResume /
-roctxProfilerPause, so `rocprofv3 --selected-regions` traces exactly those launches. Bytes per
-launch come from the step's counts at the window's start and end. `cpu_baseline` times the C
+bit for bit: `config.reruns_identical`). The second re-run is bracketed by roctxProfilerResume /
+roctxProfilerPause marks. Bytes per launch come from the step's counts at the window's start and
+end; `traffic` (HBM bytes per launch from rocprofv3 counter passes) comes from
+profiles/traffic.json when those passes were taken on the same kernel and workload. `cpu_baseline` times the C
 oracle on a bounded sample of the same workload.
 """
 
@@ -190,9 +191,9 @@ def kernel_bytes(name: str, c: dict) -> float:
 
 
 class Roctx:
-    """roctxProfilerResume / roctxProfilerPause around the eager window whose kernel launches the
-    roofline figure times, so that `rocprofv3 --selected-regions` traces exactly those launches
-    (no-ops when no profiler is attached; absent library: nothing is marked)."""
+    """roctxProfilerResume / roctxProfilerPause marks around the eager window whose kernel
+    launches the roofline figure times (no-ops when no profiler is attached; absent library:
+    nothing is marked)."""
 
     def __init__(self):
         import ctypes
@@ -250,6 +251,11 @@ def settled_run(world, run: int, steps: int, warmup: int):
     for _ in range(warmup):
         world.step(DT)
     world.wait()
+    solver = None
+    if TUNE:  # the solver's workgroup size, on the settled scene (its state restored afterwards)
+        from dynamis_amd import tune
+        solver = tune.autotune_solver(world)
+        progress(f"solver workgroup size {solver['threads']}")
     snap = world.snapshot()
     before = world_counts(world)
     ms = timed_steps([world], stream, steps)
@@ -302,7 +308,7 @@ def settled_run(world, run: int, steps: int, warmup: int):
     progress(f"eager re-runs done (roofline kernel {dominant})")
     return {"ms": ms, "stages": stages, "counts": mean, "measured": measured, "timed": timed,
             "final_states": final_states, "with_warmup_ms": float(all_ms.value), "settle": settle,
-            "reruns_identical": same}
+            "reruns_identical": same, "solver": solver}
 
 
 def sleep_off(scene):
@@ -794,6 +800,24 @@ def run_adjoint(reps: int = 3) -> dict:
 CPU_WORKERS = 16  # host cores a GPU box grants one GPU's job
 
 
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(roofline: dict, workload: str) -> None:
+    """roofline.traffic from the counter passes recorded in profiles/traffic.json: HBM bytes per
+    launch of the roofline kernel on this workload (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE,
+    each scaled by what the counter reads for a copy of known size), when they were taken."""
+    try:
+        with open(TRAFFIC) as f:
+            entry = json.load(f).get(workload, {}).get(roofline["kernel"])
+    except (OSError, ValueError):
+        return
+    if entry:
+        roofline["traffic"] = entry["bytes_per_launch"]
+        roofline["traffic_vs_algorithmic"] = round(entry["bytes_per_launch"] / max(roofline["bytes_per_launch"], 1.0), 2)
+        roofline["traffic_source"] = entry["source"]
+
+
 def world_report(wl: str, r: dict, steps: int, warmup: int, cpu_sample: int | None) -> dict:
     """The line of a single-world workload's settled window (pile, cubes)."""
     spec = WORKLOADS[wl]
@@ -802,6 +826,7 @@ def world_report(wl: str, r: dict, steps: int, warmup: int, cpu_sample: int | No
     ms = r["ms"]
     if roofline is not None:
         roofline["step"] = step_roofline(table, steps, ms / steps)
+        measured_traffic(roofline, wl)
     bodies = r["bodies"]
     sleep = None
     if wl == "pile":
@@ -827,6 +852,7 @@ def world_report(wl: str, r: dict, steps: int, warmup: int, cpu_sample: int | No
               "reference_style_note": "benchmark.rs divides by a wall time that includes its warm-up "
                                       "steps: here the warm-up and timed steps together",
               "reruns_identical": r["reruns_identical"],
+              "solver_workgroup": r.get("solver"),
               "at_end": {k: r["measured"][k] for k in ("PAIRS", "CONTACTS", "ISLANDS", "SOLVER_ROWS",
                                                       "RESTING", "ACTIVE", "REFUSED_PAIRS",
                                                       "REFUSED_CONTACTS", "ENTRY_FAULTS")}}
