@@ -402,57 +402,110 @@ __device__ inline v3 simplex_expand_dir(const SimplexPoint* s, uint32_t count) {
     return mk(0.0f, 1.0f, 0.0f);
 }
 
-template <int kLanes = 1>
-__device__ inline Closest convex_closest(const Shape& first, const Shape& second, const ShapeSet& ss,
-                                         SimplexPoint* simplex, uint32_t* out_count) {
-    uint32_t count = 0;
+// GJK's loop state between iterations: the simplex, the closest point of its hull to the origin,
+// and the next iteration's index. A pair can stop after any iteration and go on later (another
+// kernel) from this state with exactly the same arithmetic.
+struct GjkState {
+    SimplexPoint simplex[4];
+    uint32_t count;
+    SimplexResult closest;
+    uint32_t iter;
+};
+
+constexpr uint32_t kGjkIterations = 24;  // convex.wgsl:478
+
+// The result convex_closest returns when its loop runs out without an answer (or before it has
+// one): the centres and their normalized offset.
+__device__ inline Closest closest_default(const Shape& first, const Shape& second) {
     Closest r;
     r.distance = 0.0f;
     r.point_a = first.center;
     r.point_b = second.center;
     r.normal = sign_normalize(sub(second.center, first.center));
     r.penetrating = false;
+    return r;
+}
+
+// convex_closest (convex.wgsl:464-521), the part before the loop.
+template <int kLanes = 1>
+__device__ inline void gjk_begin(const Shape& first, const Shape& second, const ShapeSet& ss, GjkState* g) {
     v3 dir = sub(second.center, first.center);
     if (len3(dir) < 1e-8f) dir = mk(1.0f, 0.0f, 0.0f);
-    simplex[0] = support_pair<kLanes>(first, second, ss, dir);
-    count = 1;
-    SimplexResult closest = simplex_closest(simplex, count);
+#pragma unroll
+    for (uint32_t i = 1; i < 4; ++i) g->simplex[i].w = g->simplex[i].a = g->simplex[i].b = zero3v();
+    g->simplex[0] = support_pair<kLanes>(first, second, ss, dir);
+    g->count = 1;
+    g->closest = simplex_closest(g->simplex, g->count);
+    g->iter = 0;
+}
+
+// The loop of convex_closest from iteration g->iter up to (not including) `limit`. Returns true
+// with *r set when the loop has ended (answered, or all kGjkIterations run), false when it
+// stopped at `limit` with the state in *g.
+template <int kLanes = 1>
+__device__ inline bool gjk_run(const Shape& first, const Shape& second, const ShapeSet& ss, GjkState* g,
+                               uint32_t limit, Closest* r) {
+    *r = closest_default(first, second);
     const float tiny = 1e-4f * fmn(shape_scale(first), shape_scale(second));
-    for (uint32_t iter = 0; iter < 24u; ++iter) {
+    SimplexPoint* simplex = g->simplex;
+    uint32_t count = g->count;
+    SimplexResult closest = g->closest;
+    uint32_t iter = g->iter;
+    bool done = false;
+    for (; iter < kGjkIterations; ++iter) {
+        if (iter >= limit) break;
         if (closest.penetrating || len3(closest.v) < tiny) {
             while (count < 4u) {
                 const v3 d = simplex_expand_dir(simplex, count);
                 set_slot(simplex, count, support_pair<kLanes>(first, second, ss, d));
                 ++count;
             }
-            r.penetrating = true;
-            r.distance = 0.0f;
-            r.point_a = closest.point_a;
-            r.point_b = closest.point_b;
+            r->penetrating = true;
+            r->distance = 0.0f;
+            r->point_a = closest.point_a;
+            r->point_b = closest.point_b;
+            done = true;
             break;
         }
         const v3 nv = neg(closest.v);
         const SimplexPoint sp = support_pair<kLanes>(first, second, ss, nv);
         if (dot3(sp.w, nv) <= dot3(closest.v, nv) + 1e-4f * fmx(1.0f, dot3(closest.v, closest.v))) {
-            r.distance = len3(closest.v);
-            r.point_a = closest.point_a;
-            r.point_b = closest.point_b;
-            if (r.distance > 1e-6f) r.normal = sign_normalize(sub(closest.point_b, closest.point_a));
+            r->distance = len3(closest.v);
+            r->point_a = closest.point_a;
+            r->point_b = closest.point_b;
+            if (r->distance > 1e-6f) r->normal = sign_normalize(sub(closest.point_b, closest.point_a));
+            done = true;
             break;
         }
         count = simplex_reduce(simplex, closest.lambdas);
         if (simplex_holds(simplex, count, sp.w, shape_scale(second))) {
-            r.distance = len3(closest.v);
-            r.point_a = closest.point_a;
-            r.point_b = closest.point_b;
-            if (r.distance > 1e-6f) r.normal = sign_normalize(sub(closest.point_b, closest.point_a));
+            r->distance = len3(closest.v);
+            r->point_a = closest.point_a;
+            r->point_b = closest.point_b;
+            if (r->distance > 1e-6f) r->normal = sign_normalize(sub(closest.point_b, closest.point_a));
+            done = true;
             break;
         }
         set_slot(simplex, count, sp);
         ++count;
         closest = simplex_closest(simplex, count);
     }
-    *out_count = count;
+    g->count = count;
+    g->closest = closest;
+    g->iter = iter;
+    return done || iter >= kGjkIterations;
+}
+
+template <int kLanes = 1>
+__device__ inline Closest convex_closest(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                         SimplexPoint* simplex, uint32_t* out_count) {
+    GjkState g;
+    gjk_begin<kLanes>(first, second, ss, &g);
+    Closest r;
+    gjk_run<kLanes>(first, second, ss, &g, kGjkIterations, &r);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) simplex[i] = g.simplex[i];
+    *out_count = g.count;
     return r;
 }
 
@@ -723,9 +776,9 @@ __device__ inline Hit make_hit(float distance, v3 point, v3 normal) {
 // (returns true with the hit); for a penetrating pair returns false with GJK's final simplex,
 // which EPA (one wave) and convex_hit_finish take up.
 template <int kLanes = 1>
-__device__ inline bool convex_hit_gjk(const Shape& first, const Shape& second, const ShapeSet& ss, Hit* hit,
-                                      SimplexPoint* simplex, uint32_t* count) {
-    const Closest closest = convex_closest<kLanes>(first, second, ss, simplex, count);
+__device__ inline bool convex_hit_after_gjk(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                            const Closest& closest, Hit* hit, const SimplexPoint* simplex,
+                                            const uint32_t* count) {
     if (closest.penetrating) return false;
     const v3 centers = sub(second.center, first.center);
     const Probe probe = penetration_probe<kLanes>(first, second, ss, simplex, *count, true);
@@ -738,6 +791,13 @@ __device__ inline bool convex_hit_gjk(const Shape& first, const Shape& second, c
         *hit = make_hit(kBig, zero3v(), zero3v());  // NO_HIT
     }
     return true;
+}
+
+template <int kLanes = 1>
+__device__ inline bool convex_hit_gjk(const Shape& first, const Shape& second, const ShapeSet& ss, Hit* hit,
+                                      SimplexPoint* simplex, uint32_t* count) {
+    const Closest closest = convex_closest<kLanes>(first, second, ss, simplex, count);
+    return convex_hit_after_gjk<kLanes>(first, second, ss, closest, hit, simplex, count);
 }
 
 // convex_hit, the rest of a penetrating pair once EPA has run on the simplex.

@@ -82,7 +82,8 @@ struct W {
     uint8_t* pkind;        // per convex pair: the unordered class of its two shape kinds
     uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
     uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
-    uint32_t* work_total;  // device words: entries on the worklist of each mode, then the hull-class GJK pairs
+    uint32_t* work_total;  // device words: entries on the worklist of modes 0-2, the hull-class GJK pairs,
+                           // the resumed GJK pairs (mode 4)
     uint32_t* work_keys;   // GJK worklist sort keys: class, then the hull sources
     uint32_t* work_minor;  // the sort's unused minor lane
     uint32_t* dest;
@@ -395,6 +396,7 @@ constexpr uint32_t kPairEpa = 3;      // GJK found it penetrating; simplex parke
 constexpr uint32_t kPairEpaDone = 4;  // EPA result parked for k_narrow_finish
 constexpr uint32_t kPairHit = 5;      // GJK hit within the margin parked for k_narrow_finish
 constexpr uint32_t kPairPlane = 6;    // plane against a hull or cylinder, for k_narrow_finish
+constexpr uint32_t kPairGjkResume = 7;  // GJK's state parked for k_narrow_convex_resume
 
 // Material, identity and anchors of a generated manifold (narrowphase.wgsl:665-684).
 __device__ inline void write_contact(const W& w, uint32_t index, uint32_t fs, uint32_t ss, const Body& first,
@@ -500,6 +502,7 @@ __device__ __forceinline__ uint32_t work_class(const W& w, uint32_t index, uint3
     const uint32_t v = w.valid[index];
     const bool take = mode == 0u   ? v == kPairGjk
                       : mode == 1u ? (v == kPairEpaDone || v == kPairHit || v == kPairPlane)
+                      : mode == 4u ? v == kPairGjkResume
                                    : v == kPairEpa;
     return take ? static_cast<uint32_t>(w.pkind[index]) : kNoClass;
 }
@@ -614,46 +617,94 @@ __device__ __forceinline__ float* parked(const W& w, uint32_t index) {
     return reinterpret_cast<float*>(w.raw + index);
 }
 
-// GJK of the pairs marked kPairGjk (convex_hit, convex.wgsl:584-618), one thread per pair of the
-// worklist. Separated pairs end here; a hit within the margin is parked for k_narrow_finish and
-// a penetrating pair's simplex for k_narrow_epa. (Sharing a hull pair's support scans among
-// four lanes measured slower: the simplex arithmetic, not the scan, sets the time.)
-__global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_blocks) {
-    constexpr int kLanes = 1;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0u) w.work_total[3] = w.work_total[0] - w.class_hist[kHullClasses * class_blocks];  // hull pairs
-    if (t >= w.work_total[0]) return;
-    const bool writer = true;
-    const uint32_t index = w.worklist[t];
-    if (writer) w.valid[index] = kPairNone;
+// GJK's loop state parked in the pair's raw slot between k_narrow_convex and
+// k_narrow_convex_resume: the simplex (w, a, b of four points), the closest result (v, lambdas,
+// the penetration flag, both closest points), the simplex size and the next iteration.
+constexpr uint32_t kParkedGjkFloats = 36u + 3u + 4u + 1u + 3u + 3u + 2u;
+static_assert(kParkedGjkFloats * 4u <= sizeof(dyn_contact), "GJK's state fits the pair's raw slot");
+
+__device__ inline void park_gjk(float* park, const GjkState& g) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const v3 p[3] = {g.simplex[i].w, g.simplex[i].a, g.simplex[i].b};
+#pragma unroll
+        for (uint32_t k = 0; k < 3; ++k) {
+            park[9 * i + 3 * k] = p[k].x;
+            park[9 * i + 3 * k + 1] = p[k].y;
+            park[9 * i + 3 * k + 2] = p[k].z;
+        }
+    }
+    const SimplexResult& c = g.closest;
+    const float tail[kParkedGjkFloats - 36u] = {c.v.x, c.v.y, c.v.z, c.lambdas[0], c.lambdas[1], c.lambdas[2],
+                                                 c.lambdas[3], c.penetrating ? 1.0f : 0.0f, c.point_a.x,
+                                                 c.point_a.y, c.point_a.z, c.point_b.x, c.point_b.y, c.point_b.z,
+                                                 __uint_as_float(g.count), __uint_as_float(g.iter)};
+#pragma unroll
+    for (uint32_t k = 0; k < kParkedGjkFloats - 36u; ++k) park[36u + k] = tail[k];
+}
+
+__device__ inline void unpark_gjk(const float* park, GjkState* g) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        g->simplex[i].w = mk(park[9 * i], park[9 * i + 1], park[9 * i + 2]);
+        g->simplex[i].a = mk(park[9 * i + 3], park[9 * i + 4], park[9 * i + 5]);
+        g->simplex[i].b = mk(park[9 * i + 6], park[9 * i + 7], park[9 * i + 8]);
+    }
+    const float* t = park + 36;
+    g->closest.v = mk(t[0], t[1], t[2]);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) g->closest.lambdas[k] = t[3 + k];
+    g->closest.penetrating = t[7] != 0.0f;
+    g->closest.point_a = mk(t[8], t[9], t[10]);
+    g->closest.point_b = mk(t[11], t[12], t[13]);
+    g->count = __float_as_uint(t[14]);
+    g->iter = __float_as_uint(t[15]);
+}
+
+// GJK iterations every convex pair runs in k_narrow_convex; pairs that need more go on in
+// k_narrow_convex_resume from their parked state, packed together, so the waves of the first
+// kernel do not wait on the few pairs that run to the 24-iteration cap.
+constexpr uint32_t kGjkFirstIterations = 8;
+
+struct ConvexPair {
+    uint32_t index;
+    Shape first, second;
+    float margin;
+};
+
+__device__ inline ConvexPair load_convex_pair(const W& w, uint32_t index) {
+    ConvexPair c;
+    c.index = index;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
     const dyn_collider fc = w.co[fs], sc = w.co[ss];
     const dyn_body_state fst = w.st[fb], sst = w.st[sb];
-    float margin = w.p.contact_margin;
+    c.margin = w.p.contact_margin;
     if ((w.de[fb].flags & DYN_BODY_CCD) || (w.de[sb].flags & DYN_BODY_CCD)) {
-        margin += len3(sub(ld3(sst.velocity), ld3(fst.velocity))) * w.p.dt;  // pair_margin
+        c.margin += len3(sub(ld3(sst.velocity), ld3(fst.velocity))) * w.p.dt;  // pair_margin
     }
-    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
-    const Shape wf = world_collider(fst, fc), wsd = world_collider(sst, sc);
+    c.first = world_collider(fst, fc);
+    c.second = world_collider(sst, sc);
+    return c;
+}
+
+// The rest of convex_hit once GJK has answered: a penetrating pair's simplex is parked for
+// k_narrow_epa, a hit within the margin for k_narrow_finish; separated pairs end here.
+__device__ inline void convex_pair_after_gjk(const W& w, const ConvexPair& c, const ShapeSet& shapes,
+                                             const Closest& closest, const GjkState& g) {
     Hit hit;
-    SimplexPoint simplex[4];
-#pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) simplex[i].w = simplex[i].a = simplex[i].b = zero3v();
-    uint32_t count = 0;
-    float* park = parked(w, index);
-    if (!convex_hit_gjk<kLanes>(wf, wsd, shapes, &hit, simplex, &count)) {
-        if (!writer) return;
+    float* park = parked(w, c.index);
+    if (!convex_hit_after_gjk<1>(c.first, c.second, shapes, closest, &hit, g.simplex, &g.count)) {
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
-            park[3 * i] = simplex[i].w.x;
-            park[3 * i + 1] = simplex[i].w.y;
-            park[3 * i + 2] = simplex[i].w.z;
+            park[3 * i] = g.simplex[i].w.x;
+            park[3 * i + 1] = g.simplex[i].w.y;
+            park[3 * i + 2] = g.simplex[i].w.z;
         }
-        w.valid[index] = kPairEpa;
+        w.valid[c.index] = kPairEpa;
         return;
     }
-    if (!(hit.distance <= margin) || !writer) return;
+    if (!(hit.distance <= c.margin)) return;
     park[0] = hit.distance;
     park[1] = hit.point.x;
     park[2] = hit.point.y;
@@ -661,7 +712,46 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_
     park[4] = hit.normal.x;
     park[5] = hit.normal.y;
     park[6] = hit.normal.z;
-    w.valid[index] = kPairHit;
+    w.valid[c.index] = kPairHit;
+}
+
+// GJK of the pairs marked kPairGjk (convex_hit, convex.wgsl:584-618), one thread per pair of the
+// worklist, for its first kGjkFirstIterations iterations: a pair GJK answers by then goes on to
+// the probe and is parked for EPA or the manifold kernel (or ends); a pair that needs more parks
+// its GJK state (kPairGjkResume) for k_narrow_convex_resume.
+__global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_blocks) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0u) w.work_total[3] = w.work_total[0] - w.class_hist[kHullClasses * class_blocks];  // hull pairs
+    if (t >= w.work_total[0]) return;
+    const uint32_t index = w.worklist[t];
+    w.valid[index] = kPairNone;
+    const ConvexPair c = load_convex_pair(w, index);
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
+    GjkState g;
+    gjk_begin<1>(c.first, c.second, shapes, &g);
+    Closest closest;
+    if (!gjk_run<1>(c.first, c.second, shapes, &g, kGjkFirstIterations, &closest)) {
+        park_gjk(parked(w, index), g);
+        w.valid[index] = kPairGjkResume;
+        return;
+    }
+    convex_pair_after_gjk(w, c, shapes, closest, g);
+}
+
+// The remaining GJK iterations of the pairs k_narrow_convex parked (kPairGjkResume), from their
+// parked state, then the rest of convex_hit as in k_narrow_convex.
+__global__ void __launch_bounds__(kThreads) k_narrow_convex_resume(W w) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w.work_total[4]) return;
+    const uint32_t index = w.worklist[t];
+    w.valid[index] = kPairNone;
+    const ConvexPair c = load_convex_pair(w, index);
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
+    GjkState g;
+    unpark_gjk(parked(w, index), &g);
+    Closest closest;
+    gjk_run<1>(c.first, c.second, shapes, &g, kGjkIterations, &closest);
+    convex_pair_after_gjk(w, c, shapes, closest, g);
 }
 
 constexpr uint32_t kEpaWaves = 4;  // waves per workgroup of k_narrow_epa
@@ -2323,6 +2413,8 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
     DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[0], nullptr, 1u,
                            kWorkKeyBytes, 0u, work_scratch, work_scratch_bytes));
     LAUNCH_ON(st, k_narrow_convex, pb, w, class_blocks);
+    DYN_TRY(worklist(w, st, 4u));
+    LAUNCH_ON(st, k_narrow_convex_resume, pb, w);
     DYN_TRY(worklist(w, st, 2u));
     const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
     const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
@@ -2657,7 +2749,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.row_start, body_count);
     A(w.contrib, body_count);
     A(w.scan_total, 4);
-    A(w.work_total, 4);
+    A(w.work_total, 8);
     A(world->snap_header, 4);
     A(world->words, 8);
     A(world->entry_keys, world->entry_cap);
@@ -2903,7 +2995,7 @@ int dyn_world_step(dyn_world* world, float dt) {
 int dyn_world_read_work(dyn_world* world, uint32_t* out) {
     if (!world || !out) return dyn::fail(DYN_E_INVALID, "dyn_world_read_work: null argument");
     DYN_HIP(hipStreamSynchronize(world->stream));
-    DYN_HIP(hipMemcpy(out, world->w.work_total, 4u * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    DYN_HIP(hipMemcpy(out, world->w.work_total, 5u * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return DYN_OK;
 }
 
@@ -3081,7 +3173,7 @@ NarrowLayout narrow_layout(uint32_t pair_capacity, void* base) {
     l.worklist = reinterpret_cast<uint32_t*>(take(4u * cap));
     l.class_hist = reinterpret_cast<uint32_t*>(take(4u * kClasses * static_cast<size_t>(ceil_div(
                                                                          static_cast<uint32_t>(cap), kClassBlock))));
-    l.work_total = reinterpret_cast<uint32_t*>(take(4u * 4u));
+    l.work_total = reinterpret_cast<uint32_t*>(take(4u * 8u));
     l.work_keys = reinterpret_cast<uint32_t*>(take(4u * cap));
     l.work_minor = reinterpret_cast<uint32_t*>(take(4u * cap));
     l.sort_bytes = dyn_sort_scratch_bytes(static_cast<uint32_t>(cap), kWorkKeyBytes);

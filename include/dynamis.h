@@ -440,7 +440,8 @@ int dyn_world_edit(dyn_world* world, const uint32_t* body_ids, const dyn_body_ed
  * while dyn_profile_enable is on or after dyn_world_set_graphs(world, 0). */
 int dyn_world_step(dyn_world* world, float dt);
 /* The last step's convex-path work: out[0] pairs given to GJK, out[1] pairs given manifolds
- * by the convex path, out[2] pairs given to EPA, out[3] the GJK pairs with a hull (4 words). No counterpart in the reference (whose
+ * by the convex path, out[2] pairs given to EPA, out[3] the GJK pairs with a hull, out[4] the GJK
+ * pairs that ran past their first 8 iterations (5 words). No counterpart in the reference (whose
  * counters do not split the narrowphase); for measurement. */
 int dyn_world_read_work(dyn_world* world, uint32_t* out);
 /* Turns the graph capture of dyn_world_step on (default) or off. No counterpart in the
