@@ -101,7 +101,8 @@ def broadphase_bytes(name: str, k: int, entries: int, pairs: int) -> float:
 
 
 def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: int, points: float,
-               rows: int, gjk: int = 0, epa: int = 0, convex_manifolds: int = 0, gjk_hull: int = 0) -> float:
+               rows: int, gjk: int = 0, epa: int = 0, convex_manifolds: int = 0, gjk_hull: int = 0,
+               gjk_resumed: int = 0, epa_wave: int = 0) -> float:
     """Minimal HBM payload of one launch of each step kernel, from the step's own counts:
     N bodies, K colliders, E entries, P pairs, C contacts of m points on average, R solver rows.
     A manifold is M(m) = 48 + 56 m bytes (normal, pair and material words plus per point anchor,
@@ -117,9 +118,13 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
     if name == "k_narrow":              # two bodies and colliders per pair, manifold out
         return pairs * 316.0 + contacts * manifold
     if name == "k_narrow_convex":       # the GJK worklist: pair record + index, parked results
-        return gjk * 320.0 + manifold_pairs * 68.0
-    if name == "k_narrow_epa":          # pair record, parked simplex in, result out
+        return gjk * 320.0 + manifold_pairs * 68.0 + gjk_resumed * 208.0  # and parked GJK states
+    if name == "k_narrow_convex_resume":  # pair record + index, GJK state in, results out
+        return gjk_resumed * (320.0 + 208.0 + 68.0)
+    if name == "k_narrow_epa_small":    # pair record, parked simplex in, result out
         return epa * (316.0 + 48.0 + 20.0)
+    if name == "k_narrow_epa":          # the polytopes given a wave: index, pair record, simplex, result
+        return epa_wave * (4.0 + 316.0 + 48.0 + 20.0)
     if name == "k_narrow_finish":       # pair record, parked result, contact out
         return manifold_pairs * (316.0 + 68.0 + 352.0)
     if name in ("k_integrate", "k_advance"):
@@ -182,12 +187,14 @@ def world_counts(world) -> dict:
     work = world.convex_work()
     return {"n": len(world.scene.states), "k": len(world.scene.colliders), "entries": m["ENTRIES"],
             "pairs": m["PAIRS"], "contacts": m["CONTACTS"], "points": points, "rows": m["SOLVER_ROWS"],
-            "gjk": work["gjk"], "epa": work["epa"], "manifold": work["manifold"], "gjk_hull": work["gjk_hull"]}
+            "gjk": work["gjk"], "epa": work["epa"], "manifold": work["manifold"], "gjk_hull": work["gjk_hull"],
+            "gjk_resumed": work["gjk_resumed"], "epa_wave": work["epa_wave"]}
 
 
 def kernel_bytes(name: str, c: dict) -> float:
     return step_bytes(name, c["n"], c["k"], c["entries"], c["pairs"], c["contacts"], c["points"], c["rows"],
-                      c.get("gjk", 0), c.get("epa", 0), c.get("manifold", 0), c.get("gjk_hull", 0))
+                      c.get("gjk", 0), c.get("epa", 0), c.get("manifold", 0), c.get("gjk_hull", 0),
+                      c.get("gjk_resumed", 0), c.get("epa_wave", 0))
 
 
 class Roctx:

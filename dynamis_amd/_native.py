@@ -55,6 +55,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "dyn_broadphase": (c_int, [c_void_p, c_void_p]),
     "dyn_narrowphase_scratch_bytes": (c_size_t, [c_u32]),
     "dyn_narrowphase": (c_int, [c_void_p, c_void_p]),
+    "dyn_set_epa_small_vertices": (c_int, [c_u32]),
     "dyn_islands_scratch_bytes": (c_size_t, [c_u32]),
     "dyn_islands": (c_int, [c_void_p, c_void_p]),
     "dyn_solve_scratch_bytes": (c_size_t, [c_u32, c_u32]),

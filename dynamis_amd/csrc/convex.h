@@ -78,22 +78,23 @@ __device__ inline v3 hull_support_thread(const Shape& w, const ShapeSet& ss, v3 
         const uint32_t count = rec.x & 0xFFu;
         if (count != kSupportNoCell) {
             if (count <= kSupportInline) {
-                const uint32_t idx[4] = {(rec.x >> 8) & 0xFFFu, rec.x >> 20, rec.y & 0xFFFu, (rec.y >> 12) & 0xFFFu};
-                v3 cand[4];
-#pragma unroll
-                for (uint32_t k = 0; k < kSupportInline; ++k) {
-                    if (k < count) cand[k] = hull_vertex(ss, src, idx[k]);
-                }
-#pragma unroll
-                for (uint32_t k = 0; k < kSupportInline; ++k) {
-                    if (k < count) {
-                        const float s = dot3(cand[k], d);
-                        if (s > best_dot) {
-                            best_dot = s;
-                            best = cand[k];
-                        }
+                // the unused slots load the first candidate again (not compared)
+                const uint32_t i0 = (rec.x >> 8) & 0xFFFu;
+                const v3 c0 = hull_vertex(ss, src, i0);
+                const v3 c1 = hull_vertex(ss, src, count > 1u ? rec.x >> 20 : i0);
+                const v3 c2 = hull_vertex(ss, src, count > 2u ? rec.y & 0xFFFu : i0);
+                const v3 c3 = hull_vertex(ss, src, count > 3u ? (rec.y >> 12) & 0xFFFu : i0);
+                const auto take = [&](v3 c) {
+                    const float s = dot3(c, d);
+                    if (s > best_dot) {
+                        best_dot = s;
+                        best = c;
                     }
-                }
+                };
+                take(c0);
+                if (count > 1u) take(c1);
+                if (count > 2u) take(c2);
+                if (count > 3u) take(c3);
                 return best;
             }
             const uint16_t* list = ss.cands + rec.y;
@@ -579,11 +580,12 @@ __device__ inline void epa_best_face(const EpaShared& e, uint32_t face_count, fl
     *best_face = index == 0xFFFFFFFFu ? 0u : index;
 }
 
+// `simplex_w` holds the four simplex points' x, y, z (12 floats).
 __device__ inline Epa epa_tetrahedron(const Shape& first, const Shape& second, const ShapeSet& ss,
-                                      const SimplexPoint* simplex, uint32_t count, EpaShared& e) {
+                                      const float* simplex_w, EpaShared& e) {
     const uint32_t lane = lane_id();
-    if (count != 4u) return epa_invalid();
-    if (lane < 4u) e.w[lane] = make_float4(simplex[lane].w.x, simplex[lane].w.y, simplex[lane].w.z, 0.0f);
+    wave_sync();
+    if (lane < 4u) e.w[lane] = make_float4(simplex_w[3u * lane], simplex_w[3u * lane + 1u], simplex_w[3u * lane + 2u], 0.0f);
     wave_sync();
     uint32_t vertex_count = 4;
     uint32_t face_count = 0;
@@ -666,29 +668,33 @@ __device__ inline Epa epa_tetrahedron(const Shape& first, const Shape& second, c
             new_count = min(new_count + static_cast<uint32_t>(__popcll(mask)), 64u);
         }
         wave_sync();
-        // drop the visible faces, keeping the order of the rest
-        EpaFace kept[2];
-        uint32_t kept_at[2];
-        bool keep[2] = {false, false};
+        // drop the visible faces, keeping the order of the rest: a round's kept faces move to
+        // slots below the next round's, so each round reads, then writes
         uint32_t write = 0;
         for (uint32_t k = 0; k < 2; ++k) {
             const uint32_t i = k * kWave + lane;
             if (k * kWave >= face_count) break;
             const bool mine = i < face_count && !((seen_masks[k] >> lane) & 1ull);
             const uint64_t mask = __ballot(mine);
+            uint4 kept_ids = make_uint4(0u, 0u, 0u, 0u);
+            float4 kept_normal = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (mine) {
-                kept[k] = e.faces[i];
-                kept_at[k] = write + __popcll(mask & ((1ull << lane) - 1ull));
-                keep[k] = true;
+                kept_ids = make_uint4(e.faces[i].a, e.faces[i].b, e.faces[i].c, e.faces[i].pad);
+                kept_normal = e.faces[i].normal;
+            }
+            wave_sync();
+            if (mine) {
+                EpaFace& to = e.faces[write + __popcll(mask & ((1ull << lane) - 1ull))];
+                to.a = kept_ids.x;
+                to.b = kept_ids.y;
+                to.c = kept_ids.z;
+                to.pad = kept_ids.w;
+                to.normal = kept_normal;
             }
             write += __popcll(mask);
-        }
-        wave_sync();
-        for (uint32_t k = 0; k < 2; ++k) {
-            if (keep[k]) e.faces[kept_at[k]] = kept[k];
+            wave_sync();
         }
         face_count = write;
-        wave_sync();
         if (face_count + new_count > 128u) break;
         for (uint32_t i = lane; i < new_count; i += kWave) e.faces[face_count + i] = e.fresh[i];
         face_count += new_count;
@@ -701,6 +707,234 @@ __device__ inline Epa epa_tetrahedron(const Shape& first, const Shape& second, c
     r.depth = best_dist;
     r.point = zero3v();  // the support points of the polytope are not kept (never read)
     return r;
+}
+
+// ---- EPA of small polytopes, kEpaGroup lanes per polytope -------------------------------------
+//
+// The same expansion as epa_tetrahedron, operation for operation and in the same order, for a
+// polytope that stays within kEpaSmallVertices vertices, kEpaSmallFaces faces and
+// kEpaSmallFresh new faces per expansion: an aligned group of kEpaGroup lanes shares one
+// polytope (kWave / kEpaGroup polytopes per wave), its loops over faces and horizon edges
+// striding by the group. A polytope that would outgrow the capacities is given up (the call
+// returns false) and is expanded from its start by epa_tetrahedron, so either way the result is
+// the sequential expansion's. The support points come from support<1> (the same vertex as the
+// wave's shared scan).
+
+constexpr uint32_t kEpaGroup = 8;
+constexpr uint32_t kEpaSmallVertices = 16;
+constexpr uint32_t kEpaSmallFaces = 24;
+constexpr uint32_t kEpaSmallFresh = 16;
+
+struct alignas(16) EpaSmallFace {
+    float nx, ny, nz;
+    uint32_t abc;  // vertex indices a | b << 8 | c << 16
+};
+
+static_assert(sizeof(EpaSmallFace) == 16, "a small face is one 16-byte LDS word");
+
+struct EpaSmallShared {
+    float4 w[kEpaSmallVertices];
+    EpaSmallFace faces[kEpaSmallFaces];
+    EpaSmallFace fresh[kEpaSmallFresh];
+    uint8_t visible[kEpaSmallFaces];
+};
+
+__device__ __forceinline__ EpaSmallFace epa_small_face(uint32_t a, uint32_t b, uint32_t c, v3 unit, v3 centroid) {
+    const v3 n = dot3(unit, centroid) > 0.0f ? neg(unit) : unit;
+    EpaSmallFace f;
+    f.nx = n.x;
+    f.ny = n.y;
+    f.nz = n.z;
+    f.abc = a | (b << 8) | (c << 16);
+    return f;
+}
+
+__device__ __forceinline__ v3 small_normal(const EpaSmallFace& f) { return mk(f.nx, f.ny, f.nz); }
+__device__ __forceinline__ uint32_t small_a(const EpaSmallFace& f) { return f.abc & 0xFFu; }
+__device__ __forceinline__ uint32_t small_b(const EpaSmallFace& f) { return (f.abc >> 8) & 0xFFu; }
+__device__ __forceinline__ uint32_t small_c(const EpaSmallFace& f) { return f.abc >> 16; }
+
+__device__ __forceinline__ v3 small_w(const EpaSmallShared& e, uint32_t i) {
+    const float4 p = e.w[i];
+    return mk(p.x, p.y, p.z);
+}
+
+// Lane value of the lane DPP control `kCtrl` names (quad permutes and the half-row mirror stay
+// inside an aligned group of 8 lanes).
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, 0xF, 0xF, false));
+}
+constexpr int kDppQuadSwap1 = 0xB1;     // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadSwap2 = 0x4E;     // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror: lane 7 - l of each 8 lanes
+
+__device__ __forceinline__ void group8_min_first(float* best, uint32_t* index) {
+    uint32_t b = __float_as_uint(*best), i = *index;
+    const auto step = [&](uint32_t ob, uint32_t oi) {
+        const float of = __uint_as_float(ob), bf = __uint_as_float(b);
+        if (of < bf || (of == bf && oi < i)) {
+            b = ob;
+            i = oi;
+        }
+    };
+    step(dpp_u32<kDppQuadSwap1>(b), dpp_u32<kDppQuadSwap1>(i));
+    step(dpp_u32<kDppQuadSwap2>(b), dpp_u32<kDppQuadSwap2>(i));
+    step(dpp_u32<kDppHalfMirror>(b), dpp_u32<kDppHalfMirror>(i));
+    *best = __uint_as_float(b);
+    *index = i;
+}
+
+// This group's 8 bits of a wave ballot.
+__device__ __forceinline__ uint32_t group_ballot(bool p, uint32_t group) {
+    return static_cast<uint32_t>(__ballot(p) >> (kEpaGroup * group)) & 0xFFu;
+}
+
+__device__ inline void epa_small_best(const EpaSmallShared& e, uint32_t face_count, uint32_t q, float* best_dist,
+                                      uint32_t* best_face) {
+    float best = kBig;
+    uint32_t index = 0xFFFFFFFFu;
+    for (uint32_t i = q; i < face_count; i += kEpaGroup) {
+        const EpaSmallFace f = e.faces[i];
+        const float dist = dot3(small_normal(f), small_w(e, small_a(f)));
+        if (dist < best && len3(small_normal(f)) > 0.5f) {
+            best = dist;
+            index = i;
+        }
+    }
+    group8_min_first(&best, &index);
+    *best_dist = best;
+    *best_face = index == 0xFFFFFFFFu ? 0u : index;
+}
+
+// Returns false when the polytope outgrows the capacities (nothing is returned then), the
+// vertices' being min(vertex_limit, kEpaSmallVertices); every lane of the group calls it with the
+// same arguments and receives the same result.
+// `simplex_w` holds the four simplex points' x, y, z (12 floats).
+__device__ inline bool epa_small(const Shape& first, const Shape& second, const ShapeSet& ss,
+                                 const float* simplex_w, uint32_t vertex_limit, EpaSmallShared& e, Epa* out) {
+    const uint32_t lane = lane_id();
+    const uint32_t q = lane % kEpaGroup, group = lane / kEpaGroup;
+    const uint32_t below = (1u << q) - 1u;
+    wave_sync();
+    if (q < 4u) e.w[q] = make_float4(simplex_w[3u * q], simplex_w[3u * q + 1u], simplex_w[3u * q + 2u], 0.0f);
+    wave_sync();
+    uint32_t vertex_count = 4;
+    uint32_t face_count = 0;
+    for (uint32_t i = 0; i < 4; ++i) {
+        uint32_t a, b, c;
+        tet_face(i, &a, &b, &c);
+        const v3 normal = cross3(sub(small_w(e, b), small_w(e, a)), sub(small_w(e, c), small_w(e, a)));
+        const float length_n = len3(normal);
+        if (length_n < 1e-10f) {
+            *out = epa_invalid();
+            return true;
+        }
+        const v3 unit = dv(normal, length_n);
+        const v3 centroid = mul(add(add(small_w(e, a), small_w(e, b)), small_w(e, c)), 1.0f / 3.0f);
+        if (q == 0u) e.faces[face_count] = epa_small_face(a, b, c, unit, centroid);
+        ++face_count;
+    }
+    wave_sync();
+    float best_dist = 0.0f;
+    uint32_t best_face = 0;
+    for (uint32_t iter = 0; iter < 32u; ++iter) {
+        epa_small_best(e, face_count, q, &best_dist, &best_face);
+        const v3 bn = small_normal(e.faces[best_face]);
+        const v3 probe = sub(support<1>(first, ss, bn), support<1>(second, ss, neg(bn)));
+        const float gain = dot3(probe, bn) - best_dist;
+        if (gain < 1e-4f) break;
+        if (vertex_count >= 64u) break;
+        if (vertex_count >= vertex_limit || vertex_count >= kEpaSmallVertices) return false;
+        if (q == 0u) e.w[vertex_count] = make_float4(probe.x, probe.y, probe.z, 0.0f);
+        ++vertex_count;
+        wave_sync();
+        // faces that see the new point, in face order
+        uint32_t visible_count = 0;
+        uint32_t seen = 0;
+        for (uint32_t base = 0; base < face_count; base += kEpaGroup) {
+            const uint32_t i = base + q;
+            bool vis = false;
+            if (i < face_count) {
+                const EpaSmallFace f = e.faces[i];
+                vis = dot3(small_normal(f), sub(probe, small_w(e, small_a(f)))) > 1e-6f;
+            }
+            const uint32_t mask = group_ballot(vis, group);
+            seen |= mask << base;
+            if (vis) e.visible[visible_count + __popc(mask & below)] = static_cast<uint8_t>(i);
+            visible_count += __popc(mask);
+        }
+        wave_sync();
+        // horizon edges of the visible faces, in (face, edge) order
+        const uint32_t apex = vertex_count - 1u;
+        uint32_t new_count = 0;
+        const uint32_t items = visible_count * 3u;
+        for (uint32_t base = 0; base < items; base += kEpaGroup) {
+            const uint32_t t = base + q;
+            bool ok = false;
+            EpaSmallFace made;
+            if (t < items) {
+                const uint32_t i = t / 3u, edge = t % 3u;
+                const EpaSmallFace face = e.faces[e.visible[i]];
+                const uint32_t fa = small_a(face), fb = small_b(face), fc = small_c(face);
+                const uint32_t start = edge == 0u ? fa : (edge == 1u ? fb : fc);
+                const uint32_t end = edge == 0u ? fb : (edge == 1u ? fc : fa);
+                bool shared = false;
+                for (uint32_t j = 0; j < visible_count; ++j) {
+                    if (j == i) continue;
+                    const EpaSmallFace o = e.faces[e.visible[j]];
+                    const uint32_t oa = small_a(o), ob = small_b(o), oc = small_c(o);
+                    if ((oa == end && ob == start) || (ob == end && oc == start) || (oc == end && oa == start)) {
+                        shared = true;
+                    }
+                }
+                if (!shared) {
+                    const v3 normal = cross3(sub(small_w(e, end), small_w(e, apex)), sub(small_w(e, start), small_w(e, apex)));
+                    const float length_n = len3(normal);
+                    if (length_n > 1e-10f) {
+                        const v3 unit = dv(normal, length_n);
+                        const v3 centroid = mul(add(add(small_w(e, apex), small_w(e, start)), small_w(e, end)), 1.0f / 3.0f);
+                        made = epa_small_face(apex, start, end, unit, centroid);
+                        ok = true;
+                    }
+                }
+            }
+            const uint32_t mask = group_ballot(ok, group);
+            const uint32_t rank = new_count + __popc(mask & below);
+            if (ok && rank < kEpaSmallFresh) e.fresh[rank] = made;
+            new_count += __popc(mask);
+        }
+        if (new_count > kEpaSmallFresh) return false;
+        wave_sync();
+        // drop the visible faces, keeping the order of the rest: a round's kept faces move to
+        // slots below the next round's, so each round reads, then writes
+        uint32_t write = 0;
+        for (uint32_t base = 0; base < face_count; base += kEpaGroup) {
+            const uint32_t i = base + q;
+            const bool mine = i < face_count && !((seen >> i) & 1u);
+            const uint32_t mask = group_ballot(mine, group);
+            float4 kept = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (mine) kept = reinterpret_cast<const float4*>(e.faces)[i];
+            wave_sync();
+            if (mine) reinterpret_cast<float4*>(e.faces)[write + __popc(mask & below)] = kept;
+            write += __popc(mask);
+            wave_sync();
+        }
+        face_count = write;
+        if (face_count + new_count > 128u) break;
+        if (face_count + new_count > kEpaSmallFaces) return false;
+        for (uint32_t i = q; i < new_count; i += kEpaGroup) e.faces[face_count + i] = e.fresh[i];
+        face_count += new_count;
+        wave_sync();
+    }
+    epa_small_best(e, face_count, q, &best_dist, &best_face);
+    Epa r;
+    r.valid = best_dist < kBig;
+    r.normal = small_normal(e.faces[best_face]);
+    r.depth = best_dist;
+    r.point = zero3v();  // the support points of the polytope are not kept (never read)
+    *out = r;
+    return true;
 }
 
 // ---- penetration probe and convex_hit (convex.wgsl:523-618) ----------------------------------

@@ -83,7 +83,7 @@ struct W {
     uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
     uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
     uint32_t* work_total;  // device words: entries on the worklist of modes 0-2, the hull-class GJK pairs,
-                           // the resumed GJK pairs (mode 4)
+                           // the resumed GJK pairs (mode 4), the EPA polytopes given a wave
     uint32_t* work_keys;   // GJK worklist sort keys: class, then the hull sources
     uint32_t* work_minor;  // the sort's unused minor lane
     uint32_t* dest;
@@ -721,7 +721,10 @@ __device__ inline void convex_pair_after_gjk(const W& w, const ConvexPair& c, co
 // its GJK state (kPairGjkResume) for k_narrow_convex_resume.
 __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_blocks) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0u) w.work_total[3] = w.work_total[0] - w.class_hist[kHullClasses * class_blocks];  // hull pairs
+    if (t == 0u) {
+        w.work_total[3] = w.work_total[0] - w.class_hist[kHullClasses * class_blocks];  // hull pairs
+        w.work_total[5] = 0u;  // EPA polytopes k_narrow_epa_small gives up, counted there
+    }
     if (t >= w.work_total[0]) return;
     const uint32_t index = w.worklist[t];
     w.valid[index] = kPairNone;
@@ -754,33 +757,67 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex_resume(W w) {
     convex_pair_after_gjk(w, c, shapes, closest, g);
 }
 
-constexpr uint32_t kEpaWaves = 4;  // waves per workgroup of k_narrow_epa
+constexpr uint32_t kEpaWaves = 4;  // waves per workgroup of k_narrow_epa and k_narrow_epa_small
 
-// EPA of the pairs marked kPairEpa, one wave per pair, the waves striding over the worklist.
-// Lane 0 parks the result.
-__global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
-    __shared__ EpaShared epa[kEpaWaves];
-    const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
+// The vertex capacity k_narrow_epa_small runs with (dyn_set_epa_small_vertices).
+uint32_t g_epa_small_vertices = kEpaSmallVertices;
+
+__device__ __forceinline__ void park_epa(float* park, const Epa& r) {
+    park[12] = r.valid ? 1.0f : 0.0f;
+    park[13] = r.normal.x;
+    park[14] = r.normal.y;
+    park[15] = r.normal.z;
+    park[16] = r.depth;
+}
+
+// EPA of the pairs marked kPairEpa, kEpaGroup lanes per pair, the groups striding over the
+// worklist. A pair whose polytope outgrows the small capacities keeps its mark and is listed for
+// k_narrow_epa (work_total[5] entries of work_minor, whose sort lane is free once the GJK worklist
+// is sorted); the others are parked (lane 0 of the group) and marked kPairEpaDone.
+__global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa_small(W w, uint32_t vertex_limit) {
+    constexpr uint32_t kGroupsPerBlock = kEpaWaves * kWave / kEpaGroup;
+    __shared__ EpaSmallShared epa[kGroupsPerBlock];
+    const uint32_t slot = threadIdx.x / kEpaGroup;
+    const uint32_t q = threadIdx.x % kEpaGroup;
     const uint32_t total = w.work_total[2];
-    const uint32_t waves = gridDim.x * kEpaWaves;
+    const uint32_t groups = gridDim.x * kGroupsPerBlock;
     const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
-    for (uint32_t t = blockIdx.x * kEpaWaves + wave; t < total; t += waves) {
+    for (uint32_t t = blockIdx.x * kGroupsPerBlock + slot; t < total; t += groups) {
         const uint32_t index = w.worklist[t];
         const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
         const Shape wf = world_collider(w.st[w.owner[fs]], w.co[fs]);
         const Shape wsd = world_collider(w.st[w.owner[ss]], w.co[ss]);
         float* park = parked(w, index);
-        SimplexPoint simplex[4];
-#pragma unroll
-        for (uint32_t i = 0; i < 4; ++i) simplex[i].w = mk(park[3 * i], park[3 * i + 1], park[3 * i + 2]);
-        const Epa r = epa_tetrahedron(wf, wsd, shapes, simplex, 4u, epa[wave]);
+        Epa r;
+        if (!epa_small(wf, wsd, shapes, park, vertex_limit, epa[slot], &r)) {
+            if (q == 0u) w.work_minor[atomicAdd(&w.work_total[5], 1u)] = index;
+            continue;
+        }
+        if (q == 0u) {
+            park_epa(park, r);
+            w.valid[index] = kPairEpaDone;
+        }
+    }
+}
+
+// EPA of the pairs k_narrow_epa_small gave up (listed in work_minor), one wave per pair, the
+// waves striding over that list. Lane 0 parks the result.
+__global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
+    __shared__ EpaShared epa[kEpaWaves];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t total = w.work_total[5];
+    const uint32_t waves = gridDim.x * kEpaWaves;
+    const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
+    for (uint32_t t = blockIdx.x * kEpaWaves + wave; t < total; t += waves) {
+        const uint32_t index = w.work_minor[t];
+        const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+        const Shape wf = world_collider(w.st[w.owner[fs]], w.co[fs]);
+        const Shape wsd = world_collider(w.st[w.owner[ss]], w.co[ss]);
+        float* park = parked(w, index);
+        const Epa r = epa_tetrahedron(wf, wsd, shapes, park, epa[wave]);
         if (lane == 0u) {
-            park[12] = r.valid ? 1.0f : 0.0f;
-            park[13] = r.normal.x;
-            park[14] = r.normal.y;
-            park[15] = r.normal.z;
-            park[16] = r.depth;
+            park_epa(park, r);
             w.valid[index] = kPairEpaDone;
         }
     }
@@ -2416,8 +2453,16 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
     DYN_TRY(worklist(w, st, 4u));
     LAUNCH_ON(st, k_narrow_convex_resume, pb, w);
     DYN_TRY(worklist(w, st, 2u));
+    {
+        constexpr uint32_t kPairsPerBlock = kEpaWaves * kWave / kEpaGroup;
+        const uint32_t small_groups = min(ceil_div(w.pair_cap ? w.pair_cap : 1u, kPairsPerBlock), 2048u);
+        ProfileScope scope("k_narrow_epa_small", st);
+        k_narrow_epa_small<<<small_groups, kEpaWaves * kWave, 0, st>>>(w, g_epa_small_vertices);
+    }
+    DYN_LAUNCHED("k_narrow_epa_small");
+    // the polytopes given a wave are few (the long tail of the expansion count)
     const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
-    const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 2048u);
+    const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 256u);
     {
         ProfileScope scope("k_narrow_epa", st);
         k_narrow_epa<<<groups, kEpaWaves * kWave, 0, st>>>(w);
@@ -2995,7 +3040,7 @@ int dyn_world_step(dyn_world* world, float dt) {
 int dyn_world_read_work(dyn_world* world, uint32_t* out) {
     if (!world || !out) return dyn::fail(DYN_E_INVALID, "dyn_world_read_work: null argument");
     DYN_HIP(hipStreamSynchronize(world->stream));
-    DYN_HIP(hipMemcpy(out, world->w.work_total, 5u * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    DYN_HIP(hipMemcpy(out, world->w.work_total, 6u * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return DYN_OK;
 }
 
@@ -3186,6 +3231,14 @@ NarrowLayout narrow_layout(uint32_t pair_capacity, void* base) {
 
 size_t dyn_narrowphase_scratch_bytes(uint32_t pair_capacity) {
     return narrow_layout(pair_capacity, nullptr).bytes;
+}
+
+int dyn_set_epa_small_vertices(uint32_t vertices) {
+    if (vertices < 4u || vertices > dyn::kEpaSmallVertices) {
+        return dyn::fail(DYN_E_INVALID, "dyn_set_epa_small_vertices: outside 4..16");
+    }
+    dyn::g_epa_small_vertices = vertices;
+    return DYN_OK;
 }
 
 int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io) {

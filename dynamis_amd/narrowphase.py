@@ -58,6 +58,13 @@ def support_cells(sources: np.ndarray, vertices: np.ndarray) -> tuple[np.ndarray
     return cells, cands
 
 
+def set_epa_small_vertices(vertices: int) -> None:
+    """Test hook: the vertex capacity (4-16, default 16) of the EPA kernel that gives eight lanes
+    to a polytope; larger polytopes are expanded again by the wave-per-polytope kernel. Results
+    do not depend on it."""
+    call("dyn_set_epa_small_vertices", int(vertices))
+
+
 class Narrowphase:
     """Device copies of a scene's records; `run(pairs)` enqueues the pass over the (P, 2)
     collider pairs and `read()` returns (contacts_raw, contact_valid) for them."""

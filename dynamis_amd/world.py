@@ -410,12 +410,13 @@ class World(BodyCommands):
 
     def convex_work(self) -> dict[str, int]:
         """Pairs the last step gave to GJK (those with a hull, those GJK resumed after its first
-        iterations), to EPA and to the convex manifold kernel."""
+        iterations), to EPA (those whose polytope went to the wave kernel) and to the convex
+        manifold kernel."""
         self._ensure()
-        out = (ctypes.c_uint32 * 5)()
+        out = (ctypes.c_uint32 * 6)()
         call("dyn_world_read_work", self._handle, out)
         return {"gjk": int(out[0]), "manifold": int(out[1]), "epa": int(out[2]), "gjk_hull": int(out[3]),
-                "gjk_resumed": int(out[4])}
+                "gjk_resumed": int(out[4]), "epa_wave": int(out[5])}
 
     def _queue_edits(self, ids: np.ndarray, edits: np.ndarray) -> None:
         self._ensure()

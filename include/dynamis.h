@@ -208,6 +208,12 @@ size_t dyn_narrowphase_scratch_bytes(uint32_t pair_capacity);
  * make no contact. The same kernels run inside dyn_world_step.
  */
 int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io);
+/* Test hook, no counterpart in the reference: the vertex capacity (4 to 16, default 16) of the
+ * EPA kernel that gives eight lanes to a polytope; a polytope that would grow past it is expanded
+ * again by the kernel that gives it a wave. Results do not depend on it; lowering it routes more
+ * polytopes through the second kernel. Process-wide, read when the narrowphase is launched (a
+ * world's captured step keeps the value it was captured with). */
+int dyn_set_epa_small_vertices(uint32_t vertices);
 
 /* ---- islands (crates/dynamis-rigid/src/islands.rs:237-258, crates/dynamis-rigid/shaders/
  *      island_*.wgsl) ----------------------------------------------------------------------- */
@@ -441,8 +447,9 @@ int dyn_world_edit(dyn_world* world, const uint32_t* body_ids, const dyn_body_ed
 int dyn_world_step(dyn_world* world, float dt);
 /* The last step's convex-path work: out[0] pairs given to GJK, out[1] pairs given manifolds
  * by the convex path, out[2] pairs given to EPA, out[3] the GJK pairs with a hull, out[4] the GJK
- * pairs that ran past their first 8 iterations (5 words). No counterpart in the reference (whose
- * counters do not split the narrowphase); for measurement. */
+ * pairs that ran past their first 8 iterations, out[5] the EPA polytopes too large for the
+ * eight-lane kernel (6 words). No counterpart in the reference (whose counters do not split the
+ * narrowphase); for measurement. */
 int dyn_world_read_work(dyn_world* world, uint32_t* out);
 /* Turns the graph capture of dyn_world_step on (default) or off. No counterpart in the
  * reference, whose steps are recorded command buffers (crates/dynamis-gpu/src/recorder.rs:29). */

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from dynamis_amd.narrowphase import Narrowphase
+from dynamis_amd.narrowphase import Narrowphase, set_epa_small_vertices
 from dynamis_amd.world import World
 from tests.narrow_cases import DT, convex_path_contacts, pile_scene, with_states
 
@@ -105,3 +105,25 @@ def test_support_cells_change_no_bit(device):
         assert valid_t.tobytes() == valid_l.tobytes()
         assert raw_t[valid_t == 1].tobytes() == raw_l[valid_l == 1].tobytes()
         assert (valid_t == 1).sum() > 500
+
+
+def test_epa_fallback_changes_no_bit(device, oracle):
+    """EPA's two kernels (eight lanes per polytope up to 16 vertices, else a wave per polytope
+    from the start): with the eight-lane kernel's capacity at 4 vertices every polytope that
+    expands goes to the wave kernel, and the pass still equals the oracle's, byte for byte, on a
+    3000-body mixed pile and on a pile of hulls alone."""
+    for scene, steps in ((pile_scene(3000, seed=5, hull_sources=64, spacing=1.5), 60), (hull_pile(1500), 60)):
+        world = World()
+        world.create(scene)
+        for _ in range(steps):
+            world.step(DT)
+        states = world.states()
+        pairs = world.pairs()
+        world.close()
+        staged = with_states(scene, states)
+        try:
+            set_epa_small_vertices(4)
+            assert_stage_parity(oracle, staged, pairs, 500)
+        finally:
+            set_epa_small_vertices(16)
+        assert_stage_parity(oracle, staged, pairs, 500)
