@@ -399,15 +399,45 @@ struct PairCounter {
     __device__ __forceinline__ void operator()(uint32_t, uint32_t) { ++n; }
 };
 
-// Counts and keeps the first `limit` pairs, which the emitting pass then copies instead of
-// walking again.
-struct StagedCounter {
+// The level-links passes give each collider an aligned group of kLinkLanes lanes, which test
+// kLinkLanes consecutive run entries at a time; a group's matches are numbered in entry order
+// (ballot ranks), so its pairs come out in the order of a one-thread walk. Group sinks are
+// called by every lane of the group together, each with its own entry's outcome.
+constexpr uint32_t kLinkLanes = 8;
+constexpr uint32_t kLinkColliders = kBpThreads / kLinkLanes;  // colliders per block of the link passes
+
+__device__ __forceinline__ uint32_t link_group_bits(bool p) {
+    const uint32_t lane = lane_id();
+    return static_cast<uint32_t>(__ballot(p) >> (lane & ~(kLinkLanes - 1u))) & ((1u << kLinkLanes) - 1u);
+}
+
+// Counts the group's pairs and keeps the first kLinkStage, which the emitting pass then copies
+// instead of walking again.
+struct GroupStage {
     uint32_t n = 0;
     uint2* stage;
-    uint32_t limit;
-    __device__ __forceinline__ void operator()(uint32_t a, uint32_t b) {
-        if (n < limit) stage[n] = make_uint2(a, b);
-        ++n;
+    __device__ __forceinline__ void put(bool ok, uint32_t a, uint32_t b) {
+        const uint32_t bits = link_group_bits(ok);
+        const uint32_t at = n + __popc(bits & ((1u << (lane_id() % kLinkLanes)) - 1u));
+        if (ok && at < kLinkStage) stage[at] = make_uint2(a, b);
+        n += __popc(bits);
+    }
+};
+
+// Writes the group's pairs from `at` on, in order, up to the capacity.
+struct GroupWriter {
+    uint32_t at;
+    uint32_t capacity;
+    uint32_t* major;
+    uint32_t* minor;
+    __device__ __forceinline__ void put(bool ok, uint32_t a, uint32_t b) {
+        const uint32_t bits = link_group_bits(ok);
+        const uint32_t to = at + __popc(bits & ((1u << (lane_id() % kLinkLanes)) - 1u));
+        if (ok && to < capacity) {
+            major[to] = a;
+            minor[to] = b;
+        }
+        at += __popc(bits);
     }
 };
 
@@ -425,18 +455,27 @@ struct PairWriter {
     }
 };
 
-// pair_emit.wgsl:10-31 (node identity is excluded by the callers).
+// pair_emit.wgsl:10-31 (node identity is excluded by the callers): true with the canonical
+// pair in *lo, *hi when the two entries make a candidate pair.
+__device__ __forceinline__ bool pair_filter(uint32_t group_a, uint32_t info_a, uint32_t group_b, uint32_t info_b,
+                                            const uint32_t* __restrict__ worlds, uint32_t* lo, uint32_t* hi) {
+    if (((info_a & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return false;
+    if (((info_b & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return false;
+    if (group_a == group_b) return false;
+    if (!(info_a & DYN_ENTRY_MOBILE) && !(info_b & DYN_ENTRY_MOBILE)) return false;
+    const uint32_t a = info_a & DYN_ENTRY_INDEX_MASK;
+    const uint32_t b = info_b & DYN_ENTRY_INDEX_MASK;
+    if (worlds && worlds[a] != worlds[b]) return false;  // a batch's worlds never meet
+    *lo = min(a, b);
+    *hi = max(a, b);
+    return true;
+}
+
 template <typename Sink>
 __device__ __forceinline__ void emit_pair(Sink& sink, uint32_t group_a, uint32_t info_a, uint32_t group_b,
                                           uint32_t info_b, const uint32_t* __restrict__ worlds) {
-    if (((info_a & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
-    if (((info_b & DYN_ENTRY_KIND_MASK) >> DYN_ENTRY_KIND_SHIFT) != DYN_ENTRY_KIND_COLLIDER) return;
-    if (group_a == group_b) return;
-    if (!(info_a & DYN_ENTRY_MOBILE) && !(info_b & DYN_ENTRY_MOBILE)) return;
-    const uint32_t a = info_a & DYN_ENTRY_INDEX_MASK;
-    const uint32_t b = info_b & DYN_ENTRY_INDEX_MASK;
-    if (worlds && worlds[a] != worlds[b]) return;  // a batch's worlds never meet
-    sink(min(a, b), max(a, b));
+    uint32_t a, b;
+    if (pair_filter(group_a, info_a, group_b, info_b, worlds, &a, &b)) sink(a, b);
 }
 
 struct EntryView {
@@ -555,12 +594,13 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
     cell_mates(index, len, keys, rec, grid, worlds, writer);
 }
 
-// Coarser-level links (level_links.wgsl:43-65) of one collider. Every collider's primary entry
-// (cell offset 0) is its first slot, so the probe starts from the collider's own box, flags and
-// level instead of from the sorted array; a primary beyond capacity probes nothing, as in the
-// reference. Only strictly coarser levels are probed, which hold none of the collider's own
-// entries, so the reference's skip of the probing entry itself never applies. Entry records are
-// read in sorted order (bp_gather_kernel), so a run is read contiguously.
+// Coarser-level links (level_links.wgsl:43-65) of one collider, walked by its lane group. Every
+// collider's primary entry (cell offset 0) is its first slot, so the probe starts from the
+// collider's own box, flags and level instead of from the sorted array; a primary beyond capacity
+// probes nothing, as in the reference. Only strictly coarser levels are probed, which hold none
+// of the collider's own entries, so the reference's skip of the probing entry itself never
+// applies. Entry records are read in sorted order (bp_gather_kernel), so a run is read
+// contiguously, the group's lanes reading neighbouring records.
 template <typename Sink>
 __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_t slot,
                                             const dyn_aabb* __restrict__ aabbs,
@@ -571,6 +611,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                                             const uint32_t* __restrict__ dir, bool coarse_active,
                                             uint32_t occupied, float grid, uint32_t capacity,
                                             const uint32_t* __restrict__ worlds, Sink& sink) {
+    const uint32_t q = lane_id() % kLinkLanes;
     const uint32_t emitted = packed & 0xFFu;
     if (emitted == 0u || slot >= capacity) return;
     const uint32_t f = flags ? flags[i] : (DYN_BP_MOBILE | DYN_BP_AWAKE);
@@ -599,17 +640,24 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             // A sparse level is read whole: an entry is in the run of one of this box's cells c
             // exactly when its own cell is c and its key is c's key in this world, which
             // mate_in_cell and the key test below check, so the same pairs come out.
-            for (uint32_t e = level_first; e < level_end; ++e) {
-                const EntryView o = rec.at(e);
-                if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
-                int32_t oc[3];
-                entry_cell(o.mn, o.info, cs, oc);
-                if (oc[0] < cells.mn[0] || oc[0] > cells.mx[0] || oc[1] < cells.mn[1] || oc[1] > cells.mx[1] ||
-                    oc[2] < cells.mn[2] || oc[2] > cells.mx[2]) {
-                    continue;
+            for (uint32_t base = level_first; base < level_end; base += kLinkLanes) {
+                const uint32_t e = base + q;
+                bool ok = false;
+                uint32_t a = 0u, b = 0u;
+                if (e < level_end) {
+                    const EntryView o = rec.at(e);
+                    if (awake || (o.info & DYN_ENTRY_AWAKE)) {
+                        int32_t oc[3];
+                        entry_cell(o.mn, o.info, cs, oc);
+                        if (oc[0] >= cells.mn[0] && oc[0] <= cells.mx[0] && oc[1] >= cells.mn[1] &&
+                            oc[1] <= cells.mx[1] && oc[2] >= cells.mn[2] && oc[2] <= cells.mx[2] &&
+                            keys[e] == (cell_key(link, oc[0], oc[1], oc[2]) ^ salt) &&
+                            mate_in_cell(self, o, cs, oc)) {
+                            ok = pair_filter(self.group, self.info, o.group, o.info, worlds, &a, &b);
+                        }
+                    }
                 }
-                if (keys[e] != (cell_key(link, oc[0], oc[1], oc[2]) ^ salt)) continue;
-                if (mate_in_cell(self, o, cs, oc)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
+                sink.put(ok, a, b);
             }
             continue;
         }
@@ -619,17 +667,40 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             cell_at(cells, ordinal, cell);
             uint32_t first, end;
             key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]) ^ salt, &first, &end);
-            for (uint32_t e = first; e < end; ++e) {
-                const EntryView o = rec.at(e);
-                if (!awake && !(o.info & DYN_ENTRY_AWAKE)) continue;
-                if (mate_in_cell(self, o, cs, cell)) emit_pair(sink, self.group, self.info, o.group, o.info, worlds);
+            for (uint32_t base = first; base < end; base += kLinkLanes) {
+                const uint32_t e = base + q;
+                bool ok = false;
+                uint32_t a = 0u, b = 0u;
+                if (e < end) {
+                    const EntryView o = rec.at(e);
+                    if ((awake || (o.info & DYN_ENTRY_AWAKE)) && mate_in_cell(self, o, cs, cell)) {
+                        ok = pair_filter(self.group, self.info, o.group, o.info, worlds, &a, &b);
+                    }
+                }
+                sink.put(ok, a, b);
             }
         }
     }
 }
 
-// Level links, counting pass: one thread per collider; blocks of 256 colliders are the count
-// kernel's blocks, so block_offsets gives each collider's first entry slot.
+// The first entry slot of each collider of this block (blocks of kLinkColliders colliders): its
+// count block's offset plus the entries of the colliders before it in that block of kBpThreads.
+__device__ __forceinline__ uint32_t link_slot(const uint32_t* __restrict__ cnt_level, uint32_t n,
+                                              const uint32_t* __restrict__ block_offsets, uint32_t* s_scan,
+                                              uint32_t* s_slot) {
+    const uint32_t c0 = blockIdx.x * kLinkColliders;
+    const uint32_t seg = c0 / kBpThreads;
+    const uint32_t j = seg * kBpThreads + threadIdx.x;
+    const uint32_t before = block_exclusive_sum<kBpThreads>(j < n ? (cnt_level[j] & 0xFFu) : 0u, s_scan, nullptr);
+    const uint32_t local = c0 - seg * kBpThreads;
+    if (threadIdx.x >= local && threadIdx.x < local + kLinkColliders) {
+        s_slot[threadIdx.x - local] = block_offsets[seg] + before;
+    }
+    __syncthreads();
+    return s_slot[threadIdx.x / kLinkLanes];
+}
+
+// Level links, counting pass: kLinkLanes lanes per collider.
 __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
@@ -638,22 +709,23 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
     const uint32_t* __restrict__ worlds, uint32_t* link_counts, uint32_t* link_blocks, uint2* link_stage) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
+    __shared__ uint32_t s_slot[kLinkColliders];
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
     const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
-    const uint32_t i = blockIdx.x * kBpThreads + threadIdx.x;
-    const uint32_t packed = i < n ? cnt_level[i] : 0u;
-    const uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
+    const uint32_t slot = link_slot(cnt_level, n, block_offsets, s_scan, s_slot);
+    const uint32_t i = blockIdx.x * kLinkColliders + threadIdx.x / kLinkLanes;
+    const bool leader = threadIdx.x % kLinkLanes == 0u;
     rec.gathered = gather_pays(dir, occupied, rec.level_keys);
-    StagedCounter counter{0u, link_stage + static_cast<size_t>(i) * kLinkStage, kLinkStage};
+    GroupStage counter{0u, link_stage + static_cast<size_t>(i) * kLinkStage};
     if (i < n) {
-        level_links(i, packed, slot, aabbs, groups, flags, keys, rec, dir, coarse_active,
+        level_links(i, i < n ? cnt_level[i] : 0u, slot, aabbs, groups, flags, keys, rec, dir, coarse_active,
                     occupied, grid, capacity, worlds, counter);
-        link_counts[i] = counter.n;
+        if (leader) link_counts[i] = counter.n;
     }
     uint32_t total = 0;
-    block_exclusive_sum<kBpThreads>(counter.n, s_scan, &total);
+    block_exclusive_sum<kBpThreads>(leader && i < n ? counter.n : 0u, s_scan, &total);
     if (threadIdx.x == 0) link_blocks[blockIdx.x] = total;
 }
 
@@ -668,19 +740,24 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     const uint32_t* __restrict__ link_offsets, const uint2* __restrict__ link_stage,
     uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
+    __shared__ uint32_t s_slot[kLinkColliders];
+    __shared__ uint32_t s_at[kLinkColliders];
     const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
     const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
     const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
                                       counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
-    const uint32_t i = blockIdx.x * kBpThreads + threadIdx.x;
-    const uint32_t packed = i < n ? cnt_level[i] : 0u;
-    const uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(packed & 0xFFu, s_scan, nullptr);
+    const uint32_t slot = link_slot(cnt_level, n, block_offsets, s_scan, s_slot);
+    const uint32_t g = threadIdx.x / kLinkLanes, q = threadIdx.x % kLinkLanes;
+    const uint32_t i = blockIdx.x * kLinkColliders + g;
     const uint32_t mine = i < n ? link_counts[i] : 0u;
-    const uint32_t at = link_offsets[blockIdx.x] + block_exclusive_sum<kBpThreads>(mine, s_scan, nullptr);
+    const uint32_t excl = block_exclusive_sum<kBpThreads>(q == 0u ? mine : 0u, s_scan, nullptr);
+    if (q == 0u) s_at[g] = link_offsets[blockIdx.x] + excl;
+    __syncthreads();
+    const uint32_t at = s_at[g];
     if (mine == 0u || at >= pair_capacity) return;
     if (mine <= kLinkStage) {  // the counting pass kept them all
         const uint2* kept = link_stage + static_cast<size_t>(i) * kLinkStage;
-        for (uint32_t j = 0; j < mine && at + j < pair_capacity; ++j) {
+        for (uint32_t j = q; j < mine && at + j < pair_capacity; j += kLinkLanes) {
             const uint2 p = kept[j];
             pair_major[at + j] = p.x;
             pair_minor[at + j] = p.y;
@@ -688,8 +765,8 @@ __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
         return;
     }
     rec.gathered = gather_pays(dir, occupied, rec.level_keys);
-    PairWriter writer{at, pair_capacity, pair_major, pair_minor};
-    level_links(i, packed, slot, aabbs, groups, flags, keys, rec, dir, coarse_active, occupied,
+    GroupWriter writer{at, pair_capacity, pair_major, pair_minor};
+    level_links(i, cnt_level[i], slot, aabbs, groups, flags, keys, rec, dir, coarse_active, occupied,
                 grid, capacity, worlds, writer);
 }
 
@@ -756,7 +833,7 @@ BpLayout bp_layout(uint32_t colliders, uint32_t entry_capacity, uint32_t pair_ca
     l.res_partials = carve.take<uint32_t>(2 * kResolutionBlocks);
     const uint32_t entry_blocks = ceil_div(entry_capacity ? entry_capacity : 1u, kBpThreads);
     l.link_counts = carve.take<uint32_t>(colliders ? colliders : 1u);
-    l.link_blocks = carve.take<uint32_t>(blocks);
+    l.link_blocks = carve.take<uint32_t>(ceil_div(colliders ? colliders : 1u, kLinkColliders));
     l.mate_counts = carve.take<uint32_t>(entry_capacity ? entry_capacity : 1u);
     l.mate_blocks = carve.take<uint32_t>(entry_blocks);
     l.dir = carve.take<uint32_t>(static_cast<size_t>(kGridLevels) * kDirStride);
@@ -878,9 +955,10 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     }
     DYN_LAUNCHED("bp_gather_kernel");
     const EntryRecords rec{l.sorted, io->d_entries, io->d_entry_order, l.level_keys, false};
+    const uint32_t link_blocks = ceil_div(n, kLinkColliders);
     {
         ProfileScope scope("bp_links_count_kernel", st);
-        bp_links_count_kernel<<<blocks, kBpThreads, 0, st>>>(
+        bp_links_count_kernel<<<link_blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
             io->d_entry_keys, rec, l.dir, counters,
             io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage);
@@ -895,14 +973,14 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     DYN_LAUNCHED("bp_mates_count_kernel");
     {
         ProfileScope scope("bp_pair_offsets_kernel", st);
-        bp_pair_offsets_kernel<<<1, 1024, 0, st>>>(l.link_blocks, blocks, l.mate_blocks, entry_blocks,
+        bp_pair_offsets_kernel<<<1, 1024, 0, st>>>(l.link_blocks, link_blocks, l.mate_blocks, entry_blocks,
                                                    counters, io->pair_capacity);
     }
     DYN_LAUNCHED("bp_pair_offsets_kernel");
     if (io->pair_capacity) {
         {
             ProfileScope scope("bp_links_emit_kernel", st);
-            bp_links_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
+            bp_links_emit_kernel<<<link_blocks, kBpThreads, 0, st>>>(
                 io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
                 io->d_entry_keys, rec, l.dir, counters,
                 io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage,

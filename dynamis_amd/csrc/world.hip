@@ -81,7 +81,8 @@ struct W {
     uint32_t* valid;
     uint8_t* pkind;        // per convex pair: the unordered class of its two shape kinds
     uint32_t* worklist;    // convex pairs grouped by kind class (k_class_scatter)
-    uint32_t* class_hist;  // per kind class and block of pairs: count, then offset
+    uint32_t* class_hist;  // per kind class and block of pairs: count, then offset in the class;
+                           // then the classes' totals
     uint32_t* work_total;  // device words: entries on the worklist of modes 0-2, the hull-class GJK pairs,
                            // the resumed GJK pairs (mode 4), the EPA polytopes given a wave
     uint32_t* work_keys;   // GJK worklist sort keys: class, then the hull sources
@@ -523,12 +524,39 @@ __global__ void __launch_bounds__(kThreads) k_class_count(W w, uint32_t mode) {
     if (threadIdx.x < kClasses) w.class_hist[threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// After k_scan_blocks turned the class-major counts into offsets: each block places its pairs
-// of every class at that class's offset (in any order within the block: every pair's result
-// goes to its own slot).
+// One workgroup per kind class: the class's block counts become offsets within the class, and
+// the class's total goes to class_hist[kClasses * blocks + class].
+__global__ void __launch_bounds__(1024) k_class_scan(uint32_t* class_hist, uint32_t blocks) {
+    __shared__ uint32_t s_scan[1024 / kWave];
+    __shared__ uint32_t s_carry;
+    uint32_t* row = class_hist + static_cast<size_t>(blockIdx.x) * blocks;
+    if (threadIdx.x == 0) s_carry = 0;
+    for (uint32_t first = 0; first < blocks; first += 1024) {
+        const uint32_t b = first + threadIdx.x;
+        const uint32_t v = b < blocks ? row[b] : 0u;
+        uint32_t total = 0;
+        const uint32_t excl = block_exclusive_sum<1024>(v, s_scan, &total);
+        if (b < blocks) row[b] = s_carry + excl;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) class_hist[static_cast<size_t>(kClasses) * blocks + blockIdx.x] = s_carry;
+}
+
+// After k_class_scan: each block places its pairs of every class at the class's start (the
+// totals of the classes before it) plus its offset within the class (in any order within the
+// block: every pair's result goes to its own slot); block 0 writes the worklist's length.
 __global__ void __launch_bounds__(kThreads) k_class_scatter(W w, uint32_t mode) {
+    static_assert(kClasses <= kWave, "one wave sums the class totals");
     __shared__ uint32_t cursor[kClasses];
-    if (threadIdx.x < kClasses) cursor[threadIdx.x] = w.class_hist[threadIdx.x * gridDim.x + blockIdx.x];
+    if (threadIdx.x < kWave) {
+        const uint32_t c = threadIdx.x;
+        const uint32_t total = c < kClasses ? w.class_hist[static_cast<size_t>(kClasses) * gridDim.x + c] : 0u;
+        const uint32_t inclusive = wave_inclusive_sum(total);
+        if (c < kClasses) cursor[c] = inclusive - total + w.class_hist[c * gridDim.x + blockIdx.x];
+        if (blockIdx.x == 0u && c == kWave - 1u) w.work_total[mode] = inclusive;
+    }
     __syncthreads();
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
     const uint32_t base = blockIdx.x * kClassBlock;
@@ -722,7 +750,9 @@ __device__ inline void convex_pair_after_gjk(const W& w, const ConvexPair& c, co
 __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_blocks) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0u) {
-        w.work_total[3] = w.work_total[0] - w.class_hist[kHullClasses * class_blocks];  // hull pairs
+        uint32_t hull = 0;  // GJK pairs of the hull classes
+        for (uint32_t c = kHullClasses; c < kClasses; ++c) hull += w.class_hist[kClasses * class_blocks + c];
+        w.work_total[3] = hull;
         w.work_total[5] = 0u;  // EPA polytopes k_narrow_epa_small gives up, counted there
     }
     if (t >= w.work_total[0]) return;
@@ -2263,7 +2293,7 @@ int alloc_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_cap
     A(w.valid, pair_capacity);
     A(w.pkind, pair_capacity);
     A(w.worklist, pair_capacity);
-    A(w.class_hist, kClasses * static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)));
+    A(w.class_hist, kClasses * (static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)) + 1u));
     A(w.work_keys, pair_capacity);
     A(w.work_minor, pair_capacity);
     A(w.dest, pair_capacity);
@@ -2402,7 +2432,7 @@ int scan(dyn_world* world, const uint32_t* flags, const uint32_t* count, uint32_
 }
 
 // The worklist of convex pairs of one mode, grouped by kind class: block histograms, their
-// class-major scan (total in work_total[mode]), the scatter.
+// per-class scans, the scatter (total in work_total[mode]).
 int worklist(const dyn::W& w, hipStream_t st, uint32_t mode) {
     using namespace dyn;
     const uint32_t blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
@@ -2413,7 +2443,7 @@ int worklist(const dyn::W& w, hipStream_t st, uint32_t mode) {
     DYN_LAUNCHED("k_class_count");
     {
         ProfileScope scope("k_class_scan", st);
-        k_scan_blocks<<<1, 1024, 0, st>>>(w.class_hist, kClasses * blocks, w.work_total + mode);
+        k_class_scan<<<kClasses, 1024, 0, st>>>(w.class_hist, blocks);
     }
     DYN_LAUNCHED("k_class_scan");
     {
@@ -3216,8 +3246,8 @@ NarrowLayout narrow_layout(uint32_t pair_capacity, void* base) {
     const size_t cap = pair_capacity ? pair_capacity : 1u;
     l.pkind = reinterpret_cast<uint8_t*>(take(cap));
     l.worklist = reinterpret_cast<uint32_t*>(take(4u * cap));
-    l.class_hist = reinterpret_cast<uint32_t*>(take(4u * kClasses * static_cast<size_t>(ceil_div(
-                                                                         static_cast<uint32_t>(cap), kClassBlock))));
+    l.class_hist = reinterpret_cast<uint32_t*>(take(4u * kClasses * (static_cast<size_t>(ceil_div(
+                                                                          static_cast<uint32_t>(cap), kClassBlock)) + 1u)));
     l.work_total = reinterpret_cast<uint32_t*>(take(4u * 8u));
     l.work_keys = reinterpret_cast<uint32_t*>(take(4u * cap));
     l.work_minor = reinterpret_cast<uint32_t*>(take(4u * cap));
