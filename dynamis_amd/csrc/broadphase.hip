@@ -369,26 +369,6 @@ __global__ void __launch_bounds__(256) bp_directory_kernel(const uint32_t* __res
     dir[g] = lo;
 }
 
-// Half-open run [first, end) of `key` in the sorted keys, via the level directory.
-__device__ __forceinline__ void key_run(const uint32_t* __restrict__ keys,
-                                        const uint32_t* __restrict__ dir, uint32_t key,
-                                        uint32_t* first, uint32_t* end) {
-    const uint32_t level = key >> DYN_LEVEL_KEY_SHIFT;
-    const uint32_t bucket = (key & DYN_CELL_HASH_MASK) >> (DYN_LEVEL_KEY_SHIFT - kDirBits);
-    const uint32_t* d = dir + level * kDirStride + bucket;
-    uint32_t lo = d[0];
-    const uint32_t bucket_end = d[1];
-    uint32_t hi = bucket_end;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    *first = lo;
-    uint32_t e = lo;
-    while (e < bucket_end && keys[e] == key) ++e;
-    *end = e;
-}
-
 // Candidate pairs are written in two passes (pair_emit.wgsl:10-31 appends each pair with a device
 // atomic on one counter, which serializes): a counting pass records how many pairs every
 // collider's level links and every sorted entry's cell mates produce, one scan turns the counts
@@ -409,6 +389,26 @@ constexpr uint32_t kLinkColliders = kBpThreads / kLinkLanes;  // colliders per b
 __device__ __forceinline__ uint32_t link_group_bits(bool p) {
     const uint32_t lane = lane_id();
     return static_cast<uint32_t>(__ballot(p) >> (lane & ~(kLinkLanes - 1u))) & ((1u << kLinkLanes) - 1u);
+}
+
+// Lower bound of `key` in the sorted keys[lo, hi) by the lane group: rounds of kLinkLanes probes
+// at evenly spaced positions narrow the range about (kLinkLanes + 1)-fold, then the group reads
+// the last few keys at once.
+__device__ __forceinline__ uint32_t group_lower_bound(const uint32_t* __restrict__ keys, uint32_t lo,
+                                                      uint32_t hi, uint32_t key) {
+    const uint32_t q = lane_id() % kLinkLanes;
+    const auto at = [&](uint32_t span, uint32_t lane) {
+        return lo + static_cast<uint32_t>(static_cast<uint64_t>(span) * (lane + 1u) / (kLinkLanes + 1u));
+    };
+    while (hi - lo > kLinkLanes) {  // probe positions strictly increase across the lanes
+        const uint32_t span = hi - lo;
+        const uint32_t below = __popc(link_group_bits(keys[at(span, q)] < key));
+        const uint32_t next_lo = below == 0u ? lo : at(span, below - 1u) + 1u;
+        hi = below == kLinkLanes ? hi : at(span, below);
+        lo = next_lo;
+    }
+    const uint32_t e = lo + q;
+    return lo + __popc(link_group_bits(e < hi && keys[e] < key));
 }
 
 // Counts the group's pairs and keeps the first kLinkStage, which the emitting pass then copies
@@ -665,19 +665,27 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
         for (uint32_t ordinal = 0; ordinal < count; ++ordinal) {
             int32_t cell[3];
             cell_at(cells, ordinal, cell);
-            uint32_t first, end;
-            key_run(keys, dir, cell_key(link, cell[0], cell[1], cell[2]) ^ salt, &first, &end);
-            for (uint32_t base = first; base < end; base += kLinkLanes) {
+            // the cell's run: from the lower bound of its key in the key's directory bucket to the
+            // first other key, found chunk by chunk as the run is read
+            const uint32_t key = cell_key(link, cell[0], cell[1], cell[2]) ^ salt;
+            const uint32_t level_of_key = key >> DYN_LEVEL_KEY_SHIFT;
+            const uint32_t bucket = (key & DYN_CELL_HASH_MASK) >> (DYN_LEVEL_KEY_SHIFT - kDirBits);
+            const uint32_t* d = dir + level_of_key * kDirStride + bucket;
+            const uint32_t bucket_end = d[1];
+            for (uint32_t base = group_lower_bound(keys, d[0], bucket_end, key); base < bucket_end;
+                 base += kLinkLanes) {
                 const uint32_t e = base + q;
+                const bool in_run = e < bucket_end && keys[e] == key;
                 bool ok = false;
                 uint32_t a = 0u, b = 0u;
-                if (e < end) {
+                if (in_run) {
                     const EntryView o = rec.at(e);
                     if ((awake || (o.info & DYN_ENTRY_AWAKE)) && mate_in_cell(self, o, cs, cell)) {
                         ok = pair_filter(self.group, self.info, o.group, o.info, worlds, &a, &b);
                     }
                 }
                 sink.put(ok, a, b);
+                if (link_group_bits(in_run) != (1u << kLinkLanes) - 1u) break;  // the run ended here
             }
         }
     }
