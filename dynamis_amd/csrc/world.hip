@@ -1223,18 +1223,17 @@ __global__ void __launch_bounds__(kThreads) k_island_init(W w) {
     w.istate[i] = 0u;
 }
 
-// The root of a node, halving the path on the way (each visited node is pointed at its
-// grandparent). Parents only ever move to an ancestor and a root only to a lower root, so the
-// components, their minimum-row roots and the flattened parents after the pointer-jumping
-// rounds are those of the plain walk.
+// The root of a node by a read-only walk. Halving the path on the way (pointing each visited
+// node at its grandparent) was measured: on the settled 100k pile its stores cost more than the
+// hops they save, and the flatten rounds compress the trees after the hooks anyway. Parents only
+// ever move to an ancestor and a root only to a lower root, so the components and their
+// minimum-row roots are the same either way.
 __device__ __forceinline__ uint32_t island_find(const W& w, uint32_t node) {
     uint32_t cur = node;
     uint32_t parent = atomic_load_agent(&w.parents[cur]);
     while (parent != cur) {
-        const uint32_t grand = atomic_load_agent(&w.parents[parent]);
-        if (grand != parent) atomic_store_agent(&w.parents[cur], grand);
         cur = parent;
-        parent = grand;
+        parent = atomic_load_agent(&w.parents[cur]);
     }
     return cur;
 }
@@ -1250,6 +1249,8 @@ __device__ inline void island_couple(const W& w, uint32_t a, uint32_t b) {
     if (b >= w.d && atomic_load_agent(&w.wake[b]) != 0u) atomicOr(&w.wake[a], 1u);
     if (a >= w.d && atomic_load_agent(&w.wake[a]) != 0u) atomicOr(&w.wake[b], 1u);
     if (a >= w.d || b >= w.d || a == b) return;
+    // rows with one parent are joined already (most pairs on a seeded pile)
+    if (atomic_load_agent(&w.parents[a]) == atomic_load_agent(&w.parents[b])) return;
     uint32_t ra = island_find(w, a), rb = island_find(w, b);
     for (uint32_t attempt = 0; attempt < 256u; ++attempt) {
         if (ra == rb) return;
@@ -1303,8 +1304,8 @@ __global__ void __launch_bounds__(kThreads) k_solve_stage_check(W w, uint32_t* w
 // row's parent at the lower row (an atomic minimum, so each row ends at its lowest linked
 // neighbour below it, or itself). Parents still point only at lower rows of the same
 // component, so k_island_link's hooks and the flatten rounds end at the same minimum-row roots;
-// most rows then start one short hop from their root, which spares the hooks their long finds
-// on a pile that is one island.
+// most rows then start a few hops from their root, which spares the hooks their long finds on a
+// pile that is one island.
 __global__ void __launch_bounds__(kThreads) k_island_seed(W w, float slop) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= min(*w.ncontacts, w.contact_cap)) return;
@@ -1343,6 +1344,22 @@ __global__ void __launch_bounds__(kThreads) k_island_flatten(W w) {
     const uint32_t parent = atomic_load_agent(&w.parents[i]);
     const uint32_t gp = atomic_load_agent(&w.parents[parent]);
     atomic_store_agent(&w.parents[i], min(parent, gp));
+}
+
+// All the flatten rounds of a small world in one workgroup, a barrier after each round (within a
+// round, as across the launched rounds, a row may read a parent another row has already moved
+// up, which only shortens its path).
+constexpr uint32_t kFlattenGroupRows = 8192;
+
+__global__ void __launch_bounds__(1024) k_island_flatten_rounds(W w, uint32_t rounds) {
+    for (uint32_t r = 0; r < rounds; ++r) {
+        for (uint32_t i = threadIdx.x; i < w.d; i += 1024u) {
+            const uint32_t parent = atomic_load_agent(&w.parents[i]);
+            const uint32_t gp = atomic_load_agent(&w.parents[parent]);
+            atomic_store_agent(&w.parents[i], min(parent, gp));
+        }
+        __syncthreads();
+    }
 }
 
 // island_aggregate.wgsl + island_count.wgsl
@@ -1527,8 +1544,7 @@ __device__ __forceinline__ void commit_block(const W& w, uint32_t ci, bool fa_in
 }
 
 // solver_contact_block.wgsl:195-233
-__global__ void __launch_bounds__(kThreads) k_warm(W w) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void warm_at(const W& w, uint32_t ci) {
     if (ci >= *w.ncontacts) return;
     const dyn_contact& contact = w.contacts[ci];
     if (!block_resolves(contact)) return;
@@ -1552,6 +1568,10 @@ __global__ void __launch_bounds__(kThreads) k_warm(W w) {
                  sub(ld3(first.s.angular_velocity), ld3(fl.s.angular_velocity)),
                  sub(ld3(second.s.velocity), ld3(sl.s.velocity)),
                  sub(ld3(second.s.angular_velocity), ld3(sl.s.angular_velocity)), false);
+}
+
+__global__ void __launch_bounds__(kThreads) k_warm(W w) {
+    warm_at(w, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __device__ __forceinline__ float normal_delta(float relaxation, float damping_ratio, float mass, float truth,
@@ -1609,8 +1629,7 @@ __device__ __forceinline__ SolverBody load_solver_body(const W& w, uint32_t row)
 }
 
 // solver_block_solve.wgsl:24-54 for every claimed row: the frozen or unsplit body of a block.
-__global__ void __launch_bounds__(kThreads) k_solver_bodies(W w) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void solver_bodies_at(const W& w, uint32_t row) {
     if (row >= w.n || w.block_counts[row] == 0u) return;
     Body b = load_body(w, row);
     if (body_is_inert(b)) body_frozen(b);
@@ -1624,10 +1643,13 @@ __global__ void __launch_bounds__(kThreads) k_solver_bodies(W w) {
     p[4] = make_float4(b.s.angular_velocity[0], b.s.angular_velocity[1], b.s.angular_velocity[2], 0.0f);
 }
 
+__global__ void __launch_bounds__(kThreads) k_solver_bodies(W w) {
+    solver_bodies_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // solver_contact_block.wgsl:31-110, the parts of a block that depend on poses and masses only:
 // each point's arms from the centres of mass, its row masses, target speed and depth.
-__global__ void __launch_bounds__(kThreads) k_solve_prep(W w, uint32_t load_impulses) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t load_impulses) {
     if (ci >= *w.ncontacts) return;
     float4* row = w.srows + static_cast<size_t>(kSolveRow) * ci;
     const dyn_contact& contact = w.contacts[ci];
@@ -1706,11 +1728,14 @@ __global__ void __launch_bounds__(kThreads) k_solve_prep(W w, uint32_t load_impu
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_solve_prep(W w, uint32_t load_impulses) {
+    solve_prep_at(w, blockIdx.x * blockDim.x + threadIdx.x, load_impulses);
+}
+
 // solver_contact_block.wgsl:31-193 over the block's row: the points' impulses are applied as
 // each point is solved (every point reads only the velocities the block started from, so this
 // adds the same impulses in the same order as applying them all after the loop).
-__global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void solve_at(const W& w, uint32_t ci, uint32_t residual) {
     if (ci >= *w.ncontacts) return;
     const float4* row = w.srows + static_cast<size_t>(kSolveRow) * ci;
     const float4 h0 = row[0];
@@ -1810,9 +1835,12 @@ __global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
                  residual != 0u);
 }
 
+__global__ void __launch_bounds__(kThreads) k_solve(W w, uint32_t residual) {
+    solve_at(w, blockIdx.x * blockDim.x + threadIdx.x, residual);
+}
+
 // The iterations' impulses back into the contact records, for the archive and the next step.
-__global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void impulses_store_at(const W& w, uint32_t ci) {
     if (ci >= *w.ncontacts) return;
     dyn_contact& contact = w.contacts[ci];
     const uint32_t point_count = contact.point_count;
@@ -1831,14 +1859,17 @@ __global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
+    impulses_store_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // solver_block_apply.wgsl over every claimed row
 // List entries whose loads k_apply and k_position_apply keep in flight together: a row of up to
 // kApplyBatch blocks reads its entries (and, in k_apply, then their blocks' words) in one round
 // (integer sums: any order gives the same words).
 constexpr uint32_t kApplyBatch = 8;
 
-__global__ void __launch_bounds__(kThreads) k_apply(W w) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void apply_at(const W& w, uint32_t row) {
     if (row >= w.n) return;
     // list_counts is 0 for rows without blocks and for inert rows, which receive no words
     const uint32_t count = w.list_counts[row];
@@ -1888,9 +1919,12 @@ __global__ void __launch_bounds__(kThreads) k_apply(W w) {
     p[4] = make_float4(av.x, av.y, av.z, 0.0f);
 }
 
+__global__ void __launch_bounds__(kThreads) k_apply(W w) {
+    apply_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // substep_integrate.wgsl:15-76 over the awake dynamic rows (no fields)
-__global__ void __launch_bounds__(kThreads) k_integrate(W w) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void integrate_at(const W& w, uint32_t row) {
     if (row >= w.d) return;
     const dyn_body_desc d = w.de[row];
     dyn_body_state s = w.st[row];
@@ -1928,9 +1962,12 @@ __global__ void __launch_bounds__(kThreads) k_integrate(W w) {
     w.st[row] = s;
 }
 
+__global__ void __launch_bounds__(kThreads) k_integrate(W w) {
+    integrate_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // substep_advance.wgsl
-__global__ void __launch_bounds__(kThreads) k_advance(W w) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void advance_at(const W& w, uint32_t row) {
     if (row >= w.d) return;
     dyn_body_state s = w.st[row];
     if (!body_is_active(s, w.de[row])) return;
@@ -1942,6 +1979,10 @@ __global__ void __launch_bounds__(kThreads) k_advance(W w) {
     const float k = 0.5f * h;
     st4(s.orientation, qnorm(make_float4(q.x + k * dq.x, q.y + k * dq.y, q.z + k * dq.z, q.w + k * dq.w)));
     w.st[row] = s;
+}
+
+__global__ void __launch_bounds__(kThreads) k_advance(W w) {
+    advance_at(w, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // position_correction.wgsl:5-46, position_block.wgsl, from the block's position row
@@ -1984,8 +2025,7 @@ __device__ __forceinline__ bool position_correction(const W& w, float4 h0, const
     return true;
 }
 
-__global__ void __launch_bounds__(kThreads) k_position_block(W w) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void position_block_at(const W& w, uint32_t ci) {
     if (ci >= *w.ncontacts) return;
     const float4* prow = w.prows + 3u * ci;
     const float4 h0 = prow[0];
@@ -2004,9 +2044,12 @@ __global__ void __launch_bounds__(kThreads) k_position_block(W w) {
     if (entries.y != kNoSlot) w.pwords[entries.y] = wb;
 }
 
+__global__ void __launch_bounds__(kThreads) k_position_block(W w) {
+    position_block_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // position_apply.wgsl over every claimed row
-__global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void position_apply_at(const W& w, uint32_t row) {
     if (row >= w.n) return;
     // list_counts is 0 for rows without blocks and for inert rows; contrib, zero outside a
     // position pass, is marked only for inert rows with a contributing block
@@ -2050,6 +2093,10 @@ __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
     st4(s.orientation, qnorm(qmul(make_float4(aa.x * 0.5f, aa.y * 0.5f, aa.z * 0.5f, 1.0f), q0)));
     const v3 nr = add(mk(r0.x, r0.y, r0.z), al);
     r = make_float4(nr.x, nr.y, nr.z, 0.0f);
+}
+
+__global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
+    position_apply_at(w, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // island_sleep.wgsl
@@ -2508,6 +2555,31 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
 
 int world_step(dyn_world* world, float dt);
 
+// Pointer-jumping rounds that flatten any forest of d rows (each round at least halves every
+// depth): ilog2(max(d, 2)) + 1, as crates/dynamis-rigid/src/lib.rs:79-81 sizes the island pass.
+uint32_t flatten_rounds(uint32_t d) {
+    uint32_t rounds = 0;
+    for (uint32_t v = d < 2u ? 2u : d; v > 1u; v >>= 1) ++rounds;
+    return rounds + 1u;
+}
+
+// The flatten rounds: launched one by one, or all in one workgroup for a small world.
+int island_flatten(const dyn::W& w, hipStream_t st) {
+    using namespace dyn;
+    const uint32_t rounds = flatten_rounds(w.d);
+    if (w.d <= kFlattenGroupRows) {
+        {
+            ProfileScope scope("k_island_flatten_rounds", st);
+            k_island_flatten_rounds<<<1, 1024, 0, st>>>(w, rounds);
+        }
+        DYN_LAUNCHED("k_island_flatten_rounds");
+        return DYN_OK;
+    }
+    const uint32_t db = blocks_for(w.d);
+    for (uint32_t r = 0; r < rounds; ++r) LAUNCH_ON(st, k_island_flatten, db, w);
+    return DYN_OK;
+}
+
 // SolverReset (solver.rs:89-101) and the block set-up of SolveSubsteps (solver.rs:310): block
 // counts, the contact blocks, the rows' block lists.
 int solver_prepare(const dyn::W& w, hipStream_t st) {
@@ -2689,10 +2761,7 @@ int world_step(dyn_world* world, float dt) {
     if (!w.worlds) LAUNCH(k_island_seed, cb, w, w.p.slop);
     LAUNCH(k_island_link, cb, w, w.p.slop);
     LAUNCH(k_island_link_resting, cb, w);
-    uint32_t rounds = 0;
-    for (uint32_t v = w.d < 2u ? 2u : w.d; v > 1u; v >>= 1) ++rounds;
-    rounds += 1;  // ilog2(max(dynamic, 2)) + 1, crates/dynamis-rigid/src/lib.rs:79-81
-    for (uint32_t r = 0; r < rounds; ++r) LAUNCH(k_island_flatten, db, w);
+    DYN_TRY(island_flatten(w, st));
     LAUNCH(k_island_aggregate, db, w);
     LAUNCH(k_island_wake, db, w);
     DYN_TRY(solver_prepare(w, st));
@@ -3446,10 +3515,7 @@ int dyn_islands(void* stream, const dyn_islands_io* io) {
         LAUNCH_ON(st, k_island_seed, cb, w, io->slop);
         LAUNCH_ON(st, k_island_link, cb, w, io->slop);
     }
-    uint32_t rounds = 0;
-    for (uint32_t v = w.d < 2u ? 2u : w.d; v > 1u; v >>= 1) ++rounds;
-    rounds += 1;  // ilog2(max(dynamic, 2)) + 1, crates/dynamis-rigid/src/lib.rs:79-81
-    for (uint32_t r = 0; r < rounds; ++r) LAUNCH_ON(st, k_island_flatten, db, w);
+    DYN_TRY(island_flatten(w, st));
     LAUNCH_ON(st, k_island_aggregate, db, w);
     return DYN_OK;
 }
