@@ -669,11 +669,13 @@ def run_copy(count: int) -> int:
 
 # ---- CPU baseline ------------------------------------------------------------------------------
 
-def cpu_world(scene, states, sample_steps: int):
+def cpu_world(scene, states, sample_steps: int, parallel: bool = False):
+    """The oracle's rate over `sample_steps` steps from the given states: the serial build, or
+    with `parallel` its OpenMP build (the same results) on every thread OpenMP is given."""
     import copy
 
     from tests.oracle_binding import load
-    oracle = load()
+    oracle = load(parallel)
     start = copy.copy(scene)
     start.states = states.copy()
     world = oracle.world(start)
@@ -682,7 +684,7 @@ def cpu_world(scene, states, sample_steps: int):
         world.step(DT)
     secs = time.perf_counter() - t0
     world.close()
-    return len(scene.states) * sample_steps / secs, secs
+    return len(scene.states) * sample_steps / secs, secs, oracle.threads()
 
 
 def cpu_broadphase(passes: int):
@@ -844,11 +846,16 @@ def world_report(wl: str, r: dict, steps: int, warmup: int, cpu_sample: int | No
     sample = (2 if wl == "pile" else 100) if cpu_sample is None else cpu_sample
     cpu = None
     if sample > 0:
-        progress(f"CPU baseline: {sample} oracle steps")
-        v, secs = cpu_world(r["scene"], r["final_states"], sample)
-        cpu = {"value": v, "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{sample} steps of the same scene through the single-threaded C oracle, started "
-                         f"from the device's body states at the end of the timed steps ({secs:.1f} s)"}
+        progress(f"CPU baseline: {sample} oracle steps on every core, then on one")
+        v, secs, threads = cpu_world(r["scene"], r["final_states"], sample, parallel=True)
+        one, one_secs, _ = cpu_world(r["scene"], r["final_states"], sample)
+        cpu = {"value": v, "unit": "body-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{sample} steps of the same scene through the C oracle's OpenMP build ({threads} "
+                         f"threads over the per-pair narrowphase and the per-contact and per-row solver "
+                         f"loops; it steps bit-identically to the serial build), started from the device's "
+                         f"body states at the end of the timed steps ({secs:.1f} s)",
+               "single_core": {"value": one, "cores": 1,
+                               "sample": f"the same {sample} steps through the serial oracle ({one_secs:.1f} s)"}}
     run = r["settle"] + warmup + steps
     config = {"workload": spec["describe"], "bodies": bodies, "steps_in_run": run,
               "timed_window": f"steps {run - steps + 1}-{run} of the {run}-step run: {r['settle']} settling "

@@ -54,6 +54,9 @@ void oracle_world_counters(const oracle_world* w, uint32_t* out);
 int oracle_world_solve(oracle_world* w, dyn_contact* contacts, uint32_t count, float dt);
 /* The narrowphase alone over the given pairs and the current records (contacts_raw,
  * contact_valid); -1 when count exceeds the pair capacity or a pair names no collider. */
+/* Threads the step's parallel loops use: 1 in the serial build, OpenMP's maximum in the
+ * OpenMP build (liboracle_omp.so). */
+uint32_t oracle_threads(void);
 int oracle_world_narrowphase(oracle_world* w, const uint32_t* major, const uint32_t* minor, uint32_t count,
                              dyn_contact* raw, uint32_t* valid);
 uint32_t oracle_world_pairs(const oracle_world* w, uint32_t* major, uint32_t* minor,

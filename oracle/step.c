@@ -8,8 +8,14 @@
  * box-box SAT with clipping and plane-convex manifolds (narrowphase.wgsl); warm start from the
  * previous step's archive (contact_relay.wgsl) and from the resting archive (contact_begin.wgsl);
  * union-find islands, wake and sleep; the Jacobi mass-splitting contact solver with fixed-point
- * delta accumulation; position correction. Pairs that need GJK/EPA (hulls, cylinders, scaled
- * shapes) produce no contact here (counted in the returned narrowphase tally).
+ * delta accumulation; position correction. Hulls and cylinders take the convex path (GJK, EPA,
+ * the direction probe and the clipped feature manifolds of convex.wgsl and narrowphase.wgsl).
+ *
+ * Built with OpenMP (liboracle_omp.so, the all-core CPU baseline) the per-pair narrowphase and
+ * the per-contact and per-row solver loops run in parallel: a pair writes only its own slot, a
+ * solver block adds integer delta words (atomic adds; integer sums do not depend on order) and
+ * raises the residual maxima and contribution counts atomically, and a row's passes touch only
+ * that row, so the OpenMP build steps bit-identically to the serial one.
  *
  * Every collider sits in one grid region per step, its awake flag taken from the owner's activity
  * and wake flag (collider_entries.wgsl:9-35, immovable_entries.wgsl:8-29). The reference keeps
@@ -19,6 +25,9 @@
  */
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "omath.h"
 #include "oracle.h"
@@ -2018,14 +2027,41 @@ static v3 manifold_arm(const body_t* b, const dyn_manifold_point* p, int first) 
 
 static int block_resolves(const dyn_contact* c) { return c->point_count > 0u && c->sensor == 0u; }
 
+uint32_t oracle_threads(void) {
+#ifdef _OPENMP
+    return (uint32_t)omp_get_max_threads();
+#else
+    return 1u;
+#endif
+}
+
+/* Integer additions and maxima shared by parallel solver blocks (plain in the serial build). */
+static void add_word(uint32_t* p, uint32_t v) {
+#ifdef _OPENMP
+    __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#else
+    *p += v;
+#endif
+}
+
+static void max_word(uint32_t* p, uint32_t v) {
+#ifdef _OPENMP
+    uint32_t seen = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (v > seen && !__atomic_compare_exchange_n(p, &seen, v, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+#else
+    if (v > *p) *p = v;
+#endif
+}
+
 static void accumulate(uint32_t* deltas, uint32_t row, v3 lin, v3 ang, float scale) {
     uint32_t* d = deltas + row * 8u;
-    d[0] += solver_word(lin.x, scale);
-    d[1] += solver_word(lin.y, scale);
-    d[2] += solver_word(lin.z, scale);
-    d[3] += solver_word(ang.x, scale);
-    d[4] += solver_word(ang.y, scale);
-    d[5] += solver_word(ang.z, scale);
+    add_word(&d[0], solver_word(lin.x, scale));
+    add_word(&d[1], solver_word(lin.y, scale));
+    add_word(&d[2], solver_word(lin.z, scale));
+    add_word(&d[3], solver_word(ang.x, scale));
+    add_word(&d[4], solver_word(ang.y, scale));
+    add_word(&d[5], solver_word(ang.z, scale));
 }
 
 static void solver_blocks(oracle_world* w) {
@@ -2105,8 +2141,8 @@ static void commit_block(oracle_world* w, uint32_t fb, uint32_t sb, v3 da, v3 sa
     if (!residual) return;
     uint32_t lw = solver_word(fmaxo(vlen(da), vlen(db)), DYN_SOLVER_VELOCITY_SCALE);
     uint32_t aw = solver_word(fmaxo(vlen(sa), vlen(sbv)), DYN_SOLVER_VELOCITY_SCALE);
-    if (lw > *ctr(w, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL)) *ctr(w, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL) = lw;
-    if (aw > *ctr(w, DYN_COUNTER_SOLVE_ANGULAR_RESIDUAL)) *ctr(w, DYN_COUNTER_SOLVE_ANGULAR_RESIDUAL) = aw;
+    max_word(ctr(w, DYN_COUNTER_SOLVE_LINEAR_RESIDUAL), lw);
+    max_word(ctr(w, DYN_COUNTER_SOLVE_ANGULAR_RESIDUAL), aw);
 }
 
 static const v3 kZero = {0.0f, 0.0f, 0.0f};
@@ -2240,6 +2276,7 @@ static void warm_block(oracle_world* w, uint32_t ci) {
 
 /* solver_block_apply.wgsl */
 static void block_apply(oracle_world* w) {
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < w->nrows; i++) {
         uint32_t row = w->rows[i];
         uint32_t* d = w->vdeltas + row * 8u;
@@ -2258,6 +2295,7 @@ static void block_apply(oracle_world* w) {
 /* substep_integrate.wgsl:15-76 (no fields) */
 static void integrate(oracle_world* w) {
     float h = w->p.substep_dt;
+#pragma omp parallel for schedule(static)
     for (uint32_t li = 0; li < w->nlive; li++) {
         uint32_t row = w->live[li];
         const dyn_body_desc* d = &w->de[row];
@@ -2299,6 +2337,7 @@ static void integrate(oracle_world* w) {
 /* substep_advance.wgsl */
 static void advance(oracle_world* w) {
     float h = w->p.substep_dt;
+#pragma omp parallel for schedule(static)
     for (uint32_t li = 0; li < w->nlive; li++) {
         dyn_body_state* s = &w->st[w->live[li]];
         vstore(s->position, vadd(vload(s->position), vmul(vload(s->velocity), h)));
@@ -2314,6 +2353,7 @@ static void advance(oracle_world* w) {
 /* position_correction.wgsl:5-46 + position_block.wgsl */
 static void position_block(oracle_world* w) {
     uint32_t blocks = *ctr(w, DYN_COUNTER_BLOCKS);
+#pragma omp parallel for schedule(static)
     for (uint32_t ci = 0; ci < blocks; ci++) {
         const dyn_contact* c = &w->contacts[ci];
         if (!block_resolves(c) || c->relaxation > 0.0f) continue;
@@ -2339,8 +2379,8 @@ static void position_block(oracle_world* w) {
         v3 la = vmul(vneg(correction), first.d.inverse_mass);
         v3 lb = vmul(correction, second.d.inverse_mass);
         if (!(vdot(la, la) > 0.0f || vdot(lb, lb) > 0.0f)) continue; /* pair_holds */
-        w->contrib[fr]++;
-        w->contrib[sr]++;
+        add_word(&w->contrib[fr], 1u);
+        add_word(&w->contrib[sr], 1u);
         accumulate(w->pdeltas, fr, la, kZero, DYN_SOLVER_POSITION_SCALE);
         accumulate(w->pdeltas, sr, lb, kZero, DYN_SOLVER_POSITION_SCALE);
     }
@@ -2348,6 +2388,7 @@ static void position_block(oracle_world* w) {
 
 /* position_apply.wgsl */
 static void position_apply(oracle_world* w) {
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < w->nrows; i++) {
         uint32_t row = w->rows[i];
         uint32_t contributing = w->contrib[row];
@@ -2634,6 +2675,7 @@ void oracle_world_step(oracle_world* w, float dt) {
     prepare_and_index(w);
 
     /* narrowphase + compaction (compact_*.wgsl) */
+#pragma omp parallel for schedule(dynamic, 256)
     for (uint32_t i = 0; i < w->npairs; i++) narrow_work(w, i);
     w->ncontacts = 0;
     for (uint32_t i = 0; i < w->npairs; i++) {
@@ -2679,6 +2721,7 @@ void oracle_world_step(oracle_world* w, float dt) {
     for (uint32_t sub = 0; sub < w->p.substeps; sub++) {
         integrate(w);
         if (sub == 0) {
+#pragma omp parallel for schedule(static)
             for (uint32_t ci = 0; ci < w->ncontacts; ci++) warm_block(w, ci);
             block_apply(w);
         }
@@ -2686,6 +2729,7 @@ void oracle_world_step(oracle_world* w, float dt) {
             /* solver_block_solve.wgsl:85-87 with the round counter of solver_block_apply.wgsl:
                only the warm-started first substep reaches rounds == solve_iterations */
             int residual = sub == 0 && it + 1u == w->p.solve_iterations;
+#pragma omp parallel for schedule(static)
             for (uint32_t ci = 0; ci < w->ncontacts; ci++) solve_block(w, ci, residual);
             block_apply(w);
         }
@@ -2911,11 +2955,13 @@ int oracle_world_solve(oracle_world* w, dyn_contact* contacts, uint32_t count, f
     for (uint32_t sub = 0; sub < w->p.substeps; sub++) {
         integrate(w);
         if (sub == 0) {
+#pragma omp parallel for schedule(static)
             for (uint32_t ci = 0; ci < w->ncontacts; ci++) warm_block(w, ci);
             block_apply(w);
         }
         for (uint32_t it = 0; it < w->p.solve_iterations; it++) {
             int residual = sub == 0 && it + 1u == w->p.solve_iterations;
+#pragma omp parallel for schedule(static)
             for (uint32_t ci = 0; ci < w->ncontacts; ci++) solve_block(w, ci, residual);
             block_apply(w);
         }
@@ -2941,6 +2987,7 @@ int oracle_world_narrowphase(oracle_world* w, const uint32_t* major, const uint3
     memcpy(w->pmaj, major, (size_t)count * 4u);
     memcpy(w->pmin, minor, (size_t)count * 4u);
     w->npairs = count;
+#pragma omp parallel for schedule(dynamic, 256)
     for (uint32_t i = 0; i < count; i++) narrow_work(w, i);
     for (uint32_t i = 0; i < count; i++) {
         valid[i] = w->valid[i];

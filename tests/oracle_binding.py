@@ -18,6 +18,7 @@ from dynamis_amd.world import BodyCommands
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_SO = ORACLE_DIR / "_build" / "liboracle.so"
+ORACLE_OMP_SO = ORACLE_DIR / "_build" / "liboracle_omp.so"  # OpenMP build: same results, all cores
 
 _u32 = ctypes.c_uint32
 _p = ctypes.c_void_p
@@ -32,6 +33,8 @@ class Oracle:
     def __init__(self, handle: ctypes.CDLL):
         self.h = handle
         h = handle
+        h.oracle_threads.restype = _u32
+        h.oracle_threads.argtypes = []
         h.oracle_units_for.restype = _u32
         h.oracle_units_for.argtypes = [_u32]
         h.oracle_key_words.restype = _u32
@@ -83,6 +86,9 @@ class Oracle:
         h.oracle_inertia_inverse.argtypes = [_p, _p]
         h.oracle_relay_impulses.restype = None
         h.oracle_relay_impulses.argtypes = [_p, _p, _p, _p]
+
+    def threads(self) -> int:
+        return int(self.h.oracle_threads())
 
     def units_for(self, live: int) -> int:
         return int(self.h.oracle_units_for(live))
@@ -288,13 +294,14 @@ class OracleWorld(BodyCommands):
         self.close()
 
 
-_oracle: Oracle | None = None
+_oracles: dict[bool, Oracle] = {}
 
 
-def load() -> Oracle:
-    global _oracle
-    if _oracle is None:
-        if not ORACLE_SO.exists():
+def load(parallel: bool = False) -> Oracle:
+    """The serial oracle, or with `parallel` its OpenMP build (the all-core CPU baseline)."""
+    if parallel not in _oracles:
+        path = ORACLE_OMP_SO if parallel else ORACLE_SO
+        if not path.exists():
             build()
-        _oracle = Oracle(ctypes.CDLL(str(ORACLE_SO)))
-    return _oracle
+        _oracles[parallel] = Oracle(ctypes.CDLL(str(path)))
+    return _oracles[parallel]
