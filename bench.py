@@ -23,6 +23,8 @@ Workloads (`--workload`):
   narrowphase the narrowphase pass alone (dyn_narrowphase) over the candidate pairs of the pile's
               600th step, saved by a first run (--stage-file) so that a later run executes the pass
               and nothing else (for rocprofv3 counter passes).
+  solve       the solver alone (dyn_solve) over the contacts of the pile's 600th step, from the
+              same saved stage inputs (for rocprofv3 counter passes on the solver's kernels).
   adjoint     the differentiable rollout alone: 256 worlds of 32 boxes (half-extents
               ~ U[0.2, 0.5]) on a plane, v0 ~ N(0, 2 m/s), friction ~ U[0.2, 0.8], dt = 1/120,
               8 iterations, 64 steps with a checkpoint every 8; forward and backward times and
@@ -115,8 +117,10 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
         return contacts * (48.0 + 32.0 + 32.0)
     if name in ("k_apply", "k_position_apply"):
         return rows * 72.0
-    if name == "k_narrow":              # two bodies and colliders per pair, manifold out
-        return pairs * 316.0 + contacts * manifold
+    if name == "k_narrow":              # worklist entry, two bodies and colliders per pair, manifold out
+        return pairs * 320.0 + contacts * manifold
+    if name == "k_pair_class":          # the pair, both colliders' kinds, the class out
+        return pairs * 17.0
     if name == "k_narrow_convex":       # the GJK worklist: pair record + index, parked results
         return gjk * 320.0 + manifold_pairs * 68.0 + gjk_resumed * 208.0  # and parked GJK states
     if name == "k_narrow_convex_resume":  # pair record + index, GJK state in, results out
@@ -588,35 +592,75 @@ def run_fixed_grid(steps: int, warmup: int):
     return ms, stages, counts
 
 
-def run_narrowphase(steps: int, warmup: int, stage_file: str) -> dict:
-    """The narrowphase pass alone (dyn_narrowphase: the kernels dyn_world_step runs) over the
-    settled pile's candidate pairs. Without `stage_file` on disk, the pile runs its 600 steps
-    first and the states its last step began from, with that step's pairs, are saved there; with
-    it, they are loaded and only the pass runs, so a counter pass (rocprofv3 --pmc) sees no world
-    step at all."""
+def pile_stage(stage_file: str):
+    """The settled pile's stage inputs: the body states step 600 began from, that step's
+    candidate pairs and the contacts it solved. Run once and saved in `stage_file` (numpy .npz);
+    later runs load them, so a counter pass (rocprofv3 --pmc) over a stage sees no world step."""
     import dynamis_amd
-    from dynamis_amd.device import Event, Stream, profile_collect, profile_enable
-    from dynamis_amd.narrowphase import Narrowphase
     from dynamis_amd.world import World
 
     if dynamis_amd.device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
     dynamis_amd._native.call("dyn_set_device", 0)
     scene = pile_scene()
-    if not os.path.exists(stage_file):
-        world = World()
-        world.create(scene)
-        world.set_graphs(GRAPHS)
-        run = WORKLOADS["pile"]["run"]
-        for _ in range(run - 1):
-            world.step(DT)
-        states = world.states()
+    if os.path.exists(stage_file):
+        data = np.load(stage_file)
+        if "contacts" in data.files:
+            return scene, data
+    world = World()
+    world.create(scene)
+    world.set_graphs(GRAPHS)
+    run = WORKLOADS["pile"]["run"]
+    for _ in range(run - 1):
         world.step(DT)
-        pairs = world.pairs()
-        world.close()
-        np.savez(stage_file, states=states, pairs=pairs)
-        progress(f"saved the pile's step-{run} narrowphase input ({len(pairs)} pairs) to {stage_file}")
-    data = np.load(stage_file)
+    states = world.states()
+    world.step(DT)
+    pairs = world.pairs()
+    contacts = world.inspect_contacts()
+    world.close()
+    np.savez(stage_file, states=states, pairs=pairs, contacts=contacts)
+    progress(f"saved the pile's step-{run} stage inputs ({len(pairs)} pairs, {len(contacts)} contacts) "
+             f"to {stage_file}")
+    return scene, np.load(stage_file)
+
+
+def run_solve(steps: int, warmup: int, stage_file: str) -> dict:
+    """The solver alone (dyn_solve: SolverReset and SolveSubsteps, the kernels dyn_world_step
+    runs) over the settled pile's step-600 contacts from the states that step began from,
+    uploaded once and solved again and again on the device."""
+    from dynamis_amd.device import Event, Stream, profile_collect, profile_enable
+    from dynamis_amd.solver import Solver
+
+    scene, data = pile_stage(stage_file)
+    states, contacts = data["states"], data["contacts"]
+    stream = Stream()
+    solver = Solver(scene, len(contacts), stream=stream)
+    solver.run(states, contacts, DT)
+    for _ in range(warmup):
+        solver.again(DT)
+    stream.synchronize()
+    start, stop = Event(), Event()
+    start.record(stream)
+    for _ in range(steps):
+        solver.again(DT)
+    stop.record(stream)
+    ms = start.elapsed_ms(stop)
+    stream.synchronize()
+    profile_enable(True)
+    for _ in range(min(steps, 10)):
+        solver.again(DT)
+    stream.synchronize()
+    profile_enable(False)
+    return {"ms": ms, "contacts": int(len(contacts)), "stages": profile_collect(), "bodies": int(len(scene.states))}
+
+
+def run_narrowphase(steps: int, warmup: int, stage_file: str) -> dict:
+    """The narrowphase pass alone (dyn_narrowphase: the kernels dyn_world_step runs) over the
+    settled pile's step-600 candidate pairs from the states that step began from (pile_stage)."""
+    from dynamis_amd.device import Event, Stream, profile_collect, profile_enable
+    from dynamis_amd.narrowphase import Narrowphase
+
+    scene, data = pile_stage(stage_file)
     states, pairs = data["states"], data["pairs"]
     stream = Stream()
     stage = Narrowphase(scene, len(pairs), stream=stream)
@@ -926,7 +970,7 @@ def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--workload", choices=["pile", "cubes", "batch", "broadphase", "fixed_grid", "copy",
-                                               "adjoint", "narrowphase"], default="pile")
+                                               "adjoint", "narrowphase", "solve"], default="pile")
     parser.add_argument("--steps", type=int, default=None)
     parser.add_argument("--warmup", type=int, default=None)
     parser.add_argument("--no-graphs", action="store_true",
@@ -938,8 +982,8 @@ def main() -> int:
                         help="oracle steps (broadphase: passes) timed for the CPU baseline; 0 skips it")
     parser.add_argument("--only", action="store_true",
                         help="pile: leave out the batch, broadphase and adjoint parts of the line")
-    parser.add_argument("--stage-file", default="/tmp/dynamis_pile_narrowphase.npz",
-                        help="narrowphase: where the settled pile's narrowphase input is saved or found")
+    parser.add_argument("--stage-file", default="/tmp/dynamis_pile_stage.npz",
+                        help="narrowphase, solve: where the settled pile's stage inputs are saved or found")
     args = parser.parse_args()
     global GRAPHS, TUNE
     GRAPHS = not args.no_graphs
@@ -969,6 +1013,19 @@ def main() -> int:
                           "config": {"workload": "dyn_narrowphase over the candidate pairs of step 600 of the "
                                                  "100,000-body mixed convex pile", "pairs": r["pairs"],
                                      "contacts": r["contacts"], "bodies": r["bodies"]},
+                          "stages": {k: {"launches": v[0], "ms": v[1]} for k, v in r["stages"].items()}}))
+        return 0
+    if wl == "solve":
+        steps = max(args.steps if args.steps is not None else 20, 1)
+        warmup = max(args.warmup if args.warmup is not None else 3, 0)
+        r = run_solve(steps, warmup, args.stage_file)
+        print(json.dumps({"metric": "solver over the settled pile's contacts", "unit": "contact-solves/s",
+                          "value": r["contacts"] * steps / (r["ms"] / 1000.0), "n_gpus": 1, "steps": steps,
+                          "warmup": warmup, "ms_per_step": r["ms"] / steps, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "dyn_solve (2 substeps, 8 iterations, 10 position iterations) "
+                                                 "over the contacts of step 600 of the 100,000-body mixed "
+                                                 "convex pile", "contacts": r["contacts"], "bodies": r["bodies"]},
                           "stages": {k: {"launches": v[0], "ms": v[1]} for k, v in r["stages"].items()}}))
         return 0
     if wl == "fixed_grid":

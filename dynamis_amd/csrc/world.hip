@@ -84,7 +84,8 @@ struct W {
     uint32_t* class_hist;  // per kind class and block of pairs: count, then offset in the class;
                            // then the classes' totals
     uint32_t* work_total;  // device words: entries on the worklist of modes 0-2, the hull-class GJK pairs,
-                           // the resumed GJK pairs (mode 4), the EPA polytopes given a wave
+                           // the resumed GJK pairs (mode 4), the EPA polytopes given a wave, the
+                           // candidate pairs (mode 6)
     uint32_t* work_keys;   // GJK worklist sort keys: class, then the hull sources
     uint32_t* work_minor;  // the sort's unused minor lane
     uint32_t* dest;
@@ -497,9 +498,12 @@ __device__ __forceinline__ uint8_t kind_class(uint32_t first_kind, uint32_t seco
     return static_cast<uint8_t>(a * 6u - (a * (a - 1u)) / 2u + (b - a));
 }
 
-// The pairs a convex kernel takes: kPairGjk (mode 0), the pairs k_narrow_finish completes
-// (mode 1), kPairEpa (mode 2).
+// The pairs a narrowphase kernel takes: kPairGjk (mode 0), the pairs k_narrow_finish completes
+// (mode 1), kPairEpa (mode 2), kPairGjkResume (mode 4), every candidate pair (mode 6, for
+// k_narrow; its marks are not written yet).
+constexpr uint32_t kWorkAllPairs = 6;
 __device__ __forceinline__ uint32_t work_class(const W& w, uint32_t index, uint32_t mode) {
+    if (mode == kWorkAllPairs) return static_cast<uint32_t>(w.pkind[index]);
     const uint32_t v = w.valid[index];
     const bool take = mode == 0u   ? v == kPairGjk
                       : mode == 1u ? (v == kPairEpaDone || v == kPairHit || v == kPairPlane)
@@ -588,10 +592,22 @@ __global__ void __launch_bounds__(kThreads) k_work_keys(W w, uint32_t mode) {
 }
 
 // Analytic pairs; convex-path pairs are marked for the kernels below.
-__global__ void __launch_bounds__(kThreads) k_narrow(W w) {
+// Every candidate pair's kind class (pairs naming colliders outside the records: class 0), so
+// k_narrow's worklist groups its pairs by kind and a wave runs one generator.
+__global__ void __launch_bounds__(kThreads) k_pair_class(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
     if (index >= live) return;
+    const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
+    w.pkind[index] = (fs < w.k && ss < w.k) ? kind_class(w.co[fs].kind, w.co[ss].kind) : 0u;
+}
+
+// One thread per candidate pair of the kind-grouped worklist (every pair's result goes to its
+// own slot, so the order changes nothing).
+__global__ void __launch_bounds__(kThreads) k_narrow(W w) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w.work_total[kWorkAllPairs]) return;
+    const uint32_t index = w.worklist[t];
     w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     uint32_t fb, sb;
@@ -600,8 +616,7 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     const Body first = load_body(w, fb), second = load_body(w, sb);
     const float margin = pair_margin(w, first, second);
     const ShapeSet shapes{w.sources, w.vertices, w.support_cells, w.support_cands};
-    if (convex_path(fc, sc)) {
-        w.pkind[index] = kind_class(fc.kind, sc.kind);
+    if (convex_path(fc, sc)) {  // pkind holds the pair's class (k_pair_class)
         if (fc.kind == DYN_SHAPE_PLANE || sc.kind == DYN_SHAPE_PLANE) {
             w.valid[index] = kPairPlane;
         } else if (!bounds_apart(world_collider(first.s, fc), world_collider(second.s, sc), shapes, margin)) {
@@ -2519,6 +2534,8 @@ int worklist(const dyn::W& w, hipStream_t st, uint32_t mode) {
 int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch, size_t work_scratch_bytes) {
     using namespace dyn;
     const uint32_t pb = blocks_for(w.pair_cap);
+    LAUNCH_ON(st, k_pair_class, pb, w);
+    DYN_TRY(worklist(w, st, kWorkAllPairs));
     LAUNCH_ON(st, k_narrow, pb, w);
     if (!convex) return DYN_OK;
     DYN_TRY(worklist(w, st, 0u));

@@ -62,6 +62,11 @@ class Solver:
             self.contacts.upload(contacts)
         self.count = len(contacts)
         self.count_word.upload(np.array([self.count], np.uint32))
+        self.again(dt)
+
+    def again(self, dt: float) -> None:
+        """Enqueues the solver once more on the device's states and contacts as the last run left
+        them (no upload)."""
         io = SolveIO(self.params.ctypes.data, self.states.ptr, self.descs.ptr, self.bodies,
                      int(self.scene.dynamic_count), len(self.scene.colliders), self.capacity, self.owners.ptr,
                      self.contacts.ptr, self.count_word.ptr, self.counters.ptr, self.scratch.ptr,
