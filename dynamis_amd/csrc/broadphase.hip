@@ -27,6 +27,9 @@ constexpr uint32_t kResolutionBlocks = 1024;
 constexpr uint32_t kSparseLevel = 32;  // coarser levels of at most this many entries are read whole
 constexpr uint32_t kLinkStage = 64;    // link pairs a collider's counting pass keeps for the emitting pass
 constexpr uint32_t kGatherRun = 8;     // mean run length of a coarser level from which records are gathered
+constexpr uint32_t kLinkLanesBelow = 32768;  // up to this many colliders the link passes lack threads: 8 lanes
+constexpr uint32_t kLinkModeLanes = 1u;      // link mode word: eight lanes per collider
+constexpr uint32_t kLinkModeGathered = 2u;   // link mode word: records gathered into key order
 
 __device__ __forceinline__ uint32_t* ctr(uint32_t* counters, uint32_t slot) {
     return counters + slot * DYN_COUNTER_STRIDE_WORDS;
@@ -221,7 +224,8 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
     const uint32_t* __restrict__ cnt_level, const uint32_t* __restrict__ block_offsets,
     uint32_t* __restrict__ entry_keys, uint32_t* __restrict__ entry_order,
     dyn_grid_entry* __restrict__ entries, uint32_t capacity, uint32_t* sort_hist,
-    const uint32_t* sort_ctl, uint32_t* flags0, const uint32_t* __restrict__ worlds) {
+    const uint32_t* sort_ctl, uint32_t* flags0, const uint32_t* __restrict__ worlds,
+    uint32_t* __restrict__ first_slot) {
     __shared__ uint32_t s_hist[kEntryPasses][kSortBins];
     __shared__ uint32_t s_scan[kBpThreads / kWave];
     const uint32_t t = threadIdx.x;
@@ -233,6 +237,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_emit_kernel(
     const uint32_t level = packed >> 8;
     const uint32_t first = block_offsets[blockIdx.x] +
                            block_exclusive_sum<kBpThreads>(emitted, s_scan, nullptr);
+    if (i < n) first_slot[i] = first;  // the primary entry's slot, for the level links
     uint32_t faults = 0;
     if (emitted) {
         const float base = grid_cell_size(*ctr(counters, DYN_COUNTER_GRID_SCALE),
@@ -331,8 +336,13 @@ __global__ void __launch_bounds__(kBpThreads) bp_gather_kernel(const uint32_t* _
                                                                const uint32_t* __restrict__ dir,
                                                                const uint32_t* counters,
                                                                const uint32_t* __restrict__ level_keys,
-                                                               dyn_grid_entry* __restrict__ sorted) {
-    if (!gather_pays(dir, counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS], level_keys)) return;
+                                                               dyn_grid_entry* __restrict__ sorted,
+                                                               uint32_t colliders, uint32_t* link_mode) {
+    const bool pays = gather_pays(dir, counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS], level_keys);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the level links' lanes, and whether records are gathered
+        *link_mode = (pays || colliders <= kLinkLanesBelow ? kLinkModeLanes : 0u) | (pays ? kLinkModeGathered : 0u);
+    }
+    if (!pays) return;
     const uint32_t len = sort_ctl[kCtlLen];
     for (uint32_t e = blockIdx.x * kBpThreads + threadIdx.x; e < len; e += gridDim.x * kBpThreads) {
         const float4* src = reinterpret_cast<const float4*>(entries + order[e]);
@@ -379,60 +389,65 @@ struct PairCounter {
     __device__ __forceinline__ void operator()(uint32_t, uint32_t) { ++n; }
 };
 
-// The level-links passes give each collider an aligned group of kLinkLanes lanes, which test
-// kLinkLanes consecutive run entries at a time; a group's matches are numbered in entry order
-// (ballot ranks), so its pairs come out in the order of a one-thread walk. Group sinks are
-// called by every lane of the group together, each with its own entry's outcome.
+// The level-links passes give each collider an aligned group of kLanes lanes (8 where the
+// coarser levels hold long runs, else 1), which test kLanes consecutive run entries at a time; a
+// group's matches are numbered in entry order (ballot ranks), so its pairs come out in the order
+// of a one-thread walk. Group sinks are called by every lane of the group together, each with
+// its own entry's outcome.
 constexpr uint32_t kLinkLanes = 8;
-constexpr uint32_t kLinkColliders = kBpThreads / kLinkLanes;  // colliders per block of the link passes
+constexpr uint32_t kLinkColliders = kBpThreads / kLinkLanes;  // colliders per block, 8 lanes each
 
+template <uint32_t kLanes>
 __device__ __forceinline__ uint32_t link_group_bits(bool p) {
     const uint32_t lane = lane_id();
-    return static_cast<uint32_t>(__ballot(p) >> (lane & ~(kLinkLanes - 1u))) & ((1u << kLinkLanes) - 1u);
+    return static_cast<uint32_t>(__ballot(p) >> (lane & ~(kLanes - 1u))) & ((1u << kLanes) - 1u);
 }
 
-// Lower bound of `key` in the sorted keys[lo, hi) by the lane group: rounds of kLinkLanes probes
-// at evenly spaced positions narrow the range about (kLinkLanes + 1)-fold, then the group reads
-// the last few keys at once.
+// Lower bound of `key` in the sorted keys[lo, hi) by the lane group: rounds of kLanes probes at
+// evenly spaced positions narrow the range about (kLanes + 1)-fold (a bisection for one lane),
+// then the group reads the last few keys at once.
+template <uint32_t kLanes>
 __device__ __forceinline__ uint32_t group_lower_bound(const uint32_t* __restrict__ keys, uint32_t lo,
                                                       uint32_t hi, uint32_t key) {
-    const uint32_t q = lane_id() % kLinkLanes;
+    const uint32_t q = lane_id() % kLanes;
     const auto at = [&](uint32_t span, uint32_t lane) {
-        return lo + static_cast<uint32_t>(static_cast<uint64_t>(span) * (lane + 1u) / (kLinkLanes + 1u));
+        return lo + static_cast<uint32_t>(static_cast<uint64_t>(span) * (lane + 1u) / (kLanes + 1u));
     };
-    while (hi - lo > kLinkLanes) {  // probe positions strictly increase across the lanes
+    while (hi - lo > kLanes) {  // probe positions strictly increase across the lanes
         const uint32_t span = hi - lo;
-        const uint32_t below = __popc(link_group_bits(keys[at(span, q)] < key));
+        const uint32_t below = __popc(link_group_bits<kLanes>(keys[at(span, q)] < key));
         const uint32_t next_lo = below == 0u ? lo : at(span, below - 1u) + 1u;
-        hi = below == kLinkLanes ? hi : at(span, below);
+        hi = below == kLanes ? hi : at(span, below);
         lo = next_lo;
     }
     const uint32_t e = lo + q;
-    return lo + __popc(link_group_bits(e < hi && keys[e] < key));
+    return lo + __popc(link_group_bits<kLanes>(e < hi && keys[e] < key));
 }
 
 // Counts the group's pairs and keeps the first kLinkStage, which the emitting pass then copies
 // instead of walking again.
+template <uint32_t kLanes>
 struct GroupStage {
     uint32_t n = 0;
     uint2* stage;
     __device__ __forceinline__ void put(bool ok, uint32_t a, uint32_t b) {
-        const uint32_t bits = link_group_bits(ok);
-        const uint32_t at = n + __popc(bits & ((1u << (lane_id() % kLinkLanes)) - 1u));
+        const uint32_t bits = link_group_bits<kLanes>(ok);
+        const uint32_t at = n + __popc(bits & ((1u << (lane_id() % kLanes)) - 1u));
         if (ok && at < kLinkStage) stage[at] = make_uint2(a, b);
         n += __popc(bits);
     }
 };
 
 // Writes the group's pairs from `at` on, in order, up to the capacity.
+template <uint32_t kLanes>
 struct GroupWriter {
     uint32_t at;
     uint32_t capacity;
     uint32_t* major;
     uint32_t* minor;
     __device__ __forceinline__ void put(bool ok, uint32_t a, uint32_t b) {
-        const uint32_t bits = link_group_bits(ok);
-        const uint32_t to = at + __popc(bits & ((1u << (lane_id() % kLinkLanes)) - 1u));
+        const uint32_t bits = link_group_bits<kLanes>(ok);
+        const uint32_t to = at + __popc(bits & ((1u << (lane_id() % kLanes)) - 1u));
         if (ok && to < capacity) {
             major[to] = a;
             minor[to] = b;
@@ -601,7 +616,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_mates_emit_kernel(
 // of the collider's own entries, so the reference's skip of the probing entry itself never
 // applies. Entry records are read in sorted order (bp_gather_kernel), so a run is read
 // contiguously, the group's lanes reading neighbouring records.
-template <typename Sink>
+template <uint32_t kLanes, typename Sink>
 __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_t slot,
                                             const dyn_aabb* __restrict__ aabbs,
                                             const uint32_t* __restrict__ groups,
@@ -611,7 +626,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                                             const uint32_t* __restrict__ dir, bool coarse_active,
                                             uint32_t occupied, float grid, uint32_t capacity,
                                             const uint32_t* __restrict__ worlds, Sink& sink) {
-    const uint32_t q = lane_id() % kLinkLanes;
+    const uint32_t q = lane_id() % kLanes;
     const uint32_t emitted = packed & 0xFFu;
     if (emitted == 0u || slot >= capacity) return;
     const uint32_t f = flags ? flags[i] : (DYN_BP_MOBILE | DYN_BP_AWAKE);
@@ -640,7 +655,7 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             // A sparse level is read whole: an entry is in the run of one of this box's cells c
             // exactly when its own cell is c and its key is c's key in this world, which
             // mate_in_cell and the key test below check, so the same pairs come out.
-            for (uint32_t base = level_first; base < level_end; base += kLinkLanes) {
+            for (uint32_t base = level_first; base < level_end; base += kLanes) {
                 const uint32_t e = base + q;
                 bool ok = false;
                 uint32_t a = 0u, b = 0u;
@@ -672,8 +687,8 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
             const uint32_t bucket = (key & DYN_CELL_HASH_MASK) >> (DYN_LEVEL_KEY_SHIFT - kDirBits);
             const uint32_t* d = dir + level_of_key * kDirStride + bucket;
             const uint32_t bucket_end = d[1];
-            for (uint32_t base = group_lower_bound(keys, d[0], bucket_end, key); base < bucket_end;
-                 base += kLinkLanes) {
+            for (uint32_t base = group_lower_bound<kLanes>(keys, d[0], bucket_end, key); base < bucket_end;
+                 base += kLanes) {
                 const uint32_t e = base + q;
                 const bool in_run = e < bucket_end && keys[e] == key;
                 bool ok = false;
@@ -685,117 +700,165 @@ __device__ __forceinline__ void level_links(uint32_t i, uint32_t packed, uint32_
                     }
                 }
                 sink.put(ok, a, b);
-                if (link_group_bits(in_run) != (1u << kLinkLanes) - 1u) break;  // the run ended here
+                if (link_group_bits<kLanes>(in_run) != (1u << kLanes) - 1u) break;  // the run ended here
             }
         }
     }
 }
 
-// The first entry slot of each collider of this block (blocks of kLinkColliders colliders): its
-// count block's offset plus the entries of the colliders before it in that block of kBpThreads.
-__device__ __forceinline__ uint32_t link_slot(const uint32_t* __restrict__ cnt_level, uint32_t n,
-                                              const uint32_t* __restrict__ block_offsets, uint32_t* s_scan,
-                                              uint32_t* s_slot) {
-    const uint32_t c0 = blockIdx.x * kLinkColliders;
-    const uint32_t seg = c0 / kBpThreads;
-    const uint32_t j = seg * kBpThreads + threadIdx.x;
-    const uint32_t before = block_exclusive_sum<kBpThreads>(j < n ? (cnt_level[j] & 0xFFu) : 0u, s_scan, nullptr);
-    const uint32_t local = c0 - seg * kBpThreads;
-    if (threadIdx.x >= local && threadIdx.x < local + kLinkColliders) {
-        s_slot[threadIdx.x - local] = block_offsets[seg] + before;
-    }
-    __syncthreads();
-    return s_slot[threadIdx.x / kLinkLanes];
+// Whether the level links give a collider kLinkLanes lanes: where some coarser level holds long
+// runs of equal keys (the records are then gathered into key order too, gather_pays), which every
+// finer collider probing that level reads, or where the colliders are too few to fill the GPU one
+// lane each; otherwise the lanes would mostly wait, and one lane walks a collider's probes.
+// bp_gather_kernel writes the choice and the gather decision to one word (kLinkModeLanes,
+// kLinkModeGathered), which both link passes and the offsets read. Each pass is launched in both
+// shapes, a workgroup per chunk; the workgroups of the shape not chosen read the word and return.
+
+template <uint32_t kLanes>
+__device__ __forceinline__ uint32_t link_chunk_colliders() { return kBpThreads / kLanes; }
+
+__host__ __device__ __forceinline__ uint32_t link_chunks(uint32_t n, uint32_t lanes) {
+    return ceil_div(n ? n : 1u, kBpThreads / lanes);
 }
 
-// Level links, counting pass: kLinkLanes lanes per collider.
+struct LinkArgs {
+    const dyn_aabb* aabbs;
+    const uint32_t* groups;
+    const uint32_t* flags;
+    uint32_t n;
+    const uint32_t* cnt_level;
+    const uint32_t* first_slot;
+    const uint32_t* keys;
+    EntryRecords rec;
+    const uint32_t* dir;
+    bool coarse_active;
+    uint32_t occupied;
+    float grid;
+    uint32_t capacity;
+    const uint32_t* worlds;
+};
+
+__device__ __forceinline__ LinkArgs link_args(const dyn_aabb* aabbs, const uint32_t* groups, const uint32_t* flags,
+                                              uint32_t n, const uint32_t* cnt_level, const uint32_t* first_slot,
+                                              const uint32_t* keys, EntryRecords rec, const uint32_t* dir,
+                                              const uint32_t* counters, uint32_t capacity, const uint32_t* worlds,
+                                              uint32_t mode) {
+    rec.gathered = (mode & kLinkModeGathered) != 0u;
+    return LinkArgs{aabbs, groups, flags, n, cnt_level, first_slot, keys, rec, dir,
+                    counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u,
+                    counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS],
+                    grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
+                                   counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]),
+                    capacity, worlds};
+}
+
+// Level links, counting pass: chunks of kBpThreads / kLanes consecutive colliders, a chunk per
+// workgroup, kLanes lanes per collider; link_blocks holds each chunk's pair total.
+template <uint32_t kLanes>
 __global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
-    const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ first_slot, const uint32_t* __restrict__ keys,
     EntryRecords rec,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
-    const uint32_t* __restrict__ worlds, uint32_t* link_counts, uint32_t* link_blocks, uint2* link_stage) {
+    const uint32_t* __restrict__ worlds, const uint32_t* link_mode, uint32_t* link_counts,
+    uint32_t* link_blocks, uint2* link_stage) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
-    __shared__ uint32_t s_slot[kLinkColliders];
-    const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
-    const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
-    const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
-                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
-    const uint32_t slot = link_slot(cnt_level, n, block_offsets, s_scan, s_slot);
-    const uint32_t i = blockIdx.x * kLinkColliders + threadIdx.x / kLinkLanes;
-    const bool leader = threadIdx.x % kLinkLanes == 0u;
-    rec.gathered = gather_pays(dir, occupied, rec.level_keys);
-    GroupStage counter{0u, link_stage + static_cast<size_t>(i) * kLinkStage};
-    if (i < n) {
-        level_links(i, i < n ? cnt_level[i] : 0u, slot, aabbs, groups, flags, keys, rec, dir, coarse_active,
-                    occupied, grid, capacity, worlds, counter);
-        if (leader) link_counts[i] = counter.n;
+    const uint32_t mode = *link_mode;
+    if (((mode & kLinkModeLanes) != 0u) != (kLanes > 1u)) return;
+    const LinkArgs a = link_args(aabbs, groups, flags, n, cnt_level, first_slot, keys, rec, dir, counters,
+                                 capacity, worlds, mode);
+    const uint32_t chunks = link_chunks(n, kLanes);
+    for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        const uint32_t i = chunk * link_chunk_colliders<kLanes>() + threadIdx.x / kLanes;
+        const bool leader = threadIdx.x % kLanes == 0u && i < n;
+        GroupStage<kLanes> counter{0u, link_stage + static_cast<size_t>(i < n ? i : 0u) * kLinkStage};
+        if (i < n) {
+            level_links<kLanes>(i, cnt_level[i], first_slot[i], a.aabbs, a.groups, a.flags, a.keys, a.rec, a.dir,
+                                a.coarse_active, a.occupied, a.grid, a.capacity, a.worlds, counter);
+            if (leader) link_counts[i] = counter.n;
+        }
+        uint32_t total = 0;
+        block_exclusive_sum<kBpThreads>(leader ? counter.n : 0u, s_scan, &total);
+        if (threadIdx.x == 0) link_blocks[chunk] = total;
     }
-    uint32_t total = 0;
-    block_exclusive_sum<kBpThreads>(leader && i < n ? counter.n : 0u, s_scan, &total);
-    if (threadIdx.x == 0) link_blocks[blockIdx.x] = total;
 }
 
-// Level links, emitting pass.
+// Level links, emitting pass: the counting pass's chunks and lanes.
+template <uint32_t kLanes>
 __global__ void __launch_bounds__(kBpThreads) bp_links_emit_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
-    const uint32_t* __restrict__ block_offsets, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ first_slot, const uint32_t* __restrict__ keys,
     EntryRecords rec,
     const uint32_t* __restrict__ dir, const uint32_t* counters, uint32_t capacity,
-    const uint32_t* __restrict__ worlds, const uint32_t* __restrict__ link_counts,
+    const uint32_t* __restrict__ worlds, const uint32_t* link_mode, const uint32_t* __restrict__ link_counts,
     const uint32_t* __restrict__ link_offsets, const uint2* __restrict__ link_stage,
     uint32_t* pair_major, uint32_t* pair_minor, uint32_t pair_capacity) {
     __shared__ uint32_t s_scan[kBpThreads / kWave];
-    __shared__ uint32_t s_slot[kLinkColliders];
-    __shared__ uint32_t s_at[kLinkColliders];
-    const bool coarse_active = counters[DYN_COUNTER_COARSE_ACTIVE * DYN_COUNTER_STRIDE_WORDS] != 0u;
-    const uint32_t occupied = counters[DYN_COUNTER_GRID_LEVELS * DYN_COUNTER_STRIDE_WORDS];
-    const float grid = grid_cell_size(counters[DYN_COUNTER_GRID_SCALE * DYN_COUNTER_STRIDE_WORDS],
-                                      counters[DYN_COUNTER_GRID_EXTENT * DYN_COUNTER_STRIDE_WORDS]);
-    const uint32_t slot = link_slot(cnt_level, n, block_offsets, s_scan, s_slot);
-    const uint32_t g = threadIdx.x / kLinkLanes, q = threadIdx.x % kLinkLanes;
-    const uint32_t i = blockIdx.x * kLinkColliders + g;
-    const uint32_t mine = i < n ? link_counts[i] : 0u;
-    const uint32_t excl = block_exclusive_sum<kBpThreads>(q == 0u ? mine : 0u, s_scan, nullptr);
-    if (q == 0u) s_at[g] = link_offsets[blockIdx.x] + excl;
-    __syncthreads();
-    const uint32_t at = s_at[g];
-    if (mine == 0u || at >= pair_capacity) return;
-    if (mine <= kLinkStage) {  // the counting pass kept them all
-        const uint2* kept = link_stage + static_cast<size_t>(i) * kLinkStage;
-        for (uint32_t j = q; j < mine && at + j < pair_capacity; j += kLinkLanes) {
-            const uint2 p = kept[j];
-            pair_major[at + j] = p.x;
-            pair_minor[at + j] = p.y;
+    __shared__ uint32_t s_at[kBpThreads / kLanes];
+    const uint32_t mode = *link_mode;
+    if (((mode & kLinkModeLanes) != 0u) != (kLanes > 1u)) return;
+    const LinkArgs a = link_args(aabbs, groups, flags, n, cnt_level, first_slot, keys, rec, dir, counters,
+                                 capacity, worlds, mode);
+    const uint32_t chunks = link_chunks(n, kLanes);
+    const uint32_t g = threadIdx.x / kLanes, q = threadIdx.x % kLanes;
+    for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        const uint32_t i = chunk * link_chunk_colliders<kLanes>() + g;
+        const uint32_t mine = i < n ? link_counts[i] : 0u;
+        const uint32_t excl = block_exclusive_sum<kBpThreads>(q == 0u ? mine : 0u, s_scan, nullptr);
+        if (q == 0u) s_at[g] = link_offsets[chunk] + excl;
+        __syncthreads();
+        const uint32_t at = s_at[g];
+        if (mine != 0u && at < pair_capacity) {
+            if (mine <= kLinkStage) {  // the counting pass kept them all
+                const uint2* kept = link_stage + static_cast<size_t>(i) * kLinkStage;
+                for (uint32_t j = q; j < mine && at + j < pair_capacity; j += kLanes) {
+                    const uint2 p = kept[j];
+                    pair_major[at + j] = p.x;
+                    pair_minor[at + j] = p.y;
+                }
+            } else {
+                GroupWriter<kLanes> writer{at, pair_capacity, pair_major, pair_minor};
+                level_links<kLanes>(i, cnt_level[i], first_slot[i], a.aabbs, a.groups, a.flags, a.keys, a.rec,
+                                    a.dir, a.coarse_active, a.occupied, a.grid, a.capacity, a.worlds, writer);
+            }
         }
-        return;
+        __syncthreads();  // s_at is reused by the next chunk
     }
-    rec.gathered = gather_pays(dir, occupied, rec.level_keys);
-    GroupWriter writer{at, pair_capacity, pair_major, pair_minor};
-    level_links(i, cnt_level[i], slot, aabbs, groups, flags, keys, rec, dir, coarse_active, occupied,
-                grid, capacity, worlds, writer);
 }
 
 // One workgroup: exclusive offsets of the link blocks, then of the mate blocks after them, the
 // pair count and the refused count.
-__global__ void __launch_bounds__(1024) bp_pair_offsets_kernel(uint32_t* link_blocks, uint32_t link_n,
+__global__ void __launch_bounds__(1024) bp_pair_offsets_kernel(uint32_t* link_blocks, uint32_t colliders,
+                                                               const uint32_t* link_mode,
                                                                uint32_t* mate_blocks, uint32_t mate_n,
                                                                uint32_t* counters, uint32_t pair_capacity) {
+    const uint32_t link_n = link_chunks(colliders, (*link_mode & kLinkModeLanes) ? kLinkLanes : 1u);
+    constexpr uint32_t kItems = 8;  // block totals per thread and round
     __shared__ uint32_t s_scan[1024 / kWave];
     __shared__ uint32_t s_carry;
     if (threadIdx.x == 0) s_carry = 0;
     for (int part = 0; part < 2; ++part) {
         uint32_t* blocks = part == 0 ? link_blocks : mate_blocks;
         const uint32_t count = part == 0 ? link_n : mate_n;
-        for (uint32_t first = 0; first < count; first += 1024) {
+        for (uint32_t first = 0; first < count; first += 1024u * kItems) {
             __syncthreads();
-            const uint32_t r = first + threadIdx.x;
-            const uint32_t v = r < count ? blocks[r] : 0u;
+            const uint32_t base = first + threadIdx.x * kItems;
+            uint32_t v[kItems];
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kItems; ++j) {
+                v[j] = base + j < count ? blocks[base + j] : 0u;
+                sum += v[j];
+            }
             uint32_t total = 0;
-            const uint32_t excl = block_exclusive_sum<1024>(v, s_scan, &total);
-            if (r < count) blocks[r] = s_carry + excl;
+            uint32_t run = s_carry + block_exclusive_sum<1024>(sum, s_scan, &total);
+#pragma unroll
+            for (uint32_t j = 0; j < kItems; ++j) {
+                if (base + j < count) blocks[base + j] = run;
+                run += v[j];
+            }
             __syncthreads();
             if (threadIdx.x == 0) s_carry += total;
         }
@@ -816,6 +879,8 @@ struct BpLayout {
     uint32_t* block_offsets;
     uint32_t* res_partials;
     uint32_t* link_counts;
+    uint32_t* first_slot;
+    uint32_t* link_mode;
     uint32_t* link_blocks;
     uint32_t* mate_counts;
     uint32_t* mate_blocks;
@@ -841,6 +906,8 @@ BpLayout bp_layout(uint32_t colliders, uint32_t entry_capacity, uint32_t pair_ca
     l.res_partials = carve.take<uint32_t>(2 * kResolutionBlocks);
     const uint32_t entry_blocks = ceil_div(entry_capacity ? entry_capacity : 1u, kBpThreads);
     l.link_counts = carve.take<uint32_t>(colliders ? colliders : 1u);
+    l.first_slot = carve.take<uint32_t>(colliders ? colliders : 1u);
+    l.link_mode = carve.take<uint32_t>(1u);
     l.link_blocks = carve.take<uint32_t>(ceil_div(colliders ? colliders : 1u, kLinkColliders));
     l.mate_counts = carve.take<uint32_t>(entry_capacity ? entry_capacity : 1u);
     l.mate_blocks = carve.take<uint32_t>(entry_blocks);
@@ -934,7 +1001,7 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
         bp_emit_kernel<<<blocks, kBpThreads, 0, st>>>(
             io->d_aabbs, io->d_groups, io->d_flags, n, counters, l.cnt_level, l.block_offsets,
             io->d_entry_keys, io->d_entry_order, io->d_entries, io->entry_capacity, l.entry_sort.hist,
-            l.entry_sort.ctl, l.entry_sort.flags, io->d_worlds);
+            l.entry_sort.ctl, l.entry_sort.flags, io->d_worlds, l.first_slot);
     }
     DYN_LAUNCHED("bp_emit_kernel");
     SortDigit digits[kSortMaxPasses];
@@ -959,17 +1026,22 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
         ProfileScope scope("bp_gather_kernel", st);
         bp_gather_kernel<<<min(entry_blocks, 2048u), kBpThreads, 0, st>>>(io->d_entry_order, io->d_entries,
                                                               l.entry_sort.ctl, l.dir, counters,
-                                                              l.level_keys, l.sorted);
+                                                              l.level_keys, l.sorted, n, l.link_mode);
     }
     DYN_LAUNCHED("bp_gather_kernel");
     const EntryRecords rec{l.sorted, io->d_entries, io->d_entry_order, l.level_keys, false};
-    const uint32_t link_blocks = ceil_div(n, kLinkColliders);
+    const uint32_t thread_groups = link_chunks(n, 1u);
+    const uint32_t lane_groups = link_chunks(n, kLinkLanes);
     {
         ProfileScope scope("bp_links_count_kernel", st);
-        bp_links_count_kernel<<<link_blocks, kBpThreads, 0, st>>>(
-            io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
+        bp_links_count_kernel<1><<<thread_groups, kBpThreads, 0, st>>>(
+            io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.first_slot,
             io->d_entry_keys, rec, l.dir, counters,
-            io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage);
+            io->entry_capacity, io->d_worlds, l.link_mode, l.link_counts, l.link_blocks, l.link_stage);
+        bp_links_count_kernel<kLinkLanes><<<lane_groups, kBpThreads, 0, st>>>(
+            io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.first_slot,
+            io->d_entry_keys, rec, l.dir, counters,
+            io->entry_capacity, io->d_worlds, l.link_mode, l.link_counts, l.link_blocks, l.link_stage);
     }
     DYN_LAUNCHED("bp_links_count_kernel");
     {
@@ -981,17 +1053,22 @@ int dyn_broadphase(void* stream, const dyn_broadphase_io* io) {
     DYN_LAUNCHED("bp_mates_count_kernel");
     {
         ProfileScope scope("bp_pair_offsets_kernel", st);
-        bp_pair_offsets_kernel<<<1, 1024, 0, st>>>(l.link_blocks, link_blocks, l.mate_blocks, entry_blocks,
+        bp_pair_offsets_kernel<<<1, 1024, 0, st>>>(l.link_blocks, n, l.link_mode, l.mate_blocks, entry_blocks,
                                                    counters, io->pair_capacity);
     }
     DYN_LAUNCHED("bp_pair_offsets_kernel");
     if (io->pair_capacity) {
         {
             ProfileScope scope("bp_links_emit_kernel", st);
-            bp_links_emit_kernel<<<link_blocks, kBpThreads, 0, st>>>(
-                io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.block_offsets,
+            bp_links_emit_kernel<1><<<thread_groups, kBpThreads, 0, st>>>(
+                io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.first_slot,
                 io->d_entry_keys, rec, l.dir, counters,
-                io->entry_capacity, io->d_worlds, l.link_counts, l.link_blocks, l.link_stage,
+                io->entry_capacity, io->d_worlds, l.link_mode, l.link_counts, l.link_blocks, l.link_stage,
+                io->d_pair_major, io->d_pair_minor, io->pair_capacity);
+            bp_links_emit_kernel<kLinkLanes><<<lane_groups, kBpThreads, 0, st>>>(
+                io->d_aabbs, io->d_groups, io->d_flags, n, l.cnt_level, l.first_slot,
+                io->d_entry_keys, rec, l.dir, counters,
+                io->entry_capacity, io->d_worlds, l.link_mode, l.link_counts, l.link_blocks, l.link_stage,
                 io->d_pair_major, io->d_pair_minor, io->pair_capacity);
         }
         DYN_LAUNCHED("bp_links_emit_kernel");
