@@ -1086,6 +1086,52 @@ __device__ __forceinline__ uint32_t resolve_row(const W& w, uint32_t id, uint32_
     return row;
 }
 
+// The first index of the (a, b)-sorted keys[0, count) whose key is not below (a, b): bracketed by
+// steps of 1, 2, 4, ... out from `hint`, then bisected, which returns the same index as a bisection
+// of the whole range in a few probes when the key sits near the hint. The archive and the current
+// contacts are both in canonical pair order and mostly hold the same pairs, so an archived
+// contact's match sits near its own scaled position.
+__device__ __forceinline__ uint32_t lower_bound_near(const uint2* __restrict__ keys, uint32_t count, uint32_t hint,
+                                                     uint32_t a, uint32_t b) {
+    if (count == 0u) return 0u;
+    const auto below = [&](uint32_t i) {
+        const uint2 m = keys[i];
+        return m.x < a || (m.x == a && m.y < b);
+    };
+    if (hint >= count) hint = count - 1u;
+    uint32_t lo, hi;  // the answer lies in [lo, hi]
+    if (below(hint)) {
+        lo = hint + 1u;
+        hi = count;
+        for (uint32_t step = 1u;; step <<= 1) {
+            if (step >= count - hint) break;  // hint + step would pass the end
+            const uint32_t probe = hint + step;
+            if (!below(probe)) {
+                hi = probe;
+                break;
+            }
+            lo = probe + 1u;
+        }
+    } else {
+        lo = 0u;
+        hi = hint;
+        for (uint32_t step = 1u;; step <<= 1) {
+            if (step > hint) break;
+            const uint32_t probe = hint - step;
+            if (below(probe)) {
+                lo = probe + 1u;
+                break;
+            }
+            hi = probe;
+        }
+    }
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (below(mid)) lo = mid + 1u; else hi = mid;
+    }
+    return lo;
+}
+
 // contact_relay.wgsl:34-71 (events are not modelled: no collider here requests them)
 __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1096,13 +1142,10 @@ __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     if (resolve_row(w, held.first_body_id, held.first_generation) == DYN_NO_BODY ||
         resolve_row(w, held.second_body_id, held.second_generation) == DYN_NO_BODY) return;
     const uint32_t count = *w.ncontacts;
-    uint32_t lo = 0, hi = count;
-    // lower bound of (a, b) over the dense keys (8 bytes a contact, not its 352-byte record)
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) / 2u;
-        const uint2 m = w.ckeys[mid];
-        if (m.x < held.a || (m.x == held.a && m.y < held.b)) lo = mid + 1u; else hi = mid;
-    }
+    // lower bound of (a, b) over the dense keys (8 bytes a contact, not its 352-byte record),
+    // galloping out from where the archived contact would sit in the current list
+    const uint32_t hint = static_cast<uint32_t>(static_cast<uint64_t>(index) * count / max(*w.narchive, 1u));
+    const uint32_t lo = lower_bound_near(w.ckeys, count, hint, held.a, held.b);
     if (lo >= count) return;
     const uint2 at = w.ckeys[lo];
     if (at.x != held.a || at.y != held.b) return;
