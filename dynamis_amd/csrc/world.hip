@@ -109,7 +109,8 @@ struct W {
     uint32_t* blocks;
     float* targets;
     float4* prows;  // per contact block: the position phase's three float4 (k_blocks)
-    float4* srows;    // per contact block, per substep: kSolveRow float4 for the velocity iterations
+    float4* srows;    // per contact block, per substep: kSolveRow float4 for the velocity iterations,
+                      // section-major (solve_section)
     float4* sbodies;  // per row, per substep: orientation, frozen inverse inertia and mass, split count, velocities
     float4* simp;     // per contact block, for the step's iterations: accumulated normal, tangent 1, tangent 2
     float* resolution;
@@ -1653,6 +1654,14 @@ __device__ __forceinline__ float normal_delta(float relaxation, float damping_ra
 // iteration, and k_solve reads them back, so the iterations produce the same bits.
 
 constexpr uint32_t kSolveRow = 15;    // float4 per block: 3 of header, 3 per manifold point
+
+// A block's solver row in sections of 3 float4: the header (section 0) and each manifold point's
+// (section 1 + point), laid out section-major over the contact capacity, so the lanes of a wave
+// read one section of neighbouring blocks as one contiguous run and the sections of points a
+// block does not have are never read.
+__device__ __forceinline__ float4* solve_section(const W& w, uint32_t ci, uint32_t section) {
+    return w.srows + (static_cast<size_t>(section) * w.contact_cap + ci) * 3u;
+}
 constexpr uint32_t kSolverBody = 5;   // float4 per row: q, inertia 0-3, inertia 4-5 mass count, v, w
 constexpr uint32_t kRowActive = 1u << 5;
 
@@ -1709,7 +1718,7 @@ __global__ void __launch_bounds__(kThreads) k_solver_bodies(W w) {
 // each point's arms from the centres of mass, its row masses, target speed and depth.
 __device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t load_impulses) {
     if (ci >= *w.ncontacts) return;
-    float4* row = w.srows + static_cast<size_t>(kSolveRow) * ci;
+    float4* row = solve_section(w, ci, 0u);
     const dyn_contact& contact = w.contacts[ci];
     const uint32_t point_count = contact.point_count;
     const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
@@ -1779,7 +1788,7 @@ __device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t 
         const float truth = momentum_mass(pf, ps, aa, ab, normal);
         const float m1 = momentum_mass(split_f, split_s, aa, ab, t1) * count;
         const float m2 = momentum_mass(split_f, split_s, aa, ab, t2) * count;
-        float4* q = row + 3u + 3u * pi;
+        float4* q = solve_section(w, ci, 1u + pi);
         q[0] = make_float4(ra.x, ra.y, ra.z, nmass);
         q[1] = make_float4(rb.x, rb.y, rb.z, truth);
         q[2] = make_float4(m1, m2, w.targets[ci * 4u + pi], p.depth);
@@ -1795,7 +1804,7 @@ __global__ void __launch_bounds__(kThreads) k_solve_prep(W w, uint32_t load_impu
 // adds the same impulses in the same order as applying them all after the loop).
 __device__ __forceinline__ void solve_at(const W& w, uint32_t ci, uint32_t residual) {
     if (ci >= *w.ncontacts) return;
-    const float4* row = w.srows + static_cast<size_t>(kSolveRow) * ci;
+    const float4* row = solve_section(w, ci, 0u);
     const float4 h0 = row[0];
     const uint32_t flags = __float_as_uint(h0.z);
     if (!(flags & kRowActive)) return;  // not resolved, or both bodies inert
@@ -1828,7 +1837,7 @@ __device__ __forceinline__ void solve_at(const W& w, uint32_t ci, uint32_t resid
 #pragma unroll
     for (uint32_t pi = 0; pi < 4; ++pi) {
         if (pi >= point_count) continue;
-        const float4* q = row + 3u + 3u * pi;
+        const float4* q = solve_section(w, ci, 1u + pi);
         const float4 qa = q[0], qb = q[1], qc = q[2];
         const v3 ra = mk(qa.x, qa.y, qa.z), rb = mk(qb.x, qb.y, qb.z);
         const float nmass = qa.w, truth = qb.w, m1 = qc.x, m2 = qc.y, target = qc.z, depth = qc.w;
