@@ -1250,27 +1250,33 @@ __device__ inline float feature_radius(const Feature& f) {
     return radius;
 }
 
-// convex.wgsl:910-982: clip a feature against the side planes of a reference ring
+// convex.wgsl:910-982: clip a feature against the side planes of a reference ring. Each side
+// writes the points it keeps into the other of two buffers (the caller's and a local one), so no
+// side copies its output back; the caller's buffer holds the result on return.
 __device__ inline uint32_t clip_feature_to_ring(const Feature& inc, const Feature& ref, v3 ref_dir, v3* held,
                                                 uint32_t* held_ids) {
-    v3 kept[kCandidates];
-    uint32_t kept_ids[kCandidates];
+    v3 spare[kCandidates];
+    uint32_t spare_ids[kCandidates];
     for (uint32_t i = 0; i < inc.count; ++i) {
         held[i] = inc.p[i];
         held_ids[i] = inc.id[i];
     }
+    v3* from = held;
+    uint32_t* from_ids = held_ids;
+    v3* kept = spare;
+    uint32_t* kept_ids = spare_ids;
     uint32_t live = inc.count;
     for (uint32_t side = 0; side < ref.count; ++side) {
         const v3 current = ref.p[side], next = ref.p[(side + 1u) % ref.count];
         const v3 plane_normal = norm3(cross3(sub(next, current), ref_dir));
         uint32_t written = 0;
         if (live == 2u) {
-            const v3 p0 = held[0], p1 = held[1];
+            const v3 p0 = from[0], p1 = from[1];
             const float s0 = dot3(sub(p0, current), plane_normal) - kClipMargin;
             const float s1 = dot3(sub(p1, current), plane_normal) - kClipMargin;
             if (s0 <= 0.0f) {
                 kept[written] = p0;
-                kept_ids[written] = held_ids[0];
+                kept_ids[written] = from_ids[0];
                 ++written;
             }
             if (s0 * s1 < 0.0f && written < kCandidates) {
@@ -1280,17 +1286,17 @@ __device__ inline uint32_t clip_feature_to_ring(const Feature& inc, const Featur
             }
             if (s1 <= 0.0f && written < kCandidates) {
                 kept[written] = p1;
-                kept_ids[written] = held_ids[1];
+                kept_ids[written] = from_ids[1];
                 ++written;
             }
         } else {
             for (uint32_t i = 0; i < live; ++i) {
-                const v3 cp = held[i], np = held[(i + 1u) % live];
+                const v3 cp = from[i], np = from[(i + 1u) % live];
                 const float cs = dot3(sub(cp, current), plane_normal) - kClipMargin;
                 const float ns = dot3(sub(np, current), plane_normal) - kClipMargin;
                 if (cs <= 0.0f && written < kCandidates) {
                     kept[written] = cp;
-                    kept_ids[written] = held_ids[i];
+                    kept_ids[written] = from_ids[i];
                     ++written;
                 }
                 if (cs * ns < 0.0f && written < kCandidates) {
@@ -1301,11 +1307,19 @@ __device__ inline uint32_t clip_feature_to_ring(const Feature& inc, const Featur
             }
         }
         live = written;
-        for (uint32_t i = 0; i < written; ++i) {
-            held[i] = kept[i];
-            held_ids[i] = kept_ids[i];
-        }
+        v3* const t = from;
+        uint32_t* const t_ids = from_ids;
+        from = kept;
+        from_ids = kept_ids;
+        kept = t;
+        kept_ids = t_ids;
         if (live == 0u) break;
+    }
+    if (from != held) {
+        for (uint32_t i = 0; i < live; ++i) {
+            held[i] = from[i];
+            held_ids[i] = from_ids[i];
+        }
     }
     return live;
 }

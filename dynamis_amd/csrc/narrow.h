@@ -199,7 +199,10 @@ __device__ __forceinline__ void keep(Manifold& m, Candidates& c) {
         if (d > held || (d == held && c.feature[i] < c.feature[lead])) lead = i;
     }
     if (n > 0) swap_candidates(c, 0, lead);
-    for (uint32_t kept = 1; kept < n; ++kept) {
+    // Round k only reorders places k and above, and at most four places are pushed, so the
+    // rounds past the fourth place change nothing that is read.
+    const uint32_t placed = min(n, static_cast<uint32_t>(DYN_CONTACT_MAX_POINTS));
+    for (uint32_t kept = 1; kept < placed; ++kept) {
         uint32_t best = kept;
         float best_gap = gap_of(c, kept, kept);
         for (uint32_t i = kept + 1; i < n; ++i) {

@@ -112,6 +112,7 @@ struct W {
     float4* srows;    // per contact block, per substep: kSolveRow float4 for the velocity iterations,
                       // section-major (solve_section)
     float4* sbodies;  // per row, per substep: orientation, frozen inverse inertia and mass, split count, velocities
+    float4* sposes;   // per row, per substep: position and inert flag, local centre of mass
     float4* simp;     // per contact block, for the step's iterations: accumulated normal, tangent 1, tangent 2
     float* resolution;
     // Solver deltas without atomics: every block writes its fixed-point words to its own slots
@@ -1602,37 +1603,6 @@ __device__ __forceinline__ void commit_block(const W& w, uint32_t ci, bool fa_in
               solver_word(fmx(len3(sa), len3(sbv)), DYN_SOLVER_VELOCITY_SCALE));
 }
 
-// solver_contact_block.wgsl:195-233
-__device__ __forceinline__ void warm_at(const W& w, uint32_t ci) {
-    if (ci >= *w.ncontacts) return;
-    const dyn_contact& contact = w.contacts[ci];
-    if (!block_resolves(contact)) return;
-    const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
-    const Body fl = load_body(w, fb), sl = load_body(w, sb);
-    Body first = fl, second = sl;
-    const bool fi = body_is_inert(fl), si = body_is_inert(sl);
-    if (fi) body_frozen(first);
-    if (si) body_frozen(second);
-    const v3 normal = ld3(contact.normal);
-    v3 t1, t2;
-    make_tangents(normal, &t1, &t2);
-    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
-        const dyn_manifold_point& p = contact.points[pi];
-        const v3 aa = arm(first, p, true), ab = arm(second, p, false);
-        const v3 impulse = add(add(mul(normal, p.accumulated_normal), mul(t1, p.accumulated_tangent_1)),
-                               mul(t2, p.accumulated_tangent_2));
-        apply_pair_impulse(first, second, aa, ab, impulse);
-    }
-    commit_block(w, ci, fi, si, sub(ld3(first.s.velocity), ld3(fl.s.velocity)),
-                 sub(ld3(first.s.angular_velocity), ld3(fl.s.angular_velocity)),
-                 sub(ld3(second.s.velocity), ld3(sl.s.velocity)),
-                 sub(ld3(second.s.angular_velocity), ld3(sl.s.angular_velocity)), false);
-}
-
-__global__ void __launch_bounds__(kThreads) k_warm(W w) {
-    warm_at(w, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
 __device__ __forceinline__ float normal_delta(float relaxation, float damping_ratio, float mass, float truth,
                                               float step, float goal, float depth, float speed,
                                               float accumulated) {
@@ -1699,7 +1669,8 @@ __device__ __forceinline__ SolverBody load_solver_body(const W& w, uint32_t row)
 __device__ __forceinline__ void solver_bodies_at(const W& w, uint32_t row) {
     if (row >= w.n || w.block_counts[row] == 0u) return;
     Body b = load_body(w, row);
-    if (body_is_inert(b)) body_frozen(b);
+    const bool inert = body_is_inert(b);
+    if (inert) body_frozen(b);
     float4* p = w.sbodies + kSolverBody * row;
     p[0] = ld4(b.s.orientation);
     p[1] = make_float4(b.d.inverse_inertia[0], b.d.inverse_inertia[1], b.d.inverse_inertia[2],
@@ -1708,10 +1679,74 @@ __device__ __forceinline__ void solver_bodies_at(const W& w, uint32_t row) {
                        static_cast<float>(max(w.block_counts[row], 1u)));
     p[3] = make_float4(b.s.velocity[0], b.s.velocity[1], b.s.velocity[2], 0.0f);
     p[4] = make_float4(b.s.angular_velocity[0], b.s.angular_velocity[1], b.s.angular_velocity[2], 0.0f);
+    float4* o = w.sposes + 2u * row;
+    o[0] = make_float4(b.s.position[0], b.s.position[1], b.s.position[2], inert ? 1.0f : 0.0f);
+    o[1] = make_float4(b.d.com[0], b.d.com[1], b.d.com[2], 0.0f);
+}
+
+// A claimed row's body as the substep's blocks see it, from the row's two copies above: pose,
+// local centre of mass, velocities, and inverse mass and inertia (zero when inert). The body
+// records' other fields are not read by the per-block passes and stay zero.
+__device__ __forceinline__ Body posed_body(const W& w, uint32_t row, bool* inert) {
+    const float4* p = w.sbodies + kSolverBody * row;
+    const float4* o = w.sposes + 2u * row;
+    const float4 q = p[0], i0 = p[1], i1 = p[2], v = p[3], av = p[4], at = o[0], com = o[1];
+    Body b{};
+    st4(b.s.orientation, q);
+    b.s.position[0] = at.x;
+    b.s.position[1] = at.y;
+    b.s.position[2] = at.z;
+    b.s.velocity[0] = v.x;
+    b.s.velocity[1] = v.y;
+    b.s.velocity[2] = v.z;
+    b.s.angular_velocity[0] = av.x;
+    b.s.angular_velocity[1] = av.y;
+    b.s.angular_velocity[2] = av.z;
+    b.d.com[0] = com.x;
+    b.d.com[1] = com.y;
+    b.d.com[2] = com.z;
+    b.d.inverse_inertia[0] = i0.x;
+    b.d.inverse_inertia[1] = i0.y;
+    b.d.inverse_inertia[2] = i0.z;
+    b.d.inverse_inertia[3] = i0.w;
+    b.d.inverse_inertia[4] = i1.x;
+    b.d.inverse_inertia[5] = i1.y;
+    b.d.inverse_mass = i1.z;
+    *inert = at.w != 0.0f;
+    return b;
 }
 
 __global__ void __launch_bounds__(kThreads) k_solver_bodies(W w) {
     solver_bodies_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// solver_contact_block.wgsl:195-233
+__device__ __forceinline__ void warm_at(const W& w, uint32_t ci) {
+    if (ci >= *w.ncontacts) return;
+    const dyn_contact& contact = w.contacts[ci];
+    if (!block_resolves(contact)) return;
+    const uint32_t fb = w.blocks[2 * ci], sb = w.blocks[2 * ci + 1];
+    bool fi, si;
+    const Body fl = posed_body(w, fb, &fi), sl = posed_body(w, sb, &si);  // inert ones frozen
+    Body first = fl, second = sl;
+    const v3 normal = ld3(contact.normal);
+    v3 t1, t2;
+    make_tangents(normal, &t1, &t2);
+    for (uint32_t pi = 0; pi < contact.point_count; ++pi) {
+        const dyn_manifold_point& p = contact.points[pi];
+        const v3 aa = arm(first, p, true), ab = arm(second, p, false);
+        const v3 impulse = add(add(mul(normal, p.accumulated_normal), mul(t1, p.accumulated_tangent_1)),
+                               mul(t2, p.accumulated_tangent_2));
+        apply_pair_impulse(first, second, aa, ab, impulse);
+    }
+    commit_block(w, ci, fi, si, sub(ld3(first.s.velocity), ld3(fl.s.velocity)),
+                 sub(ld3(first.s.angular_velocity), ld3(fl.s.angular_velocity)),
+                 sub(ld3(second.s.velocity), ld3(sl.s.velocity)),
+                 sub(ld3(second.s.angular_velocity), ld3(sl.s.angular_velocity)), false);
+}
+
+__global__ void __launch_bounds__(kThreads) k_warm(W w) {
+    warm_at(w, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // solver_contact_block.wgsl:31-110, the parts of a block that depend on poses and masses only:
@@ -1741,24 +1776,16 @@ __device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t 
         imp[1] = make_float4(a1[0], a1[1], a1[2], a1[3]);
         imp[2] = make_float4(a2[0], a2[1], a2[2], a2[3]);
     }
-    Body pf = load_body(w, fb), ps = load_body(w, sb);
+    // the split bodies scale the frozen copies: an inert body's zeros stay zero
+    bool fi, si;
+    const Body pf = posed_body(w, fb, &fi), ps = posed_body(w, sb, &si);
     Body split_f = pf, split_s = ps;
-    const float c1 = static_cast<float>(max(w.block_counts[fb], 1u));
-    const float c2 = static_cast<float>(max(w.block_counts[sb], 1u));
+    const float c1 = w.sbodies[kSolverBody * fb + 2u].w, c2 = w.sbodies[kSolverBody * sb + 2u].w;
     split_f.d.inverse_mass = split_f.d.inverse_mass * c1;
     split_s.d.inverse_mass = split_s.d.inverse_mass * c2;
     for (int k = 0; k < 6; ++k) {
         split_f.d.inverse_inertia[k] = split_f.d.inverse_inertia[k] * c1;
         split_s.d.inverse_inertia[k] = split_s.d.inverse_inertia[k] * c2;
-    }
-    const bool fi = body_is_inert(pf), si = body_is_inert(ps);
-    if (fi) {
-        body_frozen(pf);
-        body_frozen(split_f);
-    }
-    if (si) {
-        body_frozen(ps);
-        body_frozen(split_s);
     }
     const uint32_t flags = point_count | (fi ? 8u : 0u) | (si ? 16u : 0u) | ((fi && si) ? 0u : kRowActive);
     const v3 normal = ld3(contact.normal);
@@ -2957,6 +2984,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.istate, body_count);
     A(w.block_counts, body_count);
     A(w.sbodies, static_cast<size_t>(kSolverBody) * body_count);
+    A(w.sposes, 2u * static_cast<size_t>(body_count));
     A(w.resolution, 4u * static_cast<size_t>(body_count));
     A(w.list_counts, body_count);
     A(w.row_start, body_count);
@@ -3482,6 +3510,7 @@ SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base)
     w.prows = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
     w.srows = reinterpret_cast<float4*>(take(sizeof(float4) * kSolveRow * c));
     w.sbodies = reinterpret_cast<float4*>(take(sizeof(float4) * kSolverBody * n));
+    w.sposes = reinterpret_cast<float4*>(take(sizeof(float4) * 2u * n));
     w.simp = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
     w.resolution = reinterpret_cast<float*>(take(4u * 4u * n));
     w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
