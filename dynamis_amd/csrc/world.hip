@@ -26,8 +26,16 @@ namespace dyn {
 namespace {
 
 constexpr int kThreads = 256;
+// Row kernels that count into a counter word stride over their rows with at most this many
+// workgroups, each adding its workgroup's count with one atomic: atomics on one word run one
+// after another, and one per wave over 1.5 M rows cost more than the rows' own work.
+constexpr uint32_t kCountBlocks = 1024;
 constexpr uint32_t kIslandWake = 1u;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kFlattenFlags = 8;   // per flatten pass: a word per round (at most 6 rounds)
+constexpr uint32_t kHookRounds = 2;     // hooking rounds (each with its flatten pass) before the link
+constexpr uint32_t kFlattenPasses = kHookRounds + 2;
+constexpr uint32_t kFlattenHops = 63;   // hops a flatten walk takes past a row's parent
 constexpr float kHalfTurn = 3.141592653589793f;
 
 __device__ __forceinline__ uint32_t* cw(uint32_t* counters, uint32_t slot) {
@@ -105,6 +113,8 @@ struct W {
     uint32_t* rest_flags;
     uint32_t* parents;
     uint32_t* istate;
+    uint32_t* flat_flags;  // kFlattenFlags words per flatten pass: set when a round left a row short of its root
+    uint2* iedges;         // per contact: the rows it links in the island pass, or kNoSlot twice (null: read the contacts)
     uint32_t* block_counts;
     uint32_t* blocks;
     float* targets;
@@ -1278,6 +1288,7 @@ __global__ void __launch_bounds__(kThreads) k_tape_record(W w, float* states, fl
 
 __global__ void __launch_bounds__(kThreads) k_island_init(W w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kFlattenPasses * kFlattenFlags) w.flat_flags[i] = 0u;  // launched with at least kThreads lanes
     if (i >= w.d) return;
     w.parents[i] = i;
     w.istate[i] = 0u;
@@ -1323,17 +1334,75 @@ __device__ inline void island_couple(const W& w, uint32_t a, uint32_t b) {
     atomicAdd(cw(w.counters, DYN_COUNTER_ISLAND_FAULTS), 1u);
 }
 
+// The rows a contact links in the island pass (island_link.wgsl:47-60): a touching, non-sensor
+// contact between colliders and bodies inside the records; kNoSlot twice otherwise.
+__device__ __forceinline__ uint2 island_edge(const W& w, uint32_t i, float slop) {
+    const uint2 none = make_uint2(kNoSlot, kNoSlot);
+    const dyn_contact& c = w.contacts[i];
+    if (c.point_count == 0u || c.sensor == 1u || !touches(c, slop)) return none;
+    // colliders or bodies outside the records (possible only for contacts handed to dyn_islands)
+    // link nothing
+    if (c.a >= w.k || c.b >= w.k) return none;
+    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
+    if (fb >= w.n || sb >= w.n) return none;
+    return make_uint2(fb, sb);
+}
+
+__device__ __forceinline__ uint2 island_edge_at(const W& w, uint32_t i, float slop) {
+    return w.iedges ? w.iedges[i] : island_edge(w, i, slop);
+}
+
+// Completes the components over every linking contact with the retrying hooks above; after the
+// seeding and hooking rounds most contacts' rows share a parent and return at once.
 __global__ void __launch_bounds__(kThreads) k_island_link(W w, float slop) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= min(*w.ncontacts, w.contact_cap)) return;
-    const dyn_contact& c = w.contacts[i];
-    if (c.point_count == 0u || c.sensor == 1u || !touches(c, slop)) return;
-    // colliders or bodies outside the records (possible only for contacts handed to dyn_islands)
-    // link nothing
-    if (c.a >= w.k || c.b >= w.k) return;
-    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
-    if (fb >= w.n || sb >= w.n) return;
-    island_couple(w, fb, sb);
+    const uint2 e = island_edge_at(w, i, slop);
+    if (e.x == kNoSlot) return;
+    island_couple(w, e.x, e.y);
+}
+
+// A hooking round over the linking contacts between simulated rows, run on a flattened forest
+// (every parent a root): a contact whose rows' parents differ points the higher parent at the
+// lower one with one atomic minimum and no retries. A hook landing on a row that is no longer a
+// root moves that row to a lower row of its component; a connection it drops is restored by
+// k_island_link, which visits every contact. Parents still point only at lower rows of the same
+// component.
+// For the first kMergedKeys distinct `key`s of the wave's lanes with `take`, one atomic minimum of
+// the smallest of their `value`s on words[key]; the lanes with other keys send their own. The
+// words end as with one atomic minimum per lane. Every lane of the wave calls it.
+constexpr uint32_t kMergedKeys = 8;
+__device__ inline void wave_min_per_key(uint32_t* words, bool take, uint32_t key, uint32_t value) {
+    uint64_t pending = __ballot(take);
+    for (uint32_t merged = 0; pending != 0u && merged < kMergedKeys; ++merged) {
+        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(pending))) - 1u;
+        const uint32_t target = __shfl(key, leader, kWave);
+        const bool mine = take && key == target;
+        uint32_t least = mine ? value : 0xFFFFFFFFu;
+#pragma unroll
+        for (int offset = kWave / 2; offset > 0; offset >>= 1) least = min(least, __shfl_xor(least, offset, kWave));
+        if (lane_id() == leader) atomicMin(&words[target], least);
+        pending &= ~__ballot(mine);
+    }
+    if ((pending >> lane_id()) & 1u) atomicMin(&words[key], value);
+}
+
+__global__ void __launch_bounds__(kThreads) k_island_hook(W w, float slop) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    uint32_t hi = 0, lo = 0;
+    if (i < min(*w.ncontacts, w.contact_cap)) {
+        const uint2 e = island_edge_at(w, i, slop);
+        if (e.x < w.d && e.y < w.d) {
+            const uint32_t ra = atomic_load_agent(&w.parents[e.x]), rb = atomic_load_agent(&w.parents[e.y]);
+            take = ra != rb;
+            hi = max(ra, rb);
+            lo = min(ra, rb);
+        }
+    }
+    // a contact's neighbours in the list mostly join the same rows, so the wave's hooks on one
+    // root go out as one atomic
+    wave_min_per_key(w.parents, take, hi, lo);
 }
 
 // dyn_solve's set-up: the inert rows' contribution marks at zero (a world keeps them zero
@@ -1360,20 +1429,24 @@ __global__ void __launch_bounds__(kThreads) k_solve_stage_check(W w, uint32_t* w
     if (c.a >= w.k || c.b >= w.k || w.owner[c.a] >= w.n || w.owner[c.b] >= w.n) atomicOr(&words[1], 1u);
 }
 
-// Before the hooking pass: every linking contact between two simulated rows points the higher
-// row's parent at the lower row (an atomic minimum, so each row ends at its lowest linked
-// neighbour below it, or itself). Parents still point only at lower rows of the same
-// component, so k_island_link's hooks and the flatten rounds end at the same minimum-row roots;
-// most rows then start a few hops from their root, which spares the hooks their long finds on a
-// pile that is one island.
+// The island pass's first kernel: each contact's linking rows go to iedges (when present), and
+// every linking contact between two simulated rows points the higher row's parent at the lower
+// row (an atomic minimum, so each row ends at its lowest linked neighbour below it, or itself).
+// Parents still point only at lower rows of the same component, so the hooks and the flatten
+// rounds end at the same minimum-row roots; most rows then start a few hops from their root.
 __global__ void __launch_bounds__(kThreads) k_island_seed(W w, float slop) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= min(*w.ncontacts, w.contact_cap)) return;
-    const dyn_contact& c = w.contacts[i];
-    if (c.point_count == 0u || c.sensor == 1u || c.a >= w.k || c.b >= w.k) return;
-    const uint32_t fb = w.owner[c.a], sb = w.owner[c.b];
-    if (fb >= w.d || sb >= w.d || fb == sb || !touches(c, slop)) return;
-    atomicMin(&w.parents[max(fb, sb)], min(fb, sb));
+    bool take = false;
+    uint32_t hi = 0, lo = 0;
+    if (i < min(*w.ncontacts, w.contact_cap)) {
+        const uint2 e = island_edge(w, i, slop);
+        if (w.iedges) w.iedges[i] = e;
+        take = e.x < w.d && e.y < w.d && e.x != e.y;
+        hi = max(e.x, e.y);
+        lo = min(e.x, e.y);
+    }
+    // a row's contacts sit together in the list but mostly as its lower side: one atomic a lane
+    if (take) atomicMin(&w.parents[hi], lo);
 }
 
 // dyn_islands' set-up: no wake or motion flags, the island counters at zero.
@@ -1397,90 +1470,129 @@ __global__ void __launch_bounds__(kThreads) k_island_link_resting(W w) {
     island_couple(w, r1, r2);
 }
 
-// island_flatten.wgsl
-__global__ void __launch_bounds__(kThreads) k_island_flatten(W w) {
+// island_flatten.wgsl, with walks of up to kFlattenHops hops in place of one: round `round` of a
+// flatten pass whose per-round flags are `flags`. Each row walks up from its parent to its root
+// or kFlattenHops hops further and points there, so a round divides every depth by at least
+// kFlattenHops + 1 (rows moved up by other rows in the same round only shorten the walks), and
+// flatten_rounds(d) rounds leave every row pointing at its root. No hooks run during a pass, so
+// a round in which every walk reached its root leaves the forest flat: the rounds after it
+// return at once.
+__global__ void __launch_bounds__(kThreads) k_island_flatten(W w, uint32_t* flags, uint32_t round) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) flags[round + 1u] = 0u;  // read by round + 2; no round writes it before round + 1
+    if (round > 0u && flags[round - 1u] == 0u) return;
     if (i >= w.d) return;
     const uint32_t parent = atomic_load_agent(&w.parents[i]);
-    const uint32_t gp = atomic_load_agent(&w.parents[parent]);
-    atomic_store_agent(&w.parents[i], min(parent, gp));
+    uint32_t at = parent;
+    uint32_t up = atomic_load_agent(&w.parents[at]);
+    for (uint32_t hop = 0; up != at && hop < kFlattenHops; ++hop) {
+        at = up;
+        up = atomic_load_agent(&w.parents[at]);
+    }
+    if (at != parent) atomic_store_agent(&w.parents[i], at);
+    if (up != at) flags[round] = 1u;  // stopped short of the root
 }
 
 // All the flatten rounds of a small world in one workgroup, a barrier after each round (within a
 // round, as across the launched rounds, a row may read a parent another row has already moved
-// up, which only shortens its path).
+// up, which only shortens its walk); it stops after the first round whose walks all reached
+// their roots.
 constexpr uint32_t kFlattenGroupRows = 8192;
-
 __global__ void __launch_bounds__(1024) k_island_flatten_rounds(W w, uint32_t rounds) {
     for (uint32_t r = 0; r < rounds; ++r) {
+        int short_of_root = 0;
         for (uint32_t i = threadIdx.x; i < w.d; i += 1024u) {
             const uint32_t parent = atomic_load_agent(&w.parents[i]);
-            const uint32_t gp = atomic_load_agent(&w.parents[parent]);
-            atomic_store_agent(&w.parents[i], min(parent, gp));
+            uint32_t at = parent;
+            uint32_t up = atomic_load_agent(&w.parents[at]);
+            for (uint32_t hop = 0; up != at && hop < kFlattenHops; ++hop) {
+                at = up;
+                up = atomic_load_agent(&w.parents[at]);
+            }
+            if (at != parent) atomic_store_agent(&w.parents[i], at);
+            if (up != at) short_of_root = 1;
         }
-        __syncthreads();
+        if (!__syncthreads_or(short_of_root)) break;
     }
+}
+
+// v summed over the workgroup (kThreads threads, every one of them calling), added to *counter
+// by thread 0 with one atomic when it is not zero.
+__device__ inline void block_count(uint32_t* counter, uint32_t v) {
+    __shared__ uint32_t partial[kThreads / kWave];
+    v = wave_sum(v);
+    if (lane_id() == 0) partial[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kThreads / kWave; ++k) total += partial[k];
+        if (total) atomicAdd(counter, total);
+    }
+    __syncthreads();  // the array is reused by the next call
 }
 
 // island_aggregate.wgsl + island_count.wgsl
 __global__ void __launch_bounds__(kThreads) k_island_aggregate(W w) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t roots = 0;
-    uint32_t root = kNoSlot;
-    bool raise = false;
-    if (i < w.d) {
-        root = w.parents[i];
-        raise = w.wake[i] != 0u || w.motion[i] != 0u;
-        roots = root == i ? 1u : 0u;
-    }
-    // One island holds many neighbouring rows: one lane per distinct root of the wave raises
-    // its flag, and not at all once the flag is there.
-    uint64_t pending = __ballot(raise);
-    while (pending != 0u) {
-        const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(pending))) - 1u;
-        const uint32_t target = __shfl(root, leader, kWave);
-        const uint64_t same = __ballot(raise && root == target);
-        if (lane_id() == leader && !(atomic_load_agent(&w.istate[target]) & kIslandWake)) {
-            atomicMax(&w.istate[target], kIslandWake);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < w.d; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t root = kNoSlot;
+        bool raise = false;
+        if (i < w.d) {
+            root = w.parents[i];
+            raise = w.wake[i] != 0u || w.motion[i] != 0u;
+            roots += root == i ? 1u : 0u;
         }
-        pending &= ~same;
+        // One island holds many neighbouring rows: one lane per distinct root of the wave raises
+        // its flag, and not at all once the flag is there.
+        // (a batch's small worlds put many roots in one wave: past kMergedKeys each lane raises
+        // its own)
+        uint64_t pending = __ballot(raise);
+        for (uint32_t merged = 0; pending != 0u && merged < kMergedKeys; ++merged) {
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(pending))) - 1u;
+            const uint32_t target = __shfl(root, leader, kWave);
+            const uint64_t same = __ballot(raise && root == target);
+            if (lane_id() == leader && !(atomic_load_agent(&w.istate[target]) & kIslandWake)) {
+                atomicMax(&w.istate[target], kIslandWake);
+            }
+            pending &= ~same;
+        }
+        if (((pending >> lane_id()) & 1u) && !(atomic_load_agent(&w.istate[root]) & kIslandWake)) {
+            atomicMax(&w.istate[root], kIslandWake);
+        }
     }
-    roots = wave_sum(roots);
-    if (lane_id() == 0 && roots) atomicAdd(cw(w.counters, DYN_COUNTER_ISLANDS), roots);
+    block_count(cw(w.counters, DYN_COUNTER_ISLANDS), roots);
 }
 
 // island_wake.wgsl
 __global__ void __launch_bounds__(kThreads) k_island_wake(W w) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t woke = 0;
-    if (i < w.d) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.d; i += gridDim.x * blockDim.x) {
         const uint32_t root = w.parents[i];
         if (w.st[i].sleeping != 0u && (w.istate[root] & kIslandWake)) {
-            woke = 1u;
+            ++woke;
             w.st[i].sleeping = 0u;
             w.st[i].sleep_timer = 0.0f;
         }
         w.wake[i] = 0u;
     }
-    woke = wave_sum(woke);
-    if (lane_id() == 0 && woke) {
-        atomicAdd(cw(w.counters, DYN_COUNTER_WOKE), woke);
-        atomicAdd(cw(w.counters, DYN_COUNTER_WOKE_DEFERRED), woke);
-    }
+    block_count(cw(w.counters, DYN_COUNTER_WOKE), woke);
+    block_count(cw(w.counters, DYN_COUNTER_WOKE_DEFERRED), woke);
 }
 
 // live.wgsl count, solver_reset.wgsl
 __global__ void __launch_bounds__(kThreads) k_solver_reset(W w) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *cw(w.counters, DYN_COUNTER_SOLVER_ROWS) = 0u;  // maintained: k_rows_count refills it
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *cw(w.counters, DYN_COUNTER_SOLVER_ROWS) = 0u;  // maintained: k_rows_count refills it
+    }
     uint32_t live = 0;
-    if (i < w.n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.n; i += gridDim.x * blockDim.x) {
         w.block_counts[i] = 0u;
         reinterpret_cast<float4*>(w.resolution)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (i < w.d && body_is_active(w.st[i], w.de[i])) live = 1u;
+        if (i < w.d && body_is_active(w.st[i], w.de[i])) ++live;
     }
-    live = wave_sum(live);
-    if (lane_id() == 0 && live) atomicAdd(cw(w.counters, DYN_COUNTER_LIVE), live);
+    block_count(cw(w.counters, DYN_COUNTER_LIVE), live);
 }
 
 // ---- solver ------------------------------------------------------------------------------------
@@ -1534,14 +1646,14 @@ __global__ void __launch_bounds__(kThreads) k_blocks(W w) {
 }
 
 __global__ void __launch_bounds__(kThreads) k_rows_count(W w) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t claimed = (i < w.n && w.block_counts[i] != 0u) ? 1u : 0u;
-    if (i < w.n) {
-        const bool listed = claimed && !(!kind_simulates(w.de[i]) || w.st[i].sleeping != 0u);
-        w.list_counts[i] = listed ? w.block_counts[i] : 0u;
+    uint32_t claimed = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.n; i += gridDim.x * blockDim.x) {
+        const uint32_t blocks = w.block_counts[i];
+        const bool listed = blocks != 0u && !(!kind_simulates(w.de[i]) || w.st[i].sleeping != 0u);
+        w.list_counts[i] = listed ? blocks : 0u;
+        claimed += blocks != 0u ? 1u : 0u;
     }
-    claimed = wave_sum(claimed);
-    if (lane_id() == 0 && claimed) atomicAdd(cw(w.counters, DYN_COUNTER_SOLVER_ROWS), claimed);
+    block_count(cw(w.counters, DYN_COUNTER_SOLVER_ROWS), claimed);
 }
 
 // The list entry of every block's non-inert sides: the row's first entry plus the block's place.
@@ -2195,36 +2307,30 @@ __global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
 
 // island_sleep.wgsl
 __global__ void __launch_bounds__(kThreads) k_sleep(W w) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t slept = 0;
-    if (i < w.d) {
-        dyn_body_state s = w.st[i];
-        if (s.sleeping == 0u && kind_simulates(w.de[i])) {
-            const uint32_t root = w.parents[i];
-            const bool disturbed = (w.istate[root] & kIslandWake) != 0u || w.wake[i] != 0u;
-            if (disturbed) {
-                s.sleep_timer = 0.0f;
-            } else {
-                s.sleep_timer = s.sleep_timer + w.p.dt;
-                if (s.sleep_timer >= w.p.sleep_time) {
-                    for (int c = 0; c < 3; ++c) {
-                        s.velocity[c] = 0.0f;
-                        s.angular_velocity[c] = 0.0f;
-                        s.force[c] = 0.0f;
-                        s.torque[c] = 0.0f;
-                        s.prev_position[c] = s.position[c];
-                    }
-                    for (int c = 0; c < 4; ++c) s.prev_orientation[c] = s.orientation[c];
-                    s.sleep_timer = 0.0f;
-                    s.sleeping = 1u;
-                    slept = 1u;
-                }
-            }
-            w.st[i] = s;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.d; i += gridDim.x * blockDim.x) {
+        dyn_body_state& s = w.st[i];
+        if (s.sleeping != 0u || !kind_simulates(w.de[i])) continue;
+        const uint32_t root = w.parents[i];
+        const bool disturbed = (w.istate[root] & kIslandWake) != 0u || w.wake[i] != 0u;
+        const float timer = disturbed ? 0.0f : s.sleep_timer + w.p.dt;
+        if (disturbed || !(timer >= w.p.sleep_time)) {  // (a NaN timer stays awake)
+            s.sleep_timer = timer;  // the only field an awake row changes here
+            continue;
         }
+        for (int c = 0; c < 3; ++c) {
+            s.velocity[c] = 0.0f;
+            s.angular_velocity[c] = 0.0f;
+            s.force[c] = 0.0f;
+            s.torque[c] = 0.0f;
+            s.prev_position[c] = s.position[c];
+        }
+        for (int c = 0; c < 4; ++c) s.prev_orientation[c] = s.orientation[c];
+        s.sleep_timer = 0.0f;
+        s.sleeping = 1u;
+        ++slept;
     }
-    slept = wave_sum(slept);
-    if (lane_id() == 0 && slept) atomicAdd(cw(w.counters, DYN_COUNTER_SLEPT), slept);
+    block_count(cw(w.counters, DYN_COUNTER_SLEPT), slept);
 }
 
 // ---- commit: thaw, archive, static wake clear, freeze (commit.rs:187-209) ---------------
@@ -2457,6 +2563,7 @@ int alloc_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_cap
     A(w.adj, 2u * static_cast<size_t>(contact_capacity));
     A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
     A(w.pwords, 2u * static_cast<size_t>(contact_capacity));
+    A(w.iedges, contact_capacity);
     uint32_t scan_limit = pair_capacity > contact_capacity ? pair_capacity : contact_capacity;
     if (body_count > scan_limit) scan_limit = body_count;  // the rows' list offsets
     A(w.scan_blocks, ceil_div(scan_limit ? scan_limit : 1u, kScanBlock) + 1u);
@@ -2542,6 +2649,8 @@ int grow_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capa
 }
 
 uint32_t blocks_for(uint32_t n) { return dyn::ceil_div(n ? n : 1u, dyn::kThreads); }
+// The grid of a row kernel that counts into a counter word (kCountBlocks).
+uint32_t count_blocks(uint32_t n) { return std::min(blocks_for(n), dyn::kCountBlocks); }
 
 int scan(const dyn::W& w, hipStream_t st, const uint32_t* flags, const uint32_t* count, uint32_t limit,
          uint32_t* dest, uint32_t* total, bool counts = false) {
@@ -2651,18 +2760,22 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
 
 int world_step(dyn_world* world, float dt);
 
-// Pointer-jumping rounds that flatten any forest of d rows (each round at least halves every
-// depth): ilog2(max(d, 2)) + 1, as crates/dynamis-rigid/src/lib.rs:79-81 sizes the island pass.
+// Launched flatten rounds that flatten any forest of d rows: a round divides every depth by at
+// least kFlattenHops + 1 = 64, and depths are below d, so the smallest r with 64^r >= d. (The
+// reference sizes its one-hop rounds as ilog2(max(N_d, 2)) + 1, crates/dynamis-rigid/src/lib.rs:
+// 79-81; both end with every row at its root.)
 uint32_t flatten_rounds(uint32_t d) {
-    uint32_t rounds = 0;
-    for (uint32_t v = d < 2u ? 2u : d; v > 1u; v >>= 1) ++rounds;
-    return rounds + 1u;
+    uint32_t rounds = 1;
+    for (uint64_t reach = dyn::kFlattenHops + 1u; reach < d; reach *= dyn::kFlattenHops + 1u) ++rounds;
+    return rounds;
 }
 
-// The flatten rounds: launched one by one, or all in one workgroup for a small world.
-int island_flatten(const dyn::W& w, hipStream_t st) {
+// A flatten pass (`pass` picks its round flags): launched round by round, or all rounds in one
+// workgroup for a small world; either way it ends once a round's walks all reach their roots.
+// With `rounds` 0 it takes flatten_rounds(w.d) rounds and leaves every row at its root.
+int island_flatten(const dyn::W& w, hipStream_t st, uint32_t pass, uint32_t rounds = 0) {
     using namespace dyn;
-    const uint32_t rounds = flatten_rounds(w.d);
+    if (rounds == 0u) rounds = flatten_rounds(w.d);
     if (w.d <= kFlattenGroupRows) {
         {
             ProfileScope scope("k_island_flatten_rounds", st);
@@ -2672,7 +2785,26 @@ int island_flatten(const dyn::W& w, hipStream_t st) {
         return DYN_OK;
     }
     const uint32_t db = blocks_for(w.d);
-    for (uint32_t r = 0; r < rounds; ++r) LAUNCH_ON(st, k_island_flatten, db, w);
+    uint32_t* flags = w.flat_flags + pass * kFlattenFlags;
+    for (uint32_t r = 0; r < rounds; ++r) LAUNCH_ON(st, k_island_flatten, db, w, flags, r);
+    return DYN_OK;
+}
+
+// The contact part of the island pass, before the resting archive's links and the last flatten:
+// the seeding pass (which also lists each contact's linking rows), then kHookRounds hooking
+// rounds, a flatten round (walks of up to 64 hops) before each and after the last, then the
+// retrying hooks over every contact. The rounds only shorten the hooks' work: neither needs a
+// flat forest, and k_island_link alone would end at the same components.
+int island_components(const dyn::W& w, hipStream_t st, float slop) {
+    using namespace dyn;
+    const uint32_t cb = blocks_for(w.contact_cap);
+    LAUNCH_ON(st, k_island_seed, cb, w, slop);
+    DYN_TRY(island_flatten(w, st, 0u, 1u));
+    for (uint32_t h = 0; h < kHookRounds; ++h) {
+        LAUNCH_ON(st, k_island_hook, cb, w, slop);
+        DYN_TRY(island_flatten(w, st, 1u + h, 1u));
+    }
+    LAUNCH_ON(st, k_island_link, cb, w, slop);
     return DYN_OK;
 }
 
@@ -2681,9 +2813,9 @@ int island_flatten(const dyn::W& w, hipStream_t st) {
 int solver_prepare(const dyn::W& w, hipStream_t st) {
     using namespace dyn;
     const uint32_t nb = blocks_for(w.n), cb = blocks_for(w.contact_cap);
-    LAUNCH_ON(st, k_solver_reset, nb, w);
+    LAUNCH_ON(st, k_solver_reset, count_blocks(w.n), w);
     LAUNCH_ON(st, k_blocks, cb, w);
-    LAUNCH_ON(st, k_rows_count, nb, w);
+    LAUNCH_ON(st, k_rows_count, count_blocks(w.n), w);
     DYN_TRY(scan(w, st, w.list_counts, nullptr, w.n, w.row_start, &w.scan_total[3], true));
     LAUNCH_ON(st, k_fill_lists, cb, w);
     return DYN_OK;
@@ -2852,21 +2984,18 @@ int world_step(dyn_world* world, float dt) {
     LAUNCH(k_relay, cb, w);
     LAUNCH(k_contact_begin, cb, w, w.p.slop);
     LAUNCH(k_island_init, db, w);
-    // A batch's islands are bounded by its worlds, whose finds are short already: the seeding
-    // pass pays only in a single world, where one island can span the whole pile.
-    if (!w.worlds) LAUNCH(k_island_seed, cb, w, w.p.slop);
-    LAUNCH(k_island_link, cb, w, w.p.slop);
+    DYN_TRY(island_components(w, st, w.p.slop));
     LAUNCH(k_island_link_resting, cb, w);
-    DYN_TRY(island_flatten(w, st));
-    LAUNCH(k_island_aggregate, db, w);
-    LAUNCH(k_island_wake, db, w);
+    DYN_TRY(island_flatten(w, st, kFlattenPasses - 1u));
+    LAUNCH(k_island_aggregate, count_blocks(w.d), w);
+    LAUNCH(k_island_wake, count_blocks(w.d), w);
     DYN_TRY(solver_prepare(w, st));
     if (world->tape_states) {
         LAUNCH(k_tape_record, blocks_for(std::max(w.n, w.contact_cap)), w, world->tape_states, world->tape_contacts,
                world->tape_count, world->tape_error);
     }
     DYN_TRY(solver_substeps(w, st, world->solver_threads));
-    LAUNCH(k_sleep, db, w);
+    LAUNCH(k_sleep, count_blocks(w.d), w);
     LAUNCH(k_thaw, cb, w);
     LAUNCH(k_archive, cb, w, world->freeze_flags);
     DYN_TRY(scan(world, w.rest_flags, w.nresting, w.contact_cap, world->keep_dest, &w.scan_total[2]));
@@ -2982,6 +3111,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.flags, collider_count);
     A(w.parents, body_count);
     A(w.istate, body_count);
+    A(w.flat_flags, kFlattenPasses * kFlattenFlags);
     A(w.block_counts, body_count);
     A(w.sbodies, static_cast<size_t>(kSolverBody) * body_count);
     A(w.sposes, 2u * static_cast<size_t>(body_count));
@@ -3575,7 +3705,7 @@ int dyn_solve(void* stream, const dyn_solve_io* io, float dt) {
 
 size_t dyn_islands_scratch_bytes(uint32_t bodies) {
     const size_t words = bodies ? bodies : 1u;
-    return 3u * ((4u * words + 255u) & ~static_cast<size_t>(255u));
+    return 3u * ((4u * words + 255u) & ~static_cast<size_t>(255u)) + 4u * dyn::kFlattenPasses * dyn::kFlattenFlags;
 }
 
 int dyn_islands(void* stream, const dyn_islands_io* io) {
@@ -3605,16 +3735,16 @@ int dyn_islands(void* stream, const dyn_islands_io* io) {
     w.wake = reinterpret_cast<uint32_t*>(base);
     w.motion = reinterpret_cast<uint32_t*>(base + part);
     w.istate = reinterpret_cast<uint32_t*>(base + 2u * part);
+    w.flat_flags = reinterpret_cast<uint32_t*>(base + 3u * part);
     const uint32_t nb = blocks_for(w.n), db = blocks_for(w.d), cb = blocks_for(w.contact_cap);
     LAUNCH_ON(st, k_islands_stage_reset, nb, w);
     if (w.d == 0u) return DYN_OK;
     LAUNCH_ON(st, k_island_init, db, w);
     if (w.contact_cap) {
-        LAUNCH_ON(st, k_island_seed, cb, w, io->slop);
-        LAUNCH_ON(st, k_island_link, cb, w, io->slop);
+        DYN_TRY(island_components(w, st, io->slop));
     }
-    DYN_TRY(island_flatten(w, st));
-    LAUNCH_ON(st, k_island_aggregate, db, w);
+    DYN_TRY(island_flatten(w, st, kFlattenPasses - 1u));
+    LAUNCH_ON(st, k_island_aggregate, count_blocks(w.d), w);
     return DYN_OK;
 }
 
