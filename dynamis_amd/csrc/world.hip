@@ -2075,14 +2075,19 @@ __global__ void __launch_bounds__(kThreads) k_impulses_store(W w) {
 // (integer sums: any order gives the same words).
 constexpr uint32_t kApplyBatch = 8;
 
-__device__ __forceinline__ void apply_at(const W& w, uint32_t row) {
+// The velocities live in the row's solver record (k_solver_bodies copies them from the body
+// state at the substep's start) while the iterations run: only the substep's last apply
+// (`commit`) writes them to the body state, for every claimed row, as the same bits an update of
+// both after every apply would leave there.
+__device__ __forceinline__ void apply_at(const W& w, uint32_t row, uint32_t commit) {
     if (row >= w.n) return;
     // list_counts is 0 for rows without blocks and for inert rows, which receive no words
     const uint32_t count = w.list_counts[row];
     const uint32_t start = w.row_start[row];
     if (count == 0u) return;
-    dyn_body_state& s = w.st[row];
-    const v3 v0 = ld3(s.velocity), av0 = ld3(s.angular_velocity);  // in flight with the list
+    float4* p = w.sbodies + kSolverBody * row;  // the solver body's copy
+    const float4 pv = p[3], pw = p[4];           // in flight with the list
+    const v3 v0 = mk(pv.x, pv.y, pv.z), av0 = mk(pw.x, pw.y, pw.z);
     const uint32_t* list = w.adj + start;
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
     for (uint32_t e = 0; e < count; e += kApplyBatch) {
@@ -2111,22 +2116,28 @@ __device__ __forceinline__ void apply_at(const W& w, uint32_t row) {
             s5 += c[j].y;
         }
     }
-    if ((s0 | s1 | s2 | s3 | s4 | s5) == 0u) return;
-    const v3 lin = mk(solver_value(s0, DYN_SOLVER_VELOCITY_SCALE), solver_value(s1, DYN_SOLVER_VELOCITY_SCALE),
-                      solver_value(s2, DYN_SOLVER_VELOCITY_SCALE));
-    const v3 ang = mk(solver_value(s3, DYN_SOLVER_VELOCITY_SCALE), solver_value(s4, DYN_SOLVER_VELOCITY_SCALE),
-                      solver_value(s5, DYN_SOLVER_VELOCITY_SCALE));
-    if (zero3(lin) && zero3(ang)) return;
-    const v3 v = add(v0, lin), av = add(av0, ang);
-    st3(s.velocity, v);
-    st3(s.angular_velocity, av);
-    float4* p = w.sbodies + kSolverBody * row;  // the solver body's copy
-    p[3] = make_float4(v.x, v.y, v.z, 0.0f);
-    p[4] = make_float4(av.x, av.y, av.z, 0.0f);
+    v3 v = v0, av = av0;
+    if ((s0 | s1 | s2 | s3 | s4 | s5) != 0u) {
+        const v3 lin = mk(solver_value(s0, DYN_SOLVER_VELOCITY_SCALE), solver_value(s1, DYN_SOLVER_VELOCITY_SCALE),
+                          solver_value(s2, DYN_SOLVER_VELOCITY_SCALE));
+        const v3 ang = mk(solver_value(s3, DYN_SOLVER_VELOCITY_SCALE), solver_value(s4, DYN_SOLVER_VELOCITY_SCALE),
+                          solver_value(s5, DYN_SOLVER_VELOCITY_SCALE));
+        if (!(zero3(lin) && zero3(ang))) {
+            v = add(v0, lin);
+            av = add(av0, ang);
+            p[3] = make_float4(v.x, v.y, v.z, 0.0f);
+            p[4] = make_float4(av.x, av.y, av.z, 0.0f);
+        }
+    }
+    if (commit) {
+        dyn_body_state& s = w.st[row];
+        st3(s.velocity, v);
+        st3(s.angular_velocity, av);
+    }
 }
 
-__global__ void __launch_bounds__(kThreads) k_apply(W w) {
-    apply_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(kThreads) k_apply(W w, uint32_t commit) {
+    apply_at(w, blockIdx.x * blockDim.x + threadIdx.x, commit);
 }
 
 // substep_integrate.wgsl:15-76 over the awake dynamic rows (no fields)
@@ -2841,16 +2852,17 @@ int solver_substeps(const dyn::W& w, hipStream_t st, uint32_t threads = dyn::kTh
         LAUNCH_TH(st, k_integrate, w.d, threads, w);
         LAUNCH_TH(st, k_solver_bodies, w.n, threads, w);
         LAUNCH_TH(st, k_solve_prep, w.contact_cap, threads, w, sub == 0 ? 1u : 0u);
+        // the substep's last apply writes the velocities back to the body states
         if (sub == 0) {
             LAUNCH_TH(st, k_warm, w.contact_cap, threads, w);
-            LAUNCH_TH(st, k_apply, w.n, threads, w);
+            LAUNCH_TH(st, k_apply, w.n, threads, w, w.p.solve_iterations == 0u ? 1u : 0u);
         }
         for (uint32_t it = 0; it < w.p.solve_iterations; ++it) {
             // solver_block_solve.wgsl:85-87: the residual round is the one where the round
             // counter (reset by integrate, bumped by every apply) equals the iteration count,
             // which only the warm-started first substep reaches.
             LAUNCH_TH(st, k_solve, w.contact_cap, threads, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
-            LAUNCH_TH(st, k_apply, w.n, threads, w);
+            LAUNCH_TH(st, k_apply, w.n, threads, w, it + 1u == w.p.solve_iterations ? 1u : 0u);
         }
         LAUNCH_TH(st, k_advance, w.d, threads, w);
         for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
