@@ -235,6 +235,12 @@ __device__ __forceinline__ void tet_face(uint32_t f, uint32_t* a, uint32_t* b, u
     *c = f == 0u ? 2u : (f == 1u ? 3u : (f == 2u ? 1u : 2u));
 }
 
+// |t[i].w - t[j].w| written from the higher slot to the lower (the same bits either way, as
+// a - b = -(b - a) exactly), so the faces that share an edge share one square root.
+__device__ __forceinline__ float tet_edge(const SimplexPoint* t, uint32_t i, uint32_t j) {
+    return len3(sub(t[i > j ? i : j].w, t[i > j ? j : i].w));
+}
+
 __device__ inline bool tetra_contains(const SimplexPoint* t, v3 o) {
 #pragma unroll
     for (uint32_t f = 0; f < 4; ++f) {
@@ -242,7 +248,7 @@ __device__ inline bool tetra_contains(const SimplexPoint* t, v3 o) {
         tet_face(f, &ia, &ib, &ic);
         const v3 a = t[ia].w, b = t[ib].w, c = t[ic].w;
         const v3 normal = cross3(sub(b, a), sub(c, a));
-        const float span = fmx(fmx(len3(sub(b, a)), len3(sub(c, a))), len3(sub(c, b)));
+        const float span = fmx(fmx(tet_edge(t, ib, ia), tet_edge(t, ic, ia)), tet_edge(t, ic, ib));
         if (len3(normal) <= 1e-6f * span * span) return false;
         if (dot3(normal, sub(o, a)) > 0.0f) return false;
     }
@@ -284,22 +290,20 @@ __device__ inline bool simplex_holds(const SimplexPoint* s, uint32_t count, v3 p
     return held;
 }
 
+// Keeps the points whose weight exceeds 1e-6, in order, in the first slots (s[0] alone when none
+// does). Slot t takes the t-th kept point: the first kept of four, the second of the last three,
+// the third of the last two; the slots past the new count are never read again.
 __device__ inline uint32_t simplex_reduce(SimplexPoint* s, const float* lambdas) {
-    SimplexPoint kept[4] = {s[0], s[1], s[2], s[3]};
-    uint32_t count = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) {
-        const bool take = lambdas[i] > 1e-6f;
-#pragma unroll
-        for (uint32_t t = 0; t < 4; ++t) kept[t] = pick_point(take && t == count, s[i], kept[t]);
-        count += take ? 1u : 0u;
-    }
-    if (count == 0u) {  // kept[0] is still s[0]
-        count = 1;
-    }
-#pragma unroll
-    for (uint32_t i = 0; i < 4; ++i) s[i] = pick_point(i < count, kept[i], s[i]);
-    return count;
+    const bool k0 = lambdas[0] > 1e-6f, k1 = lambdas[1] > 1e-6f, k2 = lambdas[2] > 1e-6f, k3 = lambdas[3] > 1e-6f;
+    const uint32_t count = (k0 ? 1u : 0u) + (k1 ? 1u : 0u) + (k2 ? 1u : 0u) + (k3 ? 1u : 0u);
+    const uint32_t before2 = (k0 ? 1u : 0u) + (k1 ? 1u : 0u);  // kept among the first two
+    const SimplexPoint first = pick_point(k0, s[0], pick_point(k1, s[1], pick_point(k2, s[2], pick_point(k3, s[3], s[0]))));
+    const SimplexPoint second = pick_point(before2 == 2u, s[1], pick_point(before2 == 1u && k2, s[2], s[3]));
+    const SimplexPoint third = pick_point(before2 == 2u && k2, s[2], s[3]);
+    s[0] = first;
+    s[1] = second;
+    s[2] = third;
+    return count == 0u ? 1u : count;
 }
 
 __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t count) {
@@ -314,10 +318,13 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
         r.v = zero3v();
         return r;
     }
+    float best = len3(r.v);  // len3(r.v), kept beside r.v
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
-        if (i < count && len3(s[i].w) < len3(r.v)) {
+        const float length = len3(s[i].w);
+        if (i < count && length < best) {
             r.v = s[i].w;
+            best = length;
 #pragma unroll
             for (int k = 0; k < 4; ++k) r.lambdas[k] = 0.0f;
             r.lambdas[i] = 1.0f;
@@ -334,8 +341,10 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
             if (denom < 1e-12f) continue;
             const float t = clampf(-dot3(a, ab) / denom, 0.0f, 1.0f);
             const v3 point = add(a, mul(ab, t));
-            if (len3(point) < len3(r.v)) {
+            const float length = len3(point);
+            if (length < best) {
                 r.v = point;
+                best = length;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) r.lambdas[k] = 0.0f;
                 r.lambdas[i] = 1.0f - t;
@@ -353,8 +362,10 @@ __device__ inline SimplexResult simplex_closest(const SimplexPoint* s, uint32_t 
                     if (k >= count) continue;
                     v3 weights;
                     const v3 point = closest_on_triangle(zero3v(), s[i].w, s[j].w, s[k].w, &weights);
-                    if (len3(point) < len3(r.v)) {
+                    const float length = len3(point);
+                    if (length < best) {
                         r.v = point;
+                        best = length;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) r.lambdas[q] = 0.0f;
                         r.lambdas[i] = weights.x;

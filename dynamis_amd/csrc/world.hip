@@ -774,7 +774,10 @@ __device__ inline void convex_pair_after_gjk(const W& w, const ConvexPair& c, co
 // worklist, for its first kGjkFirstIterations iterations: a pair GJK answers by then goes on to
 // the probe and is parked for EPA or the manifold kernel (or ends); a pair that needs more parks
 // its GJK state (kPairGjkResume) for k_narrow_convex_resume.
-__global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_blocks) {
+// Three waves a SIMD (168 registers, a few spilled): the loop is vector-ALU bound and the third
+// wave hides more of its latency than the spills cost (k_narrow_convex 708 -> 627 us and the
+// resume kernel 338 -> 277 us on the settled 100k pile; four waves, 128 registers, were slower).
+__global__ void __launch_bounds__(kThreads, 3) k_narrow_convex(W w, uint32_t class_blocks) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0u) {
         uint32_t hull = 0;  // GJK pairs of the hull classes
@@ -800,7 +803,7 @@ __global__ void __launch_bounds__(kThreads) k_narrow_convex(W w, uint32_t class_
 
 // The remaining GJK iterations of the pairs k_narrow_convex parked (kPairGjkResume), from their
 // parked state, then the rest of convex_hit as in k_narrow_convex.
-__global__ void __launch_bounds__(kThreads) k_narrow_convex_resume(W w) {
+__global__ void __launch_bounds__(kThreads, 3) k_narrow_convex_resume(W w) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= w.work_total[4]) return;
     const uint32_t index = w.worklist[t];
@@ -883,7 +886,8 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
 // The manifolds of the convex path (narrowphase.wgsl:551-664): planes against hulls and
 // cylinders, GJK hits within the margin, and penetrating pairs after EPA (the rest of
 // convex_hit first).
-__global__ void __launch_bounds__(kThreads) k_narrow_finish(W w) {
+// Four waves a SIMD: its clip polygons live in scratch either way (315 -> 298 us on the pile).
+__global__ void __launch_bounds__(kThreads, 4) k_narrow_finish(W w) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= w.work_total[1]) return;
     const uint32_t index = w.worklist[t];
