@@ -755,7 +755,9 @@ __device__ __forceinline__ LinkArgs link_args(const dyn_aabb* aabbs, const uint3
 // Level links, counting pass: chunks of kBpThreads / kLanes consecutive colliders, a chunk per
 // workgroup, kLanes lanes per collider; link_blocks holds each chunk's pair total.
 template <uint32_t kLanes>
-__global__ void __launch_bounds__(kBpThreads) bp_links_count_kernel(
+// Eight waves a SIMD (64 registers, a few spilled): the walks are latency-bound (the batch's
+// count 1.63 -> 1.42 ms; the emit pass, which writes, was slower so).
+__global__ void __launch_bounds__(kBpThreads, 8) bp_links_count_kernel(
     const dyn_aabb* __restrict__ aabbs, const uint32_t* __restrict__ groups,
     const uint32_t* __restrict__ flags, uint32_t n, const uint32_t* __restrict__ cnt_level,
     const uint32_t* __restrict__ first_slot, const uint32_t* __restrict__ keys,
