@@ -94,7 +94,6 @@ struct W {
     uint32_t* work_total;  // device words: entries on the worklist of modes 0-2, the hull-class GJK pairs,
                            // the resumed GJK pairs (mode 4), the EPA polytopes given a wave, the
                            // candidate pairs (mode 6)
-    uint32_t* work_keys;   // GJK worklist sort keys: class, then the hull sources
     uint32_t* work_minor;  // the sort's unused minor lane
     uint32_t* dest;
     dyn_contact* contacts;
@@ -123,6 +122,8 @@ struct W {
                       // section-major (solve_section)
     float4* sbodies;  // per row, per substep: orientation, frozen inverse inertia and mass, split count, velocities
     float4* sposes;   // per row, per substep: position and inert flag, local centre of mass
+    float4* bbodies;  // per row, per step (k_solver_reset): world centre of mass and inverse mass,
+                      // velocity and inert flag, angular velocity, for k_blocks
     float4* simp;     // per contact block, for the step's iterations: accumulated normal, tangent 1, tangent 2
     float* resolution;
     // Solver deltas without atomics: every block writes its fixed-point words to its own slots
@@ -584,25 +585,6 @@ __global__ void __launch_bounds__(kThreads) k_class_scatter(W w, uint32_t mode) 
     }
 }
 
-// Within a worklist's classes (GJK's and the manifolds'), pairs that share hull sources run side
-// by side, so a wave's
-// lanes mostly scan the same hull's vertices (one cache line per load instruction): the key is
-// the class, then the first and the second shape's hull source (sources from 511 on, and shapes
-// that are not hulls, share the last value). Every pair's result goes to its own slot, so the
-// order changes no result.
-constexpr uint32_t kWorkKeyBytes = 3;
-__global__ void __launch_bounds__(kThreads) k_work_keys(W w, uint32_t mode) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= w.work_total[mode]) return;
-    const uint32_t index = w.worklist[t];
-    const dyn_collider& fc = w.co[w.pmaj[index]];
-    const dyn_collider& sc = w.co[w.pmin[index]];
-    const uint32_t fa = fc.kind == DYN_SHAPE_HULL ? min(fc.source, 511u) : 511u;
-    const uint32_t fb = sc.kind == DYN_SHAPE_HULL ? min(sc.source, 511u) : 511u;
-    w.work_keys[t] = (static_cast<uint32_t>(w.pkind[index]) << 18) | (fa << 9) | fb;
-    w.work_minor[t] = 0u;
-}
-
 // Analytic pairs; convex-path pairs are marked for the kernels below.
 // Every candidate pair's kind class (pairs naming colliders outside the records: class 0), so
 // k_narrow's worklist groups its pairs by kind and a wave runs one generator.
@@ -1012,16 +994,34 @@ __global__ void __launch_bounds__(kThreads) k_scan_dest(const uint32_t* __restri
 
 // compact_scatter.wgsl: valid raw contacts in pair order; carried impulses cleared (the raw
 // record already has them at zero).
+// The workgroup lists its pairs' (raw slot, destination) and copies the records together, its
+// lanes on consecutive 16-byte pieces, instead of each lane walking its own 352-byte record.
+static_assert(sizeof(dyn_contact) % sizeof(float4) == 0, "contacts copy in 16-byte pieces");
 __global__ void __launch_bounds__(kThreads) k_contacts_scatter(W w) {
+    constexpr uint32_t kPieces = sizeof(dyn_contact) / sizeof(float4);
+    __shared__ uint2 job[kThreads];
+    __shared__ uint32_t jobs;
+    if (threadIdx.x == 0) jobs = 0u;
+    __syncthreads();
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t live = min(*cw(w.counters, DYN_COUNTER_PAIRS), w.pair_cap);
-    if (index >= live || w.valid[index] != kPairContact) return;
-    const uint32_t at = w.dest[index];
-    if (at >= w.contact_cap) return;
-    const dyn_contact& c = w.raw[index];
-    w.contacts[at] = c;
-    w.ckeys[at] = make_uint2(c.a, c.b);
-    w.matched[at] = 0u;
+    if (index < live && w.valid[index] == kPairContact) {
+        const uint32_t at = w.dest[index];
+        if (at < w.contact_cap) job[atomicAdd(&jobs, 1u)] = make_uint2(index, at);
+    }
+    __syncthreads();
+    const uint32_t n = jobs;
+    for (uint32_t item = threadIdx.x; item < n * kPieces; item += blockDim.x) {
+        const uint32_t k = item / kPieces, piece = item - k * kPieces;
+        const uint2 jb = job[k];
+        reinterpret_cast<float4*>(w.contacts + jb.y)[piece] = reinterpret_cast<const float4*>(w.raw + jb.x)[piece];
+    }
+    if (threadIdx.x < n) {
+        const uint2 jb = job[threadIdx.x];
+        const dyn_contact& c = w.raw[jb.x];
+        w.ckeys[jb.y] = make_uint2(c.a, c.b);
+        w.matched[jb.y] = 0u;
+    }
 }
 
 __global__ void k_contacts_total(W w) {
@@ -1594,7 +1594,14 @@ __global__ void __launch_bounds__(kThreads) k_solver_reset(W w) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < w.n; i += gridDim.x * blockDim.x) {
         w.block_counts[i] = 0u;
         reinterpret_cast<float4*>(w.resolution)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (i < w.d && body_is_active(w.st[i], w.de[i])) ++live;
+        const Body b = load_body(w, i);
+        if (i < w.d && body_is_active(b.s, b.d)) ++live;
+        // what k_blocks reads of a block's body, in one 48-byte record
+        const v3 com = body_com(b);
+        float4* r = w.bbodies + 3u * i;
+        r[0] = make_float4(com.x, com.y, com.z, b.d.inverse_mass);
+        r[1] = make_float4(b.s.velocity[0], b.s.velocity[1], b.s.velocity[2], body_is_inert(b) ? 1.0f : 0.0f);
+        r[2] = make_float4(b.s.angular_velocity[0], b.s.angular_velocity[1], b.s.angular_velocity[2], 0.0f);
     }
     block_count(cw(w.counters, DYN_COUNTER_LIVE), live);
 }
@@ -1619,22 +1626,28 @@ __global__ void __launch_bounds__(kThreads) k_blocks(W w) {
         w.slots[2 * i] = w.slots[2 * i + 1] = kNoSlot;
         return;
     }
-    const Body first = load_body(w, fb), second = load_body(w, sb);
+    // the bodies as k_solver_reset recorded them: centre of mass, inverse mass, velocities, inert
+    const float4* rf = w.bbodies + 3u * fb;
+    const float4* rs = w.bbodies + 3u * sb;
+    const float4 f0 = rf[0], f1 = rf[1], f2 = rf[2], s0 = rs[0], s1 = rs[1], s2 = rs[2];
+    const bool fi = f1.w != 0.0f, si = s1.w != 0.0f;
     const v3 normal = ld3(c.normal);
     {
         // position_correction.wgsl:5-46 reads only what stays fixed through the substeps
-        const bool fi = body_is_inert(first), si = body_is_inert(second);
         const uint32_t flags = c.relaxation > 0.0f ? 0u : (c.point_count | (fi ? 8u : 0u) | (si ? 16u : 0u));
-        prow[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(flags),
-                              fi ? 0.0f : first.d.inverse_mass);
-        prow[1] = make_float4(si ? 0.0f : second.d.inverse_mass, normal.x, normal.y, normal.z);
+        prow[0] = make_float4(__uint_as_float(fb), __uint_as_float(sb), __uint_as_float(flags), fi ? 0.0f : f0.w);
+        prow[1] = make_float4(si ? 0.0f : s0.w, normal.x, normal.y, normal.z);
         prow[2] = make_float4(c.points[0].depth, c.point_count > 1u ? c.points[1].depth : 0.0f,
                               c.point_count > 2u ? c.points[2].depth : 0.0f,
                               c.point_count > 3u ? c.points[3].depth : 0.0f);
     }
+    const v3 com_f = mk(f0.x, f0.y, f0.z), v_f = mk(f1.x, f1.y, f1.z), w_f = mk(f2.x, f2.y, f2.z);
+    const v3 com_s = mk(s0.x, s0.y, s0.z), v_s = mk(s1.x, s1.y, s1.z), w_s = mk(s2.x, s2.y, s2.z);
     for (uint32_t pi = 0; pi < c.point_count; ++pi) {
         const v3 pos = ld3(c.points[pi].position);
-        const float speed = dot3(relative_velocity(first, second, pos, pos), normal);
+        // relative_velocity: each body's point velocity v + w x (p - centre of mass)
+        const v3 rel = sub(add(v_s, cross3(w_s, sub(pos, com_s))), add(v_f, cross3(w_f, sub(pos, com_f))));
+        const float speed = dot3(rel, normal);
         float target = 0.0f;
         if (c.points[pi].depth < 0.0f) target = c.points[pi].depth / w.p.dt;
         else if (speed < -w.p.restitution_threshold) target = -speed * c.restitution;
@@ -1643,8 +1656,8 @@ __global__ void __launch_bounds__(kThreads) k_blocks(W w) {
     // a non-inert row's count is the length of its list; the count before this block's
     // increment is the block's place on it
     uint32_t slot_f = kNoSlot, slot_s = kNoSlot;
-    if (body_is_inert(first)) w.block_counts[fb] = 1u; else slot_f = atomicAdd(&w.block_counts[fb], 1u);
-    if (body_is_inert(second)) w.block_counts[sb] = 1u; else slot_s = atomicAdd(&w.block_counts[sb], 1u);
+    if (fi) w.block_counts[fb] = 1u; else slot_f = atomicAdd(&w.block_counts[fb], 1u);
+    if (si) w.block_counts[sb] = 1u; else slot_s = atomicAdd(&w.block_counts[sb], 1u);
     w.slots[2 * i] = slot_f;
     w.slots[2 * i + 1] = slot_s;
 }
@@ -2458,8 +2471,6 @@ struct dyn_world {
     size_t bp_scratch_bytes = 0;
     void* sort_scratch = nullptr;
     size_t sort_scratch_bytes = 0;
-    void* work_scratch = nullptr;  // the GJK worklist sort
-    size_t work_scratch_bytes = 0;
     std::vector<void*> owned;
     void* shape_sources = nullptr;   // owned separately: replaced by each shape upload
     void* shape_vertices = nullptr;
@@ -2556,7 +2567,6 @@ int alloc_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_cap
     A(w.pkind, pair_capacity);
     A(w.worklist, pair_capacity);
     A(w.class_hist, kClasses * (static_cast<size_t>(ceil_div(pair_capacity ? pair_capacity : 1u, kClassBlock)) + 1u));
-    A(w.work_keys, pair_capacity);
     A(w.work_minor, pair_capacity);
     A(w.dest, pair_capacity);
     A(w.contacts, contact_capacity);
@@ -2591,10 +2601,8 @@ int alloc_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_cap
     w.contact_cap = contact_capacity;
     world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(w.k, world->entry_cap, pair_capacity);
     world->sort_scratch_bytes = dyn_sort_scratch_bytes(contact_capacity, 2u * dyn_sort_key_words(w.id_cap ? w.id_cap : 1u));
-    world->work_scratch_bytes = dyn_sort_scratch_bytes(pair_capacity, kWorkKeyBytes);
     for (auto [ptr, bytes] : {std::pair<void**, size_t>{&world->bp_scratch, world->bp_scratch_bytes},
-                              {&world->sort_scratch, world->sort_scratch_bytes},
-                              {&world->work_scratch, world->work_scratch_bytes}}) {
+                              {&world->sort_scratch, world->sort_scratch_bytes}}) {
         *ptr = nullptr;
         DYN_HIP(hipMalloc(ptr, bytes ? bytes : 1u));
         g.push_back(*ptr);
@@ -2633,7 +2641,7 @@ int grow_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capa
     const W old = w;
     std::vector<void*> old_group;
     old_group.swap(world->streams);
-    void* old_scratch[3] = {world->bp_scratch, world->sort_scratch, world->work_scratch};
+    void* old_scratch[2] = {world->bp_scratch, world->sort_scratch};
     (void)old_scratch;
     const int rc = alloc_streams(world, pair_capacity, contact_capacity);
     if (rc != DYN_OK) {
@@ -2642,10 +2650,8 @@ int grow_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_capa
         w = old;
         world->bp_scratch_bytes = dyn_broadphase_scratch_bytes(w.k, world->entry_cap, w.pair_cap);
         world->sort_scratch_bytes = dyn_sort_scratch_bytes(w.contact_cap, 2u * dyn_sort_key_words(w.id_cap ? w.id_cap : 1u));
-        world->work_scratch_bytes = dyn_sort_scratch_bytes(w.pair_cap, kWorkKeyBytes);
         world->bp_scratch = old_scratch[0];
         world->sort_scratch = old_scratch[1];
-        world->work_scratch = old_scratch[2];
         return rc;
     }
     const size_t cap = old.contact_cap;
@@ -2734,7 +2740,7 @@ int worklist(const dyn::W& w, hipStream_t st, uint32_t mode) {
 // and the convex manifolds take up over class-grouped worklists. Afterwards valid[i] is
 // kPairContact where raw[i] holds pair i's contact and kPairNone elsewhere, for every pair below
 // the PAIRS counter (and the capacity).
-int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch, size_t work_scratch_bytes) {
+int narrowphase(const dyn::W& w, hipStream_t st, bool convex) {
     using namespace dyn;
     const uint32_t pb = blocks_for(w.pair_cap);
     LAUNCH_ON(st, k_pair_class, pb, w);
@@ -2743,9 +2749,6 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
     if (!convex) return DYN_OK;
     DYN_TRY(worklist(w, st, 0u));
     const uint32_t class_blocks = ceil_div(w.pair_cap ? w.pair_cap : 1u, kClassBlock);
-    LAUNCH_ON(st, k_work_keys, pb, w, 0u);
-    DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[0], nullptr, 1u,
-                           kWorkKeyBytes, 0u, work_scratch, work_scratch_bytes));
     LAUNCH_ON(st, k_narrow_convex, pb, w, class_blocks);
     DYN_TRY(worklist(w, st, 4u));
     LAUNCH_ON(st, k_narrow_convex_resume, pb, w);
@@ -2766,9 +2769,6 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex, void* work_scratch
     }
     DYN_LAUNCHED("k_narrow_epa");
     DYN_TRY(worklist(w, st, 1u));
-    LAUNCH_ON(st, k_work_keys, pb, w, 1u);
-    DYN_TRY(dyn_sort_pairs(st, w.work_keys, w.work_minor, w.worklist, w.pair_cap, &w.work_total[1], nullptr, 1u,
-                           kWorkKeyBytes, 0u, work_scratch, work_scratch_bytes));
     LAUNCH_ON(st, k_narrow_finish, pb, w);
     return DYN_OK;
 }
@@ -2987,7 +2987,7 @@ int world_step(dyn_world* world, float dt) {
     io.scratch_bytes = world->bp_scratch_bytes;
     io.d_worlds = w.worlds;
     DYN_TRY(dyn_broadphase(st, &io));
-    DYN_TRY(narrowphase(w, st, world->convex_colliders, world->work_scratch, world->work_scratch_bytes));
+    DYN_TRY(narrowphase(w, st, world->convex_colliders));
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
     DYN_TRY(scan(world, w.valid, pair_count, w.pair_cap, w.dest, &w.scan_total[0]));
     LAUNCH(k_contacts_scatter, pb, w);
@@ -3131,6 +3131,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.block_counts, body_count);
     A(w.sbodies, static_cast<size_t>(kSolverBody) * body_count);
     A(w.sposes, 2u * static_cast<size_t>(body_count));
+    A(w.bbodies, 3u * static_cast<size_t>(body_count));
     A(w.resolution, 4u * static_cast<size_t>(body_count));
     A(w.list_counts, body_count);
     A(w.row_start, body_count);
@@ -3538,12 +3539,11 @@ int dyn_world_read_resting(dyn_world* world, dyn_contact* out, uint32_t capacity
 
 namespace {
 
-// dyn_narrowphase's scratch: the convex worklists and their sort.
+// dyn_narrowphase's scratch: the kind classes and the worklists.
 struct NarrowLayout {
     uint8_t* pkind;
-    uint32_t *worklist, *class_hist, *work_total, *work_keys, *work_minor;
-    void* sort;
-    size_t sort_bytes, bytes;
+    uint32_t *worklist, *class_hist, *work_total, *work_minor;
+    size_t bytes;
 };
 
 NarrowLayout narrow_layout(uint32_t pair_capacity, void* base) {
@@ -3561,10 +3561,7 @@ NarrowLayout narrow_layout(uint32_t pair_capacity, void* base) {
     l.class_hist = reinterpret_cast<uint32_t*>(take(4u * kClasses * (static_cast<size_t>(ceil_div(
                                                                           static_cast<uint32_t>(cap), kClassBlock)) + 1u)));
     l.work_total = reinterpret_cast<uint32_t*>(take(4u * 8u));
-    l.work_keys = reinterpret_cast<uint32_t*>(take(4u * cap));
     l.work_minor = reinterpret_cast<uint32_t*>(take(4u * cap));
-    l.sort_bytes = dyn_sort_scratch_bytes(static_cast<uint32_t>(cap), kWorkKeyBytes);
-    l.sort = take(l.sort_bytes);
     l.bytes = at;
     return l;
 }
@@ -3626,9 +3623,8 @@ int dyn_narrowphase(void* stream, const dyn_narrowphase_io* io) {
     w.worklist = l.worklist;
     w.class_hist = l.class_hist;
     w.work_total = l.work_total;
-    w.work_keys = l.work_keys;
     w.work_minor = l.work_minor;
-    return narrowphase(w, static_cast<hipStream_t>(stream), true, l.sort, l.sort_bytes);
+    return narrowphase(w, static_cast<hipStream_t>(stream), true);
 }
 
 namespace {
@@ -3657,6 +3653,7 @@ SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base)
     w.srows = reinterpret_cast<float4*>(take(sizeof(float4) * kSolveRow * c));
     w.sbodies = reinterpret_cast<float4*>(take(sizeof(float4) * kSolverBody * n));
     w.sposes = reinterpret_cast<float4*>(take(sizeof(float4) * 2u * n));
+    w.bbodies = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * n));
     w.simp = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
     w.resolution = reinterpret_cast<float*>(take(4u * 4u * n));
     w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
