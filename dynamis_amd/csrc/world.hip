@@ -122,6 +122,7 @@ struct W {
                       // section-major (solve_section)
     float4* sbodies;  // per row, per substep: orientation, frozen inverse inertia and mass, split count, velocities
     float4* sposes;   // per row, per substep: position and inert flag, local centre of mass
+    float4* ppose;    // per claimed row, during the position iterations: position, orientation
     float4* bbodies;  // per row, per step (k_solver_reset): world centre of mass and inverse mass,
                       // velocity and inert flag, angular velocity, for k_blocks
     float4* simp;     // per contact block, for the step's iterations: accumulated normal, tangent 1, tangent 2
@@ -2283,7 +2284,11 @@ __global__ void __launch_bounds__(kThreads) k_position_block(W w) {
 }
 
 // position_apply.wgsl over every claimed row
-__device__ __forceinline__ void position_apply_at(const W& w, uint32_t row) {
+// A claimed row (count > 0) keeps its pose in ppose through the iterations: the first reads the
+// body state and writes ppose (moved or not), the last writes the body state (every claimed row,
+// as the same bits per-iteration updates of the state would leave). An inert row with a
+// contributing block, which differs from iteration to iteration, reads and writes the state.
+__device__ __forceinline__ void position_apply_at(const W& w, uint32_t row, uint32_t first, uint32_t last) {
     if (row >= w.n) return;
     // list_counts is 0 for rows without blocks and for inert rows; contrib, zero outside a
     // position pass, is marked only for inert rows with a contributing block
@@ -2291,8 +2296,19 @@ __device__ __forceinline__ void position_apply_at(const W& w, uint32_t row) {
     const uint32_t start = w.row_start[row];
     dyn_body_state& s = w.st[row];
     float4& r = reinterpret_cast<float4*>(w.resolution)[row];
-    const v3 p0 = ld3(s.position);  // in flight with the list
-    const float4 q0 = ld4(s.orientation), r0 = r;
+    float4* pose = w.ppose + 2u * row;
+    const bool in_state = count == 0u || first;
+    v3 p0;
+    float4 q0;
+    if (in_state) {  // in flight with the list
+        p0 = ld3(s.position);
+        q0 = ld4(s.orientation);
+    } else {
+        const float4 a = pose[0];
+        p0 = mk(a.x, a.y, a.z);
+        q0 = pose[1];
+    }
+    const float4 r0 = r;
     uint32_t contributing = 0;
     uint4 lo = make_uint4(0u, 0u, 0u, 0u);  // linear words, then the (zero) angular x
     uint2 hi = make_uint2(0u, 0u);          // angular y, z: the position blocks send none
@@ -2315,7 +2331,16 @@ __device__ __forceinline__ void position_apply_at(const W& w, uint32_t row) {
                 lo.z += word[j].w;
             }
         }
-        if (contributing == 0u) return;
+        if (contributing == 0u) {
+            if (first && !last) {
+                pose[0] = make_float4(p0.x, p0.y, p0.z, 0.0f);
+                pose[1] = q0;
+            } else if (last && !first) {
+                st3(s.position, p0);
+                st4(s.orientation, q0);
+            }
+            return;
+        }
     }
     const v3 lin = mk(solver_value(lo.x, DYN_SOLVER_POSITION_SCALE), solver_value(lo.y, DYN_SOLVER_POSITION_SCALE),
                       solver_value(lo.z, DYN_SOLVER_POSITION_SCALE));
@@ -2323,14 +2348,21 @@ __device__ __forceinline__ void position_apply_at(const W& w, uint32_t row) {
                       solver_value(hi.y, DYN_SOLVER_POSITION_SCALE));
     const float cf = static_cast<float>(contributing);
     const v3 al = dv(lin, cf), aa = dv(ang, cf);
-    st3(s.position, add(p0, al));
-    st4(s.orientation, qnorm(qmul(make_float4(aa.x * 0.5f, aa.y * 0.5f, aa.z * 0.5f, 1.0f), q0)));
+    const v3 p1 = add(p0, al);
+    const float4 q1 = qnorm(qmul(make_float4(aa.x * 0.5f, aa.y * 0.5f, aa.z * 0.5f, 1.0f), q0));
+    if (count == 0u || last) {
+        st3(s.position, p1);
+        st4(s.orientation, q1);
+    } else {
+        pose[0] = make_float4(p1.x, p1.y, p1.z, 0.0f);
+        pose[1] = q1;
+    }
     const v3 nr = add(mk(r0.x, r0.y, r0.z), al);
     r = make_float4(nr.x, nr.y, nr.z, 0.0f);
 }
 
-__global__ void __launch_bounds__(kThreads) k_position_apply(W w) {
-    position_apply_at(w, blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(kThreads) k_position_apply(W w, uint32_t first, uint32_t last) {
+    position_apply_at(w, blockIdx.x * blockDim.x + threadIdx.x, first, last);
 }
 
 // island_sleep.wgsl
@@ -2871,7 +2903,8 @@ int solver_substeps(const dyn::W& w, hipStream_t st, uint32_t threads = dyn::kTh
         LAUNCH_TH(st, k_advance, w.d, threads, w);
         for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
             LAUNCH_TH(st, k_position_block, w.contact_cap, threads, w);
-            LAUNCH_TH(st, k_position_apply, w.n, threads, w);
+            LAUNCH_TH(st, k_position_apply, w.n, threads, w, it == 0u ? 1u : 0u,
+                      it + 1u == w.p.position_iterations ? 1u : 0u);
         }
     }
     LAUNCH_TH(st, k_impulses_store, w.contact_cap, threads, w);
@@ -3132,6 +3165,7 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
     A(w.sbodies, static_cast<size_t>(kSolverBody) * body_count);
     A(w.sposes, 2u * static_cast<size_t>(body_count));
     A(w.bbodies, 3u * static_cast<size_t>(body_count));
+    A(w.ppose, 2u * static_cast<size_t>(body_count));
     A(w.resolution, 4u * static_cast<size_t>(body_count));
     A(w.list_counts, body_count);
     A(w.row_start, body_count);
@@ -3654,6 +3688,7 @@ SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base)
     w.sbodies = reinterpret_cast<float4*>(take(sizeof(float4) * kSolverBody * n));
     w.sposes = reinterpret_cast<float4*>(take(sizeof(float4) * 2u * n));
     w.bbodies = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * n));
+    w.ppose = reinterpret_cast<float4*>(take(sizeof(float4) * 2u * n));
     w.simp = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
     w.resolution = reinterpret_cast<float*>(take(4u * 4u * n));
     w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
