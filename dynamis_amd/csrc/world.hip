@@ -123,6 +123,8 @@ struct W {
     float4* sbodies;  // per row, per substep: orientation, frozen inverse inertia and mass, split count, velocities
     float4* sposes;   // per row, per substep: position and inert flag, local centre of mass
     float4* ppose;    // per claimed row, during the position iterations: position, orientation
+    float4* sanchor;  // per contact point, per step (first k_solve_prep): local anchors and depth,
+                      // point-major like srows
     float4* bbodies;  // per row, per step (k_solver_reset): world centre of mass and inverse mass,
                       // velocity and inert flag, angular velocity, for k_blocks
     float4* simp;     // per contact block, for the step's iterations: accumulated normal, tangent 1, tangent 2
@@ -1717,8 +1719,12 @@ __device__ __forceinline__ void apply_pair_impulse(Body& a, Body& b, v3 pa, v3 p
         add(ld3(b.s.angular_velocity), apply_inverse_inertia(b, cross3(sub(pb, body_com(b)), impulse))));
 }
 
+__device__ __forceinline__ v3 arm_at(const Body& b, v3 local) {
+    return add(qrot(ld4(b.s.orientation), local), ld3(b.s.position));
+}
+
 __device__ __forceinline__ v3 arm(const Body& b, const dyn_manifold_point& p, bool first) {
-    return add(qrot(ld4(b.s.orientation), ld3(first ? p.local_a : p.local_b)), ld3(b.s.position));
+    return arm_at(b, ld3(first ? p.local_a : p.local_b));
 }
 
 // solver_block_solve.wgsl:56-83: inert bodies receive exactly zero words (skipped).
@@ -1879,6 +1885,24 @@ __global__ void __launch_bounds__(kThreads) k_warm(W w) {
     warm_at(w, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// A block's point pi: its local anchors and depth, from the contact record or from the copy the
+// step's first k_solve_prep made.
+__device__ __forceinline__ void prep_anchor(const W& w, const dyn_contact& contact, uint32_t ci, uint32_t pi,
+                                            bool from_contact, v3* la, v3* lb, float* depth) {
+    if (from_contact) {
+        const dyn_manifold_point& p = contact.points[pi];
+        *la = ld3(p.local_a);
+        *lb = ld3(p.local_b);
+        *depth = p.depth;
+        return;
+    }
+    const float4* anchor = w.sanchor + (static_cast<size_t>(pi) * w.contact_cap + ci) * 2u;
+    const float4 a = anchor[0], b = anchor[1];
+    *la = mk(a.x, a.y, a.z);
+    *lb = mk(b.x, b.y, b.z);
+    *depth = a.w;
+}
+
 // solver_contact_block.wgsl:31-110, the parts of a block that depend on poses and masses only:
 // each point's arms from the centres of mass, its row masses, target speed and depth.
 __device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t load_impulses) {
@@ -1925,17 +1949,29 @@ __device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t 
     if (fi && si) return;  // both inert: zero words (commit_block of zeros)
     v3 t1, t2;
     make_tangents(normal, &t1, &t2);
+    // the points' local anchors and depths stay fixed through the step: the first substep copies
+    // them from the contact's 64-byte point records into 32 bytes a point (point-major), which
+    // the later substeps read instead
     v3 reach_a = zero3v(), reach_b = zero3v();
     const v3 com_f = body_com(pf), com_s = body_com(ps);
     for (uint32_t pi = 0; pi < point_count; ++pi) {
-        const dyn_manifold_point& p = contact.points[pi];
-        reach_a = add(reach_a, sub(arm(pf, p, true), com_f));
-        reach_b = add(reach_b, sub(arm(ps, p, false), com_s));
+        v3 la, lb;
+        float depth;
+        prep_anchor(w, contact, ci, pi, load_impulses != 0u, &la, &lb, &depth);
+        if (load_impulses) {
+            float4* anchor = w.sanchor + (static_cast<size_t>(pi) * w.contact_cap + ci) * 2u;
+            anchor[0] = make_float4(la.x, la.y, la.z, depth);
+            anchor[1] = make_float4(lb.x, lb.y, lb.z, 0.0f);
+        }
+        reach_a = add(reach_a, sub(arm_at(pf, la), com_f));
+        reach_b = add(reach_b, sub(arm_at(ps, lb), com_s));
     }
     const float count = static_cast<float>(max(point_count, 1u));
     for (uint32_t pi = 0; pi < point_count; ++pi) {
-        const dyn_manifold_point& p = contact.points[pi];
-        const v3 aa = arm(pf, p, true), ab = arm(ps, p, false);
+        v3 la, lb;
+        float depth;
+        prep_anchor(w, contact, ci, pi, load_impulses != 0u, &la, &lb, &depth);
+        const v3 aa = arm_at(pf, la), ab = arm_at(ps, lb);
         const v3 ra = sub(aa, body_com(pf)), rb = sub(ab, body_com(ps));  // point_velocity's arms
         const v3 arm_a = cross3(sub(aa, body_com(split_f)), normal);
         const v3 arm_b = cross3(sub(ab, body_com(split_s)), normal);
@@ -1948,7 +1984,7 @@ __device__ __forceinline__ void solve_prep_at(const W& w, uint32_t ci, uint32_t 
         float4* q = solve_section(w, ci, 1u + pi);
         q[0] = make_float4(ra.x, ra.y, ra.z, nmass);
         q[1] = make_float4(rb.x, rb.y, rb.z, truth);
-        q[2] = make_float4(m1, m2, w.targets[ci * 4u + pi], p.depth);
+        q[2] = make_float4(m1, m2, w.targets[ci * 4u + pi], depth);
     }
 }
 
@@ -2616,6 +2652,7 @@ int alloc_streams(dyn_world* world, uint32_t pair_capacity, uint32_t contact_cap
     A(w.prows, 3u * static_cast<size_t>(contact_capacity));
     A(w.srows, static_cast<size_t>(kSolveRow) * contact_capacity);
     A(w.simp, 3u * static_cast<size_t>(contact_capacity));
+    A(w.sanchor, 8u * static_cast<size_t>(contact_capacity));
     A(w.slots, 2u * static_cast<size_t>(contact_capacity));
     A(w.adj, 2u * static_cast<size_t>(contact_capacity));
     A(w.vwords, 12u * static_cast<size_t>(contact_capacity));
@@ -3690,6 +3727,7 @@ SolveLayout solve_layout(uint32_t bodies, uint32_t contact_capacity, void* base)
     w.bbodies = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * n));
     w.ppose = reinterpret_cast<float4*>(take(sizeof(float4) * 2u * n));
     w.simp = reinterpret_cast<float4*>(take(sizeof(float4) * 3u * c));
+    w.sanchor = reinterpret_cast<float4*>(take(sizeof(float4) * 8u * c));
     w.resolution = reinterpret_cast<float*>(take(4u * 4u * n));
     w.slots = reinterpret_cast<uint32_t*>(take(4u * 2u * c));
     w.list_counts = reinterpret_cast<uint32_t*>(take(4u * n));
