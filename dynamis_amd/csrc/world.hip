@@ -415,6 +415,7 @@ constexpr uint32_t kPairEpaDone = 4;  // EPA result parked for k_narrow_finish
 constexpr uint32_t kPairHit = 5;      // GJK hit within the margin parked for k_narrow_finish
 constexpr uint32_t kPairPlane = 6;    // plane against a hull or cylinder, for k_narrow_finish
 constexpr uint32_t kPairGjkResume = 7;  // GJK's state parked for k_narrow_convex_resume
+constexpr uint32_t kPairEpaWave = 8;    // EPA result of the wave kernel parked for k_narrow_finish_tail
 
 // Material, identity and anchors of a generated manifold (narrowphase.wgsl:665-684).
 __device__ inline void write_contact(const W& w, uint32_t index, uint32_t fs, uint32_t ss, const Body& first,
@@ -817,8 +818,8 @@ __device__ __forceinline__ void park_epa(float* park, const Epa& r) {
 
 // EPA of the pairs marked kPairEpa, kEpaGroup lanes per pair, the groups striding over the
 // worklist. A pair whose polytope outgrows the small capacities keeps its mark and is listed for
-// k_narrow_epa (work_total[5] entries of work_minor, whose sort lane is free once the GJK worklist
-// is sorted); the others are parked (lane 0 of the group) and marked kPairEpaDone.
+// k_narrow_epa (work_total[5] entries of work_minor); the others are parked (lane 0 of the group)
+// and marked kPairEpaDone.
 __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa_small(W w, uint32_t vertex_limit) {
     constexpr uint32_t kGroupsPerBlock = kEpaWaves * kWave / kEpaGroup;
     __shared__ EpaSmallShared epa[kGroupsPerBlock];
@@ -863,7 +864,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
         const Epa r = epa_tetrahedron(wf, wsd, shapes, park, epa[wave]);
         if (lane == 0u) {
             park_epa(park, r);
-            w.valid[index] = kPairEpaDone;
+            w.valid[index] = kPairEpaWave;
         }
     }
 }
@@ -872,12 +873,7 @@ __global__ void __launch_bounds__(kEpaWaves * kWave) k_narrow_epa(W w) {
 // cylinders, GJK hits within the margin, and penetrating pairs after EPA (the rest of
 // convex_hit first).
 // Four waves a SIMD: its clip polygons live in scratch either way (315 -> 298 us on the pile).
-__global__ void __launch_bounds__(kThreads, 4) k_narrow_finish(W w) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= w.work_total[1]) return;
-    const uint32_t index = w.worklist[t];
-    const uint32_t mark = w.valid[index];
-    if (mark != kPairEpaDone && mark != kPairHit && mark != kPairPlane) return;
+__device__ __forceinline__ void finish_pair(const W& w, uint32_t index, uint32_t mark) {
     w.valid[index] = kPairNone;
     const uint32_t fs = w.pmaj[index], ss = w.pmin[index];
     const uint32_t fb = w.owner[fs], sb = w.owner[ss];
@@ -920,6 +916,25 @@ __global__ void __launch_bounds__(kThreads, 4) k_narrow_finish(W w) {
     }
     if (m.count == 0u) return;
     write_contact(w, index, fs, ss, first, second, fc, sc, m, mirrored_features);
+}
+
+__global__ void __launch_bounds__(kThreads, 4) k_narrow_finish(W w) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= w.work_total[1]) return;
+    const uint32_t index = w.worklist[t];
+    const uint32_t mark = w.valid[index];
+    if (mark != kPairEpaDone && mark != kPairHit && mark != kPairPlane) return;
+    finish_pair(w, index, mark);
+}
+
+// The manifolds of the polytopes k_narrow_epa expanded (listed in work_minor). Their marks keep
+// them off k_narrow_finish's worklist, so the two can run side by side on two streams.
+__global__ void __launch_bounds__(kThreads, 4) k_narrow_finish_tail(W w) {
+    const uint32_t total = w.work_total[5];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t index = w.work_minor[t];
+        if (w.valid[index] == kPairEpaWave) finish_pair(w, index, kPairEpaDone);
+    }
 }
 
 // ---- stable flag compaction: per-block counts, one-workgroup scan, destination map ---------
@@ -2523,6 +2538,12 @@ __global__ void k_consume(W w) {
 
 // ---- host side: the World object behind the C ABI ---------------------------------------------
 
+// A second stream and the events that fork it from and join it to a world's stream.
+struct SideStream {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+};
+
 struct dyn_world {
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -2580,6 +2601,7 @@ struct dyn_world {
     uint32_t* snap_header = nullptr;  // device: snapshot magic, contact capacity, rows, colliders
     uint32_t* plan_host = nullptr;    // pinned: kPlanRing demand records of kPlanWords words
     hipEvent_t plan_events[4] = {nullptr, nullptr, nullptr, nullptr};
+    SideStream side{nullptr, nullptr, nullptr};  // the narrowphase's second stream (default priority)
     uint64_t issued = 0;              // dyn_world_step calls since create
     uint32_t growth_locks = 0;        // holders (differentiable rollouts) whose buffers fix the capacity
     uint32_t grows = 0;
@@ -2809,7 +2831,10 @@ int worklist(const dyn::W& w, hipStream_t st, uint32_t mode) {
 // and the convex manifolds take up over class-grouped worklists. Afterwards valid[i] is
 // kPairContact where raw[i] holds pair i's contact and kPairNone elsewhere, for every pair below
 // the PAIRS counter (and the capacity).
-int narrowphase(const dyn::W& w, hipStream_t st, bool convex) {
+// With `side` (a second stream and two events) the wave EPA kernel and the manifolds of its
+// polytopes run on the second stream beside the worklist and manifold pass of the other pairs;
+// the stream joins back before returning. Without it they run in order on `st`.
+int narrowphase(const dyn::W& w, hipStream_t st, bool convex, const SideStream* side = nullptr) {
     using namespace dyn;
     const uint32_t pb = blocks_for(w.pair_cap);
     LAUNCH_ON(st, k_pair_class, pb, w);
@@ -2832,13 +2857,22 @@ int narrowphase(const dyn::W& w, hipStream_t st, bool convex) {
     // the polytopes given a wave are few (the long tail of the expansion count)
     const uint32_t tasks = ceil_div(w.pair_cap ? w.pair_cap : 1u, static_cast<uint32_t>(kWave));
     const uint32_t groups = min(ceil_div(tasks, kEpaWaves), 256u);
+    hipStream_t tail = st;
+    if (side) {
+        DYN_HIP(hipEventRecord(side->fork, st));
+        DYN_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+        tail = side->stream;
+    }
     {
-        ProfileScope scope("k_narrow_epa", st);
-        k_narrow_epa<<<groups, kEpaWaves * kWave, 0, st>>>(w);
+        ProfileScope scope("k_narrow_epa", tail);
+        k_narrow_epa<<<groups, kEpaWaves * kWave, 0, tail>>>(w);
     }
     DYN_LAUNCHED("k_narrow_epa");
+    LAUNCH_ON(tail, k_narrow_finish_tail, min(pb, 64u), w);
+    if (side) DYN_HIP(hipEventRecord(side->join, side->stream));
     DYN_TRY(worklist(w, st, 1u));
     LAUNCH_ON(st, k_narrow_finish, pb, w);
+    if (side) DYN_HIP(hipStreamWaitEvent(st, side->join, 0));
     return DYN_OK;
 }
 
@@ -3057,7 +3091,7 @@ int world_step(dyn_world* world, float dt) {
     io.scratch_bytes = world->bp_scratch_bytes;
     io.d_worlds = w.worlds;
     DYN_TRY(dyn_broadphase(st, &io));
-    DYN_TRY(narrowphase(w, st, world->convex_colliders));
+    DYN_TRY(narrowphase(w, st, world->convex_colliders, &world->side));
     const uint32_t* pair_count = w.counters + DYN_COUNTER_PAIRS * DYN_COUNTER_STRIDE_WORDS;
     DYN_TRY(scan(world, w.valid, pair_count, w.pair_cap, w.dest, &w.scan_total[0]));
     LAUNCH(k_contacts_scatter, pb, w);
@@ -3146,6 +3180,9 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
         for (hipEvent_t& e : world->plan_events) {
             if (e) (void)hipEventDestroy(e);
         }
+        if (world->side.fork) (void)hipEventDestroy(world->side.fork);
+        if (world->side.join) (void)hipEventDestroy(world->side.join);
+        if (world->side.stream) (void)hipStreamDestroy(world->side.stream);
         if (world->own_stream && world->stream) (void)hipStreamDestroy(world->stream);
         delete world;
         return rc;
@@ -3243,6 +3280,18 @@ int dyn_world_create(dyn_world** out, void* stream, const dyn_step_params* param
             world->plan_events[i] = nullptr;
             return cleanup(fail(DYN_E_HIP, "dyn_world_create: demand events"));
         }
+    }
+    if (hipStreamCreateWithFlags(&world->side.stream, hipStreamNonBlocking) != hipSuccess) {
+        world->side.stream = nullptr;
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: second stream"));
+    }
+    if (hipEventCreateWithFlags(&world->side.fork, hipEventDisableTiming) != hipSuccess) {
+        world->side.fork = nullptr;
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: stream events"));
+    }
+    if (hipEventCreateWithFlags(&world->side.join, hipEventDisableTiming) != hipSuccess) {
+        world->side.join = nullptr;
+        return cleanup(fail(DYN_E_HIP, "dyn_world_create: stream events"));
     }
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (bytes && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = DYN_E_HIP;
@@ -3365,6 +3414,9 @@ int dyn_world_destroy(dyn_world* world) {
     if (world->support_cands) (void)hipFree(world->support_cands);
     if (world->edit_buf) (void)hipFree(world->edit_buf);
     if (world->run_buf) (void)hipFree(world->run_buf);
+    if (world->side.fork) (void)hipEventDestroy(world->side.fork);
+    if (world->side.join) (void)hipEventDestroy(world->side.join);
+    if (world->side.stream) (void)hipStreamDestroy(world->side.stream);
     if (world->own_stream) (void)hipStreamDestroy(world->stream);
     delete world;
     return DYN_OK;
