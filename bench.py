@@ -85,10 +85,10 @@ def broadphase_bytes(name: str, k: int, entries: int, pairs: int) -> float:
     record 32 bytes; box = 24 bytes of min/max; pair = 8 bytes)."""
     if name == "entry_sort_pass":        # read key+slot, write key+slot
         return 16.0 * entries
-    if name == "bp_mates_kernel":        # every sorted key read once; staged pairs written
-        return 4.0 * entries + 8.0 * pairs
-    if name == "bp_links_kernel":        # box + packed count per collider; pairs written
-        return 28.0 * k + 8.0 * pairs
+    if name in ("bp_mates_count_kernel", "bp_mates_emit_kernel"):  # every sorted key read once;
+        return 4.0 * entries + 8.0 * pairs                             # a pair's record each
+    if name in ("bp_links_count_kernel", "bp_links_emit_kernel"):  # box + packed count per collider;
+        return 28.0 * k + 8.0 * pairs                                  # a pair's record each
     if name == "bp_emit_kernel":         # box + packed count in; key, slot, record out
         return 28.0 * k + 40.0 * entries
     if name == "bp_count_kernel":        # box in, packed count out
@@ -117,6 +117,14 @@ def step_bytes(name: str, n: int, k: int, entries: int, pairs: int, contacts: in
         return contacts * (48.0 + 32.0 + 32.0)
     if name in ("k_apply", "k_position_apply"):
         return rows * 72.0
+    if name == "k_blocks":              # manifold, two bodies' 48-byte records, position row,
+        return contacts * (manifold + 96.0 + 48.0 + 16.0 + 16.0)  # targets, block and slot words
+    if name == "k_solve_prep":          # manifold, two bodies' pose and solver records, the row out
+        return contacts * (manifold + 224.0 + 48.0 + 48.0 * points)
+    if name == "k_island_seed":         # contact header and depths, owners, the linking rows out
+        return contacts * (manifold + 16.0)
+    if name in ("k_island_hook", "k_island_link"):  # the linking rows, both rows' parents
+        return contacts * 16.0
     if name == "k_narrow":              # worklist entry, two bodies and colliders per pair, manifold out
         return pairs * 320.0 + contacts * manifold
     if name == "k_pair_class":          # the pair, both colliders' kinds, the class out
