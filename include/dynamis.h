@@ -241,9 +241,11 @@ size_t dyn_islands_scratch_bytes(uint32_t bodies);
  * The island pass over contacts alone (islands.rs:237-258 without the resting-archive and
  * constraint links, wake and sleep): island_init, island_link_contacts (a touching, non-sensor
  * contact between two simulated rows joins their islands; the higher root is hooked under the
- * lower), ilog2(max(dynamic, 2)) + 1 island_flatten rounds (lib.rs:79-81), island_count. Every
- * row's parent is then its island's minimum row. Contacts naming colliders or bodies outside the
- * records link nothing. The same kernels run inside dyn_world_step.
+ * lower), the island_flatten rounds (lib.rs:79-81; here rounds that walk up to 64 hops, as many
+ * as flatten any forest of `dynamic_count` rows), island_count. Every row's parent is then its
+ * island's minimum row, as in the reference; a seeding pass and hooking rounds before the
+ * linking hooks only shorten their work. Contacts naming colliders or bodies outside the records
+ * link nothing. The same kernels run inside dyn_world_step.
  */
 int dyn_islands(void* stream, const dyn_islands_io* io);
 
