@@ -1110,6 +1110,69 @@ __device__ inline void relay_impulses(dyn_contact& r, const dyn_contact& held, u
     }
 }
 
+// An archived contact's points, read once into registers (all four slots; those past the count
+// are never used).
+struct HeldPoints {
+    uint32_t count;
+    uint32_t feature[4];
+    v3 la[4], lb[4];
+    float n[4], t1[4], t2[4];
+};
+
+__device__ __forceinline__ HeldPoints held_points(const dyn_contact& held) {
+    HeldPoints h;
+    h.count = held.point_count;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const dyn_manifold_point& p = held.points[i];
+        h.feature[i] = p.feature;
+        h.la[i] = ld3(p.local_a);
+        h.lb[i] = ld3(p.local_b);
+        h.n[i] = p.accumulated_normal;
+        h.t1[i] = p.accumulated_tangent_1;
+        h.t2[i] = p.accumulated_tangent_2;
+    }
+    return h;
+}
+
+// relay_impulses from the registers above: the same matches, in the same order, with the same
+// distances.
+__device__ inline void relay_held(dyn_contact& r, const HeldPoints& h, uint32_t* picks) {
+    uint32_t taken = 0;
+    for (uint32_t pi = 0; pi < r.point_count; ++pi) {
+        const dyn_manifold_point& point = r.points[pi];
+        const uint32_t f = point.feature, fm = feature_mirror(point.feature);
+        const v3 pa = ld3(point.local_a), pb = ld3(point.local_b);
+        uint32_t picked = kNoSlot;
+        float picked_distance = 0.0f;
+#pragma unroll
+        for (uint32_t hi = 0; hi < 4; ++hi) {
+            if (hi >= h.count || ((taken >> hi) & 1u)) continue;
+            if (!(h.feature[hi] == f || h.feature[hi] == fm)) continue;
+            const float d = len3(sub(h.la[hi], pa)) + len3(sub(h.lb[hi], pb));
+            if (picked == kNoSlot || d < picked_distance) {
+                picked = hi;
+                picked_distance = d;
+            }
+        }
+        if (picked == kNoSlot) continue;
+        taken |= 1u << picked;
+        picks[pi] = picked;
+        float n = 0.0f, t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+        for (uint32_t hi = 0; hi < 4; ++hi) {
+            if (hi == picked) {
+                n = h.n[hi];
+                t1 = h.t1[hi];
+                t2 = h.t2[hi];
+            }
+        }
+        r.points[pi].accumulated_normal = n;
+        r.points[pi].accumulated_tangent_1 = t1;
+        r.points[pi].accumulated_tangent_2 = t2;
+    }
+}
+
 // body_row.wgsl:1-13
 __device__ __forceinline__ uint32_t resolve_row(const W& w, uint32_t id, uint32_t gen) {
     if (id >= w.id_cap) return DYN_NO_BODY;
@@ -1173,6 +1236,7 @@ __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     // records are read in place and only what the relay changes is written back: the points'
     // impulses, the carried totals and the event bits (not the whole 352-byte record)
     const dyn_contact& held = w.archive[index];
+    const HeldPoints points = held_points(held);  // in flight through the checks and the search
     if (resolve_row(w, held.first_body_id, held.first_generation) == DYN_NO_BODY ||
         resolve_row(w, held.second_body_id, held.second_generation) == DYN_NO_BODY) return;
     const uint32_t count = *w.ncontacts;
@@ -1188,7 +1252,7 @@ __global__ void __launch_bounds__(kThreads) k_relay(W w) {
     if (!(held.events & kContactAnnounced)) return;
     w.matched[lo] = 1u;
     uint32_t picks[4] = {kNoSlot, kNoSlot, kNoSlot, kNoSlot};
-    relay_impulses(cur, held, picks);
+    relay_held(cur, points, picks);
     if (w.relay_src) {
 #pragma unroll
         for (uint32_t q = 0; q < 4; ++q) w.relay_src[4u * lo + q] = picks[q] == kNoSlot ? kNoSlot : 4u * index + picks[q];
