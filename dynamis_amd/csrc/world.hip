@@ -1265,7 +1265,8 @@ __global__ void __launch_bounds__(kThreads) k_relay(W w) {
 __global__ void __launch_bounds__(kThreads) k_contact_begin(W w, float slop) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     if (index >= *w.ncontacts || w.matched[index]) return;
-    dyn_contact c = w.contacts[index];
+    // changed in place: the fields below are the only ones a copy written back would change
+    dyn_contact& c = w.contacts[index];
     const uint32_t kmaj = min(c.first_body_id, c.second_body_id);
     const uint32_t kmin = max(c.first_body_id, c.second_body_id);
     const uint32_t count = *w.nresting;
@@ -1288,16 +1289,12 @@ __global__ void __launch_bounds__(kThreads) k_contact_begin(W w, float slop) {
             c.events |= kContactAnnounced;
             relay_impulses(c, rest);
             carry(c);
-            c.events |= kContactAnnounced;
         } else if (touches(c, slop)) {
             c.events |= kContactAnnounced;
         }
-        w.contacts[index] = c;
         return;
     }
-    if (touches(c, slop)) {
-        w.contacts[index].events = c.events | kContactAnnounced;
-    }
+    if (touches(c, slop)) c.events |= kContactAnnounced;
 }
 
 // ---- the differentiable rollout's tape ---------------------------------------------------------
@@ -2548,6 +2545,14 @@ __global__ void __launch_bounds__(kThreads) k_rest_gather(W w, const uint32_t* k
                                                           const uint32_t* keep_total,
                                                           const uint32_t* freeze_dest,
                                                           uint32_t* out_count) {
+    // the workgroup lists its records (source, destination) and copies them together, its lanes
+    // on consecutive 16-byte pieces
+    constexpr uint32_t kPieces = sizeof(dyn_contact) / sizeof(float4);
+    __shared__ const dyn_contact* from[2 * kThreads];
+    __shared__ uint32_t to[2 * kThreads];
+    __shared__ uint32_t jobs;
+    if (threadIdx.x == 0) jobs = 0u;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t kept = *keep_total;
     const uint32_t old = *w.nresting;
@@ -2564,7 +2569,9 @@ __global__ void __launch_bounds__(kThreads) k_rest_gather(W w, const uint32_t* k
         const uint32_t at = keep_dest[i];
         if (at < w.contact_cap) {
             const dyn_contact& c = w.resting[i];
-            w.rest_tmp[at] = c;
+            const uint32_t k = atomicAdd(&jobs, 1u);
+            from[k] = &c;
+            to[k] = at;
             w.rest_major[at] = min(c.first_body_id, c.second_body_id);
             w.rest_minor[at] = max(c.first_body_id, c.second_body_id);
             w.rest_slot[at] = at;
@@ -2574,19 +2581,39 @@ __global__ void __launch_bounds__(kThreads) k_rest_gather(W w, const uint32_t* k
         const uint32_t at = kept + freeze_dest[i];
         if (at < w.contact_cap) {
             const dyn_contact& c = w.contacts[i];
-            w.rest_tmp[at] = c;
+            const uint32_t k = atomicAdd(&jobs, 1u);
+            from[k] = &c;
+            to[k] = at;
             w.rest_major[at] = min(c.first_body_id, c.second_body_id);
             w.rest_minor[at] = max(c.first_body_id, c.second_body_id);
             w.rest_slot[at] = at;
         }
     }
+    __syncthreads();
+    const uint32_t n = jobs;
+    for (uint32_t item = threadIdx.x; item < n * kPieces; item += blockDim.x) {
+        const uint32_t k = item / kPieces, piece = item - k * kPieces;
+        reinterpret_cast<float4*>(w.rest_tmp + to[k])[piece] = reinterpret_cast<const float4*>(from[k])[piece];
+    }
 }
 
+// resting[i] = rest_tmp[rest_slot[i]], a workgroup's records copied together (lanes on
+// consecutive 16-byte pieces).
 __global__ void __launch_bounds__(kThreads) k_rest_place(W w, const uint32_t* count) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *w.nresting = *count;
-    if (i >= *count) return;
-    w.resting[i] = w.rest_tmp[w.rest_slot[i]];
+    constexpr uint32_t kPieces = sizeof(dyn_contact) / sizeof(float4);
+    __shared__ uint32_t from[kThreads];
+    const uint32_t base = blockIdx.x * blockDim.x;
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t n = *count;
+    if (i == 0) *w.nresting = n;
+    if (i < n) from[threadIdx.x] = w.rest_slot[i];
+    __syncthreads();
+    const uint32_t m = base < n ? min(n - base, static_cast<uint32_t>(kThreads)) : 0u;
+    for (uint32_t item = threadIdx.x; item < m * kPieces; item += blockDim.x) {
+        const uint32_t k = item / kPieces, piece = item - k * kPieces;
+        reinterpret_cast<float4*>(w.resting + base + k)[piece] =
+            reinterpret_cast<const float4*>(w.rest_tmp + from[k])[piece];
+    }
 }
 
 __global__ void k_consume(W w) {
