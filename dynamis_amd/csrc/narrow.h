@@ -157,11 +157,21 @@ __device__ __forceinline__ void emit(Manifold& m, v3 normal) {
     m.normal = normal;
 }
 
+__device__ __forceinline__ v3 sel3(bool take, v3 a, v3 b) {
+    return mk(take ? a.x : b.x, take ? a.y : b.y, take ? a.z : b.z);
+}
+
+// m's next point, written as a select into each of the four slots (a store through a run-time
+// index would keep the manifold in scratch memory).
 __device__ __forceinline__ void push(Manifold& m, v3 p, float depth, uint32_t feature) {
     if (m.count >= DYN_CONTACT_MAX_POINTS) return;
-    m.pos[m.count] = p;
-    m.depth[m.count] = depth;
-    m.feature[m.count] = feature;
+#pragma unroll
+    for (uint32_t j = 0; j < DYN_CONTACT_MAX_POINTS; ++j) {
+        const bool at = j == m.count;
+        m.pos[j] = sel3(at, p, m.pos[j]);
+        m.depth[j] = at ? depth : m.depth[j];
+        m.feature[j] = at ? feature : m.feature[j];
+    }
     ++m.count;
 }
 
@@ -427,6 +437,149 @@ __device__ inline v3 box_face(const BoxGeom& g, v3 face_normal, v3 corners[4], u
     return fc;
 }
 
+// Eight-slot polygons and candidate lists for the box-box clip, kept in registers: every slot is
+// addressed with a compile-time index and a write at a run-time place is a select in each slot.
+// The arithmetic and its order are clip_polygon's and keep's.
+struct Poly8 {
+    v3 p[8];
+    uint32_t id[8];
+};
+
+__device__ __forceinline__ void poly_set(Poly8& q, uint32_t at, v3 v, uint32_t id) {
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        const bool t = j == at;
+        q.p[j] = sel3(t, v, q.p[j]);
+        q.id[j] = t ? id : q.id[j];
+    }
+}
+
+// clip_polygon over a Poly8 (narrowphase.wgsl:272-305)
+__device__ inline uint32_t clip_polygon8(const Poly8& in, uint32_t count, v3 pp, v3 pn, uint32_t group,
+                                         uint32_t side, Poly8& out) {
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (i < count) {
+            const v3 cur = in.p[i];
+            const v3 nxt = i + 1u < 8u ? sel3(i + 1u < count, in.p[i + 1u < 8u ? i + 1u : 0u], in.p[0]) : in.p[0];
+            const float cd = dot3(sub(cur, pp), pn) - kClipMargin;
+            const float nd = dot3(sub(nxt, pp), pn) - kClipMargin;
+            if (cd <= 0.0f && n < 8u) {
+                poly_set(out, n, cur, in.id[i]);
+                ++n;
+            }
+            if (cd * nd < 0.0f) {
+                const float t = cd / (cd - nd);
+                if (n < 8u) {
+                    poly_set(out, n, add(cur, mul(sub(nxt, cur), t)), kFeatureClip | (group * 32u + side * 8u + i));
+                    ++n;
+                }
+            }
+        }
+    }
+    return n;
+}
+
+struct Cand8 {
+    v3 pos[8];
+    float depth[8];
+    uint32_t feature[8];
+    uint32_t count;
+};
+
+__device__ __forceinline__ void cand_push(Cand8& c, v3 p, float depth, uint32_t feature) {
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+        const bool at = j == c.count;
+        c.pos[j] = sel3(at, p, c.pos[j]);
+        c.depth[j] = at ? depth : c.depth[j];
+        c.feature[j] = at ? feature : c.feature[j];
+    }
+    ++c.count;
+}
+
+// c[slot] <-> c[other] for a compile-time slot and a run-time other >= slot whose record is
+// (op, od, of).
+template <uint32_t kSlot>
+__device__ __forceinline__ void cand_swap(Cand8& c, uint32_t other, v3 op, float od, uint32_t of) {
+    const v3 sp = c.pos[kSlot];
+    const float sd = c.depth[kSlot];
+    const uint32_t sf = c.feature[kSlot];
+#pragma unroll
+    for (uint32_t j = kSlot + 1u; j < 8u; ++j) {
+        const bool at = j == other;
+        c.pos[j] = sel3(at, sp, c.pos[j]);
+        c.depth[j] = at ? sd : c.depth[j];
+        c.feature[j] = at ? sf : c.feature[j];
+    }
+    c.pos[kSlot] = op;
+    c.depth[kSlot] = od;
+    c.feature[kSlot] = of;
+}
+
+template <uint32_t kKept>
+__device__ __forceinline__ float cand_gap(const Cand8& c, uint32_t index) {
+    float gap = 3.402823466e38f;
+#pragma unroll
+    for (uint32_t h = 0; h < kKept; ++h) gap = fmn(gap, len3(sub(c.pos[index], c.pos[h])));
+    return gap;
+}
+
+// Round kKept of keep: the farthest remaining candidate from the kept ones moves to slot kKept.
+template <uint32_t kKept>
+__device__ __forceinline__ void cand_round(Cand8& c, uint32_t n) {
+    uint32_t best = kKept;
+    float best_gap = cand_gap<kKept>(c, kKept);
+    v3 bp = c.pos[kKept];
+    float bd = c.depth[kKept];
+    uint32_t bf = c.feature[kKept];
+#pragma unroll
+    for (uint32_t i = kKept + 1u; i < 8u; ++i) {
+        if (i < n) {
+            const float g = cand_gap<kKept>(c, i);
+            if (g > best_gap) {
+                best_gap = g;
+                best = i;
+                bp = c.pos[i];
+                bd = c.depth[i];
+                bf = c.feature[i];
+            }
+        }
+    }
+    if (best != kKept) cand_swap<kKept>(c, best, bp, bd, bf);
+}
+
+// keep (core.wgsl:720-760) over a Cand8: the same lead, rounds and pushes.
+__device__ inline void keep8(Manifold& m, Cand8& c) {
+    const uint32_t n = c.count;
+    uint32_t lead = 0;
+    v3 lp = c.pos[0];
+    float ld = c.depth[0];
+    uint32_t lf = c.feature[0];
+#pragma unroll
+    for (uint32_t i = 1; i < 8; ++i) {
+        if (i < n) {
+            const float d = c.depth[i];
+            if (d > ld || (d == ld && c.feature[i] < lf)) {
+                lead = i;
+                lp = c.pos[i];
+                ld = d;
+                lf = c.feature[i];
+            }
+        }
+    }
+    if (n > 0) cand_swap<0>(c, lead, lp, ld, lf);
+    // rounds past the fourth place change nothing that is pushed
+    if (n > 1) cand_round<1>(c, n);
+    if (n > 2) cand_round<2>(c, n);
+    if (n > 3) cand_round<3>(c, n);
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (i < n && m.count < DYN_CONTACT_MAX_POINTS) push(m, c.pos[i], c.depth[i], c.feature[i]);
+    }
+}
+
 // narrowphase.wgsl:272-305
 __device__ inline uint32_t clip_polygon(const v3* pts, const uint32_t* ids, uint32_t count, v3 pp,
                                         v3 pn, uint32_t group, uint32_t side, v3* out_pts,
@@ -528,42 +681,53 @@ __device__ inline void box_box_sat(Manifold& m, const Body& a, const dyn_collide
     uint32_t inc_ids[4], inc_face = 0;
     box_face(incident, inc_normal, inc_corners, inc_ids, &inc_face);
     const uint32_t ref_field = kFeatureFaceBit | ref_face;
-    v3 pa[8], pb[8];
-    uint32_t ida[8], idb[8];
+    Poly8 pa, pb;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        pa.p[i] = pb.p[i] = zero3v();
+        pa.id[i] = pb.id[i] = 0u;
+    }
+#pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
-        pa[i] = inc_corners[i];
-        ida[i] = inc_ids[i];
+        pa.p[i] = inc_corners[i];
+        pa.id[i] = inc_ids[i];
     }
     uint32_t pc = 4;
+#pragma unroll
     for (uint32_t side = 0; side < 4; ++side) {
+        if (pc == 0u) break;
         const v3 cur = ref_corners[side], nxt = ref_corners[(side + 1u) % 4u];
         const v3 plane_normal = norm3(cross3(sub(nxt, cur), ref_normal));
-        if (side % 2u == 0u) pc = clip_polygon(pa, ida, pc, cur, plane_normal, inc_face, side, pb, idb);
-        else pc = clip_polygon(pb, idb, pc, cur, plane_normal, inc_face, side, pa, ida);
-        if (pc == 0u) break;
+        if (side % 2u == 0u) pc = clip_polygon8(pa, pc, cur, plane_normal, inc_face, side, pb);
+        else pc = clip_polygon8(pb, pc, cur, plane_normal, inc_face, side, pa);
     }
-    Candidates cand;
+    Cand8 cand;
     cand.count = 0;
-    for (uint32_t i = 0; i < pc && cand.count < kCandidates; ++i) {
-        const float pd = dot3(sub(ref_center, pa[i]), ref_normal);
-        if (pd >= -margin) {
-            cand.pos[cand.count] = sub(pa[i], mul(ref_normal, pd * 0.5f));
-            cand.depth[cand.count] = pd;
-            cand.feature[cand.count] = face_on_right ? feature_pair(kFeatureFace, ida[i], ref_field)
-                                                     : feature_pair(kFeatureFace, ref_field, ida[i]);
-            ++cand.count;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        cand.pos[i] = zero3v();
+        cand.depth[i] = 0.0f;
+        cand.feature[i] = 0u;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        if (i < pc) {
+            const float pd = dot3(sub(ref_center, pa.p[i]), ref_normal);
+            if (pd >= -margin) {
+                cand_push(cand, sub(pa.p[i], mul(ref_normal, pd * 0.5f)), pd,
+                          face_on_right ? feature_pair(kFeatureFace, pa.id[i], ref_field)
+                                        : feature_pair(kFeatureFace, ref_field, pa.id[i]));
+            }
         }
     }
     if (cand.count == 0u) {
-        for (uint32_t i = 0; i < 4 && cand.count < kCandidates; ++i) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
             const float pd = dot3(sub(ref_center, inc_corners[i]), ref_normal);
             if (pd >= -margin) {
-                cand.pos[cand.count] = sub(inc_corners[i], mul(ref_normal, pd * 0.5f));
-                cand.depth[cand.count] = pd;
-                cand.feature[cand.count] = face_on_right
-                                               ? feature_pair(kFeatureFace, inc_ids[i], ref_field)
-                                               : feature_pair(kFeatureFace, ref_field, inc_ids[i]);
-                ++cand.count;
+                cand_push(cand, sub(inc_corners[i], mul(ref_normal, pd * 0.5f)), pd,
+                          face_on_right ? feature_pair(kFeatureFace, inc_ids[i], ref_field)
+                                        : feature_pair(kFeatureFace, ref_field, inc_ids[i]));
             }
         }
     }
@@ -571,7 +735,7 @@ __device__ inline void box_box_sat(Manifold& m, const Body& a, const dyn_collide
         push(m, sub(ref_center, mul(ref_normal, face_depth * 0.5f)), face_depth, kFeaturePoint);
         return;
     }
-    keep(m, cand);
+    keep8(m, cand);
 }
 
 }  // namespace dyn
