@@ -1410,4 +1410,44 @@ __device__ inline void plane_convex(Manifold& m, const Shape& plane, const Shape
     keep(m, cand);
 }
 
+// plane_convex for the shapes k_narrow gives it (sphere, box, capsule): the same feature
+// (shape_feature's branches for those kinds), candidates and selection, with every array
+// addressed at compile-time indices so that none sits in scratch memory.
+__device__ inline void plane_analytic(Manifold& m, const Shape& plane, const Shape& convex, const ShapeSet& ss,
+                                      float margin) {
+    const v3 n = qrot(plane.rotation, mk(0.0f, 1.0f, 0.0f));
+    const float center_side = dot3(sub(convex.center, plane.center), n);
+    const v3 facing = center_side < 0.0f ? neg(n) : n;
+    emit(m, facing);
+    Feature f;
+    if (convex.kind == DYN_SHAPE_CUBOID) {
+        box_feature_ring(convex, neg(facing), &f);
+    } else if (convex.kind == DYN_SHAPE_CAPSULE) {
+        capsule_feature(convex, neg(facing), &f);
+    } else {
+        f.p[0] = support(convex, ss, neg(facing));
+        f.id[0] = 0u;
+        f.count = 1;
+    }
+    Cand8 cand;
+    cand.count = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+        cand.pos[i] = zero3v();
+        cand.depth[i] = 0.0f;
+        cand.feature[i] = 0u;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {  // a box face, a capsule side or one point
+        if (i < f.count) {
+            const float depth = dot3(sub(plane.center, f.p[i]), facing);
+            if (!(depth <= -margin)) {  // plane_convex's test, NaN included
+                cand_push(cand, add(f.p[i], mul(facing, depth * 0.5f)), depth,
+                          feature_pair(kFeatureVertex, kFeatureFaceBit | 0u, f.id[i]));
+            }
+        }
+    }
+    keep8(m, cand);
+}
+
 }  // namespace dyn

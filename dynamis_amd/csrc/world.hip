@@ -626,10 +626,10 @@ __global__ void __launch_bounds__(kThreads) k_narrow(W w) {
     m.count = 0;
     m.normal = zero3v();
     bool mirrored_features = false;
-    if (fc.kind == DYN_SHAPE_PLANE) {
-        plane_convex(m, world_collider(first.s, fc), world_collider(second.s, sc), shapes, margin);
+    if (fc.kind == DYN_SHAPE_PLANE) {  // the other shape is a sphere, box or capsule (convex_path)
+        plane_analytic(m, world_collider(first.s, fc), world_collider(second.s, sc), shapes, margin);
     } else if (sc.kind == DYN_SHAPE_PLANE) {
-        plane_convex(m, world_collider(second.s, sc), world_collider(first.s, fc), shapes, margin);
+        plane_analytic(m, world_collider(second.s, sc), world_collider(first.s, fc), shapes, margin);
         mirror(m);
         mirrored_features = true;
     } else {
