@@ -195,6 +195,28 @@ def test_rollout_is_bit_identical_to_oracle(device, oracle, name, frames):
             assert dev.inspect_contacts().tobytes() == ref.inspect_contacts().tobytes(), (name, frame)
 
 
+@pytest.mark.parametrize("solve,position,substeps", [(1, 1, 2), (1, 1, 1), (2, 1, 3), (3, 2, 1)])
+def test_iteration_counts_bit_identical(device, oracle, solve, position, substeps):
+    """Iteration counts at the edges of the solver's schedule (the parameters admit one of each
+    at least): one velocity iteration (its apply is the substep's last and writes the velocities
+    back), one position iteration (the first is the last: the pose goes from the body state to
+    the body state), one and three substeps. The device's states and contacts equal the oracle's
+    bit for bit over a settling stack and a few falling cubes."""
+    bodies = [BodyDesc.plane()] + [BodyDesc.cuboid((0.5, 0.5, 0.5), position=(0.03 * i, 0.55 + i * 1.1, 0.0))
+                                   for i in range(5)]
+    bodies += [BodyDesc.cuboid((0.4, 0.3, 0.5), position=(2.0 + 0.9 * i, 1.5 + 0.7 * i, 0.1 * i)) for i in range(4)]
+    cfg = config(solve_iterations=solve, position_iterations=position, substeps=substeps)
+    print(f"[world parity] iterations {solve}/{position}/{substeps}", flush=True)
+    scene = encode(bodies, cfg)
+    ref, dev = oracle.world(scene), device_world(scene)
+    for frame in range(90):
+        ref.step(DT)
+        dev.step(DT)
+        if frame % 15 == 14:
+            assert dev.states().tobytes() == ref.states().tobytes(), (solve, position, substeps, frame)
+            assert dev.inspect_contacts().tobytes() == ref.inspect_contacts().tobytes(), (solve, position, substeps, frame)
+
+
 def lowest_points(states, half=0.5):
     corners = np.array([[sx, sy, sz] for sx in (-half, half) for sy in (-half, half)
                         for sz in (-half, half)])
