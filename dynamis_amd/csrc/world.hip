@@ -2209,21 +2209,35 @@ constexpr uint32_t kApplyBatch = 8;
 // state at the substep's start) while the iterations run: only the substep's last apply
 // (`commit`) writes them to the body state, for every claimed row, as the same bits an update of
 // both after every apply would leave there.
-__device__ __forceinline__ void apply_at(const W& w, uint32_t row, uint32_t commit) {
-    if (row >= w.n) return;
+// Thread t serves row t >> lanes_log2 with its 2^lanes_log2 neighbours (one wave): each lane sums
+// every 2^lanes_log2-th entry of the row's list and the lanes add their sums with shuffles
+// (integer sums: the same words in any order); the first lane finishes the row. A small world
+// gets several lanes a row, so the rows' dependent loads overlap more.
+__device__ __forceinline__ uint32_t row_lanes_sum(uint32_t v, uint32_t lanes_log2) {
+    for (uint32_t o = 1; o < (1u << lanes_log2); o <<= 1) v += __shfl_xor(v, static_cast<int>(o), kWave);
+    return v;
+}
+
+__device__ __forceinline__ void apply_at(const W& w, uint32_t t, uint32_t commit, uint32_t lanes_log2) {
+    const uint32_t lanes = 1u << lanes_log2;
+    const uint32_t row = t >> lanes_log2, sub = t & (lanes - 1u);
+    const bool live = row < w.n;
     // list_counts is 0 for rows without blocks and for inert rows, which receive no words
-    const uint32_t count = w.list_counts[row];
-    const uint32_t start = w.row_start[row];
-    if (count == 0u) return;
-    float4* p = w.sbodies + kSolverBody * row;  // the solver body's copy
-    const float4 pv = p[3], pw = p[4];           // in flight with the list
+    const uint32_t count = live ? w.list_counts[row] : 0u;
+    const uint32_t start = live ? w.row_start[row] : 0u;
+    float4* p = w.sbodies + kSolverBody * (live ? row : 0u);  // the solver body's copy
+    float4 pv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pw = pv;
+    if (sub == 0u && count != 0u) {  // in flight with the list
+        pv = p[3];
+        pw = p[4];
+    }
     const v3 v0 = mk(pv.x, pv.y, pv.z), av0 = mk(pw.x, pw.y, pw.z);
     const uint32_t* list = w.adj + start;
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-    for (uint32_t e = 0; e < count; e += kApplyBatch) {
+    for (uint32_t e = sub; e < count; e += kApplyBatch * lanes) {
         uint32_t entry[kApplyBatch];
 #pragma unroll
-        for (uint32_t j = 0; j < kApplyBatch; ++j) entry[j] = e + j < count ? list[e + j] : kNoSlot;
+        for (uint32_t j = 0; j < kApplyBatch; ++j) entry[j] = e + j * lanes < count ? list[e + j * lanes] : kNoSlot;
         uint2 a[kApplyBatch], b[kApplyBatch], c[kApplyBatch];
 #pragma unroll
         for (uint32_t j = 0; j < kApplyBatch; ++j) {
@@ -2246,6 +2260,15 @@ __device__ __forceinline__ void apply_at(const W& w, uint32_t row, uint32_t comm
             s5 += c[j].y;
         }
     }
+    if (lanes_log2 != 0u) {
+        s0 = row_lanes_sum(s0, lanes_log2);
+        s1 = row_lanes_sum(s1, lanes_log2);
+        s2 = row_lanes_sum(s2, lanes_log2);
+        s3 = row_lanes_sum(s3, lanes_log2);
+        s4 = row_lanes_sum(s4, lanes_log2);
+        s5 = row_lanes_sum(s5, lanes_log2);
+    }
+    if (sub != 0u || count == 0u) return;
     v3 v = v0, av = av0;
     if ((s0 | s1 | s2 | s3 | s4 | s5) != 0u) {
         const v3 lin = mk(solver_value(s0, DYN_SOLVER_VELOCITY_SCALE), solver_value(s1, DYN_SOLVER_VELOCITY_SCALE),
@@ -2266,8 +2289,8 @@ __device__ __forceinline__ void apply_at(const W& w, uint32_t row, uint32_t comm
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_apply(W w, uint32_t commit) {
-    apply_at(w, blockIdx.x * blockDim.x + threadIdx.x, commit);
+__global__ void __launch_bounds__(kThreads) k_apply(W w, uint32_t commit, uint32_t lanes_log2) {
+    apply_at(w, blockIdx.x * blockDim.x + threadIdx.x, commit, lanes_log2);
 }
 
 // substep_integrate.wgsl:15-76 over the awake dynamic rows (no fields)
@@ -2400,49 +2423,64 @@ __global__ void __launch_bounds__(kThreads) k_position_block(W w) {
 // body state and writes ppose (moved or not), the last writes the body state (every claimed row,
 // as the same bits per-iteration updates of the state would leave). An inert row with a
 // contributing block, which differs from iteration to iteration, reads and writes the state.
-__device__ __forceinline__ void position_apply_at(const W& w, uint32_t row, uint32_t first, uint32_t last) {
-    if (row >= w.n) return;
+__device__ __forceinline__ void position_apply_at(const W& w, uint32_t t, uint32_t first, uint32_t last,
+                                                  uint32_t lanes_log2) {
+    const uint32_t lanes = 1u << lanes_log2;
+    const uint32_t row = t >> lanes_log2, sub = t & (lanes - 1u);
+    const bool live = row < w.n;
     // list_counts is 0 for rows without blocks and for inert rows; contrib, zero outside a
     // position pass, is marked only for inert rows with a contributing block
-    const uint32_t count = w.list_counts[row];
-    const uint32_t start = w.row_start[row];
-    dyn_body_state& s = w.st[row];
-    float4& r = reinterpret_cast<float4*>(w.resolution)[row];
-    float4* pose = w.ppose + 2u * row;
+    const uint32_t count = live ? w.list_counts[row] : 0u;
+    const uint32_t start = live ? w.row_start[row] : 0u;
+    const uint32_t at = live ? row : 0u;
+    dyn_body_state& s = w.st[at];
+    float4& r = reinterpret_cast<float4*>(w.resolution)[at];
+    float4* pose = w.ppose + 2u * at;
     const bool in_state = count == 0u || first;
-    v3 p0;
-    float4 q0;
-    if (in_state) {  // in flight with the list
-        p0 = ld3(s.position);
-        q0 = ld4(s.orientation);
-    } else {
-        const float4 a = pose[0];
-        p0 = mk(a.x, a.y, a.z);
-        q0 = pose[1];
+    v3 p0 = zero3v();
+    float4 q0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), r0 = q0;
+    if (live && sub == 0u) {  // in flight with the list
+        if (in_state) {
+            p0 = ld3(s.position);
+            q0 = ld4(s.orientation);
+        } else {
+            const float4 a = pose[0];
+            p0 = mk(a.x, a.y, a.z);
+            q0 = pose[1];
+        }
+        r0 = r;
     }
-    const float4 r0 = r;
     uint32_t contributing = 0;
     uint4 lo = make_uint4(0u, 0u, 0u, 0u);  // linear words, then the (zero) angular x
     uint2 hi = make_uint2(0u, 0u);          // angular y, z: the position blocks send none
+    const uint4* run = w.pwords + start;
+    for (uint32_t e = sub; e < count; e += kApplyBatch * lanes) {
+        uint4 word[kApplyBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kApplyBatch; ++j) {
+            word[j] = e + j * lanes < count ? run[e + j * lanes] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kApplyBatch; ++j) {
+            if (word[j].x == 0u) continue;  // the block did not contribute
+            ++contributing;
+            lo.x += word[j].y;
+            lo.y += word[j].z;
+            lo.z += word[j].w;
+        }
+    }
+    if (lanes_log2 != 0u) {
+        contributing = row_lanes_sum(contributing, lanes_log2);
+        lo.x = row_lanes_sum(lo.x, lanes_log2);
+        lo.y = row_lanes_sum(lo.y, lanes_log2);
+        lo.z = row_lanes_sum(lo.z, lanes_log2);
+    }
+    if (!live || sub != 0u) return;
     if (count == 0u) {
         contributing = w.contrib[row];  // inert row: only the mark
         if (contributing == 0u) return;
         w.contrib[row] = 0u;
     } else {
-        const uint4* run = w.pwords + start;
-        for (uint32_t e = 0; e < count; e += kApplyBatch) {
-            uint4 word[kApplyBatch];
-#pragma unroll
-            for (uint32_t j = 0; j < kApplyBatch; ++j) word[j] = e + j < count ? run[e + j] : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-            for (uint32_t j = 0; j < kApplyBatch; ++j) {
-                if (word[j].x == 0u) continue;  // the block did not contribute
-                ++contributing;
-                lo.x += word[j].y;
-                lo.y += word[j].z;
-                lo.z += word[j].w;
-            }
-        }
         if (contributing == 0u) {
             if (first && !last) {
                 pose[0] = make_float4(p0.x, p0.y, p0.z, 0.0f);
@@ -2473,8 +2511,8 @@ __device__ __forceinline__ void position_apply_at(const W& w, uint32_t row, uint
     r = make_float4(nr.x, nr.y, nr.z, 0.0f);
 }
 
-__global__ void __launch_bounds__(kThreads) k_position_apply(W w, uint32_t first, uint32_t last) {
-    position_apply_at(w, blockIdx.x * blockDim.x + threadIdx.x, first, last);
+__global__ void __launch_bounds__(kThreads) k_position_apply(W w, uint32_t first, uint32_t last, uint32_t lanes_log2) {
+    position_apply_at(w, blockIdx.x * blockDim.x + threadIdx.x, first, last, lanes_log2);
 }
 
 // island_sleep.wgsl
@@ -3046,6 +3084,9 @@ int solver_prepare(const dyn::W& w, hipStream_t st) {
 int solver_substeps(const dyn::W& w, hipStream_t st, uint32_t threads = dyn::kThreads) {
     using namespace dyn;
     const uint32_t nb = blocks_for(w.n);
+    // four lanes a row in the apply kernels for a world of up to 2^18 rows (too few rows to keep
+    // the GPU's memory pipes busy otherwise), one above
+    const uint32_t lanes_log2 = w.n <= (1u << 18) ? 2u : 0u;
     for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
         LAUNCH_TH(st, k_integrate, w.d, threads, w);
         LAUNCH_TH(st, k_solver_bodies, w.n, threads, w);
@@ -3053,20 +3094,21 @@ int solver_substeps(const dyn::W& w, hipStream_t st, uint32_t threads = dyn::kTh
         // the substep's last apply writes the velocities back to the body states
         if (sub == 0) {
             LAUNCH_TH(st, k_warm, w.contact_cap, threads, w);
-            LAUNCH_TH(st, k_apply, w.n, threads, w, w.p.solve_iterations == 0u ? 1u : 0u);
+            LAUNCH_TH(st, k_apply, w.n << lanes_log2, threads, w, w.p.solve_iterations == 0u ? 1u : 0u, lanes_log2);
         }
         for (uint32_t it = 0; it < w.p.solve_iterations; ++it) {
             // solver_block_solve.wgsl:85-87: the residual round is the one where the round
             // counter (reset by integrate, bumped by every apply) equals the iteration count,
             // which only the warm-started first substep reaches.
             LAUNCH_TH(st, k_solve, w.contact_cap, threads, w, (sub == 0 && it + 1u == w.p.solve_iterations) ? 1u : 0u);
-            LAUNCH_TH(st, k_apply, w.n, threads, w, it + 1u == w.p.solve_iterations ? 1u : 0u);
+            LAUNCH_TH(st, k_apply, w.n << lanes_log2, threads, w, it + 1u == w.p.solve_iterations ? 1u : 0u,
+                      lanes_log2);
         }
         LAUNCH_TH(st, k_advance, w.d, threads, w);
         for (uint32_t it = 0; it < w.p.position_iterations; ++it) {
             LAUNCH_TH(st, k_position_block, w.contact_cap, threads, w);
-            LAUNCH_TH(st, k_position_apply, w.n, threads, w, it == 0u ? 1u : 0u,
-                      it + 1u == w.p.position_iterations ? 1u : 0u);
+            LAUNCH_TH(st, k_position_apply, w.n << lanes_log2, threads, w, it == 0u ? 1u : 0u,
+                      it + 1u == w.p.position_iterations ? 1u : 0u, lanes_log2);
         }
     }
     LAUNCH_TH(st, k_impulses_store, w.contact_cap, threads, w);
