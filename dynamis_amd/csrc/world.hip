@@ -3084,9 +3084,9 @@ int solver_prepare(const dyn::W& w, hipStream_t st) {
 int solver_substeps(const dyn::W& w, hipStream_t st, uint32_t threads = dyn::kThreads) {
     using namespace dyn;
     const uint32_t nb = blocks_for(w.n);
-    // four lanes a row in the apply kernels for a world of up to 2^18 rows (too few rows to keep
-    // the GPU's memory pipes busy otherwise), one above
-    const uint32_t lanes_log2 = w.n <= (1u << 18) ? 2u : 0u;
+    // lanes a row in the apply kernels: four for a world of up to 2^18 rows (too few rows to keep
+    // the GPU's memory pipes busy otherwise; eight were slower), two above (one was slower)
+    const uint32_t lanes_log2 = w.n <= (1u << 18) ? 2u : 1u;
     for (uint32_t sub = 0; sub < w.p.substeps; ++sub) {
         LAUNCH_TH(st, k_integrate, w.d, threads, w);
         LAUNCH_TH(st, k_solver_bodies, w.n, threads, w);
